@@ -1,0 +1,286 @@
+"""Deck model of the drop-in file contract, and a generator of decks.
+
+A deck is what the reference solver reads at start-up: the positional ``input.txt``
+(parsed by ``airpore::airpore``, 3D_MPI_TEST/airpore.cpp:17-316) and the grid files
+``coordi_air{1,2,3}.dat`` / ``coordi_pore{1,2,3}.dat`` (read by
+``calculation_2D::calculation_2D``, 3D_MPI_TEST/calculation_2D.cpp:44-68, 154-204).
+
+``Deck.write(dir)`` produces files in exactly that layout (same line structure as the
+shipped ``3D_MPI_TEST/input.txt``, so the reference parser and ours read the same values),
+and ``Deck.to_c()`` builds the ``lee_deck`` struct of ``include/lee_abi.h`` directly.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+
+
+class LeeDeck(ctypes.Structure):
+    """ctypes mirror of ``lee_deck`` (include/lee_abi.h)."""
+
+    _fields_ = [
+        ("gauss_width", ctypes.c_double),
+        ("mf_jmax", ctypes.c_int), ("mf_lead_dj", ctypes.c_int), ("mf_trail_dj", ctypes.c_int),
+        ("mf_judge", ctypes.c_int),
+        ("mf_ratio", ctypes.c_double),
+        ("scheme", ctypes.c_int),
+        ("bc", ctypes.c_int * 6),
+        ("time_domain", ctypes.c_int), ("out_difft", ctypes.c_int), ("fft_m", ctypes.c_int),
+        ("frequency", ctypes.c_double),
+        ("velocity_coef", ctypes.c_double),
+        ("velocity_method", ctypes.c_int),
+        ("gamma", ctypes.c_double), ("Pav", ctypes.c_double), ("c", ctypes.c_double),
+        ("pml_alpha", ctypes.c_double * 3), ("pml_R", ctypes.c_double * 3),
+        ("pml_y_time", ctypes.c_int),
+        ("pore_q", ctypes.c_double), ("pore_porosity", ctypes.c_double),
+        ("pore_resistivity", ctypes.c_double),
+        ("src", ctypes.c_double * 3), ("rcv", ctypes.c_double * 3),
+        ("dt", ctypes.c_double),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+        ("mpi_dims", ctypes.c_int * 3),
+        ("restart", ctypes.c_int), ("restart_out", ctypes.c_int),
+        ("hill_judge", ctypes.c_int),
+        ("pore_dx", ctypes.c_double), ("pore_dy", ctypes.c_double), ("pore_dz", ctypes.c_double),
+        ("air_hdr", ctypes.c_int * 10),
+        ("air_x", c_double_p), ("air_y", c_double_p), ("air_z", c_double_p),
+        ("has_pore", ctypes.c_int),
+        ("pore_hdr", ctypes.c_int * 10),
+        ("pore_x", c_double_p), ("pore_y", c_double_p), ("pore_z", c_double_p),
+    ]
+
+
+def _ptr(a: Optional[np.ndarray]):
+    if a is None:
+        return c_double_p()
+    return a.ctypes.data_as(c_double_p)
+
+
+def _fmt(v) -> str:
+    if isinstance(v, (int, np.integer)):
+        return str(int(v))
+    return repr(float(v))
+
+
+@dataclasses.dataclass
+class Deck:
+    """All values of input.txt plus the grid arrays (air, and porous when South = 3)."""
+
+    air_x: np.ndarray
+    air_y: np.ndarray
+    air_z: np.ndarray
+    # PML point counts of the coordi_air1.dat header: IMAX1 IMAX2 JMAX1 JMAX2 KMAX1 KMAX2
+    pml_cells: Sequence[int] = (0, 0, 0, 0, 0, 0)
+    ijkmax: Optional[int] = None
+    gauss_width: float = 0.3
+    mf_jmax: Optional[int] = None
+    mf_lead_dj: int = 11
+    mf_trail_dj: int = 11
+    mf_judge: int = 0
+    mf_ratio: float = 0.75
+    scheme: int = 2
+    bc: Sequence[int] = (1, 1, 1, 1, 1, 1)  # N S E W F B
+    time_domain: int = 100
+    out_difft: int = 100
+    fft_m: int = 12
+    frequency: float = 300.0
+    velocity_coef: float = 0.0
+    velocity_method: int = 4
+    gamma: float = 1.4
+    Pav: float = 1.0e5
+    c: float = 340.0
+    pml_alpha: Sequence[float] = (2.0, 2.0, 2.0)
+    pml_R: Sequence[float] = (1000.0, 1000.0, 1000.0)
+    pml_y_time: int = 4500
+    pore_q: float = 1.7320508075688772
+    pore_porosity: float = 0.3
+    pore_resistivity: float = 2.0e5
+    src: Sequence[float] = (0.0, 0.0, 0.0)
+    rcv: Sequence[float] = (0.0, -0.23, 1.75)
+    dt: float = 4e-6
+    dx: float = 0.005
+    dy: float = 0.005
+    dz: float = 0.005
+    mpi_dims: Sequence[int] = (1, 1, 1)
+    restart: int = 0
+    restart_out: int = 5000
+    restart_infile: str = "ct003"
+    hill_judge: int = 0
+    pore_dx: float = 0.005
+    pore_dy: float = 0.005
+    pore_dz: float = 0.005
+    pore_x: Optional[np.ndarray] = None
+    pore_y: Optional[np.ndarray] = None
+    pore_z: Optional[np.ndarray] = None
+
+    def __post_init__(self):
+        self.air_x = np.ascontiguousarray(self.air_x, dtype=np.float64)
+        self.air_y = np.ascontiguousarray(self.air_y, dtype=np.float64)
+        self.air_z = np.ascontiguousarray(self.air_z, dtype=np.float64)
+        for name in ("pore_x", "pore_y", "pore_z"):
+            v = getattr(self, name)
+            if v is not None:
+                setattr(self, name, np.ascontiguousarray(v, dtype=np.float64))
+        if self.mf_jmax is None:
+            self.mf_jmax = len(self.air_y)
+        if self.ijkmax is None:
+            self.ijkmax = len(self.air_y)
+        self._keep = []
+
+    # ------------------------------------------------------------------ shape helpers
+    @property
+    def shape(self):
+        return (len(self.air_x), len(self.air_y), len(self.air_z))
+
+    @property
+    def has_pore(self) -> bool:
+        return self.pore_x is not None
+
+    def air_hdr(self):
+        i1, i2, j1, j2, k1, k2 = self.pml_cells
+        return [len(self.air_x), i1, i2, len(self.air_y), j1, j2, len(self.air_z), k1, k2, int(self.ijkmax)]
+
+    def pore_hdr(self):
+        if not self.has_pore:
+            return [0] * 10
+        return [len(self.pore_x), 0, 0, len(self.pore_y), 0, 0, len(self.pore_z), 0, 0, len(self.pore_y)]
+
+    # ------------------------------------------------------------------ C struct
+    def to_c(self) -> LeeDeck:
+        d = LeeDeck()
+        d.gauss_width = self.gauss_width
+        d.mf_jmax, d.mf_lead_dj, d.mf_trail_dj = int(self.mf_jmax), self.mf_lead_dj, self.mf_trail_dj
+        d.mf_judge, d.mf_ratio = self.mf_judge, self.mf_ratio
+        d.scheme = self.scheme
+        for q in range(6):
+            d.bc[q] = int(self.bc[q])
+        d.time_domain, d.out_difft, d.fft_m, d.frequency = self.time_domain, self.out_difft, self.fft_m, self.frequency
+        d.velocity_coef, d.velocity_method = self.velocity_coef, self.velocity_method
+        d.gamma, d.Pav, d.c = self.gamma, self.Pav, self.c
+        for q in range(3):
+            d.pml_alpha[q] = self.pml_alpha[q]
+            d.pml_R[q] = self.pml_R[q]
+            d.src[q] = self.src[q]
+            d.rcv[q] = self.rcv[q]
+            d.mpi_dims[q] = self.mpi_dims[q]
+        d.pml_y_time = self.pml_y_time
+        d.pore_q, d.pore_porosity, d.pore_resistivity = self.pore_q, self.pore_porosity, self.pore_resistivity
+        d.dt, d.dx, d.dy, d.dz = self.dt, self.dx, self.dy, self.dz
+        d.restart, d.restart_out, d.hill_judge = self.restart, self.restart_out, self.hill_judge
+        d.pore_dx, d.pore_dy, d.pore_dz = self.pore_dx, self.pore_dy, self.pore_dz
+        for q, v in enumerate(self.air_hdr()):
+            d.air_hdr[q] = v
+        # the reader takes JMAX values from coordi_air2.dat where JMAX is the frame length
+        ny = int(self.mf_jmax) if self.mf_judge == 1 else len(self.air_y)
+        self._ay = np.ascontiguousarray(self.air_y[:ny])
+        d.air_hdr[3] = len(self.air_y)
+        d.air_x, d.air_y, d.air_z = _ptr(self.air_x), _ptr(self._ay), _ptr(self.air_z)
+        d.has_pore = 1 if self.has_pore else 0
+        if self.has_pore:
+            for q, v in enumerate(self.pore_hdr()):
+                d.pore_hdr[q] = v
+            self._py = np.ascontiguousarray(self.pore_y[:ny])
+            d.pore_x, d.pore_y, d.pore_z = _ptr(self.pore_x), _ptr(self._py), _ptr(self.pore_z)
+        return d
+
+    # ------------------------------------------------------------------ files
+    def write(self, dirpath: str) -> None:
+        """Write input.txt and coordi_*.dat with the line structure of the shipped deck."""
+        os.makedirs(dirpath, exist_ok=True)
+        f = _fmt
+        lines = [
+            f"\t{f(self.gauss_width)}//gaussian_width",
+            f"\t{self.mf_jmax} {self.mf_lead_dj} {self.mf_trail_dj} {self.mf_judge} {f(self.mf_ratio)} //Moving domain:MF_JMAX; lead; trail; judge; ratio",
+            f"\t{self.scheme} //scheme:1.FB_v_p 2.FB_p_v 5.RK4",
+            "\t" + " ".join(str(int(b)) for b in self.bc) + " //NorthBC SouthBC EastBC WestBC FrontBC BackBC and",
+            "\t\t\t//B.C type: 1.rigid 2.absorbing layer 3.porous media 4.radiation 5. TDBC",
+            f"\t{self.time_domain} {self.out_difft} {self.fft_m} {f(self.frequency)} //time_domain;out_difft;FFT_m;frequency;",
+            "",
+            f"\t{f(self.velocity_coef)} {self.velocity_method} //b; velocity method",
+            f"\t{f(self.gamma)} {f(self.Pav)} {f(self.c)} //adiabatic_coef;Pav;sound_speed;",
+            f"\t{f(self.pml_alpha[0])} {f(self.pml_R[0])}//PMLx: alpha,resistivity;",
+            f"\t{f(self.pml_alpha[1])} {f(self.pml_R[1])} {self.pml_y_time} //PMLy: alpha,resistivity and time_step;",
+            f"\t{f(self.pml_alpha[2])} {f(self.pml_R[2])} //PMLz: alpha,resistivity;",
+            f"\t{f(self.pore_q)} {f(self.pore_porosity)} {f(self.pore_resistivity)} //PorePara.q;porosity;resistivity;",
+            "\t" + " ".join(f(v) for v in (*self.src, *self.rcv)) + " //source.xyz;receiver.xyz;",
+            "",
+            f"\t{f(self.dt)} //AirStep.diff_t;",
+            f"\t{f(self.dx)} {f(self.dy)} {f(self.dz)} //AirStep.diff_x;diff_y;diff_z;",
+            "",
+            "\t" + " ".join(str(int(v)) for v in self.mpi_dims) + " //mpi_xarea,mpi_yarea,mpi_zarea",
+            "",
+            "\toutput.data;",
+            f"\t{self.restart} {self.restart_out} {self.restart_infile}",
+            "",
+            f"\t{self.hill_judge} -1.0 1.0 0.0 2.5 -0.2 0.0 3.0 0.0 0.0 0 0 0 0 //Curve",
+            "\t1.79 0.877 5.1e+04 //hill.q;porosity;resistivity;",
+            "\t1.79 0.877 5.1e+04 //canopy.q;porosity;resistivity;",
+            "\t0.52 0.5712 8.0e4  //trunk.q;porosity;resistivity;",
+            "",
+            "\t-1.0 1.0 0.0 2.5 -0.2 0.0 //xstar,xstop,ystar,ystop,zstar and zstop;",
+            "",
+            "\tporous_media;",
+            f"\t{f(self.pore_dx)} {f(self.pore_dy)} {f(self.pore_dz)} //SouthStep.diff_x;diff_y;diff_z;",
+        ]
+        with open(os.path.join(dirpath, "input.txt"), "w") as fh:
+            fh.write("\n".join(lines) + "\n")
+        _write_grid(dirpath, "coordi_air", self.air_hdr(), self.air_x, self.air_y, self.air_z)
+        if self.has_pore:
+            _write_grid(dirpath, "coordi_pore", self.pore_hdr(), self.pore_x, self.pore_y, self.pore_z)
+
+
+def _write_grid(dirpath, stem, hdr, x, y, z):
+    with open(os.path.join(dirpath, stem + "1.dat"), "w") as fh:
+        fh.write(" ".join(str(int(v)) for v in hdr) + "\n")
+        fh.write(" ".join(repr(float(v)) for v in x) + "\n")
+    with open(os.path.join(dirpath, stem + "2.dat"), "w") as fh:
+        fh.write(" ".join(repr(float(v)) for v in y) + "\n")
+    with open(os.path.join(dirpath, stem + "3.dat"), "w") as fh:
+        fh.write(" ".join(repr(float(v)) for v in z) + "\n")
+
+
+# ---------------------------------------------------------------------- generators
+
+def uniform_axis(n: int, d: float, start: float = 0.0) -> np.ndarray:
+    """n coordinates start + i*d, computed per index (no accumulated rounding)."""
+    return start + d * np.arange(n, dtype=np.float64)
+
+
+def box(nx: int, ny: int, nz: int, d: float = 0.05, *, pml: int = 0, bc=None, dt=None,
+        cfl: float = 0.5, c: float = 343.0, gamma: float = 1.4, rho0: float = 1.21,
+        porous_cells: int = 0, pore_dz: Optional[float] = None, x0: Optional[float] = None,
+        y0: float = 0.0, z0: float = 0.0, **kw) -> Deck:
+    """A rectangular air box of nx*ny*nz points at spacing d, optional PML on the sides and
+    top (`pml` points each) and an optional porous ground layer of `porous_cells` points.
+
+    P0 = rho0*c^2/gamma (so that c^2/(gamma*P0) = 1/rho0), dt = cfl*d/c unless given.
+    """
+    x0 = -0.5 * (nx - 1) * d if x0 is None else x0
+    ax = uniform_axis(nx, d, x0)
+    ay = uniform_axis(ny, d, y0)
+    az = uniform_axis(nz, d, z0)
+    Pav = rho0 * c * c / gamma
+    if dt is None:
+        dt = cfl * d / c
+    if bc is None:
+        # N S E W F B: absorbing top and sides, ground rigid (or porous)
+        bc = (2 if pml else 1, 3 if porous_cells else 1, 2 if pml else 1, 2 if pml else 1,
+              2 if pml else 1, 2 if pml else 1)
+    pml_cells = (pml if bc[5] == 2 else 0, pml if bc[4] == 2 else 0,
+                 pml if bc[3] == 2 else 0, pml if bc[2] == 2 else 0,
+                 pml if bc[1] == 2 else 0, pml if bc[0] == 2 else 0)
+    kwargs = dict(air_x=ax, air_y=ay, air_z=az, pml_cells=pml_cells, bc=tuple(bc), dt=dt,
+                  dx=d, dy=d, dz=d, gamma=gamma, Pav=Pav, c=c)
+    if porous_cells:
+        pdz = d if pore_dz is None else pore_dz
+        # porous column: Kp points ending one plane above the air ground (two planes overlap
+        # through the ghost copy of calculation_2D.cpp:878-886)
+        pz = z0 + pdz * (np.arange(porous_cells, dtype=np.float64) - (porous_cells - 2))
+        kwargs.update(pore_x=ax.copy(), pore_y=ay.copy(), pore_z=pz, pore_dx=d, pore_dy=d, pore_dz=pdz)
+    kwargs.update(kw)
+    return Deck(**kwargs)

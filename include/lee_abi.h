@@ -1,0 +1,185 @@
+/*
+ * lee_abi.h -- C ABI of the MI355X linearised-Euler (LEE) + porous-ground solver.
+ *
+ * This is the drop-in boundary for the time-stepping hot path of the CPU/MPI solver
+ * azhang22/3D_MPI_TEST.  Every entry point is extern "C" with plain pointers, sizes and
+ * POD structs; no C++ or torch types cross it.  Each declaration names the reference
+ * interface it replaces (paths relative to the reference repository, 3D_MPI_TEST/).
+ *
+ * Conventions
+ *   - Status: int, 0 = LEE_OK, < 0 = error.  A failed call leaves a message readable
+ *     with lee_last_error(ctx) (or lee_global_error() when no context exists).  Nothing
+ *     is printed and nothing "continues after an error" (the reference prints and
+ *     carries on: calculation_2D.cpp:49-52, 719, 862).
+ *   - Ownership: the library owns all device memory.  Host buffers passed in belong to
+ *     the caller and are only read/written during the call.
+ *   - Threading: one host thread per context; each context owns one HIP stream.
+ *     Distinct contexts may be used from distinct threads.
+ *   - Layout: 3-D fields are SoA, x (i) fastest: idx = i + nx*(j + ny*k).  Host-side
+ *     state buffers are FP64 regardless of the device precision.
+ */
+#ifndef LEE_ABI_H
+#define LEE_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LEE_ABI_VERSION 1
+
+enum {
+    LEE_OK = 0,
+    LEE_ERR_ARG = -1,         /* invalid argument / inconsistent deck */
+    LEE_ERR_DEVICE = -2,      /* HIP runtime error */
+    LEE_ERR_IO = -3,          /* file missing or malformed */
+    LEE_ERR_UNSUPPORTED = -4, /* deck selects a path the reference never implemented */
+    LEE_ERR_NONFINITE = -5,   /* NaN/Inf detected in the state by the health check */
+    LEE_ERR_STATE = -6        /* call out of order (e.g. step before frame setup) */
+};
+
+/* Boundary types of input.txt line 4 (airpore.cpp:48-159). */
+enum { LEE_BC_RIGID = 1, LEE_BC_ABSORBING = 2, LEE_BC_POROUS = 3, LEE_BC_RADIATION = 4, LEE_BC_TDBC = 5 };
+/* Face slots of input.txt line 4, in file order (airpore.cpp:48, 52-159). */
+enum { LEE_FACE_N = 0, LEE_FACE_S = 1, LEE_FACE_E = 2, LEE_FACE_W = 3, LEE_FACE_F = 4, LEE_FACE_B = 5 };
+
+/*
+ * lee_deck: the parsed drop-in input contract -- input.txt (airpore.cpp:17-316) plus the
+ * coordi_air{1,2,3}.dat / coordi_pore{1,2,3}.dat grid files (calculation_2D.cpp:44-68,
+ * 154-204).  Field names follow the reference members they hold.
+ */
+typedef struct lee_deck {
+    double gauss_width;                          /* L1  airpore.cpp:20 */
+    int    mf_jmax, mf_lead_dj, mf_trail_dj;     /* L2  airpore.cpp:21 (MovingFrame) */
+    int    mf_judge;
+    double mf_ratio;
+    int    scheme;                               /* L3  1 FB_v_p, 2 FB_p_v, 5 RK4 (airpore.cpp:24-43) */
+    int    bc[6];                                /* L4  N S E W F B types (airpore.cpp:48) */
+    int    time_domain, out_difft, fft_m;        /* L6  airpore.cpp:171 */
+    double frequency;
+    double velocity_coef;                        /* L8  airpore.cpp:175 */
+    int    velocity_method;
+    double gamma, Pav, c;                        /* L9  adiabatic_coef, Pav, sound_speed */
+    double pml_alpha[3], pml_R[3];               /* L10-12 x, y, z: alpha, resistivity */
+    int    pml_y_time;                           /* L11 PMLy time_step (y1 gate) */
+    double pore_q, pore_porosity, pore_resistivity; /* L13 PorePara.q (sqrt tortuosity), Omega, sigma */
+    double src[3], rcv[3];                       /* L14 source.xyz, receiver.xyz */
+    double dt;                                   /* L16 AirStep.diff_t */
+    double dx, dy, dz;                           /* L17 AirStep.diff_x/y/z */
+    int    mpi_dims[3];                          /* L19 (ignored: one GPU holds the domain) */
+    int    restart, restart_out;                 /* L22 (restart is not on the hot path) */
+    int    hill_judge;                           /* L24 first field (embedded obstacles) */
+    double pore_dx, pore_dy, pore_dz;            /* L32 SouthStep (read only if South = porous) */
+    /* coordi_air1.dat header: IMAX IMAX1 IMAX2 JMAX JMAX1 JMAX2 KMAX KMAX1 KMAX2 IJKMAX */
+    int    air_hdr[10];
+    const double *air_x, *air_y, *air_z;         /* IMAX, JMAX, KMAX coordinates */
+    int    has_pore;                             /* 1 when coordi_pore*.dat were read */
+    int    pore_hdr[10];
+    const double *pore_x, *pore_y, *pore_z;
+} lee_deck;
+
+/*
+ * lee_options: everything input.txt cannot say (the sidecar).  Zero-initialise and set
+ * what you need; lee_options_default() gives the reference behaviour.
+ */
+typedef struct lee_options {
+    int precision;          /* 64 (default, reference FP64) or 32 (FP32 state, FP64 coefficients) */
+    int n_receivers;        /* 0: the reference's receiver rule (airpore.cpp:413-458) */
+    const double *receivers;/* n_receivers * (x,y,z) when n_receivers > 0 */
+    int health_every;       /* check the state for NaN/Inf every N steps (0 = never) */
+    int use_graph;          /* 1: replay each step from a captured hipGraph */
+    int device;             /* HIP device ordinal */
+    int reserved[16];
+} lee_options;
+
+typedef struct lee_info {
+    int nx, ny, nz;                 /* air frame dimensions (IMAX, frame JMAX, KMAX) */
+    int pnx, pny, pnz;              /* porous dimensions (0 when South is not porous) */
+    int ijkmax;                     /* whole propagation length in j (moving frame) */
+    int n_receivers;
+    int precision;
+    int mf_count;                   /* frame set up last (-1 before lee_frame_setup) */
+    long long steps_done;           /* time steps advanced so far */
+    long long pml_y1_counter;       /* reference PML_JMAX1 step counter */
+} lee_info;
+
+typedef struct lee_ctx lee_ctx;
+
+/* Parse input.txt + coordi_* in `dir` (airpore::airpore airpore.cpp:17-316 and the grid
+ * reader in calculation_2D::calculation_2D calculation_2D.cpp:44-68, 154-204).  Arrays in
+ * `out` are owned by the deck; release them with lee_deck_free. */
+int  lee_deck_load(const char *dir, lee_deck *out);
+void lee_deck_free(lee_deck *deck);
+void lee_options_default(lee_options *opt);
+
+/* Build a solver context: validates the deck, derives every coefficient in FP64 on the
+ * host (Aav airpore.cpp:177, PML airpore.cpp:182-247 + air.cpp:509-937, porous
+ * airpore.cpp:249-254 + porous.cpp:40-51, TDBC calculation_2D.h:71-75), allocates the
+ * SoA fields in HBM and sets the Gaussian initial condition
+ * (calculation_2D::Set_InitialCond calculation_2D.cpp:300-314).
+ * Replaces: airpore::SetInitialCond airpore.cpp:936-964. */
+int  lee_create(const lee_deck *deck, const lee_options *opt, lee_ctx **out);
+int  lee_destroy(lee_ctx *ctx);
+const char *lee_last_error(const lee_ctx *ctx);
+const char *lee_global_error(void);
+int  lee_get_info(const lee_ctx *ctx, lee_info *info);
+
+/* Receiver grid indices (ii, jj, kk per receiver, whole-domain j), as located by
+ * calculation_2D::get_position (calculation_2D.cpp:317-352) and clamped as in
+ * airpore.cpp:454-457. */
+int  lee_receiver_indices(const lee_ctx *ctx, int *ijk, int cap);
+
+/* Set up moving-frame number mf_count: shift fields and PML auxiliaries, recompute
+ * coordinates, metrics and wind.  mf_count 0 is the initial frame (zeroes the PML
+ * auxiliaries).  Replaces airpore::SetMovingFrame airpore.cpp:881-897
+ * (calculation_2D::SetMovingFrame calculation_2D.cpp:405-496, air::SetWindProfile
+ * air.cpp:1286-1304, air::SetQMove air.cpp:2447-2730). */
+int  lee_frame_setup(lee_ctx *ctx, int mf_count);
+
+/* Advance nsteps time steps.  Before each step the receivers are sampled from p_n
+ * (airpore.cpp:573-580, calculation_2D::get_pressure calculation_2D.cpp:387-403; |p| <
+ * 1e-100 is written as 0).  rec_out (may be NULL) receives nsteps*n_receivers FP64
+ * samples, step-major.  One step is airpore::Cal_pressure + airpore::Cal_velocity +
+ * airpore::UpdateInitialCond (airpore.cpp:589-614). */
+int  lee_step(lee_ctx *ctx, int nsteps, double *rec_out);
+
+/* Number of steps the reference runs in frame mf_count given temp_time steps already
+ * done (the MF_limit rule of airpore.cpp:542-560), and the number of frames
+ * (airpore.cpp:624 loop bound). */
+int  lee_frame_steps(const lee_ctx *ctx, int mf_count, int temp_time);
+int  lee_frame_count(const lee_ctx *ctx);
+
+/* The whole time loop of airpore::get_output (airpore.cpp:539-624): for each frame,
+ * frame set-up then MF_limit steps, until time_domain steps.  rec_out receives
+ * time_domain*n_receivers samples (cap = its capacity in doubles). */
+int  lee_run(lee_ctx *ctx, double *rec_out, long long cap);
+
+/* Copy the current state (time level n) to or from the host as FP64 SoA arrays.
+ * domain 0 = air (nx*ny*nz), 1 = porous ground (pnx*pny*pnz). */
+int  lee_get_state(lee_ctx *ctx, int domain, double *u, double *v, double *w, double *p);
+int  lee_set_state(lee_ctx *ctx, int domain, const double *u, const double *v,
+                   const double *w, const double *p);
+/* TDBC history Psi on the East face, [i][k][6] (calculation_2D.cpp:128-134). */
+int  lee_get_psi(lee_ctx *ctx, double *psi, long long cap);
+
+/* Device-side time of the last lee_step call (HIP events on the context stream), ms. */
+double lee_last_step_ms(const lee_ctx *ctx);
+
+/* Output writers of the drop-in file contract.
+ * ./solution/pty%04d.dat: one %.6e sample per line (airpore.cpp:468-531, 577).
+ * ./solution/p_f2_%04d.dat: zero-padded radix-2 FFT, "f |X|" at setprecision(10)
+ * (airpore::get_FFT_y1 airpore.cpp:966-1011, FFT.cpp:15-121). */
+int  lee_write_receivers(const char *dir, const double *rec, int nsteps, int nrec);
+int  lee_write_spectra(const char *dir, const double *rec, int nsteps, int nrec,
+                       int fft_m, double dt);
+/* In-place radix-2 complex FFT of 2^m points, dir = 1 forward (scaled by 1/N),
+ * -1 inverse.  Replaces FFT() FFT.cpp:15-82. */
+int  lee_fft(int dir, int m, double *re, double *im);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LEE_ABI_H */

@@ -1,0 +1,246 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Loads ``oracle/liblee_oracle.so`` (the FP64 C restatement of the reference time step) and,
+when it has been built, runs ``oracle/_ref/ref_probe`` (the reference's own stencil code).
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg use this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblee_oracle.so")
+PROBE_PATH = os.path.join(HERE, "_ref", "ref_probe")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE, "liblee_oracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_create.restype = ctypes.c_void_p
+        L.orc_create.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int]
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_error.restype = ctypes.c_char_p
+        L.orc_dims.argtypes = [ctypes.c_void_p, _ip]
+        L.orc_receiver_indices.argtypes = [ctypes.c_void_p, _ip]
+        L.orc_frame_setup.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_frame_steps.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.orc_frame_count.argtypes = [ctypes.c_void_p]
+        L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_run.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_get_state.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_set_state.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_get_psi.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_air_dp.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.orc_air_duvw.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_pore_dp.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_pore_duvw.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_q_size.restype = ctypes.c_long
+        L.orc_q_size.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_q_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_q_set.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_update_q.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_tick_gate.argtypes = [ctypes.c_void_p]
+        L.orc_wind.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_fft.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp) if a is not None else _dp()
+
+
+class Oracle:
+    """One oracle simulation built from a ``Deck``."""
+
+    def __init__(self, deck, receivers=None, fill_nn_nan=False):
+        L = lib()
+        self.deck = deck
+        self._cdeck = deck.to_c()
+        rec = None
+        n = 0
+        if receivers is not None:
+            rec = np.ascontiguousarray(receivers, dtype=np.float64).reshape(-1)
+            n = rec.size // 3
+        self._rec = rec
+        h = L.orc_create(ctypes.byref(self._cdeck), n, _p(rec), 1 if fill_nn_nan else 0)
+        if not h:
+            raise ValueError("oracle: " + L.orc_error().decode())
+        self.h = h
+        dims = (ctypes.c_int * 8)()
+        L.orc_dims(h, dims)
+        self.nx, self.ny, self.nz, self.pnx, self.pny, self.pnz, self.ijkmax, self.nrec = list(dims)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    def receiver_indices(self):
+        out = (ctypes.c_int * (3 * self.nrec))()
+        lib().orc_receiver_indices(self.h, out)
+        return np.array(out, dtype=np.int64).reshape(-1, 3)
+
+    def frame_setup(self, n):
+        lib().orc_frame_setup(self.h, n)
+
+    def frame_steps(self, mf, temp_time):
+        return lib().orc_frame_steps(self.h, mf, temp_time)
+
+    def frame_count(self):
+        return lib().orc_frame_count(self.h)
+
+    def step(self, n):
+        rec = np.zeros((n, self.nrec))
+        lib().orc_step(self.h, n, _p(rec))
+        return rec
+
+    def run(self):
+        rec = np.zeros((self.deck.time_domain, self.nrec))
+        lib().orc_run(self.h, _p(rec))
+        return rec
+
+    def state(self, domain=0):
+        shp = self.shape if domain == 0 else (self.pnz, self.pny, self.pnx)
+        arrs = [np.zeros(shp) for _ in range(4)]
+        lib().orc_get_state(self.h, domain, *[_p(a) for a in arrs])
+        return arrs
+
+    def set_state(self, u, v, w, p, domain=0):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w, p)]
+        lib().orc_set_state(self.h, domain, *[_p(a) for a in arrs])
+
+    def psi(self):
+        out = np.zeros((self.nz, self.nx, 6))
+        lib().orc_get_psi(self.h, _p(out))
+        return out
+
+    def air_dp(self, u, v, w, p):
+        out = np.zeros(self.shape)
+        args = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w, p)]
+        lib().orc_air_dp(self.h, *[_p(a) for a in args], _p(out))
+        return out
+
+    def air_duvw(self, u, v, w, p):
+        outs = [np.zeros(self.shape) for _ in range(3)]
+        args = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w, p)]
+        lib().orc_air_duvw(self.h, *[_p(a) for a in args], *[_p(o) for o in outs])
+        return outs
+
+    def pore_dp(self, u, v, w):
+        out = np.zeros((self.pnz, self.pny, self.pnx))
+        args = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w)]
+        lib().orc_pore_dp(self.h, *[_p(a) for a in args], _p(out))
+        return out
+
+    def pore_duvw(self, u, v, w, p):
+        outs = [np.zeros((self.pnz, self.pny, self.pnx)) for _ in range(3)]
+        args = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w, p)]
+        lib().orc_pore_duvw(self.h, *[_p(a) for a in args], *[_p(o) for o in outs])
+        return outs
+
+    def q_get(self, slab, field):
+        n = lib().orc_q_size(self.h, slab)
+        out = np.zeros(n)
+        if n:
+            lib().orc_q_get(self.h, slab, field, _p(out))
+        return out
+
+    def q_set(self, slab, field, buf):
+        buf = np.ascontiguousarray(buf, dtype=np.float64)
+        lib().orc_q_set(self.h, slab, field, _p(buf))
+
+    def update_q(self, which, u=None, v=None, w=None, p=None):
+        args = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None for a in (u, v, w, p)]
+        lib().orc_update_q(self.h, which, *[_p(a) for a in args])
+
+    def tick_gate(self):
+        lib().orc_tick_gate(self.h)
+
+    def wind(self):
+        outs = [np.zeros(self.shape) for _ in range(3)]
+        lib().orc_wind(self.h, *[_p(o) for o in outs])
+        return outs
+
+
+def fft(x, y, m, direction=1):
+    x = np.array(x, dtype=np.float64, copy=True)
+    y = np.array(y, dtype=np.float64, copy=True)
+    lib().orc_fft(direction, m, _p(x), _p(y))
+    return x, y
+
+
+# --------------------------------------------------------------------------- reference probe
+
+def probe_available() -> bool:
+    return os.path.exists(PROBE_PATH)
+
+
+def _rec(fh, arr):
+    arr = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+    fh.write(struct.pack("<i", arr.size))
+    fh.write(arr.tobytes())
+
+
+def run_probe(deck, fields, fields2, frame=0, counter=0, seed=7, pore=(1.7320508075688772, 0.3, 2.0e5),
+              fft_m=6, fft_re=None, fft_im=None, fft_file=None):
+    """Run the reference's own stencil code on `fields` (u, v, w, p) and return a dict of
+    named arrays (see oracle/ref_probe.cpp for the record list)."""
+    nx, ny, nz = len(deck.air_x), len(deck.air_y), len(deck.air_z)
+    hdr = deck.air_hdr()
+    ints = [nx, ny, nz, *hdr, *deck.bc, deck.velocity_method, frame, counter, deck.pml_y_time,
+            int(deck.mf_jmax), deck.mf_lead_dj, deck.mf_trail_dj, seed]
+    dbl = [deck.dt, deck.dx, deck.dy, deck.dz, deck.gamma, deck.Pav, deck.c, deck.velocity_coef,
+           *deck.pml_alpha, *deck.pml_R]
+    n = 1 << fft_m
+    if fft_re is None:
+        rng = np.random.default_rng(seed)
+        fft_re = rng.standard_normal(n)
+        fft_im = np.zeros(n)
+    with tempfile.TemporaryDirectory() as td:
+        cin, cout = os.path.join(td, "case.bin"), os.path.join(td, "out.bin")
+        with open(cin, "wb") as fh:
+            _rec(fh, np.array(ints, dtype=np.float64))
+            _rec(fh, np.array(dbl))
+            _rec(fh, deck.air_x); _rec(fh, deck.air_y); _rec(fh, deck.air_z)
+            for a in fields:
+                _rec(fh, a)
+            for a in fields2:
+                _rec(fh, a)
+            _rec(fh, np.array(pore, dtype=np.float64))
+            _rec(fh, np.array([fft_m], dtype=np.float64))
+            _rec(fh, fft_re); _rec(fh, fft_im)
+            _rec(fh, fft_re); _rec(fh, fft_im)
+        cmd = [PROBE_PATH, cin, cout] + ([fft_file] if fft_file else [])
+        subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, timeout=300)
+        out = {}
+        with open(cout, "rb") as fh:
+            data = fh.read()
+        off = 0
+        while off < len(data):
+            (ln,) = struct.unpack_from("<i", data, off); off += 4
+            name = data[off:off + ln].decode(); off += ln
+            (cnt,) = struct.unpack_from("<q", data, off); off += 8
+            out[name] = np.frombuffer(data, dtype=np.float64, count=cnt, offset=off).copy(); off += 8 * cnt
+    return out
