@@ -1,0 +1,257 @@
+"""MI355X-native time-stepping hot path of the 3D_MPI_TEST outdoor-sound FDTD solver.
+
+The compute path is ``liblee_hip.so`` (hand-written HIP kernels for gfx950 behind the C ABI
+of ``include/lee_abi.h``).  This module is a thin ctypes front end over that ABI:
+
+* :class:`Solver` -- one domain on one GPU (``lee_create`` ... ``lee_destroy``);
+* :class:`AirPore` -- mirror of the reference driver ``airpore`` (airpore.h:18-67):
+  ``AirPore(input_dir).get_output()`` runs the whole time loop and writes
+  ``solution/pty%04d.dat``; ``get_FFT_y1()`` writes ``solution/p_f2_%04d.dat``.
+
+There is no CPU fallback: if the library is missing, or no GPU is visible, the calls fail.
+"""
+from __future__ import annotations
+
+import ctypes
+import importlib
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblee_hip.so")
+
+deck = importlib.import_module(__name__ + ".deck")
+LeeDeck = deck.LeeDeck
+
+LEE_OK = 0
+ERRORS = {-1: "LEE_ERR_ARG", -2: "LEE_ERR_DEVICE", -3: "LEE_ERR_IO", -4: "LEE_ERR_UNSUPPORTED",
+          -5: "LEE_ERR_NONFINITE", -6: "LEE_ERR_STATE"}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class LeeOptions(ctypes.Structure):
+    _fields_ = [
+        ("precision", ctypes.c_int),
+        ("n_receivers", ctypes.c_int),
+        ("receivers", _dp),
+        ("health_every", ctypes.c_int),
+        ("use_graph", ctypes.c_int),
+        ("device", ctypes.c_int),
+        ("reserved", ctypes.c_int * 16),
+    ]
+
+
+class LeeInfo(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("pnx", ctypes.c_int), ("pny", ctypes.c_int), ("pnz", ctypes.c_int),
+        ("ijkmax", ctypes.c_int), ("n_receivers", ctypes.c_int), ("precision", ctypes.c_int),
+        ("mf_count", ctypes.c_int),
+        ("steps_done", ctypes.c_longlong), ("pml_y1_counter", ctypes.c_longlong),
+    ]
+
+
+class LeeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (fails loudly when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.lee_deck_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(LeeDeck)]
+        L.lee_deck_free.argtypes = [ctypes.POINTER(LeeDeck)]
+        L.lee_deck_free.restype = None
+        L.lee_options_default.argtypes = [ctypes.POINTER(LeeOptions)]
+        L.lee_options_default.restype = None
+        L.lee_create.argtypes = [ctypes.POINTER(LeeDeck), ctypes.POINTER(LeeOptions), ctypes.POINTER(vp)]
+        L.lee_destroy.argtypes = [vp]
+        L.lee_last_error.argtypes = [vp]
+        L.lee_last_error.restype = ctypes.c_char_p
+        L.lee_global_error.restype = ctypes.c_char_p
+        L.lee_get_info.argtypes = [vp, ctypes.POINTER(LeeInfo)]
+        L.lee_receiver_indices.argtypes = [vp, _ip, ctypes.c_int]
+        L.lee_frame_setup.argtypes = [vp, ctypes.c_int]
+        L.lee_step.argtypes = [vp, ctypes.c_int, _dp]
+        L.lee_frame_steps.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.lee_frame_count.argtypes = [vp]
+        L.lee_run.argtypes = [vp, _dp, ctypes.c_longlong]
+        L.lee_get_state.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.lee_set_state.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.lee_get_psi.argtypes = [vp, _dp, ctypes.c_longlong]
+        L.lee_last_step_ms.argtypes = [vp]
+        L.lee_last_step_ms.restype = ctypes.c_double
+        L.lee_write_receivers.argtypes = [ctypes.c_char_p, _dp, ctypes.c_int, ctypes.c_int]
+        L.lee_write_spectra.argtypes = [ctypes.c_char_p, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.lee_fft.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp) if a is not None else _dp()
+
+
+def _check(rc, ctx=None):
+    if rc != LEE_OK:
+        L = lib()
+        msg = (L.lee_last_error(ctx) if ctx else L.lee_global_error()) or b""
+        raise LeeError(rc, msg.decode(errors="replace"))
+    return rc
+
+
+def load_deck(dirpath: str) -> LeeDeck:
+    """Parse input.txt + coordi_* with the library's parser (a ``LeeDeck`` that owns its arrays;
+    free with ``free_deck``)."""
+    d = LeeDeck()
+    _check(lib().lee_deck_load(dirpath.encode(), ctypes.byref(d)))
+    return d
+
+
+def free_deck(d: LeeDeck) -> None:
+    lib().lee_deck_free(ctypes.byref(d))
+
+
+class Solver:
+    """One solver context on one GPU (C ABI of include/lee_abi.h)."""
+
+    def __init__(self, d, precision: int = 64, receivers=None, health_every: int = 0, device: int = 0):
+        L = lib()
+        if isinstance(d, deck.Deck):
+            self._deck_obj = d
+            self._cdeck = d.to_c()
+        else:
+            self._deck_obj = None
+            self._cdeck = d
+        opt = LeeOptions()
+        L.lee_options_default(ctypes.byref(opt))
+        opt.precision = precision
+        opt.health_every = health_every
+        opt.device = device
+        self._rec = None
+        if receivers is not None:
+            self._rec = np.ascontiguousarray(receivers, dtype=np.float64).reshape(-1)
+            opt.n_receivers = self._rec.size // 3
+            opt.receivers = _p(self._rec)
+        self._opt = opt
+        h = ctypes.c_void_p()
+        _check(L.lee_create(ctypes.byref(self._cdeck), ctypes.byref(opt), ctypes.byref(h)))
+        self.h = h
+        info = self.info()
+        self.nx, self.ny, self.nz = info.nx, info.ny, info.nz
+        self.pnx, self.pny, self.pnz = info.pnx, info.pny, info.pnz
+        self.nrec = info.n_receivers
+        self.time_domain = self._cdeck.time_domain
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().lee_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> LeeInfo:
+        i = LeeInfo()
+        _check(lib().lee_get_info(self.h, ctypes.byref(i)), self.h)
+        return i
+
+    def receiver_indices(self):
+        out = (ctypes.c_int * (3 * max(self.nrec, 1)))()
+        lib().lee_receiver_indices(self.h, out, 3 * self.nrec)
+        return np.array(out, dtype=np.int64)[: 3 * self.nrec].reshape(-1, 3)
+
+    def frame_setup(self, n: int) -> None:
+        _check(lib().lee_frame_setup(self.h, n), self.h)
+
+    def frame_steps(self, mf: int, temp_time: int) -> int:
+        return lib().lee_frame_steps(self.h, mf, temp_time)
+
+    def frame_count(self) -> int:
+        return lib().lee_frame_count(self.h)
+
+    def step(self, n: int, record: bool = True) -> Optional[np.ndarray]:
+        rec = np.zeros((n, self.nrec)) if record else None
+        _check(lib().lee_step(self.h, n, _p(rec)), self.h)
+        return rec
+
+    def run(self) -> np.ndarray:
+        rec = np.zeros((self.time_domain, self.nrec))
+        _check(lib().lee_run(self.h, _p(rec), rec.size), self.h)
+        return rec
+
+    def state(self, domain: int = 0):
+        shp = (self.nz, self.ny, self.nx) if domain == 0 else (self.pnz, self.pny, self.pnx)
+        arrs = [np.zeros(shp) for _ in range(4)]
+        _check(lib().lee_get_state(self.h, domain, *[_p(a) for a in arrs]), self.h)
+        return arrs
+
+    def set_state(self, u, v, w, p, domain: int = 0):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None for a in (u, v, w, p)]
+        _check(lib().lee_set_state(self.h, domain, *[_p(a) for a in arrs]), self.h)
+
+    def psi(self):
+        out = np.zeros((self.nz, self.nx, 6))
+        _check(lib().lee_get_psi(self.h, _p(out), out.size), self.h)
+        return out
+
+    def last_ms(self) -> float:
+        return lib().lee_last_step_ms(self.h)
+
+
+class AirPore:
+    """Mirror of the reference driver class ``airpore`` (airpore.h:18-67).
+
+    ``AirPore(dir)`` reads ``dir/input.txt`` and the ``coordi_*`` files like
+    ``airpore::airpore`` (airpore.cpp:17-316); ``get_output()`` runs the time loop of
+    ``airpore::get_output`` (airpore.cpp:400-641) on the GPU and writes the receiver series;
+    ``get_FFT_y1()`` writes the spectra like ``airpore::get_FFT_y1`` (airpore.cpp:966-1011).
+    """
+
+    def __init__(self, input_dir: str, precision: int = 64, receivers=None):
+        self.dir = input_dir
+        self._deck = load_deck(input_dir)
+        self.solver = Solver(self._deck, precision=precision, receivers=receivers)
+        self.series = None
+
+    def get_output(self) -> np.ndarray:
+        self.series = self.solver.run()
+        s = np.ascontiguousarray(self.series)
+        _check(lib().lee_write_receivers(self.dir.encode(), _p(s), s.shape[0], s.shape[1]))
+        return self.series
+
+    def get_FFT_y1(self) -> None:
+        if self.series is None:
+            raise RuntimeError("get_output must run first")
+        if self._deck.rcv[1] == 0:  # the reference writes spectra only when receiver.y != 0
+            return
+        s = np.ascontiguousarray(self.series)
+        _check(lib().lee_write_spectra(self.dir.encode(), _p(s), s.shape[0], s.shape[1],
+                                       self._deck.fft_m, self._deck.dt))
+
+    def close(self):
+        self.solver.close()
+        free_deck(self._deck)
+
+
+def fft(re, im, m: int, direction: int = 1):
+    re = np.array(re, dtype=np.float64, copy=True)
+    im = np.array(im, dtype=np.float64, copy=True)
+    _check(lib().lee_fft(direction, m, _p(re), _p(im)))
+    return re, im

@@ -1,0 +1,552 @@
+// lee_ctx.hip -- solver context and the C ABI of include/lee_abi.h.
+//
+// One context = one domain in HBM on one device, one HIP stream.  The host keeps the
+// FP64 model (lee_model.cpp) and drives the kernels of lee_kernels.hip in the phase order
+// of airpore::Cal_pressure / Cal_velocity (airpore.cpp:324-398).  Time levels n and n+1
+// are two buffer sets whose roles swap after every step instead of the reference's full
+// nn -> n copy (calculation_2D.cpp:354-369): every point of level n+1 is rewritten by the
+// interior update or by a face condition before it is read.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/lee_abi.h"
+#include "lee_kernels.h"
+#include "lee_model.h"
+
+namespace lee {
+thread_local std::string g_global_err;
+void set_global_error(const std::string &m) { g_global_err = m; }
+}  // namespace lee
+
+using namespace lee;
+
+namespace {
+
+constexpr int kRecChunk = 4096;  // receiver samples buffered on the device between copies
+
+struct DevArr {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct lee_ctx {
+    Model m;
+    int prec = 64;
+    size_t esz = 8;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    std::vector<DevArr> allocs;
+    // fields: [0..3] level A (u v w p), [4..7] level B; `cur` selects which is level n
+    void *air[8] = {};
+    void *pore[8] = {};
+    int cur = 0;
+    void *U = nullptr, *V = nullptr, *W = nullptr;
+    void *mx = nullptr, *my = nullptr, *mz = nullptr, *pmx = nullptr, *pmy = nullptr, *pmz = nullptr;
+    void *prof[6][3] = {};
+    void *Q[6][4] = {};
+    double *psi = nullptr;
+    long long *rec_idx = nullptr;
+    double *rec_buf = nullptr;
+    int *flag = nullptr;
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    int mf_count = -1;
+    long long steps = 0, counter = 0;
+    float last_ms = 0.f;
+    std::vector<double> host_rec;
+
+    ~lee_ctx()
+    {
+        // teardown: errors cannot be reported from a destructor
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &a : allocs) if (a.p) (void)hipFree(a.p);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int set_err(lee_ctx *c, int code, const std::string &m)
+{
+    if (c) c->err = m;
+    set_global_error(m);
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                         \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return set_err((c), LEE_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int dalloc(lee_ctx *c, void **out, size_t bytes)
+{
+    *out = nullptr;
+    if (bytes == 0) return LEE_OK;
+    void *p = nullptr;
+    HIPCHK(c, hipMalloc(&p, bytes));
+    HIPCHK(c, hipMemsetAsync(p, 0, bytes, c->stream));
+    c->allocs.push_back({p, bytes});
+    *out = p;
+    return LEE_OK;
+}
+
+// Upload FP64 host values as the device type.
+int upload(lee_ctx *c, void *dst, const double *src, size_t n)
+{
+    if (!n) return LEE_OK;
+    // set-up path only: wait for the copy so the host buffer may be released on return
+    if (c->prec == 64) {
+        HIPCHK(c, hipMemcpyAsync(dst, src, n * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<float> tmp(src, src + n);
+        HIPCHK(c, hipMemcpyAsync(dst, tmp.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return LEE_OK;
+}
+
+int download(lee_ctx *c, double *dst, const void *src, size_t n)
+{
+    if (!n || !dst) return LEE_OK;
+    if (c->prec == 64) {
+        HIPCHK(c, hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<float> tmp(n);
+        HIPCHK(c, hipMemcpyAsync(tmp.data(), src, n * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < n; ++i) dst[i] = tmp[i];
+    }
+    return LEE_OK;
+}
+
+size_t air_n(const lee_ctx *c) { return (size_t)c->m.nx * c->m.ny * c->m.nz; }
+size_t pore_n(const lee_ctx *c) { return (size_t)c->m.pnx * c->m.pny * c->m.pnz; }
+
+template <typename T> Grid<T> air_grid(lee_ctx *c)
+{
+    Grid<T> g;
+    const Model &m = c->m;
+    int n = c->cur * 4, o = 4 - n;
+    g.nx = m.nx; g.ny = m.ny; g.nz = m.nz;
+    g.un = (T *)c->air[n + 0]; g.vn = (T *)c->air[n + 1]; g.wn = (T *)c->air[n + 2]; g.pn = (T *)c->air[n + 3];
+    g.unn = (T *)c->air[o + 0]; g.vnn = (T *)c->air[o + 1]; g.wnn = (T *)c->air[o + 2]; g.pnn = (T *)c->air[o + 3];
+    g.mx = (const T *)c->mx; g.my = (const T *)c->my; g.mz = (const T *)c->mz;
+    g.dxi = (T)m.dxi; g.dyi = (T)m.dyi; g.dzi = (T)m.dzi; g.dt = (T)m.dt;
+    return g;
+}
+
+template <typename T> Grid<T> pore_grid(lee_ctx *c)
+{
+    Grid<T> g;
+    const Model &m = c->m;
+    int n = c->cur * 4, o = 4 - n;
+    g.nx = m.pnx; g.ny = m.pny; g.nz = m.pnz;
+    g.un = (T *)c->pore[n + 0]; g.vn = (T *)c->pore[n + 1]; g.wn = (T *)c->pore[n + 2]; g.pn = (T *)c->pore[n + 3];
+    g.unn = (T *)c->pore[o + 0]; g.vnn = (T *)c->pore[o + 1]; g.wnn = (T *)c->pore[o + 2]; g.pnn = (T *)c->pore[o + 3];
+    g.mx = (const T *)c->pmx; g.my = (const T *)c->pmy; g.mz = (const T *)c->pmz;
+    g.dxi = (T)m.pdxi; g.dyi = (T)m.pdyi; g.dzi = (T)m.pdzi; g.dt = (T)m.dt;
+    return g;
+}
+
+template <typename T> AirArgs<T> air_args(lee_ctx *c, int J1)
+{
+    const Model &m = c->m;
+    AirArgs<T> a;
+    a.g = air_grid<T>(c);
+    a.U = (const T *)c->U; a.V = (const T *)c->V; a.W = (const T *)c->W;
+    a.gam = (T)m.gamma; a.Pav = (T)m.Pav; a.gP = (T)m.gP; a.Aav = (T)m.Aav;
+    a.I1 = m.I1; a.I2 = m.I2; a.J1 = J1; a.J2 = m.J2; a.K1 = m.K1; a.K2 = m.K2;
+    DProf<T> *pr[6] = {&a.x1, &a.x2, &a.y1, &a.y2, &a.z1, &a.z2};
+    for (int s = 0; s < 6; ++s) {
+        pr[s]->s = (const T *)c->prof[s][0]; pr[s]->c = (const T *)c->prof[s][1]; pr[s]->d = (const T *)c->prof[s][2];
+        for (int f = 0; f < 4; ++f) a.Q[s][f] = (T *)c->Q[s][f];
+        a.qn[s] = m.slab[s].n;
+    }
+    return a;
+}
+
+// Upload the per-frame tables (metrics, PML profiles, wind) of frame N.
+int upload_frame(lee_ctx *c, int N)
+{
+    const Model &m = c->m;
+    FrameTables t;
+    m.frame_tables(N, t);
+    auto metric = [&](void *dst, const std::vector<double> &v) {
+        if (c->prec == 64) return upload(c, dst, v.data(), v.size());
+        std::vector<double> r(v.size(), 0.0);
+        for (size_t i = 1; i + 1 < v.size(); ++i) r[i] = 1.0 / v[i];  // separable metric in FP64
+        return upload(c, dst, r.data(), r.size());
+    };
+    int rc;
+    if ((rc = metric(c->mx, t.Xx)) || (rc = metric(c->my, t.Yy)) || (rc = metric(c->mz, t.Zz))) return rc;
+    if (m.south_porous)
+        if ((rc = metric(c->pmx, t.pXx)) || (rc = metric(c->pmy, t.pYy)) || (rc = metric(c->pmz, t.pZz))) return rc;
+    const Profile *P[6] = {&t.x1, &t.x2, &t.y1, &t.y2, &t.z1, &t.z2};
+    for (int s = 0; s < 6; ++s) {
+        if ((rc = upload(c, c->prof[s][0], P[s]->s.data(), P[s]->s.size()))) return rc;
+        if ((rc = upload(c, c->prof[s][1], P[s]->c.data(), P[s]->c.size()))) return rc;
+        if ((rc = upload(c, c->prof[s][2], P[s]->d.data(), P[s]->d.size()))) return rc;
+    }
+    if (m.has_wind) {
+        if ((rc = upload(c, c->U, t.U.data(), t.U.size())) || (rc = upload(c, c->V, t.V.data(), t.V.size())) ||
+            (rc = upload(c, c->W, t.W.data(), t.W.size())))
+            return rc;
+    }
+    // receiver linear indices in this frame
+    int nr = (int)m.ri.size();
+    std::vector<long long> idx(nr);
+    for (int r = 0; r < nr; ++r) idx[r] = m.receiver_index(r, N);
+    if (nr) HIPCHK(c, hipMemcpyAsync(c->rec_idx, idx.data(), nr * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LEE_OK;
+}
+
+template <typename T> int frame_shift(lee_ctx *c, int N)
+{
+    const Model &m = c->m;
+    int tempJ = m.d.mf_jmax - m.lead;  // calculation_2D.cpp:459 (temp_J of frame N > 0)
+    if (N == 0 || tempJ == 0) return LEE_OK;
+    int s = m.trail - 1;
+    int n = c->cur * 4, o = 4 - n;
+    // fields: level n -> (free) level n+1 buffers, then swap roles
+    for (int f = 0; f < 4; ++f)
+        launch_shift<T>((T *)c->air[o + f], (const T *)c->air[n + f], m.nx, m.ny, m.nz, tempJ, m.lead, s, c->stream);
+    if (m.south_porous)
+        for (int f = 0; f < 4; ++f)
+            launch_shift<T>((T *)c->pore[o + f], (const T *)c->pore[n + f], m.pnx, m.pny, m.pnz, tempJ, m.lead, s, c->stream);
+    c->cur ^= 1;
+    n = c->cur * 4;
+    launch_copy_plane_j<T>((T *)c->air[n + 3], m.nx, m.ny, m.nz, c->stream);
+    if (m.south_porous) launch_copy_plane_j<T>((T *)c->pore[n + 3], m.pnx, m.pny, m.pnz, c->stream);
+    // PML auxiliaries of the x and z layers (air.cpp:2522-2729); y layers stay
+    for (int sl : {0, 1, 4, 5}) {
+        const SlabGeom &g = m.slab[sl];
+        if (!g.n) continue;
+        int A = g.axis == 0 ? g.n : m.nx, C = g.axis == 2 ? g.n : m.nz;
+        size_t cnt = (size_t)A * m.ny * C;
+        for (int f = 0; f < 4; ++f) {
+            HIPCHK(c, hipMemcpyAsync(c->scratch, c->Q[sl][f], cnt * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+            launch_shift<T>((T *)c->Q[sl][f], (const T *)c->scratch, A, m.ny, C, tempJ, m.lead, s, c->stream);
+        }
+    }
+    HIPCHK(c, hipGetLastError());
+    return LEE_OK;
+}
+
+template <typename T> int run_steps(lee_ctx *c, int nsteps, double *rec_out)
+{
+    Model &m = c->m;
+    hipStream_t s = c->stream;
+    const int nr = (int)m.ri.size();
+    const bool gate = (m.judge[1] == 1 || m.judge[1] == 3);
+    HIPCHK(c, hipEventRecord(c->ev0, s));
+    int buffered = 0, written = 0;
+    for (int t = 0; t < nsteps; ++t) {
+        // receivers before the step (airpore.cpp:573-580)
+        launch_sample<T>((const T *)c->air[c->cur * 4 + 3], c->rec_idx, nr, c->rec_buf + (size_t)buffered * nr, s);
+        ++buffered;
+        // y1 time gate (air.cpp:1965-1969)
+        int J1 = m.J1;
+        if (gate) {
+            c->counter += 1;
+            J1 = c->counter >= m.pml_time ? m.J3 : 0;
+        }
+        const bool y1on = m.slab[2].n > 0 && gate && c->counter >= m.pml_time;
+        AirArgs<T> a = air_args<T>(c, J1);
+        Grid<T> &g = a.g;
+        // ---- pressure phase (airpore.cpp:366-398)
+        launch_air_p<T>(a, m.has_wind, s);
+        for (int f : {F_N, F_E, F_W, F_F, F_B}) launch_bc_p<T>(g, f, s);
+        if (!m.south_porous) {
+            launch_bc_p<T>(g, F_S, s);
+        } else {
+            PoreArgs<T> pa{pore_grid<T>(c), (T)m.p_gama, (T)m.p_invbeta, (T)m.p_eff};
+            launch_pore_p<T>(pa, s);
+            for (int f : {F_S, F_E, F_W, F_F, F_B}) launch_bc_p<T>(pa.g, f, s);
+            if (m.iface_copy) launch_iface<T>(g, pa.g, 0, s);
+        }
+        for (int sl = 0; sl < 6; ++sl) {
+            const SlabGeom &sg = m.slab[sl];
+            if (!sg.n || (sl == 2 && !y1on)) continue;
+            launch_qadd<T>((T *)c->Q[sl][3], g.pnn, sg.axis, sg.lo, sg.n, m.nx, m.ny, m.nz, (T)m.dt, s);
+        }
+        // ---- velocity phase (airpore.cpp:324-364)
+        launch_air_v<T>(a, m.has_wind, s);
+        launch_bc_v<T>(g, F_N, s);
+        if (m.east_tdbc) {
+            TdbcArgs<T> ta;
+            ta.g = g; ta.psi = c->psi;
+            for (int q = 0; q < 6; ++q) { ta.decay[q] = m.td_decay[q]; ta.gain[q] = m.td_gain[q]; }
+            ta.inv = m.td_inv; ta.Zinf = m.Zinf; ta.tb = m.dt / m.tau;
+            launch_tdbc<T>(ta, s);
+        } else {
+            launch_bc_v<T>(g, F_E, s);
+        }
+        for (int f : {F_W, F_F, F_B}) launch_bc_v<T>(g, f, s);
+        if (!m.south_porous) {
+            launch_bc_v<T>(g, F_S, s);
+        } else {
+            PoreArgs<T> pa{pore_grid<T>(c), (T)m.p_gama, (T)m.p_invbeta, (T)m.p_eff};
+            launch_pore_v<T>(pa, s);
+            for (int f : {F_S, F_E, F_W, F_F, F_B}) launch_bc_v<T>(pa.g, f, s);
+            if (m.iface_copy) launch_iface<T>(g, pa.g, 1, s);
+        }
+        const T *src[3] = {g.unn, g.vnn, g.wnn};
+        for (int sl = 0; sl < 6; ++sl) {
+            const SlabGeom &sg = m.slab[sl];
+            if (!sg.n || (sl == 2 && !y1on)) continue;
+            for (int f = 0; f < 3; ++f)
+                launch_qadd<T>((T *)c->Q[sl][f], src[f], sg.axis, sg.lo, sg.n, m.nx, m.ny, m.nz, (T)m.dt, s);
+        }
+        // ---- level n+1 becomes level n
+        c->cur ^= 1;
+        c->steps += 1;
+        if (buffered == kRecChunk || t == nsteps - 1) {
+            if (rec_out && nr) {
+                HIPCHK(c, hipMemcpyAsync(rec_out + (size_t)written * nr, c->rec_buf, (size_t)buffered * nr * 8,
+                                         hipMemcpyDeviceToHost, s));
+                HIPCHK(c, hipStreamSynchronize(s));
+            }
+            written += buffered;
+            buffered = 0;
+        }
+        if (m.opt.health_every > 0 && c->steps % m.opt.health_every == 0) {
+            HIPCHK(c, hipMemsetAsync(c->flag, 0, sizeof(int), s));
+            int n = c->cur * 4;
+            launch_finite_check<T>((const T *)c->air[n + 3], air_n(c), c->flag, s);
+            int h = 0;
+            HIPCHK(c, hipMemcpyAsync(&h, c->flag, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPCHK(c, hipStreamSynchronize(s));
+            if (h) return set_err(c, LEE_ERR_NONFINITE, "non-finite pressure after step " + std::to_string(c->steps));
+        }
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, s));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    return LEE_OK;
+}
+
+int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
+{
+    std::unique_ptr<lee_ctx> c(new lee_ctx());
+    lee_options o;
+    lee_options_default(&o);
+    if (opt) o = *opt;
+    if (o.precision != 64 && o.precision != 32) return set_err(nullptr, LEE_ERR_ARG, "precision must be 32 or 64");
+    std::string e = c->m.build(*deck, o);
+    if (!e.empty()) {
+        bool unsup = e.find("not") != std::string::npos || e.find("support") != std::string::npos ||
+                     e.find("uninitialised") != std::string::npos;
+        return set_err(nullptr, unsup ? LEE_ERR_UNSUPPORTED : LEE_ERR_ARG, e);
+    }
+    lee_ctx *C = c.get();
+    C->prec = o.precision;
+    C->esz = o.precision == 64 ? 8 : 4;
+    HIPCHK(C, hipSetDevice(o.device));
+    HIPCHK(C, hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+    HIPCHK(C, hipEventCreate(&C->ev0));
+    HIPCHK(C, hipEventCreate(&C->ev1));
+    const Model &m = C->m;
+    size_t na = air_n(C), np = pore_n(C), es = C->esz;
+    int rc;
+    for (int f = 0; f < 8; ++f)
+        if ((rc = dalloc(C, &C->air[f], na * es))) return rc;
+    if (m.south_porous)
+        for (int f = 0; f < 8; ++f)
+            if ((rc = dalloc(C, &C->pore[f], np * es))) return rc;
+    if (m.has_wind)
+        if ((rc = dalloc(C, &C->U, na * es)) || (rc = dalloc(C, &C->V, na * es)) || (rc = dalloc(C, &C->W, na * es))) return rc;
+    if ((rc = dalloc(C, &C->mx, m.nx * es)) || (rc = dalloc(C, &C->my, m.ny * es)) || (rc = dalloc(C, &C->mz, m.nz * es))) return rc;
+    if (m.south_porous)
+        if ((rc = dalloc(C, &C->pmx, m.pnx * es)) || (rc = dalloc(C, &C->pmy, m.pny * es)) || (rc = dalloc(C, &C->pmz, m.pnz * es)))
+            return rc;
+    int len[6] = {m.nx, m.nx, m.ny, m.ny, m.nz, m.nz};
+    size_t qmax = 0;
+    for (int s = 0; s < 6; ++s) {
+        for (int k = 0; k < 3; ++k)
+            if ((rc = dalloc(C, &C->prof[s][k], (size_t)len[s] * es))) return rc;
+        size_t cnt = m.slab[s].count(m.nx, m.ny, m.nz);
+        qmax = std::max(qmax, cnt);
+        for (int f = 0; f < 4; ++f)
+            if ((rc = dalloc(C, &C->Q[s][f], cnt * es))) return rc;
+    }
+    if ((rc = dalloc(C, (void **)&C->psi, (size_t)m.nx * m.nz * 6 * 8))) return rc;
+    int nr = (int)m.ri.size();
+    if ((rc = dalloc(C, (void **)&C->rec_idx, (size_t)std::max(nr, 1) * sizeof(long long)))) return rc;
+    if ((rc = dalloc(C, (void **)&C->rec_buf, (size_t)kRecChunk * std::max(nr, 1) * 8))) return rc;
+    if ((rc = dalloc(C, (void **)&C->flag, sizeof(int)))) return rc;
+    C->scratch_bytes = std::max<size_t>(qmax * es, 8);
+    if ((rc = dalloc(C, &C->scratch, C->scratch_bytes))) return rc;
+    // initial condition (calculation_2D.cpp:300-314): velocities zero, Gaussian pressure
+    std::vector<double> p0;
+    m.initial_condition(p0, false);
+    if ((rc = upload(C, C->air[3], p0.data(), p0.size()))) return rc;
+    if (m.south_porous) {
+        m.initial_condition(p0, true);
+        if ((rc = upload(C, C->pore[3], p0.data(), p0.size()))) return rc;
+    }
+    HIPCHK(C, hipStreamSynchronize(C->stream));
+    *out = c.release();
+    return LEE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lee_global_error(void) { return g_global_err.c_str(); }
+
+const char *lee_last_error(const lee_ctx *ctx) { return ctx ? ctx->err.c_str() : g_global_err.c_str(); }
+
+int lee_create(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
+{
+    if (!deck || !out) return set_err(nullptr, LEE_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    try {
+        return create_impl(deck, opt, out);
+    } catch (const std::exception &e) {
+        return set_err(nullptr, LEE_ERR_ARG, std::string("lee_create: ") + e.what());
+    }
+}
+
+int lee_destroy(lee_ctx *ctx)
+{
+    delete ctx;
+    return LEE_OK;
+}
+
+int lee_get_info(const lee_ctx *c, lee_info *info)
+{
+    if (!c || !info) return LEE_ERR_ARG;
+    const Model &m = c->m;
+    info->nx = m.nx; info->ny = m.ny; info->nz = m.nz;
+    info->pnx = m.pnx; info->pny = m.pny; info->pnz = m.pnz;
+    info->ijkmax = m.ijkmax;
+    info->n_receivers = (int)m.ri.size();
+    info->precision = c->prec;
+    info->mf_count = c->mf_count;
+    info->steps_done = c->steps;
+    info->pml_y1_counter = c->counter;
+    return LEE_OK;
+}
+
+int lee_receiver_indices(const lee_ctx *c, int *ijk, int cap)
+{
+    if (!c || !ijk) return LEE_ERR_ARG;
+    int n = (int)c->m.ri.size();
+    for (int r = 0; r < n && 3 * r + 2 < cap; ++r) {
+        ijk[3 * r] = c->m.ri[r]; ijk[3 * r + 1] = c->m.rj[r]; ijk[3 * r + 2] = c->m.rk[r];
+    }
+    return n;
+}
+
+int lee_frame_setup(lee_ctx *c, int N)
+{
+    if (!c || N < 0) return set_err(c, LEE_ERR_ARG, "bad frame number");
+    if (N >= c->m.frame_count()) return set_err(c, LEE_ERR_ARG, "frame number beyond the last frame");
+    int rc = c->prec == 64 ? frame_shift<double>(c, N) : frame_shift<float>(c, N);
+    if (rc) return rc;
+    if (N == 0)  // air::SetQMove(0) zeroes every auxiliary (air.cpp:2453-2519)
+        for (int s = 0; s < 6; ++s)
+            for (int f = 0; f < 4; ++f) {
+                size_t cnt = c->m.slab[s].count(c->m.nx, c->m.ny, c->m.nz);
+                if (cnt) HIPCHK(c, hipMemsetAsync(c->Q[s][f], 0, cnt * c->esz, c->stream));
+            }
+    rc = upload_frame(c, N);
+    if (rc) return rc;
+    c->mf_count = N;
+    return LEE_OK;
+}
+
+int lee_step(lee_ctx *c, int nsteps, double *rec_out)
+{
+    if (!c || nsteps < 0) return set_err(c, LEE_ERR_ARG, "bad arguments");
+    if (c->mf_count < 0) return set_err(c, LEE_ERR_STATE, "lee_frame_setup must run before lee_step");
+    if (nsteps == 0) return LEE_OK;
+    return c->prec == 64 ? run_steps<double>(c, nsteps, rec_out) : run_steps<float>(c, nsteps, rec_out);
+}
+
+int lee_frame_steps(const lee_ctx *c, int mf, int temp_time) { return c ? c->m.frame_steps(mf, temp_time) : 0; }
+int lee_frame_count(const lee_ctx *c) { return c ? c->m.frame_count() : 0; }
+
+int lee_run(lee_ctx *c, double *rec_out, long long cap)
+{
+    if (!c) return set_err(c, LEE_ERR_ARG, "NULL context");
+    const Model &m = c->m;
+    int nr = (int)m.ri.size();
+    if (rec_out && cap < (long long)m.d.time_domain * nr) return set_err(c, LEE_ERR_ARG, "receiver buffer too small");
+    int mf = 0, temp_time = 0, last = m.last_frame();
+    float total = 0.f;
+    do {
+        int lim = m.frame_steps(mf, temp_time);
+        int rc = lee_frame_setup(c, mf);
+        if (rc) return rc;
+        if (lim > 0) {
+            rc = lee_step(c, lim, rec_out ? rec_out + (size_t)temp_time * nr : nullptr);
+            if (rc) return rc;
+            total += c->last_ms;
+        }
+        mf += 1;
+        temp_time += lim > 0 ? lim : 0;
+    } while (mf <= last && mf < m.frame_count() && temp_time < m.d.time_domain && m.d.mf_judge == 1);
+    c->last_ms = total;
+    return LEE_OK;
+}
+
+int lee_get_state(lee_ctx *c, int domain, double *u, double *v, double *w, double *p)
+{
+    if (!c || (domain != 0 && domain != 1)) return set_err(c, LEE_ERR_ARG, "bad arguments");
+    if (domain == 1 && !c->m.south_porous) return set_err(c, LEE_ERR_ARG, "no porous domain");
+    void **f = domain == 0 ? c->air : c->pore;
+    size_t n = domain == 0 ? air_n(c) : pore_n(c);
+    int b = c->cur * 4, rc;
+    double *dst[4] = {u, v, w, p};
+    for (int q = 0; q < 4; ++q)
+        if ((rc = download(c, dst[q], f[b + q], n))) return rc;
+    return LEE_OK;
+}
+
+int lee_set_state(lee_ctx *c, int domain, const double *u, const double *v, const double *w, const double *p)
+{
+    if (!c || (domain != 0 && domain != 1)) return set_err(c, LEE_ERR_ARG, "bad arguments");
+    if (domain == 1 && !c->m.south_porous) return set_err(c, LEE_ERR_ARG, "no porous domain");
+    void **f = domain == 0 ? c->air : c->pore;
+    size_t n = domain == 0 ? air_n(c) : pore_n(c);
+    int b = c->cur * 4, rc;
+    const double *src[4] = {u, v, w, p};
+    for (int q = 0; q < 4; ++q)
+        if (src[q] && (rc = upload(c, f[b + q], src[q], n))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LEE_OK;
+}
+
+int lee_get_psi(lee_ctx *c, double *psi, long long cap)
+{
+    if (!c || !psi) return set_err(c, LEE_ERR_ARG, "bad arguments");
+    size_t n = (size_t)c->m.nx * c->m.nz * 6;
+    if ((long long)n > cap) return set_err(c, LEE_ERR_ARG, "buffer too small");
+    HIPCHK(c, hipMemcpyAsync(psi, c->psi, n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LEE_OK;
+}
+
+double lee_last_step_ms(const lee_ctx *c) { return c ? (double)c->last_ms : 0.0; }
+
+}  // extern "C"

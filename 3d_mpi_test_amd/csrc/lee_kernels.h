@@ -1,0 +1,71 @@
+// lee_kernels.h -- kernel argument blocks and launchers of the FB_p_v step on gfx950.
+//
+// Fields are SoA, x fastest: id = i + nx*(j + ny*k).  T is the state/arithmetic type
+// (double for the FP64 path, float for the FP32 path).  PML auxiliary slabs use the
+// x-fastest layout of their own box (x slab: ii + n*(j + ny*k); y slab: i + nx*(jj + n*k);
+// z slab: i + nx*(j + ny*kk)).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace lee {
+
+enum Face { F_N = 0, F_S = 1, F_E = 2, F_W = 3, F_F = 4, F_B = 5 };
+
+// Per-index PML profile: sigma, (1 - 1/beta), (1/beta - 1).
+template <typename T> struct DProf { const T *s, *c, *d; };
+
+template <typename T> struct Grid {
+    int nx, ny, nz;
+    T *un, *vn, *wn, *pn;      // time level n
+    T *unn, *vnn, *wnn, *pnn;  // time level n+1
+    // FP64: X_over_x, Y_over_y, Z_over_z (the Jacobian is formed per point as the
+    // reference does).  FP32: the reciprocals 1/X_over_x ... (= the separable metric).
+    const T *mx, *my, *mz;
+    T dxi, dyi, dzi, dt;
+};
+
+template <typename T> struct AirArgs {
+    Grid<T> g;
+    const T *U, *V, *W;        // mean wind (NULL when zero)
+    T gam, Pav, gP, Aav;
+    int I1, I2, J1, J2, K1, K2;  // active PML widths (J1 carries the y1 time gate)
+    DProf<T> x1, x2, y1, y2, z1, z2;
+    T *Q[6][4];                // slabs x1 x2 y1 y2 z1 z2, fields u v w p
+    int qn[6];                 // slab planes
+};
+
+template <typename T> struct PoreArgs {
+    Grid<T> g;
+    T gama, invbeta, eff;
+};
+
+template <typename T> struct TdbcArgs {
+    Grid<T> g;
+    double *psi;               // [i + nx*k][6], FP64
+    double decay[6], gain[6];
+    double inv, Zinf, tb;
+};
+
+template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s);
+template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStream_t s);
+template <typename T> void launch_pore_p(const PoreArgs<T> &a, hipStream_t s);
+template <typename T> void launch_pore_v(const PoreArgs<T> &a, hipStream_t s);
+template <typename T> void launch_bc_p(const Grid<T> &g, int face, hipStream_t s);
+template <typename T> void launch_bc_v(const Grid<T> &g, int face, hipStream_t s);
+template <typename T> void launch_tdbc(const TdbcArgs<T> &a, hipStream_t s);
+// which = 0: p pair, 1: u v w pairs (calculation_2D.cpp:865-985)
+template <typename T> void launch_iface(const Grid<T> &air, const Grid<T> &pore, int which, hipStream_t s);
+// Q[f] += dt * src over one slab (air.cpp:2323-2439); axis/lo/n describe the slab.
+template <typename T> void launch_qadd(T *q, const T *src, int axis, int lo, int n, int nx, int ny, int nz,
+                                       T dt, hipStream_t s);
+// Moving-frame shift along j of an (A x B x C) box (B = frame length), out of place.
+template <typename T> void launch_shift(T *dst, const T *src, int A, int B, int C, int tempJ, int lead,
+                                        int s, hipStream_t st);
+template <typename T> void launch_copy_plane_j(T *p, int nx, int ny, int nz, hipStream_t s);
+template <typename T> void launch_sample(const T *p, const long long *idx, int n, double *out, hipStream_t s);
+template <typename T> void launch_finite_check(const T *f, size_t n, int *flag, hipStream_t s);
+
+}  // namespace lee

@@ -1,0 +1,621 @@
+// lee_kernels.hip -- HIP kernels of the FB_p_v time step for gfx950 (MI355X).
+//
+// One thread per grid point, x (i) across the 64 lanes of a wavefront so every field
+// access of a wavefront is one contiguous 512 B (FP64) / 256 B (FP32) segment; the
+// y/z neighbours are re-read from L2 (the radius-1 stencil reuses each line three times
+// per plane).  Built with -ffp-contract=off: the FP64 path keeps the reference's
+// association order (air.cpp / porous.cpp), so it reproduces the CPU results bit for bit.
+//
+// The reference grid is separable (cross derivatives X_over_y ... are fixed at zero,
+// calculation_2D.cpp:102-104): of the nine inverse-metric terms only x_over_X, y_over_Y,
+// z_over_Z are non-zero, and every product with a zero term is dropped here.  Dropping
+// an exact zero from a sum leaves the value unchanged (only the sign of an exact-zero
+// result can differ).
+#include "lee_kernels.h"
+
+#include <type_traits>
+
+namespace lee {
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+template <typename T> __device__ __forceinline__ void metric(const Grid<T> &g, int i, int j, int k, T &mX, T &mY, T &mZ)
+{
+    if constexpr (std::is_same<T, double>::value) {
+        // Jacobian of calculation_2D.cpp:424-430 and Geo_over_var (df_over_var.cpp:5-9)
+        double xx = g.mx[i], yy = g.my[j], zz = g.mz[k];
+        double J = 1.0 / (xx * yy * zz);
+        mX = (yy * zz) * J;
+        mY = (xx * zz) * J;
+        mZ = (xx * yy) * J;
+    } else {
+        mX = g.mx[i]; mY = g.my[j]; mZ = g.mz[k];
+    }
+}
+
+__device__ __forceinline__ size_t qxi(int ii, int j, int k, int n, int ny) { return (size_t)ii + (size_t)n * ((size_t)j + (size_t)ny * k); }
+__device__ __forceinline__ size_t qyi(int i, int jj, int k, int nx, int n) { return (size_t)i + (size_t)nx * ((size_t)jj + (size_t)n * k); }
+__device__ __forceinline__ size_t qzi(int i, int j, int kk, int nx, int ny) { return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * kk); }
+
+// ------------------------------------------------------------------------------------
+// pressure increment + FB add: air::cal_fp (air.cpp:1953-2322) and
+// calculation_2D::Cal_pressure (calculation_2D.cpp:598-608)
+// ------------------------------------------------------------------------------------
+template <typename T, bool WIND>
+__global__ __launch_bounds__(BX * BY) void k_air_p(const AirArgs<T> a)
+{
+    const Grid<T> &g = a.g;
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    if (i >= nx - 1 || j >= ny - 1) return;
+    const size_t sxy = (size_t)nx * ny;
+    const size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+    T mX, mY, mZ;
+    metric(g, i, j, k, mX, mY, mZ);
+    const T dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, dt = g.dt, gP = a.gP;
+    const T pc = g.pn[id];
+    const T dux = g.un[id + 1] - g.un[id];
+    const T dvy = g.vn[id + nx] - g.vn[id];
+    const T dwz = g.wn[id + sxy] - g.wn[id];
+    const T div = (((mX * dux) * dxi) + ((mY * dvy) * dyi)) + ((mZ * dwz) * dzi);
+    T r;
+    if constexpr (WIND) {
+        const T Uc = a.U[id], Vc = a.V[id], Wc = a.W[id];
+        const T c1 = ((Uc * mX) * dxi) * dt, c2 = ((Vc * mY) * dyi) * dt, c3 = ((Wc * mZ) * dzi) * dt;
+        const T UX = ((mX * (a.U[id + 1] - a.U[id - 1])) * dxi) * T(0.5);
+        const T VY = ((mY * (a.V[id + nx] - a.V[id - nx])) * dyi) * T(0.5);
+        const T WZ = ((mZ * (a.W[id + sxy] - a.W[id - sxy])) * dzi) * T(0.5);
+        r = ((-c1) * (pc - g.pn[id - 1]) - c2 * (pc - g.pn[id - nx])) - c3 * (pc - g.pn[id - sxy]);
+        r += (((-a.gam) * pc) * dt) * ((UX + VY) + WZ);
+        r += (((-a.gam) * a.Pav) * dt) * div;
+    } else {
+        r = (((-a.gam) * a.Pav) * dt) * div;
+    }
+    const int n0 = a.qn[0], n1 = a.qn[1], n2 = a.qn[2], n3 = a.qn[3], n4 = a.qn[4], n5 = a.qn[5];
+    // X1 (air.cpp:2044-2057)
+    if (i < a.I1) {
+        const T sx = a.x1.s[i], cx = a.x1.c[i];
+        const size_t q = qxi(i, j, k, n0, ny);
+        const T *Qv = a.Q[0][1], *Qw = a.Q[0][2];
+        const T A = (cx * (gP * ((mX * dux) * dxi))) * dt;
+        const T B = (sx * (pc + gP * (((mY * (Qv[q + n0] - Qv[q])) * dyi) + ((mZ * (Qw[q + (size_t)n0 * ny] - Qw[q])) * dzi)))) * dt;
+        r += A - B;
+    }
+    // X2 (air.cpp:2059-2074)
+    if (i >= nx - a.I2) {
+        const int ii = i - (nx - n1);
+        const T sx = a.x2.s[i], cx = a.x2.c[i];
+        const size_t q = qxi(ii, j, k, n1, ny);
+        const T *Qv = a.Q[1][1], *Qw = a.Q[1][2];
+        const T A = (cx * (gP * ((mX * dux) * dxi))) * dt;
+        const T B = (sx * (pc + gP * (((mY * (Qv[q + n1] - Qv[q])) * dyi) + ((mZ * (Qw[q + (size_t)n1 * ny] - Qw[q])) * dzi)))) * dt;
+        r += A - B;
+    }
+    // cross term of a y or z layer with an x layer (air.cpp:2091-2115 and the like):
+    //   -(Qp_x*sx*so + gP*((1/bx-1)*so*(dQu_x/dX) + (1/bo-1)*sx*(dQ_o/dO)))*dt
+    auto xcross_y = [&](T so, T dom) {
+        if (i < a.I1) {
+            const T sx = a.x1.s[i], dxm = a.x1.d[i];
+            const size_t q = qxi(i, j, k, n0, ny);
+            const T t1 = (dxm * so) * ((mX * (a.Q[0][0][q + 1] - a.Q[0][0][q])) * dxi);
+            const T t2 = (dom * sx) * ((mY * (a.Q[0][1][q + n0] - a.Q[0][1][q])) * dyi);
+            r += (-((a.Q[0][3][q] * sx) * so + gP * (t1 + t2))) * dt;
+        } else if (i >= nx - a.I2) {
+            const int ii = i - (nx - n1);
+            const T sx = a.x2.s[i], dxm = a.x2.d[i];
+            const size_t q = qxi(ii, j, k, n1, ny);
+            const T t1 = (dxm * so) * ((mX * (a.Q[1][0][q + 1] - a.Q[1][0][q])) * dxi);
+            const T t2 = (dom * sx) * ((mY * (a.Q[1][1][q + n1] - a.Q[1][1][q])) * dyi);
+            r += (-((a.Q[1][3][q] * sx) * so + gP * (t1 + t2))) * dt;
+        }
+    };
+    auto xcross_z = [&](T so, T dom) {
+        if (i < a.I1) {
+            const T sx = a.x1.s[i], dxm = a.x1.d[i];
+            const size_t q = qxi(i, j, k, n0, ny);
+            const T t1 = (dxm * so) * ((mX * (a.Q[0][0][q + 1] - a.Q[0][0][q])) * dxi);
+            const T t2 = (dom * sx) * ((mZ * (a.Q[0][2][q + (size_t)n0 * ny] - a.Q[0][2][q])) * dzi);
+            r += (-((a.Q[0][3][q] * sx) * so + gP * (t1 + t2))) * dt;
+        } else if (i >= nx - a.I2) {
+            const int ii = i - (nx - n1);
+            const T sx = a.x2.s[i], dxm = a.x2.d[i];
+            const size_t q = qxi(ii, j, k, n1, ny);
+            const T t1 = (dxm * so) * ((mX * (a.Q[1][0][q + 1] - a.Q[1][0][q])) * dxi);
+            const T t2 = (dom * sx) * ((mZ * (a.Q[1][2][q + (size_t)n1 * ny] - a.Q[1][2][q])) * dzi);
+            r += (-((a.Q[1][3][q] * sx) * so + gP * (t1 + t2))) * dt;
+        }
+    };
+    // Y1 (air.cpp:2076-2116)
+    if (j < a.J1) {
+        const T sy = a.y1.s[j], cy = a.y1.c[j];
+        const size_t q = qyi(i, j, k, nx, n2);
+        const T *Qu = a.Q[2][0], *Qw = a.Q[2][2];
+        const T A = (cy * (gP * ((mY * dvy) * dyi))) * dt;
+        const T B = (sy * (pc + gP * (((mX * (Qu[q + 1] - Qu[q])) * dxi) + ((mZ * (Qw[q + (size_t)nx * n2] - Qw[q])) * dzi)))) * dt;
+        r += A - B;
+        xcross_y(sy, a.y1.d[j]);
+    }
+    // Y2 (air.cpp:2118-2158)
+    if (j >= ny - a.J2) {
+        const int jj = j - (ny - n3);
+        const T sy = a.y2.s[j], cy = a.y2.c[j];
+        const size_t q = qyi(i, jj, k, nx, n3);
+        const T *Qu = a.Q[3][0], *Qw = a.Q[3][2];
+        const T A = (cy * (gP * ((mY * dvy) * dyi))) * dt;
+        const T B = (sy * (pc + gP * (((mX * (Qu[q + 1] - Qu[q])) * dxi) + ((mZ * (Qw[q + (size_t)nx * n3] - Qw[q])) * dzi)))) * dt;
+        r += A - B;
+        xcross_y(sy, a.y2.d[j]);
+    }
+    // y-z cross term (air.cpp:2200-2224)
+    auto ycross_z = [&](T sz, T dzm) {
+        if (j < a.J1) {
+            const T sy = a.y1.s[j], dym = a.y1.d[j];
+            const size_t q = qyi(i, j, k, nx, n2);
+            const T t1 = (dym * sz) * ((mY * (a.Q[2][1][q + nx] - a.Q[2][1][q])) * dyi);
+            const T t2 = (dzm * sy) * ((mZ * (a.Q[2][2][q + (size_t)nx * n2] - a.Q[2][2][q])) * dzi);
+            r += (-((a.Q[2][3][q] * sy) * sz + gP * (t1 + t2))) * dt;
+        } else if (j >= ny - a.J2) {
+            const int jj = j - (ny - n3);
+            const T sy = a.y2.s[j], dym = a.y2.d[j];
+            const size_t q = qyi(i, jj, k, nx, n3);
+            const T t1 = (dym * sz) * ((mY * (a.Q[3][1][q + nx] - a.Q[3][1][q])) * dyi);
+            const T t2 = (dzm * sy) * ((mZ * (a.Q[3][2][q + (size_t)nx * n3] - a.Q[3][2][q])) * dzi);
+            r += (-((a.Q[3][3][q] * sy) * sz + gP * (t1 + t2))) * dt;
+        }
+    };
+    // Z1 (air.cpp:2160-2225)
+    if (k < a.K1) {
+        const T sz = a.z1.s[k], cz = a.z1.c[k];
+        const size_t q = qzi(i, j, k, nx, ny);
+        const T *Qu = a.Q[4][0], *Qv = a.Q[4][1];
+        const T A = (((cz * a.gam) * a.Pav) * ((mZ * dwz) * dzi)) * dt;
+        const T B = (sz * (pc + gP * (((mX * (Qu[q + 1] - Qu[q])) * dxi) + ((mY * (Qv[q + nx] - Qv[q])) * dyi)))) * dt;
+        r += A - B;
+        xcross_z(sz, a.z1.d[k]);
+        ycross_z(sz, a.z1.d[k]);
+    }
+    // Z2 (air.cpp:2227-2292)
+    if (k >= nz - a.K2) {
+        const int kk = k - (nz - n5);
+        const T sz = a.z2.s[k], cz = a.z2.c[k];
+        const size_t q = qzi(i, j, kk, nx, ny);
+        const T *Qu = a.Q[5][0], *Qv = a.Q[5][1];
+        const T A = (((cz * a.gam) * a.Pav) * ((mZ * dwz) * dzi)) * dt;
+        const T B = (sz * (pc + gP * (((mX * (Qu[q + 1] - Qu[q])) * dxi) + ((mY * (Qv[q + nx] - Qv[q])) * dyi)))) * dt;
+        r += A - B;
+        xcross_z(sz, a.z2.d[k]);
+        ycross_z(sz, a.z2.d[k]);
+    }
+    (void)n4;
+    g.pnn[id] = pc + r;
+}
+
+// ------------------------------------------------------------------------------------
+// velocity increments + FB add: air::cal_fuvw (air.cpp:1476-1949) with p at n+1, and
+// calculation_2D::Cal_velocity (calculation_2D.cpp:564-574)
+// ------------------------------------------------------------------------------------
+template <typename T, bool WIND>
+__global__ __launch_bounds__(BX * BY) void k_air_v(const AirArgs<T> a)
+{
+    const Grid<T> &g = a.g;
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    if (i >= nx - 1 || j >= ny - 1) return;
+    const size_t sxy = (size_t)nx * ny;
+    const size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+    T mX, mY, mZ;
+    metric(g, i, j, k, mX, mY, mZ);
+    const T dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, dt = g.dt, Aav = a.Aav;
+    const T *p = g.pnn;
+    const T pc = p[id];
+    const T gpx = (mX * (pc - p[id - 1])) * dxi;
+    const T gpy = (mY * (pc - p[id - nx])) * dyi;
+    const T gpz = (mZ * (pc - p[id - sxy])) * dzi;
+    const T uc = g.un[id], vc = g.vn[id], wc = g.wn[id];
+    T du, dv, dw;
+    if constexpr (WIND) {
+        const T Uc = a.U[id], Vc = a.V[id], Wc = a.W[id];
+        const T c1 = ((Uc * mX) * dxi) * dt, c2 = ((Vc * mY) * dyi) * dt, c3 = ((Wc * mZ) * dzi) * dt;
+        const T *U = a.U, *V = a.V, *W = a.W;
+        const T UX = ((mX * (U[id + 1] - U[id - 1])) * dxi) * T(0.5), UY = ((mY * (U[id + nx] - U[id - nx])) * dyi) * T(0.5),
+                UZ = ((mZ * (U[id + sxy] - U[id - sxy])) * dzi) * T(0.5);
+        const T VX = ((mX * (V[id + 1] - V[id - 1])) * dxi) * T(0.5), VY = ((mY * (V[id + nx] - V[id - nx])) * dyi) * T(0.5),
+                VZ = ((mZ * (V[id + sxy] - V[id - sxy])) * dzi) * T(0.5);
+        const T WX = ((mX * (W[id + 1] - W[id - 1])) * dxi) * T(0.5), WY = ((mY * (W[id + nx] - W[id - nx])) * dyi) * T(0.5),
+                WZ = ((mZ * (W[id + sxy] - W[id - sxy])) * dzi) * T(0.5);
+        du = ((-c1) * (g.un[id + 1] - uc) - c2 * (g.un[id + nx] - uc)) - c3 * (g.un[id + sxy] - uc);
+        du += (((-uc) * UX) * dt - (vc * UY) * dt) - (wc * UZ) * dt;
+        du += ((-Aav) * dt) * gpx;
+        dv = ((-c1) * (g.vn[id + 1] - vc) - c2 * (g.vn[id + nx] - vc)) - c3 * (g.vn[id + sxy] - vc);
+        dv += (((-uc) * VX) * dt - (vc * VY) * dt) - (wc * VZ) * dt;
+        dv += ((-Aav) * dt) * gpy;
+        dw = ((-c1) * (g.wn[id + 1] - wc) - c2 * (g.wn[id + nx] - wc)) - c3 * (g.wn[id + sxy] - wc);
+        dw += (((-uc) * WX) * dt - (vc * WY) * dt) - (wc * WZ) * dt;
+        dw += ((-Aav) * dt) * gpz;
+    } else {
+        du = ((-Aav) * dt) * gpx;
+        dv = ((-Aav) * dt) * gpy;
+        dw = ((-Aav) * dt) * gpz;
+    }
+    const int n0 = a.qn[0], n1 = a.qn[1], n2 = a.qn[2], n3 = a.qn[3], n5 = a.qn[5];
+    // backward differences of a slab field along x, y, z with the matching metric
+#define QBX(Qp, q) ((mX * ((Qp)[q] - (Qp)[(q) - 1])) * dxi)
+    // X1 / X2 (air.cpp:1579-1619)
+    if (i < a.I1) {
+        const T sx = a.x1.s[i], cx = a.x1.c[i];
+        const size_t q = qxi(i, j, k, n0, ny);
+        const T *Qp = a.Q[0][3];
+        du += ((-sx) * uc) * dt + (cx * (Aav * gpx)) * dt;
+        dv += ((-sx) * (vc + Aav * ((mY * (Qp[q] - Qp[q - n0])) * dyi))) * dt;
+        dw += ((-sx) * (wc + Aav * ((mZ * (Qp[q] - Qp[q - (size_t)n0 * ny])) * dzi))) * dt;
+    }
+    if (i >= nx - a.I2) {
+        const int ii = i - (nx - n1);
+        const T sx = a.x2.s[i], cx = a.x2.c[i];
+        const size_t q = qxi(ii, j, k, n1, ny);
+        const T *Qp = a.Q[1][3];
+        du += ((-sx) * uc) * dt + (cx * (Aav * gpx)) * dt;
+        dv += ((-sx) * (vc + Aav * ((mY * (Qp[q] - Qp[q - n1])) * dyi))) * dt;
+        dw += ((-sx) * (wc + Aav * ((mZ * (Qp[q] - Qp[q - (size_t)n1 * ny])) * dzi))) * dt;
+    }
+    // y layer with x cross terms (air.cpp:1621-1728)
+    auto ylayer = [&](T sy, T cy, T dym, const T *Qp, size_t qy, int nq) {
+        du += ((-sy) * (uc + Aav * QBX(Qp, qy))) * dt;
+        dv += ((cy * Aav) * gpy) * dt - (sy * vc) * dt;
+        dw += ((-sy) * (wc + Aav * ((mZ * (Qp[qy] - Qp[qy - (size_t)nx * nq])) * dzi))) * dt;
+        if (i < a.I1) {
+            const T sx = a.x1.s[i], dxm = a.x1.d[i];
+            const size_t q = qxi(i, j, k, n0, ny);
+            const T *Xp = a.Q[0][3];
+            du += (-((a.Q[0][0][q] * sx) * sy + ((dxm * sy) * Aav) * QBX(Xp, q))) * dt;
+            dv += (-((a.Q[0][1][q] * sx) * sy + ((dym * sx) * Aav) * ((mY * (Xp[q] - Xp[q - n0])) * dyi))) * dt;
+            dw += (((-a.Q[0][2][q]) * sx) * sy) * dt;
+        } else if (i >= nx - a.I2) {
+            const int ii = i - (nx - n1);
+            const T sx = a.x2.s[i], dxm = a.x2.d[i];
+            const size_t q = qxi(ii, j, k, n1, ny);
+            const T *Xp = a.Q[1][3];
+            du += (-((a.Q[1][0][q] * sx) * sy + ((dxm * sy) * Aav) * QBX(Xp, q))) * dt;
+            dv += (-((a.Q[1][1][q] * sx) * sy + ((dym * sx) * Aav) * ((mY * (Xp[q] - Xp[q - n1])) * dyi))) * dt;
+            dw += (((-a.Q[1][2][q]) * sx) * sy) * dt;
+        }
+    };
+    if (j < a.J1) ylayer(a.y1.s[j], a.y1.c[j], a.y1.d[j], a.Q[2][3], qyi(i, j, k, nx, n2), n2);
+    if (j >= ny - a.J2) ylayer(a.y2.s[j], a.y2.c[j], a.y2.d[j], a.Q[3][3], qyi(i, j - (ny - n3), k, nx, n3), n3);
+    // z layer with x and y cross terms (air.cpp:1730-1905)
+    auto zlayer = [&](T sz, T cz, T dzm, const T *Qp, size_t qz) {
+        du += ((-sz) * (uc + Aav * QBX(Qp, qz))) * dt;
+        dv += ((-sz) * (vc + Aav * ((mY * (Qp[qz] - Qp[qz - nx])) * dyi))) * dt;
+        dw += ((cz * Aav) * gpz) * dt - (sz * wc) * dt;
+        if (i < a.I1) {
+            const T sx = a.x1.s[i], dxm = a.x1.d[i];
+            const size_t q = qxi(i, j, k, n0, ny);
+            const T *Xp = a.Q[0][3];
+            du += (-((a.Q[0][0][q] * sx) * sz + ((dxm * sz) * Aav) * QBX(Xp, q))) * dt;
+            dv += (((-a.Q[0][1][q]) * sx) * sz) * dt;
+            dw += (-((a.Q[0][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - (size_t)n0 * ny])) * dzi))) * dt;
+        } else if (i >= nx - a.I2) {
+            const int ii = i - (nx - n1);
+            const T sx = a.x2.s[i], dxm = a.x2.d[i];
+            const size_t q = qxi(ii, j, k, n1, ny);
+            const T *Xp = a.Q[1][3];
+            du += (-((a.Q[1][0][q] * sx) * sz + ((dxm * sz) * Aav) * QBX(Xp, q))) * dt;
+            dv += (((-a.Q[1][1][q]) * sx) * sz) * dt;
+            dw += (-((a.Q[1][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - (size_t)n1 * ny])) * dzi))) * dt;
+        }
+        if (j < a.J1) {
+            const T sy = a.y1.s[j], dym = a.y1.d[j];
+            const size_t q = qyi(i, j, k, nx, n2);
+            const T *Yp = a.Q[2][3];
+            du += (((-a.Q[2][0][q]) * sy) * sz) * dt;
+            dv += (-((a.Q[2][1][q] * sy) * sz + ((dym * sz) * Aav) * ((mY * (Yp[q] - Yp[q - nx])) * dyi))) * dt;
+            dw += (-((a.Q[2][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - (size_t)nx * n2])) * dzi))) * dt;
+        } else if (j >= ny - a.J2) {
+            const int jj = j - (ny - n3);
+            const T sy = a.y2.s[j], dym = a.y2.d[j];
+            const size_t q = qyi(i, jj, k, nx, n3);
+            const T *Yp = a.Q[3][3];
+            du += (((-a.Q[3][0][q]) * sy) * sz) * dt;
+            dv += (-((a.Q[3][1][q] * sy) * sz + ((dym * sz) * Aav) * ((mY * (Yp[q] - Yp[q - nx])) * dyi))) * dt;
+            dw += (-((a.Q[3][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - (size_t)nx * n3])) * dzi))) * dt;
+        }
+    };
+    if (k < a.K1) zlayer(a.z1.s[k], a.z1.c[k], a.z1.d[k], a.Q[4][3], qzi(i, j, k, nx, ny));
+    if (k >= nz - a.K2) zlayer(a.z2.s[k], a.z2.c[k], a.z2.d[k], a.Q[5][3], qzi(i, j, k - (nz - n5), nx, ny));
+#undef QBX
+    g.unn[id] = uc + du;
+    g.vnn[id] = vc + dv;
+    g.wnn[id] = wc + dw;
+}
+
+// ------------------------------------------------------------------------------------
+// porous ground (porous.cpp:77-143) + FB add
+// ------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(BX * BY) void k_pore_p(const PoreArgs<T> a)
+{
+    const Grid<T> &g = a.g;
+    const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    const int nx = g.nx, ny = g.ny;
+    if (i >= nx - 1 || j >= ny - 1) return;
+    const size_t sxy = (size_t)nx * ny;
+    const size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+    T mX, mY, mZ;
+    metric(g, i, j, k, mX, mY, mZ);
+    const T r = (-a.gama) * ((((mX * g.dxi) * (g.un[id + 1] - g.un[id])) + ((mY * g.dyi) * (g.vn[id + nx] - g.vn[id]))) +
+                             ((mZ * g.dzi) * (g.wn[id + sxy] - g.wn[id])));
+    g.pnn[id] = g.pn[id] + r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX * BY) void k_pore_v(const PoreArgs<T> a)
+{
+    const Grid<T> &g = a.g;
+    const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    const int nx = g.nx, ny = g.ny;
+    if (i >= nx - 1 || j >= ny - 1) return;
+    const size_t sxy = (size_t)nx * ny;
+    const size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+    T mX, mY, mZ;
+    metric(g, i, j, k, mX, mY, mZ);
+    const T *p = g.pnn;
+    const T pc = p[id];
+    const T f = (((T(-1.0) * a.invbeta) * g.dt) * a.eff);
+    const T uc = g.un[id], vc = g.vn[id], wc = g.wn[id];
+    const T du = (f * ((mX * (pc - p[id - 1])) * g.dxi) + uc * a.invbeta) - uc;
+    const T dv = (f * ((mY * (pc - p[id - nx])) * g.dyi) + vc * a.invbeta) - vc;
+    const T dw = (f * ((mZ * (pc - p[id - sxy])) * g.dzi) + wc * a.invbeta) - wc;
+    g.unn[id] = uc + du;
+    g.vnn[id] = vc + dv;
+    g.wnn[id] = wc + dw;
+}
+
+// ------------------------------------------------------------------------------------
+// boundary conditions (calculation_2D.cpp:646-864); one launch per face, in the
+// reference order, so edges and corners end with the value of the last face applied.
+// ------------------------------------------------------------------------------------
+template <typename T>
+__global__ void k_bc_p(const Grid<T> g, int face)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    T *p = g.pnn;
+    auto I = [&](int i, int j, int k) { return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k); };
+    switch (face) {
+    case F_W: if (a < nx) p[I(a, 0, b)] = p[I(a, 1, b)]; break;                      // a=i, b=k
+    case F_E:
+        if (a < nx) {
+            const int jn = ny - 1;
+            p[I(a, jn, b)] = T(15.0 / 8.0) * p[I(a, jn - 1, b)] - T(5.0 / 4.0) * p[I(a, jn - 2, b)] + T(3.0 / 8.0) * p[I(a, jn - 3, b)];
+        }
+        break;
+    case F_S: if (a < nx) p[I(a, b, 0)] = p[I(a, b, 1)]; break;                      // a=i, b=j
+    case F_N: if (a < nx) p[I(a, b, nz - 1)] = p[I(a, b, nz - 2)]; break;
+    case F_F: if (a < ny) p[I(0, a, b)] = p[I(1, a, b)]; break;                      // a=j, b=k
+    case F_B: if (a < ny) p[I(nx - 1, a, b)] = p[I(nx - 2, a, b)]; break;
+    }
+}
+
+template <typename T>
+__global__ void k_bc_v(const Grid<T> g, int face)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    T *u = g.unn, *v = g.vnn, *w = g.wnn;
+    auto I = [&](int i, int j, int k) { return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k); };
+    size_t d, s;
+    switch (face) {
+    case F_W: if (a >= nx) return; d = I(a, 0, b); s = I(a, 1, b); u[d] = u[s]; v[d] = T(0); w[d] = w[s]; break;
+    case F_E: if (a >= nx) return; d = I(a, ny - 1, b); s = I(a, ny - 2, b); u[d] = u[s]; v[d] = T(0); w[d] = w[s]; break;
+    case F_S: if (a >= nx) return; d = I(a, b, 0); s = I(a, b, 1); u[d] = u[s]; v[d] = v[s]; w[d] = T(0); break;
+    case F_N: if (a >= nx) return; d = I(a, b, nz - 1); s = I(a, b, nz - 2); u[d] = u[s]; v[d] = v[s]; w[d] = T(0); break;
+    case F_F: if (a >= ny) return; d = I(0, a, b); s = I(1, a, b); u[d] = T(0); v[d] = v[s]; w[d] = w[s]; break;
+    case F_B: if (a >= ny) return; d = I(nx - 1, a, b); s = I(nx - 2, a, b); u[d] = T(0); v[d] = v[s]; w[d] = w[s]; break;
+    }
+}
+
+// East TDBC (calculation_2D.cpp:986-1028): tangential copy, then v from the recursive
+// convolution, then the FP64 history update, per face point (i,k).
+template <typename T>
+__global__ void k_tdbc(const TdbcArgs<T> a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    const Grid<T> &g = a.g;
+    const int nx = g.nx, ny = g.ny;
+    if (i >= nx) return;
+    const size_t e = (size_t)i + (size_t)nx * ((size_t)(ny - 1) + (size_t)ny * k);
+    const size_t e1 = e - nx;
+    g.unn[e] = g.unn[e1];
+    g.wnn[e] = g.wnn[e1];
+    double *psi = a.psi + ((size_t)i + (size_t)nx * k) * 6;
+    double P[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) P[q] = psi[q];
+    double temp_psi = 0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) temp_psi += a.decay[q] * P[q];
+    const double vv = a.inv * ((double)g.pnn[e] / a.Zinf - temp_psi);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) psi[q] = a.decay[q] * P[q] + (a.gain[q] * vv) * a.tb;
+    g.vnn[e] = (T)vv;
+}
+
+// Air-porous ghost planes (calculation_2D.cpp:878-886, 957-961).
+template <typename T>
+__global__ void k_iface(const Grid<T> A, const Grid<T> P, int which)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= A.nx) return;
+    const size_t a0 = (size_t)i + (size_t)A.nx * j;
+    const size_t a1 = a0 + (size_t)A.nx * A.ny;
+    const size_t pp = (size_t)P.nx * P.ny;
+    const size_t q2 = (size_t)i + (size_t)P.nx * j + pp * (P.nz - 2);
+    const size_t q1 = q2 + pp;
+    if (which == 0) {
+        A.pnn[a0] = P.pnn[q2];
+        P.pnn[q1] = A.pnn[a1];
+    } else {
+        A.unn[a0] = P.unn[q2]; P.unn[q1] = A.unn[a1];
+        A.vnn[a0] = P.vnn[q2]; P.vnn[q1] = A.vnn[a1];
+        A.wnn[a0] = P.wnn[q2]; P.wnn[q1] = A.wnn[a1];
+    }
+}
+
+template <typename T>
+__global__ void k_qadd(T *q, const T *src, int axis, int lo, int n, int nx, int ny, int nz, T dt)
+{
+    const int A = axis == 0 ? n : nx, B = axis == 1 ? n : ny, C = axis == 2 ? n : nz;
+    const size_t total = (size_t)A * B * C;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int a = (int)(t % A);
+        const size_t r = t / A;
+        const int b = (int)(r % B), c = (int)(r / B);
+        const int i = axis == 0 ? a + lo : a, j = axis == 1 ? b + lo : b, k = axis == 2 ? c + lo : c;
+        q[t] = q[t] + dt * src[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)];
+    }
+}
+
+// Frame shift along j (calculation_2D.cpp:461-477, air.cpp:2547-2573): out of place.
+template <typename T>
+__global__ void k_shift(T *dst, const T *src, int A, int B, int C, int tempJ, int lead, int s)
+{
+    const size_t total = (size_t)A * B * C;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int j = (int)((t / A) % B);
+        T v;
+        if (j < tempJ + 1 && j + lead - 1 < B - 1) v = src[t + (size_t)A * s];
+        else if (j >= tempJ + 1) v = T(0);
+        else v = src[t];
+        dst[t] = v;
+    }
+}
+
+template <typename T>
+__global__ void k_copy_plane_j(T *p, int nx, int ny, int nz)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (i >= nx) return;
+    const size_t b = (size_t)i + (size_t)nx * (size_t)ny * k;
+    p[b] = p[b + nx];
+}
+
+template <typename T>
+__global__ void k_sample(const T *p, const long long *idx, int n, double *out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const long long q = idx[r];
+    double v = q >= 0 ? (double)p[q] : 0.0;
+    if (fabs(v) < 1.0e-100) v = 0.0;
+    out[r] = v;
+}
+
+template <typename T>
+__global__ void k_finite(const T *f, size_t n, int *flag)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        if (!isfinite((double)f[t])) { atomicOr(flag, 1); return; }
+}
+
+dim3 interior_grid(int nx, int ny, int nz) { return dim3((nx - 2 + BX - 1) / BX, (ny - 2 + BY - 1) / BY, nz - 2); }
+
+unsigned flat_blocks(size_t total)
+{
+    size_t b = (total + 255) / 256;
+    if (b > 65536) b = 65536;
+    return (unsigned)(b ? b : 1);
+}
+
+}  // namespace
+
+template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s)
+{
+    dim3 g = interior_grid(a.g.nx, a.g.ny, a.g.nz), b(BX, BY);
+    if (wind) hipLaunchKernelGGL((k_air_p<T, true>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_air_p<T, false>), g, b, 0, s, a);
+}
+template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStream_t s)
+{
+    dim3 g = interior_grid(a.g.nx, a.g.ny, a.g.nz), b(BX, BY);
+    if (wind) hipLaunchKernelGGL((k_air_v<T, true>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_air_v<T, false>), g, b, 0, s, a);
+}
+template <typename T> void launch_pore_p(const PoreArgs<T> &a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_pore_p<T>), interior_grid(a.g.nx, a.g.ny, a.g.nz), dim3(BX, BY), 0, s, a);
+}
+template <typename T> void launch_pore_v(const PoreArgs<T> &a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_pore_v<T>), interior_grid(a.g.nx, a.g.ny, a.g.nz), dim3(BX, BY), 0, s, a);
+}
+template <typename T> static dim3 face_grid(const Grid<T> &g, int face)
+{
+    switch (face) {
+    case F_W: case F_E: return dim3((g.nx + 63) / 64, g.nz);
+    case F_S: case F_N: return dim3((g.nx + 63) / 64, g.ny);
+    default: return dim3((g.ny + 63) / 64, g.nz);
+    }
+}
+template <typename T> void launch_bc_p(const Grid<T> &g, int face, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_bc_p<T>), face_grid(g, face), dim3(64), 0, s, g, face);
+}
+template <typename T> void launch_bc_v(const Grid<T> &g, int face, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_bc_v<T>), face_grid(g, face), dim3(64), 0, s, g, face);
+}
+template <typename T> void launch_tdbc(const TdbcArgs<T> &a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_tdbc<T>), dim3((a.g.nx + 63) / 64, a.g.nz), dim3(64), 0, s, a);
+}
+template <typename T> void launch_iface(const Grid<T> &air, const Grid<T> &pore, int which, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_iface<T>), dim3((air.nx + 63) / 64, air.ny), dim3(64), 0, s, air, pore, which);
+}
+template <typename T> void launch_qadd(T *q, const T *src, int axis, int lo, int n, int nx, int ny, int nz, T dt, hipStream_t s)
+{
+    size_t total = (size_t)(axis == 0 ? n : nx) * (axis == 1 ? n : ny) * (axis == 2 ? n : nz);
+    hipLaunchKernelGGL((k_qadd<T>), dim3(flat_blocks(total)), dim3(256), 0, s, q, src, axis, lo, n, nx, ny, nz, dt);
+}
+template <typename T> void launch_shift(T *dst, const T *src, int A, int B, int C, int tempJ, int lead, int sh, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_shift<T>), dim3(flat_blocks((size_t)A * B * C)), dim3(256), 0, st, dst, src, A, B, C, tempJ, lead, sh);
+}
+template <typename T> void launch_copy_plane_j(T *p, int nx, int ny, int nz, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_copy_plane_j<T>), dim3((nx + 63) / 64, nz), dim3(64), 0, s, p, nx, ny, nz);
+}
+template <typename T> void launch_sample(const T *p, const long long *idx, int n, double *out, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL((k_sample<T>), dim3((n + 63) / 64), dim3(64), 0, s, p, idx, n, out);
+}
+template <typename T> void launch_finite_check(const T *f, size_t n, int *flag, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_finite<T>), dim3(flat_blocks(n)), dim3(256), 0, s, f, n, flag);
+}
+
+#define LEE_INST(T)                                                                                          \
+    template void launch_air_p<T>(const AirArgs<T> &, bool, hipStream_t);                                    \
+    template void launch_air_v<T>(const AirArgs<T> &, bool, hipStream_t);                                    \
+    template void launch_pore_p<T>(const PoreArgs<T> &, hipStream_t);                                        \
+    template void launch_pore_v<T>(const PoreArgs<T> &, hipStream_t);                                        \
+    template void launch_bc_p<T>(const Grid<T> &, int, hipStream_t);                                         \
+    template void launch_bc_v<T>(const Grid<T> &, int, hipStream_t);                                         \
+    template void launch_tdbc<T>(const TdbcArgs<T> &, hipStream_t);                                          \
+    template void launch_iface<T>(const Grid<T> &, const Grid<T> &, int, hipStream_t);                       \
+    template void launch_qadd<T>(T *, const T *, int, int, int, int, int, int, T, hipStream_t);              \
+    template void launch_shift<T>(T *, const T *, int, int, int, int, int, int, hipStream_t);                \
+    template void launch_copy_plane_j<T>(T *, int, int, int, hipStream_t);                                   \
+    template void launch_sample<T>(const T *, const long long *, int, double *, hipStream_t);                \
+    template void launch_finite_check<T>(const T *, size_t, int *, hipStream_t);
+LEE_INST(double)
+LEE_INST(float)
+#undef LEE_INST
+
+}  // namespace lee

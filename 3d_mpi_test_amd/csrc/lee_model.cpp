@@ -1,0 +1,377 @@
+// lee_model.cpp -- host-side FP64 derivation of everything static per run / per frame.
+#include "lee_model.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+namespace lee {
+
+namespace {
+
+// TDBC poles and residues of the reference (calculation_2D.h:73-75).
+const double kTdA[6] = {8.226223820e-01, 3.732304064e-01, 9.106232372e-02,
+                        4.819445335e-02, 4.225250937e-01, 1.837970986e-01};
+const double kTdR[6] = {8.681310138e-01, 1.953722814e-01, 1.108067950e-02,
+                        2.566498621e-03, 1.457297589e-04, 4.648583789e-02};
+
+// Hard-coded receiver (y, z) of airpore.cpp:434-448, x from input.
+const double kRecYZ[15][2] = {{-0.23, 1.75}, {-0.46, 0.46}, {-0.1, 0.1}, {1.75, 0.23}, {1.75, 0.46},
+                              {0.6, 0.01},   {0.7, 0.01},   {0.8, 0.01}, {0.9, 0.01},  {1.0, 0.01},
+                              {1.2, 0.01},   {1.4, 0.01},   {1.6, 0.01}, {1.8, 0.01},  {2.0, 0.01}};
+
+// PML judge of one axis (airpore.cpp:74-105, 182-247): +1 high face, +10 low face.
+int judge_of(bool hi, bool lo)
+{
+    int n = (hi ? 1 : 0) + (lo ? 10 : 0);
+    return n == 1 ? 2 : n == 10 ? 1 : n == 11 ? 3 : 0;
+}
+
+// SpeedProfile::Uav/Vav/Wav (air.cpp:71-253) for the methods the reference defines
+// without reading uninitialised members: 1, 2, 4, 5.
+double alpha2(double y, double z, double &Rr)
+{
+    Rr = std::sqrt(std::pow(y - 5.0, 2) + std::pow(z - 2.0, 2));
+    if (std::fabs(y - 5.0) <= 1.0e-6 && std::fabs(z - 2.0) <= 1.0e-6) return 0.0;
+    if (y - 5.0 >= 0.0) return std::asin((y - 5.0) / Rr);
+    return std::asin((y - 5.0) / Rr) + 3.1415926;
+}
+double windU(int m, double b, double, double y, double) { return m == 5 ? b * y : 0.0; }
+double windV(int m, double b, double, double y, double z)
+{
+    if (m == 1) return z >= 2.8 ? b * 2.8 : (z >= 0.0 ? b * z : 0.0);
+    if (m == 2) { double Rr, a = alpha2(y, z, Rr); return -1.0 * 2.0 / (Rr + 2.0) * std::sin(a); }
+    if (m == 5) return b * std::log(std::fabs(z) / 0.1 + 1);
+    return 0.0;
+}
+double windW(int m, double b, double, double y, double z)
+{
+    if (m == 2) { double Rr, a = alpha2(y, z, Rr); return 1.0 * 2.0 / (Rr + 2.0) * std::cos(a); }
+    if (m == 5) return b * y;
+    return 0.0;
+}
+
+void metrics1d(const std::vector<double> &c, double inv, std::vector<double> &out)
+{
+    int n = (int)c.size();
+    out.assign(n, 0.0);
+    for (int i = 1; i < n - 1; ++i) out[i] = (c[i + 1] - c[i - 1]) * inv * 0.5;
+}
+
+// sigma = R*pow(t,alpha), beta = 1 + 25*pow(t,alpha) (air.cpp:1580-1581) on the planes
+// of one face; t = depth/width as the reference forms it.
+void profile(Profile &P, int n, int lo, int hi_excl, double R, double alpha,
+             double (*depth)(const void *, int), const void *ctx)
+{
+    P.s.assign(n, 0.0); P.c.assign(n, 0.0); P.d.assign(n, 0.0);
+    for (int i = lo; i < hi_excl; ++i) {
+        double t = depth(ctx, i);
+        double sg = R * std::pow(t, alpha);
+        double be = 1.0 + 25.0 * std::pow(t, alpha);
+        P.s[i] = sg;
+        P.c[i] = 1.0 - 1.0 / be;
+        P.d[i] = 1.0 / be - 1.0;
+    }
+}
+
+struct Depth { const double *coord; double anchor, width; bool low; };
+double depth_fn(const void *p, int i)
+{
+    const Depth *d = static_cast<const Depth *>(p);
+    return d->low ? (d->anchor - d->coord[i]) * d->width : (d->coord[i] - d->anchor) * d->width;
+}
+
+}  // namespace
+
+std::string Model::build(const lee_deck &deck, const lee_options &o)
+{
+    d = deck;
+    opt = o;
+    char buf[256];
+    if (d.scheme != 2)
+        return "scheme " + std::to_string(d.scheme) + " is not implemented by the reference "
+               "(only FB_p_v, scheme 2, updates the fields: calculation_2D.cpp:551-612)";
+    int nb = d.bc[LEE_FACE_N];
+    if (nb != LEE_BC_RIGID && nb != LEE_BC_ABSORBING)
+        return "North boundary must be rigid or absorbing (other types skip every BC: airpore.cpp:160-169)";
+    for (int f : {LEE_FACE_W, LEE_FACE_F, LEE_FACE_B})
+        if (d.bc[f] == LEE_BC_TDBC || d.bc[f] == LEE_BC_POROUS)
+            return "TDBC/porous boundaries exist only on East/South in the reference";
+    if (d.bc[LEE_FACE_E] == LEE_BC_POROUS) return "porous East face is not implemented by the reference";
+    int sb = d.bc[LEE_FACE_S];
+    if (sb != LEE_BC_RIGID && sb != LEE_BC_ABSORBING && sb != LEE_BC_POROUS)
+        return "South boundary must be rigid, absorbing or porous";
+    if (d.hill_judge != 0) return "embedded porous obstacles (hill/canopy/trunk) are not supported";
+    if (d.restart != 0) return "restart is not supported";
+    if (d.velocity_method == 3 || d.velocity_method == 6)
+        return "wind methods 3 and 6 read uninitialised members in the reference (air.cpp:44-69)";
+    if (d.mf_lead_dj < 2) return "lead_DJ must be >= 2 (airpore.cpp:542 divides by lead_DJ-1)";
+    if (d.mf_judge == 1 && d.mf_trail_dj > d.mf_lead_dj)
+        return "trail_DJ > lead_DJ reads past the frame (calculation_2D.cpp:466-467)";
+    if (d.dt <= 0 || d.dx <= 0 || d.dy <= 0 || d.dz <= 0) return "non-positive time or grid step";
+    if (!d.air_x || !d.air_y || !d.air_z) return "missing air coordinates";
+
+    const int *h = d.air_hdr;
+    nx = h[0]; nz = h[6];
+    ijkmax = h[9];
+    int JM = h[3];
+    mf_input = d.mf_jmax; lead = d.mf_lead_dj; trail = d.mf_trail_dj;
+    if (d.mf_judge == 1) {  // calculation_2D.cpp:59-68
+        if ((ijkmax - d.mf_jmax) % (lead - 1) != 0) {
+            int nl = (ijkmax - d.mf_jmax) / (lead - 1);
+            ijkmax = d.mf_jmax + (nl + 1) * (lead - 1);
+        }
+        JM = d.mf_jmax;
+    } else {
+        ijkmax = JM;
+    }
+    ny = JM;
+    if (nx < 5 || ny < 5 || nz < 5) return "grid must be at least 5 points per axis";
+    ax.assign(d.air_x, d.air_x + nx);
+    ay.assign(d.air_y, d.air_y + ny);
+    az.assign(d.air_z, d.air_z + nz);
+    dt = d.dt;
+    dxi = 1.0 / d.dx; dyi = 1.0 / d.dy; dzi = 1.0 / d.dz;
+    gamma = d.gamma; Pav = d.Pav;
+    Aav = std::pow(d.c, 2) / d.gamma / d.Pav;  // airpore.cpp:177
+    gP = gamma * Pav;
+    has_wind = !(d.velocity_method == 4 || d.velocity_method < 1 || d.velocity_method > 6);
+    if (d.velocity_method == 1 && d.velocity_coef == 0.0) has_wind = false;
+
+    // ---- PML (air.cpp:509-937), one domain: mpi_i1 = 0, mpi_i2 = nx-1
+    I1 = h[1]; I2 = h[2]; J1 = h[4]; J2 = h[5]; J3 = h[4]; K1 = h[7]; K2 = h[8];
+    judge[0] = judge_of(d.bc[LEE_FACE_F] == LEE_BC_ABSORBING, d.bc[LEE_FACE_B] == LEE_BC_ABSORBING);
+    judge[1] = judge_of(d.bc[LEE_FACE_E] == LEE_BC_ABSORBING, d.bc[LEE_FACE_W] == LEE_BC_ABSORBING);
+    judge[2] = judge_of(d.bc[LEE_FACE_N] == LEE_BC_ABSORBING, d.bc[LEE_FACE_S] == LEE_BC_ABSORBING);
+    for (auto &s : slab) s = SlabGeom{};
+    slab[0].axis = slab[1].axis = 0; slab[2].axis = slab[3].axis = 1; slab[4].axis = slab[5].axis = 2;
+    auto setup_axis = [&](int jd, int &w1, int &w2, int n, const std::vector<double> &c, double R,
+                          double al, double &Rv, double &alv, double &a1, double &a2, double &iw1,
+                          double &iw2, SlabGeom &s1, SlabGeom &s2, const char *nm) -> std::string {
+        if (jd == 0) { w1 = 0; w2 = 0; return ""; }
+        Rv = R; alv = al;
+        if (jd == 1 || jd == 3) {
+            if (w1 < 2 || w1 > n - 2) return std::string(nm) + "1 PML width out of range";
+            iw1 = 1.0 / std::fabs(c[w1 - 1] - c[0]);
+            a1 = c[w1 - 1];
+            s1.n = (n - 1 >= w1) ? w1 + 1 : n;
+            s1.lo = 0;
+        }
+        if (jd == 2 || jd == 3) {
+            if (w2 < 2 || w2 > n - 2) return std::string(nm) + "2 PML width out of range";
+            iw2 = 1.0 / std::fabs(c[n - 1] - c[n - w2]);
+            a2 = c[n - w2];
+            s2.n = (0 <= n - w2 - 1) ? w2 + 1 : n;
+            s2.lo = n - s2.n;
+        }
+        if (jd == 1) w2 = 0;
+        if (jd == 2) w1 = 0;
+        return "";
+    };
+    std::string e;
+    e = setup_axis(judge[0], I1, I2, nx, ax, d.pml_R[0], d.pml_alpha[0], Rx, alx, px1, px2, W1, W2, slab[0], slab[1], "x");
+    if (!e.empty()) return e;
+    e = setup_axis(judge[1], J1, J2, ny, ay, d.pml_R[1], d.pml_alpha[1], Ry, aly, py1, py2, L1, L2, slab[2], slab[3], "y");
+    if (!e.empty()) return e;
+    e = setup_axis(judge[2], K1, K2, nz, az, d.pml_R[2], d.pml_alpha[2], Rz, alz, pz1, pz2, H1, H2, slab[4], slab[5], "z");
+    if (!e.empty()) return e;
+    if (judge[1] == 1 || judge[1] == 3) {
+        pml_time = d.pml_y_time;
+        J3 = h[4];
+    } else {
+        J3 = 0;
+    }
+
+    // ---- porous ground (airpore.cpp:249-251, porous.cpp:40-51)
+    south_porous = sb == LEE_BC_POROUS;
+    if (south_porous) {
+        if (!d.has_pore || !d.pore_x || !d.pore_y || !d.pore_z) return "South porous needs coordi_pore files";
+        const int *ph = d.pore_hdr;
+        pnx = ph[0]; pnz = ph[6];
+        pny = d.mf_judge == 1 ? d.mf_jmax : ph[3];
+        if (pnx != nx || pny != ny)
+            return "porous grid must share the air (i,j) index space (calculation_2D.cpp:878-886)";
+        if (pnz < 4) return "porous layer needs at least 4 planes";
+        if (d.pore_dx <= 0 || d.pore_dy <= 0 || d.pore_dz <= 0) return "non-positive porous grid step";
+        pax.assign(d.pore_x, d.pore_x + pnx);
+        pay.assign(d.pore_y, d.pore_y + pny);
+        paz.assign(d.pore_z, d.pore_z + pnz);
+        pdxi = 1.0 / d.pore_dx; pdyi = 1.0 / d.pore_dy; pdzi = 1.0 / d.pore_dz;
+        double eff_density = d.pore_q * d.pore_q / d.pore_porosity / Aav;
+        double Kp = 1.0 / d.gamma / d.Pav;
+        p_eff = 1.0 / eff_density;
+        p_beta = d.pore_resistivity * d.dt * p_eff;
+        p_gama = d.dt / Kp / d.pore_porosity;
+        p_invbeta = 1.0 / (p_beta + 1.0);
+        // (int)(dz_air/dz_pore) < 5 selects the plane copy; the other branch is commented
+        // out in the reference, which then couples nothing (calculation_2D.cpp:872-877).
+        iface_copy = (int)(d.dz / d.pore_dz) < 5.0;
+    }
+    // ---- TDBC (airpore.cpp:252-254, calculation_2D.cpp:986-1028)
+    east_tdbc = d.bc[LEE_FACE_E] == LEE_BC_TDBC;
+    {
+        double rho = d.gamma * d.Pav / 340.29 / 340.29;
+        Zinf = rho * 340.29 * d.pore_q / d.pore_porosity;
+        tau = d.gamma * rho * d.pore_q * d.pore_q / d.pore_resistivity / d.pore_porosity;
+        double tb = d.dt / tau;
+        double at = 0;
+        for (int q = 0; q < 6; ++q) {
+            at += kTdA[q] * tb;
+            td_decay[q] = std::exp(-1 * kTdR[q] * tb);
+            td_gain[q] = kTdA[q];
+        }
+        td_inv = 1.0 / (1 + at);
+        if (east_tdbc && !(std::isfinite(Zinf) && std::isfinite(tau) && tau > 0))
+            return "TDBC needs finite positive ground parameters";
+    }
+
+    // ---- receivers (airpore.cpp:413-458, calculation_2D.cpp:317-352)
+    int nrec;
+    if (opt.n_receivers > 0) {
+        if (!opt.receivers) return "receivers pointer is NULL";
+        nrec = opt.n_receivers;
+    } else {
+        nrec = (d.rcv[1] == 0) ? 1 : 5;
+    }
+    ri.assign(nrec, 1); rj.assign(nrec, 1); rk.assign(nrec, 1); rpos.assign(3 * nrec, 0.0);
+    double Yf = ay[ny - 1] - ay[0];
+    for (int r = 0; r < nrec; ++r) {
+        double rx, ry, rz;
+        if (opt.n_receivers > 0) { rx = opt.receivers[3 * r]; ry = opt.receivers[3 * r + 1]; rz = opt.receivers[3 * r + 2]; }
+        else if (r < 15) { rx = d.rcv[0]; ry = kRecYZ[r][0]; rz = kRecYZ[r][1]; }
+        else { rx = d.rcv[0]; ry = d.rcv[1]; rz = d.rcv[2]; }
+        rpos[3 * r] = rx; rpos[3 * r + 1] = ry; rpos[3 * r + 2] = rz;
+        int ii = -1, jj = -1, kk = -1;
+        int N = (int)(ry / Yf + 1.0e-6);
+        for (int i = 0; i < nx; ++i) if (rx - ax[i] <= d.dx / 8.0) { ii = i; break; }
+        int NN = N * (ny - 1);
+        if (NN >= ijkmax) { ii = jj = kk = 0; }
+        else {
+            for (int j = NN; j < ijkmax && j - NN < ny; ++j)
+                if (ry - N * Yf - ay[j - NN] <= d.dy / 8.0) { jj = j; break; }
+            for (int k = 0; k < nz; ++k) if (rz - az[k] <= d.dz / 8.0) { kk = k; break; }
+            if (kk < 0) { ii = jj = kk = 0; }
+        }
+        // airpore.cpp:455-456; an index the reference leaves undefined counts as out of domain
+        if ((ii == 0 && jj == 0 && kk == 0) || ii < 0 || jj < 0 || kk < 0) { ii = 1; jj = 1; kk = 1; }
+        if (ii > nx || jj > ijkmax || kk > nz) { ii = 1; jj = 1; kk = 1; }
+        ri[r] = ii; rj[r] = jj; rk[r] = kk;
+    }
+    (void)buf;
+    return "";
+}
+
+int Model::last_frame() const
+{
+    double x = (double)((ijkmax - mf_input) / (lead - 1)) + 1.e-6;
+    return (int)x;
+}
+
+int Model::frame_count() const
+{
+    if (d.mf_judge != 1) return 1;
+    int n = last_frame() + 1;
+    return n < 1 ? 1 : n;
+}
+
+// MF_limit of airpore.cpp:542-560 (the first frame clamped to time_domain).
+int Model::frame_steps(int mf, int temp_time) const
+{
+    int lim;
+    double per = (trail - 1) * d.dy / d.c / d.dt;
+    if (mf == last_frame() || ijkmax == mf_input) {
+        lim = d.time_domain - temp_time;
+    } else if (mf == 0) {
+        lim = (int)((mf_input - 1) * d.dy / d.c / d.dt * d.mf_ratio);
+        if (lim > d.time_domain) lim = d.time_domain;
+    } else {
+        lim = (int)per;
+        double l1 = 0.0, l2 = 0.0;
+        if (mf % 10 == 0) { l1 = per - lim; l1 = l1 * 10.0; }
+        if (mf % 100 == 0) { l2 = per - lim; l2 = l2 * 100 - (int)(l2 * 10) * 10; }
+        lim = lim + (int)l1 + (int)l2;
+        if (temp_time + lim > d.time_domain) lim = d.time_domain - temp_time;
+    }
+    return lim;
+}
+
+// calculation_2D::get_pressure (calculation_2D.cpp:387-403) for one domain.
+long long Model::receiver_index(int r, int N) const
+{
+    int sh = N * (trail - 1);
+    int ii = ri[r], jj = rj[r], kk = rk[r];
+    if (!(jj >= sh && jj < ny + sh)) return -1;
+    int jl = jj - sh;
+    if (jl > 0 && jl < ny - 1 && ii > 0 && ii < nx - 1 && kk > 0 && kk < nz - 1)
+        return (long long)ii + (long long)nx * ((long long)jl + (long long)ny * kk);
+    return -1;
+}
+
+void Model::frame_tables(int N, FrameTables &t) const
+{
+    // coordinates and metrics (calculation_2D.cpp:410-419)
+    std::vector<double> Y(ay);
+    if (N != 0)
+        for (int j = 0; j < ny; ++j) Y[j] = ay[j] + N * (trail - 1) * d.dy;
+    metrics1d(ax, dxi, t.Xx);
+    metrics1d(Y, dyi, t.Yy);
+    metrics1d(az, dzi, t.Zz);
+    if (south_porous) {
+        std::vector<double> PY(pay);
+        if (N != 0)
+            for (int j = 0; j < pny; ++j) PY[j] = pay[j] + N * (trail - 1) * d.pore_dy;
+        metrics1d(pax, pdxi, t.pXx);
+        metrics1d(PY, pdyi, t.pYy);
+        metrics1d(paz, pdzi, t.pZz);
+    }
+    // PML profiles; PML_y1/y2 follow the frame (air.cpp:1289-1292)
+    double y1a = py1, y2a = py2;
+    if (N != 0) {
+        if (J3 > 0) y1a = ay[J3 - 1] + N * (trail - 1) * d.dy;
+        if (J2 > 0) y2a = ay[ny - J2] + N * (trail - 1) * d.dy;
+    }
+    Depth dx1{ax.data(), px1, W1, true}, dx2{ax.data(), px2, W2, false};
+    Depth dy1{Y.data(), y1a, L1, true}, dy2{Y.data(), y2a, L2, false};
+    Depth dz1{az.data(), pz1, H1, true}, dz2{az.data(), pz2, H2, false};
+    profile(t.x1, nx, 0, I1, Rx, alx, depth_fn, &dx1);
+    profile(t.x2, nx, nx - I2, nx, Rx, alx, depth_fn, &dx2);
+    profile(t.y1, ny, 0, J3, Ry, aly, depth_fn, &dy1);
+    profile(t.y2, ny, ny - J2, ny, Ry, aly, depth_fn, &dy2);
+    profile(t.z1, nz, 0, K1, Rz, alz, depth_fn, &dz1);
+    profile(t.z2, nz, nz - K2, nz, Rz, alz, depth_fn, &dz2);
+    // wind (air.cpp:1293-1303)
+    t.U.clear(); t.V.clear(); t.W.clear();
+    if (has_wind) {
+        size_t n = (size_t)nx * ny * nz;
+        t.U.resize(n); t.V.resize(n); t.W.resize(n);
+        int m = d.velocity_method;
+        double b = d.velocity_coef;
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+                    double z = az[k] - 0.0;
+                    t.U[id] = windU(m, b, ax[i], Y[j], z);
+                    t.V[id] = windV(m, b, ax[i], Y[j], z);
+                    t.W[id] = windW(m, b, ax[i], Y[j], z);
+                }
+    }
+}
+
+// calculation_2D::Set_InitialCond (calculation_2D.cpp:292-314): Gaussian pulse.
+void Model::initial_condition(std::vector<double> &p, bool porous) const
+{
+    const std::vector<double> &X = porous ? pax : ax, &Y = porous ? pay : ay, &Z = porous ? paz : az;
+    int a = (int)X.size(), b = (int)Y.size(), c = (int)Z.size();
+    p.assign((size_t)a * b * c, 0.0);
+    double gc = 4.0 * std::log(2.0) / std::pow(d.gauss_width, 2);
+    for (int k = 0; k < c; ++k)
+        for (int j = 0; j < b; ++j)
+            for (int i = 0; i < a; ++i) {
+                double x = X[i] - d.src[0], y = Y[j] - d.src[1], z = Z[k] - d.src[2];
+                double r = std::sqrt(std::pow(x, 2) + std::pow(y, 2) + std::pow(z, 2));
+                p[(size_t)i + (size_t)a * ((size_t)j + (size_t)b * k)] = 1.0 * std::exp(-gc * r * r);
+            }
+}
+
+}  // namespace lee

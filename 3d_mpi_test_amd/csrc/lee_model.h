@@ -1,0 +1,85 @@
+// lee_model.h -- host-side FP64 model of one solver instance.
+//
+// Everything that is static per run or per moving frame is derived here, once, in FP64 on
+// the host with the platform libm: validation of the deck, dimensions, PML slab geometry
+// and per-index absorption profiles, porous and TDBC coefficients, wind fields, metrics,
+// receiver indices and the frame schedule.  The device kernels then only add, subtract,
+// multiply and divide, so the FP64 device path reproduces the reference arithmetic.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/lee_abi.h"
+
+namespace lee {
+
+// One PML slab of auxiliary fields Q (air.h:56-58): `n` planes normal to `axis`,
+// starting at global index `lo`.  Storage is x-fastest over the slab box.
+struct SlabGeom {
+    int axis = 0;     // 0 x, 1 y, 2 z
+    int n = 0;        // planes (0 = inactive)
+    int lo = 0;       // first global plane
+    size_t count(int nx, int ny, int nz) const
+    {
+        size_t a = axis == 0 ? n : nx, b = axis == 1 ? n : ny, c = axis == 2 ? n : nz;
+        return a * b * c;
+    }
+};
+
+// Per-index absorption profile of one PML face: sigma, (1 - 1/beta), (1/beta - 1)
+// (air.cpp:1580-1581 and the (1 -+ 1/beta) factors of air.cpp:1583-1905, 2048-2290).
+struct Profile {
+    std::vector<double> s, c, d;
+};
+
+struct FrameTables {
+    std::vector<double> Xx, Yy, Zz;       // air metrics X_over_x ... (calculation_2D.cpp:417-419)
+    std::vector<double> pXx, pYy, pZz;    // porous metrics
+    Profile x1, x2, y1, y2, z1, z2;       // PML profiles for this frame
+    std::vector<double> U, V, W;          // wind (empty when zero everywhere)
+};
+
+struct Model {
+    lee_deck d{};
+    lee_options opt{};
+    std::vector<double> ax, ay, az;       // axisbound_X/Y/Z
+    std::vector<double> pax, pay, paz;    // porous coordinates
+    int nx = 0, ny = 0, nz = 0;           // air frame dims
+    int pnx = 0, pny = 0, pnz = 0;        // porous dims
+    int ijkmax = 0;
+    double dxi = 0, dyi = 0, dzi = 0, dt = 0;
+    double pdxi = 0, pdyi = 0, pdzi = 0;
+    double gamma = 0, Pav = 0, Aav = 0, gP = 0;
+    bool south_porous = false, east_tdbc = false, iface_copy = true, has_wind = false;
+    // PML (air.cpp:509-937)
+    int judge[3] = {0, 0, 0};
+    int I1 = 0, I2 = 0, J1 = 0, J2 = 0, J3 = 0, K1 = 0, K2 = 0;
+    int pml_time = 0;
+    double Rx = 0, Ry = 0, Rz = 0, alx = 0, aly = 0, alz = 0;
+    double px1 = 0, px2 = 0, W1 = 0, W2 = 0, py1 = 0, py2 = 0, L1 = 0, L2 = 0, pz1 = 0, pz2 = 0, H1 = 0, H2 = 0;
+    SlabGeom slab[6];                     // x1 x2 y1 y2 z1 z2
+    // porous (porous.cpp:40-51)
+    double p_eff = 0, p_gama = 0, p_invbeta = 0, p_beta = 0;
+    // TDBC (calculation_2D.h:71-75, airpore.cpp:252-254)
+    double Zinf = 0, tau = 0;
+    double td_decay[6] = {}, td_gain[6] = {}, td_inv = 0;   // e^{-r tau_bar}, a*tau_bar... see .cpp
+    // moving frame
+    int mf_input = 0, lead = 0, trail = 0;
+    // receivers (whole-domain indices)
+    std::vector<int> ri, rj, rk;
+    std::vector<double> rpos;
+
+    // returns empty string on success, else the reason
+    std::string build(const lee_deck &deck, const lee_options &o);
+    void frame_tables(int N, FrameTables &t) const;
+    void initial_condition(std::vector<double> &p, bool porous) const;
+    int last_frame() const;
+    int frame_count() const;
+    int frame_steps(int mf, int temp_time) const;
+    // linear index of receiver r in frame N (x-fastest), or -1 when it samples 0
+    long long receiver_index(int r, int N) const;
+};
+
+}  // namespace lee

@@ -1,0 +1,93 @@
+"""FP64 GPU path vs the CPU oracle, through the C ABI (FB_p_v, reference semantics).
+
+The bar is <= 1e-10 relative L2 on the receiver series and on the final fields; the FP64
+device path keeps the reference's operation order without FMA contraction, so the
+differences are expected to be zero or at rounding level.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REC = [(0.0, 0.55, 0.45), (0.15, 0.75, 0.3), (-0.2, 0.35, 0.6), (0.05, 1.05, 0.2)]
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def cases(lee):
+    B = lee.deck.box
+    out = {}
+    out["rigid"] = B(24, 28, 20, 0.05, bc=(1, 1, 1, 1, 1, 1), src=(0.0, 0.6, 0.5), gauss_width=0.25, time_domain=40)
+    out["pml5"] = B(26, 30, 22, 0.05, pml=5, bc=(2, 1, 2, 2, 2, 2), src=(0.1, 0.7, 0.5), gauss_width=0.25,
+                    time_domain=40, pml_y_time=3)
+    out["tdbc"] = B(24, 30, 20, 0.05, pml=5, bc=(2, 1, 5, 2, 2, 2), src=(0.0, 1.0, 0.5), gauss_width=0.25,
+                    time_domain=40, pml_y_time=10)
+    out["porous"] = B(24, 26, 20, 0.05, pml=5, bc=(2, 3, 2, 2, 2, 2), porous_cells=8, src=(0.0, 0.6, 0.3),
+                      gauss_width=0.25, time_domain=40)
+    out["wind5"] = B(24, 28, 20, 0.05, pml=5, bc=(2, 1, 2, 2, 2, 2), src=(0.0, 0.6, 0.5), gauss_width=0.25,
+                     time_domain=40, velocity_method=5, velocity_coef=3.0)
+    out["wind2"] = B(24, 28, 20, 0.05, bc=(1, 1, 1, 1, 1, 1), src=(0.0, 0.6, 0.5), gauss_width=0.25,
+                     time_domain=40, velocity_method=2, velocity_coef=1.0)
+    return out
+
+
+@pytest.mark.parametrize("name", ["rigid", "pml5", "tdbc", "porous", "wind5", "wind2"])
+def test_fb_fp64_parity(lee, orc, name):
+    d = cases(lee)[name]
+    print(f"[gpu parity] case={name} shape={d.shape}", flush=True)
+    s = lee.Solver(d, precision=64, receivers=REC)
+    s.frame_setup(0)
+    got = s.step(d.time_domain)
+    o = orc.Oracle(d, receivers=REC)
+    o.frame_setup(0)
+    ref = o.step(d.time_domain)
+    assert np.abs(ref).max() > 0
+    e_rec = rel(got, ref)
+    fs, fo = s.state(), o.state()
+    e_f = max(rel(a, b) for a, b in zip(fs, fo))
+    same = all(np.array_equal(a, b) for a, b in zip(fs, fo))
+    if d.has_pore:
+        ps, po = s.state(1), o.state(1)
+        e_f = max(e_f, max(rel(a, b) for a, b in zip(ps, po)))
+    if name == "tdbc":
+        e_f = max(e_f, rel(s.psi(), o.psi()))
+    print(f"  rec rel-L2 {e_rec:.3e}  field rel-L2 {e_f:.3e}  bitwise {same}", flush=True)
+    s.close()
+    assert e_rec <= 1e-10
+    assert e_f <= 1e-10
+
+
+def test_moving_frame_run(lee, orc):
+    d = lee.deck.box(22, 30, 18, 0.05, pml=4, bc=(2, 1, 2, 2, 2, 2), src=(0.0, 0.5, 0.4), gauss_width=0.25,
+                     time_domain=90, mf_judge=1, mf_jmax=30, mf_lead_dj=6, mf_trail_dj=6, mf_ratio=0.7,
+                     ijkmax=70, c=343.0)
+    d.air_y = np.arange(70) * 0.05
+    print("[gpu parity] moving frame", flush=True)
+    s = lee.Solver(d, precision=64, receivers=REC)
+    assert s.frame_count() > 2
+    got = s.run()
+    o = orc.Oracle(d, receivers=REC)
+    ref = o.run()
+    e = rel(got, ref)
+    fe = max(rel(a, b) for a, b in zip(s.state(), o.state()))
+    print(f"  frames {s.frame_count()} rec rel-L2 {e:.3e} field {fe:.3e}", flush=True)
+    s.close()
+    assert e <= 1e-10 and fe <= 1e-10
+
+
+def test_fp32_tracks_fp64(lee):
+    d = cases(lee)["pml5"]
+    print("[gpu] fp32 vs fp64", flush=True)
+    a = lee.Solver(d, precision=64, receivers=REC)
+    b = lee.Solver(d, precision=32, receivers=REC)
+    a.frame_setup(0); b.frame_setup(0)
+    ra, rb = a.step(d.time_domain), b.step(d.time_domain)
+    e = rel(rb, ra)
+    print(f"  fp32 receiver rel-L2 vs fp64 {e:.3e}", flush=True)
+    a.close(); b.close()
+    assert e < 1e-4
