@@ -350,12 +350,9 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
     lee_options_default(&o);
     if (opt) o = *opt;
     if (o.precision != 64 && o.precision != 32) return set_err(nullptr, LEE_ERR_ARG, "precision must be 32 or 64");
-    std::string e = c->m.build(*deck, o);
-    if (!e.empty()) {
-        bool unsup = e.find("not") != std::string::npos || e.find("support") != std::string::npos ||
-                     e.find("uninitialised") != std::string::npos;
-        return set_err(nullptr, unsup ? LEE_ERR_UNSUPPORTED : LEE_ERR_ARG, e);
-    }
+    int code = LEE_OK;
+    std::string e = c->m.build(*deck, o, code);
+    if (!e.empty()) return set_err(nullptr, code, e);
     lee_ctx *C = c.get();
     C->prec = o.precision;
     C->esz = o.precision == 64 ? 8 : 4;

@@ -83,11 +83,11 @@ double depth_fn(const void *p, int i)
 
 }  // namespace
 
-std::string Model::build(const lee_deck &deck, const lee_options &o)
+std::string Model::build(const lee_deck &deck, const lee_options &o, int &code)
 {
     d = deck;
     opt = o;
-    char buf[256];
+    code = LEE_ERR_UNSUPPORTED;
     if (d.scheme != 2)
         return "scheme " + std::to_string(d.scheme) + " is not implemented by the reference "
                "(only FB_p_v, scheme 2, updates the fields: calculation_2D.cpp:551-612)";
@@ -105,6 +105,7 @@ std::string Model::build(const lee_deck &deck, const lee_options &o)
     if (d.restart != 0) return "restart is not supported";
     if (d.velocity_method == 3 || d.velocity_method == 6)
         return "wind methods 3 and 6 read uninitialised members in the reference (air.cpp:44-69)";
+    code = LEE_ERR_ARG;
     if (d.mf_lead_dj < 2) return "lead_DJ must be >= 2 (airpore.cpp:542 divides by lead_DJ-1)";
     if (d.mf_judge == 1 && d.mf_trail_dj > d.mf_lead_dj)
         return "trail_DJ > lead_DJ reads past the frame (calculation_2D.cpp:466-467)";
@@ -257,7 +258,7 @@ std::string Model::build(const lee_deck &deck, const lee_options &o)
         if (ii > nx || jj > ijkmax || kk > nz) { ii = 1; jj = 1; kk = 1; }
         ri[r] = ii; rj[r] = jj; rk[r] = kk;
     }
-    (void)buf;
+    code = LEE_OK;
     return "";
 }
 

@@ -64,15 +64,17 @@ struct Model {
     double p_eff = 0, p_gama = 0, p_invbeta = 0, p_beta = 0;
     // TDBC (calculation_2D.h:71-75, airpore.cpp:252-254)
     double Zinf = 0, tau = 0;
-    double td_decay[6] = {}, td_gain[6] = {}, td_inv = 0;   // e^{-r tau_bar}, a*tau_bar... see .cpp
+    // TDBC recursion constants: e^{-r_k dt/tau}, residues a_k, 1/(1 + sum_k a_k dt/tau)
+    double td_decay[6] = {}, td_gain[6] = {}, td_inv = 0;
     // moving frame
     int mf_input = 0, lead = 0, trail = 0;
     // receivers (whole-domain indices)
     std::vector<int> ri, rj, rk;
     std::vector<double> rpos;
 
-    // returns empty string on success, else the reason
-    std::string build(const lee_deck &deck, const lee_options &o);
+    // Returns an empty string on success, else the reason; `code` receives LEE_OK,
+    // LEE_ERR_UNSUPPORTED (a path the reference never implemented) or LEE_ERR_ARG.
+    std::string build(const lee_deck &deck, const lee_options &o, int &code);
     void frame_tables(int N, FrameTables &t) const;
     void initial_condition(std::vector<double> &p, bool porous) const;
     int last_frame() const;
