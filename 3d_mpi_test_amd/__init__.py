@@ -55,6 +55,16 @@ class LeeInfo(ctypes.Structure):
     ]
 
 
+class LeeKstat(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 32),
+        ("launches", ctypes.c_longlong),
+        ("total_ms", ctypes.c_double),
+        ("bytes_per_launch", ctypes.c_double),
+        ("points_per_launch", ctypes.c_double),
+    ]
+
+
 class LeeError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
@@ -97,6 +107,10 @@ def lib():
         L.lee_write_receivers.argtypes = [ctypes.c_char_p, _dp, ctypes.c_int, ctypes.c_int]
         L.lee_write_spectra.argtypes = [ctypes.c_char_p, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.lee_fft.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.lee_profile.argtypes = [vp, ctypes.c_int]
+        L.lee_kernel_stats.argtypes = [vp, ctypes.POINTER(LeeKstat), ctypes.c_int]
+        L.lee_copy_bench.argtypes = [vp, ctypes.c_longlong, ctypes.c_int]
+        L.lee_copy_bench.restype = ctypes.c_double
         _lib = L
     return _lib
 

@@ -1,0 +1,254 @@
+// lee_bc.hip -- all face conditions of one phase in one launch.
+//
+// The reference applies its faces one after another (pressure: N, E, W, F, B, then S or
+// the porous chain S, E, W, F, B and the ghost-plane copy; velocity likewise with the
+// East TDBC), each face copying from the plane next to it (calculation_2D.cpp:646-1028,
+// airpore.cpp:324-398).  Where faces meet, a later face reads values an earlier face has
+// just written.  Instead of one launch per face, each thread here takes one boundary node
+// and evaluates its final value directly by walking the face order backwards: the value
+// after faces 1..t is the last face (order <= t) containing the node applied to its
+// source nodes as they were after faces 1..order-1.  The walk ends on interior nodes (the
+// stencil output); it never needs a value the chain has not written yet, because every
+// source of a face lies strictly inside along that face's axis and has no face that comes
+// later in the order.  The arithmetic per node is the same as the sequential chain's, so
+// results are bit-identical.
+//
+// Face order (1..6):  air       N E W F B  S (rigid) / I (ghost plane k=0 <- pore Kp-2)
+//                     porous    S E W F B  I (ghost plane Kp-1 <- air k=1)
+// The same launch also does the PML-auxiliary update of the boundary nodes, the East
+// TDBC history update, and (pressure phase) the receiver samples of the step.
+#include "lee_kernels.h"
+
+namespace lee {
+
+namespace {
+
+template <typename T> __device__ __forceinline__ size_t gidx(const Grid<T> &g, int i, int j, int k)
+{
+    return (size_t)i + (size_t)g.nx * ((size_t)j + (size_t)g.ny * k);
+}
+
+// Walking the chain backwards from face `t` to face 1, every face the node lies on either
+// maps it to its source node (a copy from the plane next to the face: an index clamp),
+// ends the walk with a value (a zero normal velocity, the TDBC velocity), or -- East
+// pressure -- combines three source nodes, which are then only subject to face 1.
+// The walk needs only the stencil output (interior nodes) from memory.
+
+// ---- pressure -------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ T east_extrap(const Grid<T> &g, int i, int k)
+{
+    const T a0 = g.pnn[gidx(g, i, g.ny - 2, k)], a1 = g.pnn[gidx(g, i, g.ny - 3, k)], a2 = g.pnn[gidx(g, i, g.ny - 4, k)];
+    return T(15.0 / 8.0) * a0 - T(5.0 / 4.0) * a1 + T(3.0 / 8.0) * a2;
+}
+
+template <typename T> __device__ __forceinline__ T p_pore_walk(const BCArgs<T> &a, int i, int j, int k, int t);
+
+// air pressure after faces 1..t (t <= 6): N E W F B, then S or the ghost plane
+template <typename T, bool ALLOW_PORE> __device__ __forceinline__ T p_air_walk(const BCArgs<T> &a, int i, int j, int k, int t)
+{
+    const Grid<T> &g = a.air;
+    if (t >= 6 && k == 0 && (!a.porous || a.iface)) {
+        if constexpr (ALLOW_PORE) {
+            if (a.porous) return p_pore_walk(a, i, j, a.pore.nz - 2, 5);
+        }
+        k = 1;
+    }
+    if (t >= 5 && i == g.nx - 1) i = g.nx - 2;
+    if (t >= 4 && i == 0) i = 1;
+    if (t >= 3 && j == 0) j = 1;
+    if (t >= 2 && j == g.ny - 1) return east_extrap(g, i, k == g.nz - 1 ? g.nz - 2 : k);
+    if (t >= 1 && k == g.nz - 1) k = g.nz - 2;
+    return g.pnn[gidx(g, i, j, k)];
+}
+
+// porous pressure after faces 1..t: S E W F B, then the ghost plane Kp-1 <- air k=1
+template <typename T> __device__ __forceinline__ T p_pore_walk(const BCArgs<T> &a, int i, int j, int k, int t)
+{
+    const Grid<T> &g = a.pore;
+    if (t >= 6 && k == g.nz - 1 && a.iface) return p_air_walk<T, false>(a, i, j, 1, 5);
+    if (t >= 5 && i == g.nx - 1) i = g.nx - 2;
+    if (t >= 4 && i == 0) i = 1;
+    if (t >= 3 && j == 0) j = 1;
+    if (t >= 2 && j == g.ny - 1) return east_extrap(g, i, k == 0 ? 1 : k);
+    if (t >= 1 && k == 0) k = 1;
+    return g.pnn[gidx(g, i, j, k)];
+}
+
+// ---- velocity (component c: 0 u, 1 v, 2 w) --------------------------------------------
+template <typename T> __device__ __forceinline__ T *vfield(const Grid<T> &g, int c) { return c == 0 ? g.unn : c == 1 ? g.vnn : g.wnn; }
+
+// East TDBC velocity from the final boundary pressure and the history of the previous step
+// (calculation_2D.cpp:1019-1028).
+template <typename T> __device__ __forceinline__ double tdbc_v(const BCArgs<T> &a, int i, int k)
+{
+    const Grid<T> &g = a.air;
+    const double *P = a.psi_old + ((size_t)i + (size_t)g.nx * k) * 6;
+    double temp_psi = 0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) temp_psi += a.decay[q] * P[q];
+    return a.inv * ((double)g.pnn[gidx(g, i, g.ny - 1, k)] / a.Zinf - temp_psi);
+}
+
+template <typename T> __device__ __forceinline__ T v_pore_walk(const BCArgs<T> &a, int c, int i, int j, int k, int t);
+
+// air velocity component c after faces 1..t
+template <typename T, bool ALLOW_PORE> __device__ __forceinline__ T v_air_walk(const BCArgs<T> &a, int c, int i, int j, int k, int t)
+{
+    const Grid<T> &g = a.air;
+    if (t >= 6 && k == 0 && (!a.porous || a.iface)) {
+        if constexpr (ALLOW_PORE) {
+            if (a.porous) return v_pore_walk(a, c, i, j, a.pore.nz - 2, 5);
+        }
+        if (c == 2) return T(0);
+        k = 1;
+    }
+    if (t >= 5 && i == g.nx - 1) { if (c == 0) return T(0); i = g.nx - 2; }
+    if (t >= 4 && i == 0) { if (c == 0) return T(0); i = 1; }
+    if (t >= 3 && j == 0) { if (c == 1) return T(0); j = 1; }
+    if (t >= 2 && j == g.ny - 1) {
+        if (c == 1) return a.tdbc ? (T)tdbc_v(a, i, k) : T(0);
+        j = g.ny - 2;
+    }
+    if (t >= 1 && k == g.nz - 1) { if (c == 2) return T(0); k = g.nz - 2; }
+    return vfield(g, c)[gidx(g, i, j, k)];
+}
+
+template <typename T> __device__ __forceinline__ T v_pore_walk(const BCArgs<T> &a, int c, int i, int j, int k, int t)
+{
+    const Grid<T> &g = a.pore;
+    if (t >= 6 && k == g.nz - 1 && a.iface) return v_air_walk<T, false>(a, c, i, j, 1, 5);
+    if (t >= 5 && i == g.nx - 1) { if (c == 0) return T(0); i = g.nx - 2; }
+    if (t >= 4 && i == 0) { if (c == 0) return T(0); i = 1; }
+    if (t >= 3 && j == 0) { if (c == 1) return T(0); j = 1; }
+    if (t >= 2 && j == g.ny - 1) { if (c == 1) return T(0); j = g.ny - 2; }
+    if (t >= 1 && k == 0) { if (c == 2) return T(0); k = 1; }
+    return vfield(g, c)[gidx(g, i, j, k)];
+}
+
+// i, j, k of boundary node t of an nx*ny*nz box: the two x planes, then the two y planes
+// without their x edges, then the two z planes without x and y edges.
+__device__ __forceinline__ void boundary_node(long long t, int nx, int ny, int nz, int &i, int &j, int &k)
+{
+    const long long nA = 2LL * ny * nz, nB = 2LL * (nx - 2) * nz;
+    if (t < nA) {
+        const long long pl = (long long)ny * nz;
+        const int side = (int)(t / pl);
+        const long long r = t % pl;
+        j = (int)(r % ny); k = (int)(r / ny); i = side ? nx - 1 : 0;
+    } else if (t < nA + nB) {
+        t -= nA;
+        const long long pl = (long long)(nx - 2) * nz;
+        const int side = (int)(t / pl);
+        const long long r = t % pl;
+        i = 1 + (int)(r % (nx - 2)); k = (int)(r / (nx - 2)); j = side ? ny - 1 : 0;
+    } else {
+        t -= nA + nB;
+        const long long pl = (long long)(nx - 2) * (ny - 2);
+        const int side = (int)(t / pl);
+        const long long r = t % pl;
+        i = 1 + (int)(r % (nx - 2)); j = 1 + (int)(r / (nx - 2)); k = side ? nz - 1 : 0;
+    }
+}
+
+// Q[f] += dt*val for every PML slab containing air node (i,j,k) (air.cpp:2323-2439).
+template <typename T> __device__ __forceinline__ void q_boundary(const BCArgs<T> &a, int f, int i, int j, int k, T val)
+{
+    const Grid<T> &g = a.air;
+    for (int s = 0; s < 6; ++s) {
+        const int n = a.qn[s];
+        if (!n || (s == 2 && !a.y1on)) continue;
+        const int ax = a.qaxis[s], lo = a.qlo[s];
+        const int c = ax == 0 ? i : ax == 1 ? j : k;
+        if (c < lo || c >= lo + n) continue;
+        const int A = ax == 0 ? n : g.nx, B = ax == 1 ? n : g.ny;
+        const size_t q = (size_t)(ax == 0 ? i - lo : i) + (size_t)A * ((size_t)(ax == 1 ? j - lo : j) + (size_t)B * (ax == 2 ? k - lo : k));
+        T *Q = a.Q[s][f];
+        Q[q] = Q[q] + a.dt * val;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bc_p_all(const BCArgs<T> a)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < a.nb_air) {
+        int i, j, k;
+        boundary_node(t, a.air.nx, a.air.ny, a.air.nz, i, j, k);
+        const T v = p_air_walk<T, true>(a, i, j, k, 6);
+        a.air.pnn[gidx(a.air, i, j, k)] = v;
+        q_boundary(a, 3, i, j, k, v);
+    } else if (t < a.nb_air + a.nb_pore) {
+        int i, j, k;
+        boundary_node(t - a.nb_air, a.pore.nx, a.pore.ny, a.pore.nz, i, j, k);
+        a.pore.pnn[gidx(a.pore, i, j, k)] = p_pore_walk(a, i, j, k, 6);
+    } else if (t < a.nb_air + a.nb_pore + a.nrec) {
+        // receivers sample time level n, which this step does not modify (airpore.cpp:573-580)
+        const int r = (int)(t - a.nb_air - a.nb_pore);
+        const long long q = a.rec_idx[r];
+        double v = q >= 0 ? (double)a.air.pn[q] : 0.0;
+        if (fabs(v) < 1.0e-100) v = 0.0;
+        a.rec_out[r] = v;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < a.nb_air) {
+        int i, j, k;
+        boundary_node(t, a.air.nx, a.air.ny, a.air.nz, i, j, k);
+        const T u = v_air_walk<T, true>(a, 0, i, j, k, 6);
+        const T v = v_air_walk<T, true>(a, 1, i, j, k, 6);
+        const T w = v_air_walk<T, true>(a, 2, i, j, k, 6);
+        const size_t id = gidx(a.air, i, j, k);
+        a.air.unn[id] = u; a.air.vnn[id] = v; a.air.wnn[id] = w;
+        q_boundary(a, 0, i, j, k, u);
+        q_boundary(a, 1, i, j, k, v);
+        q_boundary(a, 2, i, j, k, w);
+        if (a.tdbc && j == a.air.ny - 1) {
+            // history update with the TDBC velocity of this node (calculation_2D.cpp:1012-1017)
+            const double vt = tdbc_v(a, i, k);
+            const size_t b = ((size_t)i + (size_t)a.air.nx * k) * 6;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) a.psi_new[b + q] = a.decay[q] * a.psi_old[b + q] + (a.gain[q] * vt) * a.tb;
+        }
+    } else if (t < a.nb_air + a.nb_pore) {
+        int i, j, k;
+        boundary_node(t - a.nb_air, a.pore.nx, a.pore.ny, a.pore.nz, i, j, k);
+        const T u = v_pore_walk(a, 0, i, j, k, 6);
+        const T v = v_pore_walk(a, 1, i, j, k, 6);
+        const T w = v_pore_walk(a, 2, i, j, k, 6);
+        const size_t id = gidx(a.pore, i, j, k);
+        a.pore.unn[id] = u; a.pore.vnn[id] = v; a.pore.wnn[id] = w;
+    }
+}
+
+}  // namespace
+
+template <typename T> long long boundary_count(int nx, int ny, int nz)
+{
+    if (nx <= 0) return 0;
+    return 2LL * ny * nz + 2LL * (nx - 2) * nz + 2LL * (nx - 2) * (ny - 2);
+}
+
+template <typename T> void launch_bc_p_all(const BCArgs<T> &a, hipStream_t s)
+{
+    const long long n = a.nb_air + a.nb_pore + a.nrec;
+    hipLaunchKernelGGL((k_bc_p_all<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+}
+
+template <typename T> void launch_bc_v_all(const BCArgs<T> &a, hipStream_t s)
+{
+    const long long n = a.nb_air + a.nb_pore;
+    hipLaunchKernelGGL((k_bc_v_all<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+}
+
+template long long boundary_count<float>(int, int, int);
+template long long boundary_count<double>(int, int, int);
+template void launch_bc_p_all<float>(const BCArgs<float> &, hipStream_t);
+template void launch_bc_p_all<double>(const BCArgs<double> &, hipStream_t);
+template void launch_bc_v_all<float>(const BCArgs<float> &, hipStream_t);
+template void launch_bc_v_all<double>(const BCArgs<double> &, hipStream_t);
+
+}  // namespace lee

@@ -54,7 +54,7 @@ struct lee_ctx {
     void *mx = nullptr, *my = nullptr, *mz = nullptr, *pmx = nullptr, *pmy = nullptr, *pmz = nullptr;
     void *prof[6][3] = {};
     void *Q[6][4] = {};
-    double *psi = nullptr;
+    double *psi = nullptr, *psi2 = nullptr;  // TDBC histories, read / written each step
     long long *rec_idx = nullptr;
     double *rec_buf = nullptr;
     int *flag = nullptr;
@@ -64,12 +64,20 @@ struct lee_ctx {
     long long steps = 0, counter = 0;
     float last_ms = 0.f;
     std::vector<double> host_rec;
+    // profiling: event pairs around the stencil launches, resolved at the end of lee_step
+    bool profile = false;
+    std::vector<hipEvent_t> evpool;
+    size_t evnext = 0;
+    struct Pending { int kind; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    lee_kstat kstat[4] = {};
 
     ~lee_ctx()
     {
         // teardown: errors cannot be reported from a destructor
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto &a : allocs) if (a.p) (void)hipFree(a.p);
+        for (auto e : evpool) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -135,6 +143,42 @@ int download(lee_ctx *c, double *dst, const void *src, size_t n)
     return LEE_OK;
 }
 
+enum { KS_AIR_P = 0, KS_AIR_V = 1, KS_PORE_P = 2, KS_PORE_V = 3 };
+
+hipEvent_t pooled_event(lee_ctx *c)
+{
+    if (c->evnext == c->evpool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->evpool.push_back(e);
+    }
+    return c->evpool[c->evnext++];
+}
+
+// Record an event before (begin=true) or after a timed launch.
+void prof_mark(lee_ctx *c, int kind, bool begin)
+{
+    if (!c->profile) return;
+    hipEvent_t e = pooled_event(c);
+    if (!e) return;
+    (void)hipEventRecord(e, c->stream);
+    if (begin) c->pending.push_back({kind, e, nullptr});
+    else c->pending.back().b = e;
+}
+
+void prof_resolve(lee_ctx *c)
+{
+    for (auto &p : c->pending) {
+        float ms = 0.f;
+        if (p.a && p.b && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            c->kstat[p.kind].launches += 1;
+            c->kstat[p.kind].total_ms += ms;
+        }
+    }
+    c->pending.clear();
+    c->evnext = 0;
+}
+
 size_t air_n(const lee_ctx *c) { return (size_t)c->m.nx * c->m.ny * c->m.nz; }
 size_t pore_n(const lee_ctx *c) { return (size_t)c->m.pnx * c->m.pny * c->m.pnz; }
 
@@ -164,7 +208,7 @@ template <typename T> Grid<T> pore_grid(lee_ctx *c)
     return g;
 }
 
-template <typename T> AirArgs<T> air_args(lee_ctx *c, int J1)
+template <typename T> AirArgs<T> air_args(lee_ctx *c, int J1, int y1on)
 {
     const Model &m = c->m;
     AirArgs<T> a;
@@ -177,8 +221,32 @@ template <typename T> AirArgs<T> air_args(lee_ctx *c, int J1)
         pr[s]->s = (const T *)c->prof[s][0]; pr[s]->c = (const T *)c->prof[s][1]; pr[s]->d = (const T *)c->prof[s][2];
         for (int f = 0; f < 4; ++f) a.Q[s][f] = (T *)c->Q[s][f];
         a.qn[s] = m.slab[s].n;
+        a.qlo[s] = m.slab[s].lo;
     }
+    a.y1on = y1on;
     return a;
+}
+
+template <typename T> BCArgs<T> bc_args(lee_ctx *c, const AirArgs<T> &aa, double *rec_out)
+{
+    const Model &m = c->m;
+    BCArgs<T> b;
+    b.air = aa.g;
+    if (m.south_porous) b.pore = pore_grid<T>(c);
+    else { std::memset(&b.pore, 0, sizeof b.pore); }
+    b.porous = m.south_porous; b.iface = m.south_porous && m.iface_copy; b.tdbc = m.east_tdbc; b.y1on = aa.y1on;
+    b.nb_air = boundary_count<T>(m.nx, m.ny, m.nz);
+    b.nb_pore = m.south_porous ? boundary_count<T>(m.pnx, m.pny, m.pnz) : 0;
+    for (int s = 0; s < 6; ++s) {
+        for (int f = 0; f < 4; ++f) b.Q[s][f] = aa.Q[s][f];
+        b.qn[s] = m.slab[s].n; b.qlo[s] = m.slab[s].lo; b.qaxis[s] = m.slab[s].axis;
+    }
+    b.dt = (T)m.dt;
+    b.psi_old = c->psi; b.psi_new = c->psi2;
+    for (int q = 0; q < 6; ++q) { b.decay[q] = m.td_decay[q]; b.gain[q] = m.td_gain[q]; }
+    b.inv = m.td_inv; b.Zinf = m.Zinf; b.tb = m.dt / m.tau;
+    b.rec_idx = c->rec_idx; b.nrec = (int)m.ri.size(); b.rec_out = rec_out;
+    return b;
 }
 
 // Upload the per-frame tables (metrics, PML profiles, wind) of frame N.
@@ -258,62 +326,39 @@ template <typename T> int run_steps(lee_ctx *c, int nsteps, double *rec_out)
     HIPCHK(c, hipEventRecord(c->ev0, s));
     int buffered = 0, written = 0;
     for (int t = 0; t < nsteps; ++t) {
-        // receivers before the step (airpore.cpp:573-580)
-        launch_sample<T>((const T *)c->air[c->cur * 4 + 3], c->rec_idx, nr, c->rec_buf + (size_t)buffered * nr, s);
-        ++buffered;
         // y1 time gate (air.cpp:1965-1969)
         int J1 = m.J1;
         if (gate) {
             c->counter += 1;
             J1 = c->counter >= m.pml_time ? m.J3 : 0;
         }
-        const bool y1on = m.slab[2].n > 0 && gate && c->counter >= m.pml_time;
-        AirArgs<T> a = air_args<T>(c, J1);
-        Grid<T> &g = a.g;
-        // ---- pressure phase (airpore.cpp:366-398)
+        const int y1on = (m.slab[2].n > 0 && gate && c->counter >= m.pml_time) ? 1 : 0;
+        AirArgs<T> a = air_args<T>(c, J1, y1on);
+        // ---- pressure phase (airpore.cpp:366-398); receivers sample level n first
+        prof_mark(c, KS_AIR_P, true);
         launch_air_p<T>(a, m.has_wind, s);
-        for (int f : {F_N, F_E, F_W, F_F, F_B}) launch_bc_p<T>(g, f, s);
-        if (!m.south_porous) {
-            launch_bc_p<T>(g, F_S, s);
-        } else {
-            PoreArgs<T> pa{pore_grid<T>(c), (T)m.p_gama, (T)m.p_invbeta, (T)m.p_eff};
+        prof_mark(c, KS_AIR_P, false);
+        PoreArgs<T> pa{};
+        if (m.south_porous) {
+            pa = PoreArgs<T>{pore_grid<T>(c), (T)m.p_gama, (T)m.p_invbeta, (T)m.p_eff};
+            prof_mark(c, KS_PORE_P, true);
             launch_pore_p<T>(pa, s);
-            for (int f : {F_S, F_E, F_W, F_F, F_B}) launch_bc_p<T>(pa.g, f, s);
-            if (m.iface_copy) launch_iface<T>(g, pa.g, 0, s);
+            prof_mark(c, KS_PORE_P, false);
         }
-        for (int sl = 0; sl < 6; ++sl) {
-            const SlabGeom &sg = m.slab[sl];
-            if (!sg.n || (sl == 2 && !y1on)) continue;
-            launch_qadd<T>((T *)c->Q[sl][3], g.pnn, sg.axis, sg.lo, sg.n, m.nx, m.ny, m.nz, (T)m.dt, s);
-        }
+        BCArgs<T> b = bc_args<T>(c, a, c->rec_buf + (size_t)buffered * nr);
+        launch_bc_p_all<T>(b, s);
+        ++buffered;
         // ---- velocity phase (airpore.cpp:324-364)
+        prof_mark(c, KS_AIR_V, true);
         launch_air_v<T>(a, m.has_wind, s);
-        launch_bc_v<T>(g, F_N, s);
-        if (m.east_tdbc) {
-            TdbcArgs<T> ta;
-            ta.g = g; ta.psi = c->psi;
-            for (int q = 0; q < 6; ++q) { ta.decay[q] = m.td_decay[q]; ta.gain[q] = m.td_gain[q]; }
-            ta.inv = m.td_inv; ta.Zinf = m.Zinf; ta.tb = m.dt / m.tau;
-            launch_tdbc<T>(ta, s);
-        } else {
-            launch_bc_v<T>(g, F_E, s);
-        }
-        for (int f : {F_W, F_F, F_B}) launch_bc_v<T>(g, f, s);
-        if (!m.south_porous) {
-            launch_bc_v<T>(g, F_S, s);
-        } else {
-            PoreArgs<T> pa{pore_grid<T>(c), (T)m.p_gama, (T)m.p_invbeta, (T)m.p_eff};
+        prof_mark(c, KS_AIR_V, false);
+        if (m.south_porous) {
+            prof_mark(c, KS_PORE_V, true);
             launch_pore_v<T>(pa, s);
-            for (int f : {F_S, F_E, F_W, F_F, F_B}) launch_bc_v<T>(pa.g, f, s);
-            if (m.iface_copy) launch_iface<T>(g, pa.g, 1, s);
+            prof_mark(c, KS_PORE_V, false);
         }
-        const T *src[3] = {g.unn, g.vnn, g.wnn};
-        for (int sl = 0; sl < 6; ++sl) {
-            const SlabGeom &sg = m.slab[sl];
-            if (!sg.n || (sl == 2 && !y1on)) continue;
-            for (int f = 0; f < 3; ++f)
-                launch_qadd<T>((T *)c->Q[sl][f], src[f], sg.axis, sg.lo, sg.n, m.nx, m.ny, m.nz, (T)m.dt, s);
-        }
+        launch_bc_v_all<T>(b, s);
+        if (m.east_tdbc) std::swap(c->psi, c->psi2);
         // ---- level n+1 becomes level n
         c->cur ^= 1;
         c->steps += 1;
@@ -340,6 +385,7 @@ template <typename T> int run_steps(lee_ctx *c, int nsteps, double *rec_out)
     HIPCHK(c, hipEventRecord(c->ev1, s));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    prof_resolve(c);
     return LEE_OK;
 }
 
@@ -385,6 +431,7 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
             if ((rc = dalloc(C, &C->Q[s][f], cnt * es))) return rc;
     }
     if ((rc = dalloc(C, (void **)&C->psi, (size_t)m.nx * m.nz * 6 * 8))) return rc;
+    if ((rc = dalloc(C, (void **)&C->psi2, (size_t)m.nx * m.nz * 6 * 8))) return rc;
     int nr = (int)m.ri.size();
     if ((rc = dalloc(C, (void **)&C->rec_idx, (size_t)std::max(nr, 1) * sizeof(long long)))) return rc;
     if ((rc = dalloc(C, (void **)&C->rec_buf, (size_t)kRecChunk * std::max(nr, 1) * 8))) return rc;
@@ -545,5 +592,67 @@ int lee_get_psi(lee_ctx *c, double *psi, long long cap)
 }
 
 double lee_last_step_ms(const lee_ctx *c) { return c ? (double)c->last_ms : 0.0; }
+
+int lee_profile(lee_ctx *c, int enable)
+{
+    if (!c) return LEE_ERR_ARG;
+    c->profile = enable != 0;
+    for (auto &k : c->kstat) { k.launches = 0; k.total_ms = 0; }
+    return LEE_OK;
+}
+
+// Compulsory bytes of one launch: every element a kernel must read or write, once.
+int lee_kernel_stats(const lee_ctx *c, lee_kstat *out, int cap)
+{
+    if (!c || !out) return LEE_ERR_ARG;
+    const Model &m = c->m;
+    const double es = (double)c->esz;
+    const double inner = (double)(m.nx - 2) * (m.ny - 2) * (m.nz - 2);
+    // PML auxiliaries per slab element: the pressure kernel reads Q_u, Q_v, Q_w, Q_p of x and y
+    // layers (Q_u, Q_v, Q_p of z layers) and rewrites Q_p; the velocity kernel reads all four
+    // and rewrites Q_u, Q_v, Q_w.
+    double qp = 0, qv = 0;
+    for (int s = 0; s < 6; ++s) {
+        double cnt = (double)m.slab[s].count(m.nx, m.ny, m.nz);
+        if (!cnt || (s == 2 && m.J3 == 0)) continue;
+        bool zs = m.slab[s].axis == 2;
+        qp += cnt * ((zs ? 3 : 4) + 1);
+        qv += cnt * (4 + 3);
+    }
+    double wind = m.has_wind ? 3.0 * m.nx * m.ny * m.nz : 0.0;
+    const double pinner = (double)std::max(m.pnx - 2, 0) * std::max(m.pny - 2, 0) * std::max(m.pnz - 2, 0);
+    const char *names[4] = {"air_p", "air_v", "pore_p", "pore_v"};
+    double bytes[4] = {(5 * inner + qp + wind) * es, (7 * inner + qv + wind) * es, 5 * pinner * es, 7 * pinner * es};
+    double pts[4] = {inner, inner, pinner, pinner};
+    int n = 0;
+    for (int k = 0; k < 4 && n < cap; ++k) {
+        lee_kstat s = c->kstat[k];
+        std::snprintf(s.name, sizeof s.name, "%s", names[k]);
+        s.bytes_per_launch = bytes[k];
+        s.points_per_launch = pts[k];
+        out[n++] = s;
+    }
+    return n;
+}
+
+double lee_copy_bench(lee_ctx *c, long long n_words, int repeats)
+{
+    if (!c || n_words <= 0 || repeats <= 0) return -1.0;
+    void *a = nullptr, *b = nullptr;
+    size_t bytes = (size_t)n_words * 4;
+    if (hipMalloc(&a, bytes) != hipSuccess) return -1.0;
+    if (hipMalloc(&b, bytes) != hipSuccess) { (void)hipFree(a); return -1.0; }
+    (void)hipMemsetAsync(a, 0, bytes, c->stream);
+    launch_copy_words((float *)b, (const float *)a, (size_t)n_words, c->stream);  // warm-up
+    (void)hipEventRecord(c->ev0, c->stream);
+    for (int r = 0; r < repeats; ++r) launch_copy_words((float *)b, (const float *)a, (size_t)n_words, c->stream);
+    (void)hipEventRecord(c->ev1, c->stream);
+    (void)hipEventSynchronize(c->ev1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return (double)ms / repeats;
+}
 
 }  // extern "C"

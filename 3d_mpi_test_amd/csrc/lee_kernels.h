@@ -35,6 +35,8 @@ template <typename T> struct AirArgs {
     DProf<T> x1, x2, y1, y2, z1, z2;
     T *Q[6][4];                // slabs x1 x2 y1 y2 z1 z2, fields u v w p
     int qn[6];                 // slab planes
+    int qlo[6];                // first global plane of each slab
+    int y1on;                  // y1 auxiliaries accumulate (gate open)
 };
 
 template <typename T> struct PoreArgs {
@@ -42,30 +44,35 @@ template <typename T> struct PoreArgs {
     T gama, invbeta, eff;
 };
 
-template <typename T> struct TdbcArgs {
-    Grid<T> g;
-    double *psi;               // [i + nx*k][6], FP64
-    double decay[6], gain[6];
-    double inv, Zinf, tb;
+// All face conditions of one phase, air and porous, in one launch (lee_bc.hip).
+template <typename T> struct BCArgs {
+    Grid<T> air, pore;         // pore.nx == 0 without a porous ground
+    int porous, iface, tdbc, y1on;
+    long long nb_air, nb_pore; // boundary nodes of each box
+    T *Q[6][4];
+    int qn[6], qlo[6], qaxis[6];
+    T dt;
+    const double *psi_old;     // East TDBC histories [i + nx*k][6], read
+    double *psi_new;           // and written (double buffered)
+    double decay[6], gain[6], inv, Zinf, tb;
+    const long long *rec_idx;  // receiver samples of time level n (pressure phase)
+    int nrec;
+    double *rec_out;
 };
+template <typename T> long long boundary_count(int nx, int ny, int nz);
+template <typename T> void launch_bc_p_all(const BCArgs<T> &a, hipStream_t s);
+template <typename T> void launch_bc_v_all(const BCArgs<T> &a, hipStream_t s);
 
 template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s);
 template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStream_t s);
 template <typename T> void launch_pore_p(const PoreArgs<T> &a, hipStream_t s);
 template <typename T> void launch_pore_v(const PoreArgs<T> &a, hipStream_t s);
-template <typename T> void launch_bc_p(const Grid<T> &g, int face, hipStream_t s);
-template <typename T> void launch_bc_v(const Grid<T> &g, int face, hipStream_t s);
-template <typename T> void launch_tdbc(const TdbcArgs<T> &a, hipStream_t s);
-// which = 0: p pair, 1: u v w pairs (calculation_2D.cpp:865-985)
-template <typename T> void launch_iface(const Grid<T> &air, const Grid<T> &pore, int which, hipStream_t s);
-// Q[f] += dt * src over one slab (air.cpp:2323-2439); axis/lo/n describe the slab.
-template <typename T> void launch_qadd(T *q, const T *src, int axis, int lo, int n, int nx, int ny, int nz,
-                                       T dt, hipStream_t s);
 // Moving-frame shift along j of an (A x B x C) box (B = frame length), out of place.
 template <typename T> void launch_shift(T *dst, const T *src, int A, int B, int C, int tempJ, int lead,
                                         int s, hipStream_t st);
 template <typename T> void launch_copy_plane_j(T *p, int nx, int ny, int nz, hipStream_t s);
 template <typename T> void launch_sample(const T *p, const long long *idx, int n, double *out, hipStream_t s);
+void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s);
 template <typename T> void launch_finite_check(const T *f, size_t n, int *flag, hipStream_t s);
 
 }  // namespace lee

@@ -39,6 +39,32 @@ __device__ __forceinline__ size_t qxi(int ii, int j, int k, int n, int ny) { ret
 __device__ __forceinline__ size_t qyi(int i, int jj, int k, int nx, int n) { return (size_t)i + (size_t)nx * ((size_t)jj + (size_t)n * k); }
 __device__ __forceinline__ size_t qzi(int i, int j, int kk, int nx, int ny) { return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * kk); }
 
+// Membership of interior point (i,j,k) in the six PML slabs and its index in each.  The
+// point's own auxiliaries are loaded at the top of a kernel and written back (Q += dt*f,
+// air.cpp:2323-2439) at the end, once the new value is known.
+template <typename T> struct OwnQ {
+    size_t idx[6];
+    bool in[6];
+};
+template <typename T> __device__ __forceinline__ OwnQ<T> own_slabs(const AirArgs<T> &a, int i, int j, int k)
+{
+    const int nx = a.g.nx, ny = a.g.ny;
+    OwnQ<T> o;
+    o.in[0] = a.qn[0] && i < a.qlo[0] + a.qn[0];
+    o.in[1] = a.qn[1] && i >= a.qlo[1];
+    o.in[2] = a.qn[2] && a.y1on && j < a.qlo[2] + a.qn[2];
+    o.in[3] = a.qn[3] && j >= a.qlo[3];
+    o.in[4] = a.qn[4] && k < a.qlo[4] + a.qn[4];
+    o.in[5] = a.qn[5] && k >= a.qlo[5];
+    o.idx[0] = o.in[0] ? qxi(i - a.qlo[0], j, k, a.qn[0], ny) : 0;
+    o.idx[1] = o.in[1] ? qxi(i - a.qlo[1], j, k, a.qn[1], ny) : 0;
+    o.idx[2] = o.in[2] ? qyi(i, j - a.qlo[2], k, nx, a.qn[2]) : 0;
+    o.idx[3] = o.in[3] ? qyi(i, j - a.qlo[3], k, nx, a.qn[3]) : 0;
+    o.idx[4] = o.in[4] ? qzi(i, j, k - a.qlo[4], nx, ny) : 0;
+    o.idx[5] = o.in[5] ? qzi(i, j, k - a.qlo[5], nx, ny) : 0;
+    return o;
+}
+
 // ------------------------------------------------------------------------------------
 // pressure increment + FB add: air::cal_fp (air.cpp:1953-2322) and
 // calculation_2D::Cal_pressure (calculation_2D.cpp:598-608)
@@ -57,6 +83,10 @@ __global__ __launch_bounds__(BX * BY) void k_air_p(const AirArgs<T> a)
     T mX, mY, mZ;
     metric(g, i, j, k, mX, mY, mZ);
     const T dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, dt = g.dt, gP = a.gP;
+    const OwnQ<T> own = own_slabs(a, i, j, k);
+    T qp_old[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) qp_old[s] = own.in[s] ? a.Q[s][3][own.idx[s]] : T(0);
     const T pc = g.pn[id];
     const T dux = g.un[id + 1] - g.un[id];
     const T dvy = g.vn[id + nx] - g.vn[id];
@@ -191,7 +221,13 @@ __global__ __launch_bounds__(BX * BY) void k_air_p(const AirArgs<T> a)
         ycross_z(sz, a.z2.d[k]);
     }
     (void)n4;
-    g.pnn[id] = pc + r;
+    const T pnew = pc + r;
+    g.pnn[id] = pnew;
+    // Q_p += dt*p of this point (air.cpp:2388-2439); the value is final, face conditions
+    // only touch boundary nodes
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+        if (own.in[s]) a.Q[s][3][own.idx[s]] = qp_old[s] + dt * pnew;
 }
 
 // ------------------------------------------------------------------------------------
@@ -212,6 +248,14 @@ __global__ __launch_bounds__(BX * BY) void k_air_v(const AirArgs<T> a)
     T mX, mY, mZ;
     metric(g, i, j, k, mX, mY, mZ);
     const T dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, dt = g.dt, Aav = a.Aav;
+    const OwnQ<T> own = own_slabs(a, i, j, k);
+    T qu_old[6], qv_old[6], qw_old[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        qu_old[s] = own.in[s] ? a.Q[s][0][own.idx[s]] : T(0);
+        qv_old[s] = own.in[s] ? a.Q[s][1][own.idx[s]] : T(0);
+        qw_old[s] = own.in[s] ? a.Q[s][2][own.idx[s]] : T(0);
+    }
     const T *p = g.pnn;
     const T pc = p[id];
     const T gpx = (mX * (pc - p[id - 1])) * dxi;
@@ -329,9 +373,18 @@ __global__ __launch_bounds__(BX * BY) void k_air_v(const AirArgs<T> a)
     if (k < a.K1) zlayer(a.z1.s[k], a.z1.c[k], a.z1.d[k], a.Q[4][3], qzi(i, j, k, nx, ny));
     if (k >= nz - a.K2) zlayer(a.z2.s[k], a.z2.c[k], a.z2.d[k], a.Q[5][3], qzi(i, j, k - (nz - n5), nx, ny));
 #undef QBX
-    g.unn[id] = uc + du;
-    g.vnn[id] = vc + dv;
-    g.wnn[id] = wc + dw;
+    const T un = uc + du, vn = vc + dv, wn = wc + dw;
+    g.unn[id] = un;
+    g.vnn[id] = vn;
+    g.wnn[id] = wn;
+    // Q_u, Q_v, Q_w += dt*(u, v, w) of this point (air.cpp:2323-2386)
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+        if (own.in[s]) {
+            a.Q[s][0][own.idx[s]] = qu_old[s] + dt * un;
+            a.Q[s][1][own.idx[s]] = qv_old[s] + dt * vn;
+            a.Q[s][2][own.idx[s]] = qw_old[s] + dt * wn;
+        }
 }
 
 // ------------------------------------------------------------------------------------
@@ -376,111 +429,6 @@ __global__ __launch_bounds__(BX * BY) void k_pore_v(const PoreArgs<T> a)
     g.wnn[id] = wc + dw;
 }
 
-// ------------------------------------------------------------------------------------
-// boundary conditions (calculation_2D.cpp:646-864); one launch per face, in the
-// reference order, so edges and corners end with the value of the last face applied.
-// ------------------------------------------------------------------------------------
-template <typename T>
-__global__ void k_bc_p(const Grid<T> g, int face)
-{
-    const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    T *p = g.pnn;
-    auto I = [&](int i, int j, int k) { return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k); };
-    switch (face) {
-    case F_W: if (a < nx) p[I(a, 0, b)] = p[I(a, 1, b)]; break;                      // a=i, b=k
-    case F_E:
-        if (a < nx) {
-            const int jn = ny - 1;
-            p[I(a, jn, b)] = T(15.0 / 8.0) * p[I(a, jn - 1, b)] - T(5.0 / 4.0) * p[I(a, jn - 2, b)] + T(3.0 / 8.0) * p[I(a, jn - 3, b)];
-        }
-        break;
-    case F_S: if (a < nx) p[I(a, b, 0)] = p[I(a, b, 1)]; break;                      // a=i, b=j
-    case F_N: if (a < nx) p[I(a, b, nz - 1)] = p[I(a, b, nz - 2)]; break;
-    case F_F: if (a < ny) p[I(0, a, b)] = p[I(1, a, b)]; break;                      // a=j, b=k
-    case F_B: if (a < ny) p[I(nx - 1, a, b)] = p[I(nx - 2, a, b)]; break;
-    }
-}
-
-template <typename T>
-__global__ void k_bc_v(const Grid<T> g, int face)
-{
-    const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    T *u = g.unn, *v = g.vnn, *w = g.wnn;
-    auto I = [&](int i, int j, int k) { return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k); };
-    size_t d, s;
-    switch (face) {
-    case F_W: if (a >= nx) return; d = I(a, 0, b); s = I(a, 1, b); u[d] = u[s]; v[d] = T(0); w[d] = w[s]; break;
-    case F_E: if (a >= nx) return; d = I(a, ny - 1, b); s = I(a, ny - 2, b); u[d] = u[s]; v[d] = T(0); w[d] = w[s]; break;
-    case F_S: if (a >= nx) return; d = I(a, b, 0); s = I(a, b, 1); u[d] = u[s]; v[d] = v[s]; w[d] = T(0); break;
-    case F_N: if (a >= nx) return; d = I(a, b, nz - 1); s = I(a, b, nz - 2); u[d] = u[s]; v[d] = v[s]; w[d] = T(0); break;
-    case F_F: if (a >= ny) return; d = I(0, a, b); s = I(1, a, b); u[d] = T(0); v[d] = v[s]; w[d] = w[s]; break;
-    case F_B: if (a >= ny) return; d = I(nx - 1, a, b); s = I(nx - 2, a, b); u[d] = T(0); v[d] = v[s]; w[d] = w[s]; break;
-    }
-}
-
-// East TDBC (calculation_2D.cpp:986-1028): tangential copy, then v from the recursive
-// convolution, then the FP64 history update, per face point (i,k).
-template <typename T>
-__global__ void k_tdbc(const TdbcArgs<T> a)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
-    const Grid<T> &g = a.g;
-    const int nx = g.nx, ny = g.ny;
-    if (i >= nx) return;
-    const size_t e = (size_t)i + (size_t)nx * ((size_t)(ny - 1) + (size_t)ny * k);
-    const size_t e1 = e - nx;
-    g.unn[e] = g.unn[e1];
-    g.wnn[e] = g.wnn[e1];
-    double *psi = a.psi + ((size_t)i + (size_t)nx * k) * 6;
-    double P[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) P[q] = psi[q];
-    double temp_psi = 0;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) temp_psi += a.decay[q] * P[q];
-    const double vv = a.inv * ((double)g.pnn[e] / a.Zinf - temp_psi);
-#pragma unroll
-    for (int q = 0; q < 6; ++q) psi[q] = a.decay[q] * P[q] + (a.gain[q] * vv) * a.tb;
-    g.vnn[e] = (T)vv;
-}
-
-// Air-porous ghost planes (calculation_2D.cpp:878-886, 957-961).
-template <typename T>
-__global__ void k_iface(const Grid<T> A, const Grid<T> P, int which)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= A.nx) return;
-    const size_t a0 = (size_t)i + (size_t)A.nx * j;
-    const size_t a1 = a0 + (size_t)A.nx * A.ny;
-    const size_t pp = (size_t)P.nx * P.ny;
-    const size_t q2 = (size_t)i + (size_t)P.nx * j + pp * (P.nz - 2);
-    const size_t q1 = q2 + pp;
-    if (which == 0) {
-        A.pnn[a0] = P.pnn[q2];
-        P.pnn[q1] = A.pnn[a1];
-    } else {
-        A.unn[a0] = P.unn[q2]; P.unn[q1] = A.unn[a1];
-        A.vnn[a0] = P.vnn[q2]; P.vnn[q1] = A.vnn[a1];
-        A.wnn[a0] = P.wnn[q2]; P.wnn[q1] = A.wnn[a1];
-    }
-}
-
-template <typename T>
-__global__ void k_qadd(T *q, const T *src, int axis, int lo, int n, int nx, int ny, int nz, T dt)
-{
-    const int A = axis == 0 ? n : nx, B = axis == 1 ? n : ny, C = axis == 2 ? n : nz;
-    const size_t total = (size_t)A * B * C;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const int a = (int)(t % A);
-        const size_t r = t / A;
-        const int b = (int)(r % B), c = (int)(r / B);
-        const int i = axis == 0 ? a + lo : a, j = axis == 1 ? b + lo : b, k = axis == 2 ? c + lo : c;
-        q[t] = q[t] + dt * src[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)];
-    }
-}
-
 // Frame shift along j (calculation_2D.cpp:461-477, air.cpp:2547-2573): out of place.
 template <typename T>
 __global__ void k_shift(T *dst, const T *src, int A, int B, int C, int tempJ, int lead, int s)
@@ -523,6 +471,27 @@ __global__ void k_finite(const T *f, size_t n, int *flag)
         if (!isfinite((double)f[t])) { atomicOr(flag, 1); return; }
 }
 
+// Streaming copy used to calibrate HBM counters: each thread moves 4 x 16 B per pass, with
+// non-temporal loads and stores (the data is touched once).
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_copy_words(v4f *dst, const v4f *src, size_t n4)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; t + 3 * stride < n4; t += 4 * stride) {
+        v4f a = __builtin_nontemporal_load(src + t);
+        v4f b = __builtin_nontemporal_load(src + t + stride);
+        v4f c = __builtin_nontemporal_load(src + t + 2 * stride);
+        v4f d = __builtin_nontemporal_load(src + t + 3 * stride);
+        __builtin_nontemporal_store(a, dst + t);
+        __builtin_nontemporal_store(b, dst + t + stride);
+        __builtin_nontemporal_store(c, dst + t + 2 * stride);
+        __builtin_nontemporal_store(d, dst + t + 3 * stride);
+    }
+    for (; t < n4; t += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + t), dst + t);
+}
+
 dim3 interior_grid(int nx, int ny, int nz) { return dim3((nx - 2 + BX - 1) / BX, (ny - 2 + BY - 1) / BY, nz - 2); }
 
 unsigned flat_blocks(size_t total)
@@ -554,35 +523,6 @@ template <typename T> void launch_pore_v(const PoreArgs<T> &a, hipStream_t s)
 {
     hipLaunchKernelGGL((k_pore_v<T>), interior_grid(a.g.nx, a.g.ny, a.g.nz), dim3(BX, BY), 0, s, a);
 }
-template <typename T> static dim3 face_grid(const Grid<T> &g, int face)
-{
-    switch (face) {
-    case F_W: case F_E: return dim3((g.nx + 63) / 64, g.nz);
-    case F_S: case F_N: return dim3((g.nx + 63) / 64, g.ny);
-    default: return dim3((g.ny + 63) / 64, g.nz);
-    }
-}
-template <typename T> void launch_bc_p(const Grid<T> &g, int face, hipStream_t s)
-{
-    hipLaunchKernelGGL((k_bc_p<T>), face_grid(g, face), dim3(64), 0, s, g, face);
-}
-template <typename T> void launch_bc_v(const Grid<T> &g, int face, hipStream_t s)
-{
-    hipLaunchKernelGGL((k_bc_v<T>), face_grid(g, face), dim3(64), 0, s, g, face);
-}
-template <typename T> void launch_tdbc(const TdbcArgs<T> &a, hipStream_t s)
-{
-    hipLaunchKernelGGL((k_tdbc<T>), dim3((a.g.nx + 63) / 64, a.g.nz), dim3(64), 0, s, a);
-}
-template <typename T> void launch_iface(const Grid<T> &air, const Grid<T> &pore, int which, hipStream_t s)
-{
-    hipLaunchKernelGGL((k_iface<T>), dim3((air.nx + 63) / 64, air.ny), dim3(64), 0, s, air, pore, which);
-}
-template <typename T> void launch_qadd(T *q, const T *src, int axis, int lo, int n, int nx, int ny, int nz, T dt, hipStream_t s)
-{
-    size_t total = (size_t)(axis == 0 ? n : nx) * (axis == 1 ? n : ny) * (axis == 2 ? n : nz);
-    hipLaunchKernelGGL((k_qadd<T>), dim3(flat_blocks(total)), dim3(256), 0, s, q, src, axis, lo, n, nx, ny, nz, dt);
-}
 template <typename T> void launch_shift(T *dst, const T *src, int A, int B, int C, int tempJ, int lead, int sh, hipStream_t st)
 {
     hipLaunchKernelGGL((k_shift<T>), dim3(flat_blocks((size_t)A * B * C)), dim3(256), 0, st, dst, src, A, B, C, tempJ, lead, sh);
@@ -600,16 +540,19 @@ template <typename T> void launch_finite_check(const T *f, size_t n, int *flag, 
     hipLaunchKernelGGL((k_finite<T>), dim3(flat_blocks(n)), dim3(256), 0, s, f, n, flag);
 }
 
+void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s)
+{
+    size_t n4 = n / 4;
+    size_t b = (n4 + 255) / 256;
+    if (b > 4096) b = 4096;
+    hipLaunchKernelGGL(k_copy_words, dim3((unsigned)b), dim3(256), 0, s, (v4f *)dst, (const v4f *)src, n4);
+}
+
 #define LEE_INST(T)                                                                                          \
     template void launch_air_p<T>(const AirArgs<T> &, bool, hipStream_t);                                    \
     template void launch_air_v<T>(const AirArgs<T> &, bool, hipStream_t);                                    \
     template void launch_pore_p<T>(const PoreArgs<T> &, hipStream_t);                                        \
     template void launch_pore_v<T>(const PoreArgs<T> &, hipStream_t);                                        \
-    template void launch_bc_p<T>(const Grid<T> &, int, hipStream_t);                                         \
-    template void launch_bc_v<T>(const Grid<T> &, int, hipStream_t);                                         \
-    template void launch_tdbc<T>(const TdbcArgs<T> &, hipStream_t);                                          \
-    template void launch_iface<T>(const Grid<T> &, const Grid<T> &, int, hipStream_t);                       \
-    template void launch_qadd<T>(T *, const T *, int, int, int, int, int, int, T, hipStream_t);              \
     template void launch_shift<T>(T *, const T *, int, int, int, int, int, int, hipStream_t);                \
     template void launch_copy_plane_j<T>(T *, int, int, int, hipStream_t);                                   \
     template void launch_sample<T>(const T *, const long long *, int, double *, hipStream_t);                \

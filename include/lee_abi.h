@@ -167,6 +167,26 @@ int  lee_get_psi(lee_ctx *ctx, double *psi, long long cap);
 /* Device-side time of the last lee_step call (HIP events on the context stream), ms. */
 double lee_last_step_ms(const lee_ctx *ctx);
 
+/* Per-kernel device time of the stencil kernels, measured with HIP events around each
+ * launch on the context stream while profiling is on.  No reference counterpart (the
+ * reference prints clock() per step, airpore.cpp:587-595); used by bench.py for the
+ * roofline.  bytes_per_launch = compulsory HBM bytes of one launch (every array element
+ * the kernel must read or write counted once). */
+typedef struct lee_kstat {
+    char name[32];
+    long long launches;
+    double total_ms;
+    double bytes_per_launch;
+    double points_per_launch;
+} lee_kstat;
+int  lee_profile(lee_ctx *ctx, int enable);
+int  lee_kernel_stats(const lee_ctx *ctx, lee_kstat *out, int cap);
+
+/* Plain device copy dst[i] = src[i] of n 4-byte words on the context stream, timed with
+ * HIP events (returns ms).  No reference counterpart: calibrates HBM counters and the
+ * copy roofline for bench.py. */
+double lee_copy_bench(lee_ctx *ctx, long long n_words, int repeats);
+
 /* Output writers of the drop-in file contract.
  * ./solution/pty%04d.dat: one %.6e sample per line (airpore.cpp:468-531, 577).
  * ./solution/p_f2_%04d.dat: zero-padded radix-2 FFT, "f |X|" at setprecision(10)
