@@ -26,6 +26,7 @@ deck = importlib.import_module(__name__ + ".deck")
 LeeDeck = deck.LeeDeck
 
 LEE_OK = 0
+PATH_AUTO, PATH_SPLIT, PATH_FUSED = 0, 1, 2
 ERRORS = {-1: "LEE_ERR_ARG", -2: "LEE_ERR_DEVICE", -3: "LEE_ERR_IO", -4: "LEE_ERR_UNSUPPORTED",
           -5: "LEE_ERR_NONFINITE", -6: "LEE_ERR_STATE"}
 
@@ -33,16 +34,8 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
 
-class LeeOptions(ctypes.Structure):
-    _fields_ = [
-        ("precision", ctypes.c_int),
-        ("n_receivers", ctypes.c_int),
-        ("receivers", _dp),
-        ("health_every", ctypes.c_int),
-        ("use_graph", ctypes.c_int),
-        ("device", ctypes.c_int),
-        ("reserved", ctypes.c_int * 16),
-    ]
+LeeOptions = deck.LeeOptions
+INT_DECK, INT_RK4 = deck.INT_DECK, deck.INT_RK4
 
 
 class LeeInfo(ctypes.Structure):
@@ -52,6 +45,7 @@ class LeeInfo(ctypes.Structure):
         ("ijkmax", ctypes.c_int), ("n_receivers", ctypes.c_int), ("precision", ctypes.c_int),
         ("mf_count", ctypes.c_int),
         ("steps_done", ctypes.c_longlong), ("pml_y1_counter", ctypes.c_longlong),
+        ("path", ctypes.c_int),
     ]
 
 
@@ -142,7 +136,8 @@ def free_deck(d: LeeDeck) -> None:
 class Solver:
     """One solver context on one GPU (C ABI of include/lee_abi.h)."""
 
-    def __init__(self, d, precision: int = 64, receivers=None, health_every: int = 0, device: int = 0):
+    def __init__(self, d, precision: int = 64, receivers=None, health_every: int = 0, device: int = 0,
+                 path: int = PATH_AUTO, integrator: Optional[int] = None):
         L = lib()
         if isinstance(d, deck.Deck):
             self._deck_obj = d
@@ -150,16 +145,8 @@ class Solver:
         else:
             self._deck_obj = None
             self._cdeck = d
-        opt = LeeOptions()
-        L.lee_options_default(ctypes.byref(opt))
-        opt.precision = precision
-        opt.health_every = health_every
-        opt.device = device
-        self._rec = None
-        if receivers is not None:
-            self._rec = np.ascontiguousarray(receivers, dtype=np.float64).reshape(-1)
-            opt.n_receivers = self._rec.size // 3
-            opt.receivers = _p(self._rec)
+        opt = deck.make_options(self._deck_obj, precision=precision, receivers=receivers,
+                                health_every=health_every, device=device, path=path, integrator=integrator)
         self._opt = opt
         h = ctypes.c_void_p()
         _check(L.lee_create(ctypes.byref(self._cdeck), ctypes.byref(opt), ctypes.byref(h)))
@@ -168,6 +155,7 @@ class Solver:
         self.nx, self.ny, self.nz = info.nx, info.ny, info.nz
         self.pnx, self.pny, self.pnz = info.pnx, info.pny, info.pnz
         self.nrec = info.n_receivers
+        self.path = info.path
         self.time_domain = self._cdeck.time_domain
 
     def close(self):

@@ -150,7 +150,8 @@ __device__ __forceinline__ void boundary_node(long long t, int nx, int ny, int n
     }
 }
 
-// Q[f] += dt*val for every PML slab containing air node (i,j,k) (air.cpp:2323-2439).
+// Q[f] <- Q[f] + dt*val (new set from old) for every PML slab containing air node (i,j,k)
+// (air.cpp:2323-2439).
 template <typename T> __device__ __forceinline__ void q_boundary(const BCArgs<T> &a, int f, int i, int j, int k, T val)
 {
     const Grid<T> &g = a.air;
@@ -162,8 +163,7 @@ template <typename T> __device__ __forceinline__ void q_boundary(const BCArgs<T>
         if (c < lo || c >= lo + n) continue;
         const int A = ax == 0 ? n : g.nx, B = ax == 1 ? n : g.ny;
         const size_t q = (size_t)(ax == 0 ? i - lo : i) + (size_t)A * ((size_t)(ax == 1 ? j - lo : j) + (size_t)B * (ax == 2 ? k - lo : k));
-        T *Q = a.Q[s][f];
-        Q[q] = Q[q] + a.dt * val;
+        a.Qn[s][f][q] = a.Qo[s][f][q] + a.dt * val;
     }
 }
 

@@ -53,7 +53,9 @@ struct lee_ctx {
     void *U = nullptr, *V = nullptr, *W = nullptr;
     void *mx = nullptr, *my = nullptr, *mz = nullptr, *pmx = nullptr, *pmy = nullptr, *pmz = nullptr;
     void *prof[6][3] = {};
-    void *Q[6][4] = {};
+    void *Q[2][6][4] = {};  // PML auxiliaries, two sets: Q[qcur] holds the values before a step
+    int qcur = 0;
+    bool fused = false;     // both air phases in one kernel (k_air_fb)
     double *psi = nullptr, *psi2 = nullptr;  // TDBC histories, read / written each step
     long long *rec_idx = nullptr;
     double *rec_buf = nullptr;
@@ -70,7 +72,7 @@ struct lee_ctx {
     size_t evnext = 0;
     struct Pending { int kind; hipEvent_t a, b; };
     std::vector<Pending> pending;
-    lee_kstat kstat[4] = {};
+    lee_kstat kstat[8] = {};
 
     ~lee_ctx()
     {
@@ -143,7 +145,7 @@ int download(lee_ctx *c, double *dst, const void *src, size_t n)
     return LEE_OK;
 }
 
-enum { KS_AIR_P = 0, KS_AIR_V = 1, KS_PORE_P = 2, KS_PORE_V = 3 };
+enum { KS_AIR_P = 0, KS_AIR_V = 1, KS_PORE_P = 2, KS_PORE_V = 3, KS_AIR_FB = 4, KS_BC_P = 5, KS_BC_V = 6, KS_N = 7 };
 
 hipEvent_t pooled_event(lee_ctx *c)
 {
@@ -219,11 +221,17 @@ template <typename T> AirArgs<T> air_args(lee_ctx *c, int J1, int y1on)
     DProf<T> *pr[6] = {&a.x1, &a.x2, &a.y1, &a.y2, &a.z1, &a.z2};
     for (int s = 0; s < 6; ++s) {
         pr[s]->s = (const T *)c->prof[s][0]; pr[s]->c = (const T *)c->prof[s][1]; pr[s]->d = (const T *)c->prof[s][2];
-        for (int f = 0; f < 4; ++f) a.Q[s][f] = (T *)c->Q[s][f];
+        for (int f = 0; f < 4; ++f) {
+            a.Qo[s][f] = (const T *)c->Q[c->qcur][s][f];
+            a.Qn[s][f] = (T *)c->Q[c->qcur ^ 1][s][f];
+        }
         a.qn[s] = m.slab[s].n;
         a.qlo[s] = m.slab[s].lo;
     }
     a.y1on = y1on;
+    a.south_pore = m.south_porous && m.iface_copy;
+    a.pore_pnn = m.south_porous ? pore_grid<T>(c).pnn : nullptr;
+    a.pore_nz = m.pnz;
     return a;
 }
 
@@ -238,7 +246,7 @@ template <typename T> BCArgs<T> bc_args(lee_ctx *c, const AirArgs<T> &aa, double
     b.nb_air = boundary_count<T>(m.nx, m.ny, m.nz);
     b.nb_pore = m.south_porous ? boundary_count<T>(m.pnx, m.pny, m.pnz) : 0;
     for (int s = 0; s < 6; ++s) {
-        for (int f = 0; f < 4; ++f) b.Q[s][f] = aa.Q[s][f];
+        for (int f = 0; f < 4; ++f) { b.Qo[s][f] = aa.Qo[s][f]; b.Qn[s][f] = aa.Qn[s][f]; }
         b.qn[s] = m.slab[s].n; b.qlo[s] = m.slab[s].lo; b.qaxis[s] = m.slab[s].axis;
     }
     b.dt = (T)m.dt;
@@ -309,8 +317,9 @@ template <typename T> int frame_shift(lee_ctx *c, int N)
         int A = g.axis == 0 ? g.n : m.nx, C = g.axis == 2 ? g.n : m.nz;
         size_t cnt = (size_t)A * m.ny * C;
         for (int f = 0; f < 4; ++f) {
-            HIPCHK(c, hipMemcpyAsync(c->scratch, c->Q[sl][f], cnt * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
-            launch_shift<T>((T *)c->Q[sl][f], (const T *)c->scratch, A, m.ny, C, tempJ, m.lead, s, c->stream);
+            void *q = c->Q[c->qcur][sl][f];
+            HIPCHK(c, hipMemcpyAsync(c->scratch, q, cnt * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+            launch_shift<T>((T *)q, (const T *)c->scratch, A, m.ny, C, tempJ, m.lead, s, c->stream);
         }
     }
     HIPCHK(c, hipGetLastError());
@@ -334,33 +343,53 @@ template <typename T> int run_steps(lee_ctx *c, int nsteps, double *rec_out)
         }
         const int y1on = (m.slab[2].n > 0 && gate && c->counter >= m.pml_time) ? 1 : 0;
         AirArgs<T> a = air_args<T>(c, J1, y1on);
-        // ---- pressure phase (airpore.cpp:366-398); receivers sample level n first
-        prof_mark(c, KS_AIR_P, true);
-        launch_air_p<T>(a, m.has_wind, s);
-        prof_mark(c, KS_AIR_P, false);
         PoreArgs<T> pa{};
-        if (m.south_porous) {
-            pa = PoreArgs<T>{pore_grid<T>(c), (T)m.p_gama, (T)m.p_invbeta, (T)m.p_eff};
-            prof_mark(c, KS_PORE_P, true);
-            launch_pore_p<T>(pa, s);
-            prof_mark(c, KS_PORE_P, false);
-        }
+        if (m.south_porous) pa = PoreArgs<T>{pore_grid<T>(c), (T)m.p_gama, (T)m.p_invbeta, (T)m.p_eff};
         BCArgs<T> b = bc_args<T>(c, a, c->rec_buf + (size_t)buffered * nr);
-        launch_bc_p_all<T>(b, s);
+        if (c->fused) {
+            // porous pressure first: the air South face reads its plane Kp-2 through the interface
+            if (m.south_porous) {
+                prof_mark(c, KS_PORE_P, true);
+                launch_pore_p<T>(pa, s);
+                prof_mark(c, KS_PORE_P, false);
+            }
+            prof_mark(c, KS_AIR_FB, true);
+            launch_air_fb<T>(a, m.has_wind, s);
+            prof_mark(c, KS_AIR_FB, false);
+            prof_mark(c, KS_BC_P, true);
+            launch_bc_p_all<T>(b, s);
+            prof_mark(c, KS_BC_P, false);
+        } else {
+            // ---- pressure phase (airpore.cpp:366-398); receivers sample level n first
+            prof_mark(c, KS_AIR_P, true);
+            launch_air_p<T>(a, m.has_wind, s);
+            prof_mark(c, KS_AIR_P, false);
+            if (m.south_porous) {
+                prof_mark(c, KS_PORE_P, true);
+                launch_pore_p<T>(pa, s);
+                prof_mark(c, KS_PORE_P, false);
+            }
+            prof_mark(c, KS_BC_P, true);
+            launch_bc_p_all<T>(b, s);
+            prof_mark(c, KS_BC_P, false);
+            // ---- velocity phase (airpore.cpp:324-364)
+            prof_mark(c, KS_AIR_V, true);
+            launch_air_v<T>(a, m.has_wind, s);
+            prof_mark(c, KS_AIR_V, false);
+        }
         ++buffered;
-        // ---- velocity phase (airpore.cpp:324-364)
-        prof_mark(c, KS_AIR_V, true);
-        launch_air_v<T>(a, m.has_wind, s);
-        prof_mark(c, KS_AIR_V, false);
         if (m.south_porous) {
             prof_mark(c, KS_PORE_V, true);
             launch_pore_v<T>(pa, s);
             prof_mark(c, KS_PORE_V, false);
         }
+        prof_mark(c, KS_BC_V, true);
         launch_bc_v_all<T>(b, s);
+        prof_mark(c, KS_BC_V, false);
         if (m.east_tdbc) std::swap(c->psi, c->psi2);
         // ---- level n+1 becomes level n
         c->cur ^= 1;
+        c->qcur ^= 1;
         c->steps += 1;
         if (buffered == kRecChunk || t == nsteps - 1) {
             if (rec_out && nr) {
@@ -402,6 +431,16 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
     lee_ctx *C = c.get();
     C->prec = o.precision;
     C->esz = o.precision == 64 ? 8 : 4;
+    {
+        const Model &mm = C->m;
+        const bool can = fused_fits<double>(mm.nx) && !(mm.south_porous && !mm.iface_copy);
+        if (o.path == LEE_PATH_FUSED && !can)
+            return set_err(nullptr, LEE_ERR_UNSUPPORTED, "the fused path needs IMAX - 2 <= 1024 and, with a porous South face, the interface copy");
+        if (o.path < LEE_PATH_AUTO || o.path > LEE_PATH_FUSED) return set_err(nullptr, LEE_ERR_ARG, "unknown path");
+        // AUTO runs the split passes: on MI355X they are faster than the fused pass (whose
+        // per-plane barrier limits the loads in flight)
+        C->fused = o.path == LEE_PATH_FUSED;
+    }
     HIPCHK(C, hipSetDevice(o.device));
     HIPCHK(C, hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
     HIPCHK(C, hipEventCreate(&C->ev0));
@@ -427,8 +466,9 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
             if ((rc = dalloc(C, &C->prof[s][k], (size_t)len[s] * es))) return rc;
         size_t cnt = m.slab[s].count(m.nx, m.ny, m.nz);
         qmax = std::max(qmax, cnt);
-        for (int f = 0; f < 4; ++f)
-            if ((rc = dalloc(C, &C->Q[s][f], cnt * es))) return rc;
+        for (int h = 0; h < 2; ++h)
+            for (int f = 0; f < 4; ++f)
+                if ((rc = dalloc(C, &C->Q[h][s][f], cnt * es))) return rc;
     }
     if ((rc = dalloc(C, (void **)&C->psi, (size_t)m.nx * m.nz * 6 * 8))) return rc;
     if ((rc = dalloc(C, (void **)&C->psi2, (size_t)m.nx * m.nz * 6 * 8))) return rc;
@@ -488,6 +528,7 @@ int lee_get_info(const lee_ctx *c, lee_info *info)
     info->mf_count = c->mf_count;
     info->steps_done = c->steps;
     info->pml_y1_counter = c->counter;
+    info->path = c->fused ? LEE_PATH_FUSED : LEE_PATH_SPLIT;
     return LEE_OK;
 }
 
@@ -508,11 +549,12 @@ int lee_frame_setup(lee_ctx *c, int N)
     int rc = c->prec == 64 ? frame_shift<double>(c, N) : frame_shift<float>(c, N);
     if (rc) return rc;
     if (N == 0)  // air::SetQMove(0) zeroes every auxiliary (air.cpp:2453-2519)
-        for (int s = 0; s < 6; ++s)
-            for (int f = 0; f < 4; ++f) {
-                size_t cnt = c->m.slab[s].count(c->m.nx, c->m.ny, c->m.nz);
-                if (cnt) HIPCHK(c, hipMemsetAsync(c->Q[s][f], 0, cnt * c->esz, c->stream));
-            }
+        for (int h = 0; h < 2; ++h)
+            for (int s = 0; s < 6; ++s)
+                for (int f = 0; f < 4; ++f) {
+                    size_t cnt = c->m.slab[s].count(c->m.nx, c->m.ny, c->m.nz);
+                    if (cnt) HIPCHK(c, hipMemsetAsync(c->Q[h][s][f], 0, cnt * c->esz, c->stream));
+                }
     rc = upload_frame(c, N);
     if (rc) return rc;
     c->mf_count = N;
@@ -621,11 +663,25 @@ int lee_kernel_stats(const lee_ctx *c, lee_kstat *out, int cap)
     }
     double wind = m.has_wind ? 3.0 * m.nx * m.ny * m.nz : 0.0;
     const double pinner = (double)std::max(m.pnx - 2, 0) * std::max(m.pny - 2, 0) * std::max(m.pnz - 2, 0);
-    const char *names[4] = {"air_p", "air_v", "pore_p", "pore_v"};
-    double bytes[4] = {(5 * inner + qp + wind) * es, (7 * inner + qv + wind) * es, 5 * pinner * es, 7 * pinner * es};
-    double pts[4] = {inner, inner, pinner, pinner};
+    // fused pass: read u, v, w, p and write them once; per slab element read the four old
+    // auxiliaries and write the four new ones
+    double qf = 0;
+    for (int s = 0; s < 6; ++s) {
+        double cnt = (double)m.slab[s].count(m.nx, m.ny, m.nz);
+        if (!cnt || (s == 2 && m.J3 == 0)) continue;
+        qf += cnt * 8;
+    }
+    // faces: every boundary node written once from one source node (pressure) or three
+    // (velocity), plus the East TDBC histories (FP64, read and written)
+    const double nb = (double)boundary_count<double>(m.nx, m.ny, m.nz) +
+                      (m.south_porous ? (double)boundary_count<double>(m.pnx, m.pny, m.pnz) : 0.0);
+    const double psi = m.east_tdbc ? (double)m.nx * m.nz * 6 * 8 * 2 : 0.0;
+    const char *names[KS_N] = {"air_p", "air_v", "pore_p", "pore_v", "air_fb", "bc_p", "bc_v"};
+    double bytes[KS_N] = {(5 * inner + qp + wind) * es, (7 * inner + qv + wind) * es, 5 * pinner * es, 7 * pinner * es,
+                          (8 * inner + qf + wind) * es, 2 * nb * es, 6 * nb * es + psi};
+    double pts[KS_N] = {inner, inner, pinner, pinner, inner, nb, nb};
     int n = 0;
-    for (int k = 0; k < 4 && n < cap; ++k) {
+    for (int k = 0; k < KS_N && n < cap; ++k) {
         lee_kstat s = c->kstat[k];
         std::snprintf(s.name, sizeof s.name, "%s", names[k]);
         s.bytes_per_launch = bytes[k];

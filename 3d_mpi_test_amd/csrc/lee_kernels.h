@@ -33,10 +33,16 @@ template <typename T> struct AirArgs {
     T gam, Pav, gP, Aav;
     int I1, I2, J1, J2, K1, K2;  // active PML widths (J1 carries the y1 time gate)
     DProf<T> x1, x2, y1, y2, z1, z2;
-    T *Q[6][4];                // slabs x1 x2 y1 y2 z1 z2, fields u v w p
+    const T *Qo[6][4];         // slabs x1 x2 y1 y2 z1 z2, fields u v w p: before the step
+    T *Qn[6][4];               // and after it
     int qn[6];                 // slab planes
     int qlo[6];                // first global plane of each slab
     int y1on;                  // y1 auxiliaries accumulate (gate open)
+    // fused step only: the South face value source (0 copy of plane k=1, 1 porous plane
+    // Kp-2 through the interface) and the porous pressure it reads
+    int south_pore;
+    const T *pore_pnn;
+    int pore_nz;
 };
 
 template <typename T> struct PoreArgs {
@@ -49,7 +55,8 @@ template <typename T> struct BCArgs {
     Grid<T> air, pore;         // pore.nx == 0 without a porous ground
     int porous, iface, tdbc, y1on;
     long long nb_air, nb_pore; // boundary nodes of each box
-    T *Q[6][4];
+    const T *Qo[6][4];
+    T *Qn[6][4];
     int qn[6], qlo[6], qaxis[6];
     T dt;
     const double *psi_old;     // East TDBC histories [i + nx*k][6], read
@@ -65,6 +72,10 @@ template <typename T> void launch_bc_v_all(const BCArgs<T> &a, hipStream_t s);
 
 template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s);
 template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStream_t s);
+// Both air phases in one pass (pressure then velocity, plane by plane); see lee_kernels.hip.
+// Returns false when the grid does not fit the fused kernel (nx - 2 > 1024).
+template <typename T> bool fused_fits(int nx);
+template <typename T> void launch_air_fb(const AirArgs<T> &a, bool wind, hipStream_t s);
 template <typename T> void launch_pore_p(const PoreArgs<T> &a, hipStream_t s);
 template <typename T> void launch_pore_v(const PoreArgs<T> &a, hipStream_t s);
 // Moving-frame shift along j of an (A x B x C) box (B = frame length), out of place.

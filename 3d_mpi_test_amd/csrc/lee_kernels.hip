@@ -13,6 +13,7 @@
 // result can differ).
 #include "lee_kernels.h"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace lee {
@@ -66,18 +67,15 @@ template <typename T> __device__ __forceinline__ OwnQ<T> own_slabs(const AirArgs
 }
 
 // ------------------------------------------------------------------------------------
-// pressure increment + FB add: air::cal_fp (air.cpp:1953-2322) and
-// calculation_2D::Cal_pressure (calculation_2D.cpp:598-608)
+// pressure increment + FB add at one interior point: air::cal_fp (air.cpp:1953-2322) and
+// calculation_2D::Cal_pressure (calculation_2D.cpp:598-608).  Writes p^{n+1} and the
+// point's Q_p (Update_PML_Qp, air.cpp:2388-2439) and returns p^{n+1}.
 // ------------------------------------------------------------------------------------
 template <typename T, bool WIND>
-__global__ __launch_bounds__(BX * BY) void k_air_p(const AirArgs<T> a)
+__device__ __forceinline__ T air_p_point(const AirArgs<T> &a, int i, int j, int k)
 {
     const Grid<T> &g = a.g;
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    if (i >= nx - 1 || j >= ny - 1) return;
     const size_t sxy = (size_t)nx * ny;
     const size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
     T mX, mY, mZ;
@@ -86,7 +84,7 @@ __global__ __launch_bounds__(BX * BY) void k_air_p(const AirArgs<T> a)
     const OwnQ<T> own = own_slabs(a, i, j, k);
     T qp_old[6];
 #pragma unroll
-    for (int s = 0; s < 6; ++s) qp_old[s] = own.in[s] ? a.Q[s][3][own.idx[s]] : T(0);
+    for (int s = 0; s < 6; ++s) qp_old[s] = own.in[s] ? a.Qo[s][3][own.idx[s]] : T(0);
     const T pc = g.pn[id];
     const T dux = g.un[id + 1] - g.un[id];
     const T dvy = g.vn[id + nx] - g.vn[id];
@@ -110,7 +108,7 @@ __global__ __launch_bounds__(BX * BY) void k_air_p(const AirArgs<T> a)
     if (i < a.I1) {
         const T sx = a.x1.s[i], cx = a.x1.c[i];
         const size_t q = qxi(i, j, k, n0, ny);
-        const T *Qv = a.Q[0][1], *Qw = a.Q[0][2];
+        const T *Qv = a.Qo[0][1], *Qw = a.Qo[0][2];
         const T A = (cx * (gP * ((mX * dux) * dxi))) * dt;
         const T B = (sx * (pc + gP * (((mY * (Qv[q + n0] - Qv[q])) * dyi) + ((mZ * (Qw[q + (size_t)n0 * ny] - Qw[q])) * dzi)))) * dt;
         r += A - B;
@@ -120,7 +118,7 @@ __global__ __launch_bounds__(BX * BY) void k_air_p(const AirArgs<T> a)
         const int ii = i - (nx - n1);
         const T sx = a.x2.s[i], cx = a.x2.c[i];
         const size_t q = qxi(ii, j, k, n1, ny);
-        const T *Qv = a.Q[1][1], *Qw = a.Q[1][2];
+        const T *Qv = a.Qo[1][1], *Qw = a.Qo[1][2];
         const T A = (cx * (gP * ((mX * dux) * dxi))) * dt;
         const T B = (sx * (pc + gP * (((mY * (Qv[q + n1] - Qv[q])) * dyi) + ((mZ * (Qw[q + (size_t)n1 * ny] - Qw[q])) * dzi)))) * dt;
         r += A - B;
@@ -131,39 +129,39 @@ __global__ __launch_bounds__(BX * BY) void k_air_p(const AirArgs<T> a)
         if (i < a.I1) {
             const T sx = a.x1.s[i], dxm = a.x1.d[i];
             const size_t q = qxi(i, j, k, n0, ny);
-            const T t1 = (dxm * so) * ((mX * (a.Q[0][0][q + 1] - a.Q[0][0][q])) * dxi);
-            const T t2 = (dom * sx) * ((mY * (a.Q[0][1][q + n0] - a.Q[0][1][q])) * dyi);
-            r += (-((a.Q[0][3][q] * sx) * so + gP * (t1 + t2))) * dt;
+            const T t1 = (dxm * so) * ((mX * (a.Qo[0][0][q + 1] - a.Qo[0][0][q])) * dxi);
+            const T t2 = (dom * sx) * ((mY * (a.Qo[0][1][q + n0] - a.Qo[0][1][q])) * dyi);
+            r += (-((a.Qo[0][3][q] * sx) * so + gP * (t1 + t2))) * dt;
         } else if (i >= nx - a.I2) {
             const int ii = i - (nx - n1);
             const T sx = a.x2.s[i], dxm = a.x2.d[i];
             const size_t q = qxi(ii, j, k, n1, ny);
-            const T t1 = (dxm * so) * ((mX * (a.Q[1][0][q + 1] - a.Q[1][0][q])) * dxi);
-            const T t2 = (dom * sx) * ((mY * (a.Q[1][1][q + n1] - a.Q[1][1][q])) * dyi);
-            r += (-((a.Q[1][3][q] * sx) * so + gP * (t1 + t2))) * dt;
+            const T t1 = (dxm * so) * ((mX * (a.Qo[1][0][q + 1] - a.Qo[1][0][q])) * dxi);
+            const T t2 = (dom * sx) * ((mY * (a.Qo[1][1][q + n1] - a.Qo[1][1][q])) * dyi);
+            r += (-((a.Qo[1][3][q] * sx) * so + gP * (t1 + t2))) * dt;
         }
     };
     auto xcross_z = [&](T so, T dom) {
         if (i < a.I1) {
             const T sx = a.x1.s[i], dxm = a.x1.d[i];
             const size_t q = qxi(i, j, k, n0, ny);
-            const T t1 = (dxm * so) * ((mX * (a.Q[0][0][q + 1] - a.Q[0][0][q])) * dxi);
-            const T t2 = (dom * sx) * ((mZ * (a.Q[0][2][q + (size_t)n0 * ny] - a.Q[0][2][q])) * dzi);
-            r += (-((a.Q[0][3][q] * sx) * so + gP * (t1 + t2))) * dt;
+            const T t1 = (dxm * so) * ((mX * (a.Qo[0][0][q + 1] - a.Qo[0][0][q])) * dxi);
+            const T t2 = (dom * sx) * ((mZ * (a.Qo[0][2][q + (size_t)n0 * ny] - a.Qo[0][2][q])) * dzi);
+            r += (-((a.Qo[0][3][q] * sx) * so + gP * (t1 + t2))) * dt;
         } else if (i >= nx - a.I2) {
             const int ii = i - (nx - n1);
             const T sx = a.x2.s[i], dxm = a.x2.d[i];
             const size_t q = qxi(ii, j, k, n1, ny);
-            const T t1 = (dxm * so) * ((mX * (a.Q[1][0][q + 1] - a.Q[1][0][q])) * dxi);
-            const T t2 = (dom * sx) * ((mZ * (a.Q[1][2][q + (size_t)n1 * ny] - a.Q[1][2][q])) * dzi);
-            r += (-((a.Q[1][3][q] * sx) * so + gP * (t1 + t2))) * dt;
+            const T t1 = (dxm * so) * ((mX * (a.Qo[1][0][q + 1] - a.Qo[1][0][q])) * dxi);
+            const T t2 = (dom * sx) * ((mZ * (a.Qo[1][2][q + (size_t)n1 * ny] - a.Qo[1][2][q])) * dzi);
+            r += (-((a.Qo[1][3][q] * sx) * so + gP * (t1 + t2))) * dt;
         }
     };
     // Y1 (air.cpp:2076-2116)
     if (j < a.J1) {
         const T sy = a.y1.s[j], cy = a.y1.c[j];
         const size_t q = qyi(i, j, k, nx, n2);
-        const T *Qu = a.Q[2][0], *Qw = a.Q[2][2];
+        const T *Qu = a.Qo[2][0], *Qw = a.Qo[2][2];
         const T A = (cy * (gP * ((mY * dvy) * dyi))) * dt;
         const T B = (sy * (pc + gP * (((mX * (Qu[q + 1] - Qu[q])) * dxi) + ((mZ * (Qw[q + (size_t)nx * n2] - Qw[q])) * dzi)))) * dt;
         r += A - B;
@@ -174,7 +172,7 @@ __global__ __launch_bounds__(BX * BY) void k_air_p(const AirArgs<T> a)
         const int jj = j - (ny - n3);
         const T sy = a.y2.s[j], cy = a.y2.c[j];
         const size_t q = qyi(i, jj, k, nx, n3);
-        const T *Qu = a.Q[3][0], *Qw = a.Q[3][2];
+        const T *Qu = a.Qo[3][0], *Qw = a.Qo[3][2];
         const T A = (cy * (gP * ((mY * dvy) * dyi))) * dt;
         const T B = (sy * (pc + gP * (((mX * (Qu[q + 1] - Qu[q])) * dxi) + ((mZ * (Qw[q + (size_t)nx * n3] - Qw[q])) * dzi)))) * dt;
         r += A - B;
@@ -185,23 +183,23 @@ __global__ __launch_bounds__(BX * BY) void k_air_p(const AirArgs<T> a)
         if (j < a.J1) {
             const T sy = a.y1.s[j], dym = a.y1.d[j];
             const size_t q = qyi(i, j, k, nx, n2);
-            const T t1 = (dym * sz) * ((mY * (a.Q[2][1][q + nx] - a.Q[2][1][q])) * dyi);
-            const T t2 = (dzm * sy) * ((mZ * (a.Q[2][2][q + (size_t)nx * n2] - a.Q[2][2][q])) * dzi);
-            r += (-((a.Q[2][3][q] * sy) * sz + gP * (t1 + t2))) * dt;
+            const T t1 = (dym * sz) * ((mY * (a.Qo[2][1][q + nx] - a.Qo[2][1][q])) * dyi);
+            const T t2 = (dzm * sy) * ((mZ * (a.Qo[2][2][q + (size_t)nx * n2] - a.Qo[2][2][q])) * dzi);
+            r += (-((a.Qo[2][3][q] * sy) * sz + gP * (t1 + t2))) * dt;
         } else if (j >= ny - a.J2) {
             const int jj = j - (ny - n3);
             const T sy = a.y2.s[j], dym = a.y2.d[j];
             const size_t q = qyi(i, jj, k, nx, n3);
-            const T t1 = (dym * sz) * ((mY * (a.Q[3][1][q + nx] - a.Q[3][1][q])) * dyi);
-            const T t2 = (dzm * sy) * ((mZ * (a.Q[3][2][q + (size_t)nx * n3] - a.Q[3][2][q])) * dzi);
-            r += (-((a.Q[3][3][q] * sy) * sz + gP * (t1 + t2))) * dt;
+            const T t1 = (dym * sz) * ((mY * (a.Qo[3][1][q + nx] - a.Qo[3][1][q])) * dyi);
+            const T t2 = (dzm * sy) * ((mZ * (a.Qo[3][2][q + (size_t)nx * n3] - a.Qo[3][2][q])) * dzi);
+            r += (-((a.Qo[3][3][q] * sy) * sz + gP * (t1 + t2))) * dt;
         }
     };
     // Z1 (air.cpp:2160-2225)
     if (k < a.K1) {
         const T sz = a.z1.s[k], cz = a.z1.c[k];
         const size_t q = qzi(i, j, k, nx, ny);
-        const T *Qu = a.Q[4][0], *Qv = a.Q[4][1];
+        const T *Qu = a.Qo[4][0], *Qv = a.Qo[4][1];
         const T A = (((cz * a.gam) * a.Pav) * ((mZ * dwz) * dzi)) * dt;
         const T B = (sz * (pc + gP * (((mX * (Qu[q + 1] - Qu[q])) * dxi) + ((mY * (Qv[q + nx] - Qv[q])) * dyi)))) * dt;
         r += A - B;
@@ -213,7 +211,7 @@ __global__ __launch_bounds__(BX * BY) void k_air_p(const AirArgs<T> a)
         const int kk = k - (nz - n5);
         const T sz = a.z2.s[k], cz = a.z2.c[k];
         const size_t q = qzi(i, j, kk, nx, ny);
-        const T *Qu = a.Q[5][0], *Qv = a.Q[5][1];
+        const T *Qu = a.Qo[5][0], *Qv = a.Qo[5][1];
         const T A = (((cz * a.gam) * a.Pav) * ((mZ * dwz) * dzi)) * dt;
         const T B = (sz * (pc + gP * (((mX * (Qu[q + 1] - Qu[q])) * dxi) + ((mY * (Qv[q + nx] - Qv[q])) * dyi)))) * dt;
         r += A - B;
@@ -223,26 +221,24 @@ __global__ __launch_bounds__(BX * BY) void k_air_p(const AirArgs<T> a)
     (void)n4;
     const T pnew = pc + r;
     g.pnn[id] = pnew;
-    // Q_p += dt*p of this point (air.cpp:2388-2439); the value is final, face conditions
-    // only touch boundary nodes
+    // Q_p += dt*p of this point; the value is final, face conditions only touch boundary nodes
 #pragma unroll
     for (int s = 0; s < 6; ++s)
-        if (own.in[s]) a.Q[s][3][own.idx[s]] = qp_old[s] + dt * pnew;
+        if (own.in[s]) a.Qn[s][3][own.idx[s]] = qp_old[s] + dt * pnew;
+    return pnew;
 }
 
 // ------------------------------------------------------------------------------------
-// velocity increments + FB add: air::cal_fuvw (air.cpp:1476-1949) with p at n+1, and
-// calculation_2D::Cal_velocity (calculation_2D.cpp:564-574)
+// velocity increments + FB add at one interior point: air::cal_fuvw (air.cpp:1476-1949)
+// with p at n+1, and calculation_2D::Cal_velocity (calculation_2D.cpp:564-574).  Reads
+// p^{n+1} and Q_p^{n+1} of the point and its backward neighbours; writes u, v, w and the
+// point's Q_u, Q_v, Q_w (Update_PML_Qvw, air.cpp:2323-2386).
 // ------------------------------------------------------------------------------------
 template <typename T, bool WIND>
-__global__ __launch_bounds__(BX * BY) void k_air_v(const AirArgs<T> a)
+__device__ __forceinline__ void air_v_point(const AirArgs<T> &a, int i, int j, int k)
 {
     const Grid<T> &g = a.g;
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    if (i >= nx - 1 || j >= ny - 1) return;
     const size_t sxy = (size_t)nx * ny;
     const size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
     T mX, mY, mZ;
@@ -252,9 +248,9 @@ __global__ __launch_bounds__(BX * BY) void k_air_v(const AirArgs<T> a)
     T qu_old[6], qv_old[6], qw_old[6];
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-        qu_old[s] = own.in[s] ? a.Q[s][0][own.idx[s]] : T(0);
-        qv_old[s] = own.in[s] ? a.Q[s][1][own.idx[s]] : T(0);
-        qw_old[s] = own.in[s] ? a.Q[s][2][own.idx[s]] : T(0);
+        qu_old[s] = own.in[s] ? a.Qo[s][0][own.idx[s]] : T(0);
+        qv_old[s] = own.in[s] ? a.Qo[s][1][own.idx[s]] : T(0);
+        qw_old[s] = own.in[s] ? a.Qo[s][2][own.idx[s]] : T(0);
     }
     const T *p = g.pnn;
     const T pc = p[id];
@@ -294,7 +290,7 @@ __global__ __launch_bounds__(BX * BY) void k_air_v(const AirArgs<T> a)
     if (i < a.I1) {
         const T sx = a.x1.s[i], cx = a.x1.c[i];
         const size_t q = qxi(i, j, k, n0, ny);
-        const T *Qp = a.Q[0][3];
+        const T *Qp = a.Qn[0][3];
         du += ((-sx) * uc) * dt + (cx * (Aav * gpx)) * dt;
         dv += ((-sx) * (vc + Aav * ((mY * (Qp[q] - Qp[q - n0])) * dyi))) * dt;
         dw += ((-sx) * (wc + Aav * ((mZ * (Qp[q] - Qp[q - (size_t)n0 * ny])) * dzi))) * dt;
@@ -303,7 +299,7 @@ __global__ __launch_bounds__(BX * BY) void k_air_v(const AirArgs<T> a)
         const int ii = i - (nx - n1);
         const T sx = a.x2.s[i], cx = a.x2.c[i];
         const size_t q = qxi(ii, j, k, n1, ny);
-        const T *Qp = a.Q[1][3];
+        const T *Qp = a.Qn[1][3];
         du += ((-sx) * uc) * dt + (cx * (Aav * gpx)) * dt;
         dv += ((-sx) * (vc + Aav * ((mY * (Qp[q] - Qp[q - n1])) * dyi))) * dt;
         dw += ((-sx) * (wc + Aav * ((mZ * (Qp[q] - Qp[q - (size_t)n1 * ny])) * dzi))) * dt;
@@ -316,22 +312,22 @@ __global__ __launch_bounds__(BX * BY) void k_air_v(const AirArgs<T> a)
         if (i < a.I1) {
             const T sx = a.x1.s[i], dxm = a.x1.d[i];
             const size_t q = qxi(i, j, k, n0, ny);
-            const T *Xp = a.Q[0][3];
-            du += (-((a.Q[0][0][q] * sx) * sy + ((dxm * sy) * Aav) * QBX(Xp, q))) * dt;
-            dv += (-((a.Q[0][1][q] * sx) * sy + ((dym * sx) * Aav) * ((mY * (Xp[q] - Xp[q - n0])) * dyi))) * dt;
-            dw += (((-a.Q[0][2][q]) * sx) * sy) * dt;
+            const T *Xp = a.Qn[0][3];
+            du += (-((a.Qo[0][0][q] * sx) * sy + ((dxm * sy) * Aav) * QBX(Xp, q))) * dt;
+            dv += (-((a.Qo[0][1][q] * sx) * sy + ((dym * sx) * Aav) * ((mY * (Xp[q] - Xp[q - n0])) * dyi))) * dt;
+            dw += (((-a.Qo[0][2][q]) * sx) * sy) * dt;
         } else if (i >= nx - a.I2) {
             const int ii = i - (nx - n1);
             const T sx = a.x2.s[i], dxm = a.x2.d[i];
             const size_t q = qxi(ii, j, k, n1, ny);
-            const T *Xp = a.Q[1][3];
-            du += (-((a.Q[1][0][q] * sx) * sy + ((dxm * sy) * Aav) * QBX(Xp, q))) * dt;
-            dv += (-((a.Q[1][1][q] * sx) * sy + ((dym * sx) * Aav) * ((mY * (Xp[q] - Xp[q - n1])) * dyi))) * dt;
-            dw += (((-a.Q[1][2][q]) * sx) * sy) * dt;
+            const T *Xp = a.Qn[1][3];
+            du += (-((a.Qo[1][0][q] * sx) * sy + ((dxm * sy) * Aav) * QBX(Xp, q))) * dt;
+            dv += (-((a.Qo[1][1][q] * sx) * sy + ((dym * sx) * Aav) * ((mY * (Xp[q] - Xp[q - n1])) * dyi))) * dt;
+            dw += (((-a.Qo[1][2][q]) * sx) * sy) * dt;
         }
     };
-    if (j < a.J1) ylayer(a.y1.s[j], a.y1.c[j], a.y1.d[j], a.Q[2][3], qyi(i, j, k, nx, n2), n2);
-    if (j >= ny - a.J2) ylayer(a.y2.s[j], a.y2.c[j], a.y2.d[j], a.Q[3][3], qyi(i, j - (ny - n3), k, nx, n3), n3);
+    if (j < a.J1) ylayer(a.y1.s[j], a.y1.c[j], a.y1.d[j], a.Qn[2][3], qyi(i, j, k, nx, n2), n2);
+    if (j >= ny - a.J2) ylayer(a.y2.s[j], a.y2.c[j], a.y2.d[j], a.Qn[3][3], qyi(i, j - (ny - n3), k, nx, n3), n3);
     // z layer with x and y cross terms (air.cpp:1730-1905)
     auto zlayer = [&](T sz, T cz, T dzm, const T *Qp, size_t qz) {
         du += ((-sz) * (uc + Aav * QBX(Qp, qz))) * dt;
@@ -340,51 +336,167 @@ __global__ __launch_bounds__(BX * BY) void k_air_v(const AirArgs<T> a)
         if (i < a.I1) {
             const T sx = a.x1.s[i], dxm = a.x1.d[i];
             const size_t q = qxi(i, j, k, n0, ny);
-            const T *Xp = a.Q[0][3];
-            du += (-((a.Q[0][0][q] * sx) * sz + ((dxm * sz) * Aav) * QBX(Xp, q))) * dt;
-            dv += (((-a.Q[0][1][q]) * sx) * sz) * dt;
-            dw += (-((a.Q[0][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - (size_t)n0 * ny])) * dzi))) * dt;
+            const T *Xp = a.Qn[0][3];
+            du += (-((a.Qo[0][0][q] * sx) * sz + ((dxm * sz) * Aav) * QBX(Xp, q))) * dt;
+            dv += (((-a.Qo[0][1][q]) * sx) * sz) * dt;
+            dw += (-((a.Qo[0][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - (size_t)n0 * ny])) * dzi))) * dt;
         } else if (i >= nx - a.I2) {
             const int ii = i - (nx - n1);
             const T sx = a.x2.s[i], dxm = a.x2.d[i];
             const size_t q = qxi(ii, j, k, n1, ny);
-            const T *Xp = a.Q[1][3];
-            du += (-((a.Q[1][0][q] * sx) * sz + ((dxm * sz) * Aav) * QBX(Xp, q))) * dt;
-            dv += (((-a.Q[1][1][q]) * sx) * sz) * dt;
-            dw += (-((a.Q[1][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - (size_t)n1 * ny])) * dzi))) * dt;
+            const T *Xp = a.Qn[1][3];
+            du += (-((a.Qo[1][0][q] * sx) * sz + ((dxm * sz) * Aav) * QBX(Xp, q))) * dt;
+            dv += (((-a.Qo[1][1][q]) * sx) * sz) * dt;
+            dw += (-((a.Qo[1][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - (size_t)n1 * ny])) * dzi))) * dt;
         }
         if (j < a.J1) {
             const T sy = a.y1.s[j], dym = a.y1.d[j];
             const size_t q = qyi(i, j, k, nx, n2);
-            const T *Yp = a.Q[2][3];
-            du += (((-a.Q[2][0][q]) * sy) * sz) * dt;
-            dv += (-((a.Q[2][1][q] * sy) * sz + ((dym * sz) * Aav) * ((mY * (Yp[q] - Yp[q - nx])) * dyi))) * dt;
-            dw += (-((a.Q[2][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - (size_t)nx * n2])) * dzi))) * dt;
+            const T *Yp = a.Qn[2][3];
+            du += (((-a.Qo[2][0][q]) * sy) * sz) * dt;
+            dv += (-((a.Qo[2][1][q] * sy) * sz + ((dym * sz) * Aav) * ((mY * (Yp[q] - Yp[q - nx])) * dyi))) * dt;
+            dw += (-((a.Qo[2][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - (size_t)nx * n2])) * dzi))) * dt;
         } else if (j >= ny - a.J2) {
             const int jj = j - (ny - n3);
             const T sy = a.y2.s[j], dym = a.y2.d[j];
             const size_t q = qyi(i, jj, k, nx, n3);
-            const T *Yp = a.Q[3][3];
-            du += (((-a.Q[3][0][q]) * sy) * sz) * dt;
-            dv += (-((a.Q[3][1][q] * sy) * sz + ((dym * sz) * Aav) * ((mY * (Yp[q] - Yp[q - nx])) * dyi))) * dt;
-            dw += (-((a.Q[3][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - (size_t)nx * n3])) * dzi))) * dt;
+            const T *Yp = a.Qn[3][3];
+            du += (((-a.Qo[3][0][q]) * sy) * sz) * dt;
+            dv += (-((a.Qo[3][1][q] * sy) * sz + ((dym * sz) * Aav) * ((mY * (Yp[q] - Yp[q - nx])) * dyi))) * dt;
+            dw += (-((a.Qo[3][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - (size_t)nx * n3])) * dzi))) * dt;
         }
     };
-    if (k < a.K1) zlayer(a.z1.s[k], a.z1.c[k], a.z1.d[k], a.Q[4][3], qzi(i, j, k, nx, ny));
-    if (k >= nz - a.K2) zlayer(a.z2.s[k], a.z2.c[k], a.z2.d[k], a.Q[5][3], qzi(i, j, k - (nz - n5), nx, ny));
+    if (k < a.K1) zlayer(a.z1.s[k], a.z1.c[k], a.z1.d[k], a.Qn[4][3], qzi(i, j, k, nx, ny));
+    if (k >= nz - a.K2) zlayer(a.z2.s[k], a.z2.c[k], a.z2.d[k], a.Qn[5][3], qzi(i, j, k - (nz - n5), nx, ny));
 #undef QBX
     const T un = uc + du, vn = vc + dv, wn = wc + dw;
     g.unn[id] = un;
     g.vnn[id] = vn;
     g.wnn[id] = wn;
-    // Q_u, Q_v, Q_w += dt*(u, v, w) of this point (air.cpp:2323-2386)
+    // Q_u, Q_v, Q_w += dt*(u, v, w) of this point
 #pragma unroll
     for (int s = 0; s < 6; ++s)
         if (own.in[s]) {
-            a.Q[s][0][own.idx[s]] = qu_old[s] + dt * un;
-            a.Q[s][1][own.idx[s]] = qv_old[s] + dt * vn;
-            a.Q[s][2][own.idx[s]] = qw_old[s] + dt * wn;
+            a.Qn[s][0][own.idx[s]] = qu_old[s] + dt * un;
+            a.Qn[s][1][own.idx[s]] = qv_old[s] + dt * vn;
+            a.Qn[s][2][own.idx[s]] = qw_old[s] + dt * wn;
         }
+}
+
+// The two phases as separate passes, one thread per interior point.
+template <typename T, bool WIND>
+__global__ __launch_bounds__(BX * BY) void k_air_p(const AirArgs<T> a)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    if (i >= a.g.nx - 1 || j >= a.g.ny - 1) return;
+    (void)air_p_point<T, WIND>(a, i, j, k);
+}
+
+template <typename T, bool WIND>
+__global__ __launch_bounds__(BX * BY) void k_air_v(const AirArgs<T> a)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    if (i >= a.g.nx - 1 || j >= a.g.ny - 1) return;
+    air_v_point<T, WIND>(a, i, j, k);
+}
+
+// ------------------------------------------------------------------------------------
+// Fused step: both phases in one pass.
+//
+// A block is FB_TY + 1 wavefronts.  Wavefront w holds row j = FB_TY*ty + w and lane l holds
+// column i = 63*tx + l, so the block covers a 64 x (FB_TY+1) patch of which the 63 x FB_TY
+// points with l >= 1 and w >= 1 are its own; lane 0 and wavefront 0 form the halo column
+// and row.  The block walks a chunk of planes upwards.  On each plane every thread first
+// computes p^{n+1} (and Q_p) at its point, then, after a barrier, the owned points compute
+// their velocities, which read p^{n+1} and Q_p at (i-1, j, k), (i, j-1, k), (i, j, k-1):
+// all written by this block before the barrier (halo points by the halo lanes, the plane
+// below by the previous iteration or, at the start of a chunk, by an extra pressure-only
+// pass over the plane below).  Halo values are recomputed from time level n and the old
+// auxiliaries, which nothing modifies during the step, so they equal the owner's bit for
+// bit.  Face nodes next to owned points (Front i = 0, West j = 0, South k = 0) get the
+// value the face pass gives them -- a copy of the adjacent interior node or, on a porous
+// South face, the porous plane Kp-2 through the interface; the face pass still runs
+// afterwards over every boundary node and stores the same values.
+// ------------------------------------------------------------------------------------
+constexpr int FB_TY = 4;
+
+// Q[f] of every active slab containing node (i,j,k) <- old + dt*val (the boundary-node part
+// of air.cpp:2323-2439).
+template <typename T> __device__ __forceinline__ void q_node(const AirArgs<T> &a, int f, int i, int j, int k, T val)
+{
+    const int nx = a.g.nx, ny = a.g.ny;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const int n = a.qn[s];
+        if (!n || (s == 2 && !a.y1on)) continue;
+        const int ax = s >> 1, lo = a.qlo[s];
+        const int c = ax == 0 ? i : ax == 1 ? j : k;
+        if (c < lo || c >= lo + n) continue;
+        const size_t q = ax == 0 ? qxi(i - lo, j, k, n, ny) : ax == 1 ? qyi(i, j - lo, k, nx, n) : qzi(i, j, k - lo, nx, ny);
+        a.Qn[s][f][q] = a.Qo[s][f][q] + a.g.dt * val;
+    }
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin (b, b+8, b+16, ... share one XCD and its
+// L2).  Give every XCD a contiguous run of tiles, so the halo a block re-reads was fetched
+// into the same L2 by its neighbour.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    const int x = b & 7, per = nb >> 3, rem = nb & 7;
+    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + (b >> 3);
+}
+
+template <typename T, bool WIND>
+__global__ __launch_bounds__(64 * (FB_TY + 1)) void k_air_fb(const AirArgs<T> a, int ntx, int nty, int kc)
+{
+    const Grid<T> &g = a.g;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tx = tile % ntx, ty = (tile / ntx) % nty, kq = tile / (ntx * nty);
+    const int i = 63 * tx + (int)threadIdx.x, j = FB_TY * ty + (int)threadIdx.y;
+    const int k0 = 1 + kq * kc, k1 = min(k0 + kc, nz - 1);
+    const bool own = threadIdx.x >= 1 && threadIdx.y >= 1 && i <= nx - 2 && j <= ny - 2;
+    // pressure task: the interior point (i, j), or the face node (0, j) / (i, 0) from its
+    // interior neighbour; the edge (0, 0) is never read by the velocity update
+    const bool face = (i == 0) != (j == 0);
+    const bool ptask = i <= nx - 2 && j <= ny - 2 && !(i == 0 && j == 0);
+    const int ip = i == 0 ? 1 : i, jp = j == 0 ? 1 : j;
+    const size_t sxy = (size_t)nx * ny;
+    // pressure (and Q_p) of plane k at this thread's point, with the face node it stands for
+    auto pressure = [&](int k, bool faces) {
+        // Keep the per-point address arithmetic here: hoisted out of the plane loop it
+        // becomes one 64-bit induction pointer per array and the kernel runs out of registers.
+        int ii = i, jj = j, ipp = ip, jpp = jp;
+        asm volatile("" : "+v"(ii), "+v"(jj), "+v"(ipp), "+v"(jpp));
+        if (!ptask || (face && !faces)) return;
+        const T p = air_p_point<T, WIND>(a, ipp, jpp, k);
+        const size_t c = (size_t)ii + (size_t)nx * jj;
+        if (face) {
+            // Front (calculation_2D.cpp:696-705) or West (676-685) face node
+            g.pnn[c + sxy * k] = p;
+            q_node(a, 3, ii, jj, k, p);
+        } else if (own && k == 1) {
+            // South face k = 0: rigid copy, or the porous plane through the interface
+            // (calculation_2D.cpp:686-695, 865-985)
+            const T ps = a.south_pore ? a.pore_pnn[c + sxy * (a.pore_nz - 2)] : p;
+            g.pnn[c] = ps;
+            q_node(a, 3, ii, jj, 0, ps);
+        }
+    };
+    // Software pipeline over the planes: one barrier per plane, after which the velocities
+    // of plane k (reading planes k and k-1) and the pressures of plane k+1 are independent.
+    if (k0 > 1) pressure(k0 - 1, false);  // the plane below the chunk: its own points only
+    pressure(k0, true);
+#pragma unroll 1
+    for (int k = k0; k < k1; ++k) {
+        __syncthreads();
+        if (own) {
+            int ii = i, jj = j;
+            asm volatile("" : "+v"(ii), "+v"(jj));
+            air_v_point<T, WIND>(a, ii, jj, k);
+        }
+        if (k + 1 < k1) pressure(k + 1, true);
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -471,25 +583,14 @@ __global__ void k_finite(const T *f, size_t n, int *flag)
         if (!isfinite((double)f[t])) { atomicOr(flag, 1); return; }
 }
 
-// Streaming copy used to calibrate HBM counters: each thread moves 4 x 16 B per pass, with
-// non-temporal loads and stores (the data is touched once).
+// Streaming copy used as the bandwidth reference and to calibrate HBM counters: one 16 B
+// vector per thread, non-temporal (the data is touched once), as many workgroups as needed.
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void k_copy_words(v4f *dst, const v4f *src, size_t n4)
 {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; t + 3 * stride < n4; t += 4 * stride) {
-        v4f a = __builtin_nontemporal_load(src + t);
-        v4f b = __builtin_nontemporal_load(src + t + stride);
-        v4f c = __builtin_nontemporal_load(src + t + 2 * stride);
-        v4f d = __builtin_nontemporal_load(src + t + 3 * stride);
-        __builtin_nontemporal_store(a, dst + t);
-        __builtin_nontemporal_store(b, dst + t + stride);
-        __builtin_nontemporal_store(c, dst + t + 2 * stride);
-        __builtin_nontemporal_store(d, dst + t + 3 * stride);
-    }
-    for (; t < n4; t += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + t), dst + t);
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n4) __builtin_nontemporal_store(__builtin_nontemporal_load(src + t), dst + t);
 }
 
 dim3 interior_grid(int nx, int ny, int nz) { return dim3((nx - 2 + BX - 1) / BX, (ny - 2 + BY - 1) / BY, nz - 2); }
@@ -515,6 +616,28 @@ template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStrea
     if (wind) hipLaunchKernelGGL((k_air_v<T, true>), g, b, 0, s, a);
     else hipLaunchKernelGGL((k_air_v<T, false>), g, b, 0, s, a);
 }
+template <typename T> bool fused_fits(int nx) { return nx >= 3; }
+
+template <typename T> void launch_air_fb(const AirArgs<T> &a, bool wind, hipStream_t s)
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+    }
+    const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+    const int ntx = (nx - 2 + 62) / 63, nty = (ny - 2 + FB_TY - 1) / FB_TY;
+    // split the planes into chunks until there are about four blocks per CU
+    int nch = std::max(1, std::min(nz - 2, (4 * cus + ntx * nty - 1) / (ntx * nty)));
+    const int kc = (nz - 2 + nch - 1) / nch;
+    nch = (nz - 2 + kc - 1) / kc;
+    dim3 grid((unsigned)(ntx * nty * nch)), block(64, FB_TY + 1);
+    if (wind) hipLaunchKernelGGL((k_air_fb<T, true>), grid, block, 0, s, a, ntx, nty, kc);
+    else hipLaunchKernelGGL((k_air_fb<T, false>), grid, block, 0, s, a, ntx, nty, kc);
+}
+
 template <typename T> void launch_pore_p(const PoreArgs<T> &a, hipStream_t s)
 {
     hipLaunchKernelGGL((k_pore_p<T>), interior_grid(a.g.nx, a.g.ny, a.g.nz), dim3(BX, BY), 0, s, a);
@@ -542,15 +665,15 @@ template <typename T> void launch_finite_check(const T *f, size_t n, int *flag, 
 
 void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s)
 {
-    size_t n4 = n / 4;
-    size_t b = (n4 + 255) / 256;
-    if (b > 4096) b = 4096;
-    hipLaunchKernelGGL(k_copy_words, dim3((unsigned)b), dim3(256), 0, s, (v4f *)dst, (const v4f *)src, n4);
+    const size_t n4 = n / 4;
+    hipLaunchKernelGGL(k_copy_words, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, s, (v4f *)dst, (const v4f *)src, n4);
 }
 
 #define LEE_INST(T)                                                                                          \
     template void launch_air_p<T>(const AirArgs<T> &, bool, hipStream_t);                                    \
     template void launch_air_v<T>(const AirArgs<T> &, bool, hipStream_t);                                    \
+    template bool fused_fits<T>(int);                                                                        \
+    template void launch_air_fb<T>(const AirArgs<T> &, bool, hipStream_t);                                   \
     template void launch_pore_p<T>(const PoreArgs<T> &, hipStream_t);                                        \
     template void launch_pore_v<T>(const PoreArgs<T> &, hipStream_t);                                        \
     template void launch_shift<T>(T *, const T *, int, int, int, int, int, int, hipStream_t);                \

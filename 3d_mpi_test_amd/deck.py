@@ -55,6 +55,55 @@ class LeeDeck(ctypes.Structure):
     ]
 
 
+class LeeOptions(ctypes.Structure):
+    """ctypes mirror of ``lee_options`` (include/lee_abi.h): the sidecar of a deck."""
+
+    _fields_ = [
+        ("precision", ctypes.c_int),
+        ("n_receivers", ctypes.c_int),
+        ("receivers", c_double_p),
+        ("health_every", ctypes.c_int),
+        ("use_graph", ctypes.c_int),
+        ("device", ctypes.c_int),
+        ("path", ctypes.c_int),
+        ("integrator", ctypes.c_int),
+        ("n_hills", ctypes.c_int),
+        ("hills", ctypes.c_double * 32),
+        ("wind_ustar", ctypes.c_double),
+        ("wind_z0", ctypes.c_double),
+        ("reserved", ctypes.c_int * 15),
+    ]
+
+
+INT_DECK, INT_RK4 = 0, 1
+
+
+def make_options(deck: Optional["Deck"] = None, precision: int = 64, receivers=None, health_every: int = 0,
+                 device: int = 0, path: int = 0, integrator: Optional[int] = None) -> LeeOptions:
+    """``lee_options`` for a deck: its extensions (integrator, hills, log wind) plus the run
+    settings.  The receiver array is kept alive on the returned struct."""
+    o = LeeOptions()
+    o.precision, o.health_every, o.device, o.path = precision, health_every, device, path
+    if receivers is not None:
+        rec = np.ascontiguousarray(receivers, dtype=np.float64).reshape(-1)
+        o._rec = rec
+        o.n_receivers = rec.size // 3
+        o.receivers = rec.ctypes.data_as(c_double_p)
+    if deck is not None:
+        o.integrator = deck.integrator if integrator is None else integrator
+        hills = list(deck.hills)
+        if len(hills) > 8:
+            raise ValueError("at most 8 hills")
+        o.n_hills = len(hills)
+        for n, h in enumerate(hills):
+            for q in range(4):
+                o.hills[4 * n + q] = float(h[q])
+        o.wind_ustar, o.wind_z0 = float(deck.wind_ustar), float(deck.wind_z0)
+    elif integrator is not None:
+        o.integrator = integrator
+    return o
+
+
 def _ptr(a: Optional[np.ndarray]):
     if a is None:
         return c_double_p()
@@ -117,6 +166,12 @@ class Deck:
     pore_x: Optional[np.ndarray] = None
     pore_y: Optional[np.ndarray] = None
     pore_z: Optional[np.ndarray] = None
+    # sidecar extensions (lee_options; not part of input.txt): time integrator, Gaussian
+    # hills (x0, y0, height, sigma) of a terrain-following grid, log wind of velocity_method 7
+    integrator: int = 0
+    hills: Sequence[Sequence[float]] = ()
+    wind_ustar: float = 0.0
+    wind_z0: float = 0.0
 
     def __post_init__(self):
         self.air_x = np.ascontiguousarray(self.air_x, dtype=np.float64)

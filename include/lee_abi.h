@@ -91,8 +91,30 @@ typedef struct lee_options {
     int health_every;       /* check the state for NaN/Inf every N steps (0 = never) */
     int use_graph;          /* 1: replay each step from a captured hipGraph */
     int device;             /* HIP device ordinal */
-    int reserved[16];
+    int path;               /* LEE_PATH_AUTO (0), LEE_PATH_SPLIT, LEE_PATH_FUSED */
+    /* ---- extensions beyond the reference (no input.txt field can express them) ---- */
+    int integrator;         /* LEE_INT_DECK (0): the deck's scheme (FB_p_v); LEE_INT_RK4 */
+    int n_hills;            /* terrain-following grid over n_hills Gaussian hills (0: flat) */
+    double hills[32];       /* per hill: x0, y0 [m, frame-0 coordinates], height, sigma [m] */
+    double wind_ustar;      /* velocity_method 7: V(z) = wind_ustar/0.41 ln(z/wind_z0) along +y, */
+    double wind_z0;         /*   z = height above the local ground, V = 0 for z <= wind_z0 */
+    int reserved[15];
 } lee_options;
+
+/* Time integrators.  LEE_INT_DECK runs the deck's scheme (the reference implements only
+ * FB_p_v, calculation_2D.cpp:551-612).  LEE_INT_RK4 is classical four-stage Runge-Kutta
+ * on the same semi-discrete operator (all four fields from one stage state, the PML
+ * auxiliaries as ODE state, face conditions re-applied after every stage); the porous drag
+ * is integrated explicitly, so sigma*dt*Omega/(q^2 rho0) must stay below 2.5.  The
+ * reference parses scheme 5 = RK4 but never integrates with it. */
+enum { LEE_INT_DECK = 0, LEE_INT_RK4 = 1 };
+#define LEE_MAX_HILLS 8
+
+/* Kernel path of the time step.  SPLIT runs the reference's two phases as separate passes
+ * over the grid; FUSED computes both phases of a plane in one pass (one read and one write
+ * of u, v, w, p per point).  Both give the same bits.  AUTO runs SPLIT, the faster of the
+ * two on MI355X.  FUSED needs the porous interface copy when South is porous. */
+enum { LEE_PATH_AUTO = 0, LEE_PATH_SPLIT = 1, LEE_PATH_FUSED = 2 };
 
 typedef struct lee_info {
     int nx, ny, nz;                 /* air frame dimensions (IMAX, frame JMAX, KMAX) */
@@ -103,6 +125,7 @@ typedef struct lee_info {
     int mf_count;                   /* frame set up last (-1 before lee_frame_setup) */
     long long steps_done;           /* time steps advanced so far */
     long long pml_y1_counter;       /* reference PML_JMAX1 step counter */
+    int path;                       /* LEE_PATH_SPLIT or LEE_PATH_FUSED, as chosen */
 } lee_info;
 
 typedef struct lee_ctx lee_ctx;

@@ -39,6 +39,15 @@ typedef struct {
     double dt;
     /* cross derivatives X_over_y ... are scalars hard-set to 0 (calculation_2D.cpp:102-104) */
     double X_y, X_z, Y_x, Y_z, Z_x, Z_y;
+    /* Terrain-following extension (no reference counterpart).  Physical height
+     *   Z(i,j,k) = Z[k] + h(i,j) * tk[k],  tk[k] = (Z[nz-1] - Z[k]) / (Z[nz-1] - Z[0])
+     * in the air (the top stays flat), tk = 1 in the porous layer (a rigid shift).  With the
+     * reference's central differences the metric terms factor exactly as
+     *   Z_over_z = Zz[k] * sh(i,j),  Z_over_x = hx(i,j) * tk[k],  Z_over_y = hy(i,j) * tk[k],
+     * sh = 1 - h/D (air) or 1 (porous), hx, hy = central differences of h. */
+    int terrain, pore_layer, nh;
+    double hills[32];
+    double *h, *sh, *hx, *hy, *tk;
 } ogrid;
 
 #define IDX(g, i, j, k) ((size_t)(i) + (size_t)(g)->nx * ((size_t)(j) + (size_t)(g)->ny * (size_t)(k)))
@@ -48,6 +57,7 @@ typedef struct { int n; double *q[4]; } oslab;
 
 struct orc_sim {
     lee_deck d;
+    lee_options opt;          /* extensions: integrator, terrain, log wind */
     ogrid air, pore;
     int south_porous, east_tdbc;
     double gamma, Pav, Aav, c;
@@ -71,6 +81,10 @@ struct orc_sim {
     int mf_count;
     /* receivers */
     int nrec; int *ri, *rj, *rk;
+    /* RK4 registers (extension): increments and accumulators of the fields [domain][u v w p]
+     * and of the auxiliaries, the auxiliaries of the stage state, and those of level n */
+    double *rk_k[2][4], *rk_a[2][4];
+    double *qk[6][4], *qa[6][4], *qs[6][4], *qsave[6][4];
 };
 
 static const double TD_A[6] = {8.226223820e-01, 3.732304064e-01, 9.106232372e-02,
@@ -114,7 +128,8 @@ static int grid_init(ogrid *g, int nx, int ny, int nz, const double *x, const do
 static void grid_free(ogrid *g)
 {
     double **ps[] = {&g->un, &g->vn, &g->wn, &g->pn, &g->unn, &g->vnn, &g->wnn, &g->pnn,
-                     &g->Jac, &g->X, &g->Y, &g->Z, &g->ax, &g->ay, &g->az, &g->Xx, &g->Yy, &g->Zz};
+                     &g->Jac, &g->X, &g->Y, &g->Z, &g->ax, &g->ay, &g->az, &g->Xx, &g->Yy, &g->Zz,
+                     &g->h, &g->sh, &g->hx, &g->hy, &g->tk};
     for (size_t i = 0; i < sizeof ps / sizeof ps[0]; ++i) { free(*ps[i]); *ps[i] = NULL; }
 }
 
@@ -123,21 +138,73 @@ static double geo(double a, double b, double c, double d, double jac) { return (
 
 typedef struct { double xX, xY, xZ, yX, yY, yZ, zX, zY, zZ; } metric9;
 
+/* Z_over_z, Z_over_x, Z_over_y at (i,j,k): the reference's scalars and 1-D array, or the
+ * terrain factors. */
+static void z_derivs(const ogrid *g, int i, int j, int k, double *Zz, double *Zx, double *Zy)
+{
+    if (!g->terrain) { *Zz = g->Zz[k]; *Zx = g->Z_x; *Zy = g->Z_y; return; }
+    size_t c = (size_t)i + (size_t)g->nx * j;
+    *Zz = g->Zz[k] * g->sh[c];
+    *Zx = g->hx[c] * g->tk[k];
+    *Zy = g->hy[c] * g->tk[k];
+}
+
 /* The nine inverse-metric terms exactly as air::cal_fp computes them (air.cpp:1975-1992). */
 static metric9 point_metric(const ogrid *g, int i, int j, int k)
 {
     metric9 m;
     double J = g->Jac[IDX(g, i, j, k)];
-    m.xX = geo(g->Yy[j], g->Zz[k], g->Y_z, g->Z_y, J);
-    m.xY = geo(g->X_z, g->Z_y, g->X_y, g->Zz[k], J);
+    double Zz, Z_x, Z_y;
+    z_derivs(g, i, j, k, &Zz, &Z_x, &Z_y);
+    m.xX = geo(g->Yy[j], Zz, g->Y_z, Z_y, J);
+    m.xY = geo(g->X_z, Z_y, g->X_y, Zz, J);
     m.xZ = geo(g->X_y, g->Y_z, g->X_z, g->Yy[j], J);
-    m.yX = geo(g->Y_z, g->Z_x, g->Y_x, g->Zz[k], J);
-    m.yY = geo(g->Xx[i], g->Zz[k], g->X_z, g->Z_x, J);
+    m.yX = geo(g->Y_z, Z_x, g->Y_x, Zz, J);
+    m.yY = geo(g->Xx[i], Zz, g->X_z, Z_x, J);
     m.yZ = geo(g->X_z, g->Y_x, g->Xx[i], g->Y_z, J);
-    m.zX = geo(g->Y_x, g->Z_y, g->Yy[j], g->Z_x, J);
-    m.zY = geo(g->X_y, g->Z_x, g->Xx[i], g->Z_y, J);
+    m.zX = geo(g->Y_x, Z_y, g->Yy[j], Z_x, J);
+    m.zY = geo(g->X_y, Z_x, g->Xx[i], Z_y, J);
     m.zZ = geo(g->Xx[i], g->Yy[j], g->X_y, g->Y_x, J);
     return m;
+}
+
+/* Sum of the Gaussian hills at world (x, y). */
+static double terrain_h(const ogrid *g, double x, double y)
+{
+    double h = 0.0;
+    for (int n = 0; n < g->nh; ++n) {
+        const double *H = &g->hills[4 * n];
+        double dx = x - H[0], dy = y - H[1];
+        h += H[2] * exp(-(dx * dx + dy * dy) / (2.0 * H[3] * H[3]));
+    }
+    return h;
+}
+
+/* Terrain tables of the current frame (X, Y, Z as set for the frame). */
+static void terrain_tables(ogrid *g)
+{
+    int nx = g->nx, ny = g->ny, nz = g->nz;
+    double top = g->Z[nz - 1], D = g->Z[nz - 1] - g->Z[0];
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            size_t c = (size_t)i + (size_t)nx * j;
+            g->h[c] = terrain_h(g, g->X[i], g->Y[j]);
+            g->sh[c] = g->pore_layer ? 1.0 : 1.0 - g->h[c] / D;
+        }
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            size_t c = (size_t)i + (size_t)nx * j;
+            g->hx[c] = (i > 0 && i < nx - 1) ? (g->h[c + 1] - g->h[c - 1]) * g->dxi * 0.5 : 0.0;
+            g->hy[c] = (j > 0 && j < ny - 1) ? (g->h[c + nx] - g->h[c - nx]) * g->dyi * 0.5 : 0.0;
+        }
+    for (int k = 0; k < nz; ++k) g->tk[k] = g->pore_layer ? 1.0 : (top - g->Z[k]) / D;
+}
+
+/* Physical height of node (i,j,k). */
+static double z_phys(const ogrid *g, int i, int j, int k)
+{
+    if (!g->terrain) return g->Z[k];
+    return g->Z[k] + g->h[(size_t)i + (size_t)g->nx * j] * g->tk[k];
 }
 
 /* Metric part of calculation_2D::SetMovingFrame (calculation_2D.cpp:410-433). */
@@ -148,13 +215,16 @@ static void grid_metrics(ogrid *g, int N, int trail)
     for (int i = 1; i < g->nx - 1; ++i) g->Xx[i] = (g->X[i + 1] - g->X[i - 1]) * g->dxi * 0.5;
     for (int j = 1; j < g->ny - 1; ++j) g->Yy[j] = (g->Y[j + 1] - g->Y[j - 1]) * g->dyi * 0.5;
     for (int k = 1; k < g->nz - 1; ++k) g->Zz[k] = (g->Z[k + 1] - g->Z[k - 1]) * g->dzi * 0.5;
+    if (g->terrain) terrain_tables(g);
     for (int k = 1; k < g->nz - 1; ++k)
         for (int j = 1; j < g->ny - 1; ++j)
             for (int i = 1; i < g->nx - 1; ++i) {
-                double J = g->Xx[i] * g->Yy[j] * g->Zz[k] + g->Z_x * g->X_y * g->Y_z +
-                           g->Y_x * g->Z_y * g->X_z;
-                J += -g->Xx[i] * g->Z_y * g->Y_z - g->Y_x * g->X_y * g->Zz[k] -
-                     g->Z_x * g->Yy[j] * g->X_z;
+                double Zz, Z_x, Z_y;
+                z_derivs(g, i, j, k, &Zz, &Z_x, &Z_y);
+                double J = g->Xx[i] * g->Yy[j] * Zz + Z_x * g->X_y * g->Y_z +
+                           g->Y_x * Z_y * g->X_z;
+                J += -g->Xx[i] * Z_y * g->Y_z - g->Y_x * g->X_y * Zz -
+                     Z_x * g->Yy[j] * g->X_z;
                 g->Jac[IDX(g, i, j, k)] = 1.0 / J;
             }
 }
@@ -188,14 +258,16 @@ static void grid_frame(ogrid *g, int N, int mf_jmax, int lead, int trail)
     }
 }
 
-/* Initial condition (calculation_2D.cpp:292-314). */
+/* Initial condition (calculation_2D.cpp:292-314).  Over terrain the source height is taken
+ * above the ground under the source and the distance is physical. */
 static void grid_ic(ogrid *g, const double *src, double gw)
 {
     double gc = 4.0 * log(2.0) / pow(gw, 2);
+    double zs = src[2] + (g->terrain ? terrain_h(g, src[0], src[1]) : 0.0);
     for (int k = 0; k < g->nz; ++k)
         for (int j = 0; j < g->ny; ++j)
             for (int i = 0; i < g->nx; ++i) {
-                double x = g->X[i] - src[0], y = g->Y[j] - src[1], z = g->Z[k] - src[2];
+                double x = g->X[i] - src[0], y = g->Y[j] - src[1], z = z_phys(g, i, j, k) - zs;
                 double r = sqrt(pow(x, 2) + pow(y, 2) + pow(z, 2));
                 size_t id = IDX(g, i, j, k);
                 g->un[id] = g->vn[id] = g->wn[id] = 0.0;
@@ -256,6 +328,14 @@ static void set_wind(orc_sim *s, int N)
             for (int i = 0; i < g->nx; ++i) {
                 double Z0 = 0.0;
                 size_t id = IDX(g, i, j, k);
+                if (s->vmethod == 7) {
+                    /* logarithmic profile along +y over the local ground (extension) */
+                    double zg = z_phys(g, i, j, k) - z_phys(g, i, j, 0);
+                    s->U[id] = 0.0;
+                    s->V[id] = zg > s->opt.wind_z0 ? s->opt.wind_ustar / 0.41 * log(zg / s->opt.wind_z0) : 0.0;
+                    s->W[id] = 0.0;
+                    continue;
+                }
                 s->U[id] = wind_U(s->vmethod, s->vcoef, g->X[i], g->Y[j], g->Z[k] - Z0);
                 s->V[id] = wind_V(s->vmethod, s->vcoef, g->X[i], g->Y[j], g->Z[k] - Z0);
                 s->W[id] = wind_W(s->vmethod, s->vcoef, g->X[i], g->Y[j], g->Z[k] - Z0);
@@ -1090,10 +1170,40 @@ static double sample(const orc_sim *s, int r)
 /* ------------------------------------------------------------------------------------ */
 /* construction                                                                          */
 /* ------------------------------------------------------------------------------------ */
+static int terrain_alloc(ogrid *g, const lee_options *o, int pore_layer)
+{
+    g->terrain = 1;
+    g->pore_layer = pore_layer;
+    g->nh = o->n_hills;
+    memcpy(g->hills, o->hills, sizeof g->hills);
+    size_t n2 = (size_t)g->nx * g->ny;
+    g->h = dalloc(n2, 0); g->sh = dalloc(n2, 1); g->hx = dalloc(n2, 0); g->hy = dalloc(n2, 0);
+    g->tk = dalloc(g->nz, 1);
+    return (g->h && g->sh && g->hx && g->hy && g->tk) ? 0 : -1;
+}
+
 orc_sim *orc_create(const lee_deck *d, int nrec, const double *rec, int fill_nn_nan)
 {
+    lee_options o;
+    memset(&o, 0, sizeof o);
+    o.precision = 64;
+    o.n_receivers = nrec;
+    o.receivers = rec;
+    return orc_create_opt(d, &o, fill_nn_nan);
+}
+
+orc_sim *orc_create_opt(const lee_deck *d, const lee_options *opt, int fill_nn_nan)
+{
     g_err[0] = 0;
-    if (d->scheme != 2) FAIL("only scheme 2 (FB_p_v) is implemented by the reference");
+    int nrec = opt->n_receivers;
+    const double *rec = opt->receivers;
+    if (opt->integrator != LEE_INT_DECK && opt->integrator != LEE_INT_RK4) FAIL("unknown integrator");
+    if (opt->n_hills < 0 || opt->n_hills > LEE_MAX_HILLS) FAIL("n_hills must be 0..%d", LEE_MAX_HILLS);
+    for (int n = 0; n < opt->n_hills; ++n)
+        if (!(opt->hills[4 * n + 3] > 0)) FAIL("hill sigma must be positive");
+    if (d->velocity_method == 7 && !(opt->wind_z0 > 0)) FAIL("velocity_method 7 needs wind_z0 > 0");
+    if (opt->integrator == LEE_INT_RK4 && d->bc[LEE_FACE_E] == LEE_BC_TDBC) FAIL("the TDBC is not defined for RK4");
+    if (opt->integrator == LEE_INT_DECK && d->scheme != 2) FAIL("only scheme 2 (FB_p_v) is implemented by the reference");
     if (d->bc[LEE_FACE_N] != LEE_BC_RIGID && d->bc[LEE_FACE_N] != LEE_BC_ABSORBING) FAIL("North must be rigid or absorbing");
     if (d->hill_judge != 0) FAIL("embedded porous obstacles are not supported");
     if (d->restart != 0) FAIL("restart is not supported");
@@ -1101,6 +1211,7 @@ orc_sim *orc_create(const lee_deck *d, int nrec, const double *rec, int fill_nn_
     orc_sim *s = (orc_sim *)calloc(1, sizeof *s);
     if (!s) FAIL("oom");
     s->d = *d;
+    s->opt = *opt;
     const int *h = d->air_hdr;
     s->mf_judge = d->mf_judge; s->lead = d->mf_lead_dj; s->trail = d->mf_trail_dj; s->ratio = d->mf_ratio;
     int JM = h[3], IJK = h[9];
@@ -1122,6 +1233,7 @@ orc_sim *orc_create(const lee_deck *d, int nrec, const double *rec, int fill_nn_
     if (grid_init(&s->air, h[0], JM, h[6], d->air_x, d->air_y, d->air_z, d->dt, d->dx, d->dy, d->dz, fill_nn_nan)) {
         free(s); FAIL("oom");
     }
+    if (opt->n_hills > 0 && terrain_alloc(&s->air, opt, 0)) { orc_destroy(s); FAIL("oom"); }
     size_t n = (size_t)h[0] * JM * h[6];
     s->U = dalloc(n, 0); s->V = dalloc(n, 0); s->W = dalloc(n, 0);
     const char *e = pml_setup(s);
@@ -1136,6 +1248,7 @@ orc_sim *orc_create(const lee_deck *d, int nrec, const double *rec, int fill_nn_
         if (grid_init(&s->pore, ph[0], pjm, ph[6], d->pore_x, d->pore_y, d->pore_z, d->dt, d->pore_dx, d->pore_dy, d->pore_dz, fill_nn_nan)) {
             orc_destroy(s); FAIL("oom");
         }
+        if (opt->n_hills > 0 && terrain_alloc(&s->pore, opt, 1)) { orc_destroy(s); FAIL("oom"); }
         double Aav = s->Aav;
         double eff_density = d->pore_q * d->pore_q / d->pore_porosity / Aav; /* airpore.cpp:250 */
         double Kp = 1.0 / d->gamma / d->Pav;                                  /* airpore.cpp:251 */
@@ -1143,6 +1256,9 @@ orc_sim *orc_create(const lee_deck *d, int nrec, const double *rec, int fill_nn_
         s->p_beta = d->pore_resistivity * d->dt * s->p_eff;                   /* porous.cpp:48 */
         s->p_gama = d->dt / Kp / d->pore_porosity;                            /* porous.cpp:49 */
         s->p_invbeta = 1.0 / (s->p_beta + 1.0);                               /* porous.cpp:50 */
+        if (opt->integrator == LEE_INT_RK4 && !(s->p_beta < 2.5)) {
+            orc_destroy(s); FAIL("explicit RK4 porous drag needs sigma*dt/rho_e < 2.5");
+        }
     } else if (d->bc[LEE_FACE_S] != LEE_BC_RIGID && d->bc[LEE_FACE_S] != LEE_BC_ABSORBING) {
         orc_destroy(s); FAIL("South must be rigid, absorbing or porous");
     }
@@ -1167,7 +1283,9 @@ orc_sim *orc_create(const lee_deck *d, int nrec, const double *rec, int fill_nn_
         else { rx = d->rcv[0]; ry = d->rcv[1]; rz = d->rcv[2]; }
         locate(s, rx, ry, rz, &s->ri[r], &s->rj[r], &s->rk[r]);
     }
-    /* initial condition (airpore.cpp:947, 956-958) */
+    /* initial condition (airpore.cpp:947, 956-958); terrain heights of frame 0 first */
+    if (s->air.terrain) terrain_tables(&s->air);
+    if (s->south_porous && s->pore.terrain) terrain_tables(&s->pore);
     grid_ic(&s->air, d->src, d->gauss_width);
     if (s->south_porous) grid_ic(&s->pore, d->src, d->gauss_width);
     s->mf_count = -1;
@@ -1181,6 +1299,10 @@ void orc_destroy(orc_sim *s)
     free(s->U); free(s->V); free(s->W); free(s->Psi);
     slab_free(&s->sx1); slab_free(&s->sx2); slab_free(&s->sy1); slab_free(&s->sy2);
     slab_free(&s->sz1); slab_free(&s->sz2);
+    for (int d = 0; d < 2; ++d)
+        for (int f = 0; f < 4; ++f) { free(s->rk_k[d][f]); free(s->rk_a[d][f]); }
+    for (int q = 0; q < 6; ++q)
+        for (int f = 0; f < 4; ++f) { free(s->qk[q][f]); free(s->qa[q][f]); free(s->qs[q][f]); }
     free(s->ri); free(s->rj); free(s->rk);
     free(s);
 }
@@ -1293,11 +1415,170 @@ static void one_step(orc_sim *s)
     }
 }
 
+/* ------------------------------------------------------------------------------------ */
+/* RK4 (extension; the reference parses scheme 5 but never integrates with it,           */
+/* calculation_2D.cpp:551-612)                                                           */
+/* ------------------------------------------------------------------------------------ */
+/* Porous velocity right-hand side times dt: the drag of porous.cpp:112-135 taken as the
+ * ODE du/dt = -(1/rho_e) grad p - sigma/rho_e u, i.e. dt*f = -dt*eff*grad p - beta*u. */
+static void pore_rk_duvw_on(orc_sim *s, const double *u, const double *v, const double *w,
+                            const double *p, double *ou, double *ov, double *ow)
+{
+    ogrid *g = &s->pore;
+    double dxi = g->dxi, dyi = g->dyi, dzi = g->dzi, dt = g->dt;
+    double f = (-dt) * s->p_eff, be = s->p_beta;
+    for (int k = 1; k < g->nz - 1; ++k)
+        for (int j = 1; j < g->ny - 1; ++j)
+            for (int i = 1; i < g->nx - 1; ++i) {
+                metric9 m = point_metric(g, i, j, k);
+                size_t id = IDX(g, i, j, k);
+                ou[id] = f * (m.xX * BDX(p) * dxi + m.yX * BDY(p) * dyi + m.zX * BDZ(p) * dzi) - be * u[id];
+                ov[id] = f * (m.xY * BDX(p) * dxi + m.yY * BDY(p) * dyi + m.zY * BDZ(p) * dzi) - be * v[id];
+                ow[id] = f * (m.xZ * BDX(p) * dxi + m.yZ * BDY(p) * dyi + m.zZ * BDZ(p) * dzi) - be * w[id];
+            }
+}
+
+static const double RK_A[3] = {0.5, 0.5, 1.0};
+static const double RK_B[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+
+/* x_next / acc update of one value at stage st (0..3): acc accumulates x_n + sum b k; the
+ * next stage state is x_n + a k, and after the last stage x_{n+1} = acc + b4 k. */
+static double rk_combine(int st, double xn, double k, double *acc)
+{
+    if (st == 0) *acc = xn + RK_B[0] * k;
+    else if (st < 3) *acc = *acc + RK_B[st] * k;
+    return st < 3 ? xn + RK_A[st] * k : *acc + RK_B[3] * k;
+}
+
+/* Face conditions of the state held in the nn arrays, in the reference's face order. */
+static void rk_faces(orc_sim *s)
+{
+    ogrid *a = &s->air;
+    bc_p(a, FN); bc_p(a, FE); bc_p(a, FW); bc_p(a, FF); bc_p(a, FB);
+    if (!s->south_porous) {
+        bc_p(a, FS);
+    } else {
+        ogrid *q = &s->pore;
+        bc_p(q, FS); bc_p(q, FE); bc_p(q, FW); bc_p(q, FF); bc_p(q, FB);
+        iface_p(s);
+    }
+    bc_v(a, FN); bc_v(a, FE); bc_v(a, FW); bc_v(a, FF); bc_v(a, FB);
+    if (!s->south_porous) {
+        bc_v(a, FS);
+    } else {
+        ogrid *q = &s->pore;
+        bc_v(q, FS); bc_v(q, FE); bc_v(q, FW); bc_v(q, FF); bc_v(q, FB);
+        iface_v(s);
+    }
+}
+
+static int rk_alloc(orc_sim *s)
+{
+    if (s->rk_k[0][0]) return 0;
+    ogrid *g[2] = {&s->air, &s->pore};
+    for (int d = 0; d < 2; ++d) {
+        if (d == 1 && !s->south_porous) break;
+        size_t n = (size_t)g[d]->nx * g[d]->ny * g[d]->nz;
+        for (int f = 0; f < 4; ++f) {
+            s->rk_k[d][f] = dalloc(n, 0); s->rk_a[d][f] = dalloc(n, 0);
+            if (!s->rk_k[d][f] || !s->rk_a[d][f]) return -1;
+        }
+    }
+    for (int q = 0; q < 6; ++q) {
+        long n = orc_q_size(s, q);
+        for (int f = 0; f < 4; ++f) {
+            s->qk[q][f] = dalloc(n, 0); s->qa[q][f] = dalloc(n, 0); s->qs[q][f] = dalloc(n, 0);
+            if (!s->qk[q][f] || !s->qa[q][f] || !s->qs[q][f]) return -1;
+        }
+    }
+    return 0;
+}
+
+/* One classical RK4 step.  The stage state lives in the nn arrays (fields, with its face
+ * values) and in qs (auxiliaries); level n stays in the n arrays and the slabs. */
+static void rk4_step(orc_sim *s)
+{
+    ogrid *g[2] = {&s->air, &s->pore};
+    oslab *sl[6] = {&s->sx1, &s->sx2, &s->sy1, &s->sy2, &s->sz1, &s->sz2};
+    const int nd = s->south_porous ? 2 : 1;
+    orc_tick_gate(s);
+    const int y1on = (s->sy1.n > 0) && (s->pml_counter >= s->pml_time);
+    for (int d = 0; d < nd; ++d) {
+        size_t n = (size_t)g[d]->nx * g[d]->ny * g[d]->nz;
+        memcpy(g[d]->unn, g[d]->un, n * 8); memcpy(g[d]->vnn, g[d]->vn, n * 8);
+        memcpy(g[d]->wnn, g[d]->wn, n * 8); memcpy(g[d]->pnn, g[d]->pn, n * 8);
+    }
+    for (int q = 0; q < 6; ++q)
+        for (int f = 0; f < 4; ++f) {
+            long n = orc_q_size(s, q);
+            if (n) memcpy(s->qs[q][f], sl[q]->q[f], n * 8);
+            s->qsave[q][f] = sl[q]->q[f];
+        }
+    for (int st = 0; st < 4; ++st) {
+        ogrid *a = &s->air;
+        /* increments of the stage state (the PML terms read the stage auxiliaries) */
+        for (int q = 0; q < 6; ++q) for (int f = 0; f < 4; ++f) sl[q]->q[f] = s->qs[q][f];
+        orc_air_dp(s, a->unn, a->vnn, a->wnn, a->pnn, s->rk_k[0][3]);
+        orc_air_duvw(s, a->unn, a->vnn, a->wnn, a->pnn, s->rk_k[0][0], s->rk_k[0][1], s->rk_k[0][2]);
+        if (s->south_porous) {
+            ogrid *p = &s->pore;
+            pore_fp_on(s, p->unn, p->vnn, p->wnn, s->rk_k[1][3]);
+            pore_rk_duvw_on(s, p->unn, p->vnn, p->wnn, p->pnn, s->rk_k[1][0], s->rk_k[1][1], s->rk_k[1][2]);
+        }
+        for (int q = 0; q < 6; ++q) for (int f = 0; f < 4; ++f) sl[q]->q[f] = s->qsave[q][f];
+        /* auxiliaries: dQ/dt = f at every slab node, f of the stage state (face values included) */
+        for (int q = 0; q < 6; ++q) {
+            oslab *b = sl[q];
+            if (!b->n || (q == 2 && !y1on)) continue;
+            int ax = q >> 1;
+            int lo = (q & 1) ? (ax == 0 ? a->nx : ax == 1 ? a->ny : a->nz) - b->n : 0;
+            const double *src[4] = {a->unn, a->vnn, a->wnn, a->pnn};
+            long cnt = orc_q_size(s, q);
+            for (long e = 0; e < cnt; ++e) {
+                long A = ax == 0 ? b->n : a->nx, B = ax == 1 ? b->n : a->ny;
+                int i = (int)(e % A), j = (int)((e / A) % B), k = (int)(e / (A * B));
+                if (ax == 0) i += lo; else if (ax == 1) j += lo; else k += lo;
+                size_t id = IDX(a, i, j, k);
+                for (int f = 0; f < 4; ++f) {
+                    double kq = a->dt * src[f][id];
+                    s->qs[q][f][e] = rk_combine(st, b->q[f][e], kq, &s->qa[q][f][e]);
+                }
+            }
+        }
+        /* fields at interior points */
+        for (int d = 0; d < nd; ++d) {
+            ogrid *G = g[d];
+            double *xn[4] = {G->un, G->vn, G->wn, G->pn}, *xs[4] = {G->unn, G->vnn, G->wnn, G->pnn};
+            for (int k = 1; k < G->nz - 1; ++k)
+                for (int j = 1; j < G->ny - 1; ++j)
+                    for (int i = 1; i < G->nx - 1; ++i) {
+                        size_t id = IDX(G, i, j, k);
+                        for (int f = 0; f < 4; ++f)
+                            xs[f][id] = rk_combine(st, xn[f][id], s->rk_k[d][f][id], &s->rk_a[d][f][id]);
+                    }
+        }
+        rk_faces(s);
+    }
+    /* the last stage state is level n+1 */
+    for (int d = 0; d < nd; ++d) {
+        size_t n = (size_t)g[d]->nx * g[d]->ny * g[d]->nz;
+        memcpy(g[d]->un, g[d]->unn, n * 8); memcpy(g[d]->vn, g[d]->vnn, n * 8);
+        memcpy(g[d]->wn, g[d]->wnn, n * 8); memcpy(g[d]->pn, g[d]->pnn, n * 8);
+    }
+    for (int q = 0; q < 6; ++q)
+        for (int f = 0; f < 4; ++f) {
+            long n = orc_q_size(s, q);
+            if (n) memcpy(sl[q]->q[f], s->qs[q][f], n * 8);
+        }
+}
+
 void orc_step(orc_sim *s, int nsteps, double *rec)
 {
+    if (s->opt.integrator == LEE_INT_RK4 && rk_alloc(s)) return;
     for (int t = 0; t < nsteps; ++t) {
         if (rec) for (int r = 0; r < s->nrec; ++r) rec[(size_t)t * s->nrec + r] = sample(s, r);
-        one_step(s);
+        if (s->opt.integrator == LEE_INT_RK4) rk4_step(s);
+        else one_step(s);
     }
 }
 
@@ -1359,6 +1640,16 @@ void orc_q_set(orc_sim *s, int slab, int field, const double *buf)
     long n = orc_q_size(s, slab);
     if (n) memcpy(all[slab]->q[field], buf, n * 8);
 }
+/* The nine inverse-metric terms and the Jacobian at an interior node (xX xY xZ yX yY yZ
+ * zX zY zZ J) of the air (domain 0) or porous (1) grid. */
+void orc_metric(const orc_sim *s, int domain, int i, int j, int k, double *out)
+{
+    const ogrid *g = domain ? &s->pore : &s->air;
+    metric9 m = point_metric(g, i, j, k);
+    double v[10] = {m.xX, m.xY, m.xZ, m.yX, m.yY, m.yZ, m.zX, m.zY, m.zZ, g->Jac[IDX(g, i, j, k)]};
+    memcpy(out, v, sizeof v);
+}
+
 void orc_wind(const orc_sim *s, double *U, double *V, double *W)
 {
     size_t n = (size_t)s->air.nx * s->air.ny * s->air.nz;

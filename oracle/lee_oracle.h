@@ -33,6 +33,9 @@ typedef struct orc_sim orc_sim;
  * zero (checks that no stale boundary value ever leaks into the state).  Returns NULL on
  * error (message via orc_error()). */
 orc_sim *orc_create(const lee_deck *d, int nrec, const double *rec, int fill_nn_nan);
+/* The same with the sidecar options: receivers, and the extensions of lee_options
+ * (integrator = LEE_INT_RK4, terrain hills, log wind of velocity_method 7). */
+orc_sim *orc_create_opt(const lee_deck *d, const lee_options *opt, int fill_nn_nan);
 void     orc_destroy(orc_sim *s);
 const char *orc_error(void);
 
@@ -73,6 +76,8 @@ void orc_update_q(orc_sim *s, int which /* 0 = Qp from p_nn, 1 = Quvw from uvw_n
 /* Advance only the y1 time-gate counter as air::cal_fp does (air.cpp:1965-1969). */
 void orc_tick_gate(orc_sim *s);
 void orc_wind(const orc_sim *s, double *U, double *V, double *W);
+/* Inverse metrics xX xY xZ yX yY yZ zX zY zZ and Jacobian at an interior node. */
+void orc_metric(const orc_sim *s, int domain, int i, int j, int k, double *out10);
 
 /* Reference FFT (FFT.cpp:15-82) and spectrum writer (FFT.cpp:84-121). */
 void orc_fft(int dir, int m, double *x, double *y);

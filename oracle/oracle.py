@@ -35,6 +35,8 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         L.orc_create.restype = ctypes.c_void_p
         L.orc_create.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int]
+        L.orc_create_opt.restype = ctypes.c_void_p
+        L.orc_create_opt.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
         L.orc_error.restype = ctypes.c_char_p
         L.orc_dims.argtypes = [ctypes.c_void_p, _ip]
@@ -58,9 +60,19 @@ def lib():
         L.orc_update_q.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp]
         L.orc_tick_gate.argtypes = [ctypes.c_void_p]
         L.orc_wind.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_metric.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_fft.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
         _lib = L
     return _lib
+
+
+def _deck_module():
+    import importlib
+    import sys
+    root = os.path.dirname(HERE)
+    if root not in sys.path:
+        sys.path.insert(0, root)
+    return importlib.import_module("3d_mpi_test_amd.deck")
 
 
 def _p(a):
@@ -70,17 +82,13 @@ def _p(a):
 class Oracle:
     """One oracle simulation built from a ``Deck``."""
 
-    def __init__(self, deck, receivers=None, fill_nn_nan=False):
+    def __init__(self, deck, receivers=None, fill_nn_nan=False, integrator=None):
         L = lib()
         self.deck = deck
         self._cdeck = deck.to_c()
-        rec = None
-        n = 0
-        if receivers is not None:
-            rec = np.ascontiguousarray(receivers, dtype=np.float64).reshape(-1)
-            n = rec.size // 3
-        self._rec = rec
-        h = L.orc_create(ctypes.byref(self._cdeck), n, _p(rec), 1 if fill_nn_nan else 0)
+        # the sidecar options (receivers, integrator, terrain, log wind) of the deck
+        self._opt = _deck_module().make_options(deck, receivers=receivers, integrator=integrator)
+        h = L.orc_create_opt(ctypes.byref(self._cdeck), ctypes.byref(self._opt), 1 if fill_nn_nan else 0)
         if not h:
             raise ValueError("oracle: " + L.orc_error().decode())
         self.h = h
@@ -177,6 +185,12 @@ class Oracle:
 
     def tick_gate(self):
         lib().orc_tick_gate(self.h)
+
+    def metric(self, i, j, k, domain=0):
+        """(xX xY xZ yX yY yZ zX zY zZ, J) at an interior node."""
+        out = np.zeros(10)
+        lib().orc_metric(self.h, domain, i, j, k, _p(out))
+        return out[:9].reshape(3, 3), out[9]
 
     def wind(self):
         outs = [np.zeros(self.shape) for _ in range(3)]

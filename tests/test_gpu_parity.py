@@ -33,14 +33,22 @@ def cases(lee):
                      time_domain=40, velocity_method=5, velocity_coef=3.0)
     out["wind2"] = B(24, 28, 20, 0.05, bc=(1, 1, 1, 1, 1, 1), src=(0.0, 0.6, 0.5), gauss_width=0.25,
                      time_domain=40, velocity_method=2, velocity_coef=1.0)
+    # long in y: the fused pass splits the planes into chunks of several planes
+    out["long"] = B(24, 200, 20, 0.05, pml=5, bc=(2, 3, 5, 2, 2, 2), porous_cells=6, src=(0.0, 1.0, 0.4),
+                    gauss_width=0.25, time_domain=30, pml_y_time=5)
     return out
 
 
-@pytest.mark.parametrize("name", ["rigid", "pml5", "tdbc", "porous", "wind5", "wind2"])
-def test_fb_fp64_parity(lee, orc, name):
+PATHS = {"split": 1, "fused": 2}
+
+
+@pytest.mark.parametrize("path", ["split", "fused"])
+@pytest.mark.parametrize("name", ["rigid", "pml5", "tdbc", "porous", "wind5", "wind2", "long"])
+def test_fb_fp64_parity(lee, orc, name, path):
     d = cases(lee)[name]
-    print(f"[gpu parity] case={name} shape={d.shape}", flush=True)
-    s = lee.Solver(d, precision=64, receivers=REC)
+    print(f"[gpu parity] case={name} path={path} shape={d.shape}", flush=True)
+    s = lee.Solver(d, precision=64, receivers=REC, path=PATHS[path])
+    assert s.path == PATHS[path]
     s.frame_setup(0)
     got = s.step(d.time_domain)
     o = orc.Oracle(d, receivers=REC)
@@ -62,13 +70,14 @@ def test_fb_fp64_parity(lee, orc, name):
     assert e_f <= 1e-10
 
 
-def test_moving_frame_run(lee, orc):
+@pytest.mark.parametrize("path", ["split", "fused"])
+def test_moving_frame_run(lee, orc, path):
     d = lee.deck.box(22, 30, 18, 0.05, pml=4, bc=(2, 1, 2, 2, 2, 2), src=(0.0, 0.5, 0.4), gauss_width=0.25,
                      time_domain=90, mf_judge=1, mf_jmax=30, mf_lead_dj=6, mf_trail_dj=6, mf_ratio=0.7,
                      ijkmax=70, c=343.0)
     d.air_y = np.arange(70) * 0.05
-    print("[gpu parity] moving frame", flush=True)
-    s = lee.Solver(d, precision=64, receivers=REC)
+    print(f"[gpu parity] moving frame path={path}", flush=True)
+    s = lee.Solver(d, precision=64, receivers=REC, path=PATHS[path])
     assert s.frame_count() > 2
     got = s.run()
     o = orc.Oracle(d, receivers=REC)
@@ -91,3 +100,22 @@ def test_fp32_tracks_fp64(lee):
     print(f"  fp32 receiver rel-L2 vs fp64 {e:.3e}", flush=True)
     a.close(); b.close()
     assert e < 1e-4
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_fused_equals_split_bitwise(lee, prec):
+    """Both kernel paths give the same bits, FP32 included."""
+    for name in ("pml5", "porous", "wind5", "long"):
+        d = cases(lee)[name]
+        print(f"[gpu] fused vs split fp{prec} case={name}", flush=True)
+        a = lee.Solver(d, precision=prec, receivers=REC, path=PATHS["split"])
+        b = lee.Solver(d, precision=prec, receivers=REC, path=PATHS["fused"])
+        a.frame_setup(0); b.frame_setup(0)
+        ra, rb = a.step(d.time_domain), b.step(d.time_domain)
+        assert np.array_equal(ra, rb), name
+        for x, y in zip(a.state(), b.state()):
+            assert np.array_equal(x, y), name
+        if d.has_pore:
+            for x, y in zip(a.state(1), b.state(1)):
+                assert np.array_equal(x, y), name
+        a.close(); b.close()
