@@ -224,7 +224,66 @@ __global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
     }
 }
 
+// RK4 stage: auxiliaries of the boundary nodes (dQ/dt = f of the stage state, with the
+// register update of k_rk_air) and the face values of the destination state.
+template <typename T>
+__device__ __forceinline__ T rk_update_q(int st, T xn, T k, T *acc, size_t q)
+{
+    const T b = (st == 0 || st == 3) ? T(1.0 / 6.0) : T(1.0 / 3.0);
+    const T al = st == 0 ? T(0.5) : st == 1 ? T(0.5) : T(1.0);
+    if (st == 0) { acc[q] = xn + b * k; return xn + al * k; }
+    if (st < 3) { acc[q] = acc[q] + b * k; return xn + al * k; }
+    return acc[q] + b * k;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_rk_bnd(const RKBndArgs<T> r)
+{
+    const BCArgs<T> &a = r.b;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < a.nb_air) {
+        int i, j, k;
+        boundary_node(t, a.air.nx, a.air.ny, a.air.nz, i, j, k);
+        const Grid<T> &g = a.air;
+        const size_t id = gidx(g, i, j, k);
+        for (int s = 0; s < 6; ++s) {
+            const int n = a.qn[s];
+            if (!n || (s == 2 && !a.y1on)) continue;
+            const int ax = a.qaxis[s], lo = a.qlo[s];
+            const int c = ax == 0 ? i : ax == 1 ? j : k;
+            if (c < lo || c >= lo + n) continue;
+            const int A = ax == 0 ? n : g.nx, B = ax == 1 ? n : g.ny;
+            const size_t q = (size_t)(ax == 0 ? i - lo : i) + (size_t)A * ((size_t)(ax == 1 ? j - lo : j) + (size_t)B * (ax == 2 ? k - lo : k));
+            for (int f = 0; f < 4; ++f)
+                r.qdst[s][f][q] = rk_update_q<T>(r.stage, r.qxn[s][f][q], a.dt * r.src[f][id], r.qacc[s][f], q);
+        }
+        const T p = p_air_walk<T, true>(a, i, j, k, 6);
+        const T u = v_air_walk<T, true>(a, 0, i, j, k, 6);
+        const T v = v_air_walk<T, true>(a, 1, i, j, k, 6);
+        const T w = v_air_walk<T, true>(a, 2, i, j, k, 6);
+        g.pnn[id] = p; g.unn[id] = u; g.vnn[id] = v; g.wnn[id] = w;
+    } else if (t < a.nb_air + a.nb_pore) {
+        int i, j, k;
+        boundary_node(t - a.nb_air, a.pore.nx, a.pore.ny, a.pore.nz, i, j, k);
+        const Grid<T> &g = a.pore;
+        const size_t id = gidx(g, i, j, k);
+        const T p = p_pore_walk(a, i, j, k, 6);
+        const T u = v_pore_walk(a, 0, i, j, k, 6);
+        const T v = v_pore_walk(a, 1, i, j, k, 6);
+        const T w = v_pore_walk(a, 2, i, j, k, 6);
+        g.pnn[id] = p; g.unn[id] = u; g.vnn[id] = v; g.wnn[id] = w;
+    }
+}
+
 }  // namespace
+
+template <typename T> void launch_rk_bnd(const RKBndArgs<T> &r, hipStream_t s)
+{
+    const long long n = r.b.nb_air + r.b.nb_pore;
+    hipLaunchKernelGGL((k_rk_bnd<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, r);
+}
+template void launch_rk_bnd<float>(const RKBndArgs<float> &, hipStream_t);
+template void launch_rk_bnd<double>(const RKBndArgs<double> &, hipStream_t);
 
 template <typename T> long long boundary_count(int nx, int ny, int nz)
 {

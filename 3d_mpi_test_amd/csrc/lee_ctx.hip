@@ -56,6 +56,12 @@ struct lee_ctx {
     void *Q[2][6][4] = {};  // PML auxiliaries, two sets: Q[qcur] holds the values before a step
     int qcur = 0;
     bool fused = false;     // both air phases in one kernel (k_air_fb)
+    // RK4 registers: second stage state, accumulator, FP32 accumulator low part and state
+    // error ([0] air, [1] porous; u v w p), and of the auxiliaries (stage state, accumulator)
+    void *rk_sb[2][4] = {}, *rk_acc[2][4] = {}, *rk_lo[2][4] = {}, *rk_err[2][4] = {};
+    void *rkq_sb[6][4] = {}, *rkq_acc[6][4] = {};
+    // terrain tables (sh, hx, hy per (i,j); tk per k) of the air [0] and porous [1] grids
+    void *ter[2][4] = {};
     double *psi = nullptr, *psi2 = nullptr;  // TDBC histories, read / written each step
     long long *rec_idx = nullptr;
     double *rec_buf = nullptr;
@@ -72,7 +78,7 @@ struct lee_ctx {
     size_t evnext = 0;
     struct Pending { int kind; hipEvent_t a, b; };
     std::vector<Pending> pending;
-    lee_kstat kstat[8] = {};
+    lee_kstat kstat[10] = {};
 
     ~lee_ctx()
     {
@@ -145,7 +151,8 @@ int download(lee_ctx *c, double *dst, const void *src, size_t n)
     return LEE_OK;
 }
 
-enum { KS_AIR_P = 0, KS_AIR_V = 1, KS_PORE_P = 2, KS_PORE_V = 3, KS_AIR_FB = 4, KS_BC_P = 5, KS_BC_V = 6, KS_N = 7 };
+enum { KS_AIR_P = 0, KS_AIR_V = 1, KS_PORE_P = 2, KS_PORE_V = 3, KS_AIR_FB = 4, KS_BC_P = 5, KS_BC_V = 6,
+       KS_RK_AIR = 7, KS_RK_PORE = 8, KS_RK_BND = 9, KS_N = 10 };
 
 hipEvent_t pooled_event(lee_ctx *c)
 {
@@ -194,6 +201,7 @@ template <typename T> Grid<T> air_grid(lee_ctx *c)
     g.unn = (T *)c->air[o + 0]; g.vnn = (T *)c->air[o + 1]; g.wnn = (T *)c->air[o + 2]; g.pnn = (T *)c->air[o + 3];
     g.mx = (const T *)c->mx; g.my = (const T *)c->my; g.mz = (const T *)c->mz;
     g.dxi = (T)m.dxi; g.dyi = (T)m.dyi; g.dzi = (T)m.dzi; g.dt = (T)m.dt;
+    g.tsh = (const T *)c->ter[0][0]; g.thx = (const T *)c->ter[0][1]; g.thy = (const T *)c->ter[0][2]; g.ttk = (const T *)c->ter[0][3];
     return g;
 }
 
@@ -207,6 +215,7 @@ template <typename T> Grid<T> pore_grid(lee_ctx *c)
     g.unn = (T *)c->pore[o + 0]; g.vnn = (T *)c->pore[o + 1]; g.wnn = (T *)c->pore[o + 2]; g.pnn = (T *)c->pore[o + 3];
     g.mx = (const T *)c->pmx; g.my = (const T *)c->pmy; g.mz = (const T *)c->pmz;
     g.dxi = (T)m.pdxi; g.dyi = (T)m.pdyi; g.dzi = (T)m.pdzi; g.dt = (T)m.dt;
+    g.tsh = (const T *)c->ter[1][0]; g.thx = (const T *)c->ter[1][1]; g.thy = (const T *)c->ter[1][2]; g.ttk = (const T *)c->ter[1][3];
     return g;
 }
 
@@ -264,7 +273,9 @@ int upload_frame(lee_ctx *c, int N)
     FrameTables t;
     m.frame_tables(N, t);
     auto metric = [&](void *dst, const std::vector<double> &v) {
-        if (c->prec == 64) return upload(c, dst, v.data(), v.size());
+        // FP32 on the separable grid stores the reciprocals (the metric itself); the terrain
+        // kernels form the Jacobian per point from X_over_x ... in either precision
+        if (c->prec == 64 || m.terrain) return upload(c, dst, v.data(), v.size());
         std::vector<double> r(v.size(), 0.0);
         for (size_t i = 1; i + 1 < v.size(); ++i) r[i] = 1.0 / v[i];  // separable metric in FP64
         return upload(c, dst, r.data(), r.size());
@@ -283,6 +294,14 @@ int upload_frame(lee_ctx *c, int N)
         if ((rc = upload(c, c->U, t.U.data(), t.U.size())) || (rc = upload(c, c->V, t.V.data(), t.V.size())) ||
             (rc = upload(c, c->W, t.W.data(), t.W.size())))
             return rc;
+    }
+    if (m.terrain) {
+        const TerrainTables *tt[2] = {&t.ter, &t.pter};
+        for (int d = 0; d < (m.south_porous ? 2 : 1); ++d) {
+            const std::vector<double> *v[4] = {&tt[d]->sh, &tt[d]->hx, &tt[d]->hy, &tt[d]->tk};
+            for (int q = 0; q < 4; ++q)
+                if ((rc = upload(c, c->ter[d][q], v[q]->data(), v[q]->size()))) return rc;
+        }
     }
     // receiver linear indices in this frame
     int nr = (int)m.ri.size();
@@ -308,6 +327,14 @@ template <typename T> int frame_shift(lee_ctx *c, int N)
             launch_shift<T>((T *)c->pore[o + f], (const T *)c->pore[n + f], m.pnx, m.pny, m.pnz, tempJ, m.lead, s, c->stream);
     c->cur ^= 1;
     n = c->cur * 4;
+    // the FP32 RK4 state error moves with the state (through the second stage buffer)
+    if (m.rk4 && c->rk_err[0][0])
+        for (int d = 0; d < (m.south_porous ? 2 : 1); ++d)
+            for (int f = 0; f < 4; ++f) {
+                const int A = d ? m.pnx : m.nx, B = d ? m.pny : m.ny, C = d ? m.pnz : m.nz;
+                launch_shift<T>((T *)c->rk_sb[d][f], (const T *)c->rk_err[d][f], A, B, C, tempJ, m.lead, s, c->stream);
+                std::swap(c->rk_sb[d][f], c->rk_err[d][f]);
+            }
     launch_copy_plane_j<T>((T *)c->air[n + 3], m.nx, m.ny, m.nz, c->stream);
     if (m.south_porous) launch_copy_plane_j<T>((T *)c->pore[n + 3], m.pnx, m.pny, m.pnz, c->stream);
     // PML auxiliaries of the x and z layers (air.cpp:2522-2729); y layers stay
@@ -326,9 +353,142 @@ template <typename T> int frame_shift(lee_ctx *c, int N)
     return LEE_OK;
 }
 
+// ---- RK4 (extension): four stages per step; level n stays in the current level buffers
+// (frame shifts, state access and receivers work as for FB), the stage states alternate
+// between the other level buffers (A) and rk_sb (B):  X -> A -> B -> A -> X.
+template <typename T> void rk_stage_buffers(lee_ctx *c, int d, int st, T *src[4], T *dst[4])
+{
+    void **lev = d ? c->pore : c->air;
+    T *X[4], *A[4], *B[4];
+    for (int f = 0; f < 4; ++f) {
+        X[f] = (T *)lev[c->cur * 4 + f];
+        A[f] = (T *)lev[(c->cur ^ 1) * 4 + f];
+        B[f] = (T *)c->rk_sb[d][f];
+    }
+    T **S[4] = {X, A, B, A}, **D[4] = {A, B, A, X};
+    for (int f = 0; f < 4; ++f) { src[f] = S[st][f]; dst[f] = D[st][f]; }
+}
+
+template <typename T> void rk_stage_q(lee_ctx *c, int st, int s, int f, T *&src, T *&dst)
+{
+    T *X = (T *)c->Q[c->qcur][s][f], *A = (T *)c->Q[c->qcur ^ 1][s][f], *B = (T *)c->rkq_sb[s][f];
+    T *S[4] = {X, A, B, A}, *D[4] = {A, B, A, X};
+    src = S[st];
+    dst = D[st];
+}
+
+template <typename T> int rk4_steps(lee_ctx *c, int nsteps, double *rec_out)
+{
+    Model &m = c->m;
+    hipStream_t s = c->stream;
+    const int nr = (int)m.ri.size();
+    const bool gate = (m.judge[1] == 1 || m.judge[1] == 3);
+    const int wind = m.has_wind ? (m.wind_v_only ? 2 : 1) : 0;
+    HIPCHK(c, hipEventRecord(c->ev0, s));
+    int buffered = 0, written = 0;
+    for (int t = 0; t < nsteps; ++t) {
+        int J1 = m.J1;
+        if (gate) {
+            c->counter += 1;
+            J1 = c->counter >= m.pml_time ? m.J3 : 0;
+        }
+        const int y1on = (m.slab[2].n > 0 && gate && c->counter >= m.pml_time) ? 1 : 0;
+        // receivers sample level n before the step (airpore.cpp:573-580)
+        launch_sample<T>((const T *)c->air[c->cur * 4 + 3], c->rec_idx, nr, c->rec_buf + (size_t)buffered * nr, s);
+        for (int st = 0; st < 4; ++st) {
+            T *src[4], *dst[4];
+            rk_stage_buffers<T>(c, 0, st, src, dst);
+            RKArgs<T> r;
+            r.a = air_args<T>(c, J1, y1on);
+            Grid<T> &g = r.a.g;
+            g.un = src[0]; g.vn = src[1]; g.wn = src[2]; g.pn = src[3];
+            g.unn = g.vnn = g.wnn = nullptr;
+            g.pnn = src[3];  // the velocity increment reads the stage pressure
+            for (int f = 0; f < 4; ++f) {
+                r.xn[f] = (const T *)c->air[c->cur * 4 + f];
+                r.acc[f] = (T *)c->rk_acc[0][f]; r.lo[f] = (T *)c->rk_lo[0][f]; r.err[f] = (T *)c->rk_err[0][f];
+                r.dst[f] = dst[f];
+            }
+            for (int q = 0; q < 6; ++q)
+                for (int f = 0; f < 4; ++f) {
+                    T *qs, *qd;
+                    rk_stage_q<T>(c, st, q, f, qs, qd);
+                    r.a.Qo[q][f] = qs; r.a.Qn[q][f] = qs;
+                    r.qxn[q][f] = (const T *)c->Q[c->qcur][q][f];
+                    r.qacc[q][f] = (T *)c->rkq_acc[q][f];
+                    r.qdst[q][f] = qd;
+                }
+            r.stage = st;
+            prof_mark(c, KS_RK_AIR, true);
+            launch_rk_air<T>(r, wind, m.terrain, s);
+            prof_mark(c, KS_RK_AIR, false);
+            T *psrc[4] = {}, *pdst[4] = {};
+            if (m.south_porous) {
+                rk_stage_buffers<T>(c, 1, st, psrc, pdst);
+                RKPoreArgs<T> pr;
+                pr.g = pore_grid<T>(c);
+                pr.g.un = psrc[0]; pr.g.vn = psrc[1]; pr.g.wn = psrc[2]; pr.g.pn = psrc[3];
+                pr.g.unn = pr.g.vnn = pr.g.wnn = pr.g.pnn = nullptr;
+                pr.gama = (T)m.p_gama; pr.eff = (T)m.p_eff; pr.beta = (T)m.p_beta;
+                for (int f = 0; f < 4; ++f) {
+                    pr.xn[f] = (const T *)c->pore[c->cur * 4 + f];
+                    pr.acc[f] = (T *)c->rk_acc[1][f]; pr.lo[f] = (T *)c->rk_lo[1][f]; pr.err[f] = (T *)c->rk_err[1][f];
+                    pr.dst[f] = pdst[f];
+                }
+                pr.stage = st;
+                prof_mark(c, KS_RK_PORE, true);
+                launch_rk_pore<T>(pr, m.terrain, s);
+                prof_mark(c, KS_RK_PORE, false);
+            }
+            // faces of the destination state and the boundary-node auxiliaries
+            RKBndArgs<T> b;
+            b.b = bc_args<T>(c, r.a, nullptr);
+            b.b.tdbc = 0;
+            b.b.nrec = 0;
+            b.b.air.unn = dst[0]; b.b.air.vnn = dst[1]; b.b.air.wnn = dst[2]; b.b.air.pnn = dst[3];
+            if (m.south_porous) {
+                b.b.pore.unn = pdst[0]; b.b.pore.vnn = pdst[1]; b.b.pore.wnn = pdst[2]; b.b.pore.pnn = pdst[3];
+            }
+            for (int f = 0; f < 4; ++f) b.src[f] = src[f];
+            for (int q = 0; q < 6; ++q)
+                for (int f = 0; f < 4; ++f) { b.qxn[q][f] = r.qxn[q][f]; b.qacc[q][f] = r.qacc[q][f]; b.qdst[q][f] = r.qdst[q][f]; }
+            b.stage = st;
+            prof_mark(c, KS_RK_BND, true);
+            launch_rk_bnd<T>(b, s);
+            prof_mark(c, KS_RK_BND, false);
+        }
+        ++buffered;
+        c->steps += 1;
+        if (buffered == kRecChunk || t == nsteps - 1) {
+            if (rec_out && nr) {
+                HIPCHK(c, hipMemcpyAsync(rec_out + (size_t)written * nr, c->rec_buf, (size_t)buffered * nr * 8,
+                                         hipMemcpyDeviceToHost, s));
+                HIPCHK(c, hipStreamSynchronize(s));
+            }
+            written += buffered;
+            buffered = 0;
+        }
+        if (m.opt.health_every > 0 && c->steps % m.opt.health_every == 0) {
+            HIPCHK(c, hipMemsetAsync(c->flag, 0, sizeof(int), s));
+            launch_finite_check<T>((const T *)c->air[c->cur * 4 + 3], air_n(c), c->flag, s);
+            int h = 0;
+            HIPCHK(c, hipMemcpyAsync(&h, c->flag, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPCHK(c, hipStreamSynchronize(s));
+            if (h) return set_err(c, LEE_ERR_NONFINITE, "non-finite pressure after step " + std::to_string(c->steps));
+        }
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, s));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    prof_resolve(c);
+    return LEE_OK;
+}
+
 template <typename T> int run_steps(lee_ctx *c, int nsteps, double *rec_out)
 {
     Model &m = c->m;
+    if (m.rk4) return rk4_steps<T>(c, nsteps, rec_out);
     hipStream_t s = c->stream;
     const int nr = (int)m.ri.size();
     const bool gate = (m.judge[1] == 1 || m.judge[1] == 3);
@@ -433,9 +593,11 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
     C->esz = o.precision == 64 ? 8 : 4;
     {
         const Model &mm = C->m;
-        const bool can = fused_fits<double>(mm.nx) && !(mm.south_porous && !mm.iface_copy);
+        const bool can = fused_fits<double>(mm.nx) && !(mm.south_porous && !mm.iface_copy) && !mm.terrain && !mm.rk4;
         if (o.path == LEE_PATH_FUSED && !can)
-            return set_err(nullptr, LEE_ERR_UNSUPPORTED, "the fused path needs IMAX - 2 <= 1024 and, with a porous South face, the interface copy");
+            return set_err(nullptr, LEE_ERR_UNSUPPORTED,
+                           "the fused path runs the FB_p_v step on a separable grid and, with a porous South face, "
+                           "needs the interface copy");
         if (o.path < LEE_PATH_AUTO || o.path > LEE_PATH_FUSED) return set_err(nullptr, LEE_ERR_ARG, "unknown path");
         // AUTO runs the split passes: on MI355X they are faster than the fused pass (whose
         // per-plane barrier limits the loads in flight)
@@ -476,6 +638,30 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
     if ((rc = dalloc(C, (void **)&C->rec_idx, (size_t)std::max(nr, 1) * sizeof(long long)))) return rc;
     if ((rc = dalloc(C, (void **)&C->rec_buf, (size_t)kRecChunk * std::max(nr, 1) * 8))) return rc;
     if ((rc = dalloc(C, (void **)&C->flag, sizeof(int)))) return rc;
+    if (m.rk4) {
+        for (int d = 0; d < (m.south_porous ? 2 : 1); ++d) {
+            size_t n = d ? np : na;
+            for (int f = 0; f < 4; ++f) {
+                if ((rc = dalloc(C, &C->rk_sb[d][f], n * es)) || (rc = dalloc(C, &C->rk_acc[d][f], n * es))) return rc;
+                if (C->prec == 32)
+                    if ((rc = dalloc(C, &C->rk_lo[d][f], n * es)) || (rc = dalloc(C, &C->rk_err[d][f], n * es))) return rc;
+            }
+        }
+        for (int s = 0; s < 6; ++s) {
+            size_t cnt = m.slab[s].count(m.nx, m.ny, m.nz);
+            for (int f = 0; f < 4; ++f)
+                if ((rc = dalloc(C, &C->rkq_sb[s][f], cnt * es)) || (rc = dalloc(C, &C->rkq_acc[s][f], cnt * es))) return rc;
+        }
+    }
+    if (m.terrain) {
+        for (int d = 0; d < (m.south_porous ? 2 : 1); ++d) {
+            size_t n2 = d ? (size_t)m.pnx * m.pny : (size_t)m.nx * m.ny;
+            size_t nk = d ? (size_t)m.pnz : (size_t)m.nz;
+            for (int q = 0; q < 3; ++q)
+                if ((rc = dalloc(C, &C->ter[d][q], n2 * es))) return rc;
+            if ((rc = dalloc(C, &C->ter[d][3], nk * es))) return rc;
+        }
+    }
     C->scratch_bytes = std::max<size_t>(qmax * es, 8);
     if ((rc = dalloc(C, &C->scratch, C->scratch_bytes))) return rc;
     // initial condition (calculation_2D.cpp:300-314): velocities zero, Gaussian pressure
@@ -676,10 +862,24 @@ int lee_kernel_stats(const lee_ctx *c, lee_kstat *out, int cap)
     const double nb = (double)boundary_count<double>(m.nx, m.ny, m.nz) +
                       (m.south_porous ? (double)boundary_count<double>(m.pnx, m.pny, m.pnz) : 0.0);
     const double psi = m.east_tdbc ? (double)m.nx * m.nz * 6 * 8 * 2 : 0.0;
-    const char *names[KS_N] = {"air_p", "air_v", "pore_p", "pore_v", "air_fb", "bc_p", "bc_v"};
+    // RK4, per stage launch (a quarter of a step): fields 64 words per point and step (x_n,
+    // stage state, accumulator: stages read 4/12/12/8 and write 8/8/8/4 words), 32 more for
+    // the FP32 compensation (accumulator low part, state error); the auxiliaries likewise 64
+    // per slab element and step; the mean flow 1 (V only) or 3 words per point and stage;
+    // the faces write 4 words per boundary node from 4 source values.
+    const double rkw = (c->prec == 32 ? 96.0 : 64.0) / 4.0;
+    double qrk = 0;
+    for (int s = 0; s < 6; ++s) {
+        double cnt = (double)m.slab[s].count(m.nx, m.ny, m.nz);
+        if (!cnt || (s == 2 && m.J3 == 0)) continue;
+        qrk += cnt * 16.0;
+    }
+    const double wrk = m.has_wind ? (m.wind_v_only ? 1.0 : 3.0) * m.nx * m.ny * m.nz : 0.0;
+    const char *names[KS_N] = {"air_p", "air_v", "pore_p", "pore_v", "air_fb", "bc_p", "bc_v", "rk_air", "rk_pore", "rk_bnd"};
     double bytes[KS_N] = {(5 * inner + qp + wind) * es, (7 * inner + qv + wind) * es, 5 * pinner * es, 7 * pinner * es,
-                          (8 * inner + qf + wind) * es, 2 * nb * es, 6 * nb * es + psi};
-    double pts[KS_N] = {inner, inner, pinner, pinner, inner, nb, nb};
+                          (8 * inner + qf + wind) * es, 2 * nb * es, 6 * nb * es + psi,
+                          (rkw * inner + qrk + wrk) * es, rkw * pinner * es, 8 * nb * es};
+    double pts[KS_N] = {inner, inner, pinner, pinner, inner, nb, nb, inner, pinner, nb};
     int n = 0;
     for (int k = 0; k < KS_N && n < cap; ++k) {
         lee_kstat s = c->kstat[k];

@@ -25,6 +25,9 @@ template <typename T> struct Grid {
     // reference does).  FP32: the reciprocals 1/X_over_x ... (= the separable metric).
     const T *mx, *my, *mz;
     T dxi, dyi, dzi, dt;
+    // terrain-following grid (extension): Z_over_z = mz[k]*tsh[c], Z_over_x = thx[c]*ttk[k],
+    // Z_over_y = thy[c]*ttk[k], c = i + nx*j; mx, my, mz then hold X_over_x ... in both precisions
+    const T *tsh, *thx, *thy, *ttk;
 };
 
 template <typename T> struct AirArgs {
@@ -66,6 +69,38 @@ template <typename T> struct BCArgs {
     int nrec;
     double *rec_out;
 };
+// ---- RK4 (extension) ------------------------------------------------------------------
+// Registers: level n (xn), the accumulator acc = xn + sum_s b_s k_s, and the destination
+// (the next stage state xn + a_s k_s, or level n+1 = acc + b_4 k_4 after the last stage).
+// FP32 adds a compensated accumulation: `lo` is the low part of acc within a step and `err`
+// the rounding error of the stored state, carried from step to step.
+template <typename T> struct RKArgs {
+    AirArgs<T> a;              // stage state in g.un .. g.pn (g.pnn = g.pn), auxiliaries Qo = Qn
+    const T *xn[4];            // u v w p at level n
+    T *acc[4], *lo[4], *err[4], *dst[4];
+    const T *qxn[6][4];        // auxiliaries at level n, accumulator, destination
+    T *qacc[6][4], *qdst[6][4];
+    int stage;                 // 0..3
+};
+template <typename T> struct RKPoreArgs {
+    Grid<T> g;                 // stage state in g.un .. g.pn
+    T gama, eff, beta;         // porous.cpp:44-50: dt/(Kp Omega), 1/rho_e, sigma dt/rho_e
+    const T *xn[4];
+    T *acc[4], *lo[4], *err[4], *dst[4];
+    int stage;
+};
+template <typename T> struct RKBndArgs {
+    BCArgs<T> b;               // destination state as air/pore "nn" fields (face walks)
+    const T *src[4];           // air stage state (right-hand side of the boundary auxiliaries)
+    const T *qxn[6][4];
+    T *qacc[6][4], *qdst[6][4];
+    int stage;
+};
+// wind: 0 none, 1 U V W fields, 2 V only (U = W = 0: velocity_method 7)
+template <typename T> void launch_rk_air(const RKArgs<T> &r, int wind, bool terrain, hipStream_t s);
+template <typename T> void launch_rk_pore(const RKPoreArgs<T> &r, bool terrain, hipStream_t s);
+template <typename T> void launch_rk_bnd(const RKBndArgs<T> &r, hipStream_t s);
+
 template <typename T> long long boundary_count(int nx, int ny, int nz);
 template <typename T> void launch_bc_p_all(const BCArgs<T> &a, hipStream_t s);
 template <typename T> void launch_bc_v_all(const BCArgs<T> &a, hipStream_t s);

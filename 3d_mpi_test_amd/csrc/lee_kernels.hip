@@ -36,13 +36,30 @@ template <typename T> __device__ __forceinline__ void metric(const Grid<T> &g, i
     }
 }
 
+// Terrain-following grid: X_over_x, Y_over_y as in the reference, Z_over_z = Zz[k]*sh(i,j),
+// Z_over_x = hx(i,j)*tk[k], Z_over_y = hy(i,j)*tk[k].  Of the nine inverse-metric terms of
+// Geo_over_var (df_over_var.cpp:5-9) five are non-zero: xX, yY, zZ and the cross terms
+// zX = -(Yy*Zx)*J, zY = -(Xx*Zy)*J.  The tables hold X_over_x ... (not reciprocals) in both
+// precisions.
+template <typename T> __device__ __forceinline__ void metric_terrain(const Grid<T> &g, int i, int j, int k, T &mX, T &mY,
+                                                                     T &mZ, T &zX, T &zY)
+{
+    const size_t c = (size_t)i + (size_t)g.nx * j;
+    const T xx = g.mx[i], yy = g.my[j], zz = g.mz[k] * g.tsh[c];
+    const T Zx = g.thx[c] * g.ttk[k], Zy = g.thy[c] * g.ttk[k];
+    const T J = T(1) / (xx * yy * zz);
+    mX = (yy * zz) * J;
+    mY = (xx * zz) * J;
+    mZ = (xx * yy) * J;
+    zX = (-(yy * Zx)) * J;
+    zY = (-(xx * Zy)) * J;
+}
+
 __device__ __forceinline__ size_t qxi(int ii, int j, int k, int n, int ny) { return (size_t)ii + (size_t)n * ((size_t)j + (size_t)ny * k); }
 __device__ __forceinline__ size_t qyi(int i, int jj, int k, int nx, int n) { return (size_t)i + (size_t)nx * ((size_t)jj + (size_t)n * k); }
 __device__ __forceinline__ size_t qzi(int i, int j, int kk, int nx, int ny) { return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * kk); }
 
-// Membership of interior point (i,j,k) in the six PML slabs and its index in each.  The
-// point's own auxiliaries are loaded at the top of a kernel and written back (Q += dt*f,
-// air.cpp:2323-2439) at the end, once the new value is known.
+// Membership of interior point (i,j,k) in the six PML slabs and its index in each.
 template <typename T> struct OwnQ {
     size_t idx[6];
     bool in[6];
@@ -67,60 +84,105 @@ template <typename T> __device__ __forceinline__ OwnQ<T> own_slabs(const AirArgs
 }
 
 // ------------------------------------------------------------------------------------
-// pressure increment + FB add at one interior point: air::cal_fp (air.cpp:1953-2322) and
-// calculation_2D::Cal_pressure (calculation_2D.cpp:598-608).  Writes p^{n+1} and the
-// point's Q_p (Update_PML_Qp, air.cpp:2388-2439) and returns p^{n+1}.
+// Pressure increment dt*R_p at one interior point: air::cal_fp (air.cpp:1953-2322).
+// Reads u, v, w, p of the time level in g.un ... g.pn and the auxiliaries a.Qo.  Terms
+// follow the reference's association; the TERRAIN terms are those of its nine-metric chain
+// (air.cpp:1975-2292) that the separable grid multiplies by zero, inserted in chain order.
 // ------------------------------------------------------------------------------------
-template <typename T, bool WIND>
-__device__ __forceinline__ T air_p_point(const AirArgs<T> &a, int i, int j, int k)
+template <typename T, int WM, bool TERRAIN>
+__device__ __forceinline__ T air_dp(const AirArgs<T> &a, int i, int j, int k)
 {
     const Grid<T> &g = a.g;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const size_t sxy = (size_t)nx * ny;
     const size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
-    T mX, mY, mZ;
-    metric(g, i, j, k, mX, mY, mZ);
+    T mX, mY, mZ, zX = T(0), zY = T(0);
+    if constexpr (TERRAIN) metric_terrain(g, i, j, k, mX, mY, mZ, zX, zY);
+    else metric(g, i, j, k, mX, mY, mZ);
     const T dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, dt = g.dt, gP = a.gP;
-    const OwnQ<T> own = own_slabs(a, i, j, k);
-    T qp_old[6];
-#pragma unroll
-    for (int s = 0; s < 6; ++s) qp_old[s] = own.in[s] ? a.Qo[s][3][own.idx[s]] : T(0);
     const T pc = g.pn[id];
     const T dux = g.un[id + 1] - g.un[id];
     const T dvy = g.vn[id + nx] - g.vn[id];
     const T dwz = g.wn[id + sxy] - g.wn[id];
-    const T div = (((mX * dux) * dxi) + ((mY * dvy) * dyi)) + ((mZ * dwz) * dzi);
+    T duz = T(0), dvz = T(0);
+    T div;
+    if constexpr (TERRAIN) {
+        duz = g.un[id + sxy] - g.un[id];
+        dvz = g.vn[id + sxy] - g.vn[id];
+        div = (((((mX * dux) * dxi) + ((zX * duz) * dzi)) + ((mY * dvy) * dyi)) + ((zY * dvz) * dzi)) + ((mZ * dwz) * dzi);
+    } else {
+        div = (((mX * dux) * dxi) + ((mY * dvy) * dyi)) + ((mZ * dwz) * dzi);
+    }
+    // d/dX of u and d/dY of v (the damped terms of the x and y layers)
+    T ddx = (mX * dux) * dxi, ddy = (mY * dvy) * dyi;
+    if constexpr (TERRAIN) {
+        ddx = ddx + (zX * duz) * dzi;
+        ddy = ddy + (zY * dvz) * dzi;
+    }
     T r;
-    if constexpr (WIND) {
+    if constexpr (WM == 1) {
         const T Uc = a.U[id], Vc = a.V[id], Wc = a.W[id];
-        const T c1 = ((Uc * mX) * dxi) * dt, c2 = ((Vc * mY) * dyi) * dt, c3 = ((Wc * mZ) * dzi) * dt;
-        const T UX = ((mX * (a.U[id + 1] - a.U[id - 1])) * dxi) * T(0.5);
-        const T VY = ((mY * (a.V[id + nx] - a.V[id - nx])) * dyi) * T(0.5);
+        const T c1 = ((Uc * mX) * dxi) * dt, c2 = ((Vc * mY) * dyi) * dt;
+        T c3;
+        T UX = ((mX * (a.U[id + 1] - a.U[id - 1])) * dxi) * T(0.5);
+        T VY = ((mY * (a.V[id + nx] - a.V[id - nx])) * dyi) * T(0.5);
         const T WZ = ((mZ * (a.W[id + sxy] - a.W[id - sxy])) * dzi) * T(0.5);
+        if constexpr (TERRAIN) {
+            c3 = ((((Uc * zX) + (Vc * zY)) + (Wc * mZ)) * dzi) * dt;
+            UX = UX + ((zX * (a.U[id + sxy] - a.U[id - sxy])) * dzi) * T(0.5);
+            VY = VY + ((zY * (a.V[id + sxy] - a.V[id - sxy])) * dzi) * T(0.5);
+        } else {
+            c3 = ((Wc * mZ) * dzi) * dt;
+        }
         r = ((-c1) * (pc - g.pn[id - 1]) - c2 * (pc - g.pn[id - nx])) - c3 * (pc - g.pn[id - sxy]);
         r += (((-a.gam) * pc) * dt) * ((UX + VY) + WZ);
+        r += (((-a.gam) * a.Pav) * dt) * div;
+    } else if constexpr (WM == 2) {
+        // mean flow along y only (U = W = 0): the terms with U or W are exact zeros
+        const T Vc = a.V[id];
+        const T c2 = ((Vc * mY) * dyi) * dt;
+        T VY = ((mY * (a.V[id + nx] - a.V[id - nx])) * dyi) * T(0.5);
+        r = -(c2 * (pc - g.pn[id - nx]));
+        if constexpr (TERRAIN) {
+            const T c3 = ((Vc * zY) * dzi) * dt;
+            VY = VY + ((zY * (a.V[id + sxy] - a.V[id - sxy])) * dzi) * T(0.5);
+            r = r - c3 * (pc - g.pn[id - sxy]);
+        }
+        r += (((-a.gam) * pc) * dt) * VY;
         r += (((-a.gam) * a.Pav) * dt) * div;
     } else {
         r = (((-a.gam) * a.Pav) * dt) * div;
     }
     const int n0 = a.qn[0], n1 = a.qn[1], n2 = a.qn[2], n3 = a.qn[3], n4 = a.qn[4], n5 = a.qn[5];
+    // forward differences of a slab field along y or z with the metric, plus the cross-metric
+    // z term over terrain: X-slab (stride n*ny per plane), Y-slab (nx*n), Z-slab (nx*ny)
+    auto fY = [&](const T *Q, size_t q, size_t sy, size_t sz) {
+        T v = (mY * (Q[q + sy] - Q[q])) * dyi;
+        if constexpr (TERRAIN) v = v + (zY * (Q[q + sz] - Q[q])) * dzi;
+        return v;
+    };
+    auto fX = [&](const T *Q, size_t q, size_t sz) {
+        T v = (mX * (Q[q + 1] - Q[q])) * dxi;
+        if constexpr (TERRAIN) v = v + (zX * (Q[q + sz] - Q[q])) * dzi;
+        return v;
+    };
     // X1 (air.cpp:2044-2057)
     if (i < a.I1) {
         const T sx = a.x1.s[i], cx = a.x1.c[i];
-        const size_t q = qxi(i, j, k, n0, ny);
+        const size_t q = qxi(i, j, k, n0, ny), sz = (size_t)n0 * ny;
         const T *Qv = a.Qo[0][1], *Qw = a.Qo[0][2];
-        const T A = (cx * (gP * ((mX * dux) * dxi))) * dt;
-        const T B = (sx * (pc + gP * (((mY * (Qv[q + n0] - Qv[q])) * dyi) + ((mZ * (Qw[q + (size_t)n0 * ny] - Qw[q])) * dzi)))) * dt;
+        const T A = (cx * (gP * ddx)) * dt;
+        const T B = (sx * (pc + gP * (fY(Qv, q, n0, sz) + ((mZ * (Qw[q + sz] - Qw[q])) * dzi)))) * dt;
         r += A - B;
     }
     // X2 (air.cpp:2059-2074)
     if (i >= nx - a.I2) {
         const int ii = i - (nx - n1);
         const T sx = a.x2.s[i], cx = a.x2.c[i];
-        const size_t q = qxi(ii, j, k, n1, ny);
+        const size_t q = qxi(ii, j, k, n1, ny), sz = (size_t)n1 * ny;
         const T *Qv = a.Qo[1][1], *Qw = a.Qo[1][2];
-        const T A = (cx * (gP * ((mX * dux) * dxi))) * dt;
-        const T B = (sx * (pc + gP * (((mY * (Qv[q + n1] - Qv[q])) * dyi) + ((mZ * (Qw[q + (size_t)n1 * ny] - Qw[q])) * dzi)))) * dt;
+        const T A = (cx * (gP * ddx)) * dt;
+        const T B = (sx * (pc + gP * (fY(Qv, q, n1, sz) + ((mZ * (Qw[q + sz] - Qw[q])) * dzi)))) * dt;
         r += A - B;
     }
     // cross term of a y or z layer with an x layer (air.cpp:2091-2115 and the like):
@@ -128,42 +190,42 @@ __device__ __forceinline__ T air_p_point(const AirArgs<T> &a, int i, int j, int 
     auto xcross_y = [&](T so, T dom) {
         if (i < a.I1) {
             const T sx = a.x1.s[i], dxm = a.x1.d[i];
-            const size_t q = qxi(i, j, k, n0, ny);
-            const T t1 = (dxm * so) * ((mX * (a.Qo[0][0][q + 1] - a.Qo[0][0][q])) * dxi);
-            const T t2 = (dom * sx) * ((mY * (a.Qo[0][1][q + n0] - a.Qo[0][1][q])) * dyi);
+            const size_t q = qxi(i, j, k, n0, ny), sz = (size_t)n0 * ny;
+            const T t1 = (dxm * so) * fX(a.Qo[0][0], q, sz);
+            const T t2 = (dom * sx) * fY(a.Qo[0][1], q, n0, sz);
             r += (-((a.Qo[0][3][q] * sx) * so + gP * (t1 + t2))) * dt;
         } else if (i >= nx - a.I2) {
             const int ii = i - (nx - n1);
             const T sx = a.x2.s[i], dxm = a.x2.d[i];
-            const size_t q = qxi(ii, j, k, n1, ny);
-            const T t1 = (dxm * so) * ((mX * (a.Qo[1][0][q + 1] - a.Qo[1][0][q])) * dxi);
-            const T t2 = (dom * sx) * ((mY * (a.Qo[1][1][q + n1] - a.Qo[1][1][q])) * dyi);
+            const size_t q = qxi(ii, j, k, n1, ny), sz = (size_t)n1 * ny;
+            const T t1 = (dxm * so) * fX(a.Qo[1][0], q, sz);
+            const T t2 = (dom * sx) * fY(a.Qo[1][1], q, n1, sz);
             r += (-((a.Qo[1][3][q] * sx) * so + gP * (t1 + t2))) * dt;
         }
     };
     auto xcross_z = [&](T so, T dom) {
         if (i < a.I1) {
             const T sx = a.x1.s[i], dxm = a.x1.d[i];
-            const size_t q = qxi(i, j, k, n0, ny);
-            const T t1 = (dxm * so) * ((mX * (a.Qo[0][0][q + 1] - a.Qo[0][0][q])) * dxi);
-            const T t2 = (dom * sx) * ((mZ * (a.Qo[0][2][q + (size_t)n0 * ny] - a.Qo[0][2][q])) * dzi);
+            const size_t q = qxi(i, j, k, n0, ny), sz = (size_t)n0 * ny;
+            const T t1 = (dxm * so) * fX(a.Qo[0][0], q, sz);
+            const T t2 = (dom * sx) * ((mZ * (a.Qo[0][2][q + sz] - a.Qo[0][2][q])) * dzi);
             r += (-((a.Qo[0][3][q] * sx) * so + gP * (t1 + t2))) * dt;
         } else if (i >= nx - a.I2) {
             const int ii = i - (nx - n1);
             const T sx = a.x2.s[i], dxm = a.x2.d[i];
-            const size_t q = qxi(ii, j, k, n1, ny);
-            const T t1 = (dxm * so) * ((mX * (a.Qo[1][0][q + 1] - a.Qo[1][0][q])) * dxi);
-            const T t2 = (dom * sx) * ((mZ * (a.Qo[1][2][q + (size_t)n1 * ny] - a.Qo[1][2][q])) * dzi);
+            const size_t q = qxi(ii, j, k, n1, ny), sz = (size_t)n1 * ny;
+            const T t1 = (dxm * so) * fX(a.Qo[1][0], q, sz);
+            const T t2 = (dom * sx) * ((mZ * (a.Qo[1][2][q + sz] - a.Qo[1][2][q])) * dzi);
             r += (-((a.Qo[1][3][q] * sx) * so + gP * (t1 + t2))) * dt;
         }
     };
     // Y1 (air.cpp:2076-2116)
     if (j < a.J1) {
         const T sy = a.y1.s[j], cy = a.y1.c[j];
-        const size_t q = qyi(i, j, k, nx, n2);
+        const size_t q = qyi(i, j, k, nx, n2), sz = (size_t)nx * n2;
         const T *Qu = a.Qo[2][0], *Qw = a.Qo[2][2];
-        const T A = (cy * (gP * ((mY * dvy) * dyi))) * dt;
-        const T B = (sy * (pc + gP * (((mX * (Qu[q + 1] - Qu[q])) * dxi) + ((mZ * (Qw[q + (size_t)nx * n2] - Qw[q])) * dzi)))) * dt;
+        const T A = (cy * (gP * ddy)) * dt;
+        const T B = (sy * (pc + gP * (fX(Qu, q, sz) + ((mZ * (Qw[q + sz] - Qw[q])) * dzi)))) * dt;
         r += A - B;
         xcross_y(sy, a.y1.d[j]);
     }
@@ -171,38 +233,48 @@ __device__ __forceinline__ T air_p_point(const AirArgs<T> &a, int i, int j, int 
     if (j >= ny - a.J2) {
         const int jj = j - (ny - n3);
         const T sy = a.y2.s[j], cy = a.y2.c[j];
-        const size_t q = qyi(i, jj, k, nx, n3);
+        const size_t q = qyi(i, jj, k, nx, n3), sz = (size_t)nx * n3;
         const T *Qu = a.Qo[3][0], *Qw = a.Qo[3][2];
-        const T A = (cy * (gP * ((mY * dvy) * dyi))) * dt;
-        const T B = (sy * (pc + gP * (((mX * (Qu[q + 1] - Qu[q])) * dxi) + ((mZ * (Qw[q + (size_t)nx * n3] - Qw[q])) * dzi)))) * dt;
+        const T A = (cy * (gP * ddy)) * dt;
+        const T B = (sy * (pc + gP * (fX(Qu, q, sz) + ((mZ * (Qw[q + sz] - Qw[q])) * dzi)))) * dt;
         r += A - B;
         xcross_y(sy, a.y2.d[j]);
     }
     // y-z cross term (air.cpp:2200-2224)
-    auto ycross_z = [&](T sz, T dzm) {
+    auto ycross_z = [&](T sz_, T dzm) {
         if (j < a.J1) {
             const T sy = a.y1.s[j], dym = a.y1.d[j];
-            const size_t q = qyi(i, j, k, nx, n2);
-            const T t1 = (dym * sz) * ((mY * (a.Qo[2][1][q + nx] - a.Qo[2][1][q])) * dyi);
-            const T t2 = (dzm * sy) * ((mZ * (a.Qo[2][2][q + (size_t)nx * n2] - a.Qo[2][2][q])) * dzi);
-            r += (-((a.Qo[2][3][q] * sy) * sz + gP * (t1 + t2))) * dt;
+            const size_t q = qyi(i, j, k, nx, n2), sz = (size_t)nx * n2;
+            const T t1 = (dym * sz_) * fY(a.Qo[2][1], q, nx, sz);
+            const T t2 = (dzm * sy) * ((mZ * (a.Qo[2][2][q + sz] - a.Qo[2][2][q])) * dzi);
+            r += (-((a.Qo[2][3][q] * sy) * sz_ + gP * (t1 + t2))) * dt;
         } else if (j >= ny - a.J2) {
             const int jj = j - (ny - n3);
             const T sy = a.y2.s[j], dym = a.y2.d[j];
-            const size_t q = qyi(i, jj, k, nx, n3);
-            const T t1 = (dym * sz) * ((mY * (a.Qo[3][1][q + nx] - a.Qo[3][1][q])) * dyi);
-            const T t2 = (dzm * sy) * ((mZ * (a.Qo[3][2][q + (size_t)nx * n3] - a.Qo[3][2][q])) * dzi);
-            r += (-((a.Qo[3][3][q] * sy) * sz + gP * (t1 + t2))) * dt;
+            const size_t q = qyi(i, jj, k, nx, n3), sz = (size_t)nx * n3;
+            const T t1 = (dym * sz_) * fY(a.Qo[3][1], q, nx, sz);
+            const T t2 = (dzm * sy) * ((mZ * (a.Qo[3][2][q + sz] - a.Qo[3][2][q])) * dzi);
+            r += (-((a.Qo[3][3][q] * sy) * sz_ + gP * (t1 + t2))) * dt;
         }
+    };
+    // z layer main terms: d/dX of Qu and d/dY of Qv in the z slab (air.cpp:2160-2292)
+    auto zmain = [&](T sz_, T cz, const T *Qu, const T *Qv, size_t q) {
+        const size_t sz = sxy;
+        const T A = (((cz * a.gam) * a.Pav) * ((mZ * dwz) * dzi)) * dt;
+        T inner;
+        if constexpr (TERRAIN) {
+            inner = (((((mX * (Qu[q + 1] - Qu[q])) * dxi) + ((zX * (Qu[q + sz] - Qu[q])) * dzi)) +
+                      ((mY * (Qv[q + nx] - Qv[q])) * dyi)) + ((zY * (Qv[q + sz] - Qv[q])) * dzi));
+        } else {
+            inner = ((mX * (Qu[q + 1] - Qu[q])) * dxi) + ((mY * (Qv[q + nx] - Qv[q])) * dyi);
+        }
+        const T B = (sz_ * (pc + gP * inner)) * dt;
+        r += A - B;
     };
     // Z1 (air.cpp:2160-2225)
     if (k < a.K1) {
         const T sz = a.z1.s[k], cz = a.z1.c[k];
-        const size_t q = qzi(i, j, k, nx, ny);
-        const T *Qu = a.Qo[4][0], *Qv = a.Qo[4][1];
-        const T A = (((cz * a.gam) * a.Pav) * ((mZ * dwz) * dzi)) * dt;
-        const T B = (sz * (pc + gP * (((mX * (Qu[q + 1] - Qu[q])) * dxi) + ((mY * (Qv[q + nx] - Qv[q])) * dyi)))) * dt;
-        r += A - B;
+        zmain(sz, cz, a.Qo[4][0], a.Qo[4][1], qzi(i, j, k, nx, ny));
         xcross_z(sz, a.z1.d[k]);
         ycross_z(sz, a.z1.d[k]);
     }
@@ -210,65 +282,86 @@ __device__ __forceinline__ T air_p_point(const AirArgs<T> &a, int i, int j, int 
     if (k >= nz - a.K2) {
         const int kk = k - (nz - n5);
         const T sz = a.z2.s[k], cz = a.z2.c[k];
-        const size_t q = qzi(i, j, kk, nx, ny);
-        const T *Qu = a.Qo[5][0], *Qv = a.Qo[5][1];
-        const T A = (((cz * a.gam) * a.Pav) * ((mZ * dwz) * dzi)) * dt;
-        const T B = (sz * (pc + gP * (((mX * (Qu[q + 1] - Qu[q])) * dxi) + ((mY * (Qv[q + nx] - Qv[q])) * dyi)))) * dt;
-        r += A - B;
+        zmain(sz, cz, a.Qo[5][0], a.Qo[5][1], qzi(i, j, kk, nx, ny));
         xcross_z(sz, a.z2.d[k]);
         ycross_z(sz, a.z2.d[k]);
     }
     (void)n4;
-    const T pnew = pc + r;
+    return r;
+}
+
+// FB pressure update at one interior point: calculation_2D::Cal_pressure
+// (calculation_2D.cpp:598-608) and Update_PML_Qp (air.cpp:2388-2439).  Writes p^{n+1} and
+// the point's Q_p and returns p^{n+1}.
+template <typename T, bool WIND, bool TERRAIN = false>
+__device__ __forceinline__ T air_p_point(const AirArgs<T> &a, int i, int j, int k)
+{
+    const Grid<T> &g = a.g;
+    const size_t id = (size_t)i + (size_t)g.nx * ((size_t)j + (size_t)g.ny * k);
+    const OwnQ<T> own = own_slabs(a, i, j, k);
+    T qp_old[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) qp_old[s] = own.in[s] ? a.Qo[s][3][own.idx[s]] : T(0);
+    const T pnew = g.pn[id] + air_dp<T, WIND ? 1 : 0, TERRAIN>(a, i, j, k);
     g.pnn[id] = pnew;
     // Q_p += dt*p of this point; the value is final, face conditions only touch boundary nodes
 #pragma unroll
     for (int s = 0; s < 6; ++s)
-        if (own.in[s]) a.Qn[s][3][own.idx[s]] = qp_old[s] + dt * pnew;
+        if (own.in[s]) a.Qn[s][3][own.idx[s]] = qp_old[s] + g.dt * pnew;
     return pnew;
 }
 
 // ------------------------------------------------------------------------------------
-// velocity increments + FB add at one interior point: air::cal_fuvw (air.cpp:1476-1949)
-// with p at n+1, and calculation_2D::Cal_velocity (calculation_2D.cpp:564-574).  Reads
-// p^{n+1} and Q_p^{n+1} of the point and its backward neighbours; writes u, v, w and the
-// point's Q_u, Q_v, Q_w (Update_PML_Qvw, air.cpp:2323-2386).
+// Velocity increments dt*R_u, dt*R_v, dt*R_w at one interior point: air::cal_fuvw
+// (air.cpp:1476-1949).  Reads u, v, w of the time level in g.un ..., the pressure in g.pnn
+// and the pressure auxiliaries in a.Qn[.][3] (FB: p^{n+1}; RK4: the stage state), the
+// velocity auxiliaries in a.Qo.
 // ------------------------------------------------------------------------------------
-template <typename T, bool WIND>
-__device__ __forceinline__ void air_v_point(const AirArgs<T> &a, int i, int j, int k)
+template <typename T, int WM, bool TERRAIN>
+__device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int k, T &du_, T &dv_, T &dw_)
 {
     const Grid<T> &g = a.g;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const size_t sxy = (size_t)nx * ny;
     const size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
-    T mX, mY, mZ;
-    metric(g, i, j, k, mX, mY, mZ);
+    T mX, mY, mZ, zX = T(0), zY = T(0);
+    if constexpr (TERRAIN) metric_terrain(g, i, j, k, mX, mY, mZ, zX, zY);
+    else metric(g, i, j, k, mX, mY, mZ);
     const T dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, dt = g.dt, Aav = a.Aav;
-    const OwnQ<T> own = own_slabs(a, i, j, k);
-    T qu_old[6], qv_old[6], qw_old[6];
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        qu_old[s] = own.in[s] ? a.Qo[s][0][own.idx[s]] : T(0);
-        qv_old[s] = own.in[s] ? a.Qo[s][1][own.idx[s]] : T(0);
-        qw_old[s] = own.in[s] ? a.Qo[s][2][own.idx[s]] : T(0);
-    }
     const T *p = g.pnn;
     const T pc = p[id];
-    const T gpx = (mX * (pc - p[id - 1])) * dxi;
-    const T gpy = (mY * (pc - p[id - nx])) * dyi;
-    const T gpz = (mZ * (pc - p[id - sxy])) * dzi;
+    const T bpz = pc - p[id - sxy];
+    T gpx = (mX * (pc - p[id - 1])) * dxi;
+    T gpy = (mY * (pc - p[id - nx])) * dyi;
+    const T gpz = (mZ * bpz) * dzi;
+    if constexpr (TERRAIN) {
+        gpx = gpx + (zX * bpz) * dzi;
+        gpy = gpy + (zY * bpz) * dzi;
+    }
     const T uc = g.un[id], vc = g.vn[id], wc = g.wn[id];
     T du, dv, dw;
-    if constexpr (WIND) {
+    if constexpr (WM == 1) {
         const T Uc = a.U[id], Vc = a.V[id], Wc = a.W[id];
-        const T c1 = ((Uc * mX) * dxi) * dt, c2 = ((Vc * mY) * dyi) * dt, c3 = ((Wc * mZ) * dzi) * dt;
+        const T c1 = ((Uc * mX) * dxi) * dt, c2 = ((Vc * mY) * dyi) * dt;
+        T c3;
+        if constexpr (TERRAIN) c3 = ((((Uc * zX) + (Vc * zY)) + (Wc * mZ)) * dzi) * dt;
+        else c3 = ((Wc * mZ) * dzi) * dt;
         const T *U = a.U, *V = a.V, *W = a.W;
-        const T UX = ((mX * (U[id + 1] - U[id - 1])) * dxi) * T(0.5), UY = ((mY * (U[id + nx] - U[id - nx])) * dyi) * T(0.5),
-                UZ = ((mZ * (U[id + sxy] - U[id - sxy])) * dzi) * T(0.5);
-        const T VX = ((mX * (V[id + 1] - V[id - 1])) * dxi) * T(0.5), VY = ((mY * (V[id + nx] - V[id - nx])) * dyi) * T(0.5),
-                VZ = ((mZ * (V[id + sxy] - V[id - sxy])) * dzi) * T(0.5);
-        const T WX = ((mX * (W[id + 1] - W[id - 1])) * dxi) * T(0.5), WY = ((mY * (W[id + nx] - W[id - nx])) * dyi) * T(0.5),
-                WZ = ((mZ * (W[id + sxy] - W[id - sxy])) * dzi) * T(0.5);
+        // central gradients of the mean flow: d/dX, d/dY, d/dZ
+        auto gx = [&](const T *F) {
+            T v = ((mX * (F[id + 1] - F[id - 1])) * dxi) * T(0.5);
+            if constexpr (TERRAIN) v = v + ((zX * (F[id + sxy] - F[id - sxy])) * dzi) * T(0.5);
+            return v;
+        };
+        auto gy = [&](const T *F) {
+            T v = ((mY * (F[id + nx] - F[id - nx])) * dyi) * T(0.5);
+            if constexpr (TERRAIN) v = v + ((zY * (F[id + sxy] - F[id - sxy])) * dzi) * T(0.5);
+            return v;
+        };
+        auto gz = [&](const T *F) { return ((mZ * (F[id + sxy] - F[id - sxy])) * dzi) * T(0.5); };
+        const T UX = gx(U), UY = gy(U), UZ = gz(U);
+        const T VX = gx(V), VY = gy(V), VZ = gz(V);
+        const T WX = gx(W), WY = gy(W), WZ = gz(W);
         du = ((-c1) * (g.un[id + 1] - uc) - c2 * (g.un[id + nx] - uc)) - c3 * (g.un[id + sxy] - uc);
         du += (((-uc) * UX) * dt - (vc * UY) * dt) - (wc * UZ) * dt;
         du += ((-Aav) * dt) * gpx;
@@ -278,51 +371,85 @@ __device__ __forceinline__ void air_v_point(const AirArgs<T> &a, int i, int j, i
         dw = ((-c1) * (g.wn[id + 1] - wc) - c2 * (g.wn[id + nx] - wc)) - c3 * (g.wn[id + sxy] - wc);
         dw += (((-uc) * WX) * dt - (vc * WY) * dt) - (wc * WZ) * dt;
         dw += ((-Aav) * dt) * gpz;
+    } else if constexpr (WM == 2) {
+        // mean flow along y only (U = W = 0): the terms with U or W or their gradients vanish
+        const T Vc = a.V[id];
+        const T c2 = ((Vc * mY) * dyi) * dt;
+        const T *V = a.V;
+        T VX = ((mX * (V[id + 1] - V[id - 1])) * dxi) * T(0.5);
+        T VY = ((mY * (V[id + nx] - V[id - nx])) * dyi) * T(0.5);
+        const T VZ = ((mZ * (V[id + sxy] - V[id - sxy])) * dzi) * T(0.5);
+        du = -(c2 * (g.un[id + nx] - uc));
+        dv = -(c2 * (g.vn[id + nx] - vc));
+        dw = -(c2 * (g.wn[id + nx] - wc));
+        if constexpr (TERRAIN) {
+            const T c3 = ((Vc * zY) * dzi) * dt;
+            VX = VX + ((zX * (V[id + sxy] - V[id - sxy])) * dzi) * T(0.5);
+            VY = VY + ((zY * (V[id + sxy] - V[id - sxy])) * dzi) * T(0.5);
+            du = du - c3 * (g.un[id + sxy] - uc);
+            dv = dv - c3 * (g.vn[id + sxy] - vc);
+            dw = dw - c3 * (g.wn[id + sxy] - wc);
+        }
+        du += ((-Aav) * dt) * gpx;
+        dv += (((-uc) * VX) * dt - (vc * VY) * dt) - (wc * VZ) * dt;
+        dv += ((-Aav) * dt) * gpy;
+        dw += ((-Aav) * dt) * gpz;
     } else {
         du = ((-Aav) * dt) * gpx;
         dv = ((-Aav) * dt) * gpy;
         dw = ((-Aav) * dt) * gpz;
     }
     const int n0 = a.qn[0], n1 = a.qn[1], n2 = a.qn[2], n3 = a.qn[3], n5 = a.qn[5];
-    // backward differences of a slab field along x, y, z with the matching metric
-#define QBX(Qp, q) ((mX * ((Qp)[q] - (Qp)[(q) - 1])) * dxi)
+    // backward differences of a slab field with the metric: d/dX (x and cross-z), d/dY
+    // (y and cross-z); `sy`, `sz` are the slab strides in y and z
+    auto bX = [&](const T *Q, size_t q, size_t sz) {
+        T v = (mX * (Q[q] - Q[q - 1])) * dxi;
+        if constexpr (TERRAIN) v = v + (zX * (Q[q] - Q[q - sz])) * dzi;
+        return v;
+    };
+    auto bY = [&](const T *Q, size_t q, size_t sy, size_t sz) {
+        T v = (mY * (Q[q] - Q[q - sy])) * dyi;
+        if constexpr (TERRAIN) v = v + (zY * (Q[q] - Q[q - sz])) * dzi;
+        return v;
+    };
     // X1 / X2 (air.cpp:1579-1619)
     if (i < a.I1) {
         const T sx = a.x1.s[i], cx = a.x1.c[i];
-        const size_t q = qxi(i, j, k, n0, ny);
+        const size_t q = qxi(i, j, k, n0, ny), sz = (size_t)n0 * ny;
         const T *Qp = a.Qn[0][3];
         du += ((-sx) * uc) * dt + (cx * (Aav * gpx)) * dt;
-        dv += ((-sx) * (vc + Aav * ((mY * (Qp[q] - Qp[q - n0])) * dyi))) * dt;
-        dw += ((-sx) * (wc + Aav * ((mZ * (Qp[q] - Qp[q - (size_t)n0 * ny])) * dzi))) * dt;
+        dv += ((-sx) * (vc + Aav * bY(Qp, q, n0, sz))) * dt;
+        dw += ((-sx) * (wc + Aav * ((mZ * (Qp[q] - Qp[q - sz])) * dzi))) * dt;
     }
     if (i >= nx - a.I2) {
         const int ii = i - (nx - n1);
         const T sx = a.x2.s[i], cx = a.x2.c[i];
-        const size_t q = qxi(ii, j, k, n1, ny);
+        const size_t q = qxi(ii, j, k, n1, ny), sz = (size_t)n1 * ny;
         const T *Qp = a.Qn[1][3];
         du += ((-sx) * uc) * dt + (cx * (Aav * gpx)) * dt;
-        dv += ((-sx) * (vc + Aav * ((mY * (Qp[q] - Qp[q - n1])) * dyi))) * dt;
-        dw += ((-sx) * (wc + Aav * ((mZ * (Qp[q] - Qp[q - (size_t)n1 * ny])) * dzi))) * dt;
+        dv += ((-sx) * (vc + Aav * bY(Qp, q, n1, sz))) * dt;
+        dw += ((-sx) * (wc + Aav * ((mZ * (Qp[q] - Qp[q - sz])) * dzi))) * dt;
     }
     // y layer with x cross terms (air.cpp:1621-1728)
     auto ylayer = [&](T sy, T cy, T dym, const T *Qp, size_t qy, int nq) {
-        du += ((-sy) * (uc + Aav * QBX(Qp, qy))) * dt;
+        const size_t szy = (size_t)nx * nq;
+        du += ((-sy) * (uc + Aav * bX(Qp, qy, szy))) * dt;
         dv += ((cy * Aav) * gpy) * dt - (sy * vc) * dt;
-        dw += ((-sy) * (wc + Aav * ((mZ * (Qp[qy] - Qp[qy - (size_t)nx * nq])) * dzi))) * dt;
+        dw += ((-sy) * (wc + Aav * ((mZ * (Qp[qy] - Qp[qy - szy])) * dzi))) * dt;
         if (i < a.I1) {
             const T sx = a.x1.s[i], dxm = a.x1.d[i];
-            const size_t q = qxi(i, j, k, n0, ny);
+            const size_t q = qxi(i, j, k, n0, ny), sz = (size_t)n0 * ny;
             const T *Xp = a.Qn[0][3];
-            du += (-((a.Qo[0][0][q] * sx) * sy + ((dxm * sy) * Aav) * QBX(Xp, q))) * dt;
-            dv += (-((a.Qo[0][1][q] * sx) * sy + ((dym * sx) * Aav) * ((mY * (Xp[q] - Xp[q - n0])) * dyi))) * dt;
+            du += (-((a.Qo[0][0][q] * sx) * sy + ((dxm * sy) * Aav) * bX(Xp, q, sz))) * dt;
+            dv += (-((a.Qo[0][1][q] * sx) * sy + ((dym * sx) * Aav) * bY(Xp, q, n0, sz))) * dt;
             dw += (((-a.Qo[0][2][q]) * sx) * sy) * dt;
         } else if (i >= nx - a.I2) {
             const int ii = i - (nx - n1);
             const T sx = a.x2.s[i], dxm = a.x2.d[i];
-            const size_t q = qxi(ii, j, k, n1, ny);
+            const size_t q = qxi(ii, j, k, n1, ny), sz = (size_t)n1 * ny;
             const T *Xp = a.Qn[1][3];
-            du += (-((a.Qo[1][0][q] * sx) * sy + ((dxm * sy) * Aav) * QBX(Xp, q))) * dt;
-            dv += (-((a.Qo[1][1][q] * sx) * sy + ((dym * sx) * Aav) * ((mY * (Xp[q] - Xp[q - n1])) * dyi))) * dt;
+            du += (-((a.Qo[1][0][q] * sx) * sy + ((dxm * sy) * Aav) * bX(Xp, q, sz))) * dt;
+            dv += (-((a.Qo[1][1][q] * sx) * sy + ((dym * sx) * Aav) * bY(Xp, q, n1, sz))) * dt;
             dw += (((-a.Qo[1][2][q]) * sx) * sy) * dt;
         }
     };
@@ -330,46 +457,66 @@ __device__ __forceinline__ void air_v_point(const AirArgs<T> &a, int i, int j, i
     if (j >= ny - a.J2) ylayer(a.y2.s[j], a.y2.c[j], a.y2.d[j], a.Qn[3][3], qyi(i, j - (ny - n3), k, nx, n3), n3);
     // z layer with x and y cross terms (air.cpp:1730-1905)
     auto zlayer = [&](T sz, T cz, T dzm, const T *Qp, size_t qz) {
-        du += ((-sz) * (uc + Aav * QBX(Qp, qz))) * dt;
-        dv += ((-sz) * (vc + Aav * ((mY * (Qp[qz] - Qp[qz - nx])) * dyi))) * dt;
+        du += ((-sz) * (uc + Aav * bX(Qp, qz, sxy))) * dt;
+        dv += ((-sz) * (vc + Aav * bY(Qp, qz, nx, sxy))) * dt;
         dw += ((cz * Aav) * gpz) * dt - (sz * wc) * dt;
         if (i < a.I1) {
             const T sx = a.x1.s[i], dxm = a.x1.d[i];
-            const size_t q = qxi(i, j, k, n0, ny);
+            const size_t q = qxi(i, j, k, n0, ny), s0 = (size_t)n0 * ny;
             const T *Xp = a.Qn[0][3];
-            du += (-((a.Qo[0][0][q] * sx) * sz + ((dxm * sz) * Aav) * QBX(Xp, q))) * dt;
+            du += (-((a.Qo[0][0][q] * sx) * sz + ((dxm * sz) * Aav) * bX(Xp, q, s0))) * dt;
             dv += (((-a.Qo[0][1][q]) * sx) * sz) * dt;
-            dw += (-((a.Qo[0][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - (size_t)n0 * ny])) * dzi))) * dt;
+            dw += (-((a.Qo[0][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - s0])) * dzi))) * dt;
         } else if (i >= nx - a.I2) {
             const int ii = i - (nx - n1);
             const T sx = a.x2.s[i], dxm = a.x2.d[i];
-            const size_t q = qxi(ii, j, k, n1, ny);
+            const size_t q = qxi(ii, j, k, n1, ny), s1 = (size_t)n1 * ny;
             const T *Xp = a.Qn[1][3];
-            du += (-((a.Qo[1][0][q] * sx) * sz + ((dxm * sz) * Aav) * QBX(Xp, q))) * dt;
+            du += (-((a.Qo[1][0][q] * sx) * sz + ((dxm * sz) * Aav) * bX(Xp, q, s1))) * dt;
             dv += (((-a.Qo[1][1][q]) * sx) * sz) * dt;
-            dw += (-((a.Qo[1][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - (size_t)n1 * ny])) * dzi))) * dt;
+            dw += (-((a.Qo[1][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - s1])) * dzi))) * dt;
         }
         if (j < a.J1) {
             const T sy = a.y1.s[j], dym = a.y1.d[j];
-            const size_t q = qyi(i, j, k, nx, n2);
+            const size_t q = qyi(i, j, k, nx, n2), s2 = (size_t)nx * n2;
             const T *Yp = a.Qn[2][3];
             du += (((-a.Qo[2][0][q]) * sy) * sz) * dt;
-            dv += (-((a.Qo[2][1][q] * sy) * sz + ((dym * sz) * Aav) * ((mY * (Yp[q] - Yp[q - nx])) * dyi))) * dt;
-            dw += (-((a.Qo[2][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - (size_t)nx * n2])) * dzi))) * dt;
+            dv += (-((a.Qo[2][1][q] * sy) * sz + ((dym * sz) * Aav) * bY(Yp, q, nx, s2))) * dt;
+            dw += (-((a.Qo[2][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - s2])) * dzi))) * dt;
         } else if (j >= ny - a.J2) {
             const int jj = j - (ny - n3);
             const T sy = a.y2.s[j], dym = a.y2.d[j];
-            const size_t q = qyi(i, jj, k, nx, n3);
+            const size_t q = qyi(i, jj, k, nx, n3), s3 = (size_t)nx * n3;
             const T *Yp = a.Qn[3][3];
             du += (((-a.Qo[3][0][q]) * sy) * sz) * dt;
-            dv += (-((a.Qo[3][1][q] * sy) * sz + ((dym * sz) * Aav) * ((mY * (Yp[q] - Yp[q - nx])) * dyi))) * dt;
-            dw += (-((a.Qo[3][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - (size_t)nx * n3])) * dzi))) * dt;
+            dv += (-((a.Qo[3][1][q] * sy) * sz + ((dym * sz) * Aav) * bY(Yp, q, nx, s3))) * dt;
+            dw += (-((a.Qo[3][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - s3])) * dzi))) * dt;
         }
     };
     if (k < a.K1) zlayer(a.z1.s[k], a.z1.c[k], a.z1.d[k], a.Qn[4][3], qzi(i, j, k, nx, ny));
     if (k >= nz - a.K2) zlayer(a.z2.s[k], a.z2.c[k], a.z2.d[k], a.Qn[5][3], qzi(i, j, k - (nz - n5), nx, ny));
-#undef QBX
-    const T un = uc + du, vn = vc + dv, wn = wc + dw;
+    du_ = du; dv_ = dv; dw_ = dw;
+}
+
+// FB velocity update at one interior point with p at n+1: calculation_2D::Cal_velocity
+// (calculation_2D.cpp:564-574) and Update_PML_Qvw (air.cpp:2323-2386).
+template <typename T, bool WIND, bool TERRAIN = false>
+__device__ __forceinline__ void air_v_point(const AirArgs<T> &a, int i, int j, int k)
+{
+    const Grid<T> &g = a.g;
+    const size_t id = (size_t)i + (size_t)g.nx * ((size_t)j + (size_t)g.ny * k);
+    const OwnQ<T> own = own_slabs(a, i, j, k);
+    T qu_old[6], qv_old[6], qw_old[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        qu_old[s] = own.in[s] ? a.Qo[s][0][own.idx[s]] : T(0);
+        qv_old[s] = own.in[s] ? a.Qo[s][1][own.idx[s]] : T(0);
+        qw_old[s] = own.in[s] ? a.Qo[s][2][own.idx[s]] : T(0);
+    }
+    T du, dv, dw;
+    air_duvw<T, WIND ? 1 : 0, TERRAIN>(a, i, j, k, du, dv, dw);
+    const T dt = g.dt;
+    const T un = g.un[id] + du, vn = g.vn[id] + dv, wn = g.wn[id] + dw;
     g.unn[id] = un;
     g.vnn[id] = vn;
     g.wnn[id] = wn;
@@ -384,20 +531,20 @@ __device__ __forceinline__ void air_v_point(const AirArgs<T> &a, int i, int j, i
 }
 
 // The two phases as separate passes, one thread per interior point.
-template <typename T, bool WIND>
+template <typename T, bool WIND, bool TERRAIN>
 __global__ __launch_bounds__(BX * BY) void k_air_p(const AirArgs<T> a)
 {
     const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
     if (i >= a.g.nx - 1 || j >= a.g.ny - 1) return;
-    (void)air_p_point<T, WIND>(a, i, j, k);
+    (void)air_p_point<T, WIND, TERRAIN>(a, i, j, k);
 }
 
-template <typename T, bool WIND>
+template <typename T, bool WIND, bool TERRAIN>
 __global__ __launch_bounds__(BX * BY) void k_air_v(const AirArgs<T> a)
 {
     const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
     if (i >= a.g.nx - 1 || j >= a.g.ny - 1) return;
-    air_v_point<T, WIND>(a, i, j, k);
+    air_v_point<T, WIND, TERRAIN>(a, i, j, k);
 }
 
 // ------------------------------------------------------------------------------------
@@ -500,9 +647,128 @@ __global__ __launch_bounds__(64 * (FB_TY + 1)) void k_air_fb(const AirArgs<T> a,
 }
 
 // ------------------------------------------------------------------------------------
+// RK4 (extension).  One kernel per stage and domain: the increment k = dt*R(stage state) at
+// every interior point, then the register update
+//   stage 0:   acc = x_n + b0 k,          dst = x_n + a0 k
+//   stage 1,2: acc = acc + b_s k,         dst = x_n + a_s k
+//   stage 3:                              dst = acc + b3 k   (= x_{n+1})
+// with (a) = (1/2, 1/2, 1), (b) = (1/6, 1/3, 1/3, 1/6).  The PML auxiliaries integrate
+// dQ/dt = f of the stage state with the same registers.  With COMP (FP32) acc carries a low
+// part through the stage sums (TwoSum) and the stored state keeps its rounding error `err`
+// into the next step (FastTwoSum), so the accumulation does not lose FP32 ulps every step.
+// ------------------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ T rk_a(int st) { return st == 0 ? T(0.5) : st == 1 ? T(0.5) : T(1.0); }
+template <typename T> __device__ __forceinline__ T rk_b(int st)
+{
+    return (st == 0 || st == 3) ? T(1.0 / 6.0) : T(1.0 / 3.0);
+}
+
+template <typename T, bool COMP>
+__device__ __forceinline__ T rk_update(int st, T xn, T k, T *acc, T *lo, T *err, size_t id)
+{
+    if constexpr (!COMP) {
+        if (st == 0) {
+            acc[id] = xn + rk_b<T>(0) * k;
+            return xn + rk_a<T>(0) * k;
+        }
+        if (st < 3) {
+            acc[id] = acc[id] + rk_b<T>(st) * k;
+            return xn + rk_a<T>(st) * k;
+        }
+        return acc[id] + rk_b<T>(3) * k;
+    } else {
+        const T y = rk_b<T>(st) * k;
+        const T a0 = st == 0 ? xn : acc[id];
+        const T sm = a0 + y;
+        const T bb = sm - a0;
+        const T e = (a0 - (sm - bb)) + (y - bb);  // TwoSum: a0 + y = sm + e exactly
+        const T l = (st == 0 ? err[id] : lo[id]) + e;
+        if (st < 3) {
+            acc[id] = sm;
+            lo[id] = l;
+            return xn + rk_a<T>(st) * k;
+        }
+        const T x = sm + l;
+        err[id] = l - (x - sm);  // FastTwoSum (|sm| >= |l|): sm + l = x + err
+        return x;
+    }
+}
+
+template <typename T, int WM, bool TERRAIN, bool COMP>
+__global__ __launch_bounds__(BX * BY) void k_rk_air(const RKArgs<T> r)
+{
+    const AirArgs<T> &a = r.a;
+    const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    if (i >= a.g.nx - 1 || j >= a.g.ny - 1) return;
+    const size_t id = (size_t)i + (size_t)a.g.nx * ((size_t)j + (size_t)a.g.ny * k);
+    const int st = r.stage;
+    T kf[4];
+    kf[3] = air_dp<T, WM, TERRAIN>(a, i, j, k);
+    air_duvw<T, WM, TERRAIN>(a, i, j, k, kf[0], kf[1], kf[2]);
+    const T xc[4] = {a.g.un[id], a.g.vn[id], a.g.wn[id], a.g.pn[id]};  // stage state at the point
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        const T xn = st == 0 ? xc[f] : r.xn[f][id];
+        r.dst[f][id] = rk_update<T, COMP>(st, xn, kf[f], r.acc[f], r.lo[f], r.err[f], id);
+    }
+    // auxiliaries of the point's own slabs: dQ/dt = f (air.cpp:2323-2439 as an ODE)
+    const OwnQ<T> own = own_slabs(a, i, j, k);
+    const T dt = a.g.dt;
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+        if (own.in[s]) {
+            const size_t q = own.idx[s];
+#pragma unroll
+            for (int f = 0; f < 4; ++f)
+                r.qdst[s][f][q] = rk_update<T, false>(st, r.qxn[s][f][q], dt * xc[f], r.qacc[s][f], nullptr, nullptr, q);
+        }
+}
+
+// Porous layer: dt*R_p = -gama div u (porous.cpp:77-95); dt*R_u = -dt eff grad p - beta u,
+// the drag of porous.cpp:112-135 as the ODE du/dt = -(1/rho_e) grad p - (sigma/rho_e) u.
+template <typename T, bool TERRAIN, bool COMP>
+__global__ __launch_bounds__(BX * BY) void k_rk_pore(const RKPoreArgs<T> r)
+{
+    const Grid<T> &g = r.g;
+    const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    const int nx = g.nx, ny = g.ny;
+    if (i >= nx - 1 || j >= ny - 1) return;
+    const size_t sxy = (size_t)nx * ny;
+    const size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+    T mX, mY, mZ, zX = T(0), zY = T(0);
+    if constexpr (TERRAIN) metric_terrain(g, i, j, k, mX, mY, mZ, zX, zY);
+    else metric(g, i, j, k, mX, mY, mZ);
+    const T dux = g.un[id + 1] - g.un[id], dvy = g.vn[id + nx] - g.vn[id], dwz = g.wn[id + sxy] - g.wn[id];
+    T div;
+    if constexpr (TERRAIN) {
+        const T duz = g.un[id + sxy] - g.un[id], dvz = g.vn[id + sxy] - g.vn[id];
+        div = (((((mX * g.dxi) * dux) + ((zX * g.dzi) * duz)) + ((mY * g.dyi) * dvy)) + ((zY * g.dzi) * dvz)) +
+              ((mZ * g.dzi) * dwz);
+    } else {
+        div = (((mX * g.dxi) * dux) + ((mY * g.dyi) * dvy)) + ((mZ * g.dzi) * dwz);
+    }
+    const T pc = g.pn[id], bz = pc - g.pn[id - sxy];
+    T gx = (mX * (pc - g.pn[id - 1])) * g.dxi, gy = (mY * (pc - g.pn[id - nx])) * g.dyi;
+    const T gz = (mZ * bz) * g.dzi;
+    if constexpr (TERRAIN) {
+        gx = gx + (zX * bz) * g.dzi;
+        gy = gy + (zY * bz) * g.dzi;
+    }
+    const T f = (-g.dt) * r.eff;
+    const T xc[4] = {g.un[id], g.vn[id], g.wn[id], pc};
+    const T kf[4] = {f * gx - r.beta * xc[0], f * gy - r.beta * xc[1], f * gz - r.beta * xc[2], (-r.gama) * div};
+    const int st = r.stage;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const T xn = st == 0 ? xc[q] : r.xn[q][id];
+        r.dst[q][id] = rk_update<T, COMP>(st, xn, kf[q], r.acc[q], r.lo[q], r.err[q], id);
+    }
+}
+
+// ------------------------------------------------------------------------------------
 // porous ground (porous.cpp:77-143) + FB add
 // ------------------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool TERRAIN>
 __global__ __launch_bounds__(BX * BY) void k_pore_p(const PoreArgs<T> a)
 {
     const Grid<T> &g = a.g;
@@ -511,14 +777,22 @@ __global__ __launch_bounds__(BX * BY) void k_pore_p(const PoreArgs<T> a)
     if (i >= nx - 1 || j >= ny - 1) return;
     const size_t sxy = (size_t)nx * ny;
     const size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
-    T mX, mY, mZ;
-    metric(g, i, j, k, mX, mY, mZ);
-    const T r = (-a.gama) * ((((mX * g.dxi) * (g.un[id + 1] - g.un[id])) + ((mY * g.dyi) * (g.vn[id + nx] - g.vn[id]))) +
-                             ((mZ * g.dzi) * (g.wn[id + sxy] - g.wn[id])));
+    T mX, mY, mZ, zX = T(0), zY = T(0);
+    if constexpr (TERRAIN) metric_terrain(g, i, j, k, mX, mY, mZ, zX, zY);
+    else metric(g, i, j, k, mX, mY, mZ);
+    T r;
+    if constexpr (TERRAIN) {
+        r = (-a.gama) * (((((mX * g.dxi) * (g.un[id + 1] - g.un[id])) + ((zX * g.dzi) * (g.un[id + sxy] - g.un[id]))) +
+                          ((mY * g.dyi) * (g.vn[id + nx] - g.vn[id]))) + ((zY * g.dzi) * (g.vn[id + sxy] - g.vn[id])) +
+                         ((mZ * g.dzi) * (g.wn[id + sxy] - g.wn[id])));
+    } else {
+        r = (-a.gama) * ((((mX * g.dxi) * (g.un[id + 1] - g.un[id])) + ((mY * g.dyi) * (g.vn[id + nx] - g.vn[id]))) +
+                         ((mZ * g.dzi) * (g.wn[id + sxy] - g.wn[id])));
+    }
     g.pnn[id] = g.pn[id] + r;
 }
 
-template <typename T>
+template <typename T, bool TERRAIN>
 __global__ __launch_bounds__(BX * BY) void k_pore_v(const PoreArgs<T> a)
 {
     const Grid<T> &g = a.g;
@@ -527,15 +801,22 @@ __global__ __launch_bounds__(BX * BY) void k_pore_v(const PoreArgs<T> a)
     if (i >= nx - 1 || j >= ny - 1) return;
     const size_t sxy = (size_t)nx * ny;
     const size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
-    T mX, mY, mZ;
-    metric(g, i, j, k, mX, mY, mZ);
+    T mX, mY, mZ, zX = T(0), zY = T(0);
+    if constexpr (TERRAIN) metric_terrain(g, i, j, k, mX, mY, mZ, zX, zY);
+    else metric(g, i, j, k, mX, mY, mZ);
     const T *p = g.pnn;
     const T pc = p[id];
     const T f = (((T(-1.0) * a.invbeta) * g.dt) * a.eff);
     const T uc = g.un[id], vc = g.vn[id], wc = g.wn[id];
-    const T du = (f * ((mX * (pc - p[id - 1])) * g.dxi) + uc * a.invbeta) - uc;
-    const T dv = (f * ((mY * (pc - p[id - nx])) * g.dyi) + vc * a.invbeta) - vc;
-    const T dw = (f * ((mZ * (pc - p[id - sxy])) * g.dzi) + wc * a.invbeta) - wc;
+    const T bz = pc - p[id - sxy];
+    T gx = (mX * (pc - p[id - 1])) * g.dxi, gy = (mY * (pc - p[id - nx])) * g.dyi;
+    if constexpr (TERRAIN) {
+        gx = gx + (zX * bz) * g.dzi;
+        gy = gy + (zY * bz) * g.dzi;
+    }
+    const T du = (f * gx + uc * a.invbeta) - uc;
+    const T dv = (f * gy + vc * a.invbeta) - vc;
+    const T dw = (f * ((mZ * bz) * g.dzi) + wc * a.invbeta) - wc;
     g.unn[id] = uc + du;
     g.vnn[id] = vc + dv;
     g.wnn[id] = wc + dw;
@@ -607,15 +888,48 @@ unsigned flat_blocks(size_t total)
 template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s)
 {
     dim3 g = interior_grid(a.g.nx, a.g.ny, a.g.nz), b(BX, BY);
-    if (wind) hipLaunchKernelGGL((k_air_p<T, true>), g, b, 0, s, a);
-    else hipLaunchKernelGGL((k_air_p<T, false>), g, b, 0, s, a);
+    const bool ter = a.g.tsh != nullptr;
+    if (ter) {
+        if (wind) hipLaunchKernelGGL((k_air_p<T, true, true>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_air_p<T, false, true>), g, b, 0, s, a);
+    } else {
+        if (wind) hipLaunchKernelGGL((k_air_p<T, true, false>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_air_p<T, false, false>), g, b, 0, s, a);
+    }
 }
 template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStream_t s)
 {
     dim3 g = interior_grid(a.g.nx, a.g.ny, a.g.nz), b(BX, BY);
-    if (wind) hipLaunchKernelGGL((k_air_v<T, true>), g, b, 0, s, a);
-    else hipLaunchKernelGGL((k_air_v<T, false>), g, b, 0, s, a);
+    const bool ter = a.g.tsh != nullptr;
+    if (ter) {
+        if (wind) hipLaunchKernelGGL((k_air_v<T, true, true>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_air_v<T, false, true>), g, b, 0, s, a);
+    } else {
+        if (wind) hipLaunchKernelGGL((k_air_v<T, true, false>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_air_v<T, false, false>), g, b, 0, s, a);
+    }
 }
+template <typename T> void launch_rk_air(const RKArgs<T> &r, int wind, bool terrain, hipStream_t s)
+{
+    constexpr bool C = std::is_same<T, float>::value;
+    dim3 g = interior_grid(r.a.g.nx, r.a.g.ny, r.a.g.nz), b(BX, BY);
+#define LEE_RK(WM, TER) hipLaunchKernelGGL((k_rk_air<T, WM, TER, C>), g, b, 0, s, r)
+    if (terrain) {
+        if (wind == 1) LEE_RK(1, true); else if (wind == 2) LEE_RK(2, true); else LEE_RK(0, true);
+    } else {
+        if (wind == 1) LEE_RK(1, false); else if (wind == 2) LEE_RK(2, false); else LEE_RK(0, false);
+    }
+#undef LEE_RK
+}
+
+template <typename T> void launch_rk_pore(const RKPoreArgs<T> &r, bool terrain, hipStream_t s)
+{
+    constexpr bool C = std::is_same<T, float>::value;
+    dim3 g = interior_grid(r.g.nx, r.g.ny, r.g.nz), b(BX, BY);
+    if (terrain) hipLaunchKernelGGL((k_rk_pore<T, true, C>), g, b, 0, s, r);
+    else hipLaunchKernelGGL((k_rk_pore<T, false, C>), g, b, 0, s, r);
+}
+
 template <typename T> bool fused_fits(int nx) { return nx >= 3; }
 
 template <typename T> void launch_air_fb(const AirArgs<T> &a, bool wind, hipStream_t s)
@@ -640,11 +954,13 @@ template <typename T> void launch_air_fb(const AirArgs<T> &a, bool wind, hipStre
 
 template <typename T> void launch_pore_p(const PoreArgs<T> &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_pore_p<T>), interior_grid(a.g.nx, a.g.ny, a.g.nz), dim3(BX, BY), 0, s, a);
+    if (a.g.tsh) hipLaunchKernelGGL((k_pore_p<T, true>), interior_grid(a.g.nx, a.g.ny, a.g.nz), dim3(BX, BY), 0, s, a);
+    else hipLaunchKernelGGL((k_pore_p<T, false>), interior_grid(a.g.nx, a.g.ny, a.g.nz), dim3(BX, BY), 0, s, a);
 }
 template <typename T> void launch_pore_v(const PoreArgs<T> &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_pore_v<T>), interior_grid(a.g.nx, a.g.ny, a.g.nz), dim3(BX, BY), 0, s, a);
+    if (a.g.tsh) hipLaunchKernelGGL((k_pore_v<T, true>), interior_grid(a.g.nx, a.g.ny, a.g.nz), dim3(BX, BY), 0, s, a);
+    else hipLaunchKernelGGL((k_pore_v<T, false>), interior_grid(a.g.nx, a.g.ny, a.g.nz), dim3(BX, BY), 0, s, a);
 }
 template <typename T> void launch_shift(T *dst, const T *src, int A, int B, int C, int tempJ, int lead, int sh, hipStream_t st)
 {
@@ -673,6 +989,8 @@ void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s)
     template void launch_air_p<T>(const AirArgs<T> &, bool, hipStream_t);                                    \
     template void launch_air_v<T>(const AirArgs<T> &, bool, hipStream_t);                                    \
     template bool fused_fits<T>(int);                                                                        \
+    template void launch_rk_air<T>(const RKArgs<T> &, int, bool, hipStream_t);                               \
+    template void launch_rk_pore<T>(const RKPoreArgs<T> &, bool, hipStream_t);                               \
     template void launch_air_fb<T>(const AirArgs<T> &, bool, hipStream_t);                                   \
     template void launch_pore_p<T>(const PoreArgs<T> &, hipStream_t);                                        \
     template void launch_pore_v<T>(const PoreArgs<T> &, hipStream_t);                                        \

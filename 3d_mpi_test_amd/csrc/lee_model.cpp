@@ -87,10 +87,19 @@ std::string Model::build(const lee_deck &deck, const lee_options &o, int &code)
 {
     d = deck;
     opt = o;
+    code = LEE_ERR_ARG;
+    if (o.integrator != LEE_INT_DECK && o.integrator != LEE_INT_RK4) return "unknown integrator";
+    if (o.n_hills < 0 || o.n_hills > LEE_MAX_HILLS) return "n_hills must be 0.." + std::to_string(LEE_MAX_HILLS);
+    for (int n = 0; n < o.n_hills; ++n)
+        if (!(o.hills[4 * n + 3] > 0)) return "hill sigma must be positive";
+    if (d.velocity_method == 7 && !(o.wind_z0 > 0)) return "velocity_method 7 needs wind_z0 > 0";
+    rk4 = o.integrator == LEE_INT_RK4;
+    terrain = o.n_hills > 0;
     code = LEE_ERR_UNSUPPORTED;
-    if (d.scheme != 2)
+    if (!rk4 && d.scheme != 2)
         return "scheme " + std::to_string(d.scheme) + " is not implemented by the reference "
                "(only FB_p_v, scheme 2, updates the fields: calculation_2D.cpp:551-612)";
+    if (rk4 && d.bc[LEE_FACE_E] == LEE_BC_TDBC) return "the East TDBC is defined for the FB_p_v step only";
     int nb = d.bc[LEE_FACE_N];
     if (nb != LEE_BC_RIGID && nb != LEE_BC_ABSORBING)
         return "North boundary must be rigid or absorbing (other types skip every BC: airpore.cpp:160-169)";
@@ -105,6 +114,7 @@ std::string Model::build(const lee_deck &deck, const lee_options &o, int &code)
     if (d.restart != 0) return "restart is not supported";
     if (d.velocity_method == 3 || d.velocity_method == 6)
         return "wind methods 3 and 6 read uninitialised members in the reference (air.cpp:44-69)";
+    if (d.velocity_method > 7) return "unknown wind method";
     code = LEE_ERR_ARG;
     if (d.mf_lead_dj < 2) return "lead_DJ must be >= 2 (airpore.cpp:542 divides by lead_DJ-1)";
     if (d.mf_judge == 1 && d.mf_trail_dj > d.mf_lead_dj)
@@ -136,8 +146,10 @@ std::string Model::build(const lee_deck &deck, const lee_options &o, int &code)
     gamma = d.gamma; Pav = d.Pav;
     Aav = std::pow(d.c, 2) / d.gamma / d.Pav;  // airpore.cpp:177
     gP = gamma * Pav;
-    has_wind = !(d.velocity_method == 4 || d.velocity_method < 1 || d.velocity_method > 6);
+    has_wind = !(d.velocity_method == 4 || d.velocity_method < 1 || d.velocity_method > 7);
     if (d.velocity_method == 1 && d.velocity_coef == 0.0) has_wind = false;
+    if (d.velocity_method == 7 && o.wind_ustar == 0.0) has_wind = false;
+    wind_v_only = d.velocity_method == 7;
 
     // ---- PML (air.cpp:509-937), one domain: mpi_i1 = 0, mpi_i2 = nx-1
     I1 = h[1]; I2 = h[2]; J1 = h[4]; J2 = h[5]; J3 = h[4]; K1 = h[7]; K2 = h[8];
@@ -204,6 +216,10 @@ std::string Model::build(const lee_deck &deck, const lee_options &o, int &code)
         p_beta = d.pore_resistivity * d.dt * p_eff;
         p_gama = d.dt / Kp / d.pore_porosity;
         p_invbeta = 1.0 / (p_beta + 1.0);
+        if (rk4 && !(p_beta < 2.5)) {
+            code = LEE_ERR_UNSUPPORTED;
+            return "explicit RK4 porous drag needs sigma*dt/rho_e < 2.5 (got " + std::to_string(p_beta) + ")";
+        }
         // (int)(dz_air/dz_pore) < 5 selects the plane copy; the other branch is commented
         // out in the reference, which then couples nothing (calculation_2D.cpp:872-877).
         iface_copy = (int)(d.dz / d.pore_dz) < 5.0;
@@ -308,6 +324,42 @@ long long Model::receiver_index(int r, int N) const
     return -1;
 }
 
+double Model::hill_height(double x, double y) const
+{
+    double h = 0.0;
+    for (int n = 0; n < opt.n_hills; ++n) {
+        const double *H = &opt.hills[4 * n];
+        double dx = x - H[0], dy = y - H[1];
+        h += H[2] * std::exp(-(dx * dx + dy * dy) / (2.0 * H[3] * H[3]));
+    }
+    return h;
+}
+
+// Terrain tables of one grid for the frame coordinates X, Y (heights in the air shrink to 0
+// at the flat top; the porous layer is shifted rigidly).
+void Model::terrain_tables(const std::vector<double> &X, const std::vector<double> &Y, const std::vector<double> &Z,
+                           double dxi_, double dyi_, bool pore_layer, TerrainTables &t) const
+{
+    const int a = (int)X.size(), b = (int)Y.size(), c = (int)Z.size();
+    const double top = Z[c - 1], D = Z[c - 1] - Z[0];
+    t.h.assign((size_t)a * b, 0.0); t.sh.assign((size_t)a * b, 1.0);
+    t.hx.assign((size_t)a * b, 0.0); t.hy.assign((size_t)a * b, 0.0);
+    t.tk.assign(c, 1.0);
+    for (int j = 0; j < b; ++j)
+        for (int i = 0; i < a; ++i) {
+            size_t q = (size_t)i + (size_t)a * j;
+            t.h[q] = hill_height(X[i], Y[j]);
+            t.sh[q] = pore_layer ? 1.0 : 1.0 - t.h[q] / D;
+        }
+    for (int j = 0; j < b; ++j)
+        for (int i = 0; i < a; ++i) {
+            size_t q = (size_t)i + (size_t)a * j;
+            t.hx[q] = (i > 0 && i < a - 1) ? (t.h[q + 1] - t.h[q - 1]) * dxi_ * 0.5 : 0.0;
+            t.hy[q] = (j > 0 && j < b - 1) ? (t.h[q + a] - t.h[q - a]) * dyi_ * 0.5 : 0.0;
+        }
+    for (int k = 0; k < c; ++k) t.tk[k] = pore_layer ? 1.0 : (top - Z[k]) / D;
+}
+
 void Model::frame_tables(int N, FrameTables &t) const
 {
     // coordinates and metrics (calculation_2D.cpp:410-419)
@@ -324,7 +376,10 @@ void Model::frame_tables(int N, FrameTables &t) const
         metrics1d(pax, pdxi, t.pXx);
         metrics1d(PY, pdyi, t.pYy);
         metrics1d(paz, pdzi, t.pZz);
+        if (terrain) terrain_tables(pax, PY, paz, pdxi, pdyi, true, t.pter);
     }
+    t.ter = TerrainTables();
+    if (terrain) terrain_tables(ax, Y, az, dxi, dyi, false, t.ter);
     // PML profiles; PML_y1/y2 follow the frame (air.cpp:1289-1292)
     double y1a = py1, y2a = py2;
     if (N != 0) {
@@ -351,6 +406,17 @@ void Model::frame_tables(int N, FrameTables &t) const
             for (int j = 0; j < ny; ++j)
                 for (int i = 0; i < nx; ++i) {
                     size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+                    if (m == 7) {
+                        // logarithmic profile along +y in the height above the local ground
+                        size_t q = (size_t)i + (size_t)nx * j;
+                        double zk = terrain ? az[k] + t.ter.h[q] * t.ter.tk[k] : az[k];
+                        double z0 = terrain ? az[0] + t.ter.h[q] * t.ter.tk[0] : az[0];
+                        double zg = zk - z0;
+                        t.U[id] = 0.0;
+                        t.V[id] = zg > opt.wind_z0 ? opt.wind_ustar / 0.41 * std::log(zg / opt.wind_z0) : 0.0;
+                        t.W[id] = 0.0;
+                        continue;
+                    }
                     double z = az[k] - 0.0;
                     t.U[id] = windU(m, b, ax[i], Y[j], z);
                     t.V[id] = windV(m, b, ax[i], Y[j], z);
@@ -359,17 +425,22 @@ void Model::frame_tables(int N, FrameTables &t) const
     }
 }
 
-// calculation_2D::Set_InitialCond (calculation_2D.cpp:292-314): Gaussian pulse.
+// calculation_2D::Set_InitialCond (calculation_2D.cpp:292-314): Gaussian pulse.  Over
+// terrain the source height is above the ground under the source and distances are physical.
 void Model::initial_condition(std::vector<double> &p, bool porous) const
 {
     const std::vector<double> &X = porous ? pax : ax, &Y = porous ? pay : ay, &Z = porous ? paz : az;
     int a = (int)X.size(), b = (int)Y.size(), c = (int)Z.size();
     p.assign((size_t)a * b * c, 0.0);
     double gc = 4.0 * std::log(2.0) / std::pow(d.gauss_width, 2);
+    TerrainTables ter;
+    if (terrain) terrain_tables(X, Y, Z, porous ? pdxi : dxi, porous ? pdyi : dyi, porous, ter);
+    const double zs = d.src[2] + (terrain ? hill_height(d.src[0], d.src[1]) : 0.0);
     for (int k = 0; k < c; ++k)
         for (int j = 0; j < b; ++j)
             for (int i = 0; i < a; ++i) {
-                double x = X[i] - d.src[0], y = Y[j] - d.src[1], z = Z[k] - d.src[2];
+                double zk = terrain ? Z[k] + ter.h[(size_t)i + (size_t)a * j] * ter.tk[k] : Z[k];
+                double x = X[i] - d.src[0], y = Y[j] - d.src[1], z = zk - zs;
                 double r = std::sqrt(std::pow(x, 2) + std::pow(y, 2) + std::pow(z, 2));
                 p[(size_t)i + (size_t)a * ((size_t)j + (size_t)b * k)] = 1.0 * std::exp(-gc * r * r);
             }

@@ -34,11 +34,19 @@ struct Profile {
     std::vector<double> s, c, d;
 };
 
+// Terrain-following grid (extension): physical height Z(i,j,k) = Z[k] + h(i,j)*tk[k], with
+// metric factors Z_over_z = Zz[k]*sh(i,j), Z_over_x = hx(i,j)*tk[k], Z_over_y = hy(i,j)*tk[k].
+struct TerrainTables {
+    std::vector<double> h, sh, hx, hy;    // [i + nx*j]
+    std::vector<double> tk;               // [k]
+};
+
 struct FrameTables {
     std::vector<double> Xx, Yy, Zz;       // air metrics X_over_x ... (calculation_2D.cpp:417-419)
     std::vector<double> pXx, pYy, pZz;    // porous metrics
     Profile x1, x2, y1, y2, z1, z2;       // PML profiles for this frame
     std::vector<double> U, V, W;          // wind (empty when zero everywhere)
+    TerrainTables ter, pter;              // air and porous terrain tables (empty when flat)
 };
 
 struct Model {
@@ -53,6 +61,8 @@ struct Model {
     double pdxi = 0, pdyi = 0, pdzi = 0;
     double gamma = 0, Pav = 0, Aav = 0, gP = 0;
     bool south_porous = false, east_tdbc = false, iface_copy = true, has_wind = false;
+    // extensions: RK4, terrain-following grid, wind along y only (velocity_method 7)
+    bool rk4 = false, terrain = false, wind_v_only = false;
     // PML (air.cpp:509-937)
     int judge[3] = {0, 0, 0};
     int I1 = 0, I2 = 0, J1 = 0, J2 = 0, J3 = 0, K1 = 0, K2 = 0;
@@ -76,6 +86,10 @@ struct Model {
     // LEE_ERR_UNSUPPORTED (a path the reference never implemented) or LEE_ERR_ARG.
     std::string build(const lee_deck &deck, const lee_options &o, int &code);
     void frame_tables(int N, FrameTables &t) const;
+    // terrain height of the hills at world (x, y)
+    double hill_height(double x, double y) const;
+    void terrain_tables(const std::vector<double> &X, const std::vector<double> &Y, const std::vector<double> &Z,
+                        double dxi_, double dyi_, bool pore_layer, TerrainTables &t) const;
     void initial_condition(std::vector<double> &p, bool porous) const;
     int last_frame() const;
     int frame_count() const;
