@@ -343,16 +343,19 @@ def box(nx: int, ny: int, nz: int, d: float = 0.05, *, pml: int = 0, bc=None, dt
 
 # ---------------------------------------------------------------------- benchmark workloads
 
-def config_c0(time_domain: int = 2000) -> Deck:
+def config_c0(time_domain: int = 2000, reference_scheme: bool = False) -> Deck:
     """96x64x48 air points at 0.05 m over an 8-point Zwikker-Kosten ground (sigma 200
     kPa s/m^2, porosity 0.3, tortuosity 3), c0 = 343 m/s, rho0 = 1.21 kg/m^3, no wind,
     Gaussian pulse of half-width 3 dx at 1.0 m, 20-point PML on the sides and top, CFL 0.5,
-    five receivers at 1.5 m height 1-4 m from the source (the top layer starts at 1.4 m, so
-    the receivers sit in it: run as specified)."""
+    RK4, five receivers at 1.5 m height 1-4 m from the source (the top layer starts at 1.4 m,
+    so the receivers sit in it: run as specified).  reference_scheme=True keeps the deck's
+    FB_p_v step, the only integrator of the reference."""
     d = 0.05
     dk = box(96, 64, 48, d, pml=20, bc=(2, 3, 2, 2, 2, 2), porous_cells=8 + 2, y0=-0.75, time_domain=time_domain,
              gauss_width=2 * 3 * d, src=(0.0, 0.0, 1.0), rcv=(0.0, 1.0, 1.5), fft_m=12, pml_y_time=0)
     dk.pore_q, dk.pore_porosity, dk.pore_resistivity = 3.0 ** 0.5, 0.3, 2.0e5
+    if not reference_scheme:
+        dk.integrator = INT_RK4
     return dk
 
 
@@ -360,11 +363,15 @@ def receivers_c0():
     return [(0.0, float(r), 1.5) for r in (1.0, 1.75, 2.5, 3.25, 4.0)]
 
 
-def config_c1(time_domain: int = 10000) -> Deck:
+def config_c1(time_domain: int = 10000, reference_scheme: bool = False) -> Deck:
     """512x256x128 air points at 0.04 m (x lateral, y propagation, z up) over a 16-point
     porous layer with the C0 ground, 20-point PML on the sides and top, moving frame on
     (frame = the 256 y-planes, shifted by 32 planes; the first frame runs half a frame
-    transit), Gaussian pulse of half-width 3 dx at 0.5 m, CFL 0.5 with c0 = 343 m/s."""
+    transit), Gaussian pulse of half-width 3 dx at 0.5 m, CFL 0.5 with c0 = 343 m/s, RK4,
+    terrain-following over a Gaussian hill of height 2 m and sigma 5 m (centred on the line
+    of receivers at y = 30 m), logarithmic wind u(z) = (0.4/0.41) ln(z/0.02) along +y.
+    Heights of the source and receivers are above the local ground.  reference_scheme=True
+    gives the same grid with the reference's capabilities only: FB_p_v, flat, no wind."""
     d = 0.04
     lead = 33
     ijk = 256 + 32 * 160
@@ -372,6 +379,10 @@ def config_c1(time_domain: int = 10000) -> Deck:
              gauss_width=2 * 3 * d, src=(0.0, 2.0, 0.5), rcv=(0.0, 6.0, 1.2), fft_m=14, pml_y_time=0,
              mf_judge=1, mf_jmax=256, mf_lead_dj=lead, mf_trail_dj=lead, mf_ratio=0.5, ijkmax=ijk)
     dk.pore_q, dk.pore_porosity, dk.pore_resistivity = 3.0 ** 0.5, 0.3, 2.0e5
+    if not reference_scheme:
+        dk.integrator = INT_RK4
+        dk.hills = [(0.0, 30.0, 2.0, 5.0)]
+        dk.velocity_method, dk.wind_ustar, dk.wind_z0 = 7, 0.4, 0.02
     return dk
 
 

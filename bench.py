@@ -1,19 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X LEE + porous-ground time step (one GPU).
 
-Workload (one "step" = one time step of the whole domain): the 512x256x128 air grid at
-0.04 m over a 16-point porous ground layer, 20-point PML on the sides and top, moving
-frame on, Gaussian pulse at 0.5 m, 32 receivers at 1.2 m (deck.config_c1).  Inputs are
-resident in HBM before the timed region; the timed region covers exactly K steps
-(including any frame moves that fall inside it) with a device synchronisation on both
-sides.  Both FP32 (headline `value`) and FP64 are measured.
+Workload (deck.config_c1): the 512x256x128 air grid at 0.04 m, terrain-following over a
+Gaussian hill (2 m high, sigma 5 m), over a 16-point Zwikker-Kosten porous layer, 20-point
+PML on the sides and top, logarithmic wind u(z) = (0.4/0.41) ln(z/0.02), moving frame on,
+Gaussian pulse of half-width 3 dx at 0.5 m, 32 receivers at 1.2 m, classical RK4 at CFL 0.5
+for 10000 steps.  One "step" is one RK4 time step of the whole domain (four stages: air,
+porous layer, faces).  FP32 (compensated accumulation) is the headline `value`; FP64 runs the
+same workload and is the reference for the accuracy figures.
 
-Metric: grid-point updates/s = (air points + porous points) x steps / wall time.
-`roofline` is the dominant kernel's compulsory HBM bytes per launch over its mean launch
-time (HIP events on the solver stream); `cpu_baseline` is the FP64 C oracle of the
-reference step on one host core over a bounded sample of the same workload.
+Timing: W untimed warm-up steps, then exactly K timed steps with a device synchronisation on
+both sides; inputs are resident in HBM.  The receiver series of the whole run (W + K steps)
+give the accuracy of FP32 against FP64: relative L2 and the largest insertion-loss
+difference |20 log10 |P32(f)| - 20 log10 |P64(f)|| over 50-1000 Hz (the reference spectrum
+of IL cancels in the difference).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--traffic-json F]
+`roofline`: the dominant kernel's compulsory HBM bytes per launch over its mean launch time
+(HIP events on the solver stream); `traffic` comes from rocprofv3 FETCH_SIZE / WRITE_SIZE
+runs when a JSON of them is given.  `cpu_baseline`: the FP64 C oracle of the same step on one
+host core over a bounded sample (the first step(s) of the same workload).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-fb] [--traffic-json F]
 """
 from __future__ import annotations
 
@@ -24,6 +31,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -31,7 +40,9 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def run_gpu(lee, deck, receivers, precision, steps, warmup):
+def run_gpu(lee, deck, receivers, precision, steps, warmup, record=True):
+    """Run warmup + steps steps of `deck`, frames moving as the reference schedules them.
+    Returns timing, kernel statistics and the receiver series of the whole run."""
     s = lee.Solver(deck, precision=precision, receivers=receivers)
     npts = s.nx * s.ny * s.nz + s.pnx * s.pny * s.pnz
     state = {"mf": 0, "temp": 0, "left": 0}
@@ -40,32 +51,27 @@ def run_gpu(lee, deck, receivers, precision, steps, warmup):
         state["left"] = s.frame_steps(state["mf"], state["temp"])
         s.frame_setup(state["mf"])
 
-    def advance(n, record=False):
-        """n steps, moving the frame whenever the current frame's budget is used up."""
+    def advance(n):
         out = []
         while n > 0:
             if state["left"] <= 0:
                 state["mf"] += 1
                 setup_frame()
             k = min(n, state["left"])
-            r = s.step(k, record=record)
-            if record:
-                out.append(r)
+            out.append(s.step(k, record=record))
             state["left"] -= k
             state["temp"] += k
             n -= k
         return out
 
     setup_frame()
-    advance(warmup)
+    series = advance(warmup)
     lee.lib().lee_profile(s.h, 1)
-    s.info()  # forces nothing; lee_step already ends with a device sync
     t0 = time.perf_counter()
-    advance(steps)
+    series += advance(steps)
     t1 = time.perf_counter()
-    wall = t1 - t0
-    stats = (lee.LeeKstat * 8)()
-    n = lee.lib().lee_kernel_stats(s.h, stats, 8)
+    stats = (lee.LeeKstat * 16)()
+    n = lee.lib().lee_kernel_stats(s.h, stats, 16)
     kst = []
     for i in range(n):
         k = stats[i]
@@ -73,9 +79,29 @@ def run_gpu(lee, deck, receivers, precision, steps, warmup):
             kst.append(dict(name=k.name.decode(), launches=k.launches, total_ms=k.total_ms,
                             bytes_per_launch=k.bytes_per_launch, points=k.points_per_launch))
     copy_ms = lee.lib().lee_copy_bench(s.h, 1 << 28, 10)  # 1 GiB read + 1 GiB write
-    info = dict(points=npts, wall=wall, kernels=kst, copy_ms=copy_ms, frames=state["mf"] + 1)
+    rec = np.concatenate([x for x in series if x is not None], axis=0) if record else None
+    info = dict(points=npts, wall=t1 - t0, kernels=kst, copy_ms=copy_ms, frames=state["mf"] + 1, series=rec)
     s.close()
     return info
+
+
+def accuracy(a, b, dt, fmin=50.0, fmax=1000.0):
+    """Relative L2 of series b against a, and the largest |IL_b - IL_a| in dB over
+    [fmin, fmax] on receivers that carry signal (bins 60 dB below the band peak skipped)."""
+    rel = float(np.linalg.norm(b - a) / max(np.linalg.norm(a), 1e-300))
+    energy = (a ** 2).sum(axis=0)
+    use = energy > 1e-6 * energy.max()
+    n = 1 << int(np.ceil(np.log2(a.shape[0])))
+    f = np.fft.rfftfreq(n, dt)
+    band = (f >= fmin) & (f <= fmax)
+    worst = 0.0
+    for r in np.nonzero(use)[0]:
+        A = np.abs(np.fft.rfft(a[:, r], n))[band]
+        B = np.abs(np.fft.rfft(b[:, r], n))[band]
+        ok = A > 1e-3 * A.max()
+        if ok.any():
+            worst = max(worst, float(np.abs(20 * np.log10(B[ok] / A[ok])).max()))
+    return rel, worst, int(use.sum())
 
 
 def cpu_baseline(deck, receivers, budget_s=20.0):
@@ -86,42 +112,45 @@ def cpu_baseline(deck, receivers, budget_s=20.0):
     npts = o.nx * o.ny * o.nz + o.pnx * o.pny * o.pnz
     t0 = time.perf_counter()
     o.step(1)
-    t1 = time.perf_counter()
-    first = t1 - t0
+    first = time.perf_counter() - t0
     more = max(0, min(9, int(budget_s / max(first, 1e-3)) - 1))
     t2 = time.perf_counter()
     if more:
         o.step(more)
-    t3 = time.perf_counter()
-    nsteps = 1 + more
-    sec = first + (t3 - t2)
-    return dict(value=npts * nsteps / sec, steps=nsteps, seconds=sec, points=npts)
+    sec = first + (time.perf_counter() - t2)
+    return dict(value=npts * (1 + more) / sec, steps=1 + more, seconds=sec, points=npts)
+
+
+def kernel_table(kernels):
+    return [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in kk.items()} for kk in kernels]
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fb", action="store_true", help="skip the reference-scheme (FB_p_v, flat) lines")
     ap.add_argument("--precisions", default="32,64")
     ap.add_argument("--traffic-json", default=None,
-                    help="JSON with HBM bytes per launch of the dominant kernel from rocprofv3 counters")
+                    help="JSON {kernel: HBM bytes per launch} from rocprofv3 FETCH_SIZE/WRITE_SIZE runs")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): the domain lives in one "
               "GPU's HBM and no multi-GPU path is built", file=sys.stderr)
         sys.exit(2)
     lee = importlib.import_module("3d_mpi_test_amd")
-    deck = lee.deck.config_c1()
+    total = a.warmup + a.steps
+    deck = lee.deck.config_c1(time_domain=total)
     rec = lee.deck.receivers_c1()
     res = {}
     for p in [int(x) for x in a.precisions.split(",")]:
-        print(f"[bench] fp{p}: {a.warmup} warm-up + {a.steps} timed steps", file=sys.stderr, flush=True)
+        print(f"[bench] C1 RK4 fp{p}: {a.warmup} warm-up + {a.steps} timed steps", file=sys.stderr, flush=True)
         res[p] = run_gpu(lee, deck, rec, p, a.steps, a.warmup)
         r = res[p]
-        print(f"[bench] fp{p}: {r['points'] * a.steps / r['wall'] / 1e9:.2f} G pt-updates/s "
-              f"({1e3 * r['wall'] / a.steps:.3f} ms/step)", file=sys.stderr, flush=True)
+        print(f"[bench] C1 RK4 fp{p}: {r['points'] * a.steps / r['wall'] / 1e9:.2f} G pt-updates/s "
+              f"({1e3 * r['wall'] / a.steps:.3f} ms/step, {r['frames']} frames)", file=sys.stderr, flush=True)
     head = 32 if 32 in res else list(res)[0]
     r = res[head]
     value = r["points"] * a.steps / r["wall"]
@@ -131,24 +160,40 @@ def main():
     traffic = None
     if a.traffic_json and os.path.exists(a.traffic_json):
         with open(a.traffic_json) as fh:
-            traffic = json.load(fh).get(dom["name"])
+            traffic = json.load(fh).get(f"{dom['name']}_fp{head}")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom["name"],
             "bytes_per_launch": dom["bytes_per_launch"], "avg_launch_ms": round(avg_s * 1e3, 4),
             "copy_kernel_GBs": round(2 * (1 << 30) / (r["copy_ms"] / 1e3) / 1e9, 1) if r["copy_ms"] > 0 else None}
+    acc = None
+    if 32 in res and 64 in res:
+        rl2, dil, nrec = accuracy(res[64]["series"], res[32]["series"], deck.dt)
+        acc = {"fp32_vs_fp64_rel_l2": rl2, "max_dIL_dB_50_1000Hz": round(dil, 5), "receivers_with_signal": nrec,
+               "steps": total}
+        print(f"[bench] FP32 vs FP64: rel-L2 {rl2:.3e}, max |dIL| {dil:.4f} dB over 50-1000 Hz", file=sys.stderr,
+              flush=True)
+    fb = {}
+    if not a.no_fb:
+        # the reference's own scheme on the same grid: FB_p_v, flat ground, no wind
+        fdeck = lee.deck.config_c1(time_domain=220, reference_scheme=True)
+        for p in (32, 64):
+            print(f"[bench] C1 reference scheme FB_p_v fp{p}", file=sys.stderr, flush=True)
+            rr = run_gpu(lee, fdeck, rec, p, 200, 20, record=False)
+            fb[f"fb_fp{p}_pt_updates_per_s"] = rr["points"] * 200 / rr["wall"]
+            fb[f"fb_fp{p}_ms_per_step"] = 1e3 * rr["wall"] / 200
+            fb[f"fb_fp{p}_kernels"] = kernel_table(rr["kernels"])
     cpu = None
     if not a.no_cpu:
         print("[bench] cpu baseline (oracle, one core)", file=sys.stderr, flush=True)
         c = cpu_baseline(deck, rec)
         cpu = {"value": round(c["value"], 1), "unit": "grid-pt updates/s", "cores": 1, "kind": "port",
-               "sample": f"C1 grid 512x256x128 + 512x256x18 porous, FB_p_v FP64, first {c['steps']} steps "
+               "sample": f"C1 (RK4, terrain, log wind, porous ground) FP64, first {c['steps']} step(s) "
                          f"({c['seconds']:.1f} s), single thread"}
     extra = {}
     for p, rr in res.items():
         extra[f"fp{p}_pt_updates_per_s"] = rr["points"] * a.steps / rr["wall"]
         extra[f"fp{p}_ms_per_step"] = 1e3 * rr["wall"] / a.steps
-        extra[f"fp{p}_kernels"] = [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in kk.items()}
-                                   for kk in rr["kernels"]]
+        extra[f"fp{p}_kernels"] = kernel_table(rr["kernels"])
     if 32 in res and 64 in res:
         extra["fp32_over_fp64"] = round(extra["fp32_pt_updates_per_s"] / extra["fp64_pt_updates_per_s"], 3)
     line = {
@@ -164,13 +209,17 @@ def main():
         "vs_baseline": None,
         "dtype": f"f{head}",
         "data": "synthetic",
-        "config": {"workload": "C1 grid 512x256x128 dx=0.04 m + 16-pt porous ground, 20-pt PML sides+top, "
-                               "moving frame on, FB_p_v (reference scheme), flat ground, no wind",
+        "config": {"workload": "C1: 512x256x128 air at 0.04 m, terrain-following over a Gaussian hill (2 m, "
+                               "sigma 5 m), 16-point porous ground (200 kPa s/m^2, 0.3, 3), 20-pt PML sides+top, "
+                               "log wind u*=0.4 m/s z0=0.02 m, moving frame, 32 receivers at 1.2 m, RK4 CFL 0.5",
                    "air_points": 512 * 256 * 128, "porous_points": r["points"] - 512 * 256 * 128,
-                   "frames_in_run": r["frames"], "parallelism": "single GPU"},
+                   "frames_in_run": r["frames"], "integrator": "RK4 (FP32: compensated accumulation)",
+                   "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "accuracy": acc,
         **extra,
+        **fb,
     }
     print(json.dumps(line))
 
