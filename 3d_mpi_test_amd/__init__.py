@@ -104,6 +104,9 @@ def lib():
         L.lee_profile.argtypes = [vp, ctypes.c_int]
         L.lee_kernel_stats.argtypes = [vp, ctypes.POINTER(LeeKstat), ctypes.c_int]
         L.lee_copy_bench.argtypes = [vp, ctypes.c_longlong, ctypes.c_int]
+        L.lee_save_restart.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
+        L.lee_load_restart.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
+        L.lee_write_contour.argtypes = [vp, ctypes.c_char_p]
         L.lee_copy_bench.restype = ctypes.c_double
         _lib = L
     return _lib
@@ -137,7 +140,7 @@ class Solver:
     """One solver context on one GPU (C ABI of include/lee_abi.h)."""
 
     def __init__(self, d, precision: int = 64, receivers=None, health_every: int = 0, device: int = 0,
-                 path: int = PATH_AUTO, integrator: Optional[int] = None):
+                 path: int = PATH_AUTO, integrator: Optional[int] = None, use_graph: bool = False):
         L = lib()
         if isinstance(d, deck.Deck):
             self._deck_obj = d
@@ -146,7 +149,8 @@ class Solver:
             self._deck_obj = None
             self._cdeck = d
         opt = deck.make_options(self._deck_obj, precision=precision, receivers=receivers,
-                                health_every=health_every, device=device, path=path, integrator=integrator)
+                                health_every=health_every, device=device, path=path, integrator=integrator,
+                                use_graph=1 if use_graph else 0)
         self._opt = opt
         h = ctypes.c_void_p()
         _check(L.lee_create(ctypes.byref(self._cdeck), ctypes.byref(opt), ctypes.byref(h)))
@@ -213,6 +217,17 @@ class Solver:
         _check(lib().lee_get_psi(self.h, _p(out), out.size), self.h)
         return out
 
+    def save_restart(self, path: str, index: int) -> None:
+        _check(lib().lee_save_restart(self.h, path.encode(), index), self.h)
+
+    def load_restart(self, path: str) -> int:
+        idx = ctypes.c_int(0)
+        _check(lib().lee_load_restart(self.h, path.encode(), ctypes.byref(idx)), self.h)
+        return idx.value
+
+    def write_contour(self, dirpath: str) -> None:
+        _check(lib().lee_write_contour(self.h, dirpath.encode()), self.h)
+
     def last_ms(self) -> float:
         return lib().lee_last_step_ms(self.h)
 
@@ -232,11 +247,45 @@ class AirPore:
         self.solver = Solver(self._deck, precision=precision, receivers=receivers)
         self.series = None
 
-    def get_output(self) -> np.ndarray:
-        self.series = self.solver.run()
+    def get_output(self, contour_dir: Optional[str] = None, restart_dir: Optional[str] = None) -> np.ndarray:
+        """The time loop of airpore::get_output (airpore.cpp:400-641).  With `contour_dir` a
+        contour file is written whenever the steps done are a multiple of out_difft
+        (airpore.cpp:615-618); with `restart_dir` a restart file ctNNN_0.dat whenever they are a
+        multiple of restart_out (airpore.cpp:620, 664-700 name the files this way)."""
+        if contour_dir is None and restart_dir is None:
+            self.series = self.solver.run()
+        else:
+            self.series = self._loop(contour_dir, restart_dir)
         s = np.ascontiguousarray(self.series)
         _check(lib().lee_write_receivers(self.dir.encode(), _p(s), s.shape[0], s.shape[1]))
         return self.series
+
+    def _loop(self, contour_dir, restart_dir):
+        sv, d = self.solver, self._deck
+        T = d.time_domain
+        out = np.zeros((T, sv.nrec))
+        every = [x for x in (d.out_difft if contour_dir else 0, d.restart_out if restart_dir else 0) if x > 0]
+        mf, done = 0, 0
+        while done < T:
+            if mf >= sv.frame_count():
+                break
+            lim = sv.frame_steps(mf, done)
+            sv.frame_setup(mf)
+            left = max(lim, 0)
+            while left > 0 and done < T:
+                nxt = min([e - done % e for e in every] + [left, T - done])
+                out[done:done + nxt] = sv.step(nxt)
+                done += nxt
+                left -= nxt
+                if contour_dir and done % d.out_difft == 0:
+                    sv.write_contour(contour_dir)
+                if restart_dir and done % d.restart_out == 0:
+                    n = done // d.restart_out
+                    sv.save_restart(os.path.join(restart_dir, f"ct{n:03d}_0.dat"), done)
+            mf += 1
+            if d.mf_judge != 1:
+                break
+        return out
 
     def get_FFT_y1(self) -> None:
         if self.series is None:

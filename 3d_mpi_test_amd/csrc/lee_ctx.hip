@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <memory>
 #include <string>
 #include <vector>
@@ -30,6 +31,7 @@ using namespace lee;
 namespace {
 
 constexpr int kRecChunk = 4096;  // receiver samples buffered on the device between copies
+constexpr int kGraphSteps = 64;  // steps per captured graph (use_graph)
 
 struct DevArr {
     void *p = nullptr;
@@ -62,6 +64,9 @@ struct lee_ctx {
     void *rkq_sb[6][4] = {}, *rkq_acc[6][4] = {};
     // terrain tables (sh, hx, hy per (i,j); tk per k) of the air [0] and porous [1] grids
     void *ter[2][4] = {};
+    // captured step chunks (use_graph): keyed by the buffer roles they start from
+    struct GraphEntry { int n, cur, qcur, gate_on; void *psi; hipGraphExec_t exec; };
+    std::vector<GraphEntry> graphs;
     double *psi = nullptr, *psi2 = nullptr;  // TDBC histories, read / written each step
     long long *rec_idx = nullptr;
     double *rec_buf = nullptr;
@@ -84,6 +89,7 @@ struct lee_ctx {
     {
         // teardown: errors cannot be reported from a destructor
         if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
         for (auto &a : allocs) if (a.p) (void)hipFree(a.p);
         for (auto e : evpool) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -377,193 +383,238 @@ template <typename T> void rk_stage_q(lee_ctx *c, int st, int s, int f, T *&src,
     dst = D[st];
 }
 
-template <typename T> int rk4_steps(lee_ctx *c, int nsteps, double *rec_out)
+// y1 time gate of the coming step (air.cpp:1965-1969): advances the counter.
+void gate_tick(lee_ctx *c, int &J1, int &y1on)
+{
+    const Model &m = c->m;
+    const bool gate = (m.judge[1] == 1 || m.judge[1] == 3);
+    J1 = m.J1;
+    if (gate) {
+        c->counter += 1;
+        J1 = c->counter >= m.pml_time ? m.J3 : 0;
+    }
+    y1on = (m.slab[2].n > 0 && gate && c->counter >= m.pml_time) ? 1 : 0;
+}
+
+// One RK4 step: launches only (no host synchronisation), receivers into rec_slot.
+template <typename T> void enqueue_rk4_step(lee_ctx *c, double *rec_slot)
 {
     Model &m = c->m;
     hipStream_t s = c->stream;
     const int nr = (int)m.ri.size();
-    const bool gate = (m.judge[1] == 1 || m.judge[1] == 3);
     const int wind = m.has_wind ? (m.wind_v_only ? 2 : 1) : 0;
-    HIPCHK(c, hipEventRecord(c->ev0, s));
-    int buffered = 0, written = 0;
-    for (int t = 0; t < nsteps; ++t) {
-        int J1 = m.J1;
-        if (gate) {
-            c->counter += 1;
-            J1 = c->counter >= m.pml_time ? m.J3 : 0;
+    int J1, y1on;
+    gate_tick(c, J1, y1on);
+    // receivers sample level n before the step (airpore.cpp:573-580)
+    launch_sample<T>((const T *)c->air[c->cur * 4 + 3], c->rec_idx, nr, rec_slot, s);
+    for (int st = 0; st < 4; ++st) {
+        T *src[4], *dst[4];
+        rk_stage_buffers<T>(c, 0, st, src, dst);
+        RKArgs<T> r;
+        r.a = air_args<T>(c, J1, y1on);
+        Grid<T> &g = r.a.g;
+        g.un = src[0]; g.vn = src[1]; g.wn = src[2]; g.pn = src[3];
+        g.unn = g.vnn = g.wnn = nullptr;
+        g.pnn = src[3];  // the velocity increment reads the stage pressure
+        for (int f = 0; f < 4; ++f) {
+            r.xn[f] = (const T *)c->air[c->cur * 4 + f];
+            r.acc[f] = (T *)c->rk_acc[0][f]; r.lo[f] = (T *)c->rk_lo[0][f]; r.err[f] = (T *)c->rk_err[0][f];
+            r.dst[f] = dst[f];
         }
-        const int y1on = (m.slab[2].n > 0 && gate && c->counter >= m.pml_time) ? 1 : 0;
-        // receivers sample level n before the step (airpore.cpp:573-580)
-        launch_sample<T>((const T *)c->air[c->cur * 4 + 3], c->rec_idx, nr, c->rec_buf + (size_t)buffered * nr, s);
-        for (int st = 0; st < 4; ++st) {
-            T *src[4], *dst[4];
-            rk_stage_buffers<T>(c, 0, st, src, dst);
-            RKArgs<T> r;
-            r.a = air_args<T>(c, J1, y1on);
-            Grid<T> &g = r.a.g;
-            g.un = src[0]; g.vn = src[1]; g.wn = src[2]; g.pn = src[3];
-            g.unn = g.vnn = g.wnn = nullptr;
-            g.pnn = src[3];  // the velocity increment reads the stage pressure
+        for (int q = 0; q < 6; ++q)
             for (int f = 0; f < 4; ++f) {
-                r.xn[f] = (const T *)c->air[c->cur * 4 + f];
-                r.acc[f] = (T *)c->rk_acc[0][f]; r.lo[f] = (T *)c->rk_lo[0][f]; r.err[f] = (T *)c->rk_err[0][f];
-                r.dst[f] = dst[f];
+                T *qs, *qd;
+                rk_stage_q<T>(c, st, q, f, qs, qd);
+                r.a.Qo[q][f] = qs; r.a.Qn[q][f] = qs;
+                r.qxn[q][f] = (const T *)c->Q[c->qcur][q][f];
+                r.qacc[q][f] = (T *)c->rkq_acc[q][f];
+                r.qdst[q][f] = qd;
             }
-            for (int q = 0; q < 6; ++q)
-                for (int f = 0; f < 4; ++f) {
-                    T *qs, *qd;
-                    rk_stage_q<T>(c, st, q, f, qs, qd);
-                    r.a.Qo[q][f] = qs; r.a.Qn[q][f] = qs;
-                    r.qxn[q][f] = (const T *)c->Q[c->qcur][q][f];
-                    r.qacc[q][f] = (T *)c->rkq_acc[q][f];
-                    r.qdst[q][f] = qd;
-                }
-            r.stage = st;
-            prof_mark(c, KS_RK_AIR, true);
-            launch_rk_air<T>(r, wind, m.terrain, s);
-            prof_mark(c, KS_RK_AIR, false);
-            T *psrc[4] = {}, *pdst[4] = {};
-            if (m.south_porous) {
-                rk_stage_buffers<T>(c, 1, st, psrc, pdst);
-                RKPoreArgs<T> pr;
-                pr.g = pore_grid<T>(c);
-                pr.g.un = psrc[0]; pr.g.vn = psrc[1]; pr.g.wn = psrc[2]; pr.g.pn = psrc[3];
-                pr.g.unn = pr.g.vnn = pr.g.wnn = pr.g.pnn = nullptr;
-                pr.gama = (T)m.p_gama; pr.eff = (T)m.p_eff; pr.beta = (T)m.p_beta;
-                for (int f = 0; f < 4; ++f) {
-                    pr.xn[f] = (const T *)c->pore[c->cur * 4 + f];
-                    pr.acc[f] = (T *)c->rk_acc[1][f]; pr.lo[f] = (T *)c->rk_lo[1][f]; pr.err[f] = (T *)c->rk_err[1][f];
-                    pr.dst[f] = pdst[f];
-                }
-                pr.stage = st;
-                prof_mark(c, KS_RK_PORE, true);
-                launch_rk_pore<T>(pr, m.terrain, s);
-                prof_mark(c, KS_RK_PORE, false);
+        r.stage = st;
+        prof_mark(c, KS_RK_AIR, true);
+        launch_rk_air<T>(r, wind, m.terrain, s);
+        prof_mark(c, KS_RK_AIR, false);
+        T *psrc[4] = {}, *pdst[4] = {};
+        if (m.south_porous) {
+            rk_stage_buffers<T>(c, 1, st, psrc, pdst);
+            RKPoreArgs<T> pr;
+            pr.g = pore_grid<T>(c);
+            pr.g.un = psrc[0]; pr.g.vn = psrc[1]; pr.g.wn = psrc[2]; pr.g.pn = psrc[3];
+            pr.g.unn = pr.g.vnn = pr.g.wnn = pr.g.pnn = nullptr;
+            pr.gama = (T)m.p_gama; pr.eff = (T)m.p_eff; pr.beta = (T)m.p_beta;
+            for (int f = 0; f < 4; ++f) {
+                pr.xn[f] = (const T *)c->pore[c->cur * 4 + f];
+                pr.acc[f] = (T *)c->rk_acc[1][f]; pr.lo[f] = (T *)c->rk_lo[1][f]; pr.err[f] = (T *)c->rk_err[1][f];
+                pr.dst[f] = pdst[f];
             }
-            // faces of the destination state and the boundary-node auxiliaries
-            RKBndArgs<T> b;
-            b.b = bc_args<T>(c, r.a, nullptr);
-            b.b.tdbc = 0;
-            b.b.nrec = 0;
-            b.b.air.unn = dst[0]; b.b.air.vnn = dst[1]; b.b.air.wnn = dst[2]; b.b.air.pnn = dst[3];
-            if (m.south_porous) {
-                b.b.pore.unn = pdst[0]; b.b.pore.vnn = pdst[1]; b.b.pore.wnn = pdst[2]; b.b.pore.pnn = pdst[3];
-            }
-            for (int f = 0; f < 4; ++f) b.src[f] = src[f];
-            for (int q = 0; q < 6; ++q)
-                for (int f = 0; f < 4; ++f) { b.qxn[q][f] = r.qxn[q][f]; b.qacc[q][f] = r.qacc[q][f]; b.qdst[q][f] = r.qdst[q][f]; }
-            b.stage = st;
-            prof_mark(c, KS_RK_BND, true);
-            launch_rk_bnd<T>(b, s);
-            prof_mark(c, KS_RK_BND, false);
+            pr.stage = st;
+            prof_mark(c, KS_RK_PORE, true);
+            launch_rk_pore<T>(pr, m.terrain, s);
+            prof_mark(c, KS_RK_PORE, false);
         }
-        ++buffered;
-        c->steps += 1;
-        if (buffered == kRecChunk || t == nsteps - 1) {
-            if (rec_out && nr) {
-                HIPCHK(c, hipMemcpyAsync(rec_out + (size_t)written * nr, c->rec_buf, (size_t)buffered * nr * 8,
-                                         hipMemcpyDeviceToHost, s));
-                HIPCHK(c, hipStreamSynchronize(s));
-            }
-            written += buffered;
-            buffered = 0;
+        // faces of the destination state and the boundary-node auxiliaries
+        RKBndArgs<T> b;
+        b.b = bc_args<T>(c, r.a, nullptr);
+        b.b.tdbc = 0;
+        b.b.nrec = 0;
+        b.b.air.unn = dst[0]; b.b.air.vnn = dst[1]; b.b.air.wnn = dst[2]; b.b.air.pnn = dst[3];
+        if (m.south_porous) {
+            b.b.pore.unn = pdst[0]; b.b.pore.vnn = pdst[1]; b.b.pore.wnn = pdst[2]; b.b.pore.pnn = pdst[3];
         }
-        if (m.opt.health_every > 0 && c->steps % m.opt.health_every == 0) {
-            HIPCHK(c, hipMemsetAsync(c->flag, 0, sizeof(int), s));
-            launch_finite_check<T>((const T *)c->air[c->cur * 4 + 3], air_n(c), c->flag, s);
-            int h = 0;
-            HIPCHK(c, hipMemcpyAsync(&h, c->flag, sizeof(int), hipMemcpyDeviceToHost, s));
-            HIPCHK(c, hipStreamSynchronize(s));
-            if (h) return set_err(c, LEE_ERR_NONFINITE, "non-finite pressure after step " + std::to_string(c->steps));
-        }
+        for (int f = 0; f < 4; ++f) b.src[f] = src[f];
+        for (int q = 0; q < 6; ++q)
+            for (int f = 0; f < 4; ++f) { b.qxn[q][f] = r.qxn[q][f]; b.qacc[q][f] = r.qacc[q][f]; b.qdst[q][f] = r.qdst[q][f]; }
+        b.stage = st;
+        prof_mark(c, KS_RK_BND, true);
+        launch_rk_bnd<T>(b, s);
+        prof_mark(c, KS_RK_BND, false);
     }
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventRecord(c->ev1, s));
-    HIPCHK(c, hipEventSynchronize(c->ev1));
-    HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
-    prof_resolve(c);
+    c->steps += 1;
+}
+
+// One FB_p_v step (airpore.cpp:589-614): launches only, receivers into rec_slot.
+template <typename T> void enqueue_fb_step(lee_ctx *c, double *rec_slot)
+{
+    Model &m = c->m;
+    hipStream_t s = c->stream;
+    int J1, y1on;
+    gate_tick(c, J1, y1on);
+    AirArgs<T> a = air_args<T>(c, J1, y1on);
+    PoreArgs<T> pa{};
+    if (m.south_porous) pa = PoreArgs<T>{pore_grid<T>(c), (T)m.p_gama, (T)m.p_invbeta, (T)m.p_eff};
+    BCArgs<T> b = bc_args<T>(c, a, rec_slot);
+    if (c->fused) {
+        // porous pressure first: the air South face reads its plane Kp-2 through the interface
+        if (m.south_porous) {
+            prof_mark(c, KS_PORE_P, true);
+            launch_pore_p<T>(pa, s);
+            prof_mark(c, KS_PORE_P, false);
+        }
+        prof_mark(c, KS_AIR_FB, true);
+        launch_air_fb<T>(a, m.has_wind, s);
+        prof_mark(c, KS_AIR_FB, false);
+        prof_mark(c, KS_BC_P, true);
+        launch_bc_p_all<T>(b, s);
+        prof_mark(c, KS_BC_P, false);
+    } else {
+        // ---- pressure phase (airpore.cpp:366-398); receivers sample level n first
+        prof_mark(c, KS_AIR_P, true);
+        launch_air_p<T>(a, m.has_wind, s);
+        prof_mark(c, KS_AIR_P, false);
+        if (m.south_porous) {
+            prof_mark(c, KS_PORE_P, true);
+            launch_pore_p<T>(pa, s);
+            prof_mark(c, KS_PORE_P, false);
+        }
+        prof_mark(c, KS_BC_P, true);
+        launch_bc_p_all<T>(b, s);
+        prof_mark(c, KS_BC_P, false);
+        // ---- velocity phase (airpore.cpp:324-364)
+        prof_mark(c, KS_AIR_V, true);
+        launch_air_v<T>(a, m.has_wind, s);
+        prof_mark(c, KS_AIR_V, false);
+    }
+    if (m.south_porous) {
+        prof_mark(c, KS_PORE_V, true);
+        launch_pore_v<T>(pa, s);
+        prof_mark(c, KS_PORE_V, false);
+    }
+    prof_mark(c, KS_BC_V, true);
+    launch_bc_v_all<T>(b, s);
+    prof_mark(c, KS_BC_V, false);
+    if (m.east_tdbc) std::swap(c->psi, c->psi2);
+    // ---- level n+1 becomes level n
+    c->cur ^= 1;
+    c->qcur ^= 1;
+    c->steps += 1;
+}
+
+template <typename T> void enqueue_step(lee_ctx *c, double *rec_slot)
+{
+    if (c->m.rk4) enqueue_rk4_step<T>(c, rec_slot);
+    else enqueue_fb_step<T>(c, rec_slot);
+}
+
+// The y1 gate is a launch argument: a captured chunk must not cross its opening.
+bool gate_constant(const lee_ctx *c, int n)
+{
+    const Model &m = c->m;
+    if (!(m.judge[1] == 1 || m.judge[1] == 3)) return true;
+    return c->counter + 1 >= m.pml_time || c->counter + n < m.pml_time;
+}
+
+void drop_graphs(lee_ctx *c)
+{
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+}
+
+// n steps from a captured graph: found by the buffer roles at its start, else captured now.
+// Replaying advances the host-side roles exactly as the capture did.
+template <typename T> int graph_steps(lee_ctx *c, int n)
+{
+    const Model &m = c->m;
+    const bool gate = (m.judge[1] == 1 || m.judge[1] == 3);
+    const int gate_on = gate && c->counter + 1 >= m.pml_time;
+    for (auto &g : c->graphs)
+        if (g.n == n && g.cur == c->cur && g.qcur == c->qcur && g.gate_on == gate_on && g.psi == (void *)c->psi) {
+            HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
+            if (!m.rk4) {
+                if (n & 1) { c->cur ^= 1; c->qcur ^= 1; if (m.east_tdbc) std::swap(c->psi, c->psi2); }
+            }
+            if (gate) c->counter += n;
+            c->steps += n;
+            return LEE_OK;
+        }
+    lee_ctx::GraphEntry e{n, c->cur, c->qcur, gate_on, (void *)c->psi, nullptr};
+    const int nr = (int)m.ri.size();
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int t = 0; t < n; ++t) enqueue_step<T>(c, c->rec_buf + (size_t)t * nr);
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamEndCapture(c->stream, &graph));
+    hipError_t ie = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(c, ie);
+    if (c->graphs.size() >= 8) {
+        (void)hipGraphExecDestroy(c->graphs.front().exec);
+        c->graphs.erase(c->graphs.begin());
+    }
+    c->graphs.push_back(e);
+    HIPCHK(c, hipGraphLaunch(e.exec, c->stream));
     return LEE_OK;
 }
 
 template <typename T> int run_steps(lee_ctx *c, int nsteps, double *rec_out)
 {
     Model &m = c->m;
-    if (m.rk4) return rk4_steps<T>(c, nsteps, rec_out);
     hipStream_t s = c->stream;
     const int nr = (int)m.ri.size();
-    const bool gate = (m.judge[1] == 1 || m.judge[1] == 3);
     HIPCHK(c, hipEventRecord(c->ev0, s));
-    int buffered = 0, written = 0;
-    for (int t = 0; t < nsteps; ++t) {
-        // y1 time gate (air.cpp:1965-1969)
-        int J1 = m.J1;
-        if (gate) {
-            c->counter += 1;
-            J1 = c->counter >= m.pml_time ? m.J3 : 0;
-        }
-        const int y1on = (m.slab[2].n > 0 && gate && c->counter >= m.pml_time) ? 1 : 0;
-        AirArgs<T> a = air_args<T>(c, J1, y1on);
-        PoreArgs<T> pa{};
-        if (m.south_porous) pa = PoreArgs<T>{pore_grid<T>(c), (T)m.p_gama, (T)m.p_invbeta, (T)m.p_eff};
-        BCArgs<T> b = bc_args<T>(c, a, c->rec_buf + (size_t)buffered * nr);
-        if (c->fused) {
-            // porous pressure first: the air South face reads its plane Kp-2 through the interface
-            if (m.south_porous) {
-                prof_mark(c, KS_PORE_P, true);
-                launch_pore_p<T>(pa, s);
-                prof_mark(c, KS_PORE_P, false);
+    int done = 0;
+    while (done < nsteps) {
+        int n = std::min(nsteps - done, kRecChunk);
+        const long long before = c->steps;
+        if (m.opt.use_graph && !c->profile) {
+            n = std::min(n, kGraphSteps);
+            if (gate_constant(c, n)) {
+                int rc = graph_steps<T>(c, n);
+                if (rc) return rc;
+            } else {
+                for (int t = 0; t < n; ++t) enqueue_step<T>(c, c->rec_buf + (size_t)t * nr);
             }
-            prof_mark(c, KS_AIR_FB, true);
-            launch_air_fb<T>(a, m.has_wind, s);
-            prof_mark(c, KS_AIR_FB, false);
-            prof_mark(c, KS_BC_P, true);
-            launch_bc_p_all<T>(b, s);
-            prof_mark(c, KS_BC_P, false);
         } else {
-            // ---- pressure phase (airpore.cpp:366-398); receivers sample level n first
-            prof_mark(c, KS_AIR_P, true);
-            launch_air_p<T>(a, m.has_wind, s);
-            prof_mark(c, KS_AIR_P, false);
-            if (m.south_porous) {
-                prof_mark(c, KS_PORE_P, true);
-                launch_pore_p<T>(pa, s);
-                prof_mark(c, KS_PORE_P, false);
-            }
-            prof_mark(c, KS_BC_P, true);
-            launch_bc_p_all<T>(b, s);
-            prof_mark(c, KS_BC_P, false);
-            // ---- velocity phase (airpore.cpp:324-364)
-            prof_mark(c, KS_AIR_V, true);
-            launch_air_v<T>(a, m.has_wind, s);
-            prof_mark(c, KS_AIR_V, false);
+            for (int t = 0; t < n; ++t) enqueue_step<T>(c, c->rec_buf + (size_t)t * nr);
         }
-        ++buffered;
-        if (m.south_porous) {
-            prof_mark(c, KS_PORE_V, true);
-            launch_pore_v<T>(pa, s);
-            prof_mark(c, KS_PORE_V, false);
+        HIPCHK(c, hipGetLastError());
+        if (rec_out && nr) {
+            HIPCHK(c, hipMemcpyAsync(rec_out + (size_t)done * nr, c->rec_buf, (size_t)n * nr * 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(c, hipStreamSynchronize(s));
         }
-        prof_mark(c, KS_BC_V, true);
-        launch_bc_v_all<T>(b, s);
-        prof_mark(c, KS_BC_V, false);
-        if (m.east_tdbc) std::swap(c->psi, c->psi2);
-        // ---- level n+1 becomes level n
-        c->cur ^= 1;
-        c->qcur ^= 1;
-        c->steps += 1;
-        if (buffered == kRecChunk || t == nsteps - 1) {
-            if (rec_out && nr) {
-                HIPCHK(c, hipMemcpyAsync(rec_out + (size_t)written * nr, c->rec_buf, (size_t)buffered * nr * 8,
-                                         hipMemcpyDeviceToHost, s));
-                HIPCHK(c, hipStreamSynchronize(s));
-            }
-            written += buffered;
-            buffered = 0;
-        }
-        if (m.opt.health_every > 0 && c->steps % m.opt.health_every == 0) {
+        done += n;
+        const int he = m.opt.health_every;
+        if (he > 0 && c->steps / he != before / he) {
             HIPCHK(c, hipMemsetAsync(c->flag, 0, sizeof(int), s));
-            int n = c->cur * 4;
-            launch_finite_check<T>((const T *)c->air[n + 3], air_n(c), c->flag, s);
+            launch_finite_check<T>((const T *)c->air[c->cur * 4 + 3], air_n(c), c->flag, s);
             int h = 0;
             HIPCHK(c, hipMemcpyAsync(&h, c->flag, sizeof(int), hipMemcpyDeviceToHost, s));
             HIPCHK(c, hipStreamSynchronize(s));
@@ -732,6 +783,7 @@ int lee_frame_setup(lee_ctx *c, int N)
 {
     if (!c || N < 0) return set_err(c, LEE_ERR_ARG, "bad frame number");
     if (N >= c->m.frame_count()) return set_err(c, LEE_ERR_ARG, "frame number beyond the last frame");
+    drop_graphs(c);  // a frame shift changes buffer roles the graphs were captured with
     int rc = c->prec == 64 ? frame_shift<double>(c, N) : frame_shift<float>(c, N);
     if (rc) return rc;
     if (N == 0)  // air::SetQMove(0) zeroes every auxiliary (air.cpp:2453-2519)
@@ -806,6 +858,312 @@ int lee_set_state(lee_ctx *c, int domain, const double *u, const double *v, cons
     for (int q = 0; q < 4; ++q)
         if (src[q] && (rc = upload(c, f[b + q], src[q], n))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LEE_OK;
+}
+
+}  // extern "C"
+
+// ---- checkpoint / restart in the reference's text format -------------------------------
+namespace {
+
+// Device array of n elements of the context precision -> FP64 host vector.
+int fetch(lee_ctx *c, const void *src, size_t n, std::vector<double> &out)
+{
+    out.assign(n, 0.0);
+    if (!src || !n) return LEE_OK;
+    return download(c, out.data(), src, n);
+}
+
+// The stream format of calculation_2D::save_restart_cal / air::save_restart_air
+// (calculation_2D.cpp:497-520, air.cpp:1305-1390): scientific, explicit sign, 10 digits,
+// width 18 for the first item only, one space after every value, a newline per section.
+void restart_stream(std::ofstream &o)
+{
+    o.setf(std::ios::scientific, std::ios::floatfield);
+    o.setf(std::ios::showpos);
+    o.precision(10);
+    o.width(18);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Reference layout: f[i][j][k] loops (i outer, k inner) for u v w p; then U V W of the mean
+// flow; then the PML slabs x1, x2 (i, j, k loops), y1, y2 (j, k, i loops), z1, z2 (k, i, j
+// loops), each Q_u Q_v Q_w Q_p per node; then the porous u v w p (airpore.cpp:783-796).
+int lee_save_restart(lee_ctx *c, const char *path, int index)
+{
+    if (!c || !path) return set_err(c, LEE_ERR_ARG, "bad arguments");
+    const Model &m = c->m;
+    int rc;
+    std::vector<double> f[4], U, V, W;
+    const int b = c->cur * 4;
+    for (int q = 0; q < 4; ++q)
+        if ((rc = fetch(c, c->air[b + q], air_n(c), f[q]))) return rc;
+    if ((rc = fetch(c, m.has_wind ? c->U : nullptr, m.has_wind ? air_n(c) : 0, U)) ||
+        (rc = fetch(c, m.has_wind ? c->V : nullptr, m.has_wind ? air_n(c) : 0, V)) ||
+        (rc = fetch(c, m.has_wind ? c->W : nullptr, m.has_wind ? air_n(c) : 0, W)))
+        return rc;
+    {
+        // header: mpi_size mpi_rank index (airpore.cpp:686-688)
+        std::ofstream h(path, std::ios::out | std::ios::binary | std::ios::trunc);
+        if (!h) return set_err(c, LEE_ERR_IO, std::string("cannot write ") + path);
+        h << 1 << " " << 0 << " " << index << std::endl;
+    }
+    std::ofstream o(path, std::ios::app | std::ios::binary);
+    if (!o) return set_err(c, LEE_ERR_IO, std::string("cannot write ") + path);
+    const int nx = m.nx, ny = m.ny, nz = m.nz;
+    auto id = [&](int i, int j, int k) { return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k); };
+    restart_stream(o);
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k)
+                for (int q = 0; q < 4; ++q) o << f[q][id(i, j, k)] << " ";
+    o << std::endl;
+    restart_stream(o);
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const size_t x = id(i, j, k);
+                o << (U.empty() ? 0.0 : U[x]) << " " << (V.empty() ? 0.0 : V[x]) << " " << (W.empty() ? 0.0 : W[x]) << " ";
+            }
+    std::vector<double> Q[4];
+    for (int s = 0; s < 6; ++s) {
+        const SlabGeom &g = m.slab[s];
+        const size_t cnt = g.count(nx, ny, nz);
+        for (int q = 0; q < 4; ++q)
+            if ((rc = fetch(c, c->Q[c->qcur][s][q], cnt, Q[q]))) return rc;
+        const int n = g.n;
+        if (!n) continue;
+        if (g.axis == 0) {
+            for (int ii = 0; ii < n; ++ii)
+                for (int j = 0; j < ny; ++j)
+                    for (int k = 0; k < nz; ++k) {
+                        const size_t x = (size_t)ii + (size_t)n * ((size_t)j + (size_t)ny * k);
+                        for (int q = 0; q < 4; ++q) o << Q[q][x] << " ";
+                    }
+        } else if (g.axis == 1) {
+            for (int jj = 0; jj < n; ++jj)
+                for (int k = 0; k < nz; ++k)
+                    for (int i = 0; i < nx; ++i) {
+                        const size_t x = (size_t)i + (size_t)nx * ((size_t)jj + (size_t)n * k);
+                        for (int q = 0; q < 4; ++q) o << Q[q][x] << " ";
+                    }
+        } else {
+            for (int kk = 0; kk < n; ++kk)
+                for (int i = 0; i < nx; ++i)
+                    for (int j = 0; j < ny; ++j) {
+                        const size_t x = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * kk);
+                        for (int q = 0; q < 4; ++q) o << Q[q][x] << " ";
+                    }
+        }
+    }
+    o << std::endl;
+    if (m.south_porous) {
+        std::vector<double> pf[4];
+        for (int q = 0; q < 4; ++q)
+            if ((rc = fetch(c, c->pore[b + q], pore_n(c), pf[q]))) return rc;
+        restart_stream(o);
+        const int px = m.pnx, py = m.pny, pz = m.pnz;
+        for (int i = 0; i < px; ++i)
+            for (int j = 0; j < py; ++j)
+                for (int k = 0; k < pz; ++k)
+                    for (int q = 0; q < 4; ++q) o << pf[q][(size_t)i + (size_t)px * ((size_t)j + (size_t)py * k)] << " ";
+        o << std::endl;
+    }
+    // extension block (not in the reference, whose reader stops before it): frame number,
+    // y1 gate counter, steps done and the East TDBC histories
+    o.unsetf(std::ios::showpos);
+    o << "LEE-EXT " << c->mf_count << " " << c->counter << " " << c->steps << std::endl;
+    const size_t np = (size_t)nx * nz * 6;
+    std::vector<double> psi(np);
+    HIPCHK(c, hipMemcpyAsync(psi.data(), c->psi, np * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    restart_stream(o);
+    for (size_t q = 0; q < np; ++q) o << psi[q] << " ";
+    o << std::endl;
+    if (!o) return set_err(c, LEE_ERR_IO, std::string("write failed: ") + path);
+    return LEE_OK;
+}
+
+// Reads what lee_save_restart writes (and the reference's files, which end before the
+// extension block: then frame, counter and histories keep their values).  The moving
+// frame of the context must already be the file's (lee_frame_setup).
+int lee_load_restart(lee_ctx *c, const char *path, int *index)
+{
+    if (!c || !path) return set_err(c, LEE_ERR_ARG, "bad arguments");
+    const Model &m = c->m;
+    std::ifstream in(path, std::ios::in | std::ios::binary);
+    if (!in) return set_err(c, LEE_ERR_IO, std::string("cannot open ") + path);
+    int size = 0, rank = 0, idx = 0;
+    if (!(in >> size >> rank >> idx)) return set_err(c, LEE_ERR_IO, "malformed restart header");
+    if (size != 1) return set_err(c, LEE_ERR_ARG, "restart written by " + std::to_string(size) + " ranks; one domain expected");
+    const int nx = m.nx, ny = m.ny, nz = m.nz;
+    auto id = [&](int i, int j, int k) { return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k); };
+    std::vector<double> f[4];
+    for (int q = 0; q < 4; ++q) f[q].assign(air_n(c), 0.0);
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k)
+                for (int q = 0; q < 4; ++q)
+                    if (!(in >> f[q][id(i, j, k)])) return set_err(c, LEE_ERR_IO, "restart file ends inside the fields");
+    double dummy;
+    for (size_t x = 0; x < 3 * air_n(c); ++x)  // the mean flow is rebuilt from the deck
+        if (!(in >> dummy)) return set_err(c, LEE_ERR_IO, "restart file ends inside the mean flow");
+    int rc;
+    for (int s = 0; s < 6; ++s) {
+        const SlabGeom &g = m.slab[s];
+        const int n = g.n;
+        if (!n) continue;
+        const size_t cnt = g.count(nx, ny, nz);
+        std::vector<double> Q[4];
+        for (int q = 0; q < 4; ++q) Q[q].assign(cnt, 0.0);
+        auto rd = [&](size_t x) {
+            for (int q = 0; q < 4; ++q)
+                if (!(in >> Q[q][x])) return false;
+            return true;
+        };
+        bool ok = true;
+        if (g.axis == 0) {
+            for (int ii = 0; ii < n && ok; ++ii)
+                for (int j = 0; j < ny && ok; ++j)
+                    for (int k = 0; k < nz && ok; ++k) ok = rd((size_t)ii + (size_t)n * ((size_t)j + (size_t)ny * k));
+        } else if (g.axis == 1) {
+            for (int jj = 0; jj < n && ok; ++jj)
+                for (int k = 0; k < nz && ok; ++k)
+                    for (int i = 0; i < nx && ok; ++i) ok = rd((size_t)i + (size_t)nx * ((size_t)jj + (size_t)n * k));
+        } else {
+            for (int kk = 0; kk < n && ok; ++kk)
+                for (int i = 0; i < nx && ok; ++i)
+                    for (int j = 0; j < ny && ok; ++j) ok = rd((size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * kk));
+        }
+        if (!ok) return set_err(c, LEE_ERR_IO, "restart file ends inside the PML auxiliaries");
+        for (int q = 0; q < 4; ++q)
+            if ((rc = upload(c, c->Q[c->qcur][s][q], Q[q].data(), cnt))) return rc;
+    }
+    const int b = c->cur * 4;
+    for (int q = 0; q < 4; ++q)
+        if ((rc = upload(c, c->air[b + q], f[q].data(), air_n(c)))) return rc;
+    if (m.south_porous) {
+        const int px = m.pnx, py = m.pny, pz = m.pnz;
+        std::vector<double> pf[4];
+        for (int q = 0; q < 4; ++q) pf[q].assign(pore_n(c), 0.0);
+        for (int i = 0; i < px; ++i)
+            for (int j = 0; j < py; ++j)
+                for (int k = 0; k < pz; ++k)
+                    for (int q = 0; q < 4; ++q)
+                        if (!(in >> pf[q][(size_t)i + (size_t)px * ((size_t)j + (size_t)py * k)]))
+                            return set_err(c, LEE_ERR_IO, "restart file ends inside the porous fields");
+        for (int q = 0; q < 4; ++q)
+            if ((rc = upload(c, c->pore[b + q], pf[q].data(), pore_n(c)))) return rc;
+    }
+    std::string tag;
+    if (in >> tag && tag == "LEE-EXT") {
+        int mf = 0;
+        long long counter = 0, steps = 0;
+        if (!(in >> mf >> counter >> steps)) return set_err(c, LEE_ERR_IO, "malformed extension block");
+        const size_t np = (size_t)nx * nz * 6;
+        std::vector<double> psi(np);
+        for (size_t q = 0; q < np; ++q)
+            if (!(in >> psi[q])) return set_err(c, LEE_ERR_IO, "restart file ends inside the TDBC histories");
+        HIPCHK(c, hipMemcpyAsync(c->psi, psi.data(), np * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->counter = counter;
+        c->steps = steps;
+        if (mf != c->mf_count) return set_err(c, LEE_ERR_STATE, "restart written in frame " + std::to_string(mf) +
+                                                                  ", context is in frame " + std::to_string(c->mf_count));
+    }
+    // the FP32 RK4 state error restarts at zero
+    for (int d = 0; d < 2; ++d)
+        for (int q = 0; q < 4; ++q)
+            if (c->rk_err[d][q]) HIPCHK(c, hipMemsetAsync(c->rk_err[d][q], 0, (d ? pore_n(c) : air_n(c)) * c->esz, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (index) *index = idx;
+    return LEE_OK;
+}
+
+}  // extern "C"
+
+// ---- contour output: airpore::get_data_contour (airpore.cpp:1018-1050) ------------------
+namespace {
+
+int out_count(int n, int stride) { return (n - 1) % stride == 0 ? (n - 1) / stride + 1 : (n - 1) / stride + 2; }
+
+// calculation_2D.cpp:207-213 (out_n = 4) and the subsample of mpi_send_data
+// (calculation_2D.cpp:1030-1105); getOstream (calculation_2D.cpp:1165-1182) prints it.
+template <typename T>
+int contour_zone(lee_ctx *c, std::ofstream &o, const T *p, int nx, int ny, int nz, const std::vector<double> &X,
+                 const std::vector<double> &Y, const std::vector<double> &Z)
+{
+    const int st = 4;
+    const int ox = out_count(nx, st), oy = out_count(ny, st), oz = out_count(nz, st);
+    const size_t n = (size_t)ox * oy * oz;
+    double *d = nullptr;
+    HIPCHK(c, hipMalloc(&d, n * sizeof(double)));
+    launch_subsample<T>(p, nx, ny, nz, st, ox, oy, oz, d, c->stream);
+    std::vector<double> h(n);
+    hipError_t e1 = hipMemcpyAsync(h.data(), d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    hipError_t e2 = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    HIPCHK(c, e1);
+    HIPCHK(c, e2);
+    auto pick = [&](const std::vector<double> &v, int o_, int nn, int q) { return q < o_ - 1 ? v[(size_t)q * st] : v[nn - 1]; };
+    o << "I=" << ox << ",J=" << oy << ",K=" << oz << ",F=POINT" << std::endl;
+    size_t r = 0;
+    for (int k = 0; k < oz; ++k)
+        for (int j = 0; j < oy; ++j)
+            for (int i = 0; i < ox; ++i, ++r)
+                o << pick(X, ox, nx, i) << " " << pick(Y, oy, ny, j) << " " << pick(Z, oz, nz, k) << " " << h[r] << " "
+                  << std::endl;
+    return LEE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Writes dir/ptNNN.dat, NNN = (steps_done + 1) / out_difft as the reference numbers its
+// files (get_data_contour receives n + temp_time + 1 and adds one more), from the pressure of
+// the current time level.  The reference writes to a fixed absolute cluster path.
+int lee_write_contour(lee_ctx *c, const char *dir)
+{
+    if (!c || !dir) return set_err(c, LEE_ERR_ARG, "bad arguments");
+    const Model &m = c->m;
+    if (m.d.out_difft <= 0) return set_err(c, LEE_ERR_ARG, "out_difft must be positive");
+    const long long nn = (c->steps + 1) / m.d.out_difft;
+    char name[64];
+    std::snprintf(name, sizeof name, "/pt%03lld.dat", nn);
+    const std::string path = std::string(dir) + name;
+    std::ofstream o(path.c_str(), std::ios::out | std::ios::trunc | std::ios::binary);
+    if (!o) return set_err(c, LEE_ERR_IO, "cannot write " + path);
+    o.setf(std::ios::scientific, std::ios::floatfield);
+    o.precision(6);
+    const int N = c->mf_count < 0 ? 0 : c->mf_count;
+    std::vector<double> Y(m.ay);
+    for (auto &y : Y) y += N * (m.trail - 1) * m.d.dy;
+    const int b = c->cur * 4 + 3;
+    int rc;
+    if (!m.south_porous) {
+        o << "VARIABLES = \"X\", \"Y\", \"Z\", \"P\"" << std::endl;
+        o << "ZONE T = \"Air Zone\",";
+        rc = c->prec == 64 ? contour_zone<double>(c, o, (const double *)c->air[b], m.nx, m.ny, m.nz, m.ax, Y, m.az)
+                           : contour_zone<float>(c, o, (const float *)c->air[b], m.nx, m.ny, m.nz, m.ax, Y, m.az);
+        if (rc) return rc;
+    } else {
+        // the reference omits the VARIABLES line when the ground is porous
+        o << "ZONE T = \"Air Zone\",";
+        rc = c->prec == 64 ? contour_zone<double>(c, o, (const double *)c->air[b], m.nx, m.ny, m.nz, m.ax, Y, m.az)
+                           : contour_zone<float>(c, o, (const float *)c->air[b], m.nx, m.ny, m.nz, m.ax, Y, m.az);
+        if (rc) return rc;
+        std::vector<double> PY(m.pay);
+        for (auto &y : PY) y += N * (m.trail - 1) * m.d.pore_dy;
+        o << "ZONE T = \"south pore Zone\",";
+        rc = c->prec == 64 ? contour_zone<double>(c, o, (const double *)c->pore[b], m.pnx, m.pny, m.pnz, m.pax, PY, m.paz)
+                           : contour_zone<float>(c, o, (const float *)c->pore[b], m.pnx, m.pny, m.pnz, m.pax, PY, m.paz);
+        if (rc) return rc;
+    }
+    if (!o) return set_err(c, LEE_ERR_IO, "write failed: " + path);
     return LEE_OK;
 }
 

@@ -857,6 +857,18 @@ __global__ void k_sample(const T *p, const long long *idx, int n, double *out)
     out[r] = v;
 }
 
+// Contour subsample (calculation_2D::mpi_send_data, calculation_2D.cpp:1030-1105): the nodes
+// with index % stride == 0 plus the last one on each axis, out[io + no_x*(jo + no_y*ko)].
+template <typename T>
+__global__ void k_subsample(const T *p, int nx, int ny, int nz, int stride, int ox, int oy, int oz, double *out)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)ox * oy * oz) return;
+    const int io = (int)(t % ox), jo = (int)((t / ox) % oy), ko = (int)(t / ((long long)ox * oy));
+    const int i = io < ox - 1 ? io * stride : nx - 1, j = jo < oy - 1 ? jo * stride : ny - 1, k = ko < oz - 1 ? ko * stride : nz - 1;
+    out[t] = (double)p[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)];
+}
+
 template <typename T>
 __global__ void k_finite(const T *f, size_t n, int *flag)
 {
@@ -974,6 +986,12 @@ template <typename T> void launch_sample(const T *p, const long long *idx, int n
 {
     if (n > 0) hipLaunchKernelGGL((k_sample<T>), dim3((n + 63) / 64), dim3(64), 0, s, p, idx, n, out);
 }
+template <typename T> void launch_subsample(const T *p, int nx, int ny, int nz, int stride, int ox, int oy, int oz,
+                                             double *out, hipStream_t s)
+{
+    const long long n = (long long)ox * oy * oz;
+    if (n > 0) hipLaunchKernelGGL((k_subsample<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, nx, ny, nz, stride, ox, oy, oz, out);
+}
 template <typename T> void launch_finite_check(const T *f, size_t n, int *flag, hipStream_t s)
 {
     hipLaunchKernelGGL((k_finite<T>), dim3(flat_blocks(n)), dim3(256), 0, s, f, n, flag);
@@ -997,7 +1015,8 @@ void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s)
     template void launch_shift<T>(T *, const T *, int, int, int, int, int, int, hipStream_t);                \
     template void launch_copy_plane_j<T>(T *, int, int, int, hipStream_t);                                   \
     template void launch_sample<T>(const T *, const long long *, int, double *, hipStream_t);                \
-    template void launch_finite_check<T>(const T *, size_t, int *, hipStream_t);
+    template void launch_finite_check<T>(const T *, size_t, int *, hipStream_t);                             \
+    template void launch_subsample<T>(const T *, int, int, int, int, int, int, int, double *, hipStream_t);
 LEE_INST(double)
 LEE_INST(float)
 #undef LEE_INST

@@ -79,11 +79,11 @@ INT_DECK, INT_RK4 = 0, 1
 
 
 def make_options(deck: Optional["Deck"] = None, precision: int = 64, receivers=None, health_every: int = 0,
-                 device: int = 0, path: int = 0, integrator: Optional[int] = None) -> LeeOptions:
+                 device: int = 0, path: int = 0, integrator: Optional[int] = None, use_graph: int = 0) -> LeeOptions:
     """``lee_options`` for a deck: its extensions (integrator, hills, log wind) plus the run
     settings.  The receiver array is kept alive on the returned struct."""
     o = LeeOptions()
-    o.precision, o.health_every, o.device, o.path = precision, health_every, device, path
+    o.precision, o.health_every, o.device, o.path, o.use_graph = precision, health_every, device, path, use_graph
     if receivers is not None:
         rec = np.ascontiguousarray(receivers, dtype=np.float64).reshape(-1)
         o._rec = rec
@@ -384,6 +384,32 @@ def config_c1(time_domain: int = 10000, reference_scheme: bool = False) -> Deck:
         dk.hills = [(0.0, 30.0, 2.0, 5.0)]
         dk.velocity_method, dk.wind_ustar, dk.wind_z0 = 7, 0.4, 0.02
     return dk
+
+
+def config_c2(time_domain: int = 2000, seed: int = 20261202) -> Deck:
+    """Precision study: 768x384x192 air points at 0.04 m (the spacing is not specified; C1's
+    is used) over a 24-point porous layer with the C0 ground, terrain-following over a random
+    smooth terrain -- a sum of 6 Gaussians with heights U[0, 1.5] m and widths U[3, 10] m at
+    centres uniform over the footprint, drawn from numpy's PCG64 with the given seed --
+    20-point PML on the sides and top, no wind, no moving frame, Gaussian pulse of half-width
+    3 dx at 0.5 m, RK4 at CFL 0.5, 2000 steps (the step count is not specified)."""
+    d = 0.04
+    dk = box(768, 384, 192, d, pml=20, bc=(2, 3, 2, 2, 2, 2), porous_cells=24 + 2, time_domain=time_domain,
+             gauss_width=2 * 3 * d, src=(0.0, 3.0, 0.5), rcv=(0.0, 6.0, 1.2), fft_m=14, pml_y_time=0)
+    dk.pore_q, dk.pore_porosity, dk.pore_resistivity = 3.0 ** 0.5, 0.3, 2.0e5
+    rng = np.random.default_rng(seed)
+    x0, x1, y0, y1 = dk.air_x[0], dk.air_x[-1], dk.air_y[0], dk.air_y[-1]
+    hills = []
+    for _ in range(6):
+        hx, hy = rng.uniform(x0, x1), rng.uniform(y0, y1)
+        hills.append((float(hx), float(hy), float(rng.uniform(0.0, 1.5)), float(rng.uniform(3.0, 10.0))))
+    dk.integrator, dk.hills = INT_RK4, hills
+    return dk
+
+
+def receivers_c2():
+    """Thirty-two receivers on a line at 1.2 m along the propagation axis (not specified)."""
+    return [(0.0, 4.0 + 0.35 * r, 1.2) for r in range(32)]
 
 
 def receivers_c1():

@@ -20,7 +20,12 @@ of IL cancels in the difference).
 runs when a JSON of them is given.  `cpu_baseline`: the FP64 C oracle of the same step on one
 host core over a bounded sample (the first step(s) of the same workload).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-fb] [--traffic-json F]
+`--workload c2` runs the precision study instead: 768x384x192 at 0.04 m over a 24-point
+porous layer and a seeded random terrain of 6 Gaussian hills, RK4, 2000 steps, FP32 against
+FP64 (deck.config_c2).
+
+    python bench.py [--gpus 1] [--workload c1|c2] [--steps K] [--warmup W] [--no-cpu] [--no-fb]
+                    [--traffic-json F]
 """
 from __future__ import annotations
 
@@ -128,7 +133,8 @@ def kernel_table(kernels):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--workload", choices=("c1", "c2"), default="c1")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (C1: 10000, C2: 2000)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fb", action="store_true", help="skip the reference-scheme (FB_p_v, flat) lines")
@@ -141,15 +147,28 @@ def main():
               "GPU's HBM and no multi-GPU path is built", file=sys.stderr)
         sys.exit(2)
     lee = importlib.import_module("3d_mpi_test_amd")
+    if a.steps is None:
+        a.steps = 10000 if a.workload == "c1" else 2000
     total = a.warmup + a.steps
-    deck = lee.deck.config_c1(time_domain=total)
-    rec = lee.deck.receivers_c1()
+    if a.workload == "c1":
+        deck = lee.deck.config_c1(time_domain=total)
+        rec = lee.deck.receivers_c1()
+        workload = ("C1: 512x256x128 air at 0.04 m, terrain-following over a Gaussian hill (2 m, sigma 5 m), "
+                    "16-point porous ground (200 kPa s/m^2, 0.3, 3), 20-pt PML sides+top, log wind u*=0.4 m/s "
+                    "z0=0.02 m, moving frame, 32 receivers at 1.2 m, RK4 CFL 0.5")
+    else:
+        deck = lee.deck.config_c2(time_domain=total)
+        rec = lee.deck.receivers_c2()
+        workload = ("C2: 768x384x192 air at 0.04 m, terrain-following over 6 seeded random Gaussian hills "
+                    "(heights U[0,1.5] m, widths U[3,10] m), 24-point porous ground, 20-pt PML sides+top, no wind, "
+                    "32 receivers at 1.2 m, RK4 CFL 0.5")
+    wl = a.workload.upper()
     res = {}
     for p in [int(x) for x in a.precisions.split(",")]:
-        print(f"[bench] C1 RK4 fp{p}: {a.warmup} warm-up + {a.steps} timed steps", file=sys.stderr, flush=True)
+        print(f"[bench] {wl} RK4 fp{p}: {a.warmup} warm-up + {a.steps} timed steps", file=sys.stderr, flush=True)
         res[p] = run_gpu(lee, deck, rec, p, a.steps, a.warmup)
         r = res[p]
-        print(f"[bench] C1 RK4 fp{p}: {r['points'] * a.steps / r['wall'] / 1e9:.2f} G pt-updates/s "
+        print(f"[bench] {wl} RK4 fp{p}: {r['points'] * a.steps / r['wall'] / 1e9:.2f} G pt-updates/s "
               f"({1e3 * r['wall'] / a.steps:.3f} ms/step, {r['frames']} frames)", file=sys.stderr, flush=True)
     head = 32 if 32 in res else list(res)[0]
     r = res[head]
@@ -173,7 +192,7 @@ def main():
         print(f"[bench] FP32 vs FP64: rel-L2 {rl2:.3e}, max |dIL| {dil:.4f} dB over 50-1000 Hz", file=sys.stderr,
               flush=True)
     fb = {}
-    if not a.no_fb:
+    if not a.no_fb and a.workload == "c1":
         # the reference's own scheme on the same grid: FB_p_v, flat ground, no wind
         fdeck = lee.deck.config_c1(time_domain=220, reference_scheme=True)
         for p in (32, 64):
@@ -187,7 +206,7 @@ def main():
         print("[bench] cpu baseline (oracle, one core)", file=sys.stderr, flush=True)
         c = cpu_baseline(deck, rec)
         cpu = {"value": round(c["value"], 1), "unit": "grid-pt updates/s", "cores": 1, "kind": "port",
-               "sample": f"C1 (RK4, terrain, log wind, porous ground) FP64, first {c['steps']} step(s) "
+               "sample": f"{wl} (RK4, terrain, porous ground) FP64, first {c['steps']} step(s) "
                          f"({c['seconds']:.1f} s), single thread"}
     extra = {}
     for p, rr in res.items():
@@ -209,10 +228,8 @@ def main():
         "vs_baseline": None,
         "dtype": f"f{head}",
         "data": "synthetic",
-        "config": {"workload": "C1: 512x256x128 air at 0.04 m, terrain-following over a Gaussian hill (2 m, "
-                               "sigma 5 m), 16-point porous ground (200 kPa s/m^2, 0.3, 3), 20-pt PML sides+top, "
-                               "log wind u*=0.4 m/s z0=0.02 m, moving frame, 32 receivers at 1.2 m, RK4 CFL 0.5",
-                   "air_points": 512 * 256 * 128, "porous_points": r["points"] - 512 * 256 * 128,
+        "config": {"workload": workload,
+                   "air_points": int(np.prod(deck.shape)), "porous_points": r["points"] - int(np.prod(deck.shape)),
                    "frames_in_run": r["frames"], "integrator": "RK4 (FP32: compensated accumulation)",
                    "parallelism": "single GPU"},
         "roofline": roof,

@@ -210,6 +210,28 @@ int  lee_kernel_stats(const lee_ctx *ctx, lee_kstat *out, int cap);
  * copy roofline for bench.py. */
 double lee_copy_bench(lee_ctx *ctx, long long n_words, int repeats);
 
+/* Checkpoint in the reference's restart text format (airpore::Save_restart airpore.cpp:664-700,
+ * airpore::save_restartfile 783-796, calculation_2D::save_restart_cal calculation_2D.cpp:497-520,
+ * air::save_restart_air air.cpp:1305-1390): a "1 0 index" line, then u v w p per node (i, j, k
+ * loops), the mean flow, the PML auxiliaries, the porous fields, all as 10-digit scientific
+ * text.  A trailing block the reference's reader never reaches adds what the reference does
+ * not save: frame number, PML y1 counter, steps done and the East TDBC histories.  The file
+ * name and directory (myfile/ctNNN_rank.dat in the reference) are the caller's. */
+int  lee_save_restart(lee_ctx *ctx, const char *path, int index);
+/* Read such a file into the context (airpore::Read_restart airpore.cpp:799-862): the fields,
+ * auxiliaries and (if present) the extension block; the mean flow is rebuilt from the deck.
+ * *index receives the step index of the header.  Call lee_frame_setup for the file's frame
+ * first; a mismatch is LEE_ERR_STATE. */
+int  lee_load_restart(lee_ctx *ctx, const char *path, int *index);
+
+/* Contour file of the current pressure (airpore::get_data_contour airpore.cpp:1018-1050,
+ * calculation_2D::mpi_send_data calculation_2D.cpp:1030-1105, getOstream 1165-1182): every
+ * 4th node plus the last on each axis, Tecplot POINT zones "X Y Z P" at scientific precision 6
+ * (an air zone, and a porous zone when South is porous), written to dir/ptNNN.dat with
+ * NNN = (steps done + 1) / out_difft.  The reference calls it when the steps done are a
+ * multiple of out_difft (airpore.cpp:615-618) and writes to a fixed absolute path. */
+int  lee_write_contour(lee_ctx *ctx, const char *dir);
+
 /* Output writers of the drop-in file contract.
  * ./solution/pty%04d.dat: one %.6e sample per line (airpore.cpp:468-531, 577).
  * ./solution/p_f2_%04d.dat: zero-padded radix-2 FFT, "f |X|" at setprecision(10)
