@@ -64,6 +64,7 @@ struct lee_ctx {
     void *rkq_sb[6][4] = {}, *rkq_acc[6][4] = {};
     // terrain tables (sh, hx, hy per (i,j); tk per k) of the air [0] and porous [1] grids
     void *ter[2][4] = {};
+    unsigned char *omask = nullptr;  // embedded obstacle material ids (FB)
     // captured step chunks (use_graph): keyed by the buffer roles they start from
     struct GraphEntry { int n, cur, qcur, gate_on; void *psi; hipGraphExec_t exec; };
     std::vector<GraphEntry> graphs;
@@ -245,6 +246,9 @@ template <typename T> AirArgs<T> air_args(lee_ctx *c, int J1, int y1on)
     }
     a.y1on = y1on;
     a.south_pore = m.south_porous && m.iface_copy;
+    a.omask = c->omask;
+    for (int q = 0; q < 6; ++q) a.ob[q] = m.ob[q];
+    for (int q = 0; q < 3; ++q) { a.ogama[q] = (T)m.o_gama[q]; a.oinvb[q] = (T)m.o_invbeta[q]; a.oeff[q] = (T)m.o_eff[q]; }
     a.pore_pnn = m.south_porous ? pore_grid<T>(c).pnn : nullptr;
     a.pore_nz = m.pnz;
     return a;
@@ -644,7 +648,8 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
     C->esz = o.precision == 64 ? 8 : 4;
     {
         const Model &mm = C->m;
-        const bool can = fused_fits<double>(mm.nx) && !(mm.south_porous && !mm.iface_copy) && !mm.terrain && !mm.rk4;
+        const bool can = fused_fits<double>(mm.nx) && !(mm.south_porous && !mm.iface_copy) && !mm.terrain && !mm.rk4 &&
+                         !mm.obstacle;
         if (o.path == LEE_PATH_FUSED && !can)
             return set_err(nullptr, LEE_ERR_UNSUPPORTED,
                            "the fused path runs the FB_p_v step on a separable grid and, with a porous South face, "
@@ -712,6 +717,11 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
                 if ((rc = dalloc(C, &C->ter[d][q], n2 * es))) return rc;
             if ((rc = dalloc(C, &C->ter[d][3], nk * es))) return rc;
         }
+    }
+    if (m.obstacle) {
+        if ((rc = dalloc(C, (void **)&C->omask, m.omask.size()))) return rc;
+        HIPCHK(C, hipMemcpyAsync(C->omask, m.omask.data(), m.omask.size(), hipMemcpyHostToDevice, C->stream));
+        HIPCHK(C, hipStreamSynchronize(C->stream));
     }
     C->scratch_bytes = std::max<size_t>(qmax * es, 8);
     if ((rc = dalloc(C, &C->scratch, C->scratch_bytes))) return rc;

@@ -146,13 +146,13 @@ int lee_deck_load(const char *dir, lee_deck *out)
     std::string restart_file;
     in >> k.restart >> k.restart_out >> restart_file;
     skip(); skip();
-    double coef;
-    int points;
+    // embedded porous obstacle (airpore.cpp:270-299): curve line, hill / canopy / trunk
+    // materials, bounding box
     in >> k.hill_judge;
-    for (int q = 0; q < 12; ++q) in >> coef;
-    in >> points; skip();
-    for (int q = 0; q < 3; ++q) { double a, b, c; in >> a >> b >> c; skip(); }
-    for (int q = 0; q < 6; ++q) in >> coef;
+    for (int q = 0; q < 12; ++q) in >> k.hill_coef[q];
+    in >> k.hill_points; skip();
+    for (int q = 0; q < 3; ++q) { in >> k.hill_mat[q][0] >> k.hill_mat[q][1] >> k.hill_mat[q][2]; skip(); }
+    for (int q = 0; q < 6; ++q) in >> k.hill_box[q];
     skip();
     skip(); skip();
     if (k.bc[LEE_FACE_S] == LEE_BC_POROUS) {

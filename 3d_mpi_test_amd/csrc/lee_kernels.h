@@ -46,6 +46,11 @@ template <typename T> struct AirArgs {
     int south_pore;
     const T *pore_pnn;
     int pore_nz;
+    // embedded porous obstacle (FB step only): material id per node of the index box ob[],
+    // NULL without obstacle; porous coefficients of the materials 1 (canopy) and 2 (trunk)
+    const unsigned char *omask;
+    int ob[6];
+    T ogama[3], oinvb[3], oeff[3];
 };
 
 template <typename T> struct PoreArgs {

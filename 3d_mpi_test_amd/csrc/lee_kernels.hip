@@ -290,6 +290,14 @@ __device__ __forceinline__ T air_dp(const AirArgs<T> &a, int i, int j, int k)
     return r;
 }
 
+// Material id of the embedded porous obstacle at (i,j,k), 0 outside it.
+template <typename T> __device__ __forceinline__ int obstacle_id(const AirArgs<T> &a, int i, int j, int k)
+{
+    if (!a.omask || i < a.ob[0] || i > a.ob[1] || j < a.ob[2] || j > a.ob[3] || k < a.ob[4] || k > a.ob[5]) return 0;
+    const int ni = a.ob[1] - a.ob[0] + 1, nj = a.ob[3] - a.ob[2] + 1;
+    return a.omask[(size_t)(i - a.ob[0]) + (size_t)ni * ((size_t)(j - a.ob[2]) + (size_t)nj * (k - a.ob[4]))];
+}
+
 // FB pressure update at one interior point: calculation_2D::Cal_pressure
 // (calculation_2D.cpp:598-608) and Update_PML_Qp (air.cpp:2388-2439).  Writes p^{n+1} and
 // the point's Q_p and returns p^{n+1}.
@@ -302,7 +310,18 @@ __device__ __forceinline__ T air_p_point(const AirArgs<T> &a, int i, int j, int 
     T qp_old[6];
 #pragma unroll
     for (int s = 0; s < 6; ++s) qp_old[s] = own.in[s] ? a.Qo[s][3][own.idx[s]] : T(0);
-    const T pnew = g.pn[id] + air_dp<T, WIND ? 1 : 0, TERRAIN>(a, i, j, k);
+    T r = air_dp<T, WIND ? 1 : 0, TERRAIN>(a, i, j, k);
+    if (const int mo = obstacle_id(a, i, j, k)) {
+        // inside the obstacle the porous-medium continuity equation replaces the increment
+        // (air.cpp:2294-2318)
+        T mX, mY, mZ;
+        metric(g, i, j, k, mX, mY, mZ);
+        const size_t sxy = (size_t)g.nx * g.ny;
+        const T div = (((mX * (g.un[id + 1] - g.un[id])) * g.dxi) + ((mY * (g.vn[id + g.nx] - g.vn[id])) * g.dyi)) +
+                      ((mZ * (g.wn[id + sxy] - g.wn[id])) * g.dzi);
+        r = (-a.ogama[mo]) * div;
+    }
+    const T pnew = g.pn[id] + r;
     g.pnn[id] = pnew;
     // Q_p += dt*p of this point; the value is final, face conditions only touch boundary nodes
 #pragma unroll
@@ -516,6 +535,20 @@ __device__ __forceinline__ void air_v_point(const AirArgs<T> &a, int i, int j, i
     T du, dv, dw;
     air_duvw<T, WIND ? 1 : 0, TERRAIN>(a, i, j, k, du, dv, dw);
     const T dt = g.dt;
+    if (const int mo = obstacle_id(a, i, j, k)) {
+        // inside the obstacle the porous-medium momentum equation replaces the increments
+        // (air.cpp:1907-1945)
+        T mX, mY, mZ;
+        metric(g, i, j, k, mX, mY, mZ);
+        const T *p = g.pnn;
+        const T pc = p[id];
+        const size_t sxy = (size_t)g.nx * g.ny;
+        const T ib = a.oinvb[mo], f = ((T(-1.0) * ib) * dt) * a.oeff[mo];
+        const T uc = g.un[id], vc = g.vn[id], wc = g.wn[id];
+        du = (f * ((mX * (pc - p[id - 1])) * g.dxi) + uc * ib) - uc;
+        dv = (f * ((mY * (pc - p[id - g.nx])) * g.dyi) + vc * ib) - vc;
+        dw = (f * ((mZ * (pc - p[id - sxy])) * g.dzi) + wc * ib) - wc;
+    }
     const T un = g.un[id] + du, vn = g.vn[id] + dv, wn = g.wn[id] + dw;
     g.unn[id] = un;
     g.vnn[id] = vn;

@@ -110,7 +110,10 @@ std::string Model::build(const lee_deck &deck, const lee_options &o, int &code)
     int sb = d.bc[LEE_FACE_S];
     if (sb != LEE_BC_RIGID && sb != LEE_BC_ABSORBING && sb != LEE_BC_POROUS)
         return "South boundary must be rigid, absorbing or porous";
-    if (d.hill_judge != 0) return "embedded porous obstacles (hill/canopy/trunk) are not supported";
+    if (d.hill_judge < 0 || d.hill_judge > 5)
+        return "curve_judge " + std::to_string(d.hill_judge) + " has no mask in the reference (air.cpp:1018-1101)";
+    obstacle = d.hill_judge != 0;
+    if (obstacle && (rk4 || terrain)) return "embedded porous obstacles run with the FB_p_v step on a separable grid";
     if (d.restart != 0) return "restart is not supported";
     if (d.velocity_method == 3 || d.velocity_method == 6)
         return "wind methods 3 and 6 read uninitialised members in the reference (air.cpp:44-69)";
@@ -275,6 +278,10 @@ std::string Model::build(const lee_deck &deck, const lee_options &o, int &code)
         ri[r] = ii; rj[r] = jj; rk[r] = kk;
     }
     code = LEE_OK;
+    if (obstacle) {
+        std::string e = obstacle_mask();
+        if (!e.empty()) { code = LEE_ERR_ARG; return e; }
+    }
     return "";
 }
 
@@ -322,6 +329,95 @@ long long Model::receiver_index(int r, int N) const
     if (jl > 0 && jl < ny - 1 && ii > 0 && ii < nx - 1 && kk > 0 && kk < nz - 1)
         return (long long)ii + (long long)nx * ((long long)jl + (long long)ny * kk);
     return -1;
+}
+
+// air::cal_hill_var (air.cpp:938-1105): material coefficients of the hill, canopy and trunk
+// (air.cpp:958-985 with airpore.cpp:282-293), the index box of the obstacle (first / last
+// node within a spacing/8 of the input bounds, air.cpp:987-1005) and the material id per
+// node of the box for the five curve kinds (air.cpp:1018-1101).
+std::string Model::obstacle_mask()
+{
+    const double Aav_ = std::pow(d.c, 2) / d.gamma / d.Pav;
+    for (int m = 0; m < 3; ++m) {
+        const double q = d.hill_mat[m][0], por = d.hill_mat[m][1], res = d.hill_mat[m][2];
+        const double effd = q * q / por / Aav_;
+        const double Kp = 1.0 / d.gamma / d.Pav;
+        o_eff[m] = 1.0 / effd;
+        const double beta = res * d.dt * o_eff[m];
+        o_gama[m] = d.dt / Kp / por;
+        o_invbeta[m] = 1.0 / (1.0 + beta);
+    }
+    auto find = [](const std::vector<double> &c, double lo, double hi, double h, int &a, int &b) {
+        a = 0; b = 0;
+        for (int i = 0; i < (int)c.size(); ++i) {
+            if (std::fabs(c[i] - lo) <= h / 8.0) a = i;
+            if (std::fabs(c[i] - hi) <= h / 8.0) { b = i; break; }
+        }
+    };
+    const double *B = d.hill_box;
+    find(ax, B[0], B[1], d.dx, ob[0], ob[1]);
+    find(ay, B[2], B[3], d.dy, ob[2], ob[3]);
+    find(az, B[4], B[5], d.dz, ob[4], ob[5]);
+    const int ni = ob[1] - ob[0] + 1, nj = ob[3] - ob[2] + 1, nk = ob[5] - ob[4] + 1;
+    if (ni <= 0 || nj <= 0 || nk <= 0) return "the obstacle bounds select an empty index box";
+    omask.assign((size_t)ni * nj * nk, 0);
+    const double *c = d.hill_coef;  // a b c d e f g h i j k l
+    const double a_ = c[0], b_ = c[1], c_ = c[2], d_ = c[3], e_ = c[4], f_ = c[5], g_ = c[6], h_ = c[7], i_ = c[8], j_ = c[9],
+                 k_ = c[10];
+    auto at = [&](int i, int j, int k) -> unsigned char & {
+        return omask[(size_t)(i - ob[0]) + (size_t)ni * ((size_t)(j - ob[2]) + (size_t)nj * (k - ob[4]))];
+    };
+    for (int i = ob[0]; i <= ob[1]; ++i)
+        for (int j = ob[2]; j <= ob[3]; ++j)
+            for (int k = ob[4]; k <= ob[5]; ++k) {
+                const double X = ax[i], Y = ay[j], Z = az[k];
+                unsigned char &m = at(i, j, k);
+                switch (d.hill_judge) {
+                case 1: {  // sphere
+                    double len = std::pow(X - c_, 2) + std::pow(Y - d_, 2) + std::pow(Z - e_, 2);
+                    if (std::sqrt(len) <= b_) m = 1;
+                    break;
+                }
+                case 2:  // box
+                    if (X >= a_ && X <= b_ && Y >= c_ && Y <= d_ && Z >= e_ && Z <= f_) m = 1;
+                    break;
+                case 3:  // three rows of vertical cylinders along y
+                    for (int jj = 0; jj < d.hill_points; ++jj) {
+                        double len = std::pow(X - c_, 2) + std::pow(Y - (d_ + jj * b_), 2);
+                        if (std::sqrt(len) <= a_ && Z <= e_) m = 1;
+                        len = std::pow(X - f_, 2) + std::pow(Y - (g_ + jj * b_), 2);
+                        if (std::sqrt(len) <= a_ && Z <= h_) m = 1;
+                        len = std::pow(X - i_, 2) + std::pow(Y - (j_ + jj * b_), 2);
+                        if (std::sqrt(len) <= a_ && Z <= k_) m = 1;
+                    }
+                    break;
+                case 4:  // array of vertical cylinders
+                    for (int ii = 0; ii < g_; ++ii)
+                        for (int jj = 0; jj < f_; ++jj) {
+                            double len = std::pow(X - (b_ + ii * e_), 2) + std::pow(Y - (c_ + jj * d_), 2);
+                            if (std::sqrt(len) <= a_) m = 1;
+                        }
+                    break;
+                case 5: {  // a tree: conical canopy (1) on a trunk (2)
+                    double len = std::pow(X - (0.0 + 0 * 2.0), 2) + std::pow(Y - (3.0 + 0 * 2.0), 2);
+                    double hgt = -25 / (9 - 1e-50) * std::sqrt(len) + 3.0;
+                    if (std::sqrt(len) <= 0.9 && Z <= hgt && Z >= 0.5) m = 1;
+                    if (std::sqrt(len) <= 0.1 && Z <= 1.5) m = 2;
+                    break;
+                }
+                default: break;
+                }
+            }
+    if (d.hill_judge == 5)
+        // the reference's second trunk pass runs with the loop counters left at 1
+        // (air.cpp:1090-1100): trunk centre (2, 5)
+        for (int i = ob[0]; i <= ob[1]; ++i)
+            for (int j = ob[2]; j <= ob[3]; ++j)
+                for (int k = ob[4]; k <= ob[5]; ++k) {
+                    double len = std::pow(ax[i] - (0.0 + 1 * 2.0), 2) + std::pow(ay[j] - (3.0 + 1 * 2.0), 2);
+                    if (std::sqrt(len) <= 0.1 && az[k] <= 1.5) at(i, j, k) = 2;
+                }
+    return std::string();
 }
 
 double Model::hill_height(double x, double y) const

@@ -63,6 +63,13 @@ struct Model {
     bool south_porous = false, east_tdbc = false, iface_copy = true, has_wind = false;
     // extensions: RK4, terrain-following grid, wind along y only (velocity_method 7)
     bool rk4 = false, terrain = false, wind_v_only = false;
+    // embedded porous obstacle (air::cal_hill_var, air.cpp:938-1105): material id per node of
+    // the box [ob[0], ob[1]] x [ob[2], ob[3]] x [ob[4], ob[5]] (global indices, i fastest);
+    // id 1 = canopy, 2 = trunk (their coefficients at index 1, 2; index 0 is the hill's)
+    bool obstacle = false;
+    int ob[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<unsigned char> omask;
+    double o_gama[3] = {}, o_invbeta[3] = {}, o_eff[3] = {};
     // PML (air.cpp:509-937)
     int judge[3] = {0, 0, 0};
     int I1 = 0, I2 = 0, J1 = 0, J2 = 0, J3 = 0, K1 = 0, K2 = 0;
@@ -86,6 +93,7 @@ struct Model {
     // LEE_ERR_UNSUPPORTED (a path the reference never implemented) or LEE_ERR_ARG.
     std::string build(const lee_deck &deck, const lee_options &o, int &code);
     void frame_tables(int N, FrameTables &t) const;
+    std::string obstacle_mask();
     // terrain height of the hills at world (x, y)
     double hill_height(double x, double y) const;
     void terrain_tables(const std::vector<double> &X, const std::vector<double> &Y, const std::vector<double> &Z,

@@ -46,6 +46,10 @@ class LeeDeck(ctypes.Structure):
         ("mpi_dims", ctypes.c_int * 3),
         ("restart", ctypes.c_int), ("restart_out", ctypes.c_int),
         ("hill_judge", ctypes.c_int),
+        ("hill_coef", ctypes.c_double * 12),
+        ("hill_points", ctypes.c_int),
+        ("hill_mat", (ctypes.c_double * 3) * 3),
+        ("hill_box", ctypes.c_double * 6),
         ("pore_dx", ctypes.c_double), ("pore_dy", ctypes.c_double), ("pore_dz", ctypes.c_double),
         ("air_hdr", ctypes.c_int * 10),
         ("air_x", c_double_p), ("air_y", c_double_p), ("air_z", c_double_p),
@@ -160,6 +164,14 @@ class Deck:
     restart_out: int = 5000
     restart_infile: str = "ct003"
     hill_judge: int = 0
+    # embedded porous obstacle: curve coefficients a..l, curve points, hill / canopy / trunk
+    # materials (q, porosity, resistivity) and bounding box (the shipped deck's values)
+    hill_coef: Sequence[float] = (-1.0, 1.0, 0.0, 2.5, -0.2, 0.0, 3.0, 0.0, 0.0, 0.0, 0.0, 0.0)
+    hill_points: int = 0
+    hill_mat: Sequence[Sequence[float]] = ((1.790799820223793, 0.8769089596793096, 5.137565e4),
+                                           (1.790799820223793, 0.8769089596793096, 5.137565e4),
+                                           (0.52, 0.5712, 8.0e4))
+    hill_box: Sequence[float] = (-1.0, 1.0, 0.0, 2.5, -0.2, 0.0)
     pore_dx: float = 0.005
     pore_dy: float = 0.005
     pore_dz: float = 0.005
@@ -227,6 +239,14 @@ class Deck:
         d.pore_q, d.pore_porosity, d.pore_resistivity = self.pore_q, self.pore_porosity, self.pore_resistivity
         d.dt, d.dx, d.dy, d.dz = self.dt, self.dx, self.dy, self.dz
         d.restart, d.restart_out, d.hill_judge = self.restart, self.restart_out, self.hill_judge
+        for q in range(12):
+            d.hill_coef[q] = float(self.hill_coef[q])
+        d.hill_points = int(self.hill_points)
+        for a in range(3):
+            for b in range(3):
+                d.hill_mat[a][b] = float(self.hill_mat[a][b])
+        for q in range(6):
+            d.hill_box[q] = float(self.hill_box[q])
         d.pore_dx, d.pore_dy, d.pore_dz = self.pore_dx, self.pore_dy, self.pore_dz
         for q, v in enumerate(self.air_hdr()):
             d.air_hdr[q] = v
@@ -272,12 +292,12 @@ class Deck:
             "\toutput.data;",
             f"\t{self.restart} {self.restart_out} {self.restart_infile}",
             "",
-            f"\t{self.hill_judge} -1.0 1.0 0.0 2.5 -0.2 0.0 3.0 0.0 0.0 0 0 0 0 //Curve",
-            "\t1.79 0.877 5.1e+04 //hill.q;porosity;resistivity;",
-            "\t1.79 0.877 5.1e+04 //canopy.q;porosity;resistivity;",
-            "\t0.52 0.5712 8.0e4  //trunk.q;porosity;resistivity;",
+            f"\t{self.hill_judge} " + " ".join(f(x) for x in self.hill_coef) + f" {int(self.hill_points)} //Curve",
+            "\t" + " ".join(f(x) for x in self.hill_mat[0]) + " //hill.q;porosity;resistivity;",
+            "\t" + " ".join(f(x) for x in self.hill_mat[1]) + " //canopy.q;porosity;resistivity;",
+            "\t" + " ".join(f(x) for x in self.hill_mat[2]) + " //trunk.q;porosity;resistivity;",
             "",
-            "\t-1.0 1.0 0.0 2.5 -0.2 0.0 //xstar,xstop,ystar,ystop,zstar and zstop;",
+            "\t" + " ".join(f(x) for x in self.hill_box) + " //xstar,xstop,ystar,ystop,zstar and zstop;",
             "",
             "\tporous_media;",
             f"\t{f(self.pore_dx)} {f(self.pore_dy)} {f(self.pore_dz)} //SouthStep.diff_x;diff_y;diff_z;",

@@ -70,7 +70,11 @@ typedef struct lee_deck {
     double dx, dy, dz;                           /* L17 AirStep.diff_x/y/z */
     int    mpi_dims[3];                          /* L19 (ignored: one GPU holds the domain) */
     int    restart, restart_out;                 /* L22 (restart is not on the hot path) */
-    int    hill_judge;                           /* L24 first field (embedded obstacles) */
+    int    hill_judge;                           /* L24 curve_judge: embedded porous obstacle 0..5 */
+    double hill_coef[12];                        /* L24 curve_coefa .. curve_coefl */
+    int    hill_points;                          /* L24 curve_points */
+    double hill_mat[3][3];                       /* L25-27 hill, canopy, trunk: q (Cs), porosity, resistivity */
+    double hill_box[6];                          /* L29 xstar xstop ystar ystop zstar zstop */
     double pore_dx, pore_dy, pore_dz;            /* L32 SouthStep (read only if South = porous) */
     /* coordi_air1.dat header: IMAX IMAX1 IMAX2 JMAX JMAX1 JMAX2 KMAX KMAX1 KMAX2 IJKMAX */
     int    air_hdr[10];

@@ -39,6 +39,13 @@ def test_parser_reads_the_shipped_deck(lee):
         assert d.dt == 4e-6 and (d.dx, d.dy, d.dz) == (0.005, 0.005, 0.005)
         assert list(d.mpi_dims) == [2, 2, 2]
         assert (d.restart, d.restart_out, d.hill_judge) == (0, 5000, 0)
+        # obstacle lines 24-29 (airpore.cpp:270-299)
+        assert list(d.hill_coef) == [-1.0, 1.0, 0.0, 2.5, -0.2, 0.0, 3.0, 0.0, 0.0, 0.0, 0.0, 0.0]
+        assert d.hill_points == 0
+        assert [list(r) for r in d.hill_mat] == [[1.790799820223793, 0.8769089596793096, 5.137565e4],
+                                                 [1.790799820223793, 0.8769089596793096, 5.137565e4],
+                                                 [0.52, 0.5712, 8.0e4]]
+        assert list(d.hill_box) == [-1.0, 1.0, 0.0, 2.5, -0.2, 0.0]
         assert list(d.air_hdr) == [141, 41, 41, 201, 41, 0, 441, 0, 41, 201]
         x = np.ctypeslib.as_array(d.air_x, shape=(141,))
         y = np.ctypeslib.as_array(d.air_y, shape=(201,))
@@ -64,7 +71,7 @@ def test_generated_deck_round_trips_through_the_parser(lee):
                     continue
                 a, b = getattr(d, name), getattr(c, name)
                 if hasattr(a, "__len__"):
-                    assert list(a) == list(b), name
+                    assert np.array_equal(np.ctypeslib.as_array(a), np.ctypeslib.as_array(b)), name
                 else:
                     assert a == b, name
             for nm, ref, n in (("air_x", g.air_x, 20), ("air_y", g.air_y, 24), ("air_z", g.air_z, 18),
@@ -119,7 +126,8 @@ def test_spectrum_writer_layout(lee):
 @pytest.mark.parametrize("change,code", [
     (dict(scheme=5), -4),       # RK4 is an unimplemented enum in the reference deck path
     (dict(scheme=1), -4),
-    (dict(hill_judge=1), -4),
+    (dict(hill_judge=6), -4),   # no obstacle mask of that kind in the reference
+    (dict(hill_judge=1, integrator=1), -4),  # obstacles run with the FB_p_v step
     (dict(restart=1), -4),
     (dict(velocity_method=3), -4),
     (dict(bc=(4, 1, 1, 1, 1, 1)), -4),
