@@ -85,6 +85,12 @@ struct orc_sim {
      * and of the auxiliaries, the auxiliaries of the stage state, and those of level n */
     double *rk_k[2][4], *rk_a[2][4];
     double *qk[6][4], *qa[6][4], *qs[6][4], *qsave[6][4];
+    /* embedded porous obstacle (air.h:58-67): curve kind, index box istar..kstop, material
+     * id per box node (0 air, 1 canopy, 2 trunk) and the per-material coefficients
+     * crpo_* (index 0 hill, 1 canopy, 2 trunk) */
+    int cr_judge, ob[6];
+    int *cr_uc;
+    double cr_gama[3], cr_invb[3], cr_eff[3];
 };
 
 static const double TD_A[6] = {8.226223820e-01, 3.732304064e-01, 9.106232372e-02,
@@ -487,6 +493,17 @@ static windterms wind_terms(const orc_sim *s, const metric9 *m, int i, int j, in
 static double sigma_of(double R, double alpha, double t) { return R * pow(t, alpha); }
 static double beta_of(double alpha, double t) { return 1.0 + 25.0 * pow(t, alpha); }
 
+/* material id of the obstacle at frame-local air node (i, j, k): the reference tests the
+ * rank-global index against the box fixed at construction (air.cpp:1907-1909, 2294-2296),
+ * so the obstacle keeps its index position when the frame moves. */
+static int cr_at(const orc_sim *s, int in, int jn, int kn)
+{
+    if (s->cr_judge == 0) return 0;
+    if (in < s->ob[0] || in > s->ob[1] || jn < s->ob[2] || jn > s->ob[3] || kn < s->ob[4] || kn > s->ob[5]) return 0;
+    int ni = s->ob[1] - s->ob[0] + 1, nj = s->ob[3] - s->ob[2] + 1;
+    return s->cr_uc[(size_t)(in - s->ob[0]) + (size_t)ni * ((size_t)(jn - s->ob[2]) + (size_t)nj * (kn - s->ob[4]))];
+}
+
 static double air_dp_point(orc_sim *s, const double *u, const double *v, const double *w,
                            const double *p, int i, int j, int k)
 {
@@ -652,6 +669,14 @@ static double air_dp_point(orc_sim *s, const double *u, const double *v, const d
 #undef XS_F
 #undef YS_F
 #undef ZS_F
+    /* inside the obstacle the increment is the rigid-frame porous continuity equation with
+     * the full nine-term divergence (air.cpp:2294-2318); mask 1 takes the canopy, mask 2 the
+     * trunk coefficients */
+    int mo = cr_at(s, in, jn, kn);
+    if (mo == 1 || mo == 2)
+        r = -s->cr_gama[mo] * (m.xX * FDX(u) * dxi + m.yX * FDY(u) * dyi + m.zX * FDZ(u) * dzi
+                             + m.xY * FDX(v) * dxi + m.yY * FDY(v) * dyi + m.zY * FDZ(v) * dzi
+                             + m.xZ * FDX(w) * dxi + m.yZ * FDY(w) * dyi + m.zZ * FDZ(w) * dzi);
     return r;
 }
 
@@ -831,6 +856,14 @@ static void air_duvw_point(orc_sim *s, const double *u, const double *v, const d
 #undef XS_B
 #undef YS_B
 #undef ZS_B
+    /* inside the obstacle: semi-implicit Darcy drag of the material (air.cpp:1907-1945) */
+    int mo = cr_at(s, in, jn, kn);
+    if (mo == 1 || mo == 2) {
+        double ib = s->cr_invb[mo], f = -1.0 * ib * dt * s->cr_eff[mo];
+        du = f * GP(m.xX, m.yX, m.zX) + uc * ib - uc;
+        dv = f * GP(m.xY, m.yY, m.zY) + vc * ib - vc;
+        dw = f * GP(m.xZ, m.yZ, m.zZ) + wc * ib - wc;
+    }
 #undef GP
     *ou = du; *ov = dv; *ow = dw;
 }
@@ -1192,6 +1225,95 @@ orc_sim *orc_create(const lee_deck *d, int nrec, const double *rec, int fill_nn_
     return orc_create_opt(d, &o, fill_nn_nan);
 }
 
+/* air::cal_hill_var (air.cpp:938-1105) with the material derivation of airpore.cpp:282-293:
+ * coefficients, index box and material mask.  Coordinates are those of the air input
+ * files (axisbound_*, calculation_2D.cpp:154-200) for the frame-0 index range. */
+static const char *obstacle_setup(orc_sim *s)
+{
+    const lee_deck *d = &s->d;
+    s->cr_judge = d->hill_judge;
+    if (s->cr_judge == 0) return NULL;
+    for (int m = 0; m < 3; ++m) {
+        double q = d->hill_mat[m][0], por = d->hill_mat[m][1], res = d->hill_mat[m][2];
+        double eff_density = q * q / por / s->Aav;      /* airpore.cpp:286 */
+        double Kp = 1.0 / s->gamma / s->Pav;             /* airpore.cpp:287 */
+        s->cr_eff[m] = 1.0 / eff_density;                /* air.cpp:958 */
+        double Beta = res * d->dt * s->cr_eff[m];        /* air.cpp:962 */
+        s->cr_gama[m] = d->dt / Kp / por;                /* air.cpp:963 */
+        s->cr_invb[m] = 1.0 / (1.0 + Beta);              /* air.cpp:964 */
+    }
+    const double *ax = d->air_x, *ay = d->air_y, *az = d->air_z;
+    int n[3] = {s->air.nx, s->air.ny, s->air.nz};
+    const double *axes[3] = {ax, ay, az};
+    double h[3] = {d->dx, d->dy, d->dz};
+    for (int a = 0; a < 3; ++a) {                        /* air.cpp:987-1005 */
+        int lo = 0, hi = 0;
+        for (int q = 0; q < n[a]; ++q) {
+            if (fabs(axes[a][q] - d->hill_box[2 * a]) <= h[a] / 8.0) lo = q;
+            if (fabs(axes[a][q] - d->hill_box[2 * a + 1]) <= h[a] / 8.0) { hi = q; break; }
+        }
+        s->ob[2 * a] = lo; s->ob[2 * a + 1] = hi;
+    }
+    int ni = s->ob[1] - s->ob[0] + 1, nj = s->ob[3] - s->ob[2] + 1, nk = s->ob[5] - s->ob[4] + 1;
+    if (ni <= 0 || nj <= 0 || nk <= 0) return "the obstacle bounds select an empty index box";
+    s->cr_uc = (int *)calloc((size_t)ni * nj * nk, sizeof(int));
+    if (!s->cr_uc) return "oom";
+    const double *c = d->hill_coef;
+    double A = c[0], B = c[1], C = c[2], D = c[3], E = c[4], F = c[5], G = c[6], H = c[7], I = c[8], J = c[9], K = c[10];
+    int ii = 0, jj = 0;
+    for (int i = s->ob[0]; i <= s->ob[1]; ++i)
+        for (int j = s->ob[2]; j <= s->ob[3]; ++j)
+            for (int k = s->ob[4]; k <= s->ob[5]; ++k) {
+                int *m = &s->cr_uc[(size_t)(i - s->ob[0]) + (size_t)ni * ((size_t)(j - s->ob[2]) + (size_t)nj * (k - s->ob[4]))];
+                double X = ax[i], Y = ay[j], Z = az[k], len, hgt;
+                *m = 0;
+                switch (s->cr_judge) {
+                case 1:   /* sphere of radius b about (c, d, e), air.cpp:1020-1031 */
+                    len = pow(X - C, 2) + pow(Y - D, 2) + pow(Z - E, 2);
+                    if (sqrt(len) <= B) *m = 1;
+                    break;
+                case 2:   /* box [a,b] x [c,d] x [e,f], air.cpp:1034-1047 */
+                    if (X >= A && X <= B && Y >= C && Y <= D && Z >= E && Z <= F) *m = 1;
+                    break;
+                case 3:   /* three rows of cylinders of radius a, pitch b, air.cpp:1050-1069 */
+                    for (jj = 0; jj < d->hill_points; jj++) {
+                        len = pow(X - C, 2) + pow(Y - (D + jj * B), 2);
+                        if (sqrt(len) <= A && Z <= E) *m = 1;
+                        len = pow(X - F, 2) + pow(Y - (G + jj * B), 2);
+                        if (sqrt(len) <= A && Z <= H) *m = 1;
+                        len = pow(X - I, 2) + pow(Y - (J + jj * B), 2);
+                        if (sqrt(len) <= A && Z <= K) *m = 1;
+                    }
+                    break;
+                case 4:   /* g x f array of cylinders of radius a, air.cpp:1071-1087 */
+                    for (ii = 0; ii < G; ii++)
+                        for (jj = 0; jj < F; jj++) {
+                            len = pow(X - (B + ii * E), 2) + pow(Y - (C + jj * D), 2);
+                            if (sqrt(len) <= A) *m = 1;
+                        }
+                    break;
+                case 5:   /* tree at (0, 3): cone canopy 0.5..3 m, trunk r 0.1 m, air.cpp:1089-1103 */
+                    for (ii = 0; ii < 1; ii++)
+                        for (jj = 0; jj < 1; jj++) {
+                            len = pow(X - (0.0 + ii * 2.0), 2) + pow(Y - (3.0 + jj * 2.0), 2);
+                            hgt = -25 / (9 - 1e-50) * sqrt(len) + 3.0;
+                            if (sqrt(len) <= 0.9 && Z <= hgt && Z >= 0.5) *m = 1;
+                            if (sqrt(len) <= 0.1 && Z <= 1.5) *m = 2;
+                        }
+                    break;
+                }
+            }
+    if (s->cr_judge == 5)   /* second trunk pass with the counters the loops left (air.cpp:1106-1115) */
+        for (int i = s->ob[0]; i <= s->ob[1]; ++i)
+            for (int j = s->ob[2]; j <= s->ob[3]; ++j)
+                for (int k = s->ob[4]; k <= s->ob[5]; ++k) {
+                    double len = pow(ax[i] - (0.0 + ii * 2.0), 2) + pow(ay[j] - (3.0 + jj * 2.0), 2);
+                    if (sqrt(len) <= 0.1 && az[k] <= 1.5)
+                        s->cr_uc[(size_t)(i - s->ob[0]) + (size_t)ni * ((size_t)(j - s->ob[2]) + (size_t)nj * (k - s->ob[4]))] = 2;
+                }
+    return NULL;
+}
+
 orc_sim *orc_create_opt(const lee_deck *d, const lee_options *opt, int fill_nn_nan)
 {
     g_err[0] = 0;
@@ -1205,7 +1327,9 @@ orc_sim *orc_create_opt(const lee_deck *d, const lee_options *opt, int fill_nn_n
     if (opt->integrator == LEE_INT_RK4 && d->bc[LEE_FACE_E] == LEE_BC_TDBC) FAIL("the TDBC is not defined for RK4");
     if (opt->integrator == LEE_INT_DECK && d->scheme != 2) FAIL("only scheme 2 (FB_p_v) is implemented by the reference");
     if (d->bc[LEE_FACE_N] != LEE_BC_RIGID && d->bc[LEE_FACE_N] != LEE_BC_ABSORBING) FAIL("North must be rigid or absorbing");
-    if (d->hill_judge != 0) FAIL("embedded porous obstacles are not supported");
+    if (d->hill_judge < 0 || d->hill_judge > 5) FAIL("curve_judge %d has no mask", d->hill_judge);
+    if (d->hill_judge != 0 && (opt->integrator == LEE_INT_RK4 || opt->n_hills > 0))
+        FAIL("embedded porous obstacles run with the FB_p_v step on a separable grid");
     if (d->restart != 0) FAIL("restart is not supported");
     if (d->velocity_method == 3 || d->velocity_method == 6) FAIL("wind methods 3 and 6 read uninitialised members");
     orc_sim *s = (orc_sim *)calloc(1, sizeof *s);
@@ -1237,6 +1361,8 @@ orc_sim *orc_create_opt(const lee_deck *d, const lee_options *opt, int fill_nn_n
     size_t n = (size_t)h[0] * JM * h[6];
     s->U = dalloc(n, 0); s->V = dalloc(n, 0); s->W = dalloc(n, 0);
     const char *e = pml_setup(s);
+    if (e) { orc_destroy(s); FAIL("%s", e); }
+    e = obstacle_setup(s);
     if (e) { orc_destroy(s); FAIL("%s", e); }
     /* porous ground */
     s->south_porous = d->bc[LEE_FACE_S] == LEE_BC_POROUS;
@@ -1304,7 +1430,19 @@ void orc_destroy(orc_sim *s)
     for (int q = 0; q < 6; ++q)
         for (int f = 0; f < 4; ++f) { free(s->qk[q][f]); free(s->qa[q][f]); free(s->qs[q][f]); }
     free(s->ri); free(s->rj); free(s->rk);
+    free(s->cr_uc);
     free(s);
+}
+
+int orc_obstacle(const orc_sim *s, int *box, int *mask)
+{
+    if (s->cr_judge == 0) return 0;
+    for (int q = 0; q < 6; ++q) box[q] = s->ob[q];
+    if (mask) {
+        size_t n = (size_t)(s->ob[1] - s->ob[0] + 1) * (s->ob[3] - s->ob[2] + 1) * (s->ob[5] - s->ob[4] + 1);
+        memcpy(mask, s->cr_uc, n * sizeof(int));
+    }
+    return s->cr_judge;
 }
 
 void orc_dims(const orc_sim *s, int *dims)

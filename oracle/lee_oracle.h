@@ -54,6 +54,9 @@ void orc_get_state(const orc_sim *s, int domain, double *u, double *v, double *w
 void orc_set_state(orc_sim *s, int domain, const double *u, const double *v,
                    const double *w, const double *p);
 void orc_get_psi(const orc_sim *s, double *psi);
+/* Embedded obstacle: returns curve_judge (0 = none); box = istar istop jstar jstop kstar
+ * kstop; mask (may be NULL) gets the material id per box node, i fastest. */
+int  orc_obstacle(const orc_sim *s, int *box, int *mask);
 
 /* Increments only (no add, no BC): the stencils air::cal_fp / air::cal_fuvw /
  * porous::cal_fp / porous::cal_fuvw applied to the given fields, written to out

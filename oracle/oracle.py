@@ -62,6 +62,7 @@ def lib():
         L.orc_wind.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.orc_metric.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_fft.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_obstacle.argtypes = [ctypes.c_void_p, _ip, _ip]
         _lib = L
     return _lib
 
@@ -192,6 +193,19 @@ class Oracle:
         lib().orc_metric(self.h, domain, i, j, k, _p(out))
         return out[:9].reshape(3, 3), out[9]
 
+    def obstacle(self):
+        """(curve_judge, box istar..kstop, mask[k, j, i]) of the embedded obstacle, or
+        (0, None, None) without one."""
+        box = np.zeros(6, np.int32)
+        kind = lib().orc_obstacle(self.h, box.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), None)
+        if kind == 0:
+            return 0, None, None
+        ni, nj, nk = box[1] - box[0] + 1, box[3] - box[2] + 1, box[5] - box[4] + 1
+        mask = np.zeros((nk, nj, ni), np.int32)
+        lib().orc_obstacle(self.h, box.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                           mask.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return kind, box, mask
+
     def wind(self):
         outs = [np.zeros(self.shape) for _ in range(3)]
         lib().orc_wind(self.h, *[_p(o) for o in outs])
@@ -218,7 +232,7 @@ def _rec(fh, arr):
 
 
 def run_probe(deck, fields, fields2, frame=0, counter=0, seed=7, pore=(1.7320508075688772, 0.3, 2.0e5),
-              fft_m=6, fft_re=None, fft_im=None, fft_file=None):
+              fft_m=6, fft_re=None, fft_im=None, fft_file=None, obstacle=False):
     """Run the reference's own stencil code on `fields` (u, v, w, p) and return a dict of
     named arrays (see oracle/ref_probe.cpp for the record list)."""
     nx, ny, nz = len(deck.air_x), len(deck.air_y), len(deck.air_z)
@@ -246,6 +260,8 @@ def run_probe(deck, fields, fields2, frame=0, counter=0, seed=7, pore=(1.7320508
             _rec(fh, np.array([fft_m], dtype=np.float64))
             _rec(fh, fft_re); _rec(fh, fft_im)
             _rec(fh, fft_re); _rec(fh, fft_im)
+            ob = [deck.hill_judge, deck.hill_points, *deck.hill_coef, *np.ravel(deck.hill_mat), *deck.hill_box]
+            _rec(fh, np.array(ob if obstacle else [0.0], dtype=np.float64))
         cmd = [PROBE_PATH, cin, cout] + ([fft_file] if fft_file else [])
         subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, timeout=300)
         out = {}

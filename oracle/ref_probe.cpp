@@ -15,6 +15,7 @@
  *   air::cal_fp (air.cpp:1953-2322), air::cal_fuvw (air.cpp:1476-1949),
  *   air::Update_PML_Qp / Qvw (air.cpp:2323-2439),
  *   porous::cal_fp / cal_fuvw (porous.cpp:77-143), FFT / FFT_output (FFT.cpp:15-121),
+ *   air::cal_hill_var (air.cpp:938-1105),
  *   Geo_over_var (df_over_var.cpp:5-9), mpe_decomp1d (decomp.cpp:4-19).
  * Calls are qualified (non-virtual), so no vtable is needed.  MPI runs as a singleton.
  *
@@ -223,6 +224,20 @@ struct AirProbe : public air {
     std::vector<double> W() { return dump(Wav); }
     double ***un() { return u_n; } double ***vn() { return v_n; } double ***wn() { return w_n; } double ***pn() { return p_n; }
     double ***unn() { return u_nn; } double ***vnn() { return v_nn; } double ***wnn() { return w_nn; } double ***pnn() { return p_nn; }
+    /* embedded obstacle: air::cal_hill_var (air.cpp:938-1105), then its box and mask */
+    void hill(hillpore h, hillpore c, hillpore t) { air::cal_hill_var(h, c, t); }
+    std::vector<double> hill_box() const
+    {
+        return {(double)istar, (double)istop, (double)jstar, (double)jstop, (double)kstar, (double)kstop};
+    }
+    std::vector<double> hill_mask() const
+    {
+        std::vector<double> m;
+        for (int k = kstar; k <= kstop; ++k)
+            for (int j = jstar; j <= jstop; ++j)
+                for (int i = istar; i <= istop; ++i) m.push_back(cr_uc[i - istar][j - jstar][k - kstar]);
+        return m;
+    }
 };
 
 struct PoreProbe : public porous {
@@ -389,6 +404,44 @@ int main(int argc, char **argv)
                 dec.push_back(n); dec.push_back(np); dec.push_back(id); dec.push_back(s); dec.push_back(e);
             }
     emit("decomp", dec);
+
+    /* embedded obstacle (record: judge points coef[12] mat[3][3] box[6]); the material
+     * parameters are derived as airpore.cpp:282-293 does, and the stencils rerun on a
+     * fresh air object with the same PML auxiliaries as above */
+    std::vector<double> ob = rd.d();
+    if (ob[0] != 0) {
+        AirProbe *b = blank<AirProbe>();
+        b->setup(g, hdr, dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], vmethod, dv[7], jmf, lead, trail, comm);
+        b->metrics(0);
+        b->pml(xb, yb, zb);
+        b->qmove(0);
+        b->fill_q(seed);
+        b->wind(0);
+        b->set_gate(counter, b->gateJ1());
+        hillpore hp[3];
+        memset(hp, 0, sizeof hp);
+        double Aav_ = pow(dv[6], 2) / dv[4] / dv[5];
+        for (int m = 0; m < 3; ++m) {
+            hillpore &h = hp[m];
+            h.curve_judge = (int)ob[0]; h.curve_points = (int)ob[1];
+            double *cf[12] = {&h.curve_coefa, &h.curve_coefb, &h.curve_coefc, &h.curve_coefd, &h.curve_coefe, &h.curve_coeff,
+                              &h.curve_coefg, &h.curve_coefh, &h.curve_coefi, &h.curve_coefj, &h.curve_coefk, &h.curve_coefl};
+            for (int q = 0; q < 12; ++q) *cf[q] = ob[2 + q];
+            h.q = ob[14 + 3 * m]; h.porosity = ob[15 + 3 * m]; h.resistivity = ob[16 + 3 * m];
+            h.eff_density = h.q * h.q / h.porosity / Aav_;
+            h.Kp = 1.0 / dv[4] / dv[5];
+            h.curve_xstar = ob[23]; h.curve_xstop = ob[24]; h.curve_ystar = ob[25];
+            h.curve_ystop = ob[26]; h.curve_zstar = ob[27]; h.curve_zstop = ob[28];
+        }
+        b->hill(hp[0], hp[1], hp[2]);
+        emit("obox", b->hill_box());
+        emit("omask", b->hill_mask());
+        b->load(b->un(), fu); b->load(b->vn(), fv); b->load(b->wn(), fw); b->load(b->pn(), fp);
+        b->fp();
+        emit("odp", b->dump(b->pnn()));
+        b->fuvw();
+        emit("odu", b->dump(b->unn())); emit("odv", b->dump(b->vnn())); emit("odw", b->dump(b->wnn()));
+    }
     fclose(g_out);
     fclose(fin);
     MPI_Finalize();
