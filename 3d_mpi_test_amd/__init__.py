@@ -107,9 +107,21 @@ def lib():
         L.lee_save_restart.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
         L.lee_load_restart.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
         L.lee_write_contour.argtypes = [vp, ctypes.c_char_p]
+        L.lee_set_source_series.argtypes = [vp, _dp, ctypes.c_longlong, ctypes.c_int]
+        L.lee_read_source_series.argtypes = [ctypes.c_char_p, _dp, ctypes.c_longlong]
+        L.lee_read_source_series.restype = ctypes.c_longlong
         L.lee_copy_bench.restype = ctypes.c_double
         _lib = L
     return _lib
+
+
+def read_source_series(dirpath: str, cap: int = 100001) -> np.ndarray:
+    """Values of dirpath/source_input.dat (calculation_2D.cpp:220-235), at most cap."""
+    out = np.zeros(cap)
+    n = lib().lee_read_source_series(dirpath.encode(), _p(out), cap)
+    if n < 0:
+        raise LeeError(int(n), lib().lee_global_error().decode())
+    return out[:n].copy()
 
 
 def _p(a):
@@ -227,6 +239,12 @@ class Solver:
 
     def write_contour(self, dirpath: str) -> None:
         _check(lib().lee_write_contour(self.h, dirpath.encode()), self.h)
+
+    def set_source_series(self, ps, period: int = 23) -> None:
+        """Time-series source: every `period` steps the pressure gains ps[N/period] times the
+        initial Gaussian (calculation_2D::UpdateSource); an empty series removes it."""
+        a = np.ascontiguousarray(np.asarray(ps, dtype=np.float64).ravel())
+        _check(lib().lee_set_source_series(self.h, _p(a) if a.size else None, a.size, period), self.h)
 
     def last_ms(self) -> float:
         return lib().lee_last_step_ms(self.h)

@@ -65,6 +65,10 @@ struct lee_ctx {
     // terrain tables (sh, hx, hy per (i,j); tk per k) of the air [0] and porous [1] grids
     void *ter[2][4] = {};
     unsigned char *omask = nullptr;  // embedded obstacle material ids (FB)
+    // time-series source: amplitudes, period in steps, node coordinates X | Y(frame) | Z
+    std::vector<double> src_ps;
+    int src_period = 0;
+    double *src_xyz = nullptr;
     // captured step chunks (use_graph): keyed by the buffer roles they start from
     struct GraphEntry { int n, cur, qcur, gate_on; void *psi; hipGraphExec_t exec; };
     std::vector<GraphEntry> graphs;
@@ -140,6 +144,15 @@ int upload(lee_ctx *c, void *dst, const double *src, size_t n)
         HIPCHK(c, hipMemcpyAsync(dst, tmp.data(), n * 4, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
+    return LEE_OK;
+}
+
+// Upload FP64 host values as FP64 whatever the state precision (coordinates).
+int upload_f64(lee_ctx *c, double *dst, const double *src, size_t n)
+{
+    if (!n) return LEE_OK;
+    HIPCHK(c, hipMemcpyAsync(dst, src, n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return LEE_OK;
 }
 
@@ -313,6 +326,8 @@ int upload_frame(lee_ctx *c, int N)
                 if ((rc = upload(c, c->ter[d][q], v[q]->data(), v[q]->size()))) return rc;
         }
     }
+    if (c->src_xyz)
+        if ((rc = upload_f64(c, c->src_xyz + m.nx, t.Y.data(), t.Y.size()))) return rc;
     // receiver linear indices in this frame
     int nr = (int)m.ri.size();
     std::vector<long long> idx(nr);
@@ -400,6 +415,19 @@ void gate_tick(lee_ctx *c, int &J1, int &y1on)
     y1on = (m.slab[2].n > 0 && gate && c->counter >= m.pml_time) ? 1 : 0;
 }
 
+// calculation_2D::UpdateSource (calculation_2D.cpp:371-385) after step N (counted from 1,
+// airpore.cpp:572): every `period` steps the pressure of level n+1 gains ps[N/period] times
+// the initial Gaussian about the deck's source (the reference's call, airpore.cpp:608-611).
+template <typename T> void inject_source(lee_ctx *c, T *p)
+{
+    const long long N = c->steps + 1;
+    if (c->src_ps.empty() || N % c->src_period != 0 || N / c->src_period >= (long long)c->src_ps.size()) return;
+    const Model &m = c->m;
+    const double gc = 4.0 * std::log(2.0) / std::pow(m.d.gauss_width, 2);  // calculation_2D.cpp:26
+    launch_add_source<T>(p, m.nx, m.ny, m.nz, c->src_xyz, c->src_xyz + m.nx, c->src_xyz + m.nx + m.ny, m.d.src[0],
+                         m.d.src[1], m.d.src[2], gc, c->src_ps[N / c->src_period], c->stream);
+}
+
 // One RK4 step: launches only (no host synchronisation), receivers into rec_slot.
 template <typename T> void enqueue_rk4_step(lee_ctx *c, double *rec_slot)
 {
@@ -473,6 +501,7 @@ template <typename T> void enqueue_rk4_step(lee_ctx *c, double *rec_slot)
         launch_rk_bnd<T>(b, s);
         prof_mark(c, KS_RK_BND, false);
     }
+    inject_source<T>(c, (T *)c->air[c->cur * 4 + 3]);
     c->steps += 1;
 }
 
@@ -527,6 +556,7 @@ template <typename T> void enqueue_fb_step(lee_ctx *c, double *rec_slot)
     launch_bc_v_all<T>(b, s);
     prof_mark(c, KS_BC_V, false);
     if (m.east_tdbc) std::swap(c->psi, c->psi2);
+    inject_source<T>(c, (T *)c->air[(c->cur ^ 1) * 4 + 3]);
     // ---- level n+1 becomes level n
     c->cur ^= 1;
     c->qcur ^= 1;
@@ -598,7 +628,7 @@ template <typename T> int run_steps(lee_ctx *c, int nsteps, double *rec_out)
     while (done < nsteps) {
         int n = std::min(nsteps - done, kRecChunk);
         const long long before = c->steps;
-        if (m.opt.use_graph && !c->profile) {
+        if (m.opt.use_graph && !c->profile && c->src_ps.empty()) {
             n = std::min(n, kGraphSteps);
             if (gate_constant(c, n)) {
                 int rc = graph_steps<T>(c, n);
@@ -815,6 +845,40 @@ int lee_step(lee_ctx *c, int nsteps, double *rec_out)
     if (c->mf_count < 0) return set_err(c, LEE_ERR_STATE, "lee_frame_setup must run before lee_step");
     if (nsteps == 0) return LEE_OK;
     return c->prec == 64 ? run_steps<double>(c, nsteps, rec_out) : run_steps<float>(c, nsteps, rec_out);
+}
+
+int lee_set_source_series(lee_ctx *c, const double *ps, long long n, int period)
+{
+    if (!c || n < 0 || (n > 0 && (!ps || period < 1))) return set_err(c, LEE_ERR_ARG, "bad source series");
+    if (n > 0 && c->m.terrain) return set_err(c, LEE_ERR_UNSUPPORTED, "the time-series source is defined on the separable grid");
+    for (long long q = 0; q < n; ++q)
+        if (!std::isfinite(ps[q])) return set_err(c, LEE_ERR_ARG, "non-finite source amplitude");
+    const Model &m = c->m;
+    if (n > 0 && !c->src_xyz) {
+        int rc = dalloc(c, (void **)&c->src_xyz, (size_t)(m.nx + m.ny + m.nz) * 8);
+        if (rc) return rc;
+        if ((rc = upload_f64(c, c->src_xyz, m.ax.data(), m.nx))) return rc;
+        if ((rc = upload_f64(c, c->src_xyz + m.nx + m.ny, m.az.data(), m.nz))) return rc;
+        FrameTables t;
+        m.frame_tables(c->mf_count < 0 ? 0 : c->mf_count, t);
+        if ((rc = upload_f64(c, c->src_xyz + m.nx, t.Y.data(), t.Y.size()))) return rc;
+    }
+    c->src_ps.assign(ps, ps + n);
+    c->src_period = n > 0 ? period : 0;
+    drop_graphs(c);
+    return LEE_OK;
+}
+
+long long lee_read_source_series(const char *dir, double *ps, long long cap)
+{
+    if (!dir || cap < 0 || (cap > 0 && !ps)) { set_global_error("bad arguments"); return LEE_ERR_ARG; }
+    std::string path = std::string(dir) + "/source_input.dat";
+    std::ifstream in(path);
+    if (!in) { set_global_error("cannot open " + path); return LEE_ERR_IO; }
+    long long n = 0;
+    double v;
+    while (n < cap && (in >> v)) ps[n++] = v;
+    return n;
 }
 
 int lee_frame_steps(const lee_ctx *c, int mf, int temp_time) { return c ? c->m.frame_steps(mf, temp_time) : 0; }

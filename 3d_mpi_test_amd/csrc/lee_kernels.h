@@ -125,6 +125,9 @@ template <typename T> void launch_copy_plane_j(T *p, int nx, int ny, int nz, hip
 template <typename T> void launch_sample(const T *p, const long long *idx, int n, double *out, hipStream_t s);
 void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s);
 template <typename T> void launch_finite_check(const T *f, size_t n, int *flag, hipStream_t s);
+template <typename T>
+void launch_add_source(T *p, int nx, int ny, int nz, const double *X, const double *Y, const double *Z, double sx,
+                       double sy, double sz, double gc, double amp, hipStream_t s);
 template <typename T> void launch_subsample(const T *p, int nx, int ny, int nz, int stride, int ox, int oy, int oz,
                                              double *out, hipStream_t s);
 

@@ -909,6 +909,24 @@ __global__ void k_finite(const T *f, size_t n, int *flag)
         if (!isfinite((double)f[t])) { atomicOr(flag, 1); return; }
 }
 
+// Time-series source, calculation_2D::UpdateSource (calculation_2D.cpp:371-385):
+// p += amp * exp(-gc r^2) at every node, r from the node to the source
+// (cal_InitialPressure, calculation_2D.cpp:292-298).  Coordinates in FP64.
+template <typename T>
+__global__ void k_add_source(T *p, int nx, int ny, int nz, const double *X, const double *Y, const double *Z,
+                             double sx, double sy, double sz, double gc, double amp)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx);
+        const size_t r = t / nx;
+        const int j = (int)(r % ny), k = (int)(r / ny);
+        const double x = X[i] - sx, y = Y[j] - sy, z = Z[k] - sz;
+        const double rr = sqrt((x * x + y * y) + z * z);
+        p[t] = p[t] + (T)(amp * (1.0 * exp((-gc) * rr * rr)));
+    }
+}
+
 // Streaming copy used as the bandwidth reference and to calibrate HBM counters: one 16 B
 // vector per thread, non-temporal (the data is touched once), as many workgroups as needed.
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -1030,6 +1048,14 @@ template <typename T> void launch_finite_check(const T *f, size_t n, int *flag, 
     hipLaunchKernelGGL((k_finite<T>), dim3(flat_blocks(n)), dim3(256), 0, s, f, n, flag);
 }
 
+template <typename T>
+void launch_add_source(T *p, int nx, int ny, int nz, const double *X, const double *Y, const double *Z, double sx,
+                       double sy, double sz, double gc, double amp, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_add_source<T>), dim3(flat_blocks((size_t)nx * ny * nz)), dim3(256), 0, s, p, nx, ny, nz, X, Y,
+                       Z, sx, sy, sz, gc, amp);
+}
+
 void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s)
 {
     const size_t n4 = n / 4;
@@ -1049,6 +1075,8 @@ void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s)
     template void launch_copy_plane_j<T>(T *, int, int, int, hipStream_t);                                   \
     template void launch_sample<T>(const T *, const long long *, int, double *, hipStream_t);                \
     template void launch_finite_check<T>(const T *, size_t, int *, hipStream_t);                             \
+    template void launch_add_source<T>(T *, int, int, int, const double *, const double *, const double *,   \
+                                       double, double, double, double, double, hipStream_t);                 \
     template void launch_subsample<T>(const T *, int, int, int, int, int, int, int, double *, hipStream_t);
 LEE_INST(double)
 LEE_INST(float)

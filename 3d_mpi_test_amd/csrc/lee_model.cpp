@@ -462,6 +462,7 @@ void Model::frame_tables(int N, FrameTables &t) const
     std::vector<double> Y(ay);
     if (N != 0)
         for (int j = 0; j < ny; ++j) Y[j] = ay[j] + N * (trail - 1) * d.dy;
+    t.Y = Y;
     metrics1d(ax, dxi, t.Xx);
     metrics1d(Y, dyi, t.Yy);
     metrics1d(az, dzi, t.Zz);

@@ -42,6 +42,7 @@ struct TerrainTables {
 };
 
 struct FrameTables {
+    std::vector<double> Y;                // air y coordinates of the frame (calculation_2D.cpp:410-415)
     std::vector<double> Xx, Yy, Zz;       // air metrics X_over_x ... (calculation_2D.cpp:417-419)
     std::vector<double> pXx, pYy, pZz;    // porous metrics
     Profile x1, x2, y1, y2, z1, z2;       // PML profiles for this frame

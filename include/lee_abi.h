@@ -236,6 +236,19 @@ int  lee_load_restart(lee_ctx *ctx, const char *path, int *index);
  * multiple of out_difft (airpore.cpp:615-618) and writes to a fixed absolute path. */
 int  lee_write_contour(lee_ctx *ctx, const char *dir);
 
+/* Time-series source (calculation_2D::UpdateSource calculation_2D.cpp:371-385, the series
+ * of source_input.dat read at calculation_2D.cpp:220-235, the call at airpore.cpp:608-611):
+ * after step N (steps counted from 1 as index_pressure, airpore.cpp:572), when
+ * N % period == 0 and N / period < n, the pressure of the new level gains
+ * ps[N / period] * exp(-gc r^2) at every air node, r the distance to the deck's source and
+ * gc = 4 ln 2 / width^2 of the initial condition.  The reference uses period 23 and ships
+ * the call switched off.  n = 0 removes the series.  Steps then run without graph replay. */
+int  lee_set_source_series(lee_ctx *ctx, const double *ps, long long n, int period);
+/* Reads up to cap whitespace-separated values of dir/source_input.dat into ps (the
+ * reference reads 100001 values into a 100000-element array); returns the count, or a
+ * negative status. */
+long long lee_read_source_series(const char *dir, double *ps, long long cap);
+
 /* Output writers of the drop-in file contract.
  * ./solution/pty%04d.dat: one %.6e sample per line (airpore.cpp:468-531, 577).
  * ./solution/p_f2_%04d.dat: zero-padded radix-2 FFT, "f |X|" at setprecision(10)

@@ -91,6 +91,10 @@ struct orc_sim {
     int cr_judge, ob[6];
     int *cr_uc;
     double cr_gama[3], cr_invb[3], cr_eff[3];
+    /* time-series source (calculation_2D.h ps, calculation_2D.cpp:220-235) and the step
+     * index of airpore.cpp:572 (index_pressure, steps done) */
+    double *src_ps; long src_n; int src_period;
+    long index_pressure;
 };
 
 static const double TD_A[6] = {8.226223820e-01, 3.732304064e-01, 9.106232372e-02,
@@ -1431,6 +1435,7 @@ void orc_destroy(orc_sim *s)
         for (int f = 0; f < 4; ++f) { free(s->qk[q][f]); free(s->qa[q][f]); free(s->qs[q][f]); }
     free(s->ri); free(s->rj); free(s->rk);
     free(s->cr_uc);
+    free(s->src_ps);
     free(s);
 }
 
@@ -1507,6 +1512,8 @@ int orc_frame_count(const orc_sim *s)
 }
 
 /* One time step: airpore::Cal_pressure, Cal_velocity, UpdateInitialCond (airpore.cpp:589-614). */
+static void update_source(orc_sim *s, double *p);
+
 static void one_step(orc_sim *s)
 {
     ogrid *a = &s->air;
@@ -1541,6 +1548,7 @@ static void one_step(orc_sim *s)
         iface_v(s);
     }
     orc_update_q(s, 1, a->unn, a->vnn, a->wnn, NULL);
+    update_source(s, a->pnn);
     /* --- nn -> n (calculation_2D.cpp:354-369) --- */
     size_t n = (size_t)a->nx * a->ny * a->nz;
     memcpy(a->un, a->unn, n * 8); memcpy(a->vn, a->vnn, n * 8);
@@ -1710,13 +1718,53 @@ static void rk4_step(orc_sim *s)
         }
 }
 
+/* calculation_2D::UpdateSource (calculation_2D.cpp:371-385) with N = index_pressure: every
+ * node of the air pressure at the new level gains ps[N/23] * cal_InitialPressure(...) when
+ * N % 23 == 0 (the period is a parameter here).  Called after the velocity phase, before
+ * nn -> n (the call site airpore.cpp:608-611); on the RK4 path after the step. */
+static void update_source(orc_sim *s, double *p)
+{
+    long N = s->index_pressure;
+    if (s->src_n == 0 || N % s->src_period != 0 || N / s->src_period >= s->src_n) return;
+    ogrid *g = &s->air;
+    double gc = 4.0 * log(2.0) / pow(s->d.gauss_width, 2);      /* calculation_2D.cpp:26 */
+    const double *src = s->d.src;
+    for (int i = 0; i < g->nx; i++)
+        for (int j = 0; j < g->ny; j++)
+            for (int k = 0; k < g->nz; k++) {
+                double x = g->X[i] - src[0], y = g->Y[j] - src[1], z = g->Z[k] - src[2];
+                double r = sqrt(pow(x, 2) + pow(y, 2) + pow(z, 2));   /* calculation_2D.cpp:292-298 */
+                size_t id = IDX(g, i, j, k);
+                p[id] = p[id] + s->src_ps[N / s->src_period] * (1.0 * exp(-gc * r * r));
+            }
+}
+
+int orc_set_source_series(orc_sim *s, const double *ps, long n, int period)
+{
+    if (n < 0 || (n > 0 && (!ps || period < 1))) return -1;
+    if (n > 0 && s->air.terrain) return -1;
+    free(s->src_ps);
+    s->src_ps = NULL; s->src_n = 0; s->src_period = 0;
+    if (n == 0) return 0;
+    s->src_ps = (double *)malloc((size_t)n * sizeof(double));
+    if (!s->src_ps) return -1;
+    memcpy(s->src_ps, ps, (size_t)n * sizeof(double));
+    s->src_n = n; s->src_period = period;
+    return 0;
+}
+
 void orc_step(orc_sim *s, int nsteps, double *rec)
 {
     if (s->opt.integrator == LEE_INT_RK4 && rk_alloc(s)) return;
     for (int t = 0; t < nsteps; ++t) {
         if (rec) for (int r = 0; r < s->nrec; ++r) rec[(size_t)t * s->nrec + r] = sample(s, r);
-        if (s->opt.integrator == LEE_INT_RK4) rk4_step(s);
-        else one_step(s);
+        s->index_pressure += 1;
+        if (s->opt.integrator == LEE_INT_RK4) {
+            rk4_step(s);
+            update_source(s, s->air.pn);
+        } else {
+            one_step(s);
+        }
     }
 }
 

@@ -63,6 +63,7 @@ def lib():
         L.orc_metric.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_fft.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.orc_obstacle.argtypes = [ctypes.c_void_p, _ip, _ip]
+        L.orc_set_source_series.argtypes = [ctypes.c_void_p, _dp, ctypes.c_long, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -192,6 +193,11 @@ class Oracle:
         out = np.zeros(10)
         lib().orc_metric(self.h, domain, i, j, k, _p(out))
         return out[:9].reshape(3, 3), out[9]
+
+    def set_source_series(self, ps, period=23):
+        a = np.ascontiguousarray(np.asarray(ps, dtype=np.float64).ravel())
+        if lib().orc_set_source_series(self.h, _p(a) if a.size else None, a.size, period) != 0:
+            raise ValueError("bad source series")
 
     def obstacle(self):
         """(curve_judge, box istar..kstop, mask[k, j, i]) of the embedded obstacle, or
