@@ -108,11 +108,91 @@ def lib():
         L.lee_load_restart.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
         L.lee_write_contour.argtypes = [vp, ctypes.c_char_p]
         L.lee_set_source_series.argtypes = [vp, _dp, ctypes.c_longlong, ctypes.c_int]
+        L.lee_ensemble_create.argtypes = [ctypes.c_int, ctypes.POINTER(LeeDeck), ctypes.POINTER(LeeOptions),
+                                          ctypes.POINTER(ctypes.c_void_p)]
+        L.lee_ensemble_destroy.argtypes = [vp]
+        L.lee_ensemble_size.argtypes = [vp]
+        L.lee_ensemble_member.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.lee_ensemble_receivers.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+        L.lee_ensemble_step.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int]
+        L.lee_ensemble_last_ms.argtypes = [vp]
+        L.lee_ensemble_last_ms.restype = ctypes.c_double
+        L.lee_ensemble_error.argtypes = [vp]
+        L.lee_ensemble_error.restype = ctypes.c_char_p
         L.lee_read_source_series.argtypes = [ctypes.c_char_p, _dp, ctypes.c_longlong]
         L.lee_read_source_series.restype = ctypes.c_longlong
         L.lee_copy_bench.restype = ctypes.c_double
         _lib = L
     return _lib
+
+
+class Ensemble:
+    """Many independent domains stepped together (lee_ensemble_*): each RK4 stage is one
+    launch per kernel kind over every member.  `decks` and per-member `receivers` lists;
+    members share precision, wind kind and grid kind."""
+
+    def __init__(self, decks, precision: int = 32, receivers=None, use_graph: bool = True):
+        L = lib()
+        n = len(decks)
+        self._cdecks = (LeeDeck * n)()
+        self._opts = (LeeOptions * n)()
+        keep = []
+        for d, dk in enumerate(decks):
+            c = dk.to_c() if isinstance(dk, deck.Deck) else dk
+            keep.append(c)
+            ctypes.memmove(ctypes.byref(self._cdecks, d * ctypes.sizeof(LeeDeck)), ctypes.byref(c), ctypes.sizeof(LeeDeck))
+            o = deck.make_options(dk if isinstance(dk, deck.Deck) else None, precision=precision,
+                                  receivers=None if receivers is None else receivers[d], integrator=deck.INT_RK4)
+            keep.append(o)
+            ctypes.memmove(ctypes.byref(self._opts, d * ctypes.sizeof(LeeOptions)), ctypes.byref(o), ctypes.sizeof(LeeOptions))
+        self._keep = keep
+        h = ctypes.c_void_p()
+        rc = L.lee_ensemble_create(n, self._cdecks, self._opts, ctypes.byref(h))
+        if rc != LEE_OK:
+            raise LeeError(rc, (L.lee_global_error() or b"").decode(errors="replace"))
+        self.h = h
+        self.n = n
+        self.use_graph = use_graph
+        off = (ctypes.c_int * (n + 1))()
+        L.lee_ensemble_receivers(self.h, off, n + 1)
+        self.offsets = list(off)
+        self.nrec = self.offsets[-1]
+        self.members = []
+        for d in range(n):
+            m = ctypes.c_void_p()
+            self._check(L.lee_ensemble_member(self.h, d, ctypes.byref(m)))
+            self.members.append(Solver._member(m, keep[2 * d]))
+
+    def _check(self, rc):
+        if rc != LEE_OK:
+            raise LeeError(rc, (lib().lee_ensemble_error(self.h) or b"").decode(errors="replace"))
+
+    def step(self, n: int, record: bool = True):
+        """n steps of every member; returns a list of (n, nrec_d) receiver series."""
+        rec = np.zeros((n, max(self.nrec, 1))) if record else None
+        self._check(lib().lee_ensemble_step(self.h, n, _p(rec), 1 if self.use_graph else 0))
+        if rec is None:
+            return None
+        return [rec[:, self.offsets[d]:self.offsets[d + 1]].copy() for d in range(self.n)]
+
+    def last_ms(self) -> float:
+        return lib().lee_ensemble_last_ms(self.h)
+
+    def points(self) -> int:
+        return sum(m.nx * m.ny * m.nz + m.pnx * m.pny * m.pnz for m in self.members)
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            for m in self.members:
+                m.h = ctypes.c_void_p()
+            lib().lee_ensemble_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def read_source_series(dirpath: str, cap: int = 100001) -> np.ndarray:
@@ -167,6 +247,10 @@ class Solver:
         h = ctypes.c_void_p()
         _check(L.lee_create(ctypes.byref(self._cdeck), ctypes.byref(opt), ctypes.byref(h)))
         self.h = h
+        self._owned = True
+        self._describe()
+
+    def _describe(self):
         info = self.info()
         self.nx, self.ny, self.nz = info.nx, info.ny, info.nz
         self.pnx, self.pny, self.pnz = info.pnx, info.pny, info.pnz
@@ -174,9 +258,20 @@ class Solver:
         self.path = info.path
         self.time_domain = self._cdeck.time_domain
 
+    @classmethod
+    def _member(cls, handle, cdeck):
+        """A view of a context owned by an Ensemble (closing it does not destroy it)."""
+        s = cls.__new__(cls)
+        s._deck_obj, s._cdeck, s._opt = None, cdeck, None
+        s.h = handle
+        s._owned = False
+        s._describe()
+        return s
+
     def close(self):
         if getattr(self, "h", None) and self.h.value:
-            lib().lee_destroy(self.h)
+            if getattr(self, "_owned", True):
+                lib().lee_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):

@@ -237,10 +237,9 @@ __device__ __forceinline__ T rk_update_q(int st, T xn, T k, T *acc, size_t q)
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_rk_bnd(const RKBndArgs<T> r)
+__device__ __forceinline__ void rk_bnd_node(const RKBndArgs<T> &r, long long t)
 {
     const BCArgs<T> &a = r.b;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < a.nb_air) {
         int i, j, k;
         boundary_node(t, a.air.nx, a.air.ny, a.air.nz, i, j, k);
@@ -275,6 +274,26 @@ __global__ __launch_bounds__(256) void k_rk_bnd(const RKBndArgs<T> r)
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_rk_bnd(const RKBndArgs<T> r)
+{
+    rk_bnd_node<T>(r, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Ensemble form: blocks first[d] .. first[d+1]-1 walk member d's boundary nodes.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rk_bnd_batch(const RKBndArgs<T> *__restrict__ args, const int *__restrict__ first,
+                                                      int n)
+{
+    int lo = 0, hi = n - 1;
+    const int b = blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    rk_bnd_node<T>(args[lo], (long long)(b - first[lo]) * blockDim.x + threadIdx.x);
+}
+
 }  // namespace
 
 template <typename T> void launch_rk_bnd(const RKBndArgs<T> &r, hipStream_t s)
@@ -282,6 +301,12 @@ template <typename T> void launch_rk_bnd(const RKBndArgs<T> &r, hipStream_t s)
     const long long n = r.b.nb_air + r.b.nb_pore;
     hipLaunchKernelGGL((k_rk_bnd<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, r);
 }
+template <typename T> void launch_rk_bnd_batch(const RKBndArgs<T> *args, const int *first, int n, int nblocks, hipStream_t s)
+{
+    if (nblocks > 0) hipLaunchKernelGGL((k_rk_bnd_batch<T>), dim3((unsigned)nblocks), dim3(256), 0, s, args, first, n);
+}
+template void launch_rk_bnd_batch<float>(const RKBndArgs<float> *, const int *, int, int, hipStream_t);
+template void launch_rk_bnd_batch<double>(const RKBndArgs<double> *, const int *, int, int, hipStream_t);
 template void launch_rk_bnd<float>(const RKBndArgs<float> &, hipStream_t);
 template void launch_rk_bnd<double>(const RKBndArgs<double> &, hipStream_t);
 

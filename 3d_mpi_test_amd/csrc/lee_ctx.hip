@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -428,75 +429,91 @@ template <typename T> void inject_source(lee_ctx *c, T *p)
                          m.d.src[1], m.d.src[2], gc, c->src_ps[N / c->src_period], c->stream);
 }
 
+// Kernel arguments of RK4 stage `st` for the y1 gate state (J1, y1on): stage state -> next
+// stage state of the air, of the porous layer, and the face walks of the destination.
+template <typename T>
+void rk4_stage_args(lee_ctx *c, int st, int J1, int y1on, RKArgs<T> &r, RKPoreArgs<T> &pr, RKBndArgs<T> &b)
+{
+    const Model &m = c->m;
+    T *src[4], *dst[4];
+    rk_stage_buffers<T>(c, 0, st, src, dst);
+    r = RKArgs<T>{};
+    r.a = air_args<T>(c, J1, y1on);
+    Grid<T> &g = r.a.g;
+    g.un = src[0]; g.vn = src[1]; g.wn = src[2]; g.pn = src[3];
+    g.unn = g.vnn = g.wnn = nullptr;
+    g.pnn = src[3];  // the velocity increment reads the stage pressure
+    for (int f = 0; f < 4; ++f) {
+        r.xn[f] = (const T *)c->air[c->cur * 4 + f];
+        r.acc[f] = (T *)c->rk_acc[0][f]; r.lo[f] = (T *)c->rk_lo[0][f]; r.err[f] = (T *)c->rk_err[0][f];
+        r.dst[f] = dst[f];
+    }
+    for (int q = 0; q < 6; ++q)
+        for (int f = 0; f < 4; ++f) {
+            T *qs, *qd;
+            rk_stage_q<T>(c, st, q, f, qs, qd);
+            r.a.Qo[q][f] = qs; r.a.Qn[q][f] = qs;
+            r.qxn[q][f] = (const T *)c->Q[c->qcur][q][f];
+            r.qacc[q][f] = (T *)c->rkq_acc[q][f];
+            r.qdst[q][f] = qd;
+        }
+    r.stage = st;
+    T *psrc[4] = {}, *pdst[4] = {};
+    pr = RKPoreArgs<T>{};
+    if (m.south_porous) {
+        rk_stage_buffers<T>(c, 1, st, psrc, pdst);
+        pr.g = pore_grid<T>(c);
+        pr.g.un = psrc[0]; pr.g.vn = psrc[1]; pr.g.wn = psrc[2]; pr.g.pn = psrc[3];
+        pr.g.unn = pr.g.vnn = pr.g.wnn = pr.g.pnn = nullptr;
+        pr.gama = (T)m.p_gama; pr.eff = (T)m.p_eff; pr.beta = (T)m.p_beta;
+        for (int f = 0; f < 4; ++f) {
+            pr.xn[f] = (const T *)c->pore[c->cur * 4 + f];
+            pr.acc[f] = (T *)c->rk_acc[1][f]; pr.lo[f] = (T *)c->rk_lo[1][f]; pr.err[f] = (T *)c->rk_err[1][f];
+            pr.dst[f] = pdst[f];
+        }
+        pr.stage = st;
+    }
+    // faces of the destination state and the boundary-node auxiliaries
+    b = RKBndArgs<T>{};
+    b.b = bc_args<T>(c, r.a, nullptr);
+    b.b.tdbc = 0;
+    b.b.nrec = 0;
+    b.b.air.unn = dst[0]; b.b.air.vnn = dst[1]; b.b.air.wnn = dst[2]; b.b.air.pnn = dst[3];
+    if (m.south_porous) {
+        b.b.pore.unn = pdst[0]; b.b.pore.vnn = pdst[1]; b.b.pore.wnn = pdst[2]; b.b.pore.pnn = pdst[3];
+    }
+    for (int f = 0; f < 4; ++f) b.src[f] = src[f];
+    for (int q = 0; q < 6; ++q)
+        for (int f = 0; f < 4; ++f) { b.qxn[q][f] = r.qxn[q][f]; b.qacc[q][f] = r.qacc[q][f]; b.qdst[q][f] = r.qdst[q][f]; }
+    b.stage = st;
+}
+
+int rk_wind_mode(const Model &m) { return m.has_wind ? (m.wind_v_only ? 2 : 1) : 0; }
+
 // One RK4 step: launches only (no host synchronisation), receivers into rec_slot.
 template <typename T> void enqueue_rk4_step(lee_ctx *c, double *rec_slot)
 {
     Model &m = c->m;
     hipStream_t s = c->stream;
     const int nr = (int)m.ri.size();
-    const int wind = m.has_wind ? (m.wind_v_only ? 2 : 1) : 0;
+    const int wind = rk_wind_mode(m);
     int J1, y1on;
     gate_tick(c, J1, y1on);
     // receivers sample level n before the step (airpore.cpp:573-580)
     launch_sample<T>((const T *)c->air[c->cur * 4 + 3], c->rec_idx, nr, rec_slot, s);
     for (int st = 0; st < 4; ++st) {
-        T *src[4], *dst[4];
-        rk_stage_buffers<T>(c, 0, st, src, dst);
         RKArgs<T> r;
-        r.a = air_args<T>(c, J1, y1on);
-        Grid<T> &g = r.a.g;
-        g.un = src[0]; g.vn = src[1]; g.wn = src[2]; g.pn = src[3];
-        g.unn = g.vnn = g.wnn = nullptr;
-        g.pnn = src[3];  // the velocity increment reads the stage pressure
-        for (int f = 0; f < 4; ++f) {
-            r.xn[f] = (const T *)c->air[c->cur * 4 + f];
-            r.acc[f] = (T *)c->rk_acc[0][f]; r.lo[f] = (T *)c->rk_lo[0][f]; r.err[f] = (T *)c->rk_err[0][f];
-            r.dst[f] = dst[f];
-        }
-        for (int q = 0; q < 6; ++q)
-            for (int f = 0; f < 4; ++f) {
-                T *qs, *qd;
-                rk_stage_q<T>(c, st, q, f, qs, qd);
-                r.a.Qo[q][f] = qs; r.a.Qn[q][f] = qs;
-                r.qxn[q][f] = (const T *)c->Q[c->qcur][q][f];
-                r.qacc[q][f] = (T *)c->rkq_acc[q][f];
-                r.qdst[q][f] = qd;
-            }
-        r.stage = st;
+        RKPoreArgs<T> pr;
+        RKBndArgs<T> b;
+        rk4_stage_args<T>(c, st, J1, y1on, r, pr, b);
         prof_mark(c, KS_RK_AIR, true);
         launch_rk_air<T>(r, wind, m.terrain, s);
         prof_mark(c, KS_RK_AIR, false);
-        T *psrc[4] = {}, *pdst[4] = {};
         if (m.south_porous) {
-            rk_stage_buffers<T>(c, 1, st, psrc, pdst);
-            RKPoreArgs<T> pr;
-            pr.g = pore_grid<T>(c);
-            pr.g.un = psrc[0]; pr.g.vn = psrc[1]; pr.g.wn = psrc[2]; pr.g.pn = psrc[3];
-            pr.g.unn = pr.g.vnn = pr.g.wnn = pr.g.pnn = nullptr;
-            pr.gama = (T)m.p_gama; pr.eff = (T)m.p_eff; pr.beta = (T)m.p_beta;
-            for (int f = 0; f < 4; ++f) {
-                pr.xn[f] = (const T *)c->pore[c->cur * 4 + f];
-                pr.acc[f] = (T *)c->rk_acc[1][f]; pr.lo[f] = (T *)c->rk_lo[1][f]; pr.err[f] = (T *)c->rk_err[1][f];
-                pr.dst[f] = pdst[f];
-            }
-            pr.stage = st;
             prof_mark(c, KS_RK_PORE, true);
             launch_rk_pore<T>(pr, m.terrain, s);
             prof_mark(c, KS_RK_PORE, false);
         }
-        // faces of the destination state and the boundary-node auxiliaries
-        RKBndArgs<T> b;
-        b.b = bc_args<T>(c, r.a, nullptr);
-        b.b.tdbc = 0;
-        b.b.nrec = 0;
-        b.b.air.unn = dst[0]; b.b.air.vnn = dst[1]; b.b.air.wnn = dst[2]; b.b.air.pnn = dst[3];
-        if (m.south_porous) {
-            b.b.pore.unn = pdst[0]; b.b.pore.vnn = pdst[1]; b.b.pore.wnn = pdst[2]; b.b.pore.pnn = pdst[3];
-        }
-        for (int f = 0; f < 4; ++f) b.src[f] = src[f];
-        for (int q = 0; q < 6; ++q)
-            for (int f = 0; f < 4; ++f) { b.qxn[q][f] = r.qxn[q][f]; b.qacc[q][f] = r.qacc[q][f]; b.qdst[q][f] = r.qdst[q][f]; }
-        b.stage = st;
         prof_mark(c, KS_RK_BND, true);
         launch_rk_bnd<T>(b, s);
         prof_mark(c, KS_RK_BND, false);
@@ -1342,5 +1359,331 @@ double lee_copy_bench(lee_ctx *c, long long n_words, int repeats)
     (void)hipFree(b);
     return (double)ms / repeats;
 }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------
+// Ensemble: many independent domains advanced together.  Every RK4 stage is three launches
+// for all members (air, porous layer, faces); each block finds its member through a block
+// prefix and reads that member's kernel arguments from a device table.  Members are full
+// contexts (state, receivers, restart files) that lee_ensemble_member exposes.
+// ------------------------------------------------------------------------------------
+struct lee_ensemble {
+    std::vector<lee_ctx *> m;
+    int prec = 32, wind = 0;
+    bool terrain = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.f;
+    std::string err;
+    std::vector<void *> allocs;
+    void *air_args[4] = {}, *pore_args[4] = {}, *bnd_args[4] = {};
+    int *first = nullptr;  // [air n+1 | pore np+1 | bnd n+1]
+    int air_blocks = 0, pore_blocks = 0, bnd_blocks = 0, n_pore = 0;
+    std::vector<int> key;  // (J1, y1on) per member the tables hold
+    int version = 0;
+    void **rec_p = nullptr;
+    long long *rec_idx = nullptr;
+    double *rec_buf = nullptr;
+    int nrec = 0;
+    std::vector<int> rec_off;
+    struct Graph { int n, version; hipGraphExec_t exec; };
+    std::vector<Graph> graphs;
+
+    ~lee_ensemble()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
+        for (void *p : allocs) (void)hipFree(p);
+        for (lee_ctx *c : m) delete c;
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int ens_err(lee_ensemble *e, int code, const std::string &msg)
+{
+    if (e) e->err = msg;
+    set_global_error(msg);
+    return code;
+}
+
+#define EHIP(e, expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return ens_err((e), LEE_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int ens_alloc(lee_ensemble *e, void **out, size_t bytes)
+{
+    void *p = nullptr;
+    EHIP(e, hipMalloc(&p, std::max<size_t>(bytes, 8)));
+    e->allocs.push_back(p);
+    *out = p;
+    return LEE_OK;
+}
+
+// Argument tables of the four stages for the members' current gate states.
+template <typename T> int ens_tables(lee_ensemble *e, const std::vector<int> &J1, const std::vector<int> &y1on)
+{
+    const int n = (int)e->m.size();
+    EHIP(e, hipStreamSynchronize(e->stream));  // no launch may still read the old tables
+    for (int st = 0; st < 4; ++st) {
+        std::vector<RKArgs<T>> ra(n);
+        std::vector<RKPoreArgs<T>> pa;
+        std::vector<RKBndArgs<T>> ba(n);
+        for (int d = 0; d < n; ++d) {
+            RKPoreArgs<T> pr;
+            rk4_stage_args<T>(e->m[d], st, J1[d], y1on[d], ra[d], pr, ba[d]);
+            if (e->m[d]->m.south_porous) pa.push_back(pr);
+        }
+        EHIP(e, hipMemcpy(e->air_args[st], ra.data(), n * sizeof(RKArgs<T>), hipMemcpyHostToDevice));
+        EHIP(e, hipMemcpy(e->bnd_args[st], ba.data(), n * sizeof(RKBndArgs<T>), hipMemcpyHostToDevice));
+        if (!pa.empty())
+            EHIP(e, hipMemcpy(e->pore_args[st], pa.data(), pa.size() * sizeof(RKPoreArgs<T>), hipMemcpyHostToDevice));
+    }
+    for (int d = 0; d < n; ++d) { e->key[2 * d] = J1[d]; e->key[2 * d + 1] = y1on[d]; }
+    e->version += 1;
+    return LEE_OK;
+}
+
+// One ensemble RK4 step (launches only once the tables are current).
+template <typename T> int ens_step(lee_ensemble *e, double *rec_slot)
+{
+    const int n = (int)e->m.size();
+    std::vector<int> J1(n), y1on(n);
+    bool stale = false;
+    for (int d = 0; d < n; ++d) {
+        gate_tick(e->m[d], J1[d], y1on[d]);
+        stale |= e->key[2 * d] != J1[d] || e->key[2 * d + 1] != y1on[d];
+    }
+    if (stale) {
+        int rc = ens_tables<T>(e, J1, y1on);
+        if (rc) return rc;
+    }
+    hipStream_t s = e->stream;
+    launch_sample_batch<T>((const T *const *)e->rec_p, e->rec_idx, e->nrec, rec_slot, s);
+    const int *fa = e->first, *fp = e->first + n + 1, *fb = e->first + n + 1 + e->n_pore + 1;
+    for (int st = 0; st < 4; ++st) {
+        launch_rk_air_batch<T>((const RKArgs<T> *)e->air_args[st], fa, n, e->air_blocks, e->wind, e->terrain, s);
+        if (e->n_pore)
+            launch_rk_pore_batch<T>((const RKPoreArgs<T> *)e->pore_args[st], fp, e->n_pore, e->pore_blocks, e->terrain, s);
+        launch_rk_bnd_batch<T>((const RKBndArgs<T> *)e->bnd_args[st], fb, n, e->bnd_blocks, s);
+    }
+    for (lee_ctx *c : e->m) c->steps += 1;
+    return LEE_OK;
+}
+
+// Steps from a captured graph when no member's gate changes inside the chunk.
+bool ens_gate_constant(const lee_ensemble *e, int n)
+{
+    for (const lee_ctx *c : e->m)
+        if (!gate_constant(c, n)) return false;
+    return true;
+}
+
+template <typename T> int ens_run(lee_ensemble *e, int nsteps, double *rec_out, int use_graph)
+{
+    hipStream_t s = e->stream;
+    EHIP(e, hipEventRecord(e->ev0, s));
+    int done = 0;
+    while (done < nsteps) {
+        int n = std::min(nsteps - done, kGraphSteps);
+        bool replayed = false;
+        if (use_graph && ens_gate_constant(e, n)) {
+            // the gate states hold for the whole chunk: bring the tables up to date before
+            // capturing (peek the coming step's state without advancing the counters)
+            std::vector<int> J1(e->m.size()), y1on(e->m.size());
+            bool stale = false;
+            for (size_t d = 0; d < e->m.size(); ++d) {
+                lee_ctx *c = e->m[d];
+                const long long save = c->counter;
+                gate_tick(c, J1[d], y1on[d]);
+                c->counter = save;
+                stale |= e->key[2 * d] != J1[d] || e->key[2 * d + 1] != y1on[d];
+            }
+            if (stale) {
+                int rc = ens_tables<T>(e, J1, y1on);
+                if (rc) return rc;
+            }
+            hipGraphExec_t exec = nullptr;
+            for (auto &g : e->graphs)
+                if (g.n == n && g.version == e->version) exec = g.exec;
+            if (exec) {
+                EHIP(e, hipGraphLaunch(exec, s));
+                for (lee_ctx *c : e->m) {
+                    const Model &m = c->m;
+                    if (m.judge[1] == 1 || m.judge[1] == 3) c->counter += n;
+                    c->steps += n;
+                }
+            } else {
+                EHIP(e, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                for (int t = 0; t < n; ++t) {
+                    int rc = ens_step<T>(e, e->rec_buf + (size_t)t * e->nrec);
+                    if (rc) { hipGraph_t g0; (void)hipStreamEndCapture(s, &g0); return rc; }
+                }
+                hipGraph_t graph = nullptr;
+                EHIP(e, hipStreamEndCapture(s, &graph));
+                lee_ensemble::Graph g{n, e->version, nullptr};
+                hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                EHIP(e, ie);
+                if (e->graphs.size() >= 4) {
+                    (void)hipGraphExecDestroy(e->graphs.front().exec);
+                    e->graphs.erase(e->graphs.begin());
+                }
+                e->graphs.push_back(g);
+                EHIP(e, hipGraphLaunch(g.exec, s));
+            }
+            replayed = true;
+        }
+        if (!replayed) {
+            n = 1;
+            int rc = ens_step<T>(e, e->rec_buf);
+            if (rc) return rc;
+        }
+        if (rec_out && e->nrec) {
+            EHIP(e, hipMemcpyAsync(rec_out + (size_t)done * e->nrec, e->rec_buf, (size_t)n * e->nrec * 8,
+                                   hipMemcpyDeviceToHost, s));
+            EHIP(e, hipStreamSynchronize(s));
+        }
+        done += n;
+    }
+    EHIP(e, hipEventRecord(e->ev1, s));
+    EHIP(e, hipEventSynchronize(e->ev1));
+    EHIP(e, hipEventElapsedTime(&e->last_ms, e->ev0, e->ev1));
+    return LEE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lee_ensemble_create(int n, const lee_deck *decks, const lee_options *opts, lee_ensemble **out)
+{
+    if (!out || n < 1 || !decks) return ens_err(nullptr, LEE_ERR_ARG, "bad arguments");
+    *out = nullptr;
+    std::unique_ptr<lee_ensemble> e(new lee_ensemble());
+    lee_ensemble *E = e.get();
+    for (int d = 0; d < n; ++d) {
+        lee_options o;
+        lee_options_default(&o);
+        if (opts) o = opts[d];
+        lee_ctx *c = nullptr;
+        int rc = create_impl(&decks[d], &o, &c);
+        if (rc) return ens_err(E, rc, "member " + std::to_string(d) + ": " + g_global_err);
+        E->m.push_back(c);
+        const Model &m = c->m;
+        if (!m.rk4) return ens_err(E, LEE_ERR_UNSUPPORTED, "ensemble members step with RK4 (integrator = LEE_INT_RK4)");
+        if (m.d.mf_judge == 1) return ens_err(E, LEE_ERR_UNSUPPORTED, "ensemble members have a fixed frame");
+        if (d == 0) {
+            E->prec = c->prec; E->wind = rk_wind_mode(m); E->terrain = m.terrain;
+        } else if (c->prec != E->prec || rk_wind_mode(m) != E->wind || m.terrain != E->terrain) {
+            return ens_err(E, LEE_ERR_UNSUPPORTED, "ensemble members share precision, wind kind and grid kind");
+        }
+        rc = lee_frame_setup(c, 0);
+        if (rc) return ens_err(E, rc, "member " + std::to_string(d) + ": " + c->err);
+        if (c->src_ps.size()) return ens_err(E, LEE_ERR_UNSUPPORTED, "no time-series source in an ensemble");
+    }
+    EHIP(E, hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+    EHIP(E, hipEventCreate(&E->ev0));
+    EHIP(E, hipEventCreate(&E->ev1));
+    int rc;
+    for (int st = 0; st < 4; ++st) {
+        const size_t sa = E->prec == 64 ? sizeof(RKArgs<double>) : sizeof(RKArgs<float>);
+        const size_t sp = E->prec == 64 ? sizeof(RKPoreArgs<double>) : sizeof(RKPoreArgs<float>);
+        const size_t sb = E->prec == 64 ? sizeof(RKBndArgs<double>) : sizeof(RKBndArgs<float>);
+        if ((rc = ens_alloc(E, &E->air_args[st], n * sa)) || (rc = ens_alloc(E, &E->pore_args[st], n * sp)) ||
+            (rc = ens_alloc(E, &E->bnd_args[st], n * sb)))
+            return rc;
+    }
+    // block prefixes: interior tiles of the air and porous grids, boundary nodes in 256s
+    std::vector<int> fa(1, 0), fp(1, 0), fb(1, 0);
+    for (lee_ctx *c : E->m) {
+        const Model &m = c->m;
+        auto tiles = [](int nx, int ny, int nz) {
+            return (long long)((nx - 2 + kTileX - 1) / kTileX) * ((ny - 2 + kTileY - 1) / kTileY) * (nz - 2);
+        };
+        long long ta = tiles(m.nx, m.ny, m.nz);
+        long long nb = (c->prec == 64 ? boundary_count<double>(m.nx, m.ny, m.nz) : boundary_count<float>(m.nx, m.ny, m.nz));
+        if (m.south_porous) {
+            fp.push_back((int)(fp.back() + tiles(m.pnx, m.pny, m.pnz)));
+            nb += c->prec == 64 ? boundary_count<double>(m.pnx, m.pny, m.pnz) : boundary_count<float>(m.pnx, m.pny, m.pnz);
+        }
+        if (fa.back() + ta > INT_MAX / 2 || fb.back() + (nb + 255) / 256 > INT_MAX / 2)
+            return ens_err(E, LEE_ERR_ARG, "ensemble too large for one launch");
+        fa.push_back((int)(fa.back() + ta));
+        fb.push_back((int)(fb.back() + (nb + 255) / 256));
+    }
+    E->air_blocks = fa.back(); E->pore_blocks = fp.back(); E->bnd_blocks = fb.back();
+    E->n_pore = (int)fp.size() - 1;
+    std::vector<int> first(fa);
+    first.insert(first.end(), fp.begin(), fp.end());
+    first.insert(first.end(), fb.begin(), fb.end());
+    if ((rc = ens_alloc(E, (void **)&E->first, first.size() * sizeof(int)))) return rc;
+    EHIP(E, hipMemcpy(E->first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice));
+    // receivers of all members, concatenated
+    std::vector<void *> rp;
+    std::vector<long long> ri;
+    for (lee_ctx *c : E->m) {
+        E->rec_off.push_back((int)ri.size());
+        const Model &m = c->m;
+        for (int r = 0; r < (int)m.ri.size(); ++r) {
+            rp.push_back(c->air[c->cur * 4 + 3]);
+            ri.push_back(m.receiver_index(r, 0));
+        }
+    }
+    E->rec_off.push_back((int)ri.size());
+    E->nrec = (int)ri.size();
+    if ((rc = ens_alloc(E, (void **)&E->rec_p, rp.size() * sizeof(void *))) ||
+        (rc = ens_alloc(E, (void **)&E->rec_idx, ri.size() * sizeof(long long))) ||
+        (rc = ens_alloc(E, (void **)&E->rec_buf, (size_t)kGraphSteps * std::max(E->nrec, 1) * 8)))
+        return rc;
+    if (E->nrec) {
+        EHIP(E, hipMemcpy(E->rec_p, rp.data(), rp.size() * sizeof(void *), hipMemcpyHostToDevice));
+        EHIP(E, hipMemcpy(E->rec_idx, ri.data(), ri.size() * sizeof(long long), hipMemcpyHostToDevice));
+    }
+    E->key.assign(2 * n, -1);  // forces the first table build
+    *out = e.release();
+    return LEE_OK;
+}
+
+int lee_ensemble_destroy(lee_ensemble *e)
+{
+    delete e;
+    return LEE_OK;
+}
+
+int lee_ensemble_size(const lee_ensemble *e) { return e ? (int)e->m.size() : 0; }
+
+int lee_ensemble_member(lee_ensemble *e, int d, lee_ctx **ctx)
+{
+    if (!e || !ctx || d < 0 || d >= (int)e->m.size()) return ens_err(e, LEE_ERR_ARG, "bad member");
+    *ctx = e->m[d];
+    return LEE_OK;
+}
+
+int lee_ensemble_receivers(const lee_ensemble *e, int *offsets, int cap)
+{
+    if (!e) return 0;
+    const int n = (int)e->rec_off.size();
+    for (int d = 0; d < n && d < cap; ++d) offsets[d] = e->rec_off[d];
+    return n;
+}
+
+int lee_ensemble_step(lee_ensemble *e, int nsteps, double *rec_out, int use_graph)
+{
+    if (!e || nsteps < 0) return ens_err(e, LEE_ERR_ARG, "bad arguments");
+    if (nsteps == 0) return LEE_OK;
+    return e->prec == 64 ? ens_run<double>(e, nsteps, rec_out, use_graph) : ens_run<float>(e, nsteps, rec_out, use_graph);
+}
+
+double lee_ensemble_last_ms(const lee_ensemble *e) { return e ? (double)e->last_ms : 0.0; }
+
+const char *lee_ensemble_error(const lee_ensemble *e) { return e ? e->err.c_str() : g_global_err.c_str(); }
 
 }  // extern "C"

@@ -105,6 +105,15 @@ template <typename T> struct RKBndArgs {
 template <typename T> void launch_rk_air(const RKArgs<T> &r, int wind, bool terrain, hipStream_t s);
 template <typename T> void launch_rk_pore(const RKPoreArgs<T> &r, bool terrain, hipStream_t s);
 template <typename T> void launch_rk_bnd(const RKBndArgs<T> &r, hipStream_t s);
+// Ensemble forms: args[d] in device memory for member d, first[d] = its first block
+// (first[n] = nblocks); interior tiles of BX x BY points for air/pore, 256 boundary nodes for bnd.
+constexpr int kTileX = 64, kTileY = 4;
+template <typename T>
+void launch_rk_air_batch(const RKArgs<T> *args, const int *first, int n, int nblocks, int wind, bool terrain, hipStream_t s);
+template <typename T>
+void launch_rk_pore_batch(const RKPoreArgs<T> *args, const int *first, int n, int nblocks, bool terrain, hipStream_t s);
+template <typename T> void launch_rk_bnd_batch(const RKBndArgs<T> *args, const int *first, int n, int nblocks, hipStream_t s);
+template <typename T> void launch_sample_batch(const T *const *p, const long long *idx, int n, double *out, hipStream_t s);
 
 template <typename T> long long boundary_count(int nx, int ny, int nz);
 template <typename T> void launch_bc_p_all(const BCArgs<T> &a, hipStream_t s);

@@ -21,6 +21,7 @@ namespace lee {
 namespace {
 
 constexpr int BX = 64, BY = 4;
+static_assert(BX == kTileX && BY == kTileY, "ensemble block prefixes are counted in BX x BY tiles");
 
 template <typename T> __device__ __forceinline__ void metric(const Grid<T> &g, int i, int j, int k, T &mX, T &mY, T &mZ)
 {
@@ -728,11 +729,9 @@ __device__ __forceinline__ T rk_update(int st, T xn, T k, T *acc, T *lo, T *err,
 }
 
 template <typename T, int WM, bool TERRAIN, bool COMP>
-__global__ __launch_bounds__(BX * BY) void k_rk_air(const RKArgs<T> r)
+__device__ __forceinline__ void rk_air_point(const RKArgs<T> &r, int i, int j, int k)
 {
     const AirArgs<T> &a = r.a;
-    const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
-    if (i >= a.g.nx - 1 || j >= a.g.ny - 1) return;
     const size_t id = (size_t)i + (size_t)a.g.nx * ((size_t)j + (size_t)a.g.ny * k);
     const int st = r.stage;
     T kf[4];
@@ -757,15 +756,49 @@ __global__ __launch_bounds__(BX * BY) void k_rk_air(const RKArgs<T> r)
         }
 }
 
+template <typename T, int WM, bool TERRAIN, bool COMP>
+__global__ __launch_bounds__(BX * BY) void k_rk_air(const RKArgs<T> r)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    if (i >= r.a.g.nx - 1 || j >= r.a.g.ny - 1) return;
+    rk_air_point<T, WM, TERRAIN, COMP>(r, i, j, k);
+}
+
+// Ensemble form: one launch steps every member.  The flat block index selects the member by
+// the block prefix `first` (first[d] .. first[d+1]-1 are member d's BX x BY tiles, x fastest,
+// then y, then interior planes) and reads that member's arguments from device memory.
+__device__ __forceinline__ int batch_member(const int *first, int n, int b)
+{
+    int lo = 0, hi = n - 1;  // largest d with first[d] <= b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T, int WM, bool TERRAIN, bool COMP>
+__global__ __launch_bounds__(BX * BY) void k_rk_air_batch(const RKArgs<T> *__restrict__ args, const int *__restrict__ first,
+                                                         int n)
+{
+    const int d = batch_member(first, n, blockIdx.x);
+    const RKArgs<T> &r = args[d];
+    const int nx = r.a.g.nx, ny = r.a.g.ny;
+    const int tx = (nx - 2 + BX - 1) / BX, ty = (ny - 2 + BY - 1) / BY;
+    const int local = blockIdx.x - first[d];
+    const int i = (local % tx) * BX + threadIdx.x + 1, j = ((local / tx) % ty) * BY + threadIdx.y + 1,
+              k = local / (tx * ty) + 1;
+    if (i >= nx - 1 || j >= ny - 1) return;
+    rk_air_point<T, WM, TERRAIN, COMP>(r, i, j, k);
+}
+
 // Porous layer: dt*R_p = -gama div u (porous.cpp:77-95); dt*R_u = -dt eff grad p - beta u,
 // the drag of porous.cpp:112-135 as the ODE du/dt = -(1/rho_e) grad p - (sigma/rho_e) u.
 template <typename T, bool TERRAIN, bool COMP>
-__global__ __launch_bounds__(BX * BY) void k_rk_pore(const RKPoreArgs<T> r)
+__device__ __forceinline__ void rk_pore_point(const RKPoreArgs<T> &r, int i, int j, int k)
 {
     const Grid<T> &g = r.g;
-    const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
     const int nx = g.nx, ny = g.ny;
-    if (i >= nx - 1 || j >= ny - 1) return;
     const size_t sxy = (size_t)nx * ny;
     const size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
     T mX, mY, mZ, zX = T(0), zY = T(0);
@@ -796,6 +829,29 @@ __global__ __launch_bounds__(BX * BY) void k_rk_pore(const RKPoreArgs<T> r)
         const T xn = st == 0 ? xc[q] : r.xn[q][id];
         r.dst[q][id] = rk_update<T, COMP>(st, xn, kf[q], r.acc[q], r.lo[q], r.err[q], id);
     }
+}
+
+template <typename T, bool TERRAIN, bool COMP>
+__global__ __launch_bounds__(BX * BY) void k_rk_pore(const RKPoreArgs<T> r)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    if (i >= r.g.nx - 1 || j >= r.g.ny - 1) return;
+    rk_pore_point<T, TERRAIN, COMP>(r, i, j, k);
+}
+
+template <typename T, bool TERRAIN, bool COMP>
+__global__ __launch_bounds__(BX * BY) void k_rk_pore_batch(const RKPoreArgs<T> *__restrict__ args,
+                                                          const int *__restrict__ first, int n)
+{
+    const int d = batch_member(first, n, blockIdx.x);
+    const RKPoreArgs<T> &r = args[d];
+    const int nx = r.g.nx, ny = r.g.ny;
+    const int tx = (nx - 2 + BX - 1) / BX, ty = (ny - 2 + BY - 1) / BY;
+    const int local = blockIdx.x - first[d];
+    const int i = (local % tx) * BX + threadIdx.x + 1, j = ((local / tx) % ty) * BY + threadIdx.y + 1,
+              k = local / (tx * ty) + 1;
+    if (i >= nx - 1 || j >= ny - 1) return;
+    rk_pore_point<T, TERRAIN, COMP>(r, i, j, k);
 }
 
 // ------------------------------------------------------------------------------------
@@ -877,6 +933,18 @@ __global__ void k_copy_plane_j(T *p, int nx, int ny, int nz)
     if (i >= nx) return;
     const size_t b = (size_t)i + (size_t)nx * (size_t)ny * k;
     p[b] = p[b + nx];
+}
+
+// Ensemble receivers: receiver r reads member field p[r] at idx[r].
+template <typename T>
+__global__ void k_sample_batch(const T *const *p, const long long *idx, int n, double *out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const long long q = idx[r];
+    double v = q >= 0 ? (double)p[r][q] : 0.0;
+    if (fabs(v) < 1.0e-100) v = 0.0;
+    out[r] = v;
 }
 
 template <typename T>
@@ -985,6 +1053,33 @@ template <typename T> void launch_rk_air(const RKArgs<T> &r, int wind, bool terr
 #undef LEE_RK
 }
 
+template <typename T>
+void launch_rk_air_batch(const RKArgs<T> *args, const int *first, int n, int nblocks, int wind, bool terrain, hipStream_t s)
+{
+    constexpr bool C = std::is_same<T, float>::value;
+    if (nblocks <= 0) return;
+    dim3 g((unsigned)nblocks), b(BX, BY);
+#define LEE_RK(WM, TER) hipLaunchKernelGGL((k_rk_air_batch<T, WM, TER, C>), g, b, 0, s, args, first, n)
+    if (terrain) {
+        if (wind == 1) LEE_RK(1, true); else if (wind == 2) LEE_RK(2, true); else LEE_RK(0, true);
+    } else {
+        if (wind == 1) LEE_RK(1, false); else if (wind == 2) LEE_RK(2, false); else LEE_RK(0, false);
+    }
+#undef LEE_RK
+}
+template <typename T>
+void launch_rk_pore_batch(const RKPoreArgs<T> *args, const int *first, int n, int nblocks, bool terrain, hipStream_t s)
+{
+    constexpr bool C = std::is_same<T, float>::value;
+    if (nblocks <= 0) return;
+    dim3 g((unsigned)nblocks), b(BX, BY);
+    if (terrain) hipLaunchKernelGGL((k_rk_pore_batch<T, true, C>), g, b, 0, s, args, first, n);
+    else hipLaunchKernelGGL((k_rk_pore_batch<T, false, C>), g, b, 0, s, args, first, n);
+}
+template <typename T> void launch_sample_batch(const T *const *p, const long long *idx, int n, double *out, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL((k_sample_batch<T>), dim3((n + 63) / 64), dim3(64), 0, s, p, idx, n, out);
+}
 template <typename T> void launch_rk_pore(const RKPoreArgs<T> &r, bool terrain, hipStream_t s)
 {
     constexpr bool C = std::is_same<T, float>::value;
@@ -1068,6 +1163,9 @@ void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s)
     template bool fused_fits<T>(int);                                                                        \
     template void launch_rk_air<T>(const RKArgs<T> &, int, bool, hipStream_t);                               \
     template void launch_rk_pore<T>(const RKPoreArgs<T> &, bool, hipStream_t);                               \
+    template void launch_rk_air_batch<T>(const RKArgs<T> *, const int *, int, int, int, bool, hipStream_t);   \
+    template void launch_rk_pore_batch<T>(const RKPoreArgs<T> *, const int *, int, int, bool, hipStream_t);   \
+    template void launch_sample_batch<T>(const T *const *, const long long *, int, double *, hipStream_t);     \
     template void launch_air_fb<T>(const AirArgs<T> &, bool, hipStream_t);                                   \
     template void launch_pore_p<T>(const PoreArgs<T> &, hipStream_t);                                        \
     template void launch_pore_v<T>(const PoreArgs<T> &, hipStream_t);                                        \

@@ -249,6 +249,25 @@ int  lee_set_source_series(lee_ctx *ctx, const double *ps, long long n, int peri
  * negative status. */
 long long lee_read_source_series(const char *dir, double *ps, long long cap);
 
+/* Ensemble: n independent domains (decks[d] with opts[d]; opts may be NULL) stepped together,
+ * each RK4 stage as one launch per kernel kind over every member -- the ensemble runner of
+ * parameter sweeps.  No counterpart in the reference, which runs one domain per MPI job.
+ * Members must share precision, wind kind (none, U V W, or V only) and grid kind (flat or
+ * terrain), integrate with RK4 and keep a fixed frame.  Each member is a full lee_ctx
+ * (lee_ensemble_member) for state, restart and contour calls between steps; do not step a
+ * member on its own.  rec_out takes nsteps rows of all members' receivers, member d's at
+ * columns offsets[d] .. offsets[d+1]-1 (lee_ensemble_receivers fills n+1 offsets).
+ * use_graph replays captured chunks of steps. */
+typedef struct lee_ensemble lee_ensemble;
+int  lee_ensemble_create(int n, const lee_deck *decks, const lee_options *opts, lee_ensemble **out);
+int  lee_ensemble_destroy(lee_ensemble *e);
+int  lee_ensemble_size(const lee_ensemble *e);
+int  lee_ensemble_member(lee_ensemble *e, int d, lee_ctx **ctx);
+int  lee_ensemble_receivers(const lee_ensemble *e, int *offsets, int cap);
+int  lee_ensemble_step(lee_ensemble *e, int nsteps, double *rec_out, int use_graph);
+double lee_ensemble_last_ms(const lee_ensemble *e);
+const char *lee_ensemble_error(const lee_ensemble *e);
+
 /* Output writers of the drop-in file contract.
  * ./solution/pty%04d.dat: one %.6e sample per line (airpore.cpp:468-531, 577).
  * ./solution/p_f2_%04d.dat: zero-padded radix-2 FFT, "f |X|" at setprecision(10)
