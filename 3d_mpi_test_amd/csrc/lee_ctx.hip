@@ -466,6 +466,9 @@ void rk4_stage_args(lee_ctx *c, int st, int J1, int y1on, RKArgs<T> &r, RKPoreAr
         pr.g.un = psrc[0]; pr.g.vn = psrc[1]; pr.g.wn = psrc[2]; pr.g.pn = psrc[3];
         pr.g.unn = pr.g.vnn = pr.g.wnn = pr.g.pnn = nullptr;
         pr.gama = (T)m.p_gama; pr.eff = (T)m.p_eff; pr.beta = (T)m.p_beta;
+        const double E = std::exp(-m.p_beta), E2 = std::exp(-0.5 * m.p_beta);
+        const double cf[5] = {E, E2, E2 * 0.5, E / 6.0, E2 / 3.0};
+        for (int q = 0; q < 5; ++q) pr.cf[q] = (T)cf[q];
         for (int f = 0; f < 4; ++f) {
             pr.xn[f] = (const T *)c->pore[c->cur * 4 + f];
             pr.acc[f] = (T *)c->rk_acc[1][f]; pr.lo[f] = (T *)c->rk_lo[1][f]; pr.err[f] = (T *)c->rk_err[1][f];

@@ -90,6 +90,7 @@ template <typename T> struct RKArgs {
 template <typename T> struct RKPoreArgs {
     Grid<T> g;                 // stage state in g.un .. g.pn
     T gama, eff, beta;         // porous.cpp:44-50: dt/(Kp Omega), 1/rho_e, sigma dt/rho_e
+    T cf[5];                   // drag integrating factors E, E2, E2/2, E/6, E2/3 (E = e^-beta)
     const T *xn[4];
     T *acc[4], *lo[4], *err[4], *dst[4];
     int stage;

@@ -728,6 +728,49 @@ __device__ __forceinline__ T rk_update(int st, T xn, T k, T *acc, T *lo, T *err,
     }
 }
 
+// Lawson (integrating-factor) RK4 for the porous velocities, x' = -(beta/dt) x + N(x) with
+// k = dt N(stage): classical RK4 in the variable e^{beta t/dt} x, so the stiff Darcy drag is
+// integrated exactly (no step limit from sigma dt / rho_e).  With cf = {E, E2, E2/2, E/6, E2/3},
+// E = e^-beta, E2 = e^-beta/2:
+//   stage 0: acc = E x_n + (E/6) k,   next = E2 x_n + (E2/2) k
+//   stage 1: acc += (E2/3) k,         next = E2 x_n + k/2
+//   stage 2: acc += (E2/3) k,         next = E x_n + E2 k
+//   stage 3: x_{n+1} = acc + k/6
+// COMP carries the low part and the state error as rk_update does (the error scaled by E).
+template <typename T, bool COMP>
+__device__ __forceinline__ T rk_update_if(int st, T xn, T k, T *acc, T *lo, T *err, size_t id, const T *cf)
+{
+    const T E = cf[0], E2 = cf[1];
+    const T w = st == 0 ? cf[3] : st < 3 ? cf[4] : T(1.0 / 6.0);
+    const T cn = st == 2 ? E : E2, an = st == 0 ? cf[2] : st == 1 ? T(0.5) : E2;
+    if constexpr (!COMP) {
+        if (st == 0) {
+            acc[id] = E * xn + w * k;
+            return cn * xn + an * k;
+        }
+        if (st < 3) {
+            acc[id] = acc[id] + w * k;
+            return cn * xn + an * k;
+        }
+        return acc[id] + w * k;
+    } else {
+        const T y = w * k;
+        const T a0 = st == 0 ? E * xn : acc[id];
+        const T sm = a0 + y;
+        const T bb = sm - a0;
+        const T e = (a0 - (sm - bb)) + (y - bb);
+        const T l = (st == 0 ? E * err[id] : lo[id]) + e;
+        if (st < 3) {
+            acc[id] = sm;
+            lo[id] = l;
+            return cn * xn + an * k;
+        }
+        const T x = sm + l;
+        err[id] = l - (x - sm);
+        return x;
+    }
+}
+
 template <typename T, int WM, bool TERRAIN, bool COMP>
 __device__ __forceinline__ void rk_air_point(const RKArgs<T> &r, int i, int j, int k)
 {
@@ -792,8 +835,8 @@ __global__ __launch_bounds__(BX * BY) void k_rk_air_batch(const RKArgs<T> *__res
     rk_air_point<T, WM, TERRAIN, COMP>(r, i, j, k);
 }
 
-// Porous layer: dt*R_p = -gama div u (porous.cpp:77-95); dt*R_u = -dt eff grad p - beta u,
-// the drag of porous.cpp:112-135 as the ODE du/dt = -(1/rho_e) grad p - (sigma/rho_e) u.
+// Porous layer: dt*R_p = -gama div u (porous.cpp:77-95); dt*R_u = -dt eff grad p, with the
+// drag of porous.cpp:112-135, du/dt = ... - (sigma/rho_e) u, in the integrating factor.
 template <typename T, bool TERRAIN, bool COMP>
 __device__ __forceinline__ void rk_pore_point(const RKPoreArgs<T> &r, int i, int j, int k)
 {
@@ -822,12 +865,16 @@ __device__ __forceinline__ void rk_pore_point(const RKPoreArgs<T> &r, int i, int
     }
     const T f = (-g.dt) * r.eff;
     const T xc[4] = {g.un[id], g.vn[id], g.wn[id], pc};
-    const T kf[4] = {f * gx - r.beta * xc[0], f * gy - r.beta * xc[1], f * gz - r.beta * xc[2], (-r.gama) * div};
+    const T kf[4] = {f * gx, f * gy, f * gz, (-r.gama) * div};
     const int st = r.stage;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 3; ++q) {
         const T xn = st == 0 ? xc[q] : r.xn[q][id];
-        r.dst[q][id] = rk_update<T, COMP>(st, xn, kf[q], r.acc[q], r.lo[q], r.err[q], id);
+        r.dst[q][id] = rk_update_if<T, COMP>(st, xn, kf[q], r.acc[q], r.lo[q], r.err[q], id, r.cf);
+    }
+    {
+        const T xn = st == 0 ? xc[3] : r.xn[3][id];
+        r.dst[3][id] = rk_update<T, COMP>(st, xn, kf[3], r.acc[3], r.lo[3], r.err[3], id);
     }
 }
 

@@ -219,10 +219,6 @@ std::string Model::build(const lee_deck &deck, const lee_options &o, int &code)
         p_beta = d.pore_resistivity * d.dt * p_eff;
         p_gama = d.dt / Kp / d.pore_porosity;
         p_invbeta = 1.0 / (p_beta + 1.0);
-        if (rk4 && !(p_beta < 2.5)) {
-            code = LEE_ERR_UNSUPPORTED;
-            return "explicit RK4 porous drag needs sigma*dt/rho_e < 2.5 (got " + std::to_string(p_beta) + ")";
-        }
         // (int)(dz_air/dz_pore) < 5 selects the plane copy; the other branch is commented
         // out in the reference, which then couples nothing (calculation_2D.cpp:872-877).
         iface_copy = (int)(d.dz / d.pore_dz) < 5.0;

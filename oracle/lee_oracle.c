@@ -76,6 +76,7 @@ struct orc_sim {
     double *Psi;                       /* [i + nx*k][6] */
     /* porous coefficients (porous.cpp:40-51) */
     double p_eff, p_beta, p_gama, p_invbeta;
+    double p_E, p_E2;                  /* RK4 integrating factors e^-beta, e^-beta/2 */
     /* moving frame (mygeneral.h:62-68) */
     int mf_judge, mf_jmax, lead, trail, ijkmax; double ratio;
     int mf_count;
@@ -1386,9 +1387,9 @@ orc_sim *orc_create_opt(const lee_deck *d, const lee_options *opt, int fill_nn_n
         s->p_beta = d->pore_resistivity * d->dt * s->p_eff;                   /* porous.cpp:48 */
         s->p_gama = d->dt / Kp / d->pore_porosity;                            /* porous.cpp:49 */
         s->p_invbeta = 1.0 / (s->p_beta + 1.0);                               /* porous.cpp:50 */
-        if (opt->integrator == LEE_INT_RK4 && !(s->p_beta < 2.5)) {
-            orc_destroy(s); FAIL("explicit RK4 porous drag needs sigma*dt/rho_e < 2.5");
-        }
+        /* RK4 takes the Darcy drag exactly (integrating factor): e^-beta per step */
+        s->p_E = exp(-s->p_beta);
+        s->p_E2 = exp(-0.5 * s->p_beta);
     } else if (d->bc[LEE_FACE_S] != LEE_BC_RIGID && d->bc[LEE_FACE_S] != LEE_BC_ABSORBING) {
         orc_destroy(s); FAIL("South must be rigid, absorbing or porous");
     }
@@ -1565,22 +1566,24 @@ static void one_step(orc_sim *s)
 /* RK4 (extension; the reference parses scheme 5 but never integrates with it,           */
 /* calculation_2D.cpp:551-612)                                                           */
 /* ------------------------------------------------------------------------------------ */
-/* Porous velocity right-hand side times dt: the drag of porous.cpp:112-135 taken as the
- * ODE du/dt = -(1/rho_e) grad p - sigma/rho_e u, i.e. dt*f = -dt*eff*grad p - beta*u. */
+/* Porous velocity right-hand side times dt without the drag: porous.cpp:112-135 as the ODE
+ * du/dt = -(1/rho_e) grad p - (sigma/rho_e) u, whose stiff linear drag the RK4 step takes
+ * exactly through the integrating factor e^{-beta t/dt} (Lawson's RK4), so dt*f here is
+ * -dt*eff*grad p only. */
 static void pore_rk_duvw_on(orc_sim *s, const double *u, const double *v, const double *w,
                             const double *p, double *ou, double *ov, double *ow)
 {
     ogrid *g = &s->pore;
     double dxi = g->dxi, dyi = g->dyi, dzi = g->dzi, dt = g->dt;
-    double f = (-dt) * s->p_eff, be = s->p_beta;
+    double f = (-dt) * s->p_eff;
     for (int k = 1; k < g->nz - 1; ++k)
         for (int j = 1; j < g->ny - 1; ++j)
             for (int i = 1; i < g->nx - 1; ++i) {
                 metric9 m = point_metric(g, i, j, k);
                 size_t id = IDX(g, i, j, k);
-                ou[id] = f * (m.xX * BDX(p) * dxi + m.yX * BDY(p) * dyi + m.zX * BDZ(p) * dzi) - be * u[id];
-                ov[id] = f * (m.xY * BDX(p) * dxi + m.yY * BDY(p) * dyi + m.zY * BDZ(p) * dzi) - be * v[id];
-                ow[id] = f * (m.xZ * BDX(p) * dxi + m.yZ * BDY(p) * dyi + m.zZ * BDZ(p) * dzi) - be * w[id];
+                ou[id] = f * (m.xX * BDX(p) * dxi + m.yX * BDY(p) * dyi + m.zX * BDZ(p) * dzi);
+                ov[id] = f * (m.xY * BDX(p) * dxi + m.yY * BDY(p) * dyi + m.zY * BDZ(p) * dzi);
+                ow[id] = f * (m.xZ * BDX(p) * dxi + m.yZ * BDY(p) * dyi + m.zZ * BDZ(p) * dzi);
             }
 }
 
@@ -1594,6 +1597,20 @@ static double rk_combine(int st, double xn, double k, double *acc)
     if (st == 0) *acc = xn + RK_B[0] * k;
     else if (st < 3) *acc = *acc + RK_B[st] * k;
     return st < 3 ? xn + RK_A[st] * k : *acc + RK_B[3] * k;
+}
+
+/* Lawson (integrating-factor) RK4 for x' = -(beta/dt) x + N(x), k = dt N(stage):
+ *   stage 0: acc = E x_n + (E/6) k1,      next = E2 x_n + (E2/2) k1
+ *   stage 1: acc += (E2/3) k2,            next = E2 x_n + (1/2) k2
+ *   stage 2: acc += (E2/3) k3,            next = E x_n + E2 k3
+ *   stage 3: x_{n+1} = acc + (1/6) k4,    E = e^-beta, E2 = e^-beta/2
+ * (classical RK4 in the variable e^{beta t/dt} x).  E = E2 = 1 gives rk_combine's values. */
+static double rk_combine_if(int st, double xn, double k, double *acc, double E, double E2)
+{
+    const double c[3] = {E2, E2, E}, a[3] = {E2 * 0.5, 0.5, E2}, w[4] = {E / 6.0, E2 / 3.0, E2 / 3.0, 1.0 / 6.0};
+    if (st == 0) *acc = E * xn + w[0] * k;
+    else if (st < 3) *acc = *acc + w[st] * k;
+    return st < 3 ? c[st] * xn + a[st] * k : *acc + w[3] * k;
 }
 
 /* Face conditions of the state held in the nn arrays, in the reference's face order. */
@@ -1700,7 +1717,9 @@ static void rk4_step(orc_sim *s)
                     for (int i = 1; i < G->nx - 1; ++i) {
                         size_t id = IDX(G, i, j, k);
                         for (int f = 0; f < 4; ++f)
-                            xs[f][id] = rk_combine(st, xn[f][id], s->rk_k[d][f][id], &s->rk_a[d][f][id]);
+                            xs[f][id] = (d == 1 && f < 3)
+                                            ? rk_combine_if(st, xn[f][id], s->rk_k[d][f][id], &s->rk_a[d][f][id], s->p_E, s->p_E2)
+                                            : rk_combine(st, xn[f][id], s->rk_k[d][f][id], &s->rk_a[d][f][id]);
                     }
         }
         rk_faces(s);
