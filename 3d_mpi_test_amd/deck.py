@@ -427,6 +427,37 @@ def config_c2(time_domain: int = 2000, seed: int = 20261202) -> Deck:
     return dk
 
 
+def config_c3(time_domain: int = 3000, seed: int = 20261202, n: int = 64):
+    """Batched ensemble: n domains with sides drawn from {64..128} x {48..96} x {48..64} at
+    0.05 m, each over its own porous layer -- flow resistivity log-uniform in 10-2000
+    kPa s/m^2, porosity U[0.2, 0.5], tortuosity U[1.5, 4] (q = sqrt(tortuosity)), depth 4-16
+    points -- a source at a height U[0.3, 2] m, a logarithmic wind with u* U[0, 0.5] m/s
+    (z0 = 0.02 m), RK4 at CFL 0.5 for 3000 steps.  Not specified and chosen here: 10-point
+    PML on the sides and top, the source at a quarter of the domain length on the centre
+    line, four receivers at 1.2 m along the centre line, numpy's PCG64 with the given seed.
+    Returns (decks, receivers)."""
+    rng = np.random.default_rng(seed)
+    d = 0.05
+    decks, recs = [], []
+    for _ in range(n):
+        nx, ny, nz = (int(rng.integers(64, 129)), int(rng.integers(48, 97)), int(rng.integers(48, 65)))
+        depth = int(rng.integers(4, 17))
+        sigma = float(10.0 ** rng.uniform(np.log10(1.0e4), np.log10(2.0e6)))
+        poro = float(rng.uniform(0.2, 0.5))
+        tort = float(rng.uniform(1.5, 4.0))
+        hs = float(rng.uniform(0.3, 2.0))
+        ustar = float(rng.uniform(0.0, 0.5))
+        ylen = (ny - 1) * d
+        dk = box(nx, ny, nz, d, pml=10, bc=(2, 3, 2, 2, 2, 2), porous_cells=depth + 2, time_domain=time_domain,
+                 gauss_width=2 * 3 * d, src=(0.0, 0.25 * ylen, hs), rcv=(0.0, 0.5 * ylen, 1.2), fft_m=12,
+                 velocity_method=7)
+        dk.pore_q, dk.pore_porosity, dk.pore_resistivity = tort ** 0.5, poro, sigma
+        dk.integrator, dk.wind_ustar, dk.wind_z0 = INT_RK4, ustar, 0.02
+        decks.append(dk)
+        recs.append([(0.0, f * ylen, 1.2) for f in (0.5, 0.625, 0.75, 0.875)])
+    return decks, recs
+
+
 def receivers_c2():
     """Thirty-two receivers on a line at 1.2 m along the propagation axis (not specified)."""
     return [(0.0, 4.0 + 0.35 * r, 1.2) for r in range(32)]

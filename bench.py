@@ -20,11 +20,14 @@ of IL cancels in the difference).
 runs when a JSON of them is given.  `cpu_baseline`: the FP64 C oracle of the same step on one
 host core over a bounded sample (the first step(s) of the same workload).
 
+`--workload c3` runs the batched ensemble: 64 seeded uneven domains (deck.config_c3) advanced
+together by the ensemble runner, FP32 (and FP64 for the accuracy figures), 3000 steps.
+
 `--workload c2` runs the precision study instead: 768x384x192 at 0.04 m over a 24-point
 porous layer and a seeded random terrain of 6 Gaussian hills, RK4, 2000 steps, FP32 against
 FP64 (deck.config_c2).
 
-    python bench.py [--gpus 1] [--workload c1|c2] [--steps K] [--warmup W] [--no-cpu] [--no-fb]
+    python bench.py [--gpus 1] [--workload c1|c2|c3] [--steps K] [--warmup W] [--no-cpu] [--no-fb]
                     [--traffic-json F]
 """
 from __future__ import annotations
@@ -126,6 +129,70 @@ def cpu_baseline(deck, receivers, budget_s=20.0):
     return dict(value=npts * (1 + more) / sec, steps=1 + more, seconds=sec, points=npts)
 
 
+def run_c3(lee, a):
+    """C3 ensemble: every member advanced by one launch per kernel kind per RK4 stage
+    (lee_ensemble_*), captured in graphs.  `value` sums the grid-point updates of all members.
+    The roofline figure is the whole step's compulsory bytes (the byte model of each member's
+    kernels, summed) over the step time."""
+    decks, recs = lee.deck.config_c3(time_domain=a.warmup + a.steps)
+    prec = [int(x) for x in a.precisions.split(",")]
+    out = {}
+    for p in prec:
+        print(f"[bench] C3 ensemble fp{p}: {len(decks)} domains, {a.warmup} warm-up + {a.steps} timed steps",
+              file=sys.stderr, flush=True)
+        e = lee.Ensemble(decks, precision=p, receivers=recs, use_graph=True)
+        npts = e.points()
+        ser = [e.step(a.warmup)]
+        t0 = time.perf_counter()
+        ser.append(e.step(a.steps))
+        wall = time.perf_counter() - t0
+        bytes_step = 0
+        for m in e.members:
+            stats = (lee.LeeKstat * 16)()
+            n = lee.lib().lee_kernel_stats(m.h, stats, 16)
+            for i in range(n):
+                if stats[i].name.decode() in ("rk_air", "rk_pore", "rk_bnd"):
+                    bytes_step += 4 * stats[i].bytes_per_launch
+        series = [np.concatenate([ser[0][d], ser[1][d]]) for d in range(len(decks))]
+        out[p] = dict(points=npts, wall=wall, bytes_step=bytes_step, series=series)
+        print(f"[bench] C3 fp{p}: {npts * a.steps / wall / 1e9:.2f} G pt-updates/s ({1e3 * wall / a.steps:.3f} ms/step)",
+              file=sys.stderr, flush=True)
+        e.close()
+    head = 32 if 32 in out else prec[0]
+    r = out[head]
+    ms = 1e3 * r["wall"] / a.steps
+    achieved = r["bytes_step"] / (ms / 1e3) / 1e9
+    acc = None
+    if 32 in out and 64 in out:
+        worst_l2, worst_il = 0.0, 0.0
+        for d in range(len(decks)):
+            rl2, dil, _ = accuracy(out[64]["series"][d], out[32]["series"][d], decks[d].dt)
+            worst_l2, worst_il = max(worst_l2, rl2), max(worst_il, dil)
+        acc = {"fp32_vs_fp64_worst_member_rel_l2": worst_l2, "worst_member_max_dIL_dB_50_1000Hz": round(worst_il, 5)}
+    line = {
+        "metric": "grid-pt updates/s + %HBM BW, FP32 vs FP64; learned-ground speedup @<=0.2 dB",
+        "value": round(r["points"] * a.steps / r["wall"], 1),
+        "unit": "grid-pt updates/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": f"f{head}", "data": "synthetic",
+        "config": {"workload": "C3: 64 seeded domains {64..128}x{48..96}x{48..64} at 0.05 m, porous layers 4-16 pts "
+                               "(sigma log-U 10-2000 kPa s/m^2, porosity U[0.2,0.5], tortuosity U[1.5,4]), source "
+                               "U[0.3,2] m, log wind u* U[0,0.5] m/s, 10-pt PML sides+top, RK4 CFL 0.5, one launch per "
+                               "kernel kind per stage, graph-replayed",
+                   "points": r["points"], "domains": len(decks), "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "whole RK4 step (rk_air + rk_pore + rk_bnd, all members)",
+                     "bytes_per_step": r["bytes_step"]},
+        "cpu_baseline": None,
+        "accuracy": acc,
+    }
+    for p, rr in out.items():
+        line[f"fp{p}_pt_updates_per_s"] = rr["points"] * a.steps / rr["wall"]
+        line[f"fp{p}_ms_per_step"] = 1e3 * rr["wall"] / a.steps
+    print(json.dumps(line))
+
+
 def kernel_table(kernels):
     return [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in kk.items()} for kk in kernels]
 
@@ -133,8 +200,8 @@ def kernel_table(kernels):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("c1", "c2"), default="c1")
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (C1: 10000, C2: 2000)")
+    ap.add_argument("--workload", choices=("c1", "c2", "c3"), default="c1")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (C1: 10000, C2: 2000, C3: 3000)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fb", action="store_true", help="skip the reference-scheme (FB_p_v, flat) lines")
@@ -148,7 +215,10 @@ def main():
         sys.exit(2)
     lee = importlib.import_module("3d_mpi_test_amd")
     if a.steps is None:
-        a.steps = 10000 if a.workload == "c1" else 2000
+        a.steps = {"c1": 10000, "c2": 2000, "c3": 3000}[a.workload]
+    if a.workload == "c3":
+        run_c3(lee, a)
+        return
     total = a.warmup + a.steps
     if a.workload == "c1":
         deck = lee.deck.config_c1(time_domain=total)
