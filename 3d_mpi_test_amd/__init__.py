@@ -108,6 +108,7 @@ def lib():
         L.lee_load_restart.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
         L.lee_write_contour.argtypes = [vp, ctypes.c_char_p]
         L.lee_set_source_series.argtypes = [vp, _dp, ctypes.c_longlong, ctypes.c_int]
+        L.lee_get_ground_psi.argtypes = [vp, _dp, ctypes.c_longlong]
         L.lee_ensemble_create.argtypes = [ctypes.c_int, ctypes.POINTER(LeeDeck), ctypes.POINTER(LeeOptions),
                                           ctypes.POINTER(ctypes.c_void_p)]
         L.lee_ensemble_destroy.argtypes = [vp]
@@ -334,6 +335,12 @@ class Solver:
 
     def write_contour(self, dirpath: str) -> None:
         _check(lib().lee_write_contour(self.h, dirpath.encode()), self.h)
+
+    def ground_psi(self, K: int) -> np.ndarray:
+        """Ground impedance histories, shape (ny, nx, K)."""
+        out = np.zeros((self.ny, self.nx, K))
+        _check(lib().lee_get_ground_psi(self.h, _p(out), out.size), self.h)
+        return out
 
     def set_source_series(self, ps, period: int = 23) -> None:
         """Time-series source: every `period` steps the pressure gains ps[N/period] times the

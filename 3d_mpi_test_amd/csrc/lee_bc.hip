@@ -47,6 +47,13 @@ template <typename T> __device__ __forceinline__ T p_pore_walk(const BCArgs<T> &
 template <typename T, bool ALLOW_PORE> __device__ __forceinline__ T p_air_walk(const BCArgs<T> &a, int i, int j, int k, int t)
 {
     const Grid<T> &g = a.air;
+    if (t >= 6 && k == 0 && a.ground) {
+        // ground impedance: the East face's quadratic extrapolation along -z from the
+        // planes k = 1, 2, 3 as faces 1..5 left them
+        const T a0 = p_air_walk<T, false>(a, i, j, 1, 5), a1 = p_air_walk<T, false>(a, i, j, 2, 5),
+                a2 = p_air_walk<T, false>(a, i, j, 3, 5);
+        return T(15.0 / 8.0) * a0 - T(5.0 / 4.0) * a1 + T(3.0 / 8.0) * a2;
+    }
     if (t >= 6 && k == 0 && (!a.porous || a.iface)) {
         if constexpr (ALLOW_PORE) {
             if (a.porous) return p_pore_walk(a, i, j, a.pore.nz - 2, 5);
@@ -89,17 +96,31 @@ template <typename T> __device__ __forceinline__ double tdbc_v(const BCArgs<T> &
     return a.inv * ((double)g.pnn[gidx(g, i, g.ny - 1, k)] / a.Zinf - temp_psi);
 }
 
+// Ground impedance: the velocity into the ground (-w at k = 0) from the final face pressure
+// and the histories of the previous step -- Cal_TDBC (calculation_2D.cpp:1019-1028) with K
+// poles.
+template <typename T> __device__ __forceinline__ double ground_v(const BCArgs<T> &a, int i, int j)
+{
+    const Grid<T> &g = a.air;
+    const double *P = a.gpsi_old + ((size_t)i + (size_t)g.nx * j) * a.gK;
+    double temp_psi = 0;
+    for (int q = 0; q < a.gK; ++q) temp_psi += a.gdecay[q] * P[q];
+    return a.ginv * ((double)g.pnn[gidx(g, i, j, 0)] / a.gZinf - temp_psi);
+}
+
 template <typename T> __device__ __forceinline__ T v_pore_walk(const BCArgs<T> &a, int c, int i, int j, int k, int t);
 
-// air velocity component c after faces 1..t
+// air velocity component c after faces 1..t; t = 7 adds the ground impedance's normal
+// velocity on the plane k = 1 (the first w that the interior divergence reads)
 template <typename T, bool ALLOW_PORE> __device__ __forceinline__ T v_air_walk(const BCArgs<T> &a, int c, int i, int j, int k, int t)
 {
     const Grid<T> &g = a.air;
+    if (t >= 7 && a.ground && k == 1 && c == 2) return (T)(-ground_v(a, i, j));
     if (t >= 6 && k == 0 && (!a.porous || a.iface)) {
         if constexpr (ALLOW_PORE) {
             if (a.porous) return v_pore_walk(a, c, i, j, a.pore.nz - 2, 5);
         }
-        if (c == 2) return T(0);
+        if (c == 2) return a.ground ? (T)(-ground_v(a, i, j)) : T(0);
         k = 1;
     }
     if (t >= 5 && i == g.nx - 1) { if (c == 0) return T(0); i = g.nx - 2; }
@@ -198,9 +219,9 @@ __global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
     if (t < a.nb_air) {
         int i, j, k;
         boundary_node(t, a.air.nx, a.air.ny, a.air.nz, i, j, k);
-        const T u = v_air_walk<T, true>(a, 0, i, j, k, 6);
-        const T v = v_air_walk<T, true>(a, 1, i, j, k, 6);
-        const T w = v_air_walk<T, true>(a, 2, i, j, k, 6);
+        const T u = v_air_walk<T, true>(a, 0, i, j, k, 7);
+        const T v = v_air_walk<T, true>(a, 1, i, j, k, 7);
+        const T w = v_air_walk<T, true>(a, 2, i, j, k, 7);
         const size_t id = gidx(a.air, i, j, k);
         a.air.unn[id] = u; a.air.vnn[id] = v; a.air.wnn[id] = w;
         q_boundary(a, 0, i, j, k, u);
@@ -213,6 +234,12 @@ __global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
 #pragma unroll
             for (int q = 0; q < 6; ++q) a.psi_new[b + q] = a.decay[q] * a.psi_old[b + q] + (a.gain[q] * vt) * a.tb;
         }
+        if (a.ground && k == 0) {
+            // Update_Psi (calculation_2D.cpp:1012-1017) with the velocity into the ground
+            const double vg = ground_v(a, i, j);
+            const size_t b = ((size_t)i + (size_t)a.air.nx * j) * a.gK;
+            for (int q = 0; q < a.gK; ++q) a.gpsi_new[b + q] = a.gdecay[q] * a.gpsi_old[b + q] + (a.ggain[q] * vg) * a.gtb;
+        }
     } else if (t < a.nb_air + a.nb_pore) {
         int i, j, k;
         boundary_node(t - a.nb_air, a.pore.nx, a.pore.ny, a.pore.nz, i, j, k);
@@ -221,6 +248,15 @@ __global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
         const T w = v_pore_walk(a, 2, i, j, k, 6);
         const size_t id = gidx(a.pore, i, j, k);
         a.pore.unn[id] = u; a.pore.vnn[id] = v; a.pore.wnn[id] = w;
+    } else if (a.ground && t < a.nb_air + a.nb_pore + (long long)(a.air.nx - 2) * (a.air.ny - 2)) {
+        // ground impedance: the normal velocity of the interior nodes of the plane k = 1
+        // (the x and y face nodes of that plane get it through their walks) and its PML
+        // auxiliaries, which the velocity stencil integrated with the value it replaced
+        const long long r = t - a.nb_air - a.nb_pore;
+        const int i = 1 + (int)(r % (a.air.nx - 2)), j = 1 + (int)(r / (a.air.nx - 2));
+        const T w = (T)(-ground_v(a, i, j));
+        a.air.wnn[gidx(a.air, i, j, 1)] = w;
+        q_boundary(a, 2, i, j, 1, w);
     }
 }
 
@@ -324,7 +360,7 @@ template <typename T> void launch_bc_p_all(const BCArgs<T> &a, hipStream_t s)
 
 template <typename T> void launch_bc_v_all(const BCArgs<T> &a, hipStream_t s)
 {
-    const long long n = a.nb_air + a.nb_pore;
+    const long long n = a.nb_air + a.nb_pore + (a.ground ? (long long)(a.air.nx - 2) * (a.air.ny - 2) : 0);
     hipLaunchKernelGGL((k_bc_v_all<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
 }
 

@@ -66,14 +66,18 @@ struct lee_ctx {
     // terrain tables (sh, hx, hy per (i,j); tk per k) of the air [0] and porous [1] grids
     void *ter[2][4] = {};
     unsigned char *omask = nullptr;  // embedded obstacle material ids (FB)
+    // log wind on the device: terrain height of the frame [nx*ny], z and tk per level [2*nz]
+    double *wind_h = nullptr, *wind_z = nullptr;
+    bool wind_flat_done = false;
     // time-series source: amplitudes, period in steps, node coordinates X | Y(frame) | Z
     std::vector<double> src_ps;
     int src_period = 0;
     double *src_xyz = nullptr;
     // captured step chunks (use_graph): keyed by the buffer roles they start from
-    struct GraphEntry { int n, cur, qcur, gate_on; void *psi; hipGraphExec_t exec; };
+    struct GraphEntry { int n, cur, qcur, gate_on; void *psi, *gpsi; hipGraphExec_t exec; };
     std::vector<GraphEntry> graphs;
     double *psi = nullptr, *psi2 = nullptr;  // TDBC histories, read / written each step
+    double *gpsi = nullptr, *gpsi2 = nullptr;  // ground impedance histories (South face)
     long long *rec_idx = nullptr;
     double *rec_buf = nullptr;
     int *flag = nullptr;
@@ -286,6 +290,9 @@ template <typename T> BCArgs<T> bc_args(lee_ctx *c, const AirArgs<T> &aa, double
     b.psi_old = c->psi; b.psi_new = c->psi2;
     for (int q = 0; q < 6; ++q) { b.decay[q] = m.td_decay[q]; b.gain[q] = m.td_gain[q]; }
     b.inv = m.td_inv; b.Zinf = m.Zinf; b.tb = m.dt / m.tau;
+    b.ground = m.ground; b.gK = m.gK; b.gpsi_old = c->gpsi; b.gpsi_new = c->gpsi2;
+    for (int q = 0; q < 8; ++q) { b.gdecay[q] = m.g_decay[q]; b.ggain[q] = m.g_gain[q]; }
+    b.ginv = m.g_inv; b.gZinf = m.gZinf; b.gtb = m.g_tb;
     b.rec_idx = c->rec_idx; b.nrec = (int)m.ri.size(); b.rec_out = rec_out;
     return b;
 }
@@ -295,7 +302,7 @@ int upload_frame(lee_ctx *c, int N)
 {
     const Model &m = c->m;
     FrameTables t;
-    m.frame_tables(N, t);
+    m.frame_tables(N, t, false);
     auto metric = [&](void *dst, const std::vector<double> &v) {
         // FP32 on the separable grid stores the reciprocals (the metric itself); the terrain
         // kernels form the Jacobian per point from X_over_x ... in either precision
@@ -314,7 +321,22 @@ int upload_frame(lee_ctx *c, int N)
         if ((rc = upload(c, c->prof[s][1], P[s]->c.data(), P[s]->c.size()))) return rc;
         if ((rc = upload(c, c->prof[s][2], P[s]->d.data(), P[s]->d.size()))) return rc;
     }
-    if (m.has_wind) {
+    if (m.has_wind && m.wind_v_only) {
+        // velocity_method 7 on the device: over terrain the ground under a node changes with
+        // the frame; on a flat grid the field is the same in every frame
+        const bool ter = m.terrain;
+        if (ter || !c->wind_flat_done) {
+            if (ter && (rc = upload_f64(c, c->wind_h, t.ter.h.data(), t.ter.h.size()))) return rc;
+            if (c->prec == 64)
+                launch_wind_log<double>((double *)c->V, m.nx, m.ny, m.nz, ter ? c->wind_h : nullptr, c->wind_z,
+                                        c->wind_z + m.nz, m.opt.wind_ustar, m.opt.wind_z0, c->stream);
+            else
+                launch_wind_log<float>((float *)c->V, m.nx, m.ny, m.nz, ter ? c->wind_h : nullptr, c->wind_z,
+                                       c->wind_z + m.nz, m.opt.wind_ustar, m.opt.wind_z0, c->stream);
+            HIPCHK(c, hipGetLastError());
+            c->wind_flat_done = true;
+        }
+    } else if (m.has_wind) {
         if ((rc = upload(c, c->U, t.U.data(), t.U.size())) || (rc = upload(c, c->V, t.V.data(), t.V.size())) ||
             (rc = upload(c, c->W, t.W.data(), t.W.size())))
             return rc;
@@ -576,6 +598,7 @@ template <typename T> void enqueue_fb_step(lee_ctx *c, double *rec_slot)
     launch_bc_v_all<T>(b, s);
     prof_mark(c, KS_BC_V, false);
     if (m.east_tdbc) std::swap(c->psi, c->psi2);
+    if (m.ground) std::swap(c->gpsi, c->gpsi2);
     inject_source<T>(c, (T *)c->air[(c->cur ^ 1) * 4 + 3]);
     // ---- level n+1 becomes level n
     c->cur ^= 1;
@@ -611,16 +634,21 @@ template <typename T> int graph_steps(lee_ctx *c, int n)
     const bool gate = (m.judge[1] == 1 || m.judge[1] == 3);
     const int gate_on = gate && c->counter + 1 >= m.pml_time;
     for (auto &g : c->graphs)
-        if (g.n == n && g.cur == c->cur && g.qcur == c->qcur && g.gate_on == gate_on && g.psi == (void *)c->psi) {
+        if (g.n == n && g.cur == c->cur && g.qcur == c->qcur && g.gate_on == gate_on && g.psi == (void *)c->psi &&
+            g.gpsi == (void *)c->gpsi) {
             HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
             if (!m.rk4) {
-                if (n & 1) { c->cur ^= 1; c->qcur ^= 1; if (m.east_tdbc) std::swap(c->psi, c->psi2); }
+                if (n & 1) {
+                    c->cur ^= 1; c->qcur ^= 1;
+                    if (m.east_tdbc) std::swap(c->psi, c->psi2);
+                    if (m.ground) std::swap(c->gpsi, c->gpsi2);
+                }
             }
             if (gate) c->counter += n;
             c->steps += n;
             return LEE_OK;
         }
-    lee_ctx::GraphEntry e{n, c->cur, c->qcur, gate_on, (void *)c->psi, nullptr};
+    lee_ctx::GraphEntry e{n, c->cur, c->qcur, gate_on, (void *)c->psi, (void *)c->gpsi, nullptr};
     const int nr = (int)m.ri.size();
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     for (int t = 0; t < n; ++t) enqueue_step<T>(c, c->rec_buf + (size_t)t * nr);
@@ -740,6 +768,10 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
     }
     if ((rc = dalloc(C, (void **)&C->psi, (size_t)m.nx * m.nz * 6 * 8))) return rc;
     if ((rc = dalloc(C, (void **)&C->psi2, (size_t)m.nx * m.nz * 6 * 8))) return rc;
+    if (m.ground) {
+        const size_t ng = (size_t)m.nx * m.ny * m.gK * 8;
+        if ((rc = dalloc(C, (void **)&C->gpsi, ng)) || (rc = dalloc(C, (void **)&C->gpsi2, ng))) return rc;
+    }
     int nr = (int)m.ri.size();
     if ((rc = dalloc(C, (void **)&C->rec_idx, (size_t)std::max(nr, 1) * sizeof(long long)))) return rc;
     if ((rc = dalloc(C, (void **)&C->rec_buf, (size_t)kRecChunk * std::max(nr, 1) * 8))) return rc;
@@ -767,6 +799,18 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
                 if ((rc = dalloc(C, &C->ter[d][q], n2 * es))) return rc;
             if ((rc = dalloc(C, &C->ter[d][3], nk * es))) return rc;
         }
+    }
+    if (m.has_wind && m.wind_v_only) {
+        if ((rc = dalloc(C, (void **)&C->wind_z, (size_t)2 * m.nz * 8))) return rc;
+        if (m.terrain && (rc = dalloc(C, (void **)&C->wind_h, (size_t)m.nx * m.ny * 8))) return rc;
+        std::vector<double> zt(2 * m.nz, 0.0);
+        if (m.terrain) {
+            FrameTables f0;
+            m.frame_tables(0, f0, false);
+            for (int k = 0; k < m.nz; ++k) zt[m.nz + k] = f0.ter.tk[k];
+        }
+        for (int k = 0; k < m.nz; ++k) zt[k] = m.az[k];
+        if ((rc = upload_f64(C, C->wind_z, zt.data(), zt.size()))) return rc;
     }
     if (m.obstacle) {
         if ((rc = dalloc(C, (void **)&C->omask, m.omask.size()))) return rc;
@@ -880,7 +924,7 @@ int lee_set_source_series(lee_ctx *c, const double *ps, long long n, int period)
         if ((rc = upload_f64(c, c->src_xyz, m.ax.data(), m.nx))) return rc;
         if ((rc = upload_f64(c, c->src_xyz + m.nx + m.ny, m.az.data(), m.nz))) return rc;
         FrameTables t;
-        m.frame_tables(c->mf_count < 0 ? 0 : c->mf_count, t);
+        m.frame_tables(c->mf_count < 0 ? 0 : c->mf_count, t, false);
         if ((rc = upload_f64(c, c->src_xyz + m.nx, t.Y.data(), t.Y.size()))) return rc;
     }
     c->src_ps.assign(ps, ps + n);
@@ -1077,6 +1121,14 @@ int lee_save_restart(lee_ctx *c, const char *path, int index)
     restart_stream(o);
     for (size_t q = 0; q < np; ++q) o << psi[q] << " ";
     o << std::endl;
+    if (m.ground) {  // ground impedance histories
+        const size_t ng = (size_t)nx * m.ny * m.gK;
+        std::vector<double> g(ng);
+        HIPCHK(c, hipMemcpyAsync(g.data(), c->gpsi, ng * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (size_t q = 0; q < ng; ++q) o << g[q] << " ";
+        o << std::endl;
+    }
     if (!o) return set_err(c, LEE_ERR_IO, std::string("write failed: ") + path);
     return LEE_OK;
 }
@@ -1163,6 +1215,14 @@ int lee_load_restart(lee_ctx *c, const char *path, int *index)
             if (!(in >> psi[q])) return set_err(c, LEE_ERR_IO, "restart file ends inside the TDBC histories");
         HIPCHK(c, hipMemcpyAsync(c->psi, psi.data(), np * 8, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (m.ground) {
+            const size_t ng = (size_t)nx * m.ny * m.gK;
+            std::vector<double> g(ng);
+            for (size_t q = 0; q < ng; ++q)
+                if (!(in >> g[q])) return set_err(c, LEE_ERR_IO, "restart file ends inside the ground histories");
+            HIPCHK(c, hipMemcpyAsync(c->gpsi, g.data(), ng * 8, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
         c->counter = counter;
         c->steps = steps;
         if (mf != c->mf_count) return set_err(c, LEE_ERR_STATE, "restart written in frame " + std::to_string(mf) +
@@ -1267,6 +1327,17 @@ int lee_get_psi(lee_ctx *c, double *psi, long long cap)
     size_t n = (size_t)c->m.nx * c->m.nz * 6;
     if ((long long)n > cap) return set_err(c, LEE_ERR_ARG, "buffer too small");
     HIPCHK(c, hipMemcpyAsync(psi, c->psi, n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LEE_OK;
+}
+
+int lee_get_ground_psi(lee_ctx *c, double *psi, long long cap)
+{
+    if (!c || !psi) return set_err(c, LEE_ERR_ARG, "bad arguments");
+    if (!c->m.ground) return set_err(c, LEE_ERR_STATE, "no ground impedance in this context");
+    size_t n = (size_t)c->m.nx * c->m.ny * c->m.gK;
+    if ((long long)n > cap) return set_err(c, LEE_ERR_ARG, "buffer too small");
+    HIPCHK(c, hipMemcpyAsync(psi, c->gpsi, n * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return LEE_OK;
 }

@@ -70,6 +70,11 @@ template <typename T> struct BCArgs {
     const double *psi_old;     // East TDBC histories [i + nx*k][6], read
     double *psi_new;           // and written (double buffered)
     double decay[6], gain[6], inv, Zinf, tb;
+    // ground impedance on the South face: histories [i + nx*j][gK], FP64, double buffered
+    int ground, gK;
+    const double *gpsi_old;
+    double *gpsi_new;
+    double gdecay[8], ggain[8], ginv, gZinf, gtb;
     const long long *rec_idx;  // receiver samples of time level n (pressure phase)
     int nrec;
     double *rec_out;
@@ -135,6 +140,9 @@ template <typename T> void launch_copy_plane_j(T *p, int nx, int ny, int nz, hip
 template <typename T> void launch_sample(const T *p, const long long *idx, int n, double *out, hipStream_t s);
 void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s);
 template <typename T> void launch_finite_check(const T *f, size_t n, int *flag, hipStream_t s);
+template <typename T>
+void launch_wind_log(T *V, int nx, int ny, int nz, const double *h, const double *az, const double *tk, double ustar,
+                     double z0, hipStream_t s);
 template <typename T>
 void launch_add_source(T *p, int nx, int ny, int nz, const double *X, const double *Y, const double *Z, double sx,
                        double sy, double sz, double gc, double amp, hipStream_t s);

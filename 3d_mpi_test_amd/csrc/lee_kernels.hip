@@ -90,7 +90,11 @@ template <typename T> __device__ __forceinline__ OwnQ<T> own_slabs(const AirArgs
 // follow the reference's association; the TERRAIN terms are those of its nine-metric chain
 // (air.cpp:1975-2292) that the separable grid multiplies by zero, inserted in chain order.
 // ------------------------------------------------------------------------------------
-template <typename T, int WM, bool TERRAIN>
+// CEN (the RK4 path): the convective derivatives V d/dY ... are central differences instead
+// of the reference's one-sided pair (backward for p, forward for u v w).  The one-sided pair
+// advects the velocity components across the flow with a downwind difference, whose modes
+// grow at up to 2|V|/dy; the central form is neutral.
+template <typename T, int WM, bool TERRAIN, bool CEN = false>
 __device__ __forceinline__ T air_dp(const AirArgs<T> &a, int i, int j, int k)
 {
     const Grid<T> &g = a.g;
@@ -135,7 +139,11 @@ __device__ __forceinline__ T air_dp(const AirArgs<T> &a, int i, int j, int k)
         } else {
             c3 = ((Wc * mZ) * dzi) * dt;
         }
-        r = ((-c1) * (pc - g.pn[id - 1]) - c2 * (pc - g.pn[id - nx])) - c3 * (pc - g.pn[id - sxy]);
+        if constexpr (CEN)
+            r = ((-c1) * ((g.pn[id + 1] - g.pn[id - 1]) * T(0.5)) - c2 * ((g.pn[id + nx] - g.pn[id - nx]) * T(0.5))) -
+                c3 * ((g.pn[id + sxy] - g.pn[id - sxy]) * T(0.5));
+        else
+            r = ((-c1) * (pc - g.pn[id - 1]) - c2 * (pc - g.pn[id - nx])) - c3 * (pc - g.pn[id - sxy]);
         r += (((-a.gam) * pc) * dt) * ((UX + VY) + WZ);
         r += (((-a.gam) * a.Pav) * dt) * div;
     } else if constexpr (WM == 2) {
@@ -143,11 +151,13 @@ __device__ __forceinline__ T air_dp(const AirArgs<T> &a, int i, int j, int k)
         const T Vc = a.V[id];
         const T c2 = ((Vc * mY) * dyi) * dt;
         T VY = ((mY * (a.V[id + nx] - a.V[id - nx])) * dyi) * T(0.5);
-        r = -(c2 * (pc - g.pn[id - nx]));
+        if constexpr (CEN) r = -(c2 * ((g.pn[id + nx] - g.pn[id - nx]) * T(0.5)));
+        else r = -(c2 * (pc - g.pn[id - nx]));
         if constexpr (TERRAIN) {
             const T c3 = ((Vc * zY) * dzi) * dt;
             VY = VY + ((zY * (a.V[id + sxy] - a.V[id - sxy])) * dzi) * T(0.5);
-            r = r - c3 * (pc - g.pn[id - sxy]);
+            if constexpr (CEN) r = r - c3 * ((g.pn[id + sxy] - g.pn[id - sxy]) * T(0.5));
+            else r = r - c3 * (pc - g.pn[id - sxy]);
         }
         r += (((-a.gam) * pc) * dt) * VY;
         r += (((-a.gam) * a.Pav) * dt) * div;
@@ -337,7 +347,7 @@ __device__ __forceinline__ T air_p_point(const AirArgs<T> &a, int i, int j, int 
 // and the pressure auxiliaries in a.Qn[.][3] (FB: p^{n+1}; RK4: the stage state), the
 // velocity auxiliaries in a.Qo.
 // ------------------------------------------------------------------------------------
-template <typename T, int WM, bool TERRAIN>
+template <typename T, int WM, bool TERRAIN, bool CEN = false>
 __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int k, T &du_, T &dv_, T &dw_)
 {
     const Grid<T> &g = a.g;
@@ -382,13 +392,21 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
         const T UX = gx(U), UY = gy(U), UZ = gz(U);
         const T VX = gx(V), VY = gy(V), VZ = gz(V);
         const T WX = gx(W), WY = gy(W), WZ = gz(W);
-        du = ((-c1) * (g.un[id + 1] - uc) - c2 * (g.un[id + nx] - uc)) - c3 * (g.un[id + sxy] - uc);
+        // convective differences of one component: forward (reference) or central
+        auto cv = [&](const T *F, T fc) {
+            if constexpr (CEN)
+                return ((-c1) * ((F[id + 1] - F[id - 1]) * T(0.5)) - c2 * ((F[id + nx] - F[id - nx]) * T(0.5))) -
+                       c3 * ((F[id + sxy] - F[id - sxy]) * T(0.5));
+            else
+                return ((-c1) * (F[id + 1] - fc) - c2 * (F[id + nx] - fc)) - c3 * (F[id + sxy] - fc);
+        };
+        du = cv(g.un, uc);
         du += (((-uc) * UX) * dt - (vc * UY) * dt) - (wc * UZ) * dt;
         du += ((-Aav) * dt) * gpx;
-        dv = ((-c1) * (g.vn[id + 1] - vc) - c2 * (g.vn[id + nx] - vc)) - c3 * (g.vn[id + sxy] - vc);
+        dv = cv(g.vn, vc);
         dv += (((-uc) * VX) * dt - (vc * VY) * dt) - (wc * VZ) * dt;
         dv += ((-Aav) * dt) * gpy;
-        dw = ((-c1) * (g.wn[id + 1] - wc) - c2 * (g.wn[id + nx] - wc)) - c3 * (g.wn[id + sxy] - wc);
+        dw = cv(g.wn, wc);
         dw += (((-uc) * WX) * dt - (vc * WY) * dt) - (wc * WZ) * dt;
         dw += ((-Aav) * dt) * gpz;
     } else if constexpr (WM == 2) {
@@ -399,16 +417,25 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
         T VX = ((mX * (V[id + 1] - V[id - 1])) * dxi) * T(0.5);
         T VY = ((mY * (V[id + nx] - V[id - nx])) * dyi) * T(0.5);
         const T VZ = ((mZ * (V[id + sxy] - V[id - sxy])) * dzi) * T(0.5);
-        du = -(c2 * (g.un[id + nx] - uc));
-        dv = -(c2 * (g.vn[id + nx] - vc));
-        dw = -(c2 * (g.wn[id + nx] - wc));
+        // d/dY (and d/dZ over terrain) of one component: forward (reference) or central
+        auto dy = [&](const T *F, T fc) {
+            if constexpr (CEN) return (F[id + nx] - F[id - nx]) * T(0.5);
+            else return F[id + nx] - fc;
+        };
+        auto dz = [&](const T *F, T fc) {
+            if constexpr (CEN) return (F[id + sxy] - F[id - sxy]) * T(0.5);
+            else return F[id + sxy] - fc;
+        };
+        du = -(c2 * dy(g.un, uc));
+        dv = -(c2 * dy(g.vn, vc));
+        dw = -(c2 * dy(g.wn, wc));
         if constexpr (TERRAIN) {
             const T c3 = ((Vc * zY) * dzi) * dt;
             VX = VX + ((zX * (V[id + sxy] - V[id - sxy])) * dzi) * T(0.5);
             VY = VY + ((zY * (V[id + sxy] - V[id - sxy])) * dzi) * T(0.5);
-            du = du - c3 * (g.un[id + sxy] - uc);
-            dv = dv - c3 * (g.vn[id + sxy] - vc);
-            dw = dw - c3 * (g.wn[id + sxy] - wc);
+            du = du - c3 * dz(g.un, uc);
+            dv = dv - c3 * dz(g.vn, vc);
+            dw = dw - c3 * dz(g.wn, wc);
         }
         du += ((-Aav) * dt) * gpx;
         dv += (((-uc) * VX) * dt - (vc * VY) * dt) - (wc * VZ) * dt;
@@ -778,8 +805,8 @@ __device__ __forceinline__ void rk_air_point(const RKArgs<T> &r, int i, int j, i
     const size_t id = (size_t)i + (size_t)a.g.nx * ((size_t)j + (size_t)a.g.ny * k);
     const int st = r.stage;
     T kf[4];
-    kf[3] = air_dp<T, WM, TERRAIN>(a, i, j, k);
-    air_duvw<T, WM, TERRAIN>(a, i, j, k, kf[0], kf[1], kf[2]);
+    kf[3] = air_dp<T, WM, TERRAIN, true>(a, i, j, k);
+    air_duvw<T, WM, TERRAIN, true>(a, i, j, k, kf[0], kf[1], kf[2]);
     const T xc[4] = {a.g.un[id], a.g.vn[id], a.g.wn[id], a.g.pn[id]};  // stage state at the point
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
@@ -1042,6 +1069,24 @@ __global__ void k_add_source(T *p, int nx, int ny, int nz, const double *X, cons
     }
 }
 
+// Logarithmic wind of velocity_method 7 for one frame: V = u*/0.41 ln(zg/z0) along +y at the
+// height zg above the local ground (V = 0 for zg <= z0), zg from the terrain height h of the
+// frame (NULL: flat) and the stretching tk of the terrain-following grid.
+template <typename T>
+__global__ void k_wind_log(T *V, int nx, int ny, int nz, const double *h, const double *az, const double *tk,
+                           double ustar, double z0)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t q = t % ((size_t)nx * ny);
+        const int k = (int)(t / ((size_t)nx * ny));
+        const double zk = h ? az[k] + h[q] * tk[k] : az[k];
+        const double zb = h ? az[0] + h[q] * tk[0] : az[0];
+        const double zg = zk - zb;
+        V[t] = (T)(zg > z0 ? ustar / 0.41 * log(zg / z0) : 0.0);
+    }
+}
+
 // Streaming copy used as the bandwidth reference and to calibrate HBM counters: one 16 B
 // vector per thread, non-temporal (the data is touched once), as many workgroups as needed.
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -1191,6 +1236,14 @@ template <typename T> void launch_finite_check(const T *f, size_t n, int *flag, 
 }
 
 template <typename T>
+void launch_wind_log(T *V, int nx, int ny, int nz, const double *h, const double *az, const double *tk, double ustar,
+                     double z0, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_wind_log<T>), dim3(flat_blocks((size_t)nx * ny * nz)), dim3(256), 0, s, V, nx, ny, nz, h, az, tk,
+                       ustar, z0);
+}
+
+template <typename T>
 void launch_add_source(T *p, int nx, int ny, int nz, const double *X, const double *Y, const double *Z, double sx,
                        double sy, double sz, double gc, double amp, hipStream_t s)
 {
@@ -1220,6 +1273,8 @@ void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s)
     template void launch_copy_plane_j<T>(T *, int, int, int, hipStream_t);                                   \
     template void launch_sample<T>(const T *, const long long *, int, double *, hipStream_t);                \
     template void launch_finite_check<T>(const T *, size_t, int *, hipStream_t);                             \
+    template void launch_wind_log<T>(T *, int, int, int, const double *, const double *, const double *,     \
+                                     double, double, hipStream_t);                                           \
     template void launch_add_source<T>(T *, int, int, int, const double *, const double *, const double *,   \
                                        double, double, double, double, double, hipStream_t);                 \
     template void launch_subsample<T>(const T *, int, int, int, int, int, int, int, double *, hipStream_t);

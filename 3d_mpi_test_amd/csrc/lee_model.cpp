@@ -240,6 +240,33 @@ std::string Model::build(const lee_deck &deck, const lee_options &o, int &code)
         if (east_tdbc && !(std::isfinite(Zinf) && std::isfinite(tau) && tau > 0))
             return "TDBC needs finite positive ground parameters";
     }
+    // ---- ground impedance (extension): South face, K poles, the East recursion's form
+    ground = o.ground_poles > 0;
+    if (ground) {
+        code = LEE_ERR_UNSUPPORTED;
+        if (rk4) return "the ground impedance is discretised for the FB_p_v step";
+        if (d.bc[LEE_FACE_S] != LEE_BC_RIGID) return "the ground impedance replaces a rigid South face";
+        if (d.mf_judge == 1) return "the ground impedance runs with a fixed frame";
+        if (obstacle) return "no embedded obstacle with the ground impedance";
+        code = LEE_ERR_ARG;
+        if (o.ground_poles > LEE_MAX_POLES) return "at most LEE_MAX_POLES ground poles";
+        if (!(o.ground_zinf > 0) || !(o.ground_tau > 0) || !std::isfinite(o.ground_zinf) || !std::isfinite(o.ground_tau))
+            return "ground_zinf and ground_tau must be finite and positive";
+        gK = o.ground_poles;
+        gZinf = o.ground_zinf;
+        g_tb = d.dt / o.ground_tau;
+        double at = 0;
+        for (int q = 0; q < gK; ++q) {
+            // a passive (positive-real) impedance keeps the recursion stable
+            if (!(o.ground_a[q] >= 0) || !(o.ground_r[q] >= 0) || !std::isfinite(o.ground_a[q]) || !std::isfinite(o.ground_r[q]))
+                return "ground poles and residues must be finite and non-negative";
+            at += o.ground_a[q] * g_tb;
+            g_decay[q] = std::exp(-1 * o.ground_r[q] * g_tb);
+            g_gain[q] = o.ground_a[q];
+        }
+        g_inv = 1.0 / (1 + at);
+        code = LEE_OK;
+    }
 
     // ---- receivers (airpore.cpp:413-458, calculation_2D.cpp:317-352)
     int nrec;
@@ -452,7 +479,7 @@ void Model::terrain_tables(const std::vector<double> &X, const std::vector<doubl
     for (int k = 0; k < c; ++k) t.tk[k] = pore_layer ? 1.0 : (top - Z[k]) / D;
 }
 
-void Model::frame_tables(int N, FrameTables &t) const
+void Model::frame_tables(int N, FrameTables &t, bool host_wind) const
 {
     // coordinates and metrics (calculation_2D.cpp:410-419)
     std::vector<double> Y(ay);
@@ -490,7 +517,7 @@ void Model::frame_tables(int N, FrameTables &t) const
     profile(t.z2, nz, nz - K2, nz, Rz, alz, depth_fn, &dz2);
     // wind (air.cpp:1293-1303)
     t.U.clear(); t.V.clear(); t.W.clear();
-    if (has_wind) {
+    if (has_wind && (host_wind || !wind_v_only)) {
         size_t n = (size_t)nx * ny * nz;
         t.U.resize(n); t.V.resize(n); t.W.resize(n);
         int m = d.velocity_method;

@@ -84,6 +84,10 @@ struct Model {
     double Zinf = 0, tau = 0;
     // TDBC recursion constants: e^{-r_k dt/tau}, residues a_k, 1/(1 + sum_k a_k dt/tau)
     double td_decay[6] = {}, td_gain[6] = {}, td_inv = 0;
+    // ground impedance on the South face (extension): the same recursion with K poles
+    bool ground = false;
+    int gK = 0;
+    double gZinf = 0, g_decay[8] = {}, g_gain[8] = {}, g_inv = 0, g_tb = 0;
     // moving frame
     int mf_input = 0, lead = 0, trail = 0;
     // receivers (whole-domain indices)
@@ -93,7 +97,8 @@ struct Model {
     // Returns an empty string on success, else the reason; `code` receives LEE_OK,
     // LEE_ERR_UNSUPPORTED (a path the reference never implemented) or LEE_ERR_ARG.
     std::string build(const lee_deck &deck, const lee_options &o, int &code);
-    void frame_tables(int N, FrameTables &t) const;
+    // host_wind = false leaves the log wind of velocity_method 7 to the device (t.U/V/W empty)
+    void frame_tables(int N, FrameTables &t, bool host_wind = true) const;
     std::string obstacle_mask();
     // terrain height of the hills at world (x, y)
     double hill_height(double x, double y) const;

@@ -75,7 +75,11 @@ class LeeOptions(ctypes.Structure):
         ("hills", ctypes.c_double * 32),
         ("wind_ustar", ctypes.c_double),
         ("wind_z0", ctypes.c_double),
-        ("reserved", ctypes.c_int * 15),
+        ("ground_poles", ctypes.c_int),
+        ("ground_a", ctypes.c_double * 8),
+        ("ground_r", ctypes.c_double * 8),
+        ("ground_zinf", ctypes.c_double),
+        ("ground_tau", ctypes.c_double),
     ]
 
 
@@ -103,6 +107,13 @@ def make_options(deck: Optional["Deck"] = None, precision: int = 64, receivers=N
             for q in range(4):
                 o.hills[4 * n + q] = float(h[q])
         o.wind_ustar, o.wind_z0 = float(deck.wind_ustar), float(deck.wind_z0)
+        poles = list(deck.ground_poles)
+        if len(poles) > 8:
+            raise ValueError("at most 8 ground poles")
+        o.ground_poles = len(poles)
+        for n, (a, r) in enumerate(poles):
+            o.ground_a[n], o.ground_r[n] = float(a), float(r)
+        o.ground_zinf, o.ground_tau = float(deck.ground_zinf), float(deck.ground_tau)
     elif integrator is not None:
         o.integrator = integrator
     return o
@@ -179,11 +190,15 @@ class Deck:
     pore_y: Optional[np.ndarray] = None
     pore_z: Optional[np.ndarray] = None
     # sidecar extensions (lee_options; not part of input.txt): time integrator, Gaussian
-    # hills (x0, y0, height, sigma) of a terrain-following grid, log wind of velocity_method 7
+    # hills (x0, y0, height, sigma) of a terrain-following grid, log wind of velocity_method 7,
+    # ground impedance (pole/residue pairs (a_k, r_k), Z_inf, tau) on the South face
     integrator: int = 0
     hills: Sequence[Sequence[float]] = ()
     wind_ustar: float = 0.0
     wind_z0: float = 0.0
+    ground_poles: Sequence[Sequence[float]] = ()
+    ground_zinf: float = 0.0
+    ground_tau: float = 0.0
 
     def __post_init__(self):
         self.air_x = np.ascontiguousarray(self.air_x, dtype=np.float64)

@@ -102,17 +102,25 @@ typedef struct lee_options {
     double hills[32];       /* per hill: x0, y0 [m, frame-0 coordinates], height, sigma [m] */
     double wind_ustar;      /* velocity_method 7: V(z) = wind_ustar/0.41 ln(z/wind_z0) along +y, */
     double wind_z0;         /*   z = height above the local ground, V = 0 for z <= wind_z0 */
-    int reserved[15];
+    /* ground impedance: the South face as a time-domain impedance boundary instead of a
+     * porous layer, Z(s) = ground_zinf (1 + sum_k ground_a[k] / (ground_r[k] + s ground_tau)),
+     * discretised as the reference's East TDBC (calculation_2D.cpp:986-1028) with the outward
+     * normal along -z: the velocity into the ground is -w at k = 0, the face pressure is the
+     * quadratic extrapolation of the East face.  FB_p_v, deck South face rigid, fixed frame. */
+    int ground_poles;       /* 0: off, else 1..LEE_MAX_POLES pole/residue pairs */
+    double ground_a[8], ground_r[8];
+    double ground_zinf, ground_tau;
 } lee_options;
 
 /* Time integrators.  LEE_INT_DECK runs the deck's scheme (the reference implements only
  * FB_p_v, calculation_2D.cpp:551-612).  LEE_INT_RK4 is classical four-stage Runge-Kutta
  * on the same semi-discrete operator (all four fields from one stage state, the PML
  * auxiliaries as ODE state, face conditions re-applied after every stage); the porous drag
- * is integrated explicitly, so sigma*dt*Omega/(q^2 rho0) must stay below 2.5.  The
- * reference parses scheme 5 = RK4 but never integrates with it. */
+ * enters through its integrating factor (Lawson's RK4: exact for the drag, no step limit
+ * from sigma dt / rho_e).  The reference parses scheme 5 = RK4 but never integrates with it. */
 enum { LEE_INT_DECK = 0, LEE_INT_RK4 = 1 };
 #define LEE_MAX_HILLS 8
+#define LEE_MAX_POLES 8
 
 /* Kernel path of the time step.  SPLIT runs the reference's two phases as separate passes
  * over the grid; FUSED computes both phases of a plane in one pass (one read and one write
@@ -191,6 +199,8 @@ int  lee_set_state(lee_ctx *ctx, int domain, const double *u, const double *v,
 /* TDBC history Psi on the East face, [i][k][6] (calculation_2D.cpp:128-134). */
 int  lee_get_psi(lee_ctx *ctx, double *psi, long long cap);
 
+/* Ground impedance histories [i + nx*j][ground_poles] (FP64), when lee_options.ground_poles > 0. */
+int  lee_get_ground_psi(lee_ctx *ctx, double *psi, long long cap);
 /* Device-side time of the last lee_step call (HIP events on the context stream), ms. */
 double lee_last_step_ms(const lee_ctx *ctx);
 

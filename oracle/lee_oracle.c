@@ -96,6 +96,10 @@ struct orc_sim {
      * index of airpore.cpp:572 (index_pressure, steps done) */
     double *src_ps; long src_n; int src_period;
     long index_pressure;
+    /* ground impedance (extension): South face, the East TDBC recursion with K poles */
+    int ground, gK;
+    double gZinf, g_tb, g_inv, g_decay[8], g_gain[8];
+    double *Gpsi;                      /* [i + nx*j][gK] */
 };
 
 static const double TD_A[6] = {8.226223820e-01, 3.732304064e-01, 9.106232372e-02,
@@ -494,6 +498,12 @@ static windterms wind_terms(const orc_sim *s, const metric9 *m, int i, int j, in
 #define BDX(F) ((F)[IDX(g, i, j, k)] - (F)[IDX(g, i - 1, j, k)])
 #define BDY(F) ((F)[IDX(g, i, j, k)] - (F)[IDX(g, i, j - 1, k)])
 #define BDZ(F) ((F)[IDX(g, i, j, k)] - (F)[IDX(g, i, j, k - 1)])
+/* central differences of the convective terms on the RK4 path (extension): the reference's
+ * one-sided pair advects the velocities across the flow downwind, which grows at up to
+ * 2|V|/dy; the central form is neutral */
+#define CDX(F) (((F)[IDX(g, i + 1, j, k)] - (F)[IDX(g, i - 1, j, k)]) * 0.5)
+#define CDY(F) (((F)[IDX(g, i, j + 1, k)] - (F)[IDX(g, i, j - 1, k)]) * 0.5)
+#define CDZ(F) (((F)[IDX(g, i, j, k + 1)] - (F)[IDX(g, i, j, k - 1)]) * 0.5)
 
 static double sigma_of(double R, double alpha, double t) { return R * pow(t, alpha); }
 static double beta_of(double alpha, double t) { return 1.0 + 25.0 * pow(t, alpha); }
@@ -522,7 +532,9 @@ static double air_dp_point(orc_sim *s, const double *u, const double *v, const d
     size_t id = IDX(g, i, j, k);
     double pc = p[id];
     double r;
-    r = -t.c1 * BDX(p) - t.c2 * BDY(p) - t.c3 * BDZ(p);
+    const int cen = s->opt.integrator == LEE_INT_RK4;
+    if (cen) r = -t.c1 * CDX(p) - t.c2 * CDY(p) - t.c3 * CDZ(p);
+    else r = -t.c1 * BDX(p) - t.c2 * BDY(p) - t.c3 * BDZ(p);
     r += -gam * pc * dt * (t.UX + t.VY + t.WZ);
     r += -gam * Pav * dt * (m.xX * FDX(u) * dxi + m.yX * FDY(u) * dyi + m.zX * FDZ(u) * dzi
                           + m.xY * FDX(v) * dxi + m.yY * FDY(v) * dyi + m.zY * FDZ(v) * dzi
@@ -700,13 +712,17 @@ static void air_duvw_point(orc_sim *s, const double *u, const double *v, const d
     size_t id = IDX(g, i, j, k);
     double uc = u[id], vc = v[id], wc = w[id];
     double du, dv, dw;
-    du = -t.c1 * FDX(u) - t.c2 * FDY(u) - t.c3 * FDZ(u);
+    const int cen = s->opt.integrator == LEE_INT_RK4;
+    if (cen) du = -t.c1 * CDX(u) - t.c2 * CDY(u) - t.c3 * CDZ(u);
+    else du = -t.c1 * FDX(u) - t.c2 * FDY(u) - t.c3 * FDZ(u);
     du += -uc * t.UX * dt - vc * t.UY * dt - wc * t.UZ * dt;
     du += -Aav * dt * (m.xX * BDX(p) * dxi + m.yX * BDY(p) * dyi + m.zX * BDZ(p) * dzi);
-    dv = -t.c1 * FDX(v) - t.c2 * FDY(v) - t.c3 * FDZ(v);
+    if (cen) dv = -t.c1 * CDX(v) - t.c2 * CDY(v) - t.c3 * CDZ(v);
+    else dv = -t.c1 * FDX(v) - t.c2 * FDY(v) - t.c3 * FDZ(v);
     dv += -uc * t.VX * dt - vc * t.VY * dt - wc * t.VZ * dt;
     dv += -Aav * dt * (m.xY * BDX(p) * dxi + m.yY * BDY(p) * dyi + m.zY * BDZ(p) * dzi);
-    dw = -t.c1 * FDX(w) - t.c2 * FDY(w) - t.c3 * FDZ(w);
+    if (cen) dw = -t.c1 * CDX(w) - t.c2 * CDY(w) - t.c3 * CDZ(w);
+    else dw = -t.c1 * FDX(w) - t.c2 * FDY(w) - t.c3 * FDZ(w);
     dw += -uc * t.WX * dt - vc * t.WY * dt - wc * t.WZ * dt;
     dw += -Aav * dt * (m.xZ * BDX(p) * dxi + m.yZ * BDY(p) * dyi + m.zZ * BDZ(p) * dzi);
 
@@ -1021,6 +1037,41 @@ static void bc_v(ogrid *g, int face)
 
 /* East TDBC: calculation_2D::UpdateBC_velocity(EastBC,Z,tau), Cal_TDBC, Update_Psi
  * (calculation_2D.cpp:986-1028). */
+/* Ground impedance, pressure: the East face's quadratic extrapolation along -z. */
+static void bc_p_ground(ogrid *g)
+{
+    for (int j = 0; j < g->ny; ++j)
+        for (int i = 0; i < g->nx; ++i)
+            g->pnn[IDX(g, i, j, 0)] = 15.0 / 8.0 * g->pnn[IDX(g, i, j, 1)] - 5.0 / 4.0 * g->pnn[IDX(g, i, j, 2)] +
+                                      3.0 / 8.0 * g->pnn[IDX(g, i, j, 3)];
+}
+
+/* Ground impedance, velocity: UpdateBC_velocity's TDBC branch (calculation_2D.cpp:986-1028)
+ * on the South face with the outward normal along -z: tangential components copied from
+ * k = 1, the velocity into the ground vg = Cal_TDBC(Psi, p) with K poles from the face
+ * pressure, then Update_Psi with vg.  The velocity that the interior divergence reads at
+ * the low face is w at k = 1 (forward differences), so w = -vg is set on the planes k = 0
+ * and k = 1 -- as the East TDBC sets the v its last interior divergence reads, with the face
+ * pressure half a cell outward of it. */
+static void bc_v_ground(orc_sim *s)
+{
+    ogrid *g = &s->air;
+    int nx = g->nx, ny = g->ny;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            size_t e = IDX(g, i, j, 0), e1 = IDX(g, i, j, 1);
+            g->unn[e] = g->unn[e1];
+            g->vnn[e] = g->vnn[e1];
+            double *Psi = &s->Gpsi[((size_t)i + (size_t)nx * j) * s->gK];
+            double temp_psi = 0;
+            for (int q = 0; q < s->gK; ++q) temp_psi += s->g_decay[q] * Psi[q];
+            double vg = s->g_inv * (g->pnn[e] / s->gZinf - temp_psi);
+            g->wnn[e] = -vg;
+            g->wnn[e1] = -vg;
+            for (int q = 0; q < s->gK; ++q) Psi[q] = s->g_decay[q] * Psi[q] + (s->g_gain[q] * vg) * s->g_tb;
+        }
+}
+
 static void bc_v_tdbc(orc_sim *s)
 {
     ogrid *g = &s->air;
@@ -1400,6 +1451,25 @@ orc_sim *orc_create_opt(const lee_deck *d, const lee_options *opt, int fill_nn_n
         s->tau = d->gamma * rho * d->pore_q * d->pore_q / d->pore_resistivity / d->pore_porosity;
     }
     s->Psi = dalloc((size_t)h[0] * h[6] * 6, 0.0); /* zeroed: the reference leaves Psi uninitialised */
+    if (opt->ground_poles > 0) {
+        if (opt->integrator == LEE_INT_RK4 || d->bc[LEE_FACE_S] != LEE_BC_RIGID || d->mf_judge == 1 || d->hill_judge) {
+            orc_destroy(s); FAIL("the ground impedance replaces a rigid South face of an FB_p_v run in a fixed frame");
+        }
+        if (opt->ground_poles > LEE_MAX_POLES || !(opt->ground_zinf > 0) || !(opt->ground_tau > 0)) {
+            orc_destroy(s); FAIL("bad ground impedance");
+        }
+        s->ground = 1; s->gK = opt->ground_poles; s->gZinf = opt->ground_zinf;
+        s->g_tb = d->dt / opt->ground_tau;
+        double at = 0;
+        for (int q = 0; q < s->gK; ++q) {
+            if (!(opt->ground_a[q] >= 0) || !(opt->ground_r[q] >= 0)) { orc_destroy(s); FAIL("bad ground poles"); }
+            at += opt->ground_a[q] * s->g_tb;
+            s->g_decay[q] = exp(-1 * opt->ground_r[q] * s->g_tb);
+            s->g_gain[q] = opt->ground_a[q];
+        }
+        s->g_inv = 1.0 / (1 + at);
+        s->Gpsi = dalloc((size_t)h[0] * JM * s->gK, 0.0);
+    }
     /* receivers */
     if (nrec > 0) {
         s->nrec = nrec;
@@ -1437,7 +1507,13 @@ void orc_destroy(orc_sim *s)
     free(s->ri); free(s->rj); free(s->rk);
     free(s->cr_uc);
     free(s->src_ps);
+    free(s->Gpsi);
     free(s);
+}
+
+void orc_get_ground_psi(const orc_sim *s, double *psi)
+{
+    if (s->ground) memcpy(psi, s->Gpsi, (size_t)s->air.nx * s->air.ny * s->gK * sizeof(double));
 }
 
 int orc_obstacle(const orc_sim *s, int *box, int *mask)
@@ -1523,7 +1599,9 @@ static void one_step(orc_sim *s)
     orc_air_dp(s, a->un, a->vn, a->wn, a->pn, a->pnn);
     add_interior(a, a->pnn, a->pn);
     bc_p(a, FN); bc_p(a, FE); bc_p(a, FW); bc_p(a, FF); bc_p(a, FB);
-    if (!s->south_porous) {
+    if (s->ground) {
+        bc_p_ground(a);
+    } else if (!s->south_porous) {
         bc_p(a, FS);
     } else {
         ogrid *q = &s->pore;
@@ -1539,7 +1617,9 @@ static void one_step(orc_sim *s)
     bc_v(a, FN);
     if (s->east_tdbc) bc_v_tdbc(s); else bc_v(a, FE);
     bc_v(a, FW); bc_v(a, FF); bc_v(a, FB);
-    if (!s->south_porous) {
+    if (s->ground) {
+        bc_v_ground(s);
+    } else if (!s->south_porous) {
         bc_v(a, FS);
     } else {
         ogrid *q = &s->pore;

@@ -59,6 +59,8 @@ void orc_get_psi(const orc_sim *s, double *psi);
 int  orc_obstacle(const orc_sim *s, int *box, int *mask);
 /* Time-series source of calculation_2D::UpdateSource; n = 0 removes it.  0 or -1. */
 int  orc_set_source_series(orc_sim *s, const double *ps, long n, int period);
+/* Ground impedance histories [i + nx*j][ground_poles] (nothing without a ground impedance). */
+void orc_get_ground_psi(const orc_sim *s, double *psi);
 
 /* Increments only (no add, no BC): the stencils air::cal_fp / air::cal_fuvw /
  * porous::cal_fp / porous::cal_fuvw applied to the given fields, written to out

@@ -64,6 +64,7 @@ def lib():
         L.orc_fft.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.orc_obstacle.argtypes = [ctypes.c_void_p, _ip, _ip]
         L.orc_set_source_series.argtypes = [ctypes.c_void_p, _dp, ctypes.c_long, ctypes.c_int]
+        L.orc_get_ground_psi.argtypes = [ctypes.c_void_p, _dp]
         _lib = L
     return _lib
 
@@ -193,6 +194,11 @@ class Oracle:
         out = np.zeros(10)
         lib().orc_metric(self.h, domain, i, j, k, _p(out))
         return out[:9].reshape(3, 3), out[9]
+
+    def ground_psi(self, K):
+        out = np.zeros((self.ny, self.nx, K))
+        lib().orc_get_ground_psi(self.h, _p(out))
+        return out
 
     def set_source_series(self, ps, period=23):
         a = np.ascontiguousarray(np.asarray(ps, dtype=np.float64).ravel())

@@ -30,7 +30,7 @@ def members(lee):
     # porous with a thicker layer, calm air
     d = B(18, 30, 20, 0.05, pml=4, bc=(2, 3, 2, 2, 2, 2), porous_cells=9, src=(-0.1, 0.8, 0.5), gauss_width=0.2,
           time_domain=40, velocity_method=7, pml_y_time=1)
-    d.wind_ustar, d.wind_z0 = 0.0, 0.02
+    d.wind_ustar, d.wind_z0 = 0.05, 0.02
     d.pore_q, d.pore_porosity, d.pore_resistivity = 1.2, 0.45, 2.0e4
     out.append(d)
     for d in out:

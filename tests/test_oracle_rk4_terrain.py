@@ -143,6 +143,26 @@ def test_rk4_with_porous_ground_converges_at_fourth_order(lee):
         assert 12.0 < d1 / d2 < 20.0
 
 
+@pytest.mark.parametrize("hills", [(), ((0.0, 1.0, 0.25, 0.3),)])
+def test_rk4_mean_flow_convection_is_neutral(lee, hills):
+    """A strong log wind (u* = 8 m/s) over 200 steps: the RK4 path's central convective
+    differences keep the field bounded, flat and over a hill, while the reference's FB_p_v
+    pair (forward differences for u, v, w: downwind for V > 0) grows without bound on the
+    same deck -- the instability the central form removes."""
+    peaks = {}
+    for integ in (1, 0):
+        d = lee.deck.box(16, 40, 16, 0.05, bc=(1, 1, 1, 1, 1, 1), src=(0.0, 1.0, 0.4), gauss_width=0.2,
+                         velocity_method=7)
+        d.wind_ustar, d.wind_z0, d.integrator = 8.0, 0.02, integ
+        d.hills = list(hills) if integ == 1 else []
+        o = O.Oracle(d, integrator=integ)
+        o.frame_setup(0)
+        o.step(200)
+        peaks[integ] = max(np.abs(a).max() for a in o.state())
+    assert peaks[1] < 1.0
+    assert peaks[0] > 1e3
+
+
 def _terrain_deck(lee, hills, **kw):
     dk = lee.deck.box(24, 26, 18, 0.05, pml=4, bc=(2, 1, 2, 2, 2, 2), src=(0.1, 0.6, 0.3), gauss_width=0.25, **kw)
     dk.hills = hills
