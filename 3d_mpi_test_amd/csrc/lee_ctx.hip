@@ -397,6 +397,20 @@ template <typename T> int frame_shift(lee_ctx *c, int N)
             launch_shift<T>((T *)q, (const T *)c->scratch, A, m.ny, C, tempJ, m.lead, s, c->stream);
         }
     }
+    // RK4: the y layers move with the frame as well (their auxiliaries meet the shifted x and
+    // z layers' in the corner cross terms; the reference's FB step leaves them in place)
+    if (m.rk4)
+        for (int sl : {2, 3}) {
+            const SlabGeom &g = m.slab[sl];
+            if (!g.n) continue;
+            size_t cnt = (size_t)m.nx * g.n * m.nz;
+            const int lo = g.lo;
+            for (int f = 0; f < 4; ++f) {
+                void *q = c->Q[c->qcur][sl][f];
+                HIPCHK(c, hipMemcpyAsync(c->scratch, q, cnt * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+                launch_shift_yslab<T>((T *)q, (const T *)c->scratch, m.nx, g.n, m.nz, lo, m.ny, tempJ, m.lead, s, c->stream);
+            }
+        }
     HIPCHK(c, hipGetLastError());
     return LEE_OK;
 }

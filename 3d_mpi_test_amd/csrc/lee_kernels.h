@@ -136,6 +136,9 @@ template <typename T> void launch_pore_v(const PoreArgs<T> &a, hipStream_t s);
 // Moving-frame shift along j of an (A x B x C) box (B = frame length), out of place.
 template <typename T> void launch_shift(T *dst, const T *src, int A, int B, int C, int tempJ, int lead,
                                         int s, hipStream_t st);
+template <typename T>
+void launch_shift_yslab(T *dst, const T *src, int nx, int n, int nz, int lo, int ny, int tempJ, int lead, int sh,
+                        hipStream_t st);
 template <typename T> void launch_copy_plane_j(T *p, int nx, int ny, int nz, hipStream_t s);
 template <typename T> void launch_sample(const T *p, const long long *idx, int n, double *out, hipStream_t s);
 void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s);

@@ -1000,6 +1000,23 @@ __global__ void k_shift(T *dst, const T *src, int A, int B, int C, int tempJ, in
     }
 }
 
+// RK4 frame shift of a y layer's auxiliaries [i + nx*(jj + n*k)], planes lo .. lo+n-1 of the
+// frame: the field's shift rule of k_shift, reading the layer's own plane jj + s, or zero
+// where that plane lies outside the layer.
+template <typename T>
+__global__ void k_shift_yslab(T *dst, const T *src, int nx, int n, int nz, int lo, int ny, int tempJ, int lead, int s)
+{
+    const size_t total = (size_t)nx * n * nz;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int jj = (int)((t / nx) % n), j = lo + jj;
+        T v;
+        if (j < tempJ + 1 && j + lead - 1 < ny - 1) v = jj + s < n ? src[t + (size_t)nx * s] : T(0);
+        else if (j >= tempJ + 1) v = T(0);
+        else v = src[t];
+        dst[t] = v;
+    }
+}
+
 template <typename T>
 __global__ void k_copy_plane_j(T *p, int nx, int ny, int nz)
 {
@@ -1216,6 +1233,13 @@ template <typename T> void launch_shift(T *dst, const T *src, int A, int B, int 
 {
     hipLaunchKernelGGL((k_shift<T>), dim3(flat_blocks((size_t)A * B * C)), dim3(256), 0, st, dst, src, A, B, C, tempJ, lead, sh);
 }
+template <typename T>
+void launch_shift_yslab(T *dst, const T *src, int nx, int n, int nz, int lo, int ny, int tempJ, int lead, int sh,
+                        hipStream_t st)
+{
+    hipLaunchKernelGGL((k_shift_yslab<T>), dim3(flat_blocks((size_t)nx * n * nz)), dim3(256), 0, st, dst, src, nx, n, nz, lo,
+                       ny, tempJ, lead, sh);
+}
 template <typename T> void launch_copy_plane_j(T *p, int nx, int ny, int nz, hipStream_t s)
 {
     hipLaunchKernelGGL((k_copy_plane_j<T>), dim3((nx + 63) / 64, nz), dim3(64), 0, s, p, nx, ny, nz);
@@ -1271,6 +1295,7 @@ void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s)
     template void launch_pore_v<T>(const PoreArgs<T> &, hipStream_t);                                        \
     template void launch_shift<T>(T *, const T *, int, int, int, int, int, int, hipStream_t);                \
     template void launch_copy_plane_j<T>(T *, int, int, int, hipStream_t);                                   \
+    template void launch_shift_yslab<T>(T *, const T *, int, int, int, int, int, int, int, int, hipStream_t);  \
     template void launch_sample<T>(const T *, const long long *, int, double *, hipStream_t);                \
     template void launch_finite_check<T>(const T *, size_t, int *, hipStream_t);                             \
     template void launch_wind_log<T>(T *, int, int, int, const double *, const double *, const double *,     \

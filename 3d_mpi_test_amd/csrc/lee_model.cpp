@@ -305,6 +305,39 @@ std::string Model::build(const lee_deck &deck, const lee_options &o, int &code)
         std::string e = obstacle_mask();
         if (!e.empty()) { code = LEE_ERR_ARG; return e; }
     }
+    if (terrain) {
+        // time-step bound of the terrain-following grid: the cells over a hill are shorter
+        // by 1 - h/D and the slopes add zX, zY to the vertical metric row, so the acoustic
+        // number c dt sqrt(1/dx^2 + 1/dy^2 + (1 + hx^2 + hy^2)/((1 - h/D) dz)^2) must stay below
+        // the integrator's limit: 1 for FB_p_v, 2 sqrt(2) / 2 for RK4 (imaginary-axis bound
+        // over the one-sided pair's spectral radius 2 c sqrt(...)).
+        // every y the frames will cover: the frame-0 axis continued at dy to ijkmax planes
+        const int na = std::max(ijkmax, ny);
+        const double D = az[nz - 1] - az[0];
+        double qmax = 0;
+        for (int j = 0; j < na; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const double x = ax[i], y = j < ny ? ay[j] : ay[ny - 1] + (j - ny + 1) * d.dy;
+                double h = 0, hx = 0, hy = 0;
+                for (int n = 0; n < o.n_hills; ++n) {
+                    const double *H = &o.hills[4 * n];
+                    const double e = H[2] * std::exp(-((x - H[0]) * (x - H[0]) + (y - H[1]) * (y - H[1])) / (2.0 * H[3] * H[3]));
+                    h += e;
+                    hx += -e * (x - H[0]) / (H[3] * H[3]);
+                    hy += -e * (y - H[1]) / (H[3] * H[3]);
+                }
+                const double sq = 1.0 - h / D;
+                if (!(sq > 0)) { code = LEE_ERR_ARG; return "a hill reaches the top of the air grid"; }
+                qmax = std::max(qmax, (1.0 + hx * hx + hy * hy) / (sq * sq));
+            }
+        const double num = d.c * d.dt * std::sqrt(1.0 / (d.dx * d.dx) + 1.0 / (d.dy * d.dy) + qmax / (d.dz * d.dz));
+        const double lim = rk4 ? std::sqrt(2.0) : 1.0;
+        if (num > lim) {
+            code = LEE_ERR_ARG;
+            return "time step too long for the terrain: acoustic number " + std::to_string(num) + " > " +
+                   std::to_string(lim) + (rk4 ? " (RK4)" : " (FB_p_v)");
+        }
+    }
     return "";
 }
 

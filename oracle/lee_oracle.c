@@ -1192,6 +1192,29 @@ static void shift_slab_z(orc_sim *s, oslab *sl, int temp_J)
                 }
 }
 
+/* RK4 path (extension): the y layers' auxiliaries move with the frame too.  SetQMove shifts
+ * only the x and z layers (air.cpp:2522-2729), so at the corners where a y layer meets an x
+ * or z layer the cross terms combine shifted and unshifted auxiliaries, and a run with PML on
+ * a y face and on an x or z face grows from frame to frame.  Here a y-layer node takes the
+ * value of the node it receives the field from (the field rule of the shift), or zero when
+ * that node lies outside the layer (it enters the layer now). */
+static void shift_slab_y(orc_sim *s, oslab *sl, int lo, int temp_J)
+{
+    ogrid *g = &s->air;
+    int sh = s->trail - 1;
+    for (int f = 0; f < 4; ++f)
+        for (int jj = 0; jj < sl->n; ++jj) {
+            int j = lo + jj;
+            for (int k = 0; k < g->nz; ++k)
+                for (int i = 0; i < g->nx; ++i) {
+                    if (j < temp_J + 1 && j + s->lead - 1 < g->ny - 1)
+                        QY(sl, f, i, jj, k) = (jj + sh < sl->n) ? QY(sl, f, i, jj + sh, k) : 0.0;
+                    else if (j >= temp_J + 1)
+                        QY(sl, f, i, jj, k) = 0.0;
+                }
+        }
+}
+
 static void set_qmove(orc_sim *s, int N)
 {
     if (N == 0) {
@@ -1210,6 +1233,10 @@ static void set_qmove(orc_sim *s, int N)
     if (s->sx2.n) shift_slab_x(s, &s->sx2, temp_J);
     if (s->sz1.n) shift_slab_z(s, &s->sz1, temp_J);
     if (s->sz2.n) shift_slab_z(s, &s->sz2, temp_J);
+    if (s->opt.integrator == LEE_INT_RK4) {
+        if (s->sy1.n) shift_slab_y(s, &s->sy1, 0, temp_J);
+        if (s->sy2.n) shift_slab_y(s, &s->sy2, s->air.ny - s->sy2.n, temp_J);
+    }
 }
 
 /* ------------------------------------------------------------------------------------ */

@@ -143,6 +143,19 @@ def test_unsupported_decks_are_rejected(lee, change, code):
     assert e.value.code == code, str(e.value)
 
 
+@pytest.mark.parametrize("integrator,height,ok", [(0, 0.3, False), (1, 0.3, True), (1, 0.55, False)])
+def test_terrain_time_step_bound(lee, integrator, height, ok):
+    """Over a hill the cells shrink by 1 - h/D: the acoustic number must stay below 1 (FB_p_v)
+    or sqrt(2) (RK4); beyond, the deck is refused before any device work."""
+    g = lee.deck.box(24, 30, 20, 0.04, bc=(1, 1, 1, 1, 1, 1), src=(0.0, 0.5, 0.4), gauss_width=0.24)
+    g.hills, g.integrator = [(0.0, 0.6, height, 0.3)], integrator
+    if ok:
+        return  # accepted decks are exercised on the GPU
+    with pytest.raises(lee.LeeError) as e:
+        lee.Solver(g)
+    assert e.value.code == -1 and "time step too long" in str(e.value)
+
+
 def test_porous_grid_must_share_the_air_index_space(lee):
     g = lee.deck.box(12, 10, 10, 0.05, bc=(1, 3, 1, 1, 1, 1), porous_cells=6)
     g.pore_x = g.pore_x[:-1]
