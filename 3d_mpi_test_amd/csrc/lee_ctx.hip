@@ -59,6 +59,7 @@ struct lee_ctx {
     void *Q[2][6][4] = {};  // PML auxiliaries, two sets: Q[qcur] holds the values before a step
     int qcur = 0;
     bool fused = false;     // both air phases in one kernel (k_air_fb)
+    int rk_variant = -1;    // RK4 air-stage launch shape (kRkTiles index), -1 until tuned
     // RK4 registers: second stage state, accumulator, FP32 accumulator low part and state
     // error ([0] air, [1] porous; u v w p), and of the auxiliaries (stage state, accumulator)
     void *rk_sb[2][4] = {}, *rk_acc[2][4] = {}, *rk_lo[2][4] = {}, *rk_err[2][4] = {};
@@ -546,7 +547,7 @@ template <typename T> void enqueue_rk4_step(lee_ctx *c, double *rec_slot)
         RKBndArgs<T> b;
         rk4_stage_args<T>(c, st, J1, y1on, r, pr, b);
         prof_mark(c, KS_RK_AIR, true);
-        launch_rk_air<T>(r, wind, m.terrain, s);
+        launch_rk_air_v<T>(r, wind, m.terrain, c->rk_variant, s);
         prof_mark(c, KS_RK_AIR, false);
         if (m.south_porous) {
             prof_mark(c, KS_RK_PORE, true);
@@ -680,10 +681,50 @@ template <typename T> int graph_steps(lee_ctx *c, int n)
     return LEE_OK;
 }
 
+// Autotuner of the RK4 air stage: times each launch shape on stage 0 of the coming step (a
+// stage-0 launch reads only level n and writes the stage registers, so repeating it changes
+// nothing the step itself does not overwrite) and keeps the fastest.
+template <typename T> int tune_rk(lee_ctx *c)
+{
+    const Model &m = c->m;
+    if (m.opt.rk_tile > 0) {
+        c->rk_variant = std::min(m.opt.rk_tile, kRkVariants) - 1;
+        return LEE_OK;
+    }
+    const long long save = c->counter;
+    int J1, y1on;
+    gate_tick(c, J1, y1on);
+    c->counter = save;
+    RKArgs<T> r;
+    RKPoreArgs<T> pr;
+    RKBndArgs<T> b;
+    rk4_stage_args<T>(c, 0, J1, y1on, r, pr, b);
+    const int wind = rk_wind_mode(m);
+    float best = 1e30f;
+    int pick = 0;
+    for (int v = 0; v < kRkVariants; ++v) {
+        launch_rk_air_v<T>(r, wind, m.terrain, v, c->stream);  // warm-up
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        for (int rep = 0; rep < 3; ++rep) launch_rk_air_v<T>(r, wind, m.terrain, v, c->stream);
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        if (ms < best) { best = ms; pick = v; }
+    }
+    HIPCHK(c, hipGetLastError());
+    c->rk_variant = pick;
+    return LEE_OK;
+}
+
 template <typename T> int run_steps(lee_ctx *c, int nsteps, double *rec_out)
 {
     Model &m = c->m;
     hipStream_t s = c->stream;
+    if (m.rk4 && c->rk_variant < 0) {
+        int rc = tune_rk<T>(c);
+        if (rc) return rc;
+    }
     const int nr = (int)m.ri.size();
     HIPCHK(c, hipEventRecord(c->ev0, s));
     int done = 0;
@@ -747,6 +788,7 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
                            "the fused path runs the FB_p_v step on a separable grid and, with a porous South face, "
                            "needs the interface copy");
         if (o.path < LEE_PATH_AUTO || o.path > LEE_PATH_FUSED) return set_err(nullptr, LEE_ERR_ARG, "unknown path");
+        if (o.rk_tile < 0 || o.rk_tile > kRkVariants) return set_err(nullptr, LEE_ERR_ARG, "rk_tile must be 0..5");
         // AUTO runs the split passes: on MI355X they are faster than the fused pass (whose
         // per-plane barrier limits the loads in flight)
         C->fused = o.path == LEE_PATH_FUSED;
@@ -884,6 +926,7 @@ int lee_get_info(const lee_ctx *c, lee_info *info)
     info->steps_done = c->steps;
     info->pml_y1_counter = c->counter;
     info->path = c->fused ? LEE_PATH_FUSED : LEE_PATH_SPLIT;
+    info->rk_tile = c->rk_variant + 1;
     return LEE_OK;
 }
 

@@ -23,6 +23,15 @@ namespace {
 constexpr int BX = 64, BY = 4;
 static_assert(BX == kTileX && BY == kTileY, "ensemble block prefixes are counted in BX x BY tiles");
 
+// Tile t of a flat grid whose blocks go round-robin to the 8 XCDs: XCD x runs the x-th
+// contiguous range of tiles, in order.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    const int x = b & 7, per = nb >> 3, rem = nb & 7;
+    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + (b >> 3);
+}
+
+
 template <typename T> __device__ __forceinline__ void metric(const Grid<T> &g, int i, int j, int k, T &mX, T &mY, T &mZ)
 {
     if constexpr (std::is_same<T, double>::value) {
@@ -648,11 +657,6 @@ template <typename T> __device__ __forceinline__ void q_node(const AirArgs<T> &a
 // Workgroups are dealt to the 8 XCDs round-robin (b, b+8, b+16, ... share one XCD and its
 // L2).  Give every XCD a contiguous run of tiles, so the halo a block re-reads was fetched
 // into the same L2 by its neighbour.
-__device__ __forceinline__ int xcd_tile(int b, int nb)
-{
-    const int x = b & 7, per = nb >> 3, rem = nb & 7;
-    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + (b >> 3);
-}
 
 template <typename T, bool WIND>
 __global__ __launch_bounds__(64 * (FB_TY + 1)) void k_air_fb(const AirArgs<T> a, int ntx, int nty, int kc)
@@ -718,6 +722,7 @@ __global__ __launch_bounds__(64 * (FB_TY + 1)) void k_air_fb(const AirArgs<T> a,
 // part through the stage sums (TwoSum) and the stored state keeps its rounding error `err`
 // into the next step (FastTwoSum), so the accumulation does not lose FP32 ulps every step.
 // ------------------------------------------------------------------------------------
+
 template <typename T> __device__ __forceinline__ T rk_a(int st) { return st == 0 ? T(0.5) : st == 1 ? T(0.5) : T(1.0); }
 template <typename T> __device__ __forceinline__ T rk_b(int st)
 {
@@ -831,6 +836,33 @@ __global__ __launch_bounds__(BX * BY) void k_rk_air(const RKArgs<T> r)
 {
     const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
     if (i >= r.a.g.nx - 1 || j >= r.a.g.ny - 1) return;
+    rk_air_point<T, WM, TERRAIN, COMP>(r, i, j, k);
+}
+
+// Tuned forms of the RK4 air stage: tiles of 64 x TY points and a flat grid.  COLUMNS orders
+// the tiles plane-fastest (a tile's column of planes one after another) and hands each XCD a
+// contiguous run of columns, so the planes k-1 and k+1 a tile reads were loaded into the same
+// XCD's L2 by the tiles just before and after it; otherwise the order is plane by plane.
+template <typename T, int WM, bool TERRAIN, bool COMP, int TY, bool COLUMNS>
+__global__ __launch_bounds__(BX * TY) void k_rk_air_t(const RKArgs<T> r)
+{
+    const int nx = r.a.g.nx, ny = r.a.g.ny, K = r.a.g.nz - 2;
+    const int tx = (nx - 2 + BX - 1) / BX, ty = (ny - 2 + TY - 1) / TY;
+    int bx, by, k;
+    if constexpr (COLUMNS) {
+        const int t = xcd_tile(blockIdx.x, gridDim.x);
+        k = t % K + 1;
+        const int col = t / K;
+        bx = col % tx;
+        by = col / tx;
+    } else {
+        const int b = blockIdx.x;
+        bx = b % tx;
+        by = (b / tx) % ty;
+        k = b / (tx * ty) + 1;
+    }
+    const int i = bx * BX + threadIdx.x + 1, j = by * TY + threadIdx.y + 1;
+    if (i >= nx - 1 || j >= ny - 1) return;
     rk_air_point<T, WM, TERRAIN, COMP>(r, i, j, k);
 }
 
@@ -1162,6 +1194,36 @@ template <typename T> void launch_rk_air(const RKArgs<T> &r, int wind, bool terr
 #undef LEE_RK
 }
 
+// Variant v of the tuned stage launch (lee_ctx keeps the autotuner's choice): v = 0 the
+// plane-by-plane 64x4 grid, 1..4 the flat grids of kRkTiles.
+template <typename T> void launch_rk_air_v(const RKArgs<T> &r, int wind, bool terrain, int v, hipStream_t s)
+{
+    if (v <= 0) { launch_rk_air<T>(r, wind, terrain, s); return; }
+    constexpr bool C = std::is_same<T, float>::value;
+    const int nx = r.a.g.nx, ny = r.a.g.ny, K = r.a.g.nz - 2;
+    const int TYv = kRkTiles[v].ty;
+    const unsigned nb = (unsigned)(((nx - 2 + BX - 1) / BX) * ((ny - 2 + TYv - 1) / TYv) * K);
+#define LEE_RKT(TY, COL)                                                                              \
+    do {                                                                                              \
+        if (terrain) {                                                                                \
+            if (wind == 1) hipLaunchKernelGGL((k_rk_air_t<T, 1, true, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, r);  \
+            else if (wind == 2) hipLaunchKernelGGL((k_rk_air_t<T, 2, true, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, r); \
+            else hipLaunchKernelGGL((k_rk_air_t<T, 0, true, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, r); \
+        } else {                                                                                      \
+            if (wind == 1) hipLaunchKernelGGL((k_rk_air_t<T, 1, false, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, r); \
+            else if (wind == 2) hipLaunchKernelGGL((k_rk_air_t<T, 2, false, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, r); \
+            else hipLaunchKernelGGL((k_rk_air_t<T, 0, false, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, r); \
+        }                                                                                             \
+    } while (0)
+    switch (v) {
+    case 1: LEE_RKT(4, true); break;
+    case 2: LEE_RKT(8, true); break;
+    case 3: LEE_RKT(8, false); break;
+    default: LEE_RKT(16, true); break;
+    }
+#undef LEE_RKT
+}
+
 template <typename T>
 void launch_rk_air_batch(const RKArgs<T> *args, const int *first, int n, int nblocks, int wind, bool terrain, hipStream_t s)
 {
@@ -1288,6 +1350,7 @@ void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s)
     template void launch_rk_air<T>(const RKArgs<T> &, int, bool, hipStream_t);                               \
     template void launch_rk_pore<T>(const RKPoreArgs<T> &, bool, hipStream_t);                               \
     template void launch_rk_air_batch<T>(const RKArgs<T> *, const int *, int, int, int, bool, hipStream_t);   \
+    template void launch_rk_air_v<T>(const RKArgs<T> &, int, bool, int, hipStream_t);                        \
     template void launch_rk_pore_batch<T>(const RKPoreArgs<T> *, const int *, int, int, bool, hipStream_t);   \
     template void launch_sample_batch<T>(const T *const *, const long long *, int, double *, hipStream_t);     \
     template void launch_air_fb<T>(const AirArgs<T> &, bool, hipStream_t);                                   \

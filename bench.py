@@ -46,6 +46,8 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# launch shapes of the RK4 air stage (lee_options.rk_tile 1..5), as the autotuner reports them
+RK_TILES = ["64x4 planes", "64x4 xcd-columns", "64x8 xcd-columns", "64x8 planes", "64x16 xcd-columns"]
 
 
 def run_gpu(lee, deck, receivers, precision, steps, warmup, record=True):
@@ -86,9 +88,11 @@ def run_gpu(lee, deck, receivers, precision, steps, warmup, record=True):
         if k.launches:
             kst.append(dict(name=k.name.decode(), launches=k.launches, total_ms=k.total_ms,
                             bytes_per_launch=k.bytes_per_launch, points=k.points_per_launch))
+    tile = s.info().rk_tile
     copy_ms = lee.lib().lee_copy_bench(s.h, 1 << 28, 10)  # 1 GiB read + 1 GiB write
     rec = np.concatenate([x for x in series if x is not None], axis=0) if record else None
-    info = dict(points=npts, wall=t1 - t0, kernels=kst, copy_ms=copy_ms, frames=state["mf"] + 1, series=rec)
+    info = dict(points=npts, wall=t1 - t0, kernels=kst, copy_ms=copy_ms, frames=state["mf"] + 1, series=rec,
+                rk_tile=RK_TILES[tile - 1] if tile > 0 else None)
     s.close()
     return info
 
@@ -301,6 +305,7 @@ def main():
         "config": {"workload": workload,
                    "air_points": int(np.prod(deck.shape)), "porous_points": r["points"] - int(np.prod(deck.shape)),
                    "frames_in_run": r["frames"], "integrator": "RK4 (FP32: compensated accumulation)",
+                   "rk_air_launch": r["rk_tile"],
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -110,6 +110,10 @@ typedef struct lee_options {
     int ground_poles;       /* 0: off, else 1..LEE_MAX_POLES pole/residue pairs */
     double ground_a[8], ground_r[8];
     double ground_zinf, ground_tau;
+    /* RK4 air-stage launch shape: 0 = autotune on the first steps (times the candidates on the
+     * context's own buffers), 1..5 = candidate 0..4 (64x4 plane order, 64x4 / 64x8 XCD columns,
+     * 64x8 plane order, 64x16 XCD columns).  All give the same bits. */
+    int rk_tile;
 } lee_options;
 
 /* Time integrators.  LEE_INT_DECK runs the deck's scheme (the reference implements only
@@ -138,6 +142,7 @@ typedef struct lee_info {
     long long steps_done;           /* time steps advanced so far */
     long long pml_y1_counter;       /* reference PML_JMAX1 step counter */
     int path;                       /* LEE_PATH_SPLIT or LEE_PATH_FUSED, as chosen */
+    int rk_tile;                    /* RK4 air-stage launch shape in use (1..5), 0 before tuning */
 } lee_info;
 
 typedef struct lee_ctx lee_ctx;

@@ -109,3 +109,29 @@ def test_fb_terrain_split_matches_oracle(lee, orc):
     print(f"  rec rel-L2 {e:.3e} field {fe:.3e}", flush=True)
     s.close()
     assert e <= 1e-10 and fe <= 1e-10
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_rk4_launch_shapes_give_the_same_bits(lee, prec):
+    """Every launch shape of the RK4 air stage (the autotuner's candidates) computes the same
+    per-point arithmetic: identical receivers and fields."""
+    d = cases(lee)["terrain"]
+    print(f"[gpu] rk4 launch shapes fp{prec}", flush=True)
+    ref = None
+    for v in range(1, 6):
+        s = lee.Solver(d, precision=prec, receivers=REC, integrator=1, rk_tile=v)
+        s.frame_setup(0)
+        got = (s.step(d.time_domain), s.state())
+        assert s.info().rk_tile == v
+        s.close()
+        if ref is None:
+            ref = got
+            continue
+        assert np.array_equal(got[0], ref[0]), v
+        for a, b in zip(got[1], ref[1]):
+            assert np.array_equal(a, b), v
+    s = lee.Solver(d, precision=prec, receivers=REC, integrator=1)
+    s.frame_setup(0)
+    s.step(2)
+    assert 1 <= s.info().rk_tile <= 5
+    s.close()
