@@ -837,8 +837,12 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
             size_t n = d ? np : na;
             for (int f = 0; f < 4; ++f) {
                 if ((rc = dalloc(C, &C->rk_sb[d][f], n * es)) || (rc = dalloc(C, &C->rk_acc[d][f], n * es))) return rc;
-                if (C->prec == 32)
-                    if ((rc = dalloc(C, &C->rk_lo[d][f], n * es)) || (rc = dalloc(C, &C->rk_err[d][f], n * es))) return rc;
+                // FP32: the carried state error; the low part of the accumulator only for the
+                // porous velocities (integrating-factor update)
+                if (C->prec == 32) {
+                    if ((rc = dalloc(C, &C->rk_err[d][f], n * es))) return rc;
+                    if (d == 1 && f < 3 && (rc = dalloc(C, &C->rk_lo[d][f], n * es))) return rc;
+                }
             }
         }
         for (int s = 0; s < 6; ++s) {
@@ -1443,11 +1447,13 @@ int lee_kernel_stats(const lee_ctx *c, lee_kstat *out, int cap)
                       (m.south_porous ? (double)boundary_count<double>(m.pnx, m.pny, m.pnz) : 0.0);
     const double psi = m.east_tdbc ? (double)m.nx * m.nz * 6 * 8 * 2 : 0.0;
     // RK4, per stage launch (a quarter of a step): fields 64 words per point and step (x_n,
-    // stage state, accumulator: stages read 4/12/12/8 and write 8/8/8/4 words), 32 more for
-    // the FP32 compensation (accumulator low part, state error); the auxiliaries likewise 64
-    // per slab element and step; the mean flow 1 (V only) or 3 words per point and stage;
-    // the faces write 4 words per boundary node from 4 source values.
-    const double rkw = (c->prec == 32 ? 96.0 : 64.0) / 4.0;
+    // stage state, accumulator: stages read 4/12/12/8 and write 8/8/8/4 words); FP32 reads x_n
+    // and the carried state error in the last stage and writes the error back (76 words); the
+    // porous velocities keep a compensated accumulator (24 words each per step, the pressure
+    // 19); the auxiliaries 64 per slab element and step; the mean flow 1 (V only) or 3 words
+    // per point and stage; the faces write 4 words per boundary node from 4 source values.
+    const double rkw = (c->prec == 32 ? 76.0 : 64.0) / 4.0;
+    const double prkw = (c->prec == 32 ? 3 * 24.0 + 19.0 : 64.0) / 4.0;
     double qrk = 0;
     for (int s = 0; s < 6; ++s) {
         double cnt = (double)m.slab[s].count(m.nx, m.ny, m.nz);
@@ -1458,7 +1464,7 @@ int lee_kernel_stats(const lee_ctx *c, lee_kstat *out, int cap)
     const char *names[KS_N] = {"air_p", "air_v", "pore_p", "pore_v", "air_fb", "bc_p", "bc_v", "rk_air", "rk_pore", "rk_bnd"};
     double bytes[KS_N] = {(5 * inner + qp + wind) * es, (7 * inner + qv + wind) * es, 5 * pinner * es, 7 * pinner * es,
                           (8 * inner + qf + wind) * es, 2 * nb * es, 6 * nb * es + psi,
-                          (rkw * inner + qrk + wrk) * es, rkw * pinner * es, 8 * nb * es};
+                          (rkw * inner + qrk + wrk) * es, prkw * pinner * es, 8 * nb * es};
     double pts[KS_N] = {inner, inner, pinner, pinner, inner, nb, nb, inner, pinner, nb};
     int n = 0;
     for (int k = 0; k < KS_N && n < cap; ++k) {

@@ -717,10 +717,12 @@ __global__ __launch_bounds__(64 * (FB_TY + 1)) void k_air_fb(const AirArgs<T> a,
 //   stage 0:   acc = x_n + b0 k,          dst = x_n + a0 k
 //   stage 1,2: acc = acc + b_s k,         dst = x_n + a_s k
 //   stage 3:                              dst = acc + b3 k   (= x_{n+1})
+// (the FP64 form; FP32 below)
 // with (a) = (1/2, 1/2, 1), (b) = (1/6, 1/3, 1/3, 1/6).  The PML auxiliaries integrate
-// dQ/dt = f of the stage state with the same registers.  With COMP (FP32) acc carries a low
-// part through the stage sums (TwoSum) and the stored state keeps its rounding error `err`
-// into the next step (FastTwoSum), so the accumulation does not lose FP32 ulps every step.
+// dQ/dt = f of the stage state with the same registers.  With COMP (FP32) acc sums only the
+// increments b_s k_s and the last stage adds them to x_n with an exact TwoSum whose rounding
+// error `err` is carried into the next step, so the state does not lose FP32 ulps every step
+// (76 words per point and step instead of the 64 + 32 of compensating every stage sum).
 // ------------------------------------------------------------------------------------
 
 template <typename T> __device__ __forceinline__ T rk_a(int st) { return st == 0 ? T(0.5) : st == 1 ? T(0.5) : T(1.0); }
@@ -743,19 +745,22 @@ __device__ __forceinline__ T rk_update(int st, T xn, T k, T *acc, T *lo, T *err,
         }
         return acc[id] + rk_b<T>(3) * k;
     } else {
-        const T y = rk_b<T>(st) * k;
-        const T a0 = st == 0 ? xn : acc[id];
-        const T sm = a0 + y;
-        const T bb = sm - a0;
-        const T e = (a0 - (sm - bb)) + (y - bb);  // TwoSum: a0 + y = sm + e exactly
-        const T l = (st == 0 ? err[id] : lo[id]) + e;
+        // acc holds the step's increment sum b1 k1 + ... (small next to the state, so its FP32
+        // rounding is small next to an ulp of the state); the last stage adds it and the
+        // carried error of the state with an exact TwoSum and keeps the new rounding error
+        (void)lo;
+        if (st == 0) {
+            acc[id] = rk_b<T>(0) * k;
+            return xn + rk_a<T>(0) * k;
+        }
         if (st < 3) {
-            acc[id] = sm;
-            lo[id] = l;
+            acc[id] = acc[id] + rk_b<T>(st) * k;
             return xn + rk_a<T>(st) * k;
         }
-        const T x = sm + l;
-        err[id] = l - (x - sm);  // FastTwoSum (|sm| >= |l|): sm + l = x + err
+        const T y = (acc[id] + rk_b<T>(3) * k) + err[id];
+        const T x = xn + y;
+        const T bb = x - xn;
+        err[id] = (xn - (x - bb)) + (y - bb);  // TwoSum: xn + y = x + err exactly
         return x;
     }
 }
@@ -815,7 +820,8 @@ __device__ __forceinline__ void rk_air_point(const RKArgs<T> &r, int i, int j, i
     const T xc[4] = {a.g.un[id], a.g.vn[id], a.g.wn[id], a.g.pn[id]};  // stage state at the point
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
-        const T xn = st == 0 ? xc[f] : r.xn[f][id];
+        // level n: the stage state itself at stage 0; not needed by the last FP64 stage
+        const T xn = st == 0 ? xc[f] : (!COMP && st == 3) ? T(0) : r.xn[f][id];
         r.dst[f][id] = rk_update<T, COMP>(st, xn, kf[f], r.acc[f], r.lo[f], r.err[f], id);
     }
     // auxiliaries of the point's own slabs: dQ/dt = f (air.cpp:2323-2439 as an ODE)
@@ -932,7 +938,7 @@ __device__ __forceinline__ void rk_pore_point(const RKPoreArgs<T> &r, int i, int
         r.dst[q][id] = rk_update_if<T, COMP>(st, xn, kf[q], r.acc[q], r.lo[q], r.err[q], id, r.cf);
     }
     {
-        const T xn = st == 0 ? xc[3] : r.xn[3][id];
+        const T xn = st == 0 ? xc[3] : (!COMP && st == 3) ? T(0) : r.xn[3][id];
         r.dst[3][id] = rk_update<T, COMP>(st, xn, kf[3], r.acc[3], r.lo[3], r.err[3], id);
     }
 }
