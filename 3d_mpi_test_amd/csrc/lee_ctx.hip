@@ -60,6 +60,12 @@ struct lee_ctx {
     int qcur = 0;
     bool fused = false;     // both air phases in one kernel (k_air_fb)
     int rk_variant = -1;    // RK4 air-stage launch shape (kRkTiles index), -1 until tuned
+    // RK4 air-stage arguments of the four stages in device memory and the host copy they were
+    // uploaded from (refreshed when buffer roles or the y1 gate change, never inside a capture)
+    void *rk_args_dev = nullptr;
+    std::vector<unsigned char> rk_args_host, rk_args_key;
+    bool capturing = false;
+    bool args_fault = false;  // rk_args_sync failed inside an enqueue (reported by run_steps)
     // RK4 registers: second stage state, accumulator, FP32 accumulator low part and state
     // error ([0] air, [1] porous; u v w p), and of the auxiliaries (stage state, accumulator)
     void *rk_sb[2][4] = {}, *rk_acc[2][4] = {}, *rk_lo[2][4] = {}, *rk_err[2][4] = {};
@@ -530,6 +536,47 @@ void rk4_stage_args(lee_ctx *c, int st, int J1, int y1on, RKArgs<T> &r, RKPoreAr
 
 int rk_wind_mode(const Model &m) { return m.has_wind ? (m.wind_v_only ? 2 : 1) : 0; }
 
+// Brings the device copy of the four stage arguments up to date for the gate state
+// (J1, y1on).  A change is rare (frame shifts, the y1 gate opening) and waits for the copy.
+// The arguments depend on the buffer roles (level and auxiliary sets, the FP32 error and
+// second stage buffers a frame shift swaps) and the gate state only: that is the key.
+template <typename T> int rk_args_sync(lee_ctx *c, int J1, int y1on)
+{
+    const size_t sz = sizeof(RKArgs<T>);
+    const uintptr_t key[8] = {(uintptr_t)c->cur, (uintptr_t)c->qcur, (uintptr_t)J1, (uintptr_t)y1on,
+                              (uintptr_t)c->rk_sb[0][0], (uintptr_t)c->rk_err[0][0], (uintptr_t)c->rk_sb[1][0],
+                              (uintptr_t)c->rk_err[1][0]};
+    std::vector<unsigned char> now((const unsigned char *)key, (const unsigned char *)key + sizeof key);
+    if (c->rk_args_dev && now == c->rk_args_key) return LEE_OK;
+    if (c->capturing) return set_err(c, LEE_ERR_STATE, "RK4 stage arguments changed inside a captured chunk");
+    if (!c->rk_args_dev) {
+        int rc = dalloc(c, &c->rk_args_dev, 4 * sz);
+        if (rc) return rc;
+    }
+    c->rk_args_host.assign(4 * sz, 0);
+    for (int st = 0; st < 4; ++st) {
+        RKArgs<T> r;
+        RKPoreArgs<T> pr;
+        RKBndArgs<T> b;
+        rk4_stage_args<T>(c, st, J1, y1on, r, pr, b);
+        std::memcpy(c->rk_args_host.data() + st * sz, &r, sz);
+    }
+    c->rk_args_key.swap(now);
+    HIPCHK(c, hipMemcpyAsync(c->rk_args_dev, c->rk_args_host.data(), 4 * sz, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LEE_OK;
+}
+
+// Stage arguments of the coming step, without advancing the gate counter.
+template <typename T> int rk_args_peek(lee_ctx *c)
+{
+    const long long save = c->counter;
+    int J1, y1on;
+    gate_tick(c, J1, y1on);
+    c->counter = save;
+    return rk_args_sync<T>(c, J1, y1on);
+}
+
 // One RK4 step: launches only (no host synchronisation), receivers into rec_slot.
 template <typename T> void enqueue_rk4_step(lee_ctx *c, double *rec_slot)
 {
@@ -539,15 +586,17 @@ template <typename T> void enqueue_rk4_step(lee_ctx *c, double *rec_slot)
     const int wind = rk_wind_mode(m);
     int J1, y1on;
     gate_tick(c, J1, y1on);
+    if (rk_args_sync<T>(c, J1, y1on)) c->args_fault = true;
     // receivers sample level n before the step (airpore.cpp:573-580)
     launch_sample<T>((const T *)c->air[c->cur * 4 + 3], c->rec_idx, nr, rec_slot, s);
+    const RKArgs<T> *rd = (const RKArgs<T> *)c->rk_args_dev;
     for (int st = 0; st < 4; ++st) {
         RKArgs<T> r;
         RKPoreArgs<T> pr;
         RKBndArgs<T> b;
         rk4_stage_args<T>(c, st, J1, y1on, r, pr, b);
         prof_mark(c, KS_RK_AIR, true);
-        launch_rk_air_v<T>(r, wind, m.terrain, c->rk_variant, s);
+        launch_rk_air_v<T>(r, rd + st, wind, m.terrain, c->rk_variant, s);
         prof_mark(c, KS_RK_AIR, false);
         if (m.south_porous) {
             prof_mark(c, KS_RK_PORE, true);
@@ -648,6 +697,10 @@ template <typename T> int graph_steps(lee_ctx *c, int n)
     const Model &m = c->m;
     const bool gate = (m.judge[1] == 1 || m.judge[1] == 3);
     const int gate_on = gate && c->counter + 1 >= m.pml_time;
+    if (m.rk4) {  // the chunk's stage arguments (constant over it) into device memory first
+        int rc = rk_args_peek<T>(c);
+        if (rc) return rc;
+    }
     for (auto &g : c->graphs)
         if (g.n == n && g.cur == c->cur && g.qcur == c->qcur && g.gate_on == gate_on && g.psi == (void *)c->psi &&
             g.gpsi == (void *)c->gpsi) {
@@ -666,9 +719,16 @@ template <typename T> int graph_steps(lee_ctx *c, int n)
     lee_ctx::GraphEntry e{n, c->cur, c->qcur, gate_on, (void *)c->psi, (void *)c->gpsi, nullptr};
     const int nr = (int)m.ri.size();
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
     for (int t = 0; t < n; ++t) enqueue_step<T>(c, c->rec_buf + (size_t)t * nr);
+    c->capturing = false;
     hipGraph_t graph = nullptr;
     HIPCHK(c, hipStreamEndCapture(c->stream, &graph));
+    if (c->args_fault) {
+        (void)hipGraphDestroy(graph);
+        c->args_fault = false;
+        return set_err(c, LEE_ERR_STATE, "RK4 stage arguments changed inside a captured chunk");
+    }
     hipError_t ie = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     HIPCHK(c, ie);
@@ -699,13 +759,16 @@ template <typename T> int tune_rk(lee_ctx *c)
     RKPoreArgs<T> pr;
     RKBndArgs<T> b;
     rk4_stage_args<T>(c, 0, J1, y1on, r, pr, b);
+    int rc = rk_args_sync<T>(c, J1, y1on);
+    if (rc) return rc;
+    const RKArgs<T> *rd = (const RKArgs<T> *)c->rk_args_dev;
     const int wind = rk_wind_mode(m);
     float best = 1e30f;
     int pick = 0;
     for (int v = 0; v < kRkVariants; ++v) {
-        launch_rk_air_v<T>(r, wind, m.terrain, v, c->stream);  // warm-up
+        launch_rk_air_v<T>(r, rd, wind, m.terrain, v, c->stream);  // warm-up
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-        for (int rep = 0; rep < 3; ++rep) launch_rk_air_v<T>(r, wind, m.terrain, v, c->stream);
+        for (int rep = 0; rep < 3; ++rep) launch_rk_air_v<T>(r, rd, wind, m.terrain, v, c->stream);
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         HIPCHK(c, hipEventSynchronize(c->ev1));
         float ms = 0.f;
@@ -741,6 +804,10 @@ template <typename T> int run_steps(lee_ctx *c, int nsteps, double *rec_out)
             }
         } else {
             for (int t = 0; t < n; ++t) enqueue_step<T>(c, c->rec_buf + (size_t)t * nr);
+        }
+        if (c->args_fault) {
+            c->args_fault = false;
+            return set_err(c, LEE_ERR_DEVICE, "RK4 stage arguments could not be uploaded: " + c->err);
         }
         HIPCHK(c, hipGetLastError());
         if (rec_out && nr) {

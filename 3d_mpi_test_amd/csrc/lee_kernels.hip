@@ -837,9 +837,12 @@ __device__ __forceinline__ void rk_air_point(const RKArgs<T> &r, int i, int j, i
         }
 }
 
+// The stage arguments (some 200 pointers) live in device memory: passed by value they
+// would fill the scalar registers and spill to vector lanes at every use.
 template <typename T, int WM, bool TERRAIN, bool COMP>
-__global__ __launch_bounds__(BX * BY) void k_rk_air(const RKArgs<T> r)
+__global__ __launch_bounds__(BX * BY) void k_rk_air(const RKArgs<T> *__restrict__ rp)
 {
+    const RKArgs<T> &r = *rp;
     const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
     if (i >= r.a.g.nx - 1 || j >= r.a.g.ny - 1) return;
     rk_air_point<T, WM, TERRAIN, COMP>(r, i, j, k);
@@ -850,8 +853,9 @@ __global__ __launch_bounds__(BX * BY) void k_rk_air(const RKArgs<T> r)
 // contiguous run of columns, so the planes k-1 and k+1 a tile reads were loaded into the same
 // XCD's L2 by the tiles just before and after it; otherwise the order is plane by plane.
 template <typename T, int WM, bool TERRAIN, bool COMP, int TY, bool COLUMNS>
-__global__ __launch_bounds__(BX * TY) void k_rk_air_t(const RKArgs<T> r)
+__global__ __launch_bounds__(BX * TY) void k_rk_air_t(const RKArgs<T> *__restrict__ rp)
 {
+    const RKArgs<T> &r = *rp;
     const int nx = r.a.g.nx, ny = r.a.g.ny, K = r.a.g.nz - 2;
     const int tx = (nx - 2 + BX - 1) / BX, ty = (ny - 2 + TY - 1) / TY;
     int bx, by, k;
@@ -1187,11 +1191,12 @@ template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStrea
         else hipLaunchKernelGGL((k_air_v<T, false, false>), g, b, 0, s, a);
     }
 }
-template <typename T> void launch_rk_air(const RKArgs<T> &r, int wind, bool terrain, hipStream_t s)
+template <typename T>
+void launch_rk_air(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool terrain, hipStream_t s)
 {
     constexpr bool C = std::is_same<T, float>::value;
     dim3 g = interior_grid(r.a.g.nx, r.a.g.ny, r.a.g.nz), b(BX, BY);
-#define LEE_RK(WM, TER) hipLaunchKernelGGL((k_rk_air<T, WM, TER, C>), g, b, 0, s, r)
+#define LEE_RK(WM, TER) hipLaunchKernelGGL((k_rk_air<T, WM, TER, C>), g, b, 0, s, rd)
     if (terrain) {
         if (wind == 1) LEE_RK(1, true); else if (wind == 2) LEE_RK(2, true); else LEE_RK(0, true);
     } else {
@@ -1202,9 +1207,10 @@ template <typename T> void launch_rk_air(const RKArgs<T> &r, int wind, bool terr
 
 // Variant v of the tuned stage launch (lee_ctx keeps the autotuner's choice): v = 0 the
 // plane-by-plane 64x4 grid, 1..4 the flat grids of kRkTiles.
-template <typename T> void launch_rk_air_v(const RKArgs<T> &r, int wind, bool terrain, int v, hipStream_t s)
+template <typename T>
+void launch_rk_air_v(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool terrain, int v, hipStream_t s)
 {
-    if (v <= 0) { launch_rk_air<T>(r, wind, terrain, s); return; }
+    if (v <= 0) { launch_rk_air<T>(r, rd, wind, terrain, s); return; }
     constexpr bool C = std::is_same<T, float>::value;
     const int nx = r.a.g.nx, ny = r.a.g.ny, K = r.a.g.nz - 2;
     const int TYv = kRkTiles[v].ty;
@@ -1212,13 +1218,13 @@ template <typename T> void launch_rk_air_v(const RKArgs<T> &r, int wind, bool te
 #define LEE_RKT(TY, COL)                                                                              \
     do {                                                                                              \
         if (terrain) {                                                                                \
-            if (wind == 1) hipLaunchKernelGGL((k_rk_air_t<T, 1, true, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, r);  \
-            else if (wind == 2) hipLaunchKernelGGL((k_rk_air_t<T, 2, true, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, r); \
-            else hipLaunchKernelGGL((k_rk_air_t<T, 0, true, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, r); \
+            if (wind == 1) hipLaunchKernelGGL((k_rk_air_t<T, 1, true, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, rd);  \
+            else if (wind == 2) hipLaunchKernelGGL((k_rk_air_t<T, 2, true, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, rd); \
+            else hipLaunchKernelGGL((k_rk_air_t<T, 0, true, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, rd); \
         } else {                                                                                      \
-            if (wind == 1) hipLaunchKernelGGL((k_rk_air_t<T, 1, false, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, r); \
-            else if (wind == 2) hipLaunchKernelGGL((k_rk_air_t<T, 2, false, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, r); \
-            else hipLaunchKernelGGL((k_rk_air_t<T, 0, false, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, r); \
+            if (wind == 1) hipLaunchKernelGGL((k_rk_air_t<T, 1, false, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, rd); \
+            else if (wind == 2) hipLaunchKernelGGL((k_rk_air_t<T, 2, false, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, rd); \
+            else hipLaunchKernelGGL((k_rk_air_t<T, 0, false, C, TY, COL>), dim3(nb), dim3(BX, TY), 0, s, rd); \
         }                                                                                             \
     } while (0)
     switch (v) {
@@ -1353,10 +1359,10 @@ void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s)
     template void launch_air_p<T>(const AirArgs<T> &, bool, hipStream_t);                                    \
     template void launch_air_v<T>(const AirArgs<T> &, bool, hipStream_t);                                    \
     template bool fused_fits<T>(int);                                                                        \
-    template void launch_rk_air<T>(const RKArgs<T> &, int, bool, hipStream_t);                               \
+    template void launch_rk_air<T>(const RKArgs<T> &, const RKArgs<T> *, int, bool, hipStream_t);            \
     template void launch_rk_pore<T>(const RKPoreArgs<T> &, bool, hipStream_t);                               \
     template void launch_rk_air_batch<T>(const RKArgs<T> *, const int *, int, int, int, bool, hipStream_t);   \
-    template void launch_rk_air_v<T>(const RKArgs<T> &, int, bool, int, hipStream_t);                        \
+    template void launch_rk_air_v<T>(const RKArgs<T> &, const RKArgs<T> *, int, bool, int, hipStream_t);     \
     template void launch_rk_pore_batch<T>(const RKPoreArgs<T> *, const int *, int, int, bool, hipStream_t);   \
     template void launch_sample_batch<T>(const T *const *, const long long *, int, double *, hipStream_t);     \
     template void launch_air_fb<T>(const AirArgs<T> &, bool, hipStream_t);                                   \
