@@ -210,7 +210,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fb", action="store_true", help="skip the reference-scheme (FB_p_v, flat) lines")
     ap.add_argument("--precisions", default="32,64")
-    ap.add_argument("--traffic-json", default=None,
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "rk4_air_c1_traffic.json"),
                     help="JSON {kernel: HBM bytes per launch} from rocprofv3 FETCH_SIZE/WRITE_SIZE runs")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -253,7 +253,7 @@ def main():
     traffic = None
     if a.traffic_json and os.path.exists(a.traffic_json):
         with open(a.traffic_json) as fh:
-            traffic = json.load(fh).get(f"{dom['name']}_fp{head}")
+            traffic = json.load(fh).get(f"{dom['name']}_fp{head}") if a.workload == "c1" else None
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom["name"],
             "bytes_per_launch": dom["bytes_per_launch"], "avg_launch_ms": round(avg_s * 1e3, 4),
