@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(_HERE, "liblee_hip.so")
 
 deck = importlib.import_module(__name__ + ".deck")
 LeeDeck = deck.LeeDeck
+ground = importlib.import_module(__name__ + ".ground")
 
 LEE_OK = 0
 PATH_AUTO, PATH_SPLIT, PATH_FUSED = 0, 1, 2

@@ -283,6 +283,29 @@ int  lee_ensemble_step(lee_ensemble *e, int nsteps, double *rec_out, int use_gra
 double lee_ensemble_last_ms(const lee_ensemble *e);
 const char *lee_ensemble_error(const lee_ensemble *e);
 
+/* Learned ground model (no counterpart in the reference): an MLP 4 -> 64 -> 64 -> 64 -> 8 with
+ * GELU, from normalised ground parameters (flow resistivity, porosity, tortuosity, layer
+ * depth) to the residues of the ground impedance's eight poles (lee_options.ground_*).
+ * Trained and evaluated on the GPU: BF16 MFMA products with FP32 accumulation, FP32 master
+ * weights, mean-squared-error loss, Adam.  Parameters are one flat FP32 array, layer by
+ * layer W[out][in] then b[out] over padded shapes 16->64, 64->64, 64->64, 64->16 (padding
+ * stays zero).  Rows of x have 4 values, rows of y / t have 8. */
+typedef struct lee_mlp lee_mlp;
+int  lee_mlp_create(int device, unsigned long long seed, lee_mlp **out);
+int  lee_mlp_destroy(lee_mlp *n);
+const char *lee_mlp_error(const lee_mlp *n);
+long long lee_mlp_param_count(const lee_mlp *n);
+int  lee_mlp_get_params(lee_mlp *n, float *w, long long cap);
+int  lee_mlp_set_params(lee_mlp *n, const float *w, long long count);
+int  lee_mlp_predict(lee_mlp *n, const float *x, int rows, float *y);
+/* Gradient of the mean squared error over rows (into grad, nparam floats, may be NULL) and
+ * the error itself; the weights are left unchanged. */
+int  lee_mlp_gradient(lee_mlp *n, const float *x, const float *t, int rows, float *grad, double *mse);
+/* epochs of shuffled mini-batches (the last partial batch of an epoch is dropped) with Adam
+ * (beta 0.9 / 0.999, eps 1e-8) at learning rate lr. */
+int  lee_mlp_train(lee_mlp *n, const float *x, const float *t, int rows, int batch, int epochs, float lr,
+                   unsigned long long seed, double *final_mse);
+
 /* Output writers of the drop-in file contract.
  * ./solution/pty%04d.dat: one %.6e sample per line (airpore.cpp:468-531, 577).
  * ./solution/p_f2_%04d.dat: zero-padded radix-2 FFT, "f |X|" at setprecision(10)
