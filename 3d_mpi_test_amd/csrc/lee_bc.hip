@@ -43,18 +43,44 @@ template <typename T> __device__ __forceinline__ T east_extrap(const Grid<T> &g,
 
 template <typename T> __device__ __forceinline__ T p_pore_walk(const BCArgs<T> &a, int i, int j, int k, int t);
 
+// Ground impedance (extension): the plane k = 0 is the ground's interface plane, as the
+// porous ground's top plane is when the air reads it through the interface copy
+// (calculation_2D.cpp:878-886).  Its interior nodes are stencil output of the porous
+// equations of the ground's top material (porous.cpp:77-143 with porous.cpp:44-50) -- the
+// faces N E W F B then treat its edge nodes as on every other plane, and there is no South
+// face.  The normal velocity (half a cell below the plane) follows the porous momentum
+// equation with the pressure half a cell further down given by the impedance through the
+// TDBC recursion of calculation_2D.cpp:986-1028 with K poles.  Flat grid: the plane's
+// metric is gmx[i] = 1/X_over_x, gmy[j] = 1/Y_over_y, gmz = 1/Z_over_z of the plane k = 1.
+template <typename T> __device__ __forceinline__ T ground_p(const BCArgs<T> &a, int i, int j)
+{
+    const Grid<T> &g = a.air;
+    const size_t e = gidx(g, i, j, 0);
+    const T ex = (T)a.gmx[i], ey = (T)a.gmy[j], ez = (T)a.gmz;
+    const T div = ex * g.dxi * (g.un[gidx(g, i + 1, j, 0)] - g.un[e]) + ey * g.dyi * (g.vn[gidx(g, i, j + 1, 0)] - g.vn[e]) +
+                  ez * g.dzi * (g.wn[gidx(g, i, j, 1)] - g.wn[e]);
+    return g.pn[e] + (-(T)a.g_gama * div);
+}
+
+// air pressure of a node as the stencil (or the ground plane) left it
+template <typename T> __device__ __forceinline__ T p_air_base(const BCArgs<T> &a, int i, int j, int k)
+{
+    if (k == 0 && a.ground) return ground_p(a, i, j);
+    return a.air.pnn[gidx(a.air, i, j, k)];
+}
+
+template <typename T> __device__ __forceinline__ T east_extrap_air(const BCArgs<T> &a, int i, int k)
+{
+    const Grid<T> &g = a.air;
+    const T a0 = p_air_base(a, i, g.ny - 2, k), a1 = p_air_base(a, i, g.ny - 3, k), a2 = p_air_base(a, i, g.ny - 4, k);
+    return T(15.0 / 8.0) * a0 - T(5.0 / 4.0) * a1 + T(3.0 / 8.0) * a2;
+}
+
 // air pressure after faces 1..t (t <= 6): N E W F B, then S or the ghost plane
 template <typename T, bool ALLOW_PORE> __device__ __forceinline__ T p_air_walk(const BCArgs<T> &a, int i, int j, int k, int t)
 {
     const Grid<T> &g = a.air;
-    if (t >= 6 && k == 0 && a.ground) {
-        // ground impedance: the East face's quadratic extrapolation along -z from the
-        // planes k = 1, 2, 3 as faces 1..5 left them
-        const T a0 = p_air_walk<T, false>(a, i, j, 1, 5), a1 = p_air_walk<T, false>(a, i, j, 2, 5),
-                a2 = p_air_walk<T, false>(a, i, j, 3, 5);
-        return T(15.0 / 8.0) * a0 - T(5.0 / 4.0) * a1 + T(3.0 / 8.0) * a2;
-    }
-    if (t >= 6 && k == 0 && (!a.porous || a.iface)) {
+    if (t >= 6 && k == 0 && !a.ground && (!a.porous || a.iface)) {
         if constexpr (ALLOW_PORE) {
             if (a.porous) return p_pore_walk(a, i, j, a.pore.nz - 2, 5);
         }
@@ -63,9 +89,9 @@ template <typename T, bool ALLOW_PORE> __device__ __forceinline__ T p_air_walk(c
     if (t >= 5 && i == g.nx - 1) i = g.nx - 2;
     if (t >= 4 && i == 0) i = 1;
     if (t >= 3 && j == 0) j = 1;
-    if (t >= 2 && j == g.ny - 1) return east_extrap(g, i, k == g.nz - 1 ? g.nz - 2 : k);
+    if (t >= 2 && j == g.ny - 1) return east_extrap_air(a, i, k == g.nz - 1 ? g.nz - 2 : k);
     if (t >= 1 && k == g.nz - 1) k = g.nz - 2;
-    return g.pnn[gidx(g, i, j, k)];
+    return p_air_base(a, i, j, k);
 }
 
 // porous pressure after faces 1..t: S E W F B, then the ghost plane Kp-1 <- air k=1
@@ -96,31 +122,49 @@ template <typename T> __device__ __forceinline__ double tdbc_v(const BCArgs<T> &
     return a.inv * ((double)g.pnn[gidx(g, i, g.ny - 1, k)] / a.Zinf - temp_psi);
 }
 
-// Ground impedance: the velocity into the ground (-w at k = 0) from the final face pressure
-// and the histories of the previous step -- Cal_TDBC (calculation_2D.cpp:1019-1028) with K
-// poles.
-template <typename T> __device__ __forceinline__ double ground_v(const BCArgs<T> &a, int i, int j)
+// Ground plane velocities of interior node (i, j) from the final pressures of the plane:
+// tangential components by the porous momentum equation, the normal one (double: it feeds
+// the FP64 histories) with the impedance's pressure half a cell below it,
+//   w' = ib (w - cpl (p' - p_h)),  p_h = Zinf (1 + tb sum a) v + Zinf sum_k e^{-r_k tb} Psi_k,
+// v = -w', solved for w'.
+template <typename T> __device__ __forceinline__ double ground_w(const BCArgs<T> &a, int i, int j)
 {
     const Grid<T> &g = a.air;
+    const size_t e = gidx(g, i, j, 0);
     const double *P = a.gpsi_old + ((size_t)i + (size_t)g.nx * j) * a.gK;
     double temp_psi = 0;
     for (int q = 0; q < a.gK; ++q) temp_psi += a.gdecay[q] * P[q];
-    return a.ginv * ((double)g.pnn[gidx(g, i, j, 0)] / a.gZinf - temp_psi);
+    const double hist = a.gZinf * temp_psi;
+    return ((double)g.wn[e] - a.gcpl * ((double)g.pnn[e] - hist)) * a.g_ib / (1.0 + a.g_ib * a.gcpl * a.gzi);
+}
+
+template <typename T> __device__ __forceinline__ T ground_vel(const BCArgs<T> &a, int c, int i, int j)
+{
+    const Grid<T> &g = a.air;
+    const size_t e = gidx(g, i, j, 0);
+    const T ib = (T)a.g_ib, dt = a.dt, eff = (T)a.g_eff;
+    if (c == 0) {
+        const T ex = (T)a.gmx[i];
+        return g.un[e] + (T(-1.0) * ib * dt * eff * (ex * (g.pnn[e] - g.pnn[gidx(g, i - 1, j, 0)]) * g.dxi) + g.un[e] * ib - g.un[e]);
+    }
+    if (c == 1) {
+        const T ey = (T)a.gmy[j];
+        return g.vn[e] + (T(-1.0) * ib * dt * eff * (ey * (g.pnn[e] - g.pnn[gidx(g, i, j - 1, 0)]) * g.dyi) + g.vn[e] * ib - g.vn[e]);
+    }
+    return (T)ground_w(a, i, j);
 }
 
 template <typename T> __device__ __forceinline__ T v_pore_walk(const BCArgs<T> &a, int c, int i, int j, int k, int t);
 
-// air velocity component c after faces 1..t; t = 7 adds the ground impedance's normal
-// velocity on the plane k = 1 (the first w that the interior divergence reads)
+// air velocity component c after faces 1..t
 template <typename T, bool ALLOW_PORE> __device__ __forceinline__ T v_air_walk(const BCArgs<T> &a, int c, int i, int j, int k, int t)
 {
     const Grid<T> &g = a.air;
-    if (t >= 7 && a.ground && k == 1 && c == 2) return (T)(-ground_v(a, i, j));
-    if (t >= 6 && k == 0 && (!a.porous || a.iface)) {
+    if (t >= 6 && k == 0 && !a.ground && (!a.porous || a.iface)) {
         if constexpr (ALLOW_PORE) {
             if (a.porous) return v_pore_walk(a, c, i, j, a.pore.nz - 2, 5);
         }
-        if (c == 2) return a.ground ? (T)(-ground_v(a, i, j)) : T(0);
+        if (c == 2) return T(0);
         k = 1;
     }
     if (t >= 5 && i == g.nx - 1) { if (c == 0) return T(0); i = g.nx - 2; }
@@ -131,6 +175,7 @@ template <typename T, bool ALLOW_PORE> __device__ __forceinline__ T v_air_walk(c
         j = g.ny - 2;
     }
     if (t >= 1 && k == g.nz - 1) { if (c == 2) return T(0); k = g.nz - 2; }
+    if (k == 0 && a.ground) return ground_vel(a, c, i, j);
     return vfield(g, c)[gidx(g, i, j, k)];
 }
 
@@ -219,9 +264,9 @@ __global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
     if (t < a.nb_air) {
         int i, j, k;
         boundary_node(t, a.air.nx, a.air.ny, a.air.nz, i, j, k);
-        const T u = v_air_walk<T, true>(a, 0, i, j, k, 7);
-        const T v = v_air_walk<T, true>(a, 1, i, j, k, 7);
-        const T w = v_air_walk<T, true>(a, 2, i, j, k, 7);
+        const T u = v_air_walk<T, true>(a, 0, i, j, k, 6);
+        const T v = v_air_walk<T, true>(a, 1, i, j, k, 6);
+        const T w = v_air_walk<T, true>(a, 2, i, j, k, 6);
         const size_t id = gidx(a.air, i, j, k);
         a.air.unn[id] = u; a.air.vnn[id] = v; a.air.wnn[id] = w;
         q_boundary(a, 0, i, j, k, u);
@@ -234,9 +279,9 @@ __global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
 #pragma unroll
             for (int q = 0; q < 6; ++q) a.psi_new[b + q] = a.decay[q] * a.psi_old[b + q] + (a.gain[q] * vt) * a.tb;
         }
-        if (a.ground && k == 0) {
+        if (a.ground && k == 0 && i > 0 && i < a.air.nx - 1 && j > 0 && j < a.air.ny - 1) {
             // Update_Psi (calculation_2D.cpp:1012-1017) with the velocity into the ground
-            const double vg = ground_v(a, i, j);
+            const double vg = -ground_w(a, i, j);
             const size_t b = ((size_t)i + (size_t)a.air.nx * j) * a.gK;
             for (int q = 0; q < a.gK; ++q) a.gpsi_new[b + q] = a.gdecay[q] * a.gpsi_old[b + q] + (a.ggain[q] * vg) * a.gtb;
         }
@@ -248,15 +293,6 @@ __global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
         const T w = v_pore_walk(a, 2, i, j, k, 6);
         const size_t id = gidx(a.pore, i, j, k);
         a.pore.unn[id] = u; a.pore.vnn[id] = v; a.pore.wnn[id] = w;
-    } else if (a.ground && t < a.nb_air + a.nb_pore + (long long)(a.air.nx - 2) * (a.air.ny - 2)) {
-        // ground impedance: the normal velocity of the interior nodes of the plane k = 1
-        // (the x and y face nodes of that plane get it through their walks) and its PML
-        // auxiliaries, which the velocity stencil integrated with the value it replaced
-        const long long r = t - a.nb_air - a.nb_pore;
-        const int i = 1 + (int)(r % (a.air.nx - 2)), j = 1 + (int)(r / (a.air.nx - 2));
-        const T w = (T)(-ground_v(a, i, j));
-        a.air.wnn[gidx(a.air, i, j, 1)] = w;
-        q_boundary(a, 2, i, j, 1, w);
     }
 }
 
@@ -360,7 +396,7 @@ template <typename T> void launch_bc_p_all(const BCArgs<T> &a, hipStream_t s)
 
 template <typename T> void launch_bc_v_all(const BCArgs<T> &a, hipStream_t s)
 {
-    const long long n = a.nb_air + a.nb_pore + (a.ground ? (long long)(a.air.nx - 2) * (a.air.ny - 2) : 0);
+    const long long n = a.nb_air + a.nb_pore;
     hipLaunchKernelGGL((k_bc_v_all<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
 }
 

@@ -85,6 +85,8 @@ struct lee_ctx {
     std::vector<GraphEntry> graphs;
     double *psi = nullptr, *psi2 = nullptr;  // TDBC histories, read / written each step
     double *gpsi = nullptr, *gpsi2 = nullptr;  // ground impedance histories (South face)
+    double *gmx = nullptr, *gmy = nullptr;     // ground plane metric 1/X_over_x, 1/Y_over_y
+    double gmz = 0;                            // and 1/Z_over_z of the plane k = 1
     long long *rec_idx = nullptr;
     double *rec_buf = nullptr;
     int *flag = nullptr;
@@ -300,6 +302,10 @@ template <typename T> BCArgs<T> bc_args(lee_ctx *c, const AirArgs<T> &aa, double
     b.ground = m.ground; b.gK = m.gK; b.gpsi_old = c->gpsi; b.gpsi_new = c->gpsi2;
     for (int q = 0; q < 8; ++q) { b.gdecay[q] = m.g_decay[q]; b.ggain[q] = m.g_gain[q]; }
     b.ginv = m.g_inv; b.gZinf = m.gZinf; b.gtb = m.g_tb;
+    b.gmx = c->gmx; b.gmy = c->gmy; b.gmz = c->gmz;
+    b.g_gama = m.g_gama; b.g_eff = m.g_eff; b.g_ib = m.g_ib;
+    b.gcpl = 2.0 * m.dt * m.g_eff * c->gmz * m.dzi;
+    b.gzi = m.ground ? m.gZinf / m.g_inv : 0.0;
     b.rec_idx = c->rec_idx; b.nrec = (int)m.ri.size(); b.rec_out = rec_out;
     return b;
 }
@@ -322,6 +328,16 @@ int upload_frame(lee_ctx *c, int N)
     if ((rc = metric(c->mx, t.Xx)) || (rc = metric(c->my, t.Yy)) || (rc = metric(c->mz, t.Zz))) return rc;
     if (m.south_porous)
         if ((rc = metric(c->pmx, t.pXx)) || (rc = metric(c->pmy, t.pYy)) || (rc = metric(c->pmz, t.pZz))) return rc;
+    if (m.ground) {
+        std::vector<double> gx(m.nx, 0.0), gy(m.ny, 0.0);
+        for (int i = 1; i + 1 < m.nx; ++i) gx[i] = 1.0 / t.Xx[i];
+        for (int j = 1; j + 1 < m.ny; ++j) gy[j] = 1.0 / t.Yy[j];
+        c->gmz = 1.0 / t.Zz[1];
+        // FP64 in both precisions (the plane's coefficients are cast per use)
+        HIPCHK(c, hipMemcpyAsync(c->gmx, gx.data(), gx.size() * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->gmy, gy.data(), gy.size() * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     const Profile *P[6] = {&t.x1, &t.x2, &t.y1, &t.y2, &t.z1, &t.z2};
     for (int s = 0; s < 6; ++s) {
         if ((rc = upload(c, c->prof[s][0], P[s]->s.data(), P[s]->s.size()))) return rc;
@@ -849,7 +865,7 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
     {
         const Model &mm = C->m;
         const bool can = fused_fits<double>(mm.nx) && !(mm.south_porous && !mm.iface_copy) && !mm.terrain && !mm.rk4 &&
-                         !mm.obstacle;
+                         !mm.obstacle && !mm.ground;
         if (o.path == LEE_PATH_FUSED && !can)
             return set_err(nullptr, LEE_ERR_UNSUPPORTED,
                            "the fused path runs the FB_p_v step on a separable grid and, with a porous South face, "
@@ -894,6 +910,8 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
     if (m.ground) {
         const size_t ng = (size_t)m.nx * m.ny * m.gK * 8;
         if ((rc = dalloc(C, (void **)&C->gpsi, ng)) || (rc = dalloc(C, (void **)&C->gpsi2, ng))) return rc;
+        if ((rc = dalloc(C, (void **)&C->gmx, (size_t)m.nx * 8)) || (rc = dalloc(C, (void **)&C->gmy, (size_t)m.ny * 8)))
+            return rc;
     }
     int nr = (int)m.ri.size();
     if ((rc = dalloc(C, (void **)&C->rec_idx, (size_t)std::max(nr, 1) * sizeof(long long)))) return rc;

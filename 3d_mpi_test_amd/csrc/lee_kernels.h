@@ -70,11 +70,16 @@ template <typename T> struct BCArgs {
     const double *psi_old;     // East TDBC histories [i + nx*k][6], read
     double *psi_new;           // and written (double buffered)
     double decay[6], gain[6], inv, Zinf, tb;
-    // ground impedance on the South face: histories [i + nx*j][gK], FP64, double buffered
+    // ground impedance on the South face: histories [i + nx*j][gK], FP64, double buffered;
+    // the ground interface plane k = 0: its metric (gmx[i] = 1/X_over_x, gmy[j], gmz), the
+    // porous coefficients of the top material (dt/(Kp porosity), 1/rho_e, 1/(1 + beta)),
+    // 2 dt/(rho_e dz) of the half-cell momentum and Zinf (1 + tb sum_k a_k)
     int ground, gK;
     const double *gpsi_old;
     double *gpsi_new;
     double gdecay[8], ggain[8], ginv, gZinf, gtb;
+    const double *gmx, *gmy;
+    double gmz, g_gama, g_eff, g_ib, gcpl, gzi;
     const long long *rec_idx;  // receiver samples of time level n (pressure phase)
     int nrec;
     double *rec_out;

@@ -248,7 +248,19 @@ std::string Model::build(const lee_deck &deck, const lee_options &o, int &code)
         if (d.bc[LEE_FACE_S] != LEE_BC_RIGID) return "the ground impedance replaces a rigid South face";
         if (d.mf_judge == 1) return "the ground impedance runs with a fixed frame";
         if (obstacle) return "no embedded obstacle with the ground impedance";
+        if (terrain) return "the ground interface plane is flat (no terrain with the ground impedance)";
         code = LEE_ERR_ARG;
+        if (!(d.pore_q > 0) || !(d.pore_porosity > 0) || !(d.pore_resistivity >= 0) || !std::isfinite(d.pore_q) ||
+            !std::isfinite(d.pore_porosity) || !std::isfinite(d.pore_resistivity))
+            return "the ground impedance needs its top material: pore_q > 0, pore_porosity > 0, pore_resistivity >= 0";
+        {
+            // porous.cpp:44-50 for the top material (airpore.cpp:250-251)
+            double eff_density = d.pore_q * d.pore_q / d.pore_porosity / Aav;
+            double Kp = 1.0 / d.gamma / d.Pav;
+            g_eff = 1.0 / eff_density;
+            g_gama = d.dt / Kp / d.pore_porosity;
+            g_ib = 1.0 / (d.pore_resistivity * d.dt * g_eff + 1.0);
+        }
         if (o.ground_poles > LEE_MAX_POLES) return "at most LEE_MAX_POLES ground poles";
         if (!(o.ground_zinf > 0) || !(o.ground_tau > 0) || !std::isfinite(o.ground_zinf) || !std::isfinite(o.ground_tau))
             return "ground_zinf and ground_tau must be finite and positive";

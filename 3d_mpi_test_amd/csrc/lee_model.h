@@ -85,9 +85,12 @@ struct Model {
     // TDBC recursion constants: e^{-r_k dt/tau}, residues a_k, 1/(1 + sum_k a_k dt/tau)
     double td_decay[6] = {}, td_gain[6] = {}, td_inv = 0;
     // ground impedance on the South face (extension): the same recursion with K poles
+    // closing the ground interface plane k = 0, which carries the porous equations of the
+    // ground's top material (deck pore_q, pore_porosity, pore_resistivity)
     bool ground = false;
     int gK = 0;
     double gZinf = 0, g_decay[8] = {}, g_gain[8] = {}, g_inv = 0, g_tb = 0;
+    double g_gama = 0, g_eff = 0, g_ib = 0;
     // moving frame
     int mf_input = 0, lead = 0, trail = 0;
     // receivers (whole-domain indices)

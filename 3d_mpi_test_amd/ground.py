@@ -1,18 +1,29 @@
 """Learned ground: ground parameters -> pole/residue pairs of the ground impedance boundary.
 
-The full solve models the ground as the porous layer of porous.cpp under the air (South
-face porous, FB_p_v).  The learned ground replaces that layer by the South-face impedance of
-lee_options.ground_* (K = 8 real poles, Z(s) = Z_inf (1 + sum_k a_k / (r_k + s tau)),
-Z_inf = rho0 c q / porosity, tau = rho_e / sigma with rho_e = q^2 rho0 / porosity) whose
-residues a_k on a fixed pole grid r_k come from a small network trained on the GPU
-(lee_mlp, BF16 MFMA).  The training data are reflection coefficients measured in runs of the
-full air + porous solve: a plane pulse in a rigid-walled column falls on the layer, the
-reflected and incident pulses at a receiver give R(f) (divided by the same ratio over a rigid
-ground, which removes the air path's numerical dispersion), and Z(f) = rho c (1 + R)/(1 - R)
-is fitted with non-negative residues (passive, so the boundary recursion stays stable).
+The full solve models the ground as the porous layer of porous.cpp under the air (South face
+porous, FB_p_v, the air reading the layer's top plane through the interface copy of
+calculation_2D.cpp:878-886).  The learned ground drops the layer: the air's plane k = 0
+becomes the ground's interface plane, which carries the porous equations of the top
+material, and the layer under it is replaced by the impedance of lee_options.ground_*
+(K = 8 real poles, Z(s) = Z_inf (1 + sum_k a_k / (r_k + s tau)), Z_inf = rho0 c q/porosity,
+tau = rho_e / sigma, rho_e = q^2 rho0 / porosity).  The poles sit on a fixed grid in units
+of 1/tau (R_GRID); a small network (lee_mlp: BF16 MFMA, trained on the GPU) maps (sigma,
+porosity, tortuosity, layer depth) to the residues.
 
-Outside the training hull (flow resistivity below HULL_SIGMA_MIN, or a fit worse than
-HULL_FIT_R in reflection coefficient) the model declines and the caller keeps the full porous solve.
+Training data come from runs of the full air + porous solve: a plane pulse in a rigid-walled
+column falls on the layer, and the incident and reflected pulses at a receiver give R(f)
+(divided by the same ratio over a rigid floor, which removes the air path's dispersion).
+The boundary's own discretisation is taken into account exactly: for a fixed closure the
+column's reflection is a Moebius function of the recursion's discrete impedance Z_d(f)
+(one scalar relation closes a linear system), identified per frequency from four runs of
+the impedance closure with known Z_d; inverting it turns the measured R(f) into the Z_d(f)
+the boundary must realise, and Z_d is linear in the residues, so a non-negative least
+squares fit (weighted by |dR/dZ_d|) gives them -- non-negative residues keep the boundary
+passive and its recursion stable.
+
+The full porous solve stays the judge and the fallback: `LearnedGround.decide` predicts the
+residues, measures the reflection they give against the porous column's, and returns the
+impedance deck only inside the training hull and within the error bound.
 """
 from __future__ import annotations
 
@@ -28,6 +39,8 @@ K = 8
 R_GRID = np.concatenate([[0.0], np.logspace(-2.5, 2.0, K - 1)])
 HULL_SIGMA_MIN = 3.0e4
 HULL_FIT_R = 0.05
+F_BAND = (50.0, 1000.0)       # the accuracy band
+F_FIT = (20.0, 1500.0)        # the fitted band
 # C3 / C4 ground-parameter distribution
 SIGMA_RANGE = (1.0e4, 2.0e6)
 POROSITY_RANGE = (0.2, 0.5)
@@ -39,12 +52,12 @@ def _lee():
     return importlib.import_module(__package__ or "3d_mpi_test_amd")
 
 
-@dataclass
+@dataclass(frozen=True)
 class Ground:
     sigma: float        # flow resistivity [Pa s / m^2]
     porosity: float
     tortuosity: float   # q^2
-    depth: int          # porous layer points
+    depth: int          # porous layer points below the interface plane
 
 
 def normalise(g: Ground):
@@ -56,6 +69,12 @@ def normalise(g: Ground):
     return np.array([2 * ls - 1, 2 * po - 1, 2 * to - 1, 2 * de - 1], dtype=np.float32)
 
 
+def in_hull(g: Ground):
+    """Inside the sampled parameter box, with a flow resistivity the pole grid resolves."""
+    return (HULL_SIGMA_MIN <= g.sigma <= SIGMA_RANGE[1] and POROSITY_RANGE[0] <= g.porosity <= POROSITY_RANGE[1]
+            and TORTUOSITY_RANGE[0] <= g.tortuosity <= TORTUOSITY_RANGE[1] and DEPTH_POINTS[0] <= g.depth <= DEPTH_POINTS[1])
+
+
 def zinf_tau(g: Ground, rho0: float, c: float):
     """Z_inf and tau of the porous equations (porous.cpp:40-51 with airpore.cpp:249-251)."""
     rho_e = g.tortuosity * rho0 / g.porosity
@@ -63,6 +82,7 @@ def zinf_tau(g: Ground, rho0: float, c: float):
 
 
 def sample_grounds(n: int, seed: int):
+    """Grounds drawn from the C3 distribution: sigma log-uniform, the rest uniform."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):
@@ -72,21 +92,51 @@ def sample_grounds(n: int, seed: int):
     return out
 
 
+def set_top_material(deck, g: Ground):
+    deck.pore_q, deck.pore_porosity, deck.pore_resistivity = float(np.sqrt(g.tortuosity)), g.porosity, g.sigma
+
+
+def porous_deck(deck, g: Ground):
+    """`deck` (a Deck built by deck.box with porous_cells = g.depth + 2) with g's material."""
+    set_top_material(deck, g)
+    return deck
+
+
+def impedance_deck(deck, g: Ground, residues, zinf=None, tau=None):
+    """Set the ground impedance of `deck` (rigid South face) from residues on R_GRID: Z_inf and
+    tau of g's material unless given, the top material of g on the interface plane."""
+    rho = deck.gamma * deck.Pav / deck.c ** 2
+    zi, ta = zinf_tau(g, rho, deck.c)
+    deck.ground_poles = [(float(max(a, 0.0)), float(r)) for a, r in zip(residues, R_GRID)]
+    deck.ground_zinf = float(zi if zinf is None else zinf)
+    deck.ground_tau = float(ta if tau is None else tau)
+    set_top_material(deck, g)
+    return deck
+
+
 # ---------------------------------------------------------------------- reflection data
-def column_deck(g: Ground | None, dz=0.04, n_air=320, steps=2048):
-    """A 6 x 6 column of n_air points with rigid side walls and a 20-point PML at the top,
-    over the ground (the porous layer of g, or a rigid floor for g = None); FB_p_v at
-    CFL 0.5 in 3-D."""
+# reflection column: plane pulse released at Z_SRC, receiver at Z_REC, N_AIR points at DZ; the
+# reflected window closes at T_END_M / c, before the pulse's upward tail comes back from the top
+Z_SRC, Z_REC, N_AIR, DZ, COLUMN_STEPS, T_END_M = 6.0, 3.0, 640, 0.04, 2048, 40.0
+
+
+def column_deck(kind: str, g: Ground | None = None, residues=None, zinf=None, tau=None):
+    """A 6 x 6 column of N_AIR points with rigid side walls and a 20-point PML at the top over
+    a rigid floor (kind 'rigid'), g's porous layer ('porous') or the impedance closure with
+    g's top material and the given residues ('impedance'); FB_p_v at CFL 0.5 in 3-D."""
     lee = _lee()
-    kw = dict(pml=20, bc=(2, 3 if g else 1, 1, 1, 1, 1), porous_cells=(g.depth + 2) if g else 0,
-              time_domain=steps, src=(0.0, 0.0, -100.0))
-    d = lee.deck.box(6, 6, n_air, dz, **kw)
-    if g:
-        d.pore_q, d.pore_porosity, d.pore_resistivity = float(np.sqrt(g.tortuosity)), g.porosity, g.sigma
+    porous = kind == "porous"
+    d = lee.deck.box(6, 6, N_AIR, DZ, pml=20, bc=(2, 3 if porous else 1, 1, 1, 1, 1),
+                     porous_cells=(g.depth + 2) if porous else 0, time_domain=COLUMN_STEPS, src=(0.0, 0.0, -100.0))
+    if porous:
+        porous_deck(d, g)
+    elif kind == "impedance":
+        impedance_deck(d, g, residues, zinf, tau)
     return d
 
 
 def _plane_pulse(solver, d, z0, width):
+    """A pulse travelling down: p = exp(-((z - z0)/width)^2), w = -p/(rho0 c)."""
     rho = d.gamma * d.Pav / d.c ** 2
     z = np.asarray(d.air_z)[:, None, None] * np.ones((len(d.air_z), len(d.air_y), len(d.air_x)))
     p = np.exp(-((z - z0) / width) ** 2)
@@ -94,28 +144,32 @@ def _plane_pulse(solver, d, z0, width):
     solver.set_state(zero, zero, -p / (rho * d.c), p)
 
 
-def column_series(g: Ground | None, dz=0.04, n_air=320, z_src=6.0, z_rec=3.0, steps=2048, precision=64):
-    """Pressure at z_rec of a plane pulse from z_src falling on the ground."""
+def column_series(d, precision=64):
+    """Receiver pressure at Z_REC of the plane pulse from Z_SRC in column deck d."""
     lee = _lee()
-    d = column_deck(g, dz, n_air, steps)
-    rec = [(float(d.air_x[2]), float(d.air_y[2]), z_rec)]
+    rec = [(float(d.air_x[2]), float(d.air_y[2]), Z_REC)]
     s = lee.Solver(d, precision=precision, receivers=rec, use_graph=True)
-    s.frame_setup(0)
-    _plane_pulse(s, d, z_src, 3 * dz)
-    out = s.step(steps)[:, 0]
-    s.close()
-    return out, d.dt
+    try:
+        s.frame_setup(0)
+        _plane_pulse(s, d, Z_SRC, 3 * DZ)
+        return s.step(d.time_domain)[:, 0]
+    finally:
+        s.close()
 
 
-def reflection(series, ref_series, dt, c, z_src, z_rec, nfft=4096):
+def reflection(series, ref_series, dt, c, z_src=Z_SRC, z_rec=Z_REC, t_end=None, nfft=8192):
     """R(f) of the ground from the receiver series over it and over a rigid floor: the
-    incident and reflected windows are split half-way between their arrivals."""
+    incident and reflected windows are split half-way between their arrivals; the reflected
+    window ends at t_end (default: the end of the series), before any later arrival."""
     t_split = ((z_src - z_rec) / c + (z_src + z_rec) / c) / 2
     n_split = int(t_split / dt)
+    n_end = len(series) if t_end is None else min(len(series), int(t_end / dt))
+
     def ratio(x):
-        inc = np.zeros(nfft); ref = np.zeros(nfft)
+        inc = np.zeros(nfft)
+        ref = np.zeros(nfft)
         inc[:n_split] = x[:n_split]
-        ref[:len(x) - n_split] = x[n_split:]
+        ref[:n_end - n_split] = x[n_split:n_end]
         with np.errstate(divide="ignore", invalid="ignore"):
             return np.fft.rfft(ref) / np.fft.rfft(inc)
     f = np.fft.rfftfreq(nfft, dt)
@@ -123,54 +177,113 @@ def reflection(series, ref_series, dt, c, z_src, z_rec, nfft=4096):
         return f, ratio(series) / ratio(ref_series)
 
 
-def fit_residues(f, Z, zinf, tau, z0, fmin=20.0, fmax=2000.0):
-    """Non-negative residues on R_GRID for Z(f) / Z_inf - 1, each frequency weighted by the
-    sensitivity of the reflection coefficient to Z (|dR/dZ| = 2 z0 / |Z + z0|^2, z0 = rho0 c);
-    returns (a, max |R_fit - R| over 50-1000 Hz)."""
-    sel = (f >= fmin) & (f <= fmax)
-    st = 2j * np.pi * f[sel] * tau
-    y = Z[sel] / zinf - 1.0
-    A = 1.0 / (R_GRID[None, :] + st[:, None])
-    w = 2.0 * z0 * zinf / np.abs(Z[sel] + z0) ** 2
-    M = np.vstack([(A * w[:, None]).real, (A * w[:, None]).imag])
-    b = np.concatenate([(y * w).real, (y * w).imag])
-    a, _ = nnls(M, b)
-    return a, reflection_error(f, Z, impedance(f, a, zinf, tau), z0)
+def discrete_impedance(f, a, zinf, tau, dt):
+    """Z_d(f) of the recursion: Zinf (1 + sum_k a_k tb / (1 - e^{-r_k tb} z^-1)), z = e^{i w dt},
+    tb = dt/tau (the z-transform of Cal_TDBC / Update_Psi, calculation_2D.cpp:1012-1028)."""
+    tb = dt / tau
+    zi = np.exp(-2j * np.pi * np.asarray(f) * dt)
+    B = tb / (1.0 - np.exp(-R_GRID[None, :] * tb) * zi[:, None])
+    return zinf * (1.0 + B @ np.asarray(a, float))
 
 
-def impedance(f, a, zinf, tau):
-    """Z(f) of residues a on R_GRID."""
-    return zinf * (1 + (1.0 / (R_GRID[None, :] + 2j * np.pi * np.asarray(f)[:, None] * tau)) @ np.asarray(a, float))
+def moebius(Zs, Rs):
+    """Per-frequency coefficients (al, be, ga, de) of R = (al + be Z)/(ga + de Z) from n >= 3
+    pairs of (Z_d, R) arrays: the null vector of [1, Z, -R, -R Z]."""
+    Zs, Rs = np.asarray(Zs), np.asarray(Rs)
+    out = np.zeros((Zs.shape[1], 4), complex)
+    for i in range(Zs.shape[1]):
+        M = np.stack([np.ones(len(Zs)), Zs[:, i], -Rs[:, i], -Rs[:, i] * Zs[:, i]], axis=1)
+        out[i] = np.linalg.svd(M)[2][-1].conj()
+    return out
 
 
-def reflection_error(f, Z, Zf, z0):
-    """max |R_fit - R| of the plane-wave reflection coefficient over 50-1000 Hz."""
-    band = (f >= 50) & (f <= 1000)
-    R = (Z[band] - z0) / (Z[band] + z0)
-    Rf = (Zf[band] - z0) / (Zf[band] + z0)
-    return float(np.abs(Rf - R).max())
+def moebius_apply(coef, Z):
+    return (coef[:, 0] + coef[:, 1] * Z) / (coef[:, 2] + coef[:, 3] * Z)
 
 
-def training_set(grounds, dz=0.04, precision=64):
-    """Reflection data of the full air + porous solve for each ground, fitted residues and
-    fit errors max |R_fit - R|: (X [n x 4], A [n x 8], err [n])."""
-    lee = _lee()
-    d0 = column_deck(None, dz)
-    rho = d0.gamma * d0.Pav / d0.c ** 2
-    ref, dt = column_series(None, dz, precision=precision)
+@dataclass
+class Reflection:
+    """A ground's measured porous reflection and its impedance closure's Moebius map."""
+    g: Ground
+    f: np.ndarray        # fitted-band frequencies
+    R: np.ndarray        # porous column reflection
+    coef: np.ndarray     # Moebius map Z_d -> R of the closure
+    zinf: float
+    tau: float
+    dt: float
+
+    def predict(self, a):
+        """Reflection of the impedance closure with residues a."""
+        return moebius_apply(self.coef, discrete_impedance(self.f, a, self.zinf, self.tau, self.dt))
+
+    def error(self, a):
+        """max |R_a - R_porous| over F_BAND."""
+        b = (self.f >= F_BAND[0]) & (self.f <= F_BAND[1])
+        return float(np.abs(self.predict(a)[b] - self.R[b]).max())
+
+    def fit(self):
+        """Non-negative residues for the porous reflection; returns (a, error)."""
+        co, R, f = self.coef, self.R, self.f
+        Zreq = (co[:, 0] - co[:, 2] * R) / (co[:, 3] * R - co[:, 1])
+        tb = self.dt / self.tau
+        zi = np.exp(-2j * np.pi * f * self.dt)
+        A = self.zinf * tb / (1.0 - np.exp(-R_GRID[None, :] * tb) * zi[:, None])
+        w = np.abs((co[:, 1] * co[:, 2] - co[:, 0] * co[:, 3]) / (co[:, 2] + co[:, 3] * Zreq) ** 2)
+        w = np.where(np.isfinite(w), w, 0.0)
+        y = Zreq - self.zinf
+        M = np.vstack([(A * w[:, None]).real, (A * w[:, None]).imag])
+        rhs = np.concatenate([(y * w).real, (y * w).imag])
+        ok = np.isfinite(rhs)
+        a, _ = nnls(M[ok], rhs[ok])
+        return a, self.error(a)
+
+
+class Columns:
+    """Column runs on the GPU: the rigid reference once, then per ground the porous column
+    and four impedance-closure columns of known Z_d."""
+
+    def __init__(self, precision=64):
+        self.precision = precision
+        d = column_deck("rigid")
+        self.dt, self.c = d.dt, d.c
+        self.rho = d.gamma * d.Pav / d.c ** 2
+        self.ref = column_series(d, precision)
+
+    def _R(self, series):
+        f, R = reflection(series, self.ref, self.dt, self.c, t_end=T_END_M / self.c)
+        sel = (f >= F_FIT[0]) & (f <= F_FIT[1])
+        return f[sel], R[sel]
+
+    def porous(self, g: Ground):
+        return self._R(column_series(column_deck("porous", g), self.precision))
+
+    def impedance(self, g: Ground, a, zinf=None, tau=None):
+        return self._R(column_series(column_deck("impedance", g, a, zinf, tau), self.precision))
+
+    def measure(self, g: Ground) -> Reflection:
+        z0 = self.rho * self.c
+        zinf, tau = zinf_tau(g, self.rho, self.c)
+        Zs, Rs = [], []
+        a1 = np.zeros(K)
+        a1[3] = 1.0
+        for zc, a in ((z0, np.zeros(K)), (4 * z0, np.zeros(K)), (16 * z0, np.zeros(K)), (4 * z0, a1)):
+            f, R = self.impedance(g, a, zc, tau)
+            Zs.append(discrete_impedance(f, a, zc, tau, self.dt))
+            Rs.append(R)
+        f, R = self.porous(g)
+        return Reflection(g, f, R, moebius(Zs, Rs), zinf, tau, self.dt)
+
+
+def training_set(grounds, columns: Columns | None = None, progress=None):
+    """(X [n x 4], A [n x K] fitted residues, E [n] fit errors) from the column runs."""
+    cols = columns or Columns()
     X, A, E = [], [], []
-    for g in grounds:
-        s, _ = column_series(g, dz, precision=precision)
-        f, R = reflection(s, ref, dt, d0.c, 6.0, 3.0)
-        Z = rho * d0.c * (1 + R) / (1 - R)
-        zinf, tau = zinf_tau(g, rho, d0.c)
-        a, e = fit_residues(f, Z, zinf, tau, rho * d0.c)
-        X.append(normalise(g)); A.append(a.astype(np.float32)); E.append(e)
-    return np.array(X), np.array(A), np.array(E)
-
-
-def in_hull(g: Ground):
-    return g.sigma >= HULL_SIGMA_MIN
+    for n, g in enumerate(grounds):
+        a, e = cols.measure(g).fit()
+        X.append(normalise(g)); A.append(a); E.append(e)
+        if progress:
+            progress(n, g, e)
+    return np.array(X, np.float32), np.array(A), np.array(E)
 
 
 # ---------------------------------------------------------------------- network
@@ -255,10 +368,30 @@ def _setup(L):
     L._mlp_ready = True
 
 
-def ground_options(deck, g: Ground, residues):
-    """Set the ground impedance fields of `deck` (a Deck) from residues on R_GRID."""
-    rho = deck.gamma * deck.Pav / deck.c ** 2
-    zinf, tau = zinf_tau(g, rho, deck.c)
-    deck.ground_poles = [(float(max(a, 0.0)), float(r)) for a, r in zip(residues, R_GRID)]
-    deck.ground_zinf, deck.ground_tau = float(zinf), float(tau)
-    return deck
+
+class LearnedGround:
+    """The trained network with its decision rule.  Targets are sqrt(a_k) (a = t^2 >= 0 keeps
+    the boundary passive)."""
+
+    def __init__(self, seed=1, device=0):
+        self.net = Network(seed=seed, device=device)
+
+    def train(self, X, A, epochs=400, batch=32, lr=2e-3, seed=7):
+        return self.net.train(X, np.sqrt(np.maximum(A, 0.0)).astype(np.float32), batch=batch, epochs=epochs, lr=lr,
+                              seed=seed)
+
+    def residues(self, grounds):
+        t = self.net.predict(np.array([normalise(g) for g in grounds], np.float32))
+        return t.astype(np.float64) ** 2
+
+    def decide(self, g: Ground, columns: Columns, bound=HULL_FIT_R):
+        """(residues or None, judged error): None outside the hull or above the bound, where
+        the caller keeps the full porous solve."""
+        if not in_hull(g):
+            return None, float("nan")
+        a = self.residues([g])[0]
+        e = columns.measure(g).error(a)
+        return (a if e <= bound else None), e
+
+    def close(self):
+        self.net.close()

@@ -96,8 +96,10 @@ struct orc_sim {
      * index of airpore.cpp:572 (index_pressure, steps done) */
     double *src_ps; long src_n; int src_period;
     long index_pressure;
-    /* ground impedance (extension): South face, the East TDBC recursion with K poles */
+    /* ground impedance (extension): South face, the East TDBC recursion with K poles, closing
+     * a ground interface plane k = 0 that carries the porous equations of the ground's top */
     int ground, gK;
+    double g_gama, g_eff, g_ib;
     double gZinf, g_tb, g_inv, g_decay[8], g_gain[8];
     double *Gpsi;                      /* [i + nx*j][gK] */
 };
@@ -1037,37 +1039,52 @@ static void bc_v(ogrid *g, int face)
 
 /* East TDBC: calculation_2D::UpdateBC_velocity(EastBC,Z,tau), Cal_TDBC, Update_Psi
  * (calculation_2D.cpp:986-1028). */
-/* Ground impedance, pressure: the East face's quadratic extrapolation along -z. */
-static void bc_p_ground(ogrid *g)
+/* Ground impedance (extension).  The plane k = 0 is the ground's interface plane, as the
+ * porous ground's top plane is when the air reads it through the interface copy
+ * (calculation_2D.cpp:878-886): its pressure and tangential velocities follow the porous
+ * equations of the ground's top material (porous.cpp:77-143 with porous.cpp:44-50), and its
+ * normal velocity w (half a cell below the plane) follows the porous momentum equation with
+ * the pressure half a cell further down given by the impedance, Z(s) = Zinf (1 + sum_k
+ * a_k / (r_k + s tau)), through the TDBC recursion of calculation_2D.cpp:986-1028 with K
+ * poles.  The interior nodes of the plane are updated like stencil output, before the faces
+ * (N E W F B, there is no South face), which then treat the plane's edge nodes as on every
+ * other plane.  Flat grid: the plane's metric is 1/X_over_x, 1/Y_over_y and 1/Z_over_z of
+ * the plane k = 1. */
+static void bc_p_ground(orc_sim *s)
 {
-    for (int j = 0; j < g->ny; ++j)
-        for (int i = 0; i < g->nx; ++i)
-            g->pnn[IDX(g, i, j, 0)] = 15.0 / 8.0 * g->pnn[IDX(g, i, j, 1)] - 5.0 / 4.0 * g->pnn[IDX(g, i, j, 2)] +
-                                      3.0 / 8.0 * g->pnn[IDX(g, i, j, 3)];
+    ogrid *g = &s->air;
+    for (int j = 1; j < g->ny - 1; ++j)
+        for (int i = 1; i < g->nx - 1; ++i) {
+            size_t e = IDX(g, i, j, 0);
+            double ex = 1.0 / g->Xx[i], ey = 1.0 / g->Yy[j], ez = 1.0 / g->Zz[1];
+            double div = ex * g->dxi * (g->un[IDX(g, i + 1, j, 0)] - g->un[e]) +
+                         ey * g->dyi * (g->vn[IDX(g, i, j + 1, 0)] - g->vn[e]) +
+                         ez * g->dzi * (g->wn[IDX(g, i, j, 1)] - g->wn[e]);
+            g->pnn[e] = g->pn[e] + (-s->g_gama * div);
+        }
 }
 
-/* Ground impedance, velocity: UpdateBC_velocity's TDBC branch (calculation_2D.cpp:986-1028)
- * on the South face with the outward normal along -z: tangential components copied from
- * k = 1, the velocity into the ground vg = Cal_TDBC(Psi, p) with K poles from the face
- * pressure, then Update_Psi with vg.  The velocity that the interior divergence reads at
- * the low face is w at k = 1 (forward differences), so w = -vg is set on the planes k = 0
- * and k = 1 -- as the East TDBC sets the v its last interior divergence reads, with the face
- * pressure half a cell outward of it. */
 static void bc_v_ground(orc_sim *s)
 {
     ogrid *g = &s->air;
-    int nx = g->nx, ny = g->ny;
-    for (int i = 0; i < nx; ++i)
-        for (int j = 0; j < ny; ++j) {
-            size_t e = IDX(g, i, j, 0), e1 = IDX(g, i, j, 1);
-            g->unn[e] = g->unn[e1];
-            g->vnn[e] = g->vnn[e1];
+    int nx = g->nx;
+    double ib = s->g_ib, dt = g->dt, eff = s->g_eff;
+    double ez = 1.0 / g->Zz[1];
+    double cpl = 2.0 * dt * eff * ez * g->dzi;   /* dt/rho_e over half a cell */
+    double zi = s->gZinf / s->g_inv;             /* Zinf (1 + tb sum_k a_k) */
+    for (int j = 1; j < g->ny - 1; ++j)
+        for (int i = 1; i < nx - 1; ++i) {
+            size_t e = IDX(g, i, j, 0);
+            double ex = 1.0 / g->Xx[i], ey = 1.0 / g->Yy[j];
+            g->unn[e] = g->un[e] + (-1.0 * ib * dt * eff * (ex * (g->pnn[e] - g->pnn[IDX(g, i - 1, j, 0)]) * g->dxi) + g->un[e] * ib - g->un[e]);
+            g->vnn[e] = g->vn[e] + (-1.0 * ib * dt * eff * (ey * (g->pnn[e] - g->pnn[IDX(g, i, j - 1, 0)]) * g->dyi) + g->vn[e] * ib - g->vn[e]);
             double *Psi = &s->Gpsi[((size_t)i + (size_t)nx * j) * s->gK];
             double temp_psi = 0;
             for (int q = 0; q < s->gK; ++q) temp_psi += s->g_decay[q] * Psi[q];
-            double vg = s->g_inv * (g->pnn[e] / s->gZinf - temp_psi);
-            g->wnn[e] = -vg;
-            g->wnn[e1] = -vg;
+            double hist = s->gZinf * temp_psi;
+            double w = (g->wn[e] - cpl * (g->pnn[e] - hist)) * ib / (1.0 + ib * cpl * zi);
+            g->wnn[e] = w;
+            double vg = -w;
             for (int q = 0; q < s->gK; ++q) Psi[q] = s->g_decay[q] * Psi[q] + (s->g_gain[q] * vg) * s->g_tb;
         }
 }
@@ -1485,6 +1502,16 @@ orc_sim *orc_create_opt(const lee_deck *d, const lee_options *opt, int fill_nn_n
         if (opt->ground_poles > LEE_MAX_POLES || !(opt->ground_zinf > 0) || !(opt->ground_tau > 0)) {
             orc_destroy(s); FAIL("bad ground impedance");
         }
+        if (!(d->pore_q > 0) || !(d->pore_porosity > 0) || !(d->pore_resistivity >= 0) || opt->n_hills > 0) {
+            orc_destroy(s); FAIL("the ground impedance needs the top material (pore_q, pore_porosity, pore_resistivity) and a flat ground");
+        }
+        {
+            double eff_density = d->pore_q * d->pore_q / d->pore_porosity / s->Aav;
+            double Kp = 1.0 / d->gamma / d->Pav;
+            s->g_eff = 1.0 / eff_density;
+            s->g_gama = d->dt / Kp / d->pore_porosity;
+            s->g_ib = 1.0 / (d->pore_resistivity * d->dt * s->g_eff + 1.0);
+        }
         s->ground = 1; s->gK = opt->ground_poles; s->gZinf = opt->ground_zinf;
         s->g_tb = d->dt / opt->ground_tau;
         double at = 0;
@@ -1625,9 +1652,10 @@ static void one_step(orc_sim *s)
     orc_tick_gate(s);
     orc_air_dp(s, a->un, a->vn, a->wn, a->pn, a->pnn);
     add_interior(a, a->pnn, a->pn);
+    if (s->ground) bc_p_ground(s);
     bc_p(a, FN); bc_p(a, FE); bc_p(a, FW); bc_p(a, FF); bc_p(a, FB);
     if (s->ground) {
-        bc_p_ground(a);
+        /* no South face: the ground plane's interior nodes are stencil output */
     } else if (!s->south_porous) {
         bc_p(a, FS);
     } else {
@@ -1641,11 +1669,12 @@ static void one_step(orc_sim *s)
     /* --- velocity (airpore.cpp:324-364) --- */
     orc_air_duvw(s, a->un, a->vn, a->wn, a->pnn, a->unn, a->vnn, a->wnn);
     add_interior(a, a->unn, a->un); add_interior(a, a->vnn, a->vn); add_interior(a, a->wnn, a->wn);
+    if (s->ground) bc_v_ground(s);
     bc_v(a, FN);
     if (s->east_tdbc) bc_v_tdbc(s); else bc_v(a, FE);
     bc_v(a, FW); bc_v(a, FF); bc_v(a, FB);
     if (s->ground) {
-        bc_v_ground(s);
+        /* no South face */
     } else if (!s->south_porous) {
         bc_v(a, FS);
     } else {

@@ -1,15 +1,10 @@
 """The learned ground network on the GPU (lee_mlp: BF16 MFMA products, FP32 accumulation)
 against the CPU oracle of the same network: forward pass, loss and gradients, with row counts
 that are and are not multiples of the 16-row tile; then a short training run."""
-import os
-import sys
-
 import numpy as np
 import pytest
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-sys.path.insert(0, os.path.join(ROOT, "oracle"))
-import mlp_oracle as mo  # noqa: E402
+from oracle import mlp_oracle as mo
 
 pytestmark = pytest.mark.gpu
 

@@ -1,15 +1,24 @@
-"""Ground impedance (South-face TDBC with K poles) in the oracle.  Pinned by the recursion's
-invariant and by the physics of a plane wave at normal incidence: a matched impedance
-(Z = rho0 c, no poles' weight) absorbs it, a rigid face returns it, and an impedance with a
-known pole set reflects with the coefficient R = (Z - rho0 c)/(Z + rho0 c) at low frequency."""
+"""Ground impedance (South-face TDBC with K poles closing a ground interface plane) in the
+oracle.  Pinned by (1) an exact equivalence: with an air-like top material (q = 1, porosity 1,
+no flow resistance) and an infinite impedance, the interface plane is an air plane over a
+rigid wall half a cell below it -- the rigid floor of the reference one plane lower; (2) the
+closure's defining relations between consecutive steps (porous momentum over half a cell
+with the impedance's pressure, the recursion's history sum); (3) plane waves at normal
+incidence: a matched impedance (Z = rho0 c) absorbs, a resistive one reflects with
+R = (Z - rho0 c)/(Z + rho0 c)."""
 import numpy as np
 import pytest
 
 from oracle import oracle as O
 
 
+AIR_TOP = dict(pore_q=1.0, pore_porosity=1.0, pore_resistivity=0.0)
+
+
 def _column(lee, nz=160, **ground):
     d = lee.deck.box(4, 4, nz, 0.02, bc=(1, 1, 1, 1, 1, 1), src=(0.0, 0.0, -50.0))
+    if "ground_poles" in ground:
+        ground = {**AIR_TOP, **ground}
     for k, v in ground.items():
         setattr(d, k, v)
     return d
@@ -50,22 +59,67 @@ def test_resistive_ground_reflection_coefficient(lee):
     assert abs(r - 0.5) < 0.03
 
 
-def test_ground_recursion_invariant(lee):
-    """Cal_TDBC / Update_Psi with K poles keep -w + sum_k Psi_k = p / Z_inf at every face node
-    after every step (the relation the recursion is built on)."""
+def test_air_top_and_infinite_impedance_is_the_rigid_floor_one_plane_lower(lee):
+    """The same physical nodes on two grids: (a) the ground closure at z = 0 with an air-like
+    top material and Z = 1e30 (the normal velocity half a cell below the plane stays 0), and
+    (b) the reference's rigid South face on the grid extended by one plane below (its wall
+    sits half a cell under its plane k = 1, which is z = 0).  Receiver series agree to
+    rounding (the plane's metric is 1/X_over_x where the interior forms the Jacobian)."""
+    d = 0.05
+    kw = dict(bc=(1, 1, 1, 1, 1, 1), src=(0.05, 0.6, 0.3), gauss_width=0.2, time_domain=120)
+    a = lee.deck.box(16, 18, 14, d, ground_poles=[(0.0, 1.0)], ground_zinf=1e30, ground_tau=1e-3, **AIR_TOP, **kw)
+    b = lee.deck.box(16, 18, 15, d, z0=-d, **kw)
+    rec = [(0.05, 0.3, 0.05), (-0.1, 0.55, 0.1), (0.2, 0.7, 0.35)]
+    oa, ob = O.Oracle(a, receivers=rec), O.Oracle(b, receivers=rec)
+    oa.frame_setup(0); ob.frame_setup(0)
+    ra, rb = oa.step(120), ob.step(120)
+    assert np.abs(rb).max() > 1e-3
+    assert np.linalg.norm(ra - rb) <= 1e-10 * np.linalg.norm(rb)
+    pa, pb = oa.state()[3], ob.state()[3][1:]
+    assert np.linalg.norm(pa - pb) <= 1e-10 * np.linalg.norm(pb)
+
+
+def _ground_deck(lee):
     rng = np.random.default_rng(3)
     poles = [(float(a), float(r)) for a, r in zip(rng.uniform(0.05, 0.8, 8), rng.uniform(0.0, 2.0, 8))]
-    d = lee.deck.box(12, 14, 16, 0.05, bc=(1, 1, 1, 1, 1, 1), src=(0.0, 0.3, 0.2), gauss_width=0.2,
-                     ground_poles=poles, ground_zinf=700.0, ground_tau=2e-4)
+    return lee.deck.box(12, 14, 16, 0.05, bc=(1, 1, 1, 1, 1, 1), src=(0.0, 0.3, 0.2), gauss_width=0.2,
+                        ground_poles=poles, ground_zinf=700.0, ground_tau=2e-4, pore_q=1.6, pore_porosity=0.4,
+                        pore_resistivity=8.0e4), poles
+
+
+def test_ground_closure_relations(lee):
+    """Between consecutive steps, on the interior nodes of the plane k = 0:
+      w' = ib (w - cpl (p' - p_h)),  p_h = Zinf (-w' + sum_k Psi'_k)   (half-cell momentum,
+      the impedance's pressure through the recursion p_h / Zinf = v + sum_k Psi'_k),
+      Psi'_k = e^{-r_k tb} Psi_k + a_k tb v,  v = -w'
+      p' = p - gama (dx u + dy v + (w(k=1) - w) / dz)   (porous continuity of the top material)."""
+    d, poles = _ground_deck(lee)
     o = O.Oracle(d)
     o.frame_setup(0)
-    for _ in range(3):
-        o.step(7)
-        u, v, w, p = o.state()
-        psi = o.ground_psi(8)
-        lhs = -w[0] + psi.sum(axis=2)
-        assert np.allclose(lhs, p[0] / 700.0, rtol=1e-12, atol=1e-18)
-        assert np.abs(psi).max() > 0
+    o.step(9)
+    u0, v0, w0, p0 = o.state()
+    psi0 = o.ground_psi(8)
+    o.step(1)
+    u1, v1, w1, p1 = o.state()
+    psi1 = o.ground_psi(8)
+    rho0 = d.gamma * d.Pav / d.c ** 2
+    eff = d.pore_porosity / (d.pore_q ** 2 * rho0) * 1.0
+    ib = 1.0 / (d.pore_resistivity * d.dt * eff + 1.0)
+    cpl = 2.0 * d.dt * eff / d.dz
+    gama = d.dt * d.gamma * d.Pav / d.pore_porosity
+    sl = (slice(1, -1), slice(1, -1))
+    vg = -w1[0][sl]
+    ph = 700.0 * (vg + psi1[sl].sum(axis=2))
+    lhs = w1[0][sl]
+    rhs = ib * (w0[0][sl] - cpl * (p1[0][sl] - ph))
+    assert np.abs(lhs - rhs).max() <= 1e-9 * np.abs(lhs).max() and np.abs(lhs).max() > 0
+    tb = d.dt / d.ground_tau
+    for q, (a, r) in enumerate(poles):
+        ref = np.exp(-r * tb) * psi0[sl][..., q] + a * tb * vg
+        assert np.allclose(psi1[sl][..., q], ref, rtol=1e-12, atol=1e-22)
+    div = (u0[0][1:-1, 2:] - u0[0][1:-1, 1:-1]) / d.dx + (v0[0][2:, 1:-1] - v0[0][1:-1, 1:-1]) / d.dy + \
+          (w0[1][sl] - w0[0][sl]) / d.dz
+    assert np.allclose(p1[0][sl], p0[0][sl] - gama * div, rtol=1e-9, atol=1e-9 * np.abs(p1[0]).max())
 
 
 def test_ground_impedance_is_rejected_where_undefined(lee):
@@ -74,6 +128,15 @@ def test_ground_impedance_is_rejected_where_undefined(lee):
     with pytest.raises(ValueError):
         O.Oracle(d)
     d = lee.deck.box(12, 14, 16, 0.05, bc=(1, 1, 1, 1, 1, 1), ground_poles=[(-0.1, 1.0)], ground_zinf=700.0,
-                     ground_tau=2e-4)
+                     ground_tau=2e-4, **AIR_TOP)
+    with pytest.raises(ValueError):
+        O.Oracle(d)
+    d = lee.deck.box(12, 14, 16, 0.05, bc=(1, 1, 1, 1, 1, 1), ground_poles=[(0.1, 1.0)], ground_zinf=700.0,
+                     ground_tau=2e-4, pore_porosity=0.0)   # no top material
+    with pytest.raises(ValueError):
+        O.Oracle(d)
+    d = lee.deck.box(12, 14, 16, 0.05, bc=(1, 1, 1, 1, 1, 1), ground_poles=[(0.1, 1.0)], ground_zinf=700.0,
+                     ground_tau=2e-4, **AIR_TOP)
+    d.hills = [(0.0, 0.3, 0.1, 0.2)]   # the interface plane is flat
     with pytest.raises(ValueError):
         O.Oracle(d)

@@ -1,16 +1,11 @@
 """CPU oracle of the learned ground network: bfloat16 rounding against known answers and
 torch, the backward pass against finite differences, and the pole-fit helpers on inputs with
 known answers.  No GPU."""
-import os
-import sys
-
 import numpy as np
 import pytest
 import torch
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-sys.path.insert(0, os.path.join(ROOT, "oracle"))
-import mlp_oracle as mo  # noqa: E402
+from oracle import mlp_oracle as mo
 
 
 def test_bf16_rounding_known_answers():
@@ -80,16 +75,49 @@ def test_padded_parameters_have_zero_gradient():
     assert not L[-1][0][mo.N_OUT:, :].any() and not L[-1][1][mo.N_OUT:].any()  # padded outputs
 
 
-def test_pole_fit_recovers_known_residues(lee):
-    ground = lee.ground
+def test_moebius_map_and_residue_fit_recover_known_answers(lee):
+    """A reflection built from a known Moebius map of the discrete impedance of known
+    residues: the map is identified from four (Z_d, R) pairs, and the fit returns the
+    residues (the model class contains the answer, so the error vanishes)."""
+    G = lee.ground
     rng = np.random.default_rng(6)
-    a = rng.uniform(0, 2, ground.K) * (rng.uniform(size=ground.K) > 0.3)
-    zinf, tau, z0 = 900.0, 2e-4, 415.0
-    f = np.linspace(5, 3000, 2000)
-    Z = ground.impedance(f, a, zinf, tau)
-    got, err = ground.fit_residues(f, Z, zinf, tau, z0)
-    assert err < 1e-6
-    assert np.allclose(got, a, atol=1e-6)
+    f = np.linspace(G.F_FIT[0], G.F_FIT[1], 300)
+    dt, zinf, tau = 5.8e-5, 1200.0, 3e-4
+    co = rng.standard_normal((1, 4)) + 1j * rng.standard_normal((1, 4))
+    co = np.repeat(co, len(f), axis=0) * (1 + 0.1 * np.sin(f / 100.0))[:, None]
+    Zs, Rs = [], []
+    for zc, a in ((415.0, np.zeros(8)), (1660.0, np.zeros(8)), (6640.0, np.zeros(8)), (1660.0, np.eye(8)[3])):
+        Z = G.discrete_impedance(f, a, zc, tau, dt)
+        Zs.append(Z); Rs.append(G.moebius_apply(co, Z))
+    got = G.moebius(Zs, Rs)
+    Zt = G.discrete_impedance(f, rng.uniform(0, 1, 8), 900.0, tau, dt)
+    assert np.allclose(G.moebius_apply(got, Zt), G.moebius_apply(co, Zt), rtol=1e-9, atol=1e-12)
+    a_true = rng.uniform(0, 2, 8) * (rng.uniform(size=8) > 0.3)
+    R = G.moebius_apply(co, G.discrete_impedance(f, a_true, zinf, tau, dt))
+    ref = G.Reflection(G.Ground(1e5, 0.3, 2.0, 8), f, R, got, zinf, tau, dt)
+    a, err = ref.fit()
+    assert err < 1e-8 and np.allclose(a, a_true, atol=1e-6)
+
+
+def test_discrete_impedance_is_the_recursion(lee):
+    """Z_d of discrete_impedance against a time-domain run of Cal_TDBC / Update_Psi with a
+    sinusoidal velocity: p_n / Zinf = v_n (1 + tb sum a) + sum_k e^{-r_k tb} Psi_k."""
+    G = lee.ground
+    dt, zinf, tau = 5.8e-5, 900.0, 2e-4
+    a = np.array([0.3, 0.0, 0.5, 0.0, 1.2, 0.0, 0.2, 0.7])
+    tb = dt / tau
+    dec = np.exp(-G.R_GRID * tb)
+    f0 = 400.0
+    n = np.arange(50)
+    v = np.exp(2j * np.pi * f0 * n * dt)
+    # histories started in their periodic state (Psi = a tb v / (e^{i w dt} - e^{-r tb}))
+    psi = a * tb * v[0] / (np.exp(2j * np.pi * f0 * dt) - dec)
+    p = np.zeros(len(n), complex)
+    for m in range(len(n)):
+        p[m] = zinf * (v[m] * (1 + tb * a.sum()) + (dec * psi).sum())
+        psi = dec * psi + a * tb * v[m]
+    Zd = G.discrete_impedance(np.array([f0]), a, zinf, tau, dt)[0]
+    assert np.abs(p / v - Zd).max() < 1e-9 * abs(Zd)
 
 
 def test_reflection_of_a_scaled_echo(lee):
