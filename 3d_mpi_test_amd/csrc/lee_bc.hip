@@ -127,15 +127,20 @@ template <typename T> __device__ __forceinline__ double tdbc_v(const BCArgs<T> &
 // the FP64 histories) with the impedance's pressure half a cell below it,
 //   w' = ib (w - cpl (p' - p_h)),  p_h = Zinf (1 + tb sum a) v + Zinf sum_k e^{-r_k tb} Psi_k,
 // v = -w', solved for w'.
-template <typename T> __device__ __forceinline__ double ground_w(const BCArgs<T> &a, int i, int j)
+template <typename T> __device__ __forceinline__ double ground_w_of(const BCArgs<T> &a, int i, int j, double temp_psi)
 {
     const Grid<T> &g = a.air;
     const size_t e = gidx(g, i, j, 0);
-    const double *P = a.gpsi_old + ((size_t)i + (size_t)g.nx * j) * a.gK;
-    double temp_psi = 0;
-    for (int q = 0; q < a.gK; ++q) temp_psi += a.gdecay[q] * P[q];
     const double hist = a.gZinf * temp_psi;
     return ((double)g.wn[e] - a.gcpl * ((double)g.pnn[e] - hist)) * a.g_ib / (1.0 + a.g_ib * a.gcpl * a.gzi);
+}
+
+template <typename T> __device__ __forceinline__ double ground_w(const BCArgs<T> &a, int i, int j)
+{
+    const double *P = a.gpsi_old + ((size_t)i + (size_t)a.air.nx * j) * a.gK;
+    double temp_psi = 0;
+    for (int q = 0; q < a.gK; ++q) temp_psi += a.gdecay[q] * P[q];
+    return ground_w_of(a, i, j, temp_psi);
 }
 
 template <typename T> __device__ __forceinline__ T ground_vel(const BCArgs<T> &a, int c, int i, int j)
@@ -264,10 +269,33 @@ __global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
     if (t < a.nb_air) {
         int i, j, k;
         boundary_node(t, a.air.nx, a.air.ny, a.air.nz, i, j, k);
+        const size_t id = gidx(a.air, i, j, k);
+        if (a.ground && k == 0 && i > 0 && i < a.air.nx - 1 && j > 0 && j < a.air.ny - 1) {
+            // interior node of the ground plane: its velocities, then Update_Psi
+            // (calculation_2D.cpp:1012-1017) with the velocity into the ground, reading the
+            // histories once
+            const T u = ground_vel(a, 0, i, j), v = ground_vel(a, 1, i, j);
+            const size_t b = ((size_t)i + (size_t)a.air.nx * j) * a.gK;
+            double P[8];
+            double temp_psi = 0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (q < a.gK) { P[q] = a.gpsi_old[b + q]; temp_psi += a.gdecay[q] * P[q]; }
+            const double wd = ground_w_of(a, i, j, temp_psi);
+            const T w = (T)wd;
+            a.air.unn[id] = u; a.air.vnn[id] = v; a.air.wnn[id] = w;
+            q_boundary(a, 0, i, j, k, u);
+            q_boundary(a, 1, i, j, k, v);
+            q_boundary(a, 2, i, j, k, w);
+            const double vg = -wd;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (q < a.gK) a.gpsi_new[b + q] = a.gdecay[q] * P[q] + (a.ggain[q] * vg) * a.gtb;
+            return;
+        }
         const T u = v_air_walk<T, true>(a, 0, i, j, k, 6);
         const T v = v_air_walk<T, true>(a, 1, i, j, k, 6);
         const T w = v_air_walk<T, true>(a, 2, i, j, k, 6);
-        const size_t id = gidx(a.air, i, j, k);
         a.air.unn[id] = u; a.air.vnn[id] = v; a.air.wnn[id] = w;
         q_boundary(a, 0, i, j, k, u);
         q_boundary(a, 1, i, j, k, v);
@@ -278,12 +306,6 @@ __global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
             const size_t b = ((size_t)i + (size_t)a.air.nx * k) * 6;
 #pragma unroll
             for (int q = 0; q < 6; ++q) a.psi_new[b + q] = a.decay[q] * a.psi_old[b + q] + (a.gain[q] * vt) * a.tb;
-        }
-        if (a.ground && k == 0 && i > 0 && i < a.air.nx - 1 && j > 0 && j < a.air.ny - 1) {
-            // Update_Psi (calculation_2D.cpp:1012-1017) with the velocity into the ground
-            const double vg = -ground_w(a, i, j);
-            const size_t b = ((size_t)i + (size_t)a.air.nx * j) * a.gK;
-            for (int q = 0; q < a.gK; ++q) a.gpsi_new[b + q] = a.gdecay[q] * a.gpsi_old[b + q] + (a.ggain[q] * vg) * a.gtb;
         }
     } else if (t < a.nb_air + a.nb_pore) {
         int i, j, k;

@@ -109,14 +109,17 @@ __global__ void k_mse_back(const float *y, const float *t, float *dy, int rows, 
     }
     if (threadIdx.x == 0) loss[blockIdx.x] = part[0];
 }
-// db(n) = sum over rows of dz(row, n), one thread per column
-__global__ void k_colsum(const float *dz, float *db, int rows, int cols)
+// db(n) = sum over rows of dz(row, n): one 64-lane wave per column, lanes stride the rows,
+// then a butterfly over the wave
+__global__ __launch_bounds__(256) void k_colsum(const float *dz, float *db, int rows, int cols)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (n >= cols) return;
     float s = 0.f;
-    for (int i = 0; i < rows; ++i) s += dz[(long)i * cols + n];
-    db[n] = s;
+    for (int i = lane; i < rows; i += 64) s += dz[(long)i * cols + n];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) db[n] = s;
 }
 // Adam on FP32 parameters (m, v moments), step t (1-based)
 __global__ void k_adam(float *w, const float *g, float *m, float *v, long n, float lr, float b1, float b2, float eps,
@@ -242,17 +245,16 @@ void mlp_backward(lee_mlp *n, int R)
     const float *dcur = n->dy;
     for (int l = lee_mlp::L - 1; l >= 0; --l) {
         const int o = n->dout[l], i = n->din[l];
-        float *dzl = n->dz[l];
+        const float *dzl = n->dy;  // linear output layer: dZ = dY
         if (l < lee_mlp::L - 1) {
             const long cnt = (long)R * o;
-            hipLaunchKernelGGL(k_gelu_back, dim3(blocks_for(cnt)), dim3(256), 0, n->stream, n->z[l], dcur, dzl, cnt);
-        } else {
-            (void)hipMemcpyAsync(dzl, dcur, (size_t)R * o * sizeof(float), hipMemcpyDeviceToDevice, n->stream);
+            hipLaunchKernelGGL(k_gelu_back, dim3(blocks_for(cnt)), dim3(256), 0, n->stream, n->z[l], dcur, n->dz[l], cnt);
+            dzl = n->dz[l];
         }
         const float *prev = l ? n->a[l - 1] : n->x;
         // dW [o x i] = dZ^T [o x R] * prev [R x i]
         gemm(n, dzl, 1, o, prev, i, 1, n->g + n->woff[l], i, nullptr, o, i, R);
-        hipLaunchKernelGGL(k_colsum, dim3((o + 63) / 64), dim3(64), 0, n->stream, dzl, n->g + n->boff[l], R, o);
+        hipLaunchKernelGGL(k_colsum, dim3((o + 3) / 4), dim3(256), 0, n->stream, dzl, n->g + n->boff[l], R, o);
         if (l > 0) {
             // dA_prev [R x i] = dZ [R x o] * W [o x i]
             gemm(n, dzl, o, 1, n->w + n->woff[l], i, 1, n->da[l], i, nullptr, R, i, o);

@@ -476,6 +476,30 @@ def config_c3(time_domain: int = 3000, seed: int = 20261202, n: int = 64):
     return decks, recs
 
 
+def config_c4(ground, kind: str, residues=None, time_domain: int = 8000) -> Deck:
+    """Learned ground: 1024x512x160 air points at 0.04 m (the spacing is not specified; C1's
+    is used), flat ground, 20-point PML on the sides and top, no wind, no moving frame,
+    Gaussian pulse of half-width 3 dx at 0.5 m, 3 m into the domain, FB_p_v (the ground
+    impedance is discretised for it) at CFL 0.5, 8000 steps.  kind 'porous': the full solve
+    over the porous layer of `ground` (a ground.Ground, depth points + the two interface
+    planes); kind 'impedance': air only, the ground interface plane closed by the impedance
+    with `residues` on ground.R_GRID."""
+    from . import ground as G
+    d = 0.04
+    kw = dict(pml=20, time_domain=time_domain, gauss_width=2 * 3 * d, src=(0.0, 3.0, 0.5), rcv=(0.0, 6.0, 1.2),
+              fft_m=14, pml_y_time=0)
+    if kind == "porous":
+        dk = box(1024, 512, 160, d, bc=(2, 3, 2, 2, 2, 2), porous_cells=ground.depth + 2, **kw)
+        return G.porous_deck(dk, ground)
+    dk = box(1024, 512, 160, d, bc=(2, 1, 2, 2, 2, 2), **kw)
+    return G.impedance_deck(dk, ground, residues)
+
+
+def receivers_c4():
+    """Sixteen receivers 1-15 m from the source at 0.3 m and 1.2 m height (not specified)."""
+    return [(0.0, 3.0 + r, h) for r in (1.0, 2.0, 3.0, 4.5, 6.0, 8.0, 11.0, 15.0) for h in (0.3, 1.2)]
+
+
 def receivers_c2():
     """Thirty-two receivers on a line at 1.2 m along the propagation axis (not specified)."""
     return [(0.0, 4.0 + 0.35 * r, 1.2) for r in range(32)]

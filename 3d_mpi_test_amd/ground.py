@@ -41,6 +41,7 @@ HULL_SIGMA_MIN = 3.0e4
 HULL_FIT_R = 0.05
 F_BAND = (50.0, 1000.0)       # the accuracy band
 F_FIT = (20.0, 1500.0)        # the fitted band
+USE_GRAPH = True              # column and judge runs replay captured HIP graphs
 # C3 / C4 ground-parameter distribution
 SIGMA_RANGE = (1.0e4, 2.0e6)
 POROSITY_RANGE = (0.2, 0.5)
@@ -102,12 +103,14 @@ def porous_deck(deck, g: Ground):
     return deck
 
 
-def impedance_deck(deck, g: Ground, residues, zinf=None, tau=None):
-    """Set the ground impedance of `deck` (rigid South face) from residues on R_GRID: Z_inf and
-    tau of g's material unless given, the top material of g on the interface plane."""
+def impedance_deck(deck, g: Ground, residues, zinf=None, tau=None, poles=None):
+    """Set the ground impedance of `deck` (rigid South face) from residues on R_GRID (or on
+    `poles`): Z_inf and tau of g's material unless given, the top material of g on the
+    interface plane."""
     rho = deck.gamma * deck.Pav / deck.c ** 2
     zi, ta = zinf_tau(g, rho, deck.c)
-    deck.ground_poles = [(float(max(a, 0.0)), float(r)) for a, r in zip(residues, R_GRID)]
+    r = R_GRID if poles is None else poles
+    deck.ground_poles = [(float(max(a, 0.0)), float(rr)) for a, rr in zip(residues, r)]
     deck.ground_zinf = float(zi if zinf is None else zinf)
     deck.ground_tau = float(ta if tau is None else tau)
     set_top_material(deck, g)
@@ -148,7 +151,7 @@ def column_series(d, precision=64):
     """Receiver pressure at Z_REC of the plane pulse from Z_SRC in column deck d."""
     lee = _lee()
     rec = [(float(d.air_x[2]), float(d.air_y[2]), Z_REC)]
-    s = lee.Solver(d, precision=precision, receivers=rec, use_graph=True)
+    s = lee.Solver(d, precision=precision, receivers=rec, use_graph=USE_GRAPH)
     try:
         s.frame_setup(0)
         _plane_pulse(s, d, Z_SRC, 3 * DZ)
@@ -177,13 +180,21 @@ def reflection(series, ref_series, dt, c, z_src=Z_SRC, z_rec=Z_REC, t_end=None, 
         return f, ratio(series) / ratio(ref_series)
 
 
-def discrete_impedance(f, a, zinf, tau, dt):
+def discrete_impedance(f, a, zinf, tau, dt, poles=None):
     """Z_d(f) of the recursion: Zinf (1 + sum_k a_k tb / (1 - e^{-r_k tb} z^-1)), z = e^{i w dt},
-    tb = dt/tau (the z-transform of Cal_TDBC / Update_Psi, calculation_2D.cpp:1012-1028)."""
+    tb = dt/tau (the z-transform of Cal_TDBC / Update_Psi, calculation_2D.cpp:1012-1028);
+    poles r_k default to R_GRID."""
+    r = R_GRID if poles is None else np.asarray(poles, float)
     tb = dt / tau
     zi = np.exp(-2j * np.pi * np.asarray(f) * dt)
-    B = tb / (1.0 - np.exp(-R_GRID[None, :] * tb) * zi[:, None])
+    B = tb / (1.0 - np.exp(-r[None, :] * tb) * zi[:, None])
     return zinf * (1.0 + B @ np.asarray(a, float))
+
+
+# the reference's six-pole TDBC coefficients (calculation_2D.h:73-75): its fit of the
+# semi-infinite layer's sqrt(1 + 1/(s tau)), the same for every ground
+SIX_POLE_A = (8.226223820e-01, 3.732304064e-01, 9.106232372e-02, 4.819445335e-02, 4.225250937e-01, 1.837970986e-01)
+SIX_POLE_R = (8.681310138e-01, 1.953722814e-01, 1.108067950e-02, 2.566498621e-03, 1.457297589e-04, 4.648583789e-02)
 
 
 def moebius(Zs, Rs):
@@ -216,10 +227,11 @@ class Reflection:
         """Reflection of the impedance closure with residues a."""
         return moebius_apply(self.coef, discrete_impedance(self.f, a, self.zinf, self.tau, self.dt))
 
-    def error(self, a):
-        """max |R_a - R_porous| over F_BAND."""
+    def error(self, a, poles=None):
+        """max |R_a - R_porous| over F_BAND (residues a on R_GRID, or on `poles`)."""
         b = (self.f >= F_BAND[0]) & (self.f <= F_BAND[1])
-        return float(np.abs(self.predict(a)[b] - self.R[b]).max())
+        Rp = moebius_apply(self.coef, discrete_impedance(self.f, a, self.zinf, self.tau, self.dt, poles))
+        return float(np.abs(Rp[b] - self.R[b]).max())
 
     def fit(self):
         """Non-negative residues for the porous reflection; returns (a, error)."""
@@ -369,6 +381,64 @@ def _setup(L):
 
 
 
+def il_errors(a, b, dt, fmin=F_BAND[0], fmax=F_BAND[1]):
+    """Insertion-loss differences of receiver series b against a (columns = receivers, only
+    receivers carrying signal), in dB over [fmin, fmax]: (largest narrow-band difference, bins
+    60 dB below the band peak skipped; largest one-third-octave band-level difference, centre
+    frequencies 1000 * 2^(n/3) in the band, edges at 2^(+-1/6))."""
+    energy = (a ** 2).sum(axis=0)
+    use = np.nonzero(energy > 1e-6 * energy.max())[0]
+    n = 1 << int(np.ceil(np.log2(a.shape[0])))
+    f = np.fft.rfftfreq(n, dt)
+    band = (f >= fmin) & (f <= fmax)
+    fc = [1000.0 * 2 ** (m / 3.0) for m in range(-30, 1)]
+    fc = [x for x in fc if fmin * 0.999 <= x <= fmax * 1.001]
+    narrow = third = 0.0
+    for r in use:
+        A = np.abs(np.fft.rfft(a[:, r], n))
+        B = np.abs(np.fft.rfft(b[:, r], n))
+        ok = band & (A > 1e-3 * A[band].max())
+        if ok.any():
+            narrow = max(narrow, float(np.abs(20 * np.log10(B[ok] / A[ok])).max()))
+        for c in fc:
+            sel = (f >= c * 2 ** (-1 / 6)) & (f < c * 2 ** (1 / 6))
+            if sel.any():
+                third = max(third, float(abs(10 * np.log10((B[sel] ** 2).sum() / (A[sel] ** 2).sum()))))
+    return narrow, third
+
+
+# ---------------------------------------------------------------------- 3-D judge
+JUDGE_REC = [(0.0, y, h) for y in (2.6, 3.4) for h in (0.1, 0.6, 1.2)]
+
+
+def judge_deck(g: Ground, kind: str, residues=None, steps=1024, poles=None):
+    """A 120 x 120 x 60 box at 0.04 m (20-point PML on the sides and top), Gaussian pulse at
+    0.6 m, over g's porous layer ('porous') or the impedance closure ('impedance')."""
+    lee = _lee()
+    kw = dict(pml=20, src=(0.0, 1.6, 0.6), gauss_width=0.24, time_domain=steps)
+    if kind == "porous":
+        d = lee.deck.box(120, 120, 60, DZ, bc=(2, 3, 2, 2, 2, 2), porous_cells=g.depth + 2, **kw)
+        return porous_deck(d, g)
+    d = lee.deck.box(120, 120, 60, DZ, bc=(2, 1, 2, 2, 2, 2), **kw)
+    return impedance_deck(d, g, residues, poles=poles)
+
+
+def judge_3d(g: Ground, residues, precision=64, poles=None):
+    """IL differences (narrow-band, one-third octave) of the impedance closure with
+    `residues` (on R_GRID or `poles`) against the full porous solve in the judge box."""
+    lee = _lee()
+    out = []
+    for kind in ("porous", "impedance"):
+        d = judge_deck(g, kind, residues, poles=poles)
+        s = lee.Solver(d, precision=precision, receivers=JUDGE_REC, use_graph=USE_GRAPH)
+        try:
+            s.frame_setup(0)
+            out.append(s.step(d.time_domain))
+        finally:
+            s.close()
+    return il_errors(out[0], out[1], d.dt)
+
+
 class LearnedGround:
     """The trained network with its decision rule.  Targets are sqrt(a_k) (a = t^2 >= 0 keeps
     the boundary passive)."""
@@ -384,14 +454,19 @@ class LearnedGround:
         t = self.net.predict(np.array([normalise(g) for g in grounds], np.float32))
         return t.astype(np.float64) ** 2
 
-    def decide(self, g: Ground, columns: Columns, bound=HULL_FIT_R):
-        """(residues or None, judged error): None outside the hull or above the bound, where
-        the caller keeps the full porous solve."""
-        if not in_hull(g):
-            return None, float("nan")
+    def decide(self, g: Ground, columns: Columns, bound_r=HULL_FIT_R, bound_db=0.2):
+        """(residues or None, report): the full porous solve judges the predicted residues --
+        the column reflection error and the one-third-octave IL difference in the judge box;
+        None (keep the full porous solve) outside the hull or above either bound."""
+        rep = {"in_hull": in_hull(g)}
+        if not rep["in_hull"]:
+            return None, rep
         a = self.residues([g])[0]
-        e = columns.measure(g).error(a)
-        return (a if e <= bound else None), e
+        rep["reflection_error"] = columns.measure(g).error(a)
+        if rep["reflection_error"] > bound_r:
+            return None, rep
+        rep["il_narrow_dB"], rep["il_third_octave_dB"] = judge_3d(g, a)
+        return (a if rep["il_third_octave_dB"] <= bound_db else None), rep
 
     def close(self):
         self.net.close()

@@ -23,11 +23,15 @@ host core over a bounded sample (the first step(s) of the same workload).
 `--workload c3` runs the batched ensemble: 64 seeded uneven domains (deck.config_c3) advanced
 together by the ensemble runner, FP32 (and FP64 for the accuracy figures), 3000 steps.
 
+`--workload c4` runs the learned ground: the training pipeline on the GPU, the judge, then the
+1024x512x160 FB_p_v domain FP32 over the full porous layer and air-only with the learned
+impedance (speed-up and IL difference between the two).
+
 `--workload c2` runs the precision study instead: 768x384x192 at 0.04 m over a 24-point
 porous layer and a seeded random terrain of 6 Gaussian hills, RK4, 2000 steps, FP32 against
 FP64 (deck.config_c2).
 
-    python bench.py [--gpus 1] [--workload c1|c2|c3] [--steps K] [--warmup W] [--no-cpu] [--no-fb]
+    python bench.py [--gpus 1] [--workload c1|c2|c3|c4] [--steps K] [--warmup W] [--no-cpu] [--no-fb]
                     [--traffic-json F]
 """
 from __future__ import annotations
@@ -197,6 +201,85 @@ def run_c3(lee, a):
     print(json.dumps(line))
 
 
+def run_c4(lee, a):
+    """C4 learned ground.  The pipeline end to end on the GPU: column runs of the full air +
+    porous solve for 128 seeded grounds inside the hull (reflection data and the closure's
+    calibration), residue fits, the network trained by the BF16-MFMA kernels, then the ground
+    drawn for C4 (the first draw of the C3 distribution with seed 20261202) judged by the full
+    porous solve (column reflection error and the one-third-octave IL in the judge box).  The
+    timed runs are the 1024x512x160 domain over the full porous layer and air-only with the
+    learned impedance, FP32, FB_p_v; `value` is the learned run's grid-point updates per second
+    of the air domain and `learned_speedup` the porous run's step time over the learned run's."""
+    G = lee.ground
+    if a.eager:
+        G.USE_GRAPH = False
+    t0 = time.perf_counter()
+    cols = G.Columns()
+    train = [g for g in G.sample_grounds(512, 20261203) if G.in_hull(g)][:128]
+    X, A, E = G.training_set(train, cols)
+    t_data = time.perf_counter() - t0
+    keep = E <= G.HULL_FIT_R
+    lg = G.LearnedGround(seed=1)
+    t1 = time.perf_counter()
+    mse = lg.train(X[keep], A[keep], epochs=600, batch=32, lr=2e-3)
+    t_train = time.perf_counter() - t1
+    print(f"[bench] C4 training data: {len(train)} grounds in {t_data:.1f} s (fit error median {np.median(E):.4f}, "
+          f"max {E.max():.4f}); network trained in {t_train:.2f} s (mse {mse:.2e})", file=sys.stderr, flush=True)
+    g = G.sample_grounds(1, 20261202)[0]
+    res, rep = lg.decide(g, cols)
+    print(f"[bench] C4 ground {g}: judge {rep}", file=sys.stderr, flush=True)
+    a_pred = lg.residues([g])[0]
+    lg.close()
+    total = a.warmup + a.steps
+    rec = lee.deck.receivers_c4()
+    runs = {}
+    for kind in ("porous", "impedance"):
+        deck = lee.deck.config_c4(g, kind, a_pred, time_domain=total)
+        print(f"[bench] C4 {kind} fp32: {a.warmup} warm-up + {a.steps} timed steps", file=sys.stderr, flush=True)
+        runs[kind] = run_gpu(lee, deck, rec, 32, a.steps, a.warmup)
+        r = runs[kind]
+        print(f"[bench] C4 {kind}: {r['points'] * a.steps / r['wall'] / 1e9:.2f} G pt-updates/s "
+              f"({1e3 * r['wall'] / a.steps:.3f} ms/step)", file=sys.stderr, flush=True)
+    lp, li = runs["porous"], runs["impedance"]
+    narrow, third = G.il_errors(lp["series"], li["series"], deck.dt)
+    rl2 = float(np.linalg.norm(li["series"] - lp["series"]) / np.linalg.norm(lp["series"]))
+    speedup = lp["wall"] / li["wall"]
+    dom = max(li["kernels"], key=lambda k: k["total_ms"])
+    avg_s = dom["total_ms"] / dom["launches"] / 1e3
+    achieved = dom["bytes_per_launch"] / avg_s / 1e9
+    air = int(np.prod(deck.shape))
+    line = {
+        "metric": "grid-pt updates/s + %HBM BW, FP32 vs FP64; learned-ground speedup @<=0.2 dB",
+        "value": round(li["points"] * a.steps / li["wall"], 1),
+        "unit": "grid-pt updates/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(1e3 * li["wall"] / a.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "C4: 1024x512x160 air at 0.04 m, flat ground, 20-pt PML sides+top, FB_p_v CFL 0.5, "
+                               "ground drawn from the C3 distribution (seed 20261202); learned 8-pole impedance on "
+                               "the ground interface plane vs the full porous layer",
+                   "air_points": air, "ground": vars(g), "porous_points": lp["points"] - air,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom["name"],
+                     "bytes_per_launch": dom["bytes_per_launch"], "avg_launch_ms": round(avg_s * 1e3, 4)},
+        "cpu_baseline": None,
+        "learned_speedup": round(speedup, 3),
+        "porous_pt_updates_per_s": lp["points"] * a.steps / lp["wall"],
+        "porous_ms_per_step": 1e3 * lp["wall"] / a.steps,
+        "accuracy": {"learned_vs_porous_rel_l2": rl2, "max_dIL_dB_50_1000Hz": round(narrow, 4),
+                     "max_dIL_third_octave_dB_50_1000Hz": round(third, 4), "steps": total},
+        "judge": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in rep.items()},
+        "learned_used": res is not None,
+        "training": {"grounds": len(train), "data_s": round(t_data, 2), "train_s": round(t_train, 3),
+                     "fit_error_median": float(np.median(E)), "fit_error_max": float(E.max()), "mse": mse},
+        "impedance_kernels": kernel_table(li["kernels"]),
+        "porous_kernels": kernel_table(lp["kernels"]),
+    }
+    print(f"[bench] C4 learned vs porous: speedup {speedup:.3f}, rel-L2 {rl2:.3e}, max dIL {narrow:.3f} dB "
+          f"(one-third octave {third:.3f} dB)", file=sys.stderr, flush=True)
+    print(json.dumps(line))
+
+
 def kernel_table(kernels):
     return [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in kk.items()} for kk in kernels]
 
@@ -204,12 +287,13 @@ def kernel_table(kernels):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("c1", "c2", "c3"), default="c1")
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (C1: 10000, C2: 2000, C3: 3000)")
+    ap.add_argument("--workload", choices=("c1", "c2", "c3", "c4"), default="c1")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (C1: 10000, C2: 2000, C3: 3000, C4: 8000)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fb", action="store_true", help="skip the reference-scheme (FB_p_v, flat) lines")
     ap.add_argument("--precisions", default="32,64")
+    ap.add_argument("--eager", action="store_true", help="C4: launch the column and judge runs eagerly (no graphs)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "rk4_air_c1_traffic.json"),
                     help="JSON {kernel: HBM bytes per launch} from rocprofv3 FETCH_SIZE/WRITE_SIZE runs")
     a = ap.parse_args()
@@ -219,9 +303,12 @@ def main():
         sys.exit(2)
     lee = importlib.import_module("3d_mpi_test_amd")
     if a.steps is None:
-        a.steps = {"c1": 10000, "c2": 2000, "c3": 3000}[a.workload]
+        a.steps = {"c1": 10000, "c2": 2000, "c3": 3000, "c4": 8000}[a.workload]
     if a.workload == "c3":
         run_c3(lee, a)
+        return
+    if a.workload == "c4":
+        run_c4(lee, a)
         return
     total = a.warmup + a.steps
     if a.workload == "c1":

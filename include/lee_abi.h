@@ -102,11 +102,16 @@ typedef struct lee_options {
     double hills[32];       /* per hill: x0, y0 [m, frame-0 coordinates], height, sigma [m] */
     double wind_ustar;      /* velocity_method 7: V(z) = wind_ustar/0.41 ln(z/wind_z0) along +y, */
     double wind_z0;         /*   z = height above the local ground, V = 0 for z <= wind_z0 */
-    /* ground impedance: the South face as a time-domain impedance boundary instead of a
-     * porous layer, Z(s) = ground_zinf (1 + sum_k ground_a[k] / (ground_r[k] + s ground_tau)),
-     * discretised as the reference's East TDBC (calculation_2D.cpp:986-1028) with the outward
-     * normal along -z: the velocity into the ground is -w at k = 0, the face pressure is the
-     * quadratic extrapolation of the East face.  FB_p_v, deck South face rigid, fixed frame. */
+    /* ground impedance (no counterpart in the reference): the air's plane k = 0 becomes the
+     * ground's interface plane -- its pressure and tangential velocities follow the porous
+     * equations of the deck's top material (pore_q, pore_porosity, pore_resistivity;
+     * porous.cpp:44-50, 77-143), as the porous layer's top plane does through the interface
+     * copy (calculation_2D.cpp:878-886) -- and the layer below it is the impedance
+     * Z(s) = ground_zinf (1 + sum_k ground_a[k] / (ground_r[k] + s ground_tau)) through the
+     * East TDBC's recursion (calculation_2D.cpp:986-1028) with K poles: the normal velocity
+     * half a cell below the plane follows the porous momentum equation with the impedance's
+     * pressure half a cell further down.  FB_p_v, deck South face rigid, fixed frame, flat
+     * grid, no obstacle; residues and poles >= 0 (passive). */
     int ground_poles;       /* 0: off, else 1..LEE_MAX_POLES pole/residue pairs */
     double ground_a[8], ground_r[8];
     double ground_zinf, ground_tau;
@@ -285,7 +290,8 @@ const char *lee_ensemble_error(const lee_ensemble *e);
 
 /* Learned ground model (no counterpart in the reference): an MLP 4 -> 64 -> 64 -> 64 -> 8 with
  * GELU, from normalised ground parameters (flow resistivity, porosity, tortuosity, layer
- * depth) to the residues of the ground impedance's eight poles (lee_options.ground_*).
+ * depth) to the residues of the ground impedance's eight poles (lee_options.ground_*; the
+ * Python driver trains it on the square roots of the residues).
  * Trained and evaluated on the GPU: BF16 MFMA products with FP32 accumulation, FP32 master
  * weights, mean-squared-error loss, Adam.  Parameters are one flat FP32 array, layer by
  * layer W[out][in] then b[out] over padded shapes 16->64, 64->64, 64->64, 64->16 (padding

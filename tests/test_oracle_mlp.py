@@ -134,3 +134,13 @@ def test_reflection_of_a_scaled_echo(lee):
     f, R = ground.reflection(inc - 0.4 * echo, inc + echo, dt, c, z_src, z_rec)
     band = (f > 50) & (f < 1000)
     assert np.allclose(R[band], -0.4, atol=1e-9)
+
+
+def test_il_errors(lee):
+    G = lee.ground
+    rng = np.random.default_rng(8)
+    a = rng.standard_normal((4096, 3)) * np.hanning(4096)[:, None]
+    n, t = G.il_errors(a, a, 1e-4)
+    assert n == 0.0 and t == 0.0
+    n, t = G.il_errors(a, 1.1 * a, 1e-4)
+    assert n == pytest.approx(20 * np.log10(1.1), rel=1e-9) and t == pytest.approx(20 * np.log10(1.1), rel=1e-9)
