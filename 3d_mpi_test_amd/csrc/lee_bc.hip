@@ -321,13 +321,13 @@ __global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
 // RK4 stage: auxiliaries of the boundary nodes (dQ/dt = f of the stage state, with the
 // register update of k_rk_air) and the face values of the destination state.
 template <typename T>
-__device__ __forceinline__ T rk_update_q(int st, T xn, T k, T *acc, size_t q)
+__device__ __forceinline__ T rk_q_regs(int st, T xn, T k, T acc_in, T &acc_out)
 {
     const T b = (st == 0 || st == 3) ? T(1.0 / 6.0) : T(1.0 / 3.0);
     const T al = st == 0 ? T(0.5) : st == 1 ? T(0.5) : T(1.0);
-    if (st == 0) { acc[q] = xn + b * k; return xn + al * k; }
-    if (st < 3) { acc[q] = acc[q] + b * k; return xn + al * k; }
-    return acc[q] + b * k;
+    if (st == 0) { acc_out = xn + b * k; return xn + al * k; }
+    if (st < 3) { acc_out = acc_in + b * k; return xn + al * k; }
+    return acc_in + b * k;
 }
 
 template <typename T>
@@ -339,6 +339,15 @@ __device__ __forceinline__ void rk_bnd_node(const RKBndArgs<T> &r, long long t)
         boundary_node(t, a.air.nx, a.air.ny, a.air.nz, i, j, k);
         const Grid<T> &g = a.air;
         const size_t id = gidx(g, i, j, k);
+        // reads first (the face walks, the node's stage state), then the stores
+        const T p = p_air_walk<T, true>(a, i, j, k, 6);
+        const T u = v_air_walk<T, true>(a, 0, i, j, k, 6);
+        const T v = v_air_walk<T, true>(a, 1, i, j, k, 6);
+        const T w = v_air_walk<T, true>(a, 2, i, j, k, 6);
+        T sv[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) sv[f] = r.src[f][id];
+        g.pnn[id] = p; g.unn[id] = u; g.vnn[id] = v; g.wnn[id] = w;
         for (int s = 0; s < 6; ++s) {
             const int n = a.qn[s];
             if (!n || (s == 2 && !a.y1on)) continue;
@@ -347,14 +356,19 @@ __device__ __forceinline__ void rk_bnd_node(const RKBndArgs<T> &r, long long t)
             if (c < lo || c >= lo + n) continue;
             const int A = ax == 0 ? n : g.nx, B = ax == 1 ? n : g.ny;
             const size_t q = (size_t)(ax == 0 ? i - lo : i) + (size_t)A * ((size_t)(ax == 1 ? j - lo : j) + (size_t)B * (ax == 2 ? k - lo : k));
-            for (int f = 0; f < 4; ++f)
-                r.qdst[s][f][q] = rk_update_q<T>(r.stage, r.qxn[s][f][q], a.dt * r.src[f][id], r.qacc[s][f], q);
+            T qx[4], qa[4];
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                qx[f] = r.qxn[s][f][q];
+                qa[f] = r.stage != 0 ? r.qacc[s][f][q] : T(0);
+            }
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                T ao;
+                r.qdst[s][f][q] = rk_q_regs<T>(r.stage, qx[f], a.dt * sv[f], qa[f], ao);
+                if (r.stage < 3) r.qacc[s][f][q] = ao;
+            }
         }
-        const T p = p_air_walk<T, true>(a, i, j, k, 6);
-        const T u = v_air_walk<T, true>(a, 0, i, j, k, 6);
-        const T v = v_air_walk<T, true>(a, 1, i, j, k, 6);
-        const T w = v_air_walk<T, true>(a, 2, i, j, k, 6);
-        g.pnn[id] = p; g.unn[id] = u; g.vnn[id] = v; g.wnn[id] = w;
     } else if (t < a.nb_air + a.nb_pore) {
         int i, j, k;
         boundary_node(t - a.nb_air, a.pore.nx, a.pore.ny, a.pore.nz, i, j, k);

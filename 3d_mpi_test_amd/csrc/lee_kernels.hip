@@ -731,36 +731,36 @@ template <typename T> __device__ __forceinline__ T rk_b(int st)
     return (st == 0 || st == 3) ? T(1.0 / 6.0) : T(1.0 / 3.0);
 }
 
+// The register update on values: the caller reads the registers, so that a point's loads
+// can all be issued before its first store (the field pointers may alias as far as the
+// compiler knows, so a load after a store waits for it).
 template <typename T, bool COMP>
-__device__ __forceinline__ T rk_update(int st, T xn, T k, T *acc, T *lo, T *err, size_t id)
+__device__ __forceinline__ T rk_regs(int st, T xn, T k, T acc_in, T err_in, T &acc_out, T &err_out)
 {
     if constexpr (!COMP) {
+        (void)err_in; (void)err_out;
         if (st == 0) {
-            acc[id] = xn + rk_b<T>(0) * k;
+            acc_out = xn + rk_b<T>(0) * k;
             return xn + rk_a<T>(0) * k;
         }
         if (st < 3) {
-            acc[id] = acc[id] + rk_b<T>(st) * k;
+            acc_out = acc_in + rk_b<T>(st) * k;
             return xn + rk_a<T>(st) * k;
         }
-        return acc[id] + rk_b<T>(3) * k;
+        return acc_in + rk_b<T>(3) * k;
     } else {
-        // acc holds the step's increment sum b1 k1 + ... (small next to the state, so its FP32
-        // rounding is small next to an ulp of the state); the last stage adds it and the
-        // carried error of the state with an exact TwoSum and keeps the new rounding error
-        (void)lo;
         if (st == 0) {
-            acc[id] = rk_b<T>(0) * k;
+            acc_out = rk_b<T>(0) * k;
             return xn + rk_a<T>(0) * k;
         }
         if (st < 3) {
-            acc[id] = acc[id] + rk_b<T>(st) * k;
+            acc_out = acc_in + rk_b<T>(st) * k;
             return xn + rk_a<T>(st) * k;
         }
-        const T y = (acc[id] + rk_b<T>(3) * k) + err[id];
+        const T y = (acc_in + rk_b<T>(3) * k) + err_in;
         const T x = xn + y;
         const T bb = x - xn;
-        err[id] = (xn - (x - bb)) + (y - bb);  // TwoSum: xn + y = x + err exactly
+        err_out = (xn - (x - bb)) + (y - bb);  // TwoSum: xn + y = x + err exactly
         return x;
     }
 }
@@ -773,37 +773,39 @@ __device__ __forceinline__ T rk_update(int st, T xn, T k, T *acc, T *lo, T *err,
 //   stage 1: acc += (E2/3) k,         next = E2 x_n + k/2
 //   stage 2: acc += (E2/3) k,         next = E x_n + E2 k
 //   stage 3: x_{n+1} = acc + k/6
-// COMP carries the low part and the state error as rk_update does (the error scaled by E).
+// COMP carries the low part and the state error as rk_regs does (the error scaled by E).
 template <typename T, bool COMP>
-__device__ __forceinline__ T rk_update_if(int st, T xn, T k, T *acc, T *lo, T *err, size_t id, const T *cf)
+__device__ __forceinline__ T rk_if_regs(int st, T xn, T k, T acc_in, T lo_in, T err_in, const T *cf, T &acc_out,
+                                        T &lo_out, T &err_out)
 {
     const T E = cf[0], E2 = cf[1];
     const T w = st == 0 ? cf[3] : st < 3 ? cf[4] : T(1.0 / 6.0);
     const T cn = st == 2 ? E : E2, an = st == 0 ? cf[2] : st == 1 ? T(0.5) : E2;
     if constexpr (!COMP) {
+        (void)lo_in; (void)err_in; (void)lo_out; (void)err_out;
         if (st == 0) {
-            acc[id] = E * xn + w * k;
+            acc_out = E * xn + w * k;
             return cn * xn + an * k;
         }
         if (st < 3) {
-            acc[id] = acc[id] + w * k;
+            acc_out = acc_in + w * k;
             return cn * xn + an * k;
         }
-        return acc[id] + w * k;
+        return acc_in + w * k;
     } else {
         const T y = w * k;
-        const T a0 = st == 0 ? E * xn : acc[id];
+        const T a0 = st == 0 ? E * xn : acc_in;
         const T sm = a0 + y;
         const T bb = sm - a0;
         const T e = (a0 - (sm - bb)) + (y - bb);
-        const T l = (st == 0 ? E * err[id] : lo[id]) + e;
+        const T l = (st == 0 ? E * err_in : lo_in) + e;
         if (st < 3) {
-            acc[id] = sm;
-            lo[id] = l;
+            acc_out = sm;
+            lo_out = l;
             return cn * xn + an * k;
         }
         const T x = sm + l;
-        err[id] = l - (x - sm);
+        err_out = l - (x - sm);
         return x;
     }
 }
@@ -818,22 +820,45 @@ __device__ __forceinline__ void rk_air_point(const RKArgs<T> &r, int i, int j, i
     kf[3] = air_dp<T, WM, TERRAIN, true>(a, i, j, k);
     air_duvw<T, WM, TERRAIN, true>(a, i, j, k, kf[0], kf[1], kf[2]);
     const T xc[4] = {a.g.un[id], a.g.vn[id], a.g.wn[id], a.g.pn[id]};  // stage state at the point
+    // every register read of the point first, then the arithmetic, then the stores
+    T xn[4], ac[4], er[4];
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         // level n: the stage state itself at stage 0; not needed by the last FP64 stage
-        const T xn = st == 0 ? xc[f] : (!COMP && st == 3) ? T(0) : r.xn[f][id];
-        r.dst[f][id] = rk_update<T, COMP>(st, xn, kf[f], r.acc[f], r.lo[f], r.err[f], id);
+        xn[f] = st == 0 ? xc[f] : (!COMP && st == 3) ? T(0) : r.xn[f][id];
+        ac[f] = st == 0 ? T(0) : r.acc[f][id];
+        er[f] = (COMP && st == 3) ? r.err[f][id] : T(0);
     }
-    // auxiliaries of the point's own slabs: dQ/dt = f (air.cpp:2323-2439 as an ODE)
+    T d[4], ao[4], eo[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) d[f] = rk_regs<T, COMP>(st, xn[f], kf[f], ac[f], er[f], ao[f], eo[f]);
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        r.dst[f][id] = d[f];
+        if (st < 3) r.acc[f][id] = ao[f];
+        if (COMP && st == 3) r.err[f][id] = eo[f];
+    }
+    // auxiliaries of the point's own slabs: dQ/dt = f (air.cpp:2323-2439 as an ODE), each
+    // slab's reads before its stores
     const OwnQ<T> own = own_slabs(a, i, j, k);
     const T dt = a.g.dt;
 #pragma unroll
     for (int s = 0; s < 6; ++s)
         if (own.in[s]) {
             const size_t q = own.idx[s];
+            T qx[4], qa[4];
 #pragma unroll
-            for (int f = 0; f < 4; ++f)
-                r.qdst[s][f][q] = rk_update<T, false>(st, r.qxn[s][f][q], dt * xc[f], r.qacc[s][f], nullptr, nullptr, q);
+            for (int f = 0; f < 4; ++f) {
+                qx[f] = r.qxn[s][f][q];
+                qa[f] = st != 0 ? r.qacc[s][f][q] : T(0);
+            }
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                T qao, unused;
+                const T qd = rk_regs<T, false>(st, qx[f], dt * xc[f], qa[f], T(0), qao, unused);
+                r.qdst[s][f][q] = qd;
+                if (st < 3) r.qacc[s][f][q] = qao;
+            }
         }
 }
 
@@ -936,14 +961,35 @@ __device__ __forceinline__ void rk_pore_point(const RKPoreArgs<T> &r, int i, int
     const T xc[4] = {g.un[id], g.vn[id], g.wn[id], pc};
     const T kf[4] = {f * gx, f * gy, f * gz, (-r.gama) * div};
     const int st = r.stage;
+    // every register read first, then the arithmetic, then the stores
+    T xn[4], ac[4], lo[4], er[4];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        const T xn = st == 0 ? xc[q] : r.xn[q][id];
-        r.dst[q][id] = rk_update_if<T, COMP>(st, xn, kf[q], r.acc[q], r.lo[q], r.err[q], id, r.cf);
+    for (int q = 0; q < 3; ++q) {  // velocities: integrating-factor update
+        xn[q] = st == 0 ? xc[q] : r.xn[q][id];
+        ac[q] = st == 0 ? T(0) : r.acc[q][id];
+        lo[q] = (COMP && st != 0) ? r.lo[q][id] : T(0);
+        er[q] = (COMP && st == 0) ? r.err[q][id] : T(0);
     }
-    {
-        const T xn = st == 0 ? xc[3] : (!COMP && st == 3) ? T(0) : r.xn[3][id];
-        r.dst[3][id] = rk_update<T, COMP>(st, xn, kf[3], r.acc[3], r.lo[3], r.err[3], id);
+    xn[3] = st == 0 ? xc[3] : (!COMP && st == 3) ? T(0) : r.xn[3][id];
+    ac[3] = st == 0 ? T(0) : r.acc[3][id];
+    lo[3] = T(0);
+    er[3] = (COMP && st == 3) ? r.err[3][id] : T(0);
+    T d[4], ao[4], lout[4], eo[4];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) d[q] = rk_if_regs<T, COMP>(st, xn[q], kf[q], ac[q], lo[q], er[q], r.cf, ao[q], lout[q], eo[q]);
+    d[3] = rk_regs<T, COMP>(st, xn[3], kf[3], ac[3], er[3], ao[3], eo[3]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        r.dst[q][id] = d[q];
+        if (st < 3) r.acc[q][id] = ao[q];
+    }
+    if constexpr (COMP) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            if (st < 3) r.lo[q][id] = lout[q];
+            else r.err[q][id] = eo[q];
+        }
+        if (st == 3) r.err[3][id] = eo[3];
     }
 }
 
