@@ -765,6 +765,42 @@ __device__ __forceinline__ T rk_regs(int st, T xn, T k, T acc_in, T err_in, T &a
     }
 }
 
+// Pointer form of the register update, kept for the porous stage: there, reading each
+// field's registers next to its update measured faster than reading them all first.
+template <typename T, bool COMP>
+__device__ __forceinline__ T rk_update(int st, T xn, T k, T *acc, T *lo, T *err, size_t id)
+{
+    if constexpr (!COMP) {
+        if (st == 0) {
+            acc[id] = xn + rk_b<T>(0) * k;
+            return xn + rk_a<T>(0) * k;
+        }
+        if (st < 3) {
+            acc[id] = acc[id] + rk_b<T>(st) * k;
+            return xn + rk_a<T>(st) * k;
+        }
+        return acc[id] + rk_b<T>(3) * k;
+    } else {
+        // acc holds the step's increment sum b1 k1 + ... (small next to the state, so its FP32
+        // rounding is small next to an ulp of the state); the last stage adds it and the
+        // carried error of the state with an exact TwoSum and keeps the new rounding error
+        (void)lo;
+        if (st == 0) {
+            acc[id] = rk_b<T>(0) * k;
+            return xn + rk_a<T>(0) * k;
+        }
+        if (st < 3) {
+            acc[id] = acc[id] + rk_b<T>(st) * k;
+            return xn + rk_a<T>(st) * k;
+        }
+        const T y = (acc[id] + rk_b<T>(3) * k) + err[id];
+        const T x = xn + y;
+        const T bb = x - xn;
+        err[id] = (xn - (x - bb)) + (y - bb);  // TwoSum: xn + y = x + err exactly
+        return x;
+    }
+}
+
 // Lawson (integrating-factor) RK4 for the porous velocities, x' = -(beta/dt) x + N(x) with
 // k = dt N(stage): classical RK4 in the variable e^{beta t/dt} x, so the stiff Darcy drag is
 // integrated exactly (no step limit from sigma dt / rho_e).  With cf = {E, E2, E2/2, E/6, E2/3},
@@ -775,37 +811,35 @@ __device__ __forceinline__ T rk_regs(int st, T xn, T k, T acc_in, T err_in, T &a
 //   stage 3: x_{n+1} = acc + k/6
 // COMP carries the low part and the state error as rk_regs does (the error scaled by E).
 template <typename T, bool COMP>
-__device__ __forceinline__ T rk_if_regs(int st, T xn, T k, T acc_in, T lo_in, T err_in, const T *cf, T &acc_out,
-                                        T &lo_out, T &err_out)
+__device__ __forceinline__ T rk_update_if(int st, T xn, T k, T *acc, T *lo, T *err, size_t id, const T *cf)
 {
     const T E = cf[0], E2 = cf[1];
     const T w = st == 0 ? cf[3] : st < 3 ? cf[4] : T(1.0 / 6.0);
     const T cn = st == 2 ? E : E2, an = st == 0 ? cf[2] : st == 1 ? T(0.5) : E2;
     if constexpr (!COMP) {
-        (void)lo_in; (void)err_in; (void)lo_out; (void)err_out;
         if (st == 0) {
-            acc_out = E * xn + w * k;
+            acc[id] = E * xn + w * k;
             return cn * xn + an * k;
         }
         if (st < 3) {
-            acc_out = acc_in + w * k;
+            acc[id] = acc[id] + w * k;
             return cn * xn + an * k;
         }
-        return acc_in + w * k;
+        return acc[id] + w * k;
     } else {
         const T y = w * k;
-        const T a0 = st == 0 ? E * xn : acc_in;
+        const T a0 = st == 0 ? E * xn : acc[id];
         const T sm = a0 + y;
         const T bb = sm - a0;
         const T e = (a0 - (sm - bb)) + (y - bb);
-        const T l = (st == 0 ? E * err_in : lo_in) + e;
+        const T l = (st == 0 ? E * err[id] : lo[id]) + e;
         if (st < 3) {
-            acc_out = sm;
-            lo_out = l;
+            acc[id] = sm;
+            lo[id] = l;
             return cn * xn + an * k;
         }
         const T x = sm + l;
-        err_out = l - (x - sm);
+        err[id] = l - (x - sm);
         return x;
     }
 }
@@ -961,35 +995,14 @@ __device__ __forceinline__ void rk_pore_point(const RKPoreArgs<T> &r, int i, int
     const T xc[4] = {g.un[id], g.vn[id], g.wn[id], pc};
     const T kf[4] = {f * gx, f * gy, f * gz, (-r.gama) * div};
     const int st = r.stage;
-    // every register read first, then the arithmetic, then the stores
-    T xn[4], ac[4], lo[4], er[4];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {  // velocities: integrating-factor update
-        xn[q] = st == 0 ? xc[q] : r.xn[q][id];
-        ac[q] = st == 0 ? T(0) : r.acc[q][id];
-        lo[q] = (COMP && st != 0) ? r.lo[q][id] : T(0);
-        er[q] = (COMP && st == 0) ? r.err[q][id] : T(0);
+    for (int q = 0; q < 3; ++q) {
+        const T xn = st == 0 ? xc[q] : r.xn[q][id];
+        r.dst[q][id] = rk_update_if<T, COMP>(st, xn, kf[q], r.acc[q], r.lo[q], r.err[q], id, r.cf);
     }
-    xn[3] = st == 0 ? xc[3] : (!COMP && st == 3) ? T(0) : r.xn[3][id];
-    ac[3] = st == 0 ? T(0) : r.acc[3][id];
-    lo[3] = T(0);
-    er[3] = (COMP && st == 3) ? r.err[3][id] : T(0);
-    T d[4], ao[4], lout[4], eo[4];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) d[q] = rk_if_regs<T, COMP>(st, xn[q], kf[q], ac[q], lo[q], er[q], r.cf, ao[q], lout[q], eo[q]);
-    d[3] = rk_regs<T, COMP>(st, xn[3], kf[3], ac[3], er[3], ao[3], eo[3]);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        r.dst[q][id] = d[q];
-        if (st < 3) r.acc[q][id] = ao[q];
-    }
-    if constexpr (COMP) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            if (st < 3) r.lo[q][id] = lout[q];
-            else r.err[q][id] = eo[q];
-        }
-        if (st == 3) r.err[3][id] = eo[3];
+    {
+        const T xn = st == 0 ? xc[3] : (!COMP && st == 3) ? T(0) : r.xn[3][id];
+        r.dst[3][id] = rk_update<T, COMP>(st, xn, kf[3], r.acc[3], r.lo[3], r.err[3], id);
     }
 }
 
