@@ -37,8 +37,14 @@ from scipy.optimize import nnls
 K = 8
 # pole grid in units of 1/tau: one pole at zero (the layer's compliance over a rigid backing)
 R_GRID = np.concatenate([[0.0], np.logspace(-2.5, 2.0, K - 1)])
-HULL_SIGMA_MIN = 3.0e4
+# Below ~200 kPa s/m^2 the layer is extended-reacting at the grazing angles of outdoor
+# propagation (its refraction index is small enough that the wave runs along inside it), which
+# a locally reacting impedance cannot represent: measured in the judge box, the one-third-
+# octave IL difference grows with the angle of incidence there (0.03 dB normal, 0.35 dB at
+# 69 degrees for 74 kPa s/m^2) and stays <= 0.02 dB at all angles for 500-1500 kPa s/m^2.
+HULL_SIGMA_MIN = 2.0e5
 HULL_FIT_R = 0.05
+JUDGE_BOUND_DB = 0.2          # one-third-octave IL bound of the judge box
 F_BAND = (50.0, 1000.0)       # the accuracy band
 F_FIT = (20.0, 1500.0)        # the fitted band
 USE_GRAPH = True              # column and judge runs replay captured HIP graphs
@@ -233,8 +239,10 @@ class Reflection:
         Rp = moebius_apply(self.coef, discrete_impedance(self.f, a, self.zinf, self.tau, self.dt, poles))
         return float(np.abs(Rp[b] - self.R[b]).max())
 
-    def fit(self):
-        """Non-negative residues for the porous reflection; returns (a, error)."""
+    def fit(self, ridge=0.0):
+        """Non-negative residues for the porous reflection; returns (a, error).  `ridge` adds
+        ridge * (mean column norm)^2 * |a|^2 to the least squares (smoother residues across
+        grounds, at some cost in fit error)."""
         co, R, f = self.coef, self.R, self.f
         Zreq = (co[:, 0] - co[:, 2] * R) / (co[:, 3] * R - co[:, 1])
         tb = self.dt / self.tau
@@ -246,7 +254,12 @@ class Reflection:
         M = np.vstack([(A * w[:, None]).real, (A * w[:, None]).imag])
         rhs = np.concatenate([(y * w).real, (y * w).imag])
         ok = np.isfinite(rhs)
-        a, _ = nnls(M[ok], rhs[ok])
+        M, rhs = M[ok], rhs[ok]
+        if ridge > 0:
+            scale = np.sqrt(ridge) * np.linalg.norm(M, axis=0).mean()
+            M = np.vstack([M, scale * np.eye(K)])
+            rhs = np.concatenate([rhs, np.zeros(K)])
+        a, _ = nnls(M, rhs)
         return a, self.error(a)
 
 
@@ -286,12 +299,19 @@ class Columns:
         return Reflection(g, f, R, moebius(Zs, Rs), zinf, tau, self.dt)
 
 
-def training_set(grounds, columns: Columns | None = None, progress=None):
+# ridge weight of the training targets' fits: the plain non-negative fit's active set jumps
+# between neighbouring grounds, which the network cannot follow; with this weight the fit
+# errors stay the same (median 1.1e-3) and the network's held-out reflection error drops from
+# 3.4e-3 to 1.4e-3 (median; 90th percentile 1.2e-2 to 5.5e-3) on 64 held-out grounds
+TRAIN_RIDGE = 1e-4
+
+
+def training_set(grounds, columns: Columns | None = None, progress=None, ridge=TRAIN_RIDGE):
     """(X [n x 4], A [n x K] fitted residues, E [n] fit errors) from the column runs."""
     cols = columns or Columns()
     X, A, E = [], [], []
     for n, g in enumerate(grounds):
-        a, e = cols.measure(g).fit()
+        a, e = cols.measure(g).fit(ridge)
         X.append(normalise(g)); A.append(a); E.append(e)
         if progress:
             progress(n, g, e)
@@ -446,15 +466,17 @@ class LearnedGround:
     def __init__(self, seed=1, device=0):
         self.net = Network(seed=seed, device=device)
 
-    def train(self, X, A, epochs=400, batch=32, lr=2e-3, seed=7):
-        return self.net.train(X, np.sqrt(np.maximum(A, 0.0)).astype(np.float32), batch=batch, epochs=epochs, lr=lr,
-                              seed=seed)
+    def train(self, X, A, epochs=600, batch=32, lr=2e-3, seed=7):
+        """`epochs` at lr, then as many at lr/4 (Adam's moments carry over)."""
+        t = np.sqrt(np.maximum(A, 0.0)).astype(np.float32)
+        self.net.train(X, t, batch=batch, epochs=epochs, lr=lr, seed=seed)
+        return self.net.train(X, t, batch=batch, epochs=epochs, lr=lr / 4, seed=seed + 1)
 
     def residues(self, grounds):
         t = self.net.predict(np.array([normalise(g) for g in grounds], np.float32))
         return t.astype(np.float64) ** 2
 
-    def decide(self, g: Ground, columns: Columns, bound_r=HULL_FIT_R, bound_db=0.2):
+    def decide(self, g: Ground, columns: Columns, bound_r=HULL_FIT_R, bound_db=JUDGE_BOUND_DB):
         """(residues or None, report): the full porous solve judges the predicted residues --
         the column reflection error and the one-third-octave IL difference in the judge box;
         None (keep the full porous solve) outside the hull or above either bound."""
