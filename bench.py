@@ -17,8 +17,9 @@ of IL cancels in the difference).
 
 `roofline`: the dominant kernel's compulsory HBM bytes per launch over its mean launch time
 (HIP events on the solver stream); `traffic` comes from rocprofv3 FETCH_SIZE / WRITE_SIZE
-runs when a JSON of them is given.  `cpu_baseline`: the FP64 C oracle of the same step on one
-host core over a bounded sample (the first step(s) of the same workload).
+runs when a JSON of them is given.  `cpu_baseline`: the FP64 C oracle of the same step on the host's
+OpenMP threads (and on one thread beside it) over a bounded sample (the first step(s) of the
+same workload).
 
 `--workload c3` runs the batched ensemble: 64 seeded uneven domains (deck.config_c3) advanced
 together by the ensemble runner, FP32 (and FP64 for the accuracy figures), 3000 steps.
@@ -120,9 +121,15 @@ def accuracy(a, b, dt, fmin=50.0, fmax=1000.0):
     return rel, worst, int(use.sum())
 
 
-def cpu_baseline(deck, receivers, budget_s=20.0):
+def cpu_baseline(deck, receivers, budget_s=20.0, threads=None):
+    """The FP64 oracle of the same step on the host: the first step(s) of the workload within
+    about `budget_s` seconds, on `threads` host threads (default: OpenMP's count, which
+    OMP_NUM_THREADS sets)."""
     from oracle import oracle as orc
 
+    if threads:
+        orc.set_threads(threads)
+    used = orc.get_threads()
     o = orc.Oracle(deck, receivers=receivers)
     o.frame_setup(0)
     npts = o.nx * o.ny * o.nz + o.pnx * o.pny * o.pnz
@@ -134,7 +141,7 @@ def cpu_baseline(deck, receivers, budget_s=20.0):
     if more:
         o.step(more)
     sec = first + (time.perf_counter() - t2)
-    return dict(value=npts * (1 + more) / sec, steps=1 + more, seconds=sec, points=npts)
+    return dict(value=npts * (1 + more) / sec, steps=1 + more, seconds=sec, points=npts, threads=used)
 
 
 def run_c3(lee, a):
@@ -381,11 +388,17 @@ def main():
             fb[f"fb_fp{p}_kernels"] = kernel_table(rr["kernels"])
     cpu = None
     if not a.no_cpu:
-        print("[bench] cpu baseline (oracle, one core)", file=sys.stderr, flush=True)
-        c = cpu_baseline(deck, rec)
-        cpu = {"value": round(c["value"], 1), "unit": "grid-pt updates/s", "cores": 1, "kind": "port",
+        from oracle import oracle as orc
+        nthr = orc.get_threads()
+        print(f"[bench] cpu baseline (oracle, {nthr} threads, then 1)", file=sys.stderr, flush=True)
+        c = cpu_baseline(deck, rec, threads=nthr)
+        c1 = cpu_baseline(deck, rec, budget_s=15.0, threads=1)
+        orc.set_threads(nthr)
+        cpu = {"value": round(c["value"], 1), "unit": "grid-pt updates/s", "cores": c["threads"], "kind": "port",
                "sample": f"{wl} (RK4, terrain, porous ground) FP64, first {c['steps']} step(s) "
-                         f"({c['seconds']:.1f} s), single thread"}
+                         f"({c['seconds']:.1f} s) on {c['threads']} OpenMP threads",
+               "single_thread_value": round(c1["value"], 1),
+               "single_thread_sample": f"first {c1['steps']} step(s) ({c1['seconds']:.1f} s), one thread"}
     extra = {}
     for p, rr in res.items():
         extra[f"fp{p}_pt_updates_per_s"] = rr["points"] * a.steps / rr["wall"]

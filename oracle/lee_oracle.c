@@ -14,6 +14,7 @@
 #include "lee_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -895,6 +896,7 @@ void orc_air_dp(orc_sim *s, const double *u, const double *v, const double *w,
                 const double *p, double *dp)
 {
     const ogrid *g = &s->air;
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < g->nz - 1; ++k)
         for (int j = 1; j < g->ny - 1; ++j)
             for (int i = 1; i < g->nx - 1; ++i)
@@ -905,6 +907,7 @@ void orc_air_duvw(orc_sim *s, const double *u, const double *v, const double *w,
                   const double *p, double *du, double *dv, double *dw)
 {
     const ogrid *g = &s->air;
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < g->nz - 1; ++k)
         for (int j = 1; j < g->ny - 1; ++j)
             for (int i = 1; i < g->nx - 1; ++i) {
@@ -920,6 +923,7 @@ static void pore_fp_on(orc_sim *s, const double *u, const double *v, const doubl
 {
     ogrid *g = &s->pore;
     double dxi = g->dxi, dyi = g->dyi, dzi = g->dzi;
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < g->nz - 1; ++k)
         for (int j = 1; j < g->ny - 1; ++j)
             for (int i = 1; i < g->nx - 1; ++i) {
@@ -936,6 +940,7 @@ static void pore_fuvw_on(orc_sim *s, const double *u, const double *v, const dou
     ogrid *g = &s->pore;
     double dxi = g->dxi, dyi = g->dyi, dzi = g->dzi, dt = g->dt;
     double ib = s->p_invbeta, eff = s->p_eff;
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < g->nz - 1; ++k)
         for (int j = 1; j < g->ny - 1; ++j)
             for (int i = 1; i < g->nx - 1; ++i) {
@@ -966,6 +971,7 @@ void orc_pore_duvw(orc_sim *s, const double *u, const double *v, const double *w
 /* FB add loops of calculation_2D::Cal_pressure / Cal_velocity (calculation_2D.cpp:598-608, 564-574). */
 static void add_interior(ogrid *g, double *fnn, const double *fn)
 {
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < g->nz - 1; ++k)
         for (int j = 1; j < g->ny - 1; ++j)
             for (int i = 1; i < g->nx - 1; ++i) {
@@ -1712,6 +1718,7 @@ static void pore_rk_duvw_on(orc_sim *s, const double *u, const double *v, const 
     ogrid *g = &s->pore;
     double dxi = g->dxi, dyi = g->dyi, dzi = g->dzi, dt = g->dt;
     double f = (-dt) * s->p_eff;
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < g->nz - 1; ++k)
         for (int j = 1; j < g->ny - 1; ++j)
             for (int i = 1; i < g->nx - 1; ++i) {
@@ -1833,6 +1840,7 @@ static void rk4_step(orc_sim *s)
             int lo = (q & 1) ? (ax == 0 ? a->nx : ax == 1 ? a->ny : a->nz) - b->n : 0;
             const double *src[4] = {a->unn, a->vnn, a->wnn, a->pnn};
             long cnt = orc_q_size(s, q);
+#pragma omp parallel for schedule(static)
             for (long e = 0; e < cnt; ++e) {
                 long A = ax == 0 ? b->n : a->nx, B = ax == 1 ? b->n : a->ny;
                 int i = (int)(e % A), j = (int)((e / A) % B), k = (int)(e / (A * B));
@@ -1848,6 +1856,7 @@ static void rk4_step(orc_sim *s)
         for (int d = 0; d < nd; ++d) {
             ogrid *G = g[d];
             double *xn[4] = {G->un, G->vn, G->wn, G->pn}, *xs[4] = {G->unn, G->vnn, G->wnn, G->pnn};
+#pragma omp parallel for schedule(static)
             for (int k = 1; k < G->nz - 1; ++k)
                 for (int j = 1; j < G->ny - 1; ++j)
                     for (int i = 1; i < G->nx - 1; ++i) {
@@ -1999,6 +2008,11 @@ void orc_wind(const orc_sim *s, double *U, double *V, double *W)
 
 /* FFT (FFT.cpp:15-82): bit reversal then iterative radix-2 butterflies with the
  * recursive twiddle update c1,c2 <- sqrt((1 +- c1)/2). */
+/* Host threads of the stencil loops (OpenMP; every loop writes disjoint points, so results
+ * do not depend on the count). */
+void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
+
 void orc_fft(int dir, int m, double *x, double *y)
 {
     long n = 1;

@@ -87,6 +87,8 @@ void orc_wind(const orc_sim *s, double *U, double *V, double *W);
 void orc_metric(const orc_sim *s, int domain, int i, int j, int k, double *out10);
 
 /* Reference FFT (FFT.cpp:15-82) and spectrum writer (FFT.cpp:84-121). */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 void orc_fft(int dir, int m, double *x, double *y);
 
 #ifdef __cplusplus

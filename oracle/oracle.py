@@ -23,6 +23,15 @@ _ip = ctypes.POINTER(ctypes.c_int)
 _lib = None
 
 
+def set_threads(n: int) -> None:
+    """Host threads of the oracle's stencil loops (results do not depend on it)."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return lib().orc_get_threads()
+
+
 def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE, "liblee_oracle.so"], check=True)
 
@@ -53,6 +62,8 @@ def lib():
         L.orc_air_duvw.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_pore_dp.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_pore_duvw.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
         L.orc_q_size.restype = ctypes.c_long
         L.orc_q_size.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_q_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
