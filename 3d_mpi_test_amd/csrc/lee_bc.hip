@@ -198,26 +198,23 @@ template <typename T> __device__ __forceinline__ T v_pore_walk(const BCArgs<T> &
 
 // i, j, k of boundary node t of an nx*ny*nz box: the two x planes, then the two y planes
 // without their x edges, then the two z planes without x and y edges.
-__device__ __forceinline__ void boundary_node(long long t, int nx, int ny, int nz, int &i, int &j, int &k)
+__device__ __forceinline__ void boundary_node(long long t64, int nx, int ny, int nz, int &i, int &j, int &k)
 {
-    const long long nA = 2LL * ny * nz, nB = 2LL * (nx - 2) * nz;
+    // 32-bit divisions: the boundary counts fit (lee_create refuses grids whose counts do not)
+    unsigned t = (unsigned)t64;
+    const unsigned uy = (unsigned)ny, ux2 = (unsigned)(nx - 2), uz = (unsigned)nz;
+    const unsigned nA = 2u * uy * uz, nB = 2u * ux2 * uz;
     if (t < nA) {
-        const long long pl = (long long)ny * nz;
-        const int side = (int)(t / pl);
-        const long long r = t % pl;
-        j = (int)(r % ny); k = (int)(r / ny); i = side ? nx - 1 : 0;
+        const unsigned pl = uy * uz, side = t / pl, r = t - side * pl, q = r / uy;
+        j = (int)(r - q * uy); k = (int)q; i = side ? nx - 1 : 0;
     } else if (t < nA + nB) {
         t -= nA;
-        const long long pl = (long long)(nx - 2) * nz;
-        const int side = (int)(t / pl);
-        const long long r = t % pl;
-        i = 1 + (int)(r % (nx - 2)); k = (int)(r / (nx - 2)); j = side ? ny - 1 : 0;
+        const unsigned pl = ux2 * uz, side = t / pl, r = t - side * pl, q = r / ux2;
+        i = 1 + (int)(r - q * ux2); k = (int)q; j = side ? ny - 1 : 0;
     } else {
         t -= nA + nB;
-        const long long pl = (long long)(nx - 2) * (ny - 2);
-        const int side = (int)(t / pl);
-        const long long r = t % pl;
-        i = 1 + (int)(r % (nx - 2)); j = 1 + (int)(r / (nx - 2)); k = side ? nz - 1 : 0;
+        const unsigned pl = ux2 * (uy - 2u), side = t / pl, r = t - side * pl, q = r / ux2;
+        i = 1 + (int)(r - q * ux2); j = 1 + (int)q; k = side ? nz - 1 : 0;
     }
 }
 

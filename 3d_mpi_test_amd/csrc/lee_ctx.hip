@@ -875,6 +875,10 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
         // AUTO runs the split passes: on MI355X they are faster than the fused pass (whose
         // per-plane barrier limits the loads in flight)
         C->fused = o.path == LEE_PATH_FUSED;
+        // the boundary kernels decode node indices in 32 bits
+        const long long nb = boundary_count<double>(mm.nx, mm.ny, mm.nz) +
+                             (mm.south_porous ? boundary_count<double>(mm.pnx, mm.pny, mm.pnz) : 0) + (long long)mm.ri.size();
+        if (nb >= (1LL << 31)) return set_err(nullptr, LEE_ERR_ARG, "grid too large: more than 2^31 boundary nodes");
     }
     HIPCHK(C, hipSetDevice(o.device));
     HIPCHK(C, hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
