@@ -403,7 +403,8 @@ def receivers_c0():
 
 def config_c1(time_domain: int = 10000, reference_scheme: bool = False) -> Deck:
     """512x256x128 air points at 0.04 m (x lateral, y propagation, z up) over a 16-point
-    porous layer with the C0 ground, 20-point PML on the sides and top, moving frame on
+    porous layer (200 kPa s/m^2, porosity 0.3, tortuosity 3), 20-point PML on the sides and
+    top, moving frame on
     (frame = the 256 y-planes, shifted by 32 planes; the first frame runs half a frame
     transit), Gaussian pulse of half-width 3 dx at 0.5 m, CFL 0.5 with c0 = 343 m/s, RK4,
     terrain-following over a Gaussian hill of height 2 m and sigma 5 m (centred on the line
@@ -425,8 +426,9 @@ def config_c1(time_domain: int = 10000, reference_scheme: bool = False) -> Deck:
 
 
 def config_c2(time_domain: int = 2000, seed: int = 20261202) -> Deck:
-    """Precision study: 768x384x192 air points at 0.04 m (the spacing is not specified; C1's
-    is used) over a 24-point porous layer with the C0 ground, terrain-following over a random
+    """Precision study: 768x384x192 air points at 0.04 m (the spacing is not specified; the
+    512x256x128 benchmark's is used) over a 24-point porous layer (200 kPa s/m^2, porosity 0.3,
+    tortuosity 3), terrain-following over a random
     smooth terrain -- a sum of 6 Gaussians with heights U[0, 1.5] m and widths U[3, 10] m at
     centres uniform over the footprint, drawn from numpy's PCG64 with the given seed --
     20-point PML on the sides and top, no wind, no moving frame, Gaussian pulse of half-width
@@ -477,9 +479,10 @@ def config_c3(time_domain: int = 3000, seed: int = 20261202, n: int = 64):
 
 
 def config_c4(ground, kind: str, residues=None, time_domain: int = 8000) -> Deck:
-    """Learned ground: 1024x512x160 air points at 0.04 m (the spacing is not specified; C1's
-    is used), flat ground, 20-point PML on the sides and top, no wind, no moving frame,
-    Gaussian pulse of half-width 3 dx at 0.5 m, 3 m into the domain, FB_p_v (the ground
+    """Learned ground: 1024x512x160 air points at 0.04 m (the spacing is not specified; the
+    512x256x128 benchmark's is used), flat ground, 20-point PML on the sides and top, no
+    wind, no moving frame, Gaussian pulse of half-width 3 dx at 0.5 m, 3 m into the domain,
+    FB_p_v (the ground
     impedance is discretised for it) at CFL 0.5, 8000 steps.  kind 'porous': the full solve
     over the porous layer of `ground` (a ground.Ground, depth points + the two interface
     planes); kind 'impedance': air only, the ground interface plane closed by the impedance

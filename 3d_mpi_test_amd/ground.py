@@ -48,7 +48,7 @@ JUDGE_BOUND_DB = 0.2          # one-third-octave IL bound of the judge box
 F_BAND = (50.0, 1000.0)       # the accuracy band
 F_FIT = (20.0, 1500.0)        # the fitted band
 USE_GRAPH = True              # column and judge runs replay captured HIP graphs
-# C3 / C4 ground-parameter distribution
+# ground-parameter distribution of the ensemble sweep and the learned-ground workload
 SIGMA_RANGE = (1.0e4, 2.0e6)
 POROSITY_RANGE = (0.2, 0.5)
 TORTUOSITY_RANGE = (1.5, 4.0)
@@ -89,7 +89,8 @@ def zinf_tau(g: Ground, rho0: float, c: float):
 
 
 def sample_grounds(n: int, seed: int):
-    """Grounds drawn from the C3 distribution: sigma log-uniform, the rest uniform."""
+    """Grounds drawn from the sweep distribution: sigma log-uniform in SIGMA_RANGE, porosity,
+    tortuosity and depth uniform in their ranges."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):

@@ -1,5 +1,5 @@
-"""Long-run stability of the RK4 path on the features the C1 workload combines: PML on all
-faces under a moving frame, a strong logarithmic wind, terrain, porous ground.  The
+"""Long-run stability of the RK4 path on the features the 512x256x128 benchmark workload
+combines: PML on all faces under a moving frame, a strong logarithmic wind, terrain, porous ground.  The
 reference's FB_p_v differencing grows in two of these settings (downwind convection of the
 cross-flow velocities; y-layer auxiliaries left in place by the frame shift); the RK4 path
 must stay bounded over thousands of steps."""
