@@ -404,9 +404,8 @@ def receivers_c0():
 def config_c1(time_domain: int = 10000, reference_scheme: bool = False) -> Deck:
     """512x256x128 air points at 0.04 m (x lateral, y propagation, z up) over a 16-point
     porous layer (200 kPa s/m^2, porosity 0.3, tortuosity 3), 20-point PML on the sides and
-    top, moving frame on
-    (frame = the 256 y-planes, shifted by 32 planes; the first frame runs half a frame
-    transit), Gaussian pulse of half-width 3 dx at 0.5 m, CFL 0.5 with c0 = 343 m/s, RK4,
+    top, moving frame on (frame = the 256 y-planes, shifted by 32 planes; the first frame runs
+    half a frame transit), Gaussian pulse of half-width 3 dx at 0.5 m, CFL 0.5 with c0 = 343 m/s, RK4,
     terrain-following over a Gaussian hill of height 2 m and sigma 5 m (centred on the line
     of receivers at y = 30 m), logarithmic wind u(z) = (0.4/0.41) ln(z/0.02) along +y.
     Heights of the source and receivers are above the local ground.  reference_scheme=True
