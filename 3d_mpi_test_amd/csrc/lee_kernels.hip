@@ -94,6 +94,168 @@ template <typename T> __device__ __forceinline__ OwnQ<T> own_slabs(const AirArgs
 }
 
 // ------------------------------------------------------------------------------------
+// The interior arithmetic of the pressure and velocity increments for the mean flows of
+// velocity_method 0 and 7 (none, or V along y only), from the point's neighbour values.  The
+// one-point stencils below and the four-point form of the RK4 stage both call these, so every
+// launch shape computes the same bits.
+// ------------------------------------------------------------------------------------
+template <typename T> struct Nb {
+    T mX, mY, mZ, zX, zY;                        // inverse metric (zX = zY = 0 on a flat grid)
+    T pc, p_im, p_jp, p_jm, p_kp, p_km;          // p and its neighbours (i-1, j+-1, k+-1)
+    T uc, u_ip, u_jp, u_jm, u_kp, u_km;          // u
+    T vc, v_jp, v_jm, v_kp, v_km;                // v
+    T wc, w_jp, w_jm, w_kp, w_km;                // w
+    T Vc, V_ip, V_im, V_jp, V_jm, V_kp, V_km;    // mean flow along y
+};
+
+// dt*R_p without the layer terms (air.cpp:1953-2043); also the damped terms ddx, ddy and
+// the difference dwz the layer terms use.
+template <typename T, int WM, bool TERRAIN, bool CEN>
+__device__ __forceinline__ T dp_core(const AirArgs<T> &a, const Nb<T> &n, T &ddx, T &ddy, T &dwz)
+{
+    const Grid<T> &g = a.g;
+    const T dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, dt = g.dt;
+    const T mX = n.mX, mY = n.mY, mZ = n.mZ, zX = n.zX, zY = n.zY, pc = n.pc;
+    const T dux = n.u_ip - n.uc;
+    const T dvy = n.v_jp - n.vc;
+    dwz = n.w_kp - n.wc;
+    T duz = T(0), dvz = T(0);
+    T div;
+    if constexpr (TERRAIN) {
+        duz = n.u_kp - n.uc;
+        dvz = n.v_kp - n.vc;
+        div = (((((mX * dux) * dxi) + ((zX * duz) * dzi)) + ((mY * dvy) * dyi)) + ((zY * dvz) * dzi)) + ((mZ * dwz) * dzi);
+    } else {
+        div = (((mX * dux) * dxi) + ((mY * dvy) * dyi)) + ((mZ * dwz) * dzi);
+    }
+    ddx = (mX * dux) * dxi;
+    ddy = (mY * dvy) * dyi;
+    if constexpr (TERRAIN) {
+        ddx = ddx + (zX * duz) * dzi;
+        ddy = ddy + (zY * dvz) * dzi;
+    }
+    T r;
+    if constexpr (WM == 2) {
+        // mean flow along y only (U = W = 0): the terms with U or W are exact zeros
+        const T Vc = n.Vc;
+        const T c2 = ((Vc * mY) * dyi) * dt;
+        T VY = ((mY * (n.V_jp - n.V_jm)) * dyi) * T(0.5);
+        if constexpr (CEN) r = -(c2 * ((n.p_jp - n.p_jm) * T(0.5)));
+        else r = -(c2 * (pc - n.p_jm));
+        if constexpr (TERRAIN) {
+            const T c3 = ((Vc * zY) * dzi) * dt;
+            VY = VY + ((zY * (n.V_kp - n.V_km)) * dzi) * T(0.5);
+            if constexpr (CEN) r = r - c3 * ((n.p_kp - n.p_km) * T(0.5));
+            else r = r - c3 * (pc - n.p_km);
+        }
+        r += (((-a.gam) * pc) * dt) * VY;
+        r += (((-a.gam) * a.Pav) * dt) * div;
+    } else {
+        r = (((-a.gam) * a.Pav) * dt) * div;
+    }
+    return r;
+}
+
+// dt*R_u, dt*R_v, dt*R_w without the layer terms (air.cpp:1476-1578); also the pressure
+// gradients gpx, gpy, gpz the layer terms use.
+template <typename T, int WM, bool TERRAIN, bool CEN>
+__device__ __forceinline__ void duvw_core(const AirArgs<T> &a, const Nb<T> &n, T &gpx, T &gpy, T &gpz, T &du, T &dv,
+                                          T &dw)
+{
+    const Grid<T> &g = a.g;
+    const T dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, dt = g.dt, Aav = a.Aav;
+    const T mX = n.mX, mY = n.mY, mZ = n.mZ, zX = n.zX, zY = n.zY, pc = n.pc;
+    const T bpz = pc - n.p_km;
+    gpx = (mX * (pc - n.p_im)) * dxi;
+    gpy = (mY * (pc - n.p_jm)) * dyi;
+    gpz = (mZ * bpz) * dzi;
+    if constexpr (TERRAIN) {
+        gpx = gpx + (zX * bpz) * dzi;
+        gpy = gpy + (zY * bpz) * dzi;
+    }
+    const T uc = n.uc, vc = n.vc, wc = n.wc;
+    if constexpr (WM == 2) {
+        // mean flow along y only (U = W = 0): the terms with U or W or their gradients vanish
+        const T Vc = n.Vc;
+        const T c2 = ((Vc * mY) * dyi) * dt;
+        T VX = ((mX * (n.V_ip - n.V_im)) * dxi) * T(0.5);
+        T VY = ((mY * (n.V_jp - n.V_jm)) * dyi) * T(0.5);
+        const T VZ = ((mZ * (n.V_kp - n.V_km)) * dzi) * T(0.5);
+        // d/dY (and d/dZ over terrain) of one component: forward (reference) or central
+        auto d1 = [&](T fp, T fm, T fc) {
+            if constexpr (CEN) return (fp - fm) * T(0.5);
+            else return fp - fc;
+        };
+        du = -(c2 * d1(n.u_jp, n.u_jm, uc));
+        dv = -(c2 * d1(n.v_jp, n.v_jm, vc));
+        dw = -(c2 * d1(n.w_jp, n.w_jm, wc));
+        if constexpr (TERRAIN) {
+            const T c3 = ((Vc * zY) * dzi) * dt;
+            VX = VX + ((zX * (n.V_kp - n.V_km)) * dzi) * T(0.5);
+            VY = VY + ((zY * (n.V_kp - n.V_km)) * dzi) * T(0.5);
+            du = du - c3 * d1(n.u_kp, n.u_km, uc);
+            dv = dv - c3 * d1(n.v_kp, n.v_km, vc);
+            dw = dw - c3 * d1(n.w_kp, n.w_km, wc);
+        }
+        du += ((-Aav) * dt) * gpx;
+        dv += (((-uc) * VX) * dt - (vc * VY) * dt) - (wc * VZ) * dt;
+        dv += ((-Aav) * dt) * gpy;
+        dw += ((-Aav) * dt) * gpz;
+    } else {
+        du = ((-Aav) * dt) * gpx;
+        dv = ((-Aav) * dt) * gpy;
+        dw = ((-Aav) * dt) * gpz;
+    }
+}
+
+// Neighbour values of one point for dp_core (DP) and/or duvw_core (DUVW), read from memory;
+// p of the pressure increment from g.pn, of the velocity increment from g.pnn.
+template <typename T, int WM, bool TERRAIN, bool CEN, bool DP, bool DUVW>
+__device__ __forceinline__ void load_nb(const AirArgs<T> &a, int i, int j, int k, Nb<T> &n)
+{
+    const Grid<T> &g = a.g;
+    const size_t nx = (size_t)g.nx, sxy = (size_t)g.nx * g.ny;
+    const size_t id = (size_t)i + nx * ((size_t)j + (size_t)g.ny * k);
+    n.zX = T(0); n.zY = T(0);
+    if constexpr (TERRAIN) metric_terrain(g, i, j, k, n.mX, n.mY, n.mZ, n.zX, n.zY);
+    else metric(g, i, j, k, n.mX, n.mY, n.mZ);
+    const T *u = g.un, *v = g.vn, *w = g.wn;
+    n.uc = u[id]; n.vc = v[id]; n.wc = w[id];
+    if constexpr (DP) {
+        const T *p = g.pn;
+        n.pc = p[id];
+        n.u_ip = u[id + 1]; n.v_jp = v[id + nx]; n.w_kp = w[id + sxy];
+        if constexpr (TERRAIN) { n.u_kp = u[id + sxy]; n.v_kp = v[id + sxy]; }
+        if constexpr (WM == 2) {
+            n.Vc = a.V[id]; n.V_jp = a.V[id + nx]; n.V_jm = a.V[id - nx];
+            n.p_jm = p[id - nx];
+            if constexpr (CEN) n.p_jp = p[id + nx];
+            if constexpr (TERRAIN) {
+                n.V_kp = a.V[id + sxy]; n.V_km = a.V[id - sxy];
+                n.p_km = p[id - sxy];
+                if constexpr (CEN) n.p_kp = p[id + sxy];
+            }
+        }
+    }
+    if constexpr (DUVW) {
+        const T *p = g.pnn;
+        n.pc = p[id];
+        n.p_im = p[id - 1]; n.p_jm = p[id - nx]; n.p_km = p[id - sxy];
+        if constexpr (WM == 2) {
+            n.Vc = a.V[id];
+            n.V_ip = a.V[id + 1]; n.V_im = a.V[id - 1]; n.V_jp = a.V[id + nx]; n.V_jm = a.V[id - nx];
+            n.V_kp = a.V[id + sxy]; n.V_km = a.V[id - sxy];
+            n.u_jp = u[id + nx]; n.v_jp = v[id + nx]; n.w_jp = w[id + nx];
+            if constexpr (CEN) { n.u_jm = u[id - nx]; n.v_jm = v[id - nx]; n.w_jm = w[id - nx]; }
+            if constexpr (TERRAIN) {
+                n.u_kp = u[id + sxy]; n.v_kp = v[id + sxy]; n.w_kp = w[id + sxy];
+                if constexpr (CEN) { n.u_km = u[id - sxy]; n.v_km = v[id - sxy]; n.w_km = w[id - sxy]; }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
 // Pressure increment dt*R_p at one interior point: air::cal_fp (air.cpp:1953-2322).
 // Reads u, v, w, p of the time level in g.un ... g.pn and the auxiliaries a.Qo.  Terms
 // follow the reference's association; the TERRAIN terms are those of its nine-metric chain
@@ -115,63 +277,51 @@ __device__ __forceinline__ T air_dp(const AirArgs<T> &a, int i, int j, int k)
     else metric(g, i, j, k, mX, mY, mZ);
     const T dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, dt = g.dt, gP = a.gP;
     const T pc = g.pn[id];
-    const T dux = g.un[id + 1] - g.un[id];
-    const T dvy = g.vn[id + nx] - g.vn[id];
-    const T dwz = g.wn[id + sxy] - g.wn[id];
-    T duz = T(0), dvz = T(0);
-    T div;
-    if constexpr (TERRAIN) {
-        duz = g.un[id + sxy] - g.un[id];
-        dvz = g.vn[id + sxy] - g.vn[id];
-        div = (((((mX * dux) * dxi) + ((zX * duz) * dzi)) + ((mY * dvy) * dyi)) + ((zY * dvz) * dzi)) + ((mZ * dwz) * dzi);
+    T ddx, ddy, dwz, r;
+    if constexpr (WM != 1) {
+        Nb<T> n;
+        load_nb<T, WM, TERRAIN, CEN, true, false>(a, i, j, k, n);
+        r = dp_core<T, WM, TERRAIN, CEN>(a, n, ddx, ddy, dwz);
     } else {
-        div = (((mX * dux) * dxi) + ((mY * dvy) * dyi)) + ((mZ * dwz) * dzi);
-    }
-    // d/dX of u and d/dY of v (the damped terms of the x and y layers)
-    T ddx = (mX * dux) * dxi, ddy = (mY * dvy) * dyi;
-    if constexpr (TERRAIN) {
-        ddx = ddx + (zX * duz) * dzi;
-        ddy = ddy + (zY * dvz) * dzi;
-    }
-    T r;
-    if constexpr (WM == 1) {
-        const T Uc = a.U[id], Vc = a.V[id], Wc = a.W[id];
-        const T c1 = ((Uc * mX) * dxi) * dt, c2 = ((Vc * mY) * dyi) * dt;
-        T c3;
-        T UX = ((mX * (a.U[id + 1] - a.U[id - 1])) * dxi) * T(0.5);
-        T VY = ((mY * (a.V[id + nx] - a.V[id - nx])) * dyi) * T(0.5);
-        const T WZ = ((mZ * (a.W[id + sxy] - a.W[id - sxy])) * dzi) * T(0.5);
+        const T dux = g.un[id + 1] - g.un[id];
+        const T dvy = g.vn[id + nx] - g.vn[id];
+        dwz = g.wn[id + sxy] - g.wn[id];
+        T duz = T(0), dvz = T(0);
+        T div;
         if constexpr (TERRAIN) {
-            c3 = ((((Uc * zX) + (Vc * zY)) + (Wc * mZ)) * dzi) * dt;
-            UX = UX + ((zX * (a.U[id + sxy] - a.U[id - sxy])) * dzi) * T(0.5);
-            VY = VY + ((zY * (a.V[id + sxy] - a.V[id - sxy])) * dzi) * T(0.5);
+            duz = g.un[id + sxy] - g.un[id];
+            dvz = g.vn[id + sxy] - g.vn[id];
+            div = (((((mX * dux) * dxi) + ((zX * duz) * dzi)) + ((mY * dvy) * dyi)) + ((zY * dvz) * dzi)) + ((mZ * dwz) * dzi);
         } else {
-            c3 = ((Wc * mZ) * dzi) * dt;
+            div = (((mX * dux) * dxi) + ((mY * dvy) * dyi)) + ((mZ * dwz) * dzi);
         }
-        if constexpr (CEN)
-            r = ((-c1) * ((g.pn[id + 1] - g.pn[id - 1]) * T(0.5)) - c2 * ((g.pn[id + nx] - g.pn[id - nx]) * T(0.5))) -
-                c3 * ((g.pn[id + sxy] - g.pn[id - sxy]) * T(0.5));
-        else
-            r = ((-c1) * (pc - g.pn[id - 1]) - c2 * (pc - g.pn[id - nx])) - c3 * (pc - g.pn[id - sxy]);
-        r += (((-a.gam) * pc) * dt) * ((UX + VY) + WZ);
-        r += (((-a.gam) * a.Pav) * dt) * div;
-    } else if constexpr (WM == 2) {
-        // mean flow along y only (U = W = 0): the terms with U or W are exact zeros
-        const T Vc = a.V[id];
-        const T c2 = ((Vc * mY) * dyi) * dt;
-        T VY = ((mY * (a.V[id + nx] - a.V[id - nx])) * dyi) * T(0.5);
-        if constexpr (CEN) r = -(c2 * ((g.pn[id + nx] - g.pn[id - nx]) * T(0.5)));
-        else r = -(c2 * (pc - g.pn[id - nx]));
+        // d/dX of u and d/dY of v (the damped terms of the x and y layers)
+        ddx = (mX * dux) * dxi;
+        ddy = (mY * dvy) * dyi;
         if constexpr (TERRAIN) {
-            const T c3 = ((Vc * zY) * dzi) * dt;
-            VY = VY + ((zY * (a.V[id + sxy] - a.V[id - sxy])) * dzi) * T(0.5);
-            if constexpr (CEN) r = r - c3 * ((g.pn[id + sxy] - g.pn[id - sxy]) * T(0.5));
-            else r = r - c3 * (pc - g.pn[id - sxy]);
+            ddx = ddx + (zX * duz) * dzi;
+            ddy = ddy + (zY * dvz) * dzi;
         }
-        r += (((-a.gam) * pc) * dt) * VY;
-        r += (((-a.gam) * a.Pav) * dt) * div;
-    } else {
-        r = (((-a.gam) * a.Pav) * dt) * div;
+            const T Uc = a.U[id], Vc = a.V[id], Wc = a.W[id];
+            const T c1 = ((Uc * mX) * dxi) * dt, c2 = ((Vc * mY) * dyi) * dt;
+            T c3;
+            T UX = ((mX * (a.U[id + 1] - a.U[id - 1])) * dxi) * T(0.5);
+            T VY = ((mY * (a.V[id + nx] - a.V[id - nx])) * dyi) * T(0.5);
+            const T WZ = ((mZ * (a.W[id + sxy] - a.W[id - sxy])) * dzi) * T(0.5);
+            if constexpr (TERRAIN) {
+                c3 = ((((Uc * zX) + (Vc * zY)) + (Wc * mZ)) * dzi) * dt;
+                UX = UX + ((zX * (a.U[id + sxy] - a.U[id - sxy])) * dzi) * T(0.5);
+                VY = VY + ((zY * (a.V[id + sxy] - a.V[id - sxy])) * dzi) * T(0.5);
+            } else {
+                c3 = ((Wc * mZ) * dzi) * dt;
+            }
+            if constexpr (CEN)
+                r = ((-c1) * ((g.pn[id + 1] - g.pn[id - 1]) * T(0.5)) - c2 * ((g.pn[id + nx] - g.pn[id - nx]) * T(0.5))) -
+                    c3 * ((g.pn[id + sxy] - g.pn[id - sxy]) * T(0.5));
+            else
+                r = ((-c1) * (pc - g.pn[id - 1]) - c2 * (pc - g.pn[id - nx])) - c3 * (pc - g.pn[id - sxy]);
+            r += (((-a.gam) * pc) * dt) * ((UX + VY) + WZ);
+            r += (((-a.gam) * a.Pav) * dt) * div;
     }
     const int n0 = a.qn[0], n1 = a.qn[1], n2 = a.qn[2], n3 = a.qn[3], n4 = a.qn[4], n5 = a.qn[5];
     // forward differences of a slab field along y or z with the metric, plus the cross-metric
@@ -369,91 +519,59 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
     const T dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, dt = g.dt, Aav = a.Aav;
     const T *p = g.pnn;
     const T pc = p[id];
-    const T bpz = pc - p[id - sxy];
-    T gpx = (mX * (pc - p[id - 1])) * dxi;
-    T gpy = (mY * (pc - p[id - nx])) * dyi;
-    const T gpz = (mZ * bpz) * dzi;
-    if constexpr (TERRAIN) {
-        gpx = gpx + (zX * bpz) * dzi;
-        gpy = gpy + (zY * bpz) * dzi;
-    }
     const T uc = g.un[id], vc = g.vn[id], wc = g.wn[id];
-    T du, dv, dw;
-    if constexpr (WM == 1) {
-        const T Uc = a.U[id], Vc = a.V[id], Wc = a.W[id];
-        const T c1 = ((Uc * mX) * dxi) * dt, c2 = ((Vc * mY) * dyi) * dt;
-        T c3;
-        if constexpr (TERRAIN) c3 = ((((Uc * zX) + (Vc * zY)) + (Wc * mZ)) * dzi) * dt;
-        else c3 = ((Wc * mZ) * dzi) * dt;
-        const T *U = a.U, *V = a.V, *W = a.W;
-        // central gradients of the mean flow: d/dX, d/dY, d/dZ
-        auto gx = [&](const T *F) {
-            T v = ((mX * (F[id + 1] - F[id - 1])) * dxi) * T(0.5);
-            if constexpr (TERRAIN) v = v + ((zX * (F[id + sxy] - F[id - sxy])) * dzi) * T(0.5);
-            return v;
-        };
-        auto gy = [&](const T *F) {
-            T v = ((mY * (F[id + nx] - F[id - nx])) * dyi) * T(0.5);
-            if constexpr (TERRAIN) v = v + ((zY * (F[id + sxy] - F[id - sxy])) * dzi) * T(0.5);
-            return v;
-        };
-        auto gz = [&](const T *F) { return ((mZ * (F[id + sxy] - F[id - sxy])) * dzi) * T(0.5); };
-        const T UX = gx(U), UY = gy(U), UZ = gz(U);
-        const T VX = gx(V), VY = gy(V), VZ = gz(V);
-        const T WX = gx(W), WY = gy(W), WZ = gz(W);
-        // convective differences of one component: forward (reference) or central
-        auto cv = [&](const T *F, T fc) {
-            if constexpr (CEN)
-                return ((-c1) * ((F[id + 1] - F[id - 1]) * T(0.5)) - c2 * ((F[id + nx] - F[id - nx]) * T(0.5))) -
-                       c3 * ((F[id + sxy] - F[id - sxy]) * T(0.5));
-            else
-                return ((-c1) * (F[id + 1] - fc) - c2 * (F[id + nx] - fc)) - c3 * (F[id + sxy] - fc);
-        };
-        du = cv(g.un, uc);
-        du += (((-uc) * UX) * dt - (vc * UY) * dt) - (wc * UZ) * dt;
-        du += ((-Aav) * dt) * gpx;
-        dv = cv(g.vn, vc);
-        dv += (((-uc) * VX) * dt - (vc * VY) * dt) - (wc * VZ) * dt;
-        dv += ((-Aav) * dt) * gpy;
-        dw = cv(g.wn, wc);
-        dw += (((-uc) * WX) * dt - (vc * WY) * dt) - (wc * WZ) * dt;
-        dw += ((-Aav) * dt) * gpz;
-    } else if constexpr (WM == 2) {
-        // mean flow along y only (U = W = 0): the terms with U or W or their gradients vanish
-        const T Vc = a.V[id];
-        const T c2 = ((Vc * mY) * dyi) * dt;
-        const T *V = a.V;
-        T VX = ((mX * (V[id + 1] - V[id - 1])) * dxi) * T(0.5);
-        T VY = ((mY * (V[id + nx] - V[id - nx])) * dyi) * T(0.5);
-        const T VZ = ((mZ * (V[id + sxy] - V[id - sxy])) * dzi) * T(0.5);
-        // d/dY (and d/dZ over terrain) of one component: forward (reference) or central
-        auto dy = [&](const T *F, T fc) {
-            if constexpr (CEN) return (F[id + nx] - F[id - nx]) * T(0.5);
-            else return F[id + nx] - fc;
-        };
-        auto dz = [&](const T *F, T fc) {
-            if constexpr (CEN) return (F[id + sxy] - F[id - sxy]) * T(0.5);
-            else return F[id + sxy] - fc;
-        };
-        du = -(c2 * dy(g.un, uc));
-        dv = -(c2 * dy(g.vn, vc));
-        dw = -(c2 * dy(g.wn, wc));
-        if constexpr (TERRAIN) {
-            const T c3 = ((Vc * zY) * dzi) * dt;
-            VX = VX + ((zX * (V[id + sxy] - V[id - sxy])) * dzi) * T(0.5);
-            VY = VY + ((zY * (V[id + sxy] - V[id - sxy])) * dzi) * T(0.5);
-            du = du - c3 * dz(g.un, uc);
-            dv = dv - c3 * dz(g.vn, vc);
-            dw = dw - c3 * dz(g.wn, wc);
-        }
-        du += ((-Aav) * dt) * gpx;
-        dv += (((-uc) * VX) * dt - (vc * VY) * dt) - (wc * VZ) * dt;
-        dv += ((-Aav) * dt) * gpy;
-        dw += ((-Aav) * dt) * gpz;
+    T gpx, gpy, gpz, du, dv, dw;
+    if constexpr (WM != 1) {
+        Nb<T> n;
+        load_nb<T, WM, TERRAIN, CEN, false, true>(a, i, j, k, n);
+        duvw_core<T, WM, TERRAIN, CEN>(a, n, gpx, gpy, gpz, du, dv, dw);
     } else {
-        du = ((-Aav) * dt) * gpx;
-        dv = ((-Aav) * dt) * gpy;
-        dw = ((-Aav) * dt) * gpz;
+        const T bpz = pc - p[id - sxy];
+        gpx = (mX * (pc - p[id - 1])) * dxi;
+        gpy = (mY * (pc - p[id - nx])) * dyi;
+        gpz = (mZ * bpz) * dzi;
+        if constexpr (TERRAIN) {
+            gpx = gpx + (zX * bpz) * dzi;
+            gpy = gpy + (zY * bpz) * dzi;
+        }
+            const T Uc = a.U[id], Vc = a.V[id], Wc = a.W[id];
+            const T c1 = ((Uc * mX) * dxi) * dt, c2 = ((Vc * mY) * dyi) * dt;
+            T c3;
+            if constexpr (TERRAIN) c3 = ((((Uc * zX) + (Vc * zY)) + (Wc * mZ)) * dzi) * dt;
+            else c3 = ((Wc * mZ) * dzi) * dt;
+            const T *U = a.U, *V = a.V, *W = a.W;
+            // central gradients of the mean flow: d/dX, d/dY, d/dZ
+            auto gx = [&](const T *F) {
+                T v = ((mX * (F[id + 1] - F[id - 1])) * dxi) * T(0.5);
+                if constexpr (TERRAIN) v = v + ((zX * (F[id + sxy] - F[id - sxy])) * dzi) * T(0.5);
+                return v;
+            };
+            auto gy = [&](const T *F) {
+                T v = ((mY * (F[id + nx] - F[id - nx])) * dyi) * T(0.5);
+                if constexpr (TERRAIN) v = v + ((zY * (F[id + sxy] - F[id - sxy])) * dzi) * T(0.5);
+                return v;
+            };
+            auto gz = [&](const T *F) { return ((mZ * (F[id + sxy] - F[id - sxy])) * dzi) * T(0.5); };
+            const T UX = gx(U), UY = gy(U), UZ = gz(U);
+            const T VX = gx(V), VY = gy(V), VZ = gz(V);
+            const T WX = gx(W), WY = gy(W), WZ = gz(W);
+            // convective differences of one component: forward (reference) or central
+            auto cv = [&](const T *F, T fc) {
+                if constexpr (CEN)
+                    return ((-c1) * ((F[id + 1] - F[id - 1]) * T(0.5)) - c2 * ((F[id + nx] - F[id - nx]) * T(0.5))) -
+                           c3 * ((F[id + sxy] - F[id - sxy]) * T(0.5));
+                else
+                    return ((-c1) * (F[id + 1] - fc) - c2 * (F[id + nx] - fc)) - c3 * (F[id + sxy] - fc);
+            };
+            du = cv(g.un, uc);
+            du += (((-uc) * UX) * dt - (vc * UY) * dt) - (wc * UZ) * dt;
+            du += ((-Aav) * dt) * gpx;
+            dv = cv(g.vn, vc);
+            dv += (((-uc) * VX) * dt - (vc * VY) * dt) - (wc * VZ) * dt;
+            dv += ((-Aav) * dt) * gpy;
+            dw = cv(g.wn, wc);
+            dw += (((-uc) * WX) * dt - (vc * WY) * dt) - (wc * WZ) * dt;
+            dw += ((-Aav) * dt) * gpz;
     }
     const int n0 = a.qn[0], n1 = a.qn[1], n2 = a.qn[2], n3 = a.qn[3], n5 = a.qn[5];
     // backward differences of a slab field with the metric: d/dX (x and cross-z), d/dY
