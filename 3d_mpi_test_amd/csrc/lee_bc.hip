@@ -108,7 +108,7 @@ template <typename T> __device__ __forceinline__ T p_pore_walk(const BCArgs<T> &
 }
 
 // ---- velocity (component c: 0 u, 1 v, 2 w) --------------------------------------------
-template <typename T> __device__ __forceinline__ T *vfield(const Grid<T> &g, int c) { return c == 0 ? g.unn : c == 1 ? g.vnn : g.wnn; }
+template <typename T> __device__ __forceinline__ gptr<T> vfield(const Grid<T> &g, int c) { return c == 0 ? g.unn : c == 1 ? g.vnn : g.wnn; }
 
 // East TDBC velocity from the final boundary pressure and the history of the previous step
 // (calculation_2D.cpp:1019-1028).

@@ -9,46 +9,68 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <type_traits>
 
 namespace lee {
+
+// A device-memory pointer in the argument blocks.  The blocks live in device memory, so a
+// pointer read from them is a generic (flat) address to the compiler: every access through it
+// would be a flat load or store, which also counts against the scalar-load counter, so each
+// wait for a pointer fetched from the block would wait for every field load in flight too.
+// Indexing through gptr reads and writes through the global address space (global_load /
+// global_store with a scalar base).  On the host it converts to and from T * like the raw
+// pointer it holds.
+template <typename T> struct gptr {
+    T *p;
+    gptr() = default;
+    __host__ __device__ constexpr gptr(T *q) : p(q) {}
+    __host__ __device__ constexpr gptr(std::nullptr_t) : p(nullptr) {}
+    template <typename U, typename = typename std::enable_if<std::is_convertible<U *, T *>::value>::type>
+    __host__ __device__ constexpr gptr(gptr<U> q) : p(q.p) {}
+    __host__ __device__ constexpr operator T *() const { return p; }
+    template <typename I> __device__ __forceinline__ __attribute__((address_space(1))) T &operator[](I i) const
+    {
+        return ((__attribute__((address_space(1))) T *)p)[i];
+    }
+};
 
 enum Face { F_N = 0, F_S = 1, F_E = 2, F_W = 3, F_F = 4, F_B = 5 };
 
 // Per-index PML profile: sigma, (1 - 1/beta), (1/beta - 1).
-template <typename T> struct DProf { const T *s, *c, *d; };
+template <typename T> struct DProf { gptr<const T> s, c, d; };
 
 template <typename T> struct Grid {
     int nx, ny, nz;
-    T *un, *vn, *wn, *pn;      // time level n
-    T *unn, *vnn, *wnn, *pnn;  // time level n+1
+    gptr<T> un, vn, wn, pn;  // time level n
+    gptr<T> unn, vnn, wnn, pnn;  // time level n+1
     // FP64: X_over_x, Y_over_y, Z_over_z (the Jacobian is formed per point as the
     // reference does).  FP32: the reciprocals 1/X_over_x ... (= the separable metric).
-    const T *mx, *my, *mz;
+    gptr<const T> mx, my, mz;
     T dxi, dyi, dzi, dt;
     // terrain-following grid (extension): Z_over_z = mz[k]*tsh[c], Z_over_x = thx[c]*ttk[k],
     // Z_over_y = thy[c]*ttk[k], c = i + nx*j; mx, my, mz then hold X_over_x ... in both precisions
-    const T *tsh, *thx, *thy, *ttk;
+    gptr<const T> tsh, thx, thy, ttk;
 };
 
 template <typename T> struct AirArgs {
     Grid<T> g;
-    const T *U, *V, *W;        // mean wind (NULL when zero)
+    gptr<const T> U, V, W;  // mean wind (NULL when zero)
     T gam, Pav, gP, Aav;
     int I1, I2, J1, J2, K1, K2;  // active PML widths (J1 carries the y1 time gate)
     DProf<T> x1, x2, y1, y2, z1, z2;
-    const T *Qo[6][4];         // slabs x1 x2 y1 y2 z1 z2, fields u v w p: before the step
-    T *Qn[6][4];               // and after it
+    gptr<const T> Qo[6][4];  // slabs x1 x2 y1 y2 z1 z2, fields u v w p: before the step
+    gptr<T> Qn[6][4];  // and after it
     int qn[6];                 // slab planes
     int qlo[6];                // first global plane of each slab
     int y1on;                  // y1 auxiliaries accumulate (gate open)
     // fused step only: the South face value source (0 copy of plane k=1, 1 porous plane
     // Kp-2 through the interface) and the porous pressure it reads
     int south_pore;
-    const T *pore_pnn;
+    gptr<const T> pore_pnn;
     int pore_nz;
     // embedded porous obstacle (FB step only): material id per node of the index box ob[],
     // NULL without obstacle; porous coefficients of the materials 1 (canopy) and 2 (trunk)
-    const unsigned char *omask;
+    gptr<const unsigned char> omask;
     int ob[6];
     T ogama[3], oinvb[3], oeff[3];
 };
@@ -63,26 +85,26 @@ template <typename T> struct BCArgs {
     Grid<T> air, pore;         // pore.nx == 0 without a porous ground
     int porous, iface, tdbc, y1on;
     long long nb_air, nb_pore; // boundary nodes of each box
-    const T *Qo[6][4];
-    T *Qn[6][4];
+    gptr<const T> Qo[6][4];
+    gptr<T> Qn[6][4];
     int qn[6], qlo[6], qaxis[6];
     T dt;
-    const double *psi_old;     // East TDBC histories [i + nx*k][6], read
-    double *psi_new;           // and written (double buffered)
+    gptr<const double> psi_old;  // East TDBC histories [i + nx*k][6], read
+    gptr<double> psi_new;  // and written (double buffered)
     double decay[6], gain[6], inv, Zinf, tb;
     // ground impedance on the South face: histories [i + nx*j][gK], FP64, double buffered;
     // the ground interface plane k = 0: its metric (gmx[i] = 1/X_over_x, gmy[j], gmz), the
     // porous coefficients of the top material (dt/(Kp porosity), 1/rho_e, 1/(1 + beta)),
     // 2 dt/(rho_e dz) of the half-cell momentum and Zinf (1 + tb sum_k a_k)
     int ground, gK;
-    const double *gpsi_old;
-    double *gpsi_new;
+    gptr<const double> gpsi_old;
+    gptr<double> gpsi_new;
     double gdecay[8], ggain[8], ginv, gZinf, gtb;
-    const double *gmx, *gmy;
+    gptr<const double> gmx, gmy;
     double gmz, g_gama, g_eff, g_ib, gcpl, gzi;
-    const long long *rec_idx;  // receiver samples of time level n (pressure phase)
+    gptr<const long long> rec_idx;  // receiver samples of time level n (pressure phase)
     int nrec;
-    double *rec_out;
+    gptr<double> rec_out;
 };
 // ---- RK4 (extension) ------------------------------------------------------------------
 // Registers: level n (xn), the accumulator acc = xn + sum_s b_s k_s, and the destination
@@ -91,25 +113,25 @@ template <typename T> struct BCArgs {
 // the rounding error of the stored state, carried from step to step.
 template <typename T> struct RKArgs {
     AirArgs<T> a;              // stage state in g.un .. g.pn (g.pnn = g.pn), auxiliaries Qo = Qn
-    const T *xn[4];            // u v w p at level n
-    T *acc[4], *lo[4], *err[4], *dst[4];
-    const T *qxn[6][4];        // auxiliaries at level n, accumulator, destination
-    T *qacc[6][4], *qdst[6][4];
+    gptr<const T> xn[4];  // u v w p at level n
+    gptr<T> acc[4], lo[4], err[4], dst[4];
+    gptr<const T> qxn[6][4];  // auxiliaries at level n, accumulator, destination
+    gptr<T> qacc[6][4], qdst[6][4];
     int stage;                 // 0..3
 };
 template <typename T> struct RKPoreArgs {
     Grid<T> g;                 // stage state in g.un .. g.pn
     T gama, eff, beta;         // porous.cpp:44-50: dt/(Kp Omega), 1/rho_e, sigma dt/rho_e
     T cf[5];                   // drag integrating factors E, E2, E2/2, E/6, E2/3 (E = e^-beta)
-    const T *xn[4];
-    T *acc[4], *lo[4], *err[4], *dst[4];
+    gptr<const T> xn[4];
+    gptr<T> acc[4], lo[4], err[4], dst[4];
     int stage;
 };
 template <typename T> struct RKBndArgs {
     BCArgs<T> b;               // destination state as air/pore "nn" fields (face walks)
-    const T *src[4];           // air stage state (right-hand side of the boundary auxiliaries)
-    const T *qxn[6][4];
-    T *qacc[6][4], *qdst[6][4];
+    gptr<const T> src[4];  // air stage state (right-hand side of the boundary auxiliaries)
+    gptr<const T> qxn[6][4];
+    gptr<T> qacc[6][4], qdst[6][4];
     int stage;
 };
 // wind: 0 none, 1 U V W fields, 2 V only (U = W = 0: velocity_method 7)

@@ -219,10 +219,10 @@ __device__ __forceinline__ void load_nb(const AirArgs<T> &a, int i, int j, int k
     n.zX = T(0); n.zY = T(0);
     if constexpr (TERRAIN) metric_terrain(g, i, j, k, n.mX, n.mY, n.mZ, n.zX, n.zY);
     else metric(g, i, j, k, n.mX, n.mY, n.mZ);
-    const T *u = g.un, *v = g.vn, *w = g.wn;
+    gptr<const T> u = g.un, v = g.vn, w = g.wn;
     n.uc = u[id]; n.vc = v[id]; n.wc = w[id];
     if constexpr (DP) {
-        const T *p = g.pn;
+        gptr<const T> p = g.pn;
         n.pc = p[id];
         n.u_ip = u[id + 1]; n.v_jp = v[id + nx]; n.w_kp = w[id + sxy];
         if constexpr (TERRAIN) { n.u_kp = u[id + sxy]; n.v_kp = v[id + sxy]; }
@@ -238,7 +238,7 @@ __device__ __forceinline__ void load_nb(const AirArgs<T> &a, int i, int j, int k
         }
     }
     if constexpr (DUVW) {
-        const T *p = g.pnn;
+        gptr<const T> p = g.pnn;
         n.pc = p[id];
         n.p_im = p[id - 1]; n.p_jm = p[id - nx]; n.p_km = p[id - sxy];
         if constexpr (WM == 2) {
@@ -326,12 +326,12 @@ __device__ __forceinline__ T air_dp(const AirArgs<T> &a, int i, int j, int k)
     const int n0 = a.qn[0], n1 = a.qn[1], n2 = a.qn[2], n3 = a.qn[3], n4 = a.qn[4], n5 = a.qn[5];
     // forward differences of a slab field along y or z with the metric, plus the cross-metric
     // z term over terrain: X-slab (stride n*ny per plane), Y-slab (nx*n), Z-slab (nx*ny)
-    auto fY = [&](const T *Q, size_t q, size_t sy, size_t sz) {
+    auto fY = [&](gptr<const T> Q, size_t q, size_t sy, size_t sz) {
         T v = (mY * (Q[q + sy] - Q[q])) * dyi;
         if constexpr (TERRAIN) v = v + (zY * (Q[q + sz] - Q[q])) * dzi;
         return v;
     };
-    auto fX = [&](const T *Q, size_t q, size_t sz) {
+    auto fX = [&](gptr<const T> Q, size_t q, size_t sz) {
         T v = (mX * (Q[q + 1] - Q[q])) * dxi;
         if constexpr (TERRAIN) v = v + (zX * (Q[q + sz] - Q[q])) * dzi;
         return v;
@@ -340,7 +340,7 @@ __device__ __forceinline__ T air_dp(const AirArgs<T> &a, int i, int j, int k)
     if (i < a.I1) {
         const T sx = a.x1.s[i], cx = a.x1.c[i];
         const size_t q = qxi(i, j, k, n0, ny), sz = (size_t)n0 * ny;
-        const T *Qv = a.Qo[0][1], *Qw = a.Qo[0][2];
+        gptr<const T> Qv = a.Qo[0][1], Qw = a.Qo[0][2];
         const T A = (cx * (gP * ddx)) * dt;
         const T B = (sx * (pc + gP * (fY(Qv, q, n0, sz) + ((mZ * (Qw[q + sz] - Qw[q])) * dzi)))) * dt;
         r += A - B;
@@ -350,7 +350,7 @@ __device__ __forceinline__ T air_dp(const AirArgs<T> &a, int i, int j, int k)
         const int ii = i - (nx - n1);
         const T sx = a.x2.s[i], cx = a.x2.c[i];
         const size_t q = qxi(ii, j, k, n1, ny), sz = (size_t)n1 * ny;
-        const T *Qv = a.Qo[1][1], *Qw = a.Qo[1][2];
+        gptr<const T> Qv = a.Qo[1][1], Qw = a.Qo[1][2];
         const T A = (cx * (gP * ddx)) * dt;
         const T B = (sx * (pc + gP * (fY(Qv, q, n1, sz) + ((mZ * (Qw[q + sz] - Qw[q])) * dzi)))) * dt;
         r += A - B;
@@ -393,7 +393,7 @@ __device__ __forceinline__ T air_dp(const AirArgs<T> &a, int i, int j, int k)
     if (j < a.J1) {
         const T sy = a.y1.s[j], cy = a.y1.c[j];
         const size_t q = qyi(i, j, k, nx, n2), sz = (size_t)nx * n2;
-        const T *Qu = a.Qo[2][0], *Qw = a.Qo[2][2];
+        gptr<const T> Qu = a.Qo[2][0], Qw = a.Qo[2][2];
         const T A = (cy * (gP * ddy)) * dt;
         const T B = (sy * (pc + gP * (fX(Qu, q, sz) + ((mZ * (Qw[q + sz] - Qw[q])) * dzi)))) * dt;
         r += A - B;
@@ -404,7 +404,7 @@ __device__ __forceinline__ T air_dp(const AirArgs<T> &a, int i, int j, int k)
         const int jj = j - (ny - n3);
         const T sy = a.y2.s[j], cy = a.y2.c[j];
         const size_t q = qyi(i, jj, k, nx, n3), sz = (size_t)nx * n3;
-        const T *Qu = a.Qo[3][0], *Qw = a.Qo[3][2];
+        gptr<const T> Qu = a.Qo[3][0], Qw = a.Qo[3][2];
         const T A = (cy * (gP * ddy)) * dt;
         const T B = (sy * (pc + gP * (fX(Qu, q, sz) + ((mZ * (Qw[q + sz] - Qw[q])) * dzi)))) * dt;
         r += A - B;
@@ -428,7 +428,7 @@ __device__ __forceinline__ T air_dp(const AirArgs<T> &a, int i, int j, int k)
         }
     };
     // z layer main terms: d/dX of Qu and d/dY of Qv in the z slab (air.cpp:2160-2292)
-    auto zmain = [&](T sz_, T cz, const T *Qu, const T *Qv, size_t q) {
+    auto zmain = [&](T sz_, T cz, gptr<const T> Qu, gptr<const T> Qv, size_t q) {
         const size_t sz = sxy;
         const T A = (((cz * a.gam) * a.Pav) * ((mZ * dwz) * dzi)) * dt;
         T inner;
@@ -517,7 +517,7 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
     if constexpr (TERRAIN) metric_terrain(g, i, j, k, mX, mY, mZ, zX, zY);
     else metric(g, i, j, k, mX, mY, mZ);
     const T dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, dt = g.dt, Aav = a.Aav;
-    const T *p = g.pnn;
+    gptr<const T> p = g.pnn;
     const T pc = p[id];
     const T uc = g.un[id], vc = g.vn[id], wc = g.wn[id];
     T gpx, gpy, gpz, du, dv, dw;
@@ -539,24 +539,24 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
             T c3;
             if constexpr (TERRAIN) c3 = ((((Uc * zX) + (Vc * zY)) + (Wc * mZ)) * dzi) * dt;
             else c3 = ((Wc * mZ) * dzi) * dt;
-            const T *U = a.U, *V = a.V, *W = a.W;
+            gptr<const T> U = a.U, V = a.V, W = a.W;
             // central gradients of the mean flow: d/dX, d/dY, d/dZ
-            auto gx = [&](const T *F) {
+            auto gx = [&](gptr<const T> F) {
                 T v = ((mX * (F[id + 1] - F[id - 1])) * dxi) * T(0.5);
                 if constexpr (TERRAIN) v = v + ((zX * (F[id + sxy] - F[id - sxy])) * dzi) * T(0.5);
                 return v;
             };
-            auto gy = [&](const T *F) {
+            auto gy = [&](gptr<const T> F) {
                 T v = ((mY * (F[id + nx] - F[id - nx])) * dyi) * T(0.5);
                 if constexpr (TERRAIN) v = v + ((zY * (F[id + sxy] - F[id - sxy])) * dzi) * T(0.5);
                 return v;
             };
-            auto gz = [&](const T *F) { return ((mZ * (F[id + sxy] - F[id - sxy])) * dzi) * T(0.5); };
+            auto gz = [&](gptr<const T> F) { return ((mZ * (F[id + sxy] - F[id - sxy])) * dzi) * T(0.5); };
             const T UX = gx(U), UY = gy(U), UZ = gz(U);
             const T VX = gx(V), VY = gy(V), VZ = gz(V);
             const T WX = gx(W), WY = gy(W), WZ = gz(W);
             // convective differences of one component: forward (reference) or central
-            auto cv = [&](const T *F, T fc) {
+            auto cv = [&](gptr<const T> F, T fc) {
                 if constexpr (CEN)
                     return ((-c1) * ((F[id + 1] - F[id - 1]) * T(0.5)) - c2 * ((F[id + nx] - F[id - nx]) * T(0.5))) -
                            c3 * ((F[id + sxy] - F[id - sxy]) * T(0.5));
@@ -576,12 +576,12 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
     const int n0 = a.qn[0], n1 = a.qn[1], n2 = a.qn[2], n3 = a.qn[3], n5 = a.qn[5];
     // backward differences of a slab field with the metric: d/dX (x and cross-z), d/dY
     // (y and cross-z); `sy`, `sz` are the slab strides in y and z
-    auto bX = [&](const T *Q, size_t q, size_t sz) {
+    auto bX = [&](gptr<const T> Q, size_t q, size_t sz) {
         T v = (mX * (Q[q] - Q[q - 1])) * dxi;
         if constexpr (TERRAIN) v = v + (zX * (Q[q] - Q[q - sz])) * dzi;
         return v;
     };
-    auto bY = [&](const T *Q, size_t q, size_t sy, size_t sz) {
+    auto bY = [&](gptr<const T> Q, size_t q, size_t sy, size_t sz) {
         T v = (mY * (Q[q] - Q[q - sy])) * dyi;
         if constexpr (TERRAIN) v = v + (zY * (Q[q] - Q[q - sz])) * dzi;
         return v;
@@ -590,7 +590,7 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
     if (i < a.I1) {
         const T sx = a.x1.s[i], cx = a.x1.c[i];
         const size_t q = qxi(i, j, k, n0, ny), sz = (size_t)n0 * ny;
-        const T *Qp = a.Qn[0][3];
+        gptr<const T> Qp = a.Qn[0][3];
         du += ((-sx) * uc) * dt + (cx * (Aav * gpx)) * dt;
         dv += ((-sx) * (vc + Aav * bY(Qp, q, n0, sz))) * dt;
         dw += ((-sx) * (wc + Aav * ((mZ * (Qp[q] - Qp[q - sz])) * dzi))) * dt;
@@ -599,13 +599,13 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
         const int ii = i - (nx - n1);
         const T sx = a.x2.s[i], cx = a.x2.c[i];
         const size_t q = qxi(ii, j, k, n1, ny), sz = (size_t)n1 * ny;
-        const T *Qp = a.Qn[1][3];
+        gptr<const T> Qp = a.Qn[1][3];
         du += ((-sx) * uc) * dt + (cx * (Aav * gpx)) * dt;
         dv += ((-sx) * (vc + Aav * bY(Qp, q, n1, sz))) * dt;
         dw += ((-sx) * (wc + Aav * ((mZ * (Qp[q] - Qp[q - sz])) * dzi))) * dt;
     }
     // y layer with x cross terms (air.cpp:1621-1728)
-    auto ylayer = [&](T sy, T cy, T dym, const T *Qp, size_t qy, int nq) {
+    auto ylayer = [&](T sy, T cy, T dym, gptr<const T> Qp, size_t qy, int nq) {
         const size_t szy = (size_t)nx * nq;
         du += ((-sy) * (uc + Aav * bX(Qp, qy, szy))) * dt;
         dv += ((cy * Aav) * gpy) * dt - (sy * vc) * dt;
@@ -613,7 +613,7 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
         if (i < a.I1) {
             const T sx = a.x1.s[i], dxm = a.x1.d[i];
             const size_t q = qxi(i, j, k, n0, ny), sz = (size_t)n0 * ny;
-            const T *Xp = a.Qn[0][3];
+            gptr<const T> Xp = a.Qn[0][3];
             du += (-((a.Qo[0][0][q] * sx) * sy + ((dxm * sy) * Aav) * bX(Xp, q, sz))) * dt;
             dv += (-((a.Qo[0][1][q] * sx) * sy + ((dym * sx) * Aav) * bY(Xp, q, n0, sz))) * dt;
             dw += (((-a.Qo[0][2][q]) * sx) * sy) * dt;
@@ -621,7 +621,7 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
             const int ii = i - (nx - n1);
             const T sx = a.x2.s[i], dxm = a.x2.d[i];
             const size_t q = qxi(ii, j, k, n1, ny), sz = (size_t)n1 * ny;
-            const T *Xp = a.Qn[1][3];
+            gptr<const T> Xp = a.Qn[1][3];
             du += (-((a.Qo[1][0][q] * sx) * sy + ((dxm * sy) * Aav) * bX(Xp, q, sz))) * dt;
             dv += (-((a.Qo[1][1][q] * sx) * sy + ((dym * sx) * Aav) * bY(Xp, q, n1, sz))) * dt;
             dw += (((-a.Qo[1][2][q]) * sx) * sy) * dt;
@@ -630,14 +630,14 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
     if (j < a.J1) ylayer(a.y1.s[j], a.y1.c[j], a.y1.d[j], a.Qn[2][3], qyi(i, j, k, nx, n2), n2);
     if (j >= ny - a.J2) ylayer(a.y2.s[j], a.y2.c[j], a.y2.d[j], a.Qn[3][3], qyi(i, j - (ny - n3), k, nx, n3), n3);
     // z layer with x and y cross terms (air.cpp:1730-1905)
-    auto zlayer = [&](T sz, T cz, T dzm, const T *Qp, size_t qz) {
+    auto zlayer = [&](T sz, T cz, T dzm, gptr<const T> Qp, size_t qz) {
         du += ((-sz) * (uc + Aav * bX(Qp, qz, sxy))) * dt;
         dv += ((-sz) * (vc + Aav * bY(Qp, qz, nx, sxy))) * dt;
         dw += ((cz * Aav) * gpz) * dt - (sz * wc) * dt;
         if (i < a.I1) {
             const T sx = a.x1.s[i], dxm = a.x1.d[i];
             const size_t q = qxi(i, j, k, n0, ny), s0 = (size_t)n0 * ny;
-            const T *Xp = a.Qn[0][3];
+            gptr<const T> Xp = a.Qn[0][3];
             du += (-((a.Qo[0][0][q] * sx) * sz + ((dxm * sz) * Aav) * bX(Xp, q, s0))) * dt;
             dv += (((-a.Qo[0][1][q]) * sx) * sz) * dt;
             dw += (-((a.Qo[0][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - s0])) * dzi))) * dt;
@@ -645,7 +645,7 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
             const int ii = i - (nx - n1);
             const T sx = a.x2.s[i], dxm = a.x2.d[i];
             const size_t q = qxi(ii, j, k, n1, ny), s1 = (size_t)n1 * ny;
-            const T *Xp = a.Qn[1][3];
+            gptr<const T> Xp = a.Qn[1][3];
             du += (-((a.Qo[1][0][q] * sx) * sz + ((dxm * sz) * Aav) * bX(Xp, q, s1))) * dt;
             dv += (((-a.Qo[1][1][q]) * sx) * sz) * dt;
             dw += (-((a.Qo[1][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - s1])) * dzi))) * dt;
@@ -653,7 +653,7 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
         if (j < a.J1) {
             const T sy = a.y1.s[j], dym = a.y1.d[j];
             const size_t q = qyi(i, j, k, nx, n2), s2 = (size_t)nx * n2;
-            const T *Yp = a.Qn[2][3];
+            gptr<const T> Yp = a.Qn[2][3];
             du += (((-a.Qo[2][0][q]) * sy) * sz) * dt;
             dv += (-((a.Qo[2][1][q] * sy) * sz + ((dym * sz) * Aav) * bY(Yp, q, nx, s2))) * dt;
             dw += (-((a.Qo[2][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - s2])) * dzi))) * dt;
@@ -661,7 +661,7 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
             const int jj = j - (ny - n3);
             const T sy = a.y2.s[j], dym = a.y2.d[j];
             const size_t q = qyi(i, jj, k, nx, n3), s3 = (size_t)nx * n3;
-            const T *Yp = a.Qn[3][3];
+            gptr<const T> Yp = a.Qn[3][3];
             du += (((-a.Qo[3][0][q]) * sy) * sz) * dt;
             dv += (-((a.Qo[3][1][q] * sy) * sz + ((dym * sz) * Aav) * bY(Yp, q, nx, s3))) * dt;
             dw += (-((a.Qo[3][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - s3])) * dzi))) * dt;
@@ -695,7 +695,7 @@ __device__ __forceinline__ void air_v_point(const AirArgs<T> &a, int i, int j, i
         // (air.cpp:1907-1945)
         T mX, mY, mZ;
         metric(g, i, j, k, mX, mY, mZ);
-        const T *p = g.pnn;
+        gptr<const T> p = g.pnn;
         const T pc = p[id];
         const size_t sxy = (size_t)g.nx * g.ny;
         const T ib = a.oinvb[mo], f = ((T(-1.0) * ib) * dt) * a.oeff[mo];
@@ -886,7 +886,7 @@ __device__ __forceinline__ T rk_regs(int st, T xn, T k, T acc_in, T err_in, T &a
 // Pointer form of the register update, kept for the porous stage: there, reading each
 // field's registers next to its update measured faster than reading them all first.
 template <typename T, bool COMP>
-__device__ __forceinline__ T rk_update(int st, T xn, T k, T *acc, T *lo, T *err, size_t id)
+__device__ __forceinline__ T rk_update(int st, T xn, T k, gptr<T> acc, gptr<T> lo, gptr<T> err, size_t id)
 {
     if constexpr (!COMP) {
         if (st == 0) {
@@ -929,7 +929,7 @@ __device__ __forceinline__ T rk_update(int st, T xn, T k, T *acc, T *lo, T *err,
 //   stage 3: x_{n+1} = acc + k/6
 // COMP carries the low part and the state error as rk_regs does (the error scaled by E).
 template <typename T, bool COMP>
-__device__ __forceinline__ T rk_update_if(int st, T xn, T k, T *acc, T *lo, T *err, size_t id, const T *cf)
+__device__ __forceinline__ T rk_update_if(int st, T xn, T k, gptr<T> acc, gptr<T> lo, gptr<T> err, size_t id, gptr<const T> cf)
 {
     const T E = cf[0], E2 = cf[1];
     const T w = st == 0 ? cf[3] : st < 3 ? cf[4] : T(1.0 / 6.0);
@@ -1186,7 +1186,7 @@ __global__ __launch_bounds__(BX * BY) void k_pore_v(const PoreArgs<T> a)
     T mX, mY, mZ, zX = T(0), zY = T(0);
     if constexpr (TERRAIN) metric_terrain(g, i, j, k, mX, mY, mZ, zX, zY);
     else metric(g, i, j, k, mX, mY, mZ);
-    const T *p = g.pnn;
+    gptr<const T> p = g.pnn;
     const T pc = p[id];
     const T f = (((T(-1.0) * a.invbeta) * g.dt) * a.eff);
     const T uc = g.un[id], vc = g.vn[id], wc = g.wn[id];
