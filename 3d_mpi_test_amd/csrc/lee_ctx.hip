@@ -1832,7 +1832,7 @@ int lee_ensemble_create(int n, const lee_deck *decks, const lee_options *opts, l
     for (lee_ctx *c : E->m) {
         const Model &m = c->m;
         auto tiles = [](int nx, int ny, int nz) {
-            return (long long)((nx - 2 + kTileX - 1) / kTileX) * ((ny - 2 + kTileY - 1) / kTileY) * (nz - 2);
+            return (long long)((nx + kTileX - 1) / kTileX) * ((ny - 2 + kTileY - 1) / kTileY) * (nz - 2);
         };
         long long ta = tiles(m.nx, m.ny, m.nz);
         long long nb = (c->prec == 64 ? boundary_count<double>(m.nx, m.ny, m.nz) : boundary_count<float>(m.nx, m.ny, m.nz));

@@ -722,16 +722,16 @@ __device__ __forceinline__ void air_v_point(const AirArgs<T> &a, int i, int j, i
 template <typename T, bool WIND, bool TERRAIN>
 __global__ __launch_bounds__(BX * BY) void k_air_p(const AirArgs<T> a)
 {
-    const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
-    if (i >= a.g.nx - 1 || j >= a.g.ny - 1) return;
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    if (i < 1 || i >= a.g.nx - 1 || j >= a.g.ny - 1) return;
     (void)air_p_point<T, WIND, TERRAIN>(a, i, j, k);
 }
 
 template <typename T, bool WIND, bool TERRAIN>
 __global__ __launch_bounds__(BX * BY) void k_air_v(const AirArgs<T> a)
 {
-    const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
-    if (i >= a.g.nx - 1 || j >= a.g.ny - 1) return;
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    if (i < 1 || i >= a.g.nx - 1 || j >= a.g.ny - 1) return;
     air_v_point<T, WIND, TERRAIN>(a, i, j, k);
 }
 
@@ -1020,8 +1020,8 @@ template <typename T, int WM, bool TERRAIN, bool COMP>
 __global__ __launch_bounds__(BX * BY) void k_rk_air(const RKArgs<T> *__restrict__ rp)
 {
     const RKArgs<T> &r = *rp;
-    const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
-    if (i >= r.a.g.nx - 1 || j >= r.a.g.ny - 1) return;
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    if (i < 1 || i >= r.a.g.nx - 1 || j >= r.a.g.ny - 1) return;
     rk_air_point<T, WM, TERRAIN, COMP>(r, i, j, k);
 }
 
@@ -1034,7 +1034,7 @@ __global__ __launch_bounds__(BX * TY) void k_rk_air_t(const RKArgs<T> *__restric
 {
     const RKArgs<T> &r = *rp;
     const int nx = r.a.g.nx, ny = r.a.g.ny, K = r.a.g.nz - 2;
-    const int tx = (nx - 2 + BX - 1) / BX, ty = (ny - 2 + TY - 1) / TY;
+    const int tx = (nx + BX - 1) / BX, ty = (ny - 2 + TY - 1) / TY;
     int bx, by, k;
     if constexpr (COLUMNS) {
         const int t = xcd_tile(blockIdx.x, gridDim.x);
@@ -1048,8 +1048,8 @@ __global__ __launch_bounds__(BX * TY) void k_rk_air_t(const RKArgs<T> *__restric
         by = (b / tx) % ty;
         k = b / (tx * ty) + 1;
     }
-    const int i = bx * BX + threadIdx.x + 1, j = by * TY + threadIdx.y + 1;
-    if (i >= nx - 1 || j >= ny - 1) return;
+    const int i = bx * BX + threadIdx.x, j = by * TY + threadIdx.y + 1;
+    if (i < 1 || i >= nx - 1 || j >= ny - 1) return;
     rk_air_point<T, WM, TERRAIN, COMP>(r, i, j, k);
 }
 
@@ -1073,11 +1073,11 @@ __global__ __launch_bounds__(BX * BY) void k_rk_air_batch(const RKArgs<T> *__res
     const int d = batch_member(first, n, blockIdx.x);
     const RKArgs<T> &r = args[d];
     const int nx = r.a.g.nx, ny = r.a.g.ny;
-    const int tx = (nx - 2 + BX - 1) / BX, ty = (ny - 2 + BY - 1) / BY;
+    const int tx = (nx + BX - 1) / BX, ty = (ny - 2 + BY - 1) / BY;
     const int local = blockIdx.x - first[d];
-    const int i = (local % tx) * BX + threadIdx.x + 1, j = ((local / tx) % ty) * BY + threadIdx.y + 1,
+    const int i = (local % tx) * BX + threadIdx.x, j = ((local / tx) % ty) * BY + threadIdx.y + 1,
               k = local / (tx * ty) + 1;
-    if (i >= nx - 1 || j >= ny - 1) return;
+    if (i < 1 || i >= nx - 1 || j >= ny - 1) return;
     rk_air_point<T, WM, TERRAIN, COMP>(r, i, j, k);
 }
 
@@ -1127,8 +1127,8 @@ __device__ __forceinline__ void rk_pore_point(const RKPoreArgs<T> &r, int i, int
 template <typename T, bool TERRAIN, bool COMP>
 __global__ __launch_bounds__(BX * BY) void k_rk_pore(const RKPoreArgs<T> r)
 {
-    const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
-    if (i >= r.g.nx - 1 || j >= r.g.ny - 1) return;
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    if (i < 1 || i >= r.g.nx - 1 || j >= r.g.ny - 1) return;
     rk_pore_point<T, TERRAIN, COMP>(r, i, j, k);
 }
 
@@ -1139,11 +1139,11 @@ __global__ __launch_bounds__(BX * BY) void k_rk_pore_batch(const RKPoreArgs<T> *
     const int d = batch_member(first, n, blockIdx.x);
     const RKPoreArgs<T> &r = args[d];
     const int nx = r.g.nx, ny = r.g.ny;
-    const int tx = (nx - 2 + BX - 1) / BX, ty = (ny - 2 + BY - 1) / BY;
+    const int tx = (nx + BX - 1) / BX, ty = (ny - 2 + BY - 1) / BY;
     const int local = blockIdx.x - first[d];
-    const int i = (local % tx) * BX + threadIdx.x + 1, j = ((local / tx) % ty) * BY + threadIdx.y + 1,
+    const int i = (local % tx) * BX + threadIdx.x, j = ((local / tx) % ty) * BY + threadIdx.y + 1,
               k = local / (tx * ty) + 1;
-    if (i >= nx - 1 || j >= ny - 1) return;
+    if (i < 1 || i >= nx - 1 || j >= ny - 1) return;
     rk_pore_point<T, TERRAIN, COMP>(r, i, j, k);
 }
 
@@ -1154,9 +1154,9 @@ template <typename T, bool TERRAIN>
 __global__ __launch_bounds__(BX * BY) void k_pore_p(const PoreArgs<T> a)
 {
     const Grid<T> &g = a.g;
-    const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
     const int nx = g.nx, ny = g.ny;
-    if (i >= nx - 1 || j >= ny - 1) return;
+    if (i < 1 || i >= nx - 1 || j >= ny - 1) return;
     const size_t sxy = (size_t)nx * ny;
     const size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
     T mX, mY, mZ, zX = T(0), zY = T(0);
@@ -1178,9 +1178,9 @@ template <typename T, bool TERRAIN>
 __global__ __launch_bounds__(BX * BY) void k_pore_v(const PoreArgs<T> a)
 {
     const Grid<T> &g = a.g;
-    const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
     const int nx = g.nx, ny = g.ny;
-    if (i >= nx - 1 || j >= ny - 1) return;
+    if (i < 1 || i >= nx - 1 || j >= ny - 1) return;
     const size_t sxy = (size_t)nx * ny;
     const size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
     T mX, mY, mZ, zX = T(0), zY = T(0);
@@ -1333,7 +1333,9 @@ __global__ __launch_bounds__(256) void k_copy_words(v4f *dst, const v4f *src, si
     if (t < n4) __builtin_nontemporal_store(__builtin_nontemporal_load(src + t), dst + t);
 }
 
-dim3 interior_grid(int nx, int ny, int nz) { return dim3((nx - 2 + BX - 1) / BX, (ny - 2 + BY - 1) / BY, nz - 2); }
+// Interior tiles: x tiles start at i = 0 so that a tile's rows cover whole 128-byte lines (the
+// face threads i = 0 and i = nx - 1 return at once); y and z start at the first interior plane.
+dim3 interior_grid(int nx, int ny, int nz) { return dim3((nx + BX - 1) / BX, (ny - 2 + BY - 1) / BY, nz - 2); }
 
 unsigned flat_blocks(size_t total)
 {
@@ -1391,7 +1393,7 @@ void launch_rk_air_v(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool ter
     constexpr bool C = std::is_same<T, float>::value;
     const int nx = r.a.g.nx, ny = r.a.g.ny, K = r.a.g.nz - 2;
     const int TYv = kRkTiles[v].ty;
-    const unsigned nb = (unsigned)(((nx - 2 + BX - 1) / BX) * ((ny - 2 + TYv - 1) / TYv) * K);
+    const unsigned nb = (unsigned)(((nx + BX - 1) / BX) * ((ny - 2 + TYv - 1) / TYv) * K);
 #define LEE_RKT(TY, COL)                                                                              \
     do {                                                                                              \
         if (terrain) {                                                                                \
