@@ -200,22 +200,27 @@ template <typename T> __device__ __forceinline__ T v_pore_walk(const BCArgs<T> &
 // without their x edges, then the two z planes without x and y edges.
 __device__ __forceinline__ void boundary_node(long long t64, int nx, int ny, int nz, int &i, int &j, int &k)
 {
-    // 32-bit divisions: the boundary counts fit (lee_create refuses grids whose counts do not)
+    // 32-bit divisions: the boundary counts fit (lee_create refuses grids whose counts do not).
+    // Every branch sets all three coordinates as values: assigning a different output in each
+    // branch let the optimiser merge the stores into one through a selected pointer, which put
+    // i, j, k in private (scratch) memory.
     unsigned t = (unsigned)t64;
     const unsigned uy = (unsigned)ny, ux2 = (unsigned)(nx - 2), uz = (unsigned)nz;
     const unsigned nA = 2u * uy * uz, nB = 2u * ux2 * uz;
+    int ii, jj, kk;
     if (t < nA) {
         const unsigned pl = uy * uz, side = t / pl, r = t - side * pl, q = r / uy;
-        j = (int)(r - q * uy); k = (int)q; i = side ? nx - 1 : 0;
+        ii = side ? nx - 1 : 0; jj = (int)(r - q * uy); kk = (int)q;
     } else if (t < nA + nB) {
         t -= nA;
         const unsigned pl = ux2 * uz, side = t / pl, r = t - side * pl, q = r / ux2;
-        i = 1 + (int)(r - q * ux2); k = (int)q; j = side ? ny - 1 : 0;
+        ii = 1 + (int)(r - q * ux2); jj = side ? ny - 1 : 0; kk = (int)q;
     } else {
         t -= nA + nB;
         const unsigned pl = ux2 * (uy - 2u), side = t / pl, r = t - side * pl, q = r / ux2;
-        i = 1 + (int)(r - q * ux2); j = 1 + (int)q; k = side ? nz - 1 : 0;
+        ii = 1 + (int)(r - q * ux2); jj = 1 + (int)q; kk = side ? nz - 1 : 0;
     }
+    i = ii; j = jj; k = kk;
 }
 
 // Q[f] <- Q[f] + dt*val (new set from old) for every PML slab containing air node (i,j,k)
