@@ -135,11 +135,13 @@ template <typename T> __device__ __forceinline__ double ground_w_of(const BCArgs
     return ((double)g.wn[e] - a.gcpl * ((double)g.pnn[e] - hist)) * a.g_ib / (1.0 + a.g_ib * a.gcpl * a.gzi);
 }
 
+// The ground histories are pole-major, Psi_q of node (i, j) at q * nx * ny + i + nx * j, so
+// that each pole's reads and writes of a wave are contiguous.
 template <typename T> __device__ __forceinline__ double ground_w(const BCArgs<T> &a, int i, int j)
 {
-    const double *P = a.gpsi_old + ((size_t)i + (size_t)a.air.nx * j) * a.gK;
+    const size_t node = (size_t)i + (size_t)a.air.nx * j, nn = (size_t)a.air.nx * a.air.ny;
     double temp_psi = 0;
-    for (int q = 0; q < a.gK; ++q) temp_psi += a.gdecay[q] * P[q];
+    for (int q = 0; q < a.gK; ++q) temp_psi += a.gdecay[q] * a.gpsi_old[q * nn + node];
     return ground_w_of(a, i, j, temp_psi);
 }
 
@@ -277,12 +279,12 @@ __global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
             // (calculation_2D.cpp:1012-1017) with the velocity into the ground, reading the
             // histories once
             const T u = ground_vel(a, 0, i, j), v = ground_vel(a, 1, i, j);
-            const size_t b = ((size_t)i + (size_t)a.air.nx * j) * a.gK;
+            const size_t b = (size_t)i + (size_t)a.air.nx * j, nn = (size_t)a.air.nx * a.air.ny;
             double P[8];
             double temp_psi = 0;
 #pragma unroll
             for (int q = 0; q < 8; ++q)
-                if (q < a.gK) { P[q] = a.gpsi_old[b + q]; temp_psi += a.gdecay[q] * P[q]; }
+                if (q < a.gK) { P[q] = a.gpsi_old[q * nn + b]; temp_psi += a.gdecay[q] * P[q]; }
             const double wd = ground_w_of(a, i, j, temp_psi);
             const T w = (T)wd;
             a.air.unn[id] = u; a.air.vnn[id] = v; a.air.wnn[id] = w;
@@ -292,7 +294,7 @@ __global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
             const double vg = -wd;
 #pragma unroll
             for (int q = 0; q < 8; ++q)
-                if (q < a.gK) a.gpsi_new[b + q] = a.gdecay[q] * P[q] + (a.ggain[q] * vg) * a.gtb;
+                if (q < a.gK) a.gpsi_new[q * nn + b] = a.gdecay[q] * P[q] + (a.ggain[q] * vg) * a.gtb;
             return;
         }
         const T u = v_air_walk<T, true>(a, 0, i, j, k, 6);

@@ -441,6 +441,18 @@ template <typename T> int frame_shift(lee_ctx *c, int N)
 // ---- RK4 (extension): four stages per step; level n stays in the current level buffers
 // (frame shifts, state access and receivers work as for FB), the stage states alternate
 // between the other level buffers (A) and rk_sb (B):  X -> A -> B -> A -> X.
+// Ground impedance histories: pole-major on the device (Psi_q of node n at q * nn + n, so a
+// wave's accesses to one pole are contiguous), node-major in restart files and in
+// lee_get_ground_psi (n * K + q).  to_nodes: device order -> node-major, else the reverse.
+static void ground_psi_layout(const double *src, double *dst, size_t nn, int K, bool to_nodes)
+{
+    for (size_t n = 0; n < nn; ++n)
+        for (int q = 0; q < K; ++q) {
+            if (to_nodes) dst[n * K + q] = src[(size_t)q * nn + n];
+            else dst[(size_t)q * nn + n] = src[n * K + q];
+        }
+}
+
 template <typename T> void rk_stage_buffers(lee_ctx *c, int d, int st, T *src[4], T *dst[4])
 {
     void **lev = d ? c->pore : c->air;
@@ -1273,9 +1285,10 @@ int lee_save_restart(lee_ctx *c, const char *path, int index)
     o << std::endl;
     if (m.ground) {  // ground impedance histories
         const size_t ng = (size_t)nx * m.ny * m.gK;
-        std::vector<double> g(ng);
-        HIPCHK(c, hipMemcpyAsync(g.data(), c->gpsi, ng * 8, hipMemcpyDeviceToHost, c->stream));
+        std::vector<double> d(ng), g(ng);
+        HIPCHK(c, hipMemcpyAsync(d.data(), c->gpsi, ng * 8, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        ground_psi_layout(d.data(), g.data(), (size_t)nx * m.ny, m.gK, true);
         for (size_t q = 0; q < ng; ++q) o << g[q] << " ";
         o << std::endl;
     }
@@ -1367,10 +1380,11 @@ int lee_load_restart(lee_ctx *c, const char *path, int *index)
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (m.ground) {
             const size_t ng = (size_t)nx * m.ny * m.gK;
-            std::vector<double> g(ng);
+            std::vector<double> g(ng), d(ng);
             for (size_t q = 0; q < ng; ++q)
                 if (!(in >> g[q])) return set_err(c, LEE_ERR_IO, "restart file ends inside the ground histories");
-            HIPCHK(c, hipMemcpyAsync(c->gpsi, g.data(), ng * 8, hipMemcpyHostToDevice, c->stream));
+            ground_psi_layout(g.data(), d.data(), (size_t)nx * m.ny, m.gK, false);
+            HIPCHK(c, hipMemcpyAsync(c->gpsi, d.data(), ng * 8, hipMemcpyHostToDevice, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
         }
         c->counter = counter;
@@ -1487,8 +1501,10 @@ int lee_get_ground_psi(lee_ctx *c, double *psi, long long cap)
     if (!c->m.ground) return set_err(c, LEE_ERR_STATE, "no ground impedance in this context");
     size_t n = (size_t)c->m.nx * c->m.ny * c->m.gK;
     if ((long long)n > cap) return set_err(c, LEE_ERR_ARG, "buffer too small");
-    HIPCHK(c, hipMemcpyAsync(psi, c->gpsi, n * 8, hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> d(n);
+    HIPCHK(c, hipMemcpyAsync(d.data(), c->gpsi, n * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    ground_psi_layout(d.data(), psi, (size_t)c->m.nx * c->m.ny, c->m.gK, true);
     return LEE_OK;
 }
 

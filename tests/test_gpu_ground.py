@@ -49,3 +49,29 @@ def test_ground_impedance_fp32(lee):
     print(f"[gpu ground] fp32 vs fp64 receivers {e:.2e}", flush=True)
     a.close(); b.close()
     assert e < 1e-4
+
+
+def test_ground_histories_survive_a_restart(lee, tmp_path):
+    """The restart file carries the ground histories node-major ([i + nx*j][K]) whatever the
+    device layout: a fresh context that loads it holds the saved histories (to the file's ten
+    significant digits) and continues the run."""
+    d = deck(lee, 5)
+    print("[gpu ground] restart", flush=True)
+    a = lee.Solver(d, precision=64, receivers=REC)
+    a.frame_setup(0)
+    a.step(30)
+    path = str(tmp_path / "ct001_0.dat")
+    a.save_restart(path, 30)
+    saved = a.ground_psi(8)
+    rest = a.step(30)
+    b = lee.Solver(d, precision=64, receivers=REC)
+    b.frame_setup(0)
+    assert b.load_restart(path) == 30
+    loaded = b.ground_psi(8)
+    cont = b.step(30)
+    e_psi, e_run = rel(loaded, saved), rel(cont, rest)
+    print(f"  histories {e_psi:.2e} continuation {e_run:.2e}", flush=True)
+    a.close()
+    b.close()
+    assert np.abs(saved).max() > 0
+    assert e_psi <= 1e-9 and e_run <= 1e-6
