@@ -343,7 +343,10 @@ __device__ __forceinline__ void rk_bnd_node(const RKBndArgs<T> &r, long long t)
         boundary_node(t, a.air.nx, a.air.ny, a.air.nz, i, j, k);
         const Grid<T> &g = a.air;
         const size_t id = gidx(g, i, j, k);
-        // reads first (the face walks, the node's stage state), then the stores
+        // every read first -- the face walks, the node's stage state, the auxiliaries of each
+        // PML slab holding the node -- then the stores: a read after a store waits for it (the
+        // arrays may alias as far as the compiler knows), which made each slab a memory round
+        // trip of its own
         const T p = p_air_walk<T, true>(a, i, j, k, 6);
         const T u = v_air_walk<T, true>(a, 0, i, j, k, 6);
         const T v = v_air_walk<T, true>(a, 1, i, j, k, 6);
@@ -351,28 +354,35 @@ __device__ __forceinline__ void rk_bnd_node(const RKBndArgs<T> &r, long long t)
         T sv[4];
 #pragma unroll
         for (int f = 0; f < 4; ++f) sv[f] = r.src[f][id];
-        g.pnn[id] = p; g.unn[id] = u; g.vnn[id] = v; g.wnn[id] = w;
+        bool in[6];
+        size_t qi[6];
+        T qx[6][4], qa[6][4];
+#pragma unroll
         for (int s = 0; s < 6; ++s) {
-            const int n = a.qn[s];
-            if (!n || (s == 2 && !a.y1on)) continue;
-            const int ax = a.qaxis[s], lo = a.qlo[s];
+            const int n = a.qn[s], ax = a.qaxis[s], lo = a.qlo[s];
             const int c = ax == 0 ? i : ax == 1 ? j : k;
-            if (c < lo || c >= lo + n) continue;
+            in[s] = n && !(s == 2 && !a.y1on) && c >= lo && c < lo + n;
             const int A = ax == 0 ? n : g.nx, B = ax == 1 ? n : g.ny;
-            const size_t q = (size_t)(ax == 0 ? i - lo : i) + (size_t)A * ((size_t)(ax == 1 ? j - lo : j) + (size_t)B * (ax == 2 ? k - lo : k));
-            T qx[4], qa[4];
+            qi[s] = (size_t)(ax == 0 ? i - lo : i) + (size_t)A * ((size_t)(ax == 1 ? j - lo : j) + (size_t)B * (ax == 2 ? k - lo : k));
+            if (in[s]) {
 #pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                qx[f] = r.qxn[s][f][q];
-                qa[f] = r.stage != 0 ? r.qacc[s][f][q] : T(0);
-            }
-#pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                T ao;
-                r.qdst[s][f][q] = rk_q_regs<T>(r.stage, qx[f], a.dt * sv[f], qa[f], ao);
-                if (r.stage < 3) r.qacc[s][f][q] = ao;
+                for (int f = 0; f < 4; ++f) {
+                    qx[s][f] = r.qxn[s][f][qi[s]];
+                    qa[s][f] = r.stage != 0 ? r.qacc[s][f][qi[s]] : T(0);
+                }
             }
         }
+        g.pnn[id] = p; g.unn[id] = u; g.vnn[id] = v; g.wnn[id] = w;
+#pragma unroll
+        for (int s = 0; s < 6; ++s)
+            if (in[s]) {
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    T ao;
+                    r.qdst[s][f][qi[s]] = rk_q_regs<T>(r.stage, qx[s][f], a.dt * sv[f], qa[s][f], ao);
+                    if (r.stage < 3) r.qacc[s][f][qi[s]] = ao;
+                }
+            }
     } else if (t < a.nb_air + a.nb_pore) {
         int i, j, k;
         boundary_node(t - a.nb_air, a.pore.nx, a.pore.ny, a.pore.nz, i, j, k);
@@ -387,9 +397,9 @@ __device__ __forceinline__ void rk_bnd_node(const RKBndArgs<T> &r, long long t)
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_rk_bnd(const RKBndArgs<T> r)
+__global__ __launch_bounds__(256) void k_rk_bnd(const RKBndArgs<T> *__restrict__ rp)
 {
-    rk_bnd_node<T>(r, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+    rk_bnd_node<T>(*rp, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Ensemble form: blocks first[d] .. first[d+1]-1 walk member d's boundary nodes.
@@ -408,10 +418,10 @@ __global__ __launch_bounds__(256) void k_rk_bnd_batch(const RKBndArgs<T> *__rest
 
 }  // namespace
 
-template <typename T> void launch_rk_bnd(const RKBndArgs<T> &r, hipStream_t s)
+template <typename T> void launch_rk_bnd(const RKBndArgs<T> &r, const RKBndArgs<T> *rd, hipStream_t s)
 {
     const long long n = r.b.nb_air + r.b.nb_pore;
-    hipLaunchKernelGGL((k_rk_bnd<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, r);
+    hipLaunchKernelGGL((k_rk_bnd<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rd);
 }
 template <typename T> void launch_rk_bnd_batch(const RKBndArgs<T> *args, const int *first, int n, int nblocks, hipStream_t s)
 {
@@ -419,8 +429,8 @@ template <typename T> void launch_rk_bnd_batch(const RKBndArgs<T> *args, const i
 }
 template void launch_rk_bnd_batch<float>(const RKBndArgs<float> *, const int *, int, int, hipStream_t);
 template void launch_rk_bnd_batch<double>(const RKBndArgs<double> *, const int *, int, int, hipStream_t);
-template void launch_rk_bnd<float>(const RKBndArgs<float> &, hipStream_t);
-template void launch_rk_bnd<double>(const RKBndArgs<double> &, hipStream_t);
+template void launch_rk_bnd<float>(const RKBndArgs<float> &, const RKBndArgs<float> *, hipStream_t);
+template void launch_rk_bnd<double>(const RKBndArgs<double> &, const RKBndArgs<double> *, hipStream_t);
 
 template <typename T> long long boundary_count(int nx, int ny, int nz)
 {

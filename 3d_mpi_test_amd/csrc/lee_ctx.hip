@@ -564,13 +564,14 @@ void rk4_stage_args(lee_ctx *c, int st, int J1, int y1on, RKArgs<T> &r, RKPoreAr
 
 int rk_wind_mode(const Model &m) { return m.has_wind ? (m.wind_v_only ? 2 : 1) : 0; }
 
-// Brings the device copy of the four stage arguments up to date for the gate state
-// (J1, y1on).  A change is rare (frame shifts, the y1 gate opening) and waits for the copy.
+// Brings the device copy of the four stage arguments (air stage, then boundary stage: four
+// RKArgs followed by four RKBndArgs) up to date for the gate state (J1, y1on).  A change is rare (frame shifts, the y1 gate opening) and waits for the copy.
 // The arguments depend on the buffer roles (level and auxiliary sets, the FP32 error and
 // second stage buffers a frame shift swaps) and the gate state only: that is the key.
 template <typename T> int rk_args_sync(lee_ctx *c, int J1, int y1on)
 {
-    const size_t sz = sizeof(RKArgs<T>);
+    const size_t sz = sizeof(RKArgs<T>), szb = sizeof(RKBndArgs<T>);
+    static_assert(sizeof(RKArgs<T>) % alignof(RKBndArgs<T>) == 0, "boundary blocks follow the air blocks");
     const uintptr_t key[8] = {(uintptr_t)c->cur, (uintptr_t)c->qcur, (uintptr_t)J1, (uintptr_t)y1on,
                               (uintptr_t)c->rk_sb[0][0], (uintptr_t)c->rk_err[0][0], (uintptr_t)c->rk_sb[1][0],
                               (uintptr_t)c->rk_err[1][0]};
@@ -578,19 +579,21 @@ template <typename T> int rk_args_sync(lee_ctx *c, int J1, int y1on)
     if (c->rk_args_dev && now == c->rk_args_key) return LEE_OK;
     if (c->capturing) return set_err(c, LEE_ERR_STATE, "RK4 stage arguments changed inside a captured chunk");
     if (!c->rk_args_dev) {
-        int rc = dalloc(c, &c->rk_args_dev, 4 * sz);
+        int rc = dalloc(c, &c->rk_args_dev, 4 * (sz + szb));
         if (rc) return rc;
     }
-    c->rk_args_host.assign(4 * sz, 0);
+    c->rk_args_host.assign(4 * (sz + szb), 0);
     for (int st = 0; st < 4; ++st) {
         RKArgs<T> r;
         RKPoreArgs<T> pr;
         RKBndArgs<T> b;
         rk4_stage_args<T>(c, st, J1, y1on, r, pr, b);
         std::memcpy(c->rk_args_host.data() + st * sz, &r, sz);
+        std::memcpy(c->rk_args_host.data() + 4 * sz + st * szb, &b, szb);
     }
     c->rk_args_key.swap(now);
-    HIPCHK(c, hipMemcpyAsync(c->rk_args_dev, c->rk_args_host.data(), 4 * sz, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->rk_args_dev, c->rk_args_host.data(), 4 * (sz + szb), hipMemcpyHostToDevice,
+                             c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return LEE_OK;
 }
@@ -618,6 +621,7 @@ template <typename T> void enqueue_rk4_step(lee_ctx *c, double *rec_slot)
     // receivers sample level n before the step (airpore.cpp:573-580)
     launch_sample<T>((const T *)c->air[c->cur * 4 + 3], c->rec_idx, nr, rec_slot, s);
     const RKArgs<T> *rd = (const RKArgs<T> *)c->rk_args_dev;
+    const RKBndArgs<T> *bd = (const RKBndArgs<T> *)((const unsigned char *)c->rk_args_dev + 4 * sizeof(RKArgs<T>));
     for (int st = 0; st < 4; ++st) {
         RKArgs<T> r;
         RKPoreArgs<T> pr;
@@ -632,7 +636,7 @@ template <typename T> void enqueue_rk4_step(lee_ctx *c, double *rec_slot)
             prof_mark(c, KS_RK_PORE, false);
         }
         prof_mark(c, KS_RK_BND, true);
-        launch_rk_bnd<T>(b, s);
+        launch_rk_bnd<T>(b, bd + st, s);
         prof_mark(c, KS_RK_BND, false);
     }
     inject_source<T>(c, (T *)c->air[c->cur * 4 + 3]);

@@ -147,7 +147,9 @@ constexpr int kRkVariants = 5;
 template <typename T>
 void launch_rk_air_v(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool terrain, int v, hipStream_t s);
 template <typename T> void launch_rk_pore(const RKPoreArgs<T> &r, bool terrain, hipStream_t s);
-template <typename T> void launch_rk_bnd(const RKBndArgs<T> &r, hipStream_t s);
+// r: the stage arguments on the host (node count), rd: the same in device memory (read by the
+// kernel; by value the block was copied to scratch for the slab indexing)
+template <typename T> void launch_rk_bnd(const RKBndArgs<T> &r, const RKBndArgs<T> *rd, hipStream_t s);
 // Ensemble forms: args[d] in device memory for member d, first[d] = its first block
 // (first[n] = nblocks); interior tiles of BX x BY points for air/pore, 256 boundary nodes for bnd.
 constexpr int kTileX = 64, kTileY = 4;
