@@ -31,12 +31,15 @@ def cases(lee):
           time_domain=30, velocity_method=7)
     t.hills, t.wind_ustar, t.wind_z0 = HILLS, 0.4, 0.02
     out["terrain"] = t
+    # x layers of 5 points on a grid 10 points wide: the slabs x1 and x2 share the points i = 4, 5
+    out["overlap"] = B(10, 30, 22, 0.05, pml=5, bc=(2, 1, 2, 2, 2, 2), src=(0.0, 0.7, 0.5), gauss_width=0.25,
+                       time_domain=30)
     for d in out.values():
         d.integrator = 1
     return out
 
 
-@pytest.mark.parametrize("name", ["pml", "porous", "wind5", "terrain"])
+@pytest.mark.parametrize("name", ["pml", "porous", "wind5", "terrain", "overlap"])
 def test_rk4_fp64_parity(lee, orc, name):
     d = cases(lee)[name]
     print(f"[gpu rk4] case={name} shape={d.shape}", flush=True)
