@@ -225,21 +225,41 @@ __device__ __forceinline__ void boundary_node(long long t64, int nx, int ny, int
     i = ii; j = jj; k = kk;
 }
 
-// Q[f] <- Q[f] + dt*val (new set from old) for every PML slab containing air node (i,j,k)
-// (air.cpp:2323-2439).
-template <typename T> __device__ __forceinline__ void q_boundary(const BCArgs<T> &a, int f, int i, int j, int k, T val)
+// Q[f] <- Q[f] + dt*val[f] (new set from old) for the fields f0 .. f0+NF-1 of every PML slab
+// containing air node (i,j,k) (air.cpp:2323-2439), in two phases so that the node's reads all
+// precede its stores (a read after a store waits for it): qb_read before the node's field
+// stores, qb_write after them.
+template <typename T, int NF> struct QBnd {
+    bool in[6];
+    size_t q[6];
+    T o[6][NF];
+};
+template <typename T, int NF>
+__device__ __forceinline__ void qb_read(const BCArgs<T> &a, int f0, int i, int j, int k, QBnd<T, NF> &b)
 {
     const Grid<T> &g = a.air;
+#pragma unroll
     for (int s = 0; s < 6; ++s) {
-        const int n = a.qn[s];
-        if (!n || (s == 2 && !a.y1on)) continue;
-        const int ax = a.qaxis[s], lo = a.qlo[s];
+        const int n = a.qn[s], ax = a.qaxis[s], lo = a.qlo[s];
         const int c = ax == 0 ? i : ax == 1 ? j : k;
-        if (c < lo || c >= lo + n) continue;
+        b.in[s] = n && !(s == 2 && !a.y1on) && c >= lo && c < lo + n;
         const int A = ax == 0 ? n : g.nx, B = ax == 1 ? n : g.ny;
-        const size_t q = (size_t)(ax == 0 ? i - lo : i) + (size_t)A * ((size_t)(ax == 1 ? j - lo : j) + (size_t)B * (ax == 2 ? k - lo : k));
-        a.Qn[s][f][q] = a.Qo[s][f][q] + a.dt * val;
+        b.q[s] = (size_t)(ax == 0 ? i - lo : i) + (size_t)A * ((size_t)(ax == 1 ? j - lo : j) + (size_t)B * (ax == 2 ? k - lo : k));
+        if (b.in[s]) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) b.o[s][f] = a.Qo[s][f0 + f][b.q[s]];
+        }
     }
+}
+template <typename T, int NF>
+__device__ __forceinline__ void qb_write(const BCArgs<T> &a, int f0, const QBnd<T, NF> &b, const T (&val)[NF])
+{
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+        if (b.in[s]) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) a.Qn[s][f0 + f][b.q[s]] = b.o[s][f] + a.dt * val[f];
+        }
 }
 
 template <typename T>
@@ -250,8 +270,11 @@ __global__ __launch_bounds__(256) void k_bc_p_all(const BCArgs<T> a)
         int i, j, k;
         boundary_node(t, a.air.nx, a.air.ny, a.air.nz, i, j, k);
         const T v = p_air_walk<T, true>(a, i, j, k, 6);
+        QBnd<T, 1> qb;
+        qb_read<T, 1>(a, 3, i, j, k, qb);
         a.air.pnn[gidx(a.air, i, j, k)] = v;
-        q_boundary(a, 3, i, j, k, v);
+        const T vals[1] = {v};
+        qb_write<T, 1>(a, 3, qb, vals);
     } else if (t < a.nb_air + a.nb_pore) {
         int i, j, k;
         boundary_node(t - a.nb_air, a.pore.nx, a.pore.ny, a.pore.nz, i, j, k);
@@ -287,10 +310,11 @@ __global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
                 if (q < a.gK) { P[q] = a.gpsi_old[q * nn + b]; temp_psi += a.gdecay[q] * P[q]; }
             const double wd = ground_w_of(a, i, j, temp_psi);
             const T w = (T)wd;
+            QBnd<T, 3> qb;
+            qb_read<T, 3>(a, 0, i, j, k, qb);
             a.air.unn[id] = u; a.air.vnn[id] = v; a.air.wnn[id] = w;
-            q_boundary(a, 0, i, j, k, u);
-            q_boundary(a, 1, i, j, k, v);
-            q_boundary(a, 2, i, j, k, w);
+            const T vals[3] = {u, v, w};
+            qb_write<T, 3>(a, 0, qb, vals);
             const double vg = -wd;
 #pragma unroll
             for (int q = 0; q < 8; ++q)
@@ -300,10 +324,11 @@ __global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
         const T u = v_air_walk<T, true>(a, 0, i, j, k, 6);
         const T v = v_air_walk<T, true>(a, 1, i, j, k, 6);
         const T w = v_air_walk<T, true>(a, 2, i, j, k, 6);
+        QBnd<T, 3> qb;
+        qb_read<T, 3>(a, 0, i, j, k, qb);
         a.air.unn[id] = u; a.air.vnn[id] = v; a.air.wnn[id] = w;
-        q_boundary(a, 0, i, j, k, u);
-        q_boundary(a, 1, i, j, k, v);
-        q_boundary(a, 2, i, j, k, w);
+        const T vals[3] = {u, v, w};
+        qb_write<T, 3>(a, 0, qb, vals);
         if (a.tdbc && j == a.air.ny - 1) {
             // history update with the TDBC velocity of this node (calculation_2D.cpp:1012-1017)
             const double vt = tdbc_v(a, i, k);
