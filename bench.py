@@ -225,7 +225,12 @@ def run_c4(lee, a):
     t0 = time.perf_counter()
     cols = G.Columns()
     train = [gd for gd in G.sample_grounds(2048, 20261203) if G.in_hull(gd)][:256]
-    X, A, E = G.training_set(train, cols)
+    def progress(n, gd, e):
+        if (n + 1) % 16 == 0:
+            print(f"[bench] C4 training data: {n + 1}/{len(train)} grounds, {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
+
+    X, A, E = G.training_set(train, cols, progress=progress)
     t_data = time.perf_counter() - t0
     keep = E <= G.HULL_FIT_R
     lg = G.LearnedGround(seed=1)
