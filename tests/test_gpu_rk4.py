@@ -34,12 +34,15 @@ def cases(lee):
     # x layers of 5 points on a grid 10 points wide: the slabs x1 and x2 share the points i = 4, 5
     out["overlap"] = B(10, 30, 22, 0.05, pml=5, bc=(2, 1, 2, 2, 2, 2), src=(0.0, 0.7, 0.5), gauss_width=0.25,
                        time_domain=30)
+    # 70 points across x: two x tiles of 64 points, the second one partial, over a porous ground
+    out["wide"] = B(70, 24, 16, 0.05, pml=5, bc=(2, 3, 2, 2, 2, 2), porous_cells=6, src=(0.4, 0.6, 0.3),
+                    gauss_width=0.25, time_domain=30)
     for d in out.values():
         d.integrator = 1
     return out
 
 
-@pytest.mark.parametrize("name", ["pml", "porous", "wind5", "terrain", "overlap"])
+@pytest.mark.parametrize("name", ["pml", "porous", "wind5", "terrain", "overlap", "wide"])
 def test_rk4_fp64_parity(lee, orc, name):
     d = cases(lee)[name]
     print(f"[gpu rk4] case={name} shape={d.shape}", flush=True)
@@ -107,6 +110,25 @@ def test_fb_terrain_split_matches_oracle(lee, orc):
     o = orc.Oracle(d, receivers=REC)
     o.frame_setup(0)
     ref = o.step(d.time_domain)
+    e = rel(got, ref)
+    fe = max(rel(x, y) for x, y in zip(s.state(), o.state()))
+    print(f"  rec rel-L2 {e:.3e} field {fe:.3e}", flush=True)
+    s.close()
+    assert e <= 1e-10 and fe <= 1e-10
+
+
+def test_fb_two_x_tiles_match_oracle(lee, orc):
+    """The FB_p_v step on a grid two x tiles wide (64 + 6 points; the tiles start at i = 0)."""
+    d = cases(lee)["wide"]
+    d.integrator = 0
+    print("[gpu] FB two x tiles", flush=True)
+    s = lee.Solver(d, precision=64, receivers=REC)
+    s.frame_setup(0)
+    got = s.step(d.time_domain)
+    o = orc.Oracle(d, receivers=REC)
+    o.frame_setup(0)
+    ref = o.step(d.time_domain)
+    assert np.abs(ref).max() > 0
     e = rel(got, ref)
     fe = max(rel(x, y) for x, y in zip(s.state(), o.state()))
     print(f"  rec rel-L2 {e:.3e} field {fe:.3e}", flush=True)
