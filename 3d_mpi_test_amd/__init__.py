@@ -339,8 +339,14 @@ class Solver:
     def write_contour(self, dirpath: str) -> None:
         _check(lib().lee_write_contour(self.h, dirpath.encode()), self.h)
 
-    def ground_psi(self, K: int) -> np.ndarray:
-        """Ground impedance histories, shape (ny, nx, K)."""
+    def ground_psi(self, K: Optional[int] = None) -> np.ndarray:
+        """Ground impedance histories, shape (ny, nx, K), K = the context's pole count (a
+        different K is an error: the library writes nx*ny*K values node-major)."""
+        have = self._opt.ground_poles if self._opt is not None else 0
+        if K is None:
+            K = have
+        if K != have or K <= 0:
+            raise ValueError(f"the context has {have} ground poles, asked for {K}")
         out = np.zeros((self.ny, self.nx, K))
         _check(lib().lee_get_ground_psi(self.h, _p(out), out.size), self.h)
         return out

@@ -92,7 +92,9 @@ template <typename T> struct BCArgs {
     gptr<const double> psi_old;  // East TDBC histories [i + nx*k][6], read
     gptr<double> psi_new;  // and written (double buffered)
     double decay[6], gain[6], inv, Zinf, tb;
-    // ground impedance on the South face: histories [i + nx*j][gK], FP64, double buffered;
+    // ground impedance on the South face: histories FP64, double buffered, pole-major on the
+    // device (pole q of ground node n = i + nx*j at q*nx*ny + n, so a wave's accesses to one
+    // pole are contiguous; restart files and lee_get_ground_psi use node-major [n][gK]);
     // the ground interface plane k = 0: its metric (gmx[i] = 1/X_over_x, gmy[j], gmz), the
     // porous coefficients of the top material (dt/(Kp porosity), 1/rho_e, 1/(1 + beta)),
     // 2 dt/(rho_e dz) of the half-cell momentum and Zinf (1 + tb sum_k a_k)

@@ -190,11 +190,34 @@ int mlp_alloc(lee_mlp *n, float **p, long count)
     return LEE_OK;
 }
 
+// Give back a buffer of the batch (the stream must be idle).
+void mlp_release(lee_mlp *n, float *&p)
+{
+    if (!p) return;
+    for (auto it = n->allocs.begin(); it != n->allocs.end(); ++it)
+        if (*it == (void *)p) {
+            n->allocs.erase(it);
+            break;
+        }
+    (void)hipFree(p);
+    p = nullptr;
+}
+
+// Batch buffers for `rows` rows: kept while they are large enough; on growth the old set is
+// freed and the new one sized with headroom (1.5x), so a run of growing batches reallocates
+// a few times, not once per call.
 int mlp_batch(lee_mlp *n, int rows)
 {
     if (rows <= n->cap) return LEE_OK;
     int rc;
-    const int R = (rows + 15) / 16 * 16;
+    const long want = n->cap > 0 ? std::max((long)rows, (long)n->cap * 3 / 2) : (long)rows;
+    const int R = (int)((want + 15) / 16 * 16);
+    MHIP(n, hipStreamSynchronize(n->stream));
+    mlp_release(n, n->x); mlp_release(n, n->t); mlp_release(n, n->dy); mlp_release(n, n->loss);
+    for (int l = 0; l < lee_mlp::L; ++l) {
+        mlp_release(n, n->z[l]); mlp_release(n, n->a[l]); mlp_release(n, n->dz[l]); mlp_release(n, n->da[l]);
+    }
+    n->cap = 0;
     if ((rc = mlp_alloc(n, &n->x, (long)R * n->din[0])) || (rc = mlp_alloc(n, &n->t, (long)R * n->n_out)) ||
         (rc = mlp_alloc(n, &n->dy, (long)R * n->dout[lee_mlp::L - 1])) || (rc = mlp_alloc(n, &n->loss, 4096)))
         return rc;

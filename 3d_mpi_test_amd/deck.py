@@ -344,22 +344,38 @@ def uniform_axis(n: int, d: float, start: float = 0.0) -> np.ndarray:
     return start + d * np.arange(n, dtype=np.float64)
 
 
+def stretched_axis(n: int, d: float, ratio: float, start: float = 0.0) -> np.ndarray:
+    """n coordinates whose spacing starts at d and grows geometrically by `ratio` per cell:
+    start + d (ratio^i - 1)/(ratio - 1) (uniform for ratio 1)."""
+    if ratio == 1.0:
+        return uniform_axis(n, d, start)
+    i = np.arange(n, dtype=np.float64)
+    return start + d * (ratio ** i - 1.0) / (ratio - 1.0)
+
+
 def box(nx: int, ny: int, nz: int, d: float = 0.05, *, pml: int = 0, bc=None, dt=None,
         cfl: float = 0.5, c: float = 343.0, gamma: float = 1.4, rho0: float = 1.21,
         porous_cells: int = 0, pore_dz: Optional[float] = None, x0: Optional[float] = None,
-        y0: float = 0.0, z0: float = 0.0, **kw) -> Deck:
-    """A rectangular air box of nx*ny*nz points at spacing d, optional PML on the sides and
-    top (`pml` points each) and an optional porous ground layer of `porous_cells` points.
+        y0: float = 0.0, z0: float = 0.0, spacing: Optional[Sequence[float]] = None,
+        stretch: Optional[Sequence[float]] = None, **kw) -> Deck:
+    """A rectangular air box of nx*ny*nz points, optional PML on the sides and top (`pml`
+    points each) and an optional porous ground layer of `porous_cells` points.
 
-    P0 = rho0*c^2/gamma (so that c^2/(gamma*P0) = 1/rho0), dt = cfl*d/c unless given.
+    The spacing is d on every axis, or spacing = (dx, dy, dz) per axis; stretch = (rx, ry, rz)
+    grows each axis's spacing geometrically from its first cell (the deck's diff_x ... stay the
+    first spacing, as the reference takes the computational step from input.txt and the
+    metric from the coordinate files).  P0 = rho0*c^2/gamma (so that c^2/(gamma*P0) = 1/rho0),
+    dt = cfl*min(spacing)/c unless given.
     """
-    x0 = -0.5 * (nx - 1) * d if x0 is None else x0
-    ax = uniform_axis(nx, d, x0)
-    ay = uniform_axis(ny, d, y0)
-    az = uniform_axis(nz, d, z0)
+    dx, dy, dz = (d, d, d) if spacing is None else (float(spacing[0]), float(spacing[1]), float(spacing[2]))
+    rx, ry, rz = (1.0, 1.0, 1.0) if stretch is None else (float(stretch[0]), float(stretch[1]), float(stretch[2]))
+    x0 = -0.5 * (nx - 1) * dx if x0 is None else x0
+    ax = stretched_axis(nx, dx, rx, x0)
+    ay = stretched_axis(ny, dy, ry, y0)
+    az = stretched_axis(nz, dz, rz, z0)
     Pav = rho0 * c * c / gamma
     if dt is None:
-        dt = cfl * d / c
+        dt = cfl * min(dx, dy, dz) / c
     if bc is None:
         # N S E W F B: absorbing top and sides, ground rigid (or porous)
         bc = (2 if pml else 1, 3 if porous_cells else 1, 2 if pml else 1, 2 if pml else 1,
@@ -368,13 +384,13 @@ def box(nx: int, ny: int, nz: int, d: float = 0.05, *, pml: int = 0, bc=None, dt
                  pml if bc[3] == 2 else 0, pml if bc[2] == 2 else 0,
                  pml if bc[1] == 2 else 0, pml if bc[0] == 2 else 0)
     kwargs = dict(air_x=ax, air_y=ay, air_z=az, pml_cells=pml_cells, bc=tuple(bc), dt=dt,
-                  dx=d, dy=d, dz=d, gamma=gamma, Pav=Pav, c=c)
+                  dx=dx, dy=dy, dz=dz, gamma=gamma, Pav=Pav, c=c)
     if porous_cells:
-        pdz = d if pore_dz is None else pore_dz
+        pdz = dz if pore_dz is None else pore_dz
         # porous column: Kp points ending one plane above the air ground (two planes overlap
         # through the ghost copy of calculation_2D.cpp:878-886)
         pz = z0 + pdz * (np.arange(porous_cells, dtype=np.float64) - (porous_cells - 2))
-        kwargs.update(pore_x=ax.copy(), pore_y=ay.copy(), pore_z=pz, pore_dx=d, pore_dy=d, pore_dz=pdz)
+        kwargs.update(pore_x=ax.copy(), pore_y=ay.copy(), pore_z=pz, pore_dx=dx, pore_dy=dy, pore_dz=pdz)
     kwargs.update(kw)
     return Deck(**kwargs)
 

@@ -102,11 +102,36 @@ def run_gpu(lee, deck, receivers, precision, steps, warmup, record=True):
     return info
 
 
+def run_series(lee, deck, receivers, precision, nsteps):
+    """Receiver series of nsteps steps of `deck` (frames moving as the reference schedules
+    them), untimed: the accuracy pass."""
+    s = lee.Solver(deck, precision=precision, receivers=receivers)
+    out, mf, temp = [], 0, 0
+    left = s.frame_steps(mf, temp)
+    s.frame_setup(mf)
+    n = nsteps
+    while n > 0:
+        if left <= 0:
+            mf += 1
+            left = s.frame_steps(mf, temp)
+            s.frame_setup(mf)
+        k = min(n, left)
+        out.append(s.step(k))
+        left -= k
+        temp += k
+        n -= k
+    s.close()
+    return np.concatenate(out, axis=0)
+
+
 def accuracy(a, b, dt, fmin=50.0, fmax=1000.0):
     """Relative L2 of series b against a, and the largest |IL_b - IL_a| in dB over
-    [fmin, fmax] on receivers that carry signal (bins 60 dB below the band peak skipped)."""
-    rel = float(np.linalg.norm(b - a) / max(np.linalg.norm(a), 1e-300))
+    [fmin, fmax] on receivers that carry signal (bins 60 dB below the band peak skipped).
+    None when no receiver carries signal (nothing was measured)."""
     energy = (a ** 2).sum(axis=0)
+    if not energy.max() > 0:
+        return None
+    rel = float(np.linalg.norm(b - a) / max(np.linalg.norm(a), 1e-300))
     use = energy > 1e-6 * energy.max()
     n = 1 << int(np.ceil(np.log2(a.shape[0])))
     f = np.fft.rfftfreq(n, dt)
@@ -121,10 +146,21 @@ def accuracy(a, b, dt, fmin=50.0, fmax=1000.0):
     return rel, worst, int(use.sum())
 
 
-def cpu_baseline(deck, receivers, budget_s=20.0, threads=None):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(deck, receivers, budget_s=20.0, threads=None, max_steps=10):
     """The FP64 oracle of the same step on the host: the first step(s) of the workload within
-    about `budget_s` seconds, on `threads` host threads (default: OpenMP's count, which
-    OMP_NUM_THREADS sets)."""
+    about `budget_s` seconds (at most max_steps), on `threads` host threads (default:
+    OpenMP's count, which OMP_NUM_THREADS sets)."""
     from oracle import oracle as orc
 
     if threads:
@@ -136,7 +172,7 @@ def cpu_baseline(deck, receivers, budget_s=20.0, threads=None):
     t0 = time.perf_counter()
     o.step(1)
     first = time.perf_counter() - t0
-    more = max(0, min(9, int(budget_s / max(first, 1e-3)) - 1))
+    more = max(0, min(max_steps - 1, int(budget_s / max(first, 1e-3)) - 1))
     t2 = time.perf_counter()
     if more:
         o.step(more)
@@ -180,10 +216,16 @@ def run_c3(lee, a):
     acc = None
     if 32 in out and 64 in out:
         worst_l2, worst_il = 0.0, 0.0
+        measured = 0
         for d in range(len(decks)):
-            rl2, dil, _ = accuracy(out[64]["series"][d], out[32]["series"][d], decks[d].dt)
-            worst_l2, worst_il = max(worst_l2, rl2), max(worst_il, dil)
-        acc = {"fp32_vs_fp64_worst_member_rel_l2": worst_l2, "worst_member_max_dIL_dB_50_1000Hz": round(worst_il, 5)}
+            m = accuracy(out[64]["series"][d], out[32]["series"][d], decks[d].dt)
+            if m is None:
+                continue
+            measured += 1
+            worst_l2, worst_il = max(worst_l2, m[0]), max(worst_il, m[1])
+        if measured:
+            acc = {"fp32_vs_fp64_worst_member_rel_l2": worst_l2,
+                   "worst_member_max_dIL_dB_50_1000Hz": round(worst_il, 5), "members_with_signal": measured}
     line = {
         "metric": "grid-pt updates/s + %HBM BW, FP32 vs FP64; learned-ground speedup @<=0.2 dB",
         "value": round(r["points"] * a.steps / r["wall"], 1),
@@ -323,6 +365,9 @@ def main():
     ap.add_argument("--no-fb", action="store_true", help="skip the reference-scheme (FB_p_v, flat) lines")
     ap.add_argument("--precisions", default="32,64")
     ap.add_argument("--eager", action="store_true", help="C4: launch the column and judge runs eagerly (no graphs)")
+    ap.add_argument("--accuracy-steps", type=int, default=None,
+                    help="C1/C2: steps of the untimed FP32-vs-FP64 accuracy pass when the timed run is shorter "
+                         "(default: the whole workload, 10000 / 2000; 0 skips it)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "rk4_air_c1_traffic.json"),
                     help="JSON {kernel: HBM bytes per launch} from rocprofv3 FETCH_SIZE/WRITE_SIZE runs")
     a = ap.parse_args()
@@ -376,11 +421,23 @@ def main():
             "copy_kernel_GBs": round(2 * (1 << 30) / (r["copy_ms"] / 1e3) / 1e9, 1) if r["copy_ms"] > 0 else None}
     acc = None
     if 32 in res and 64 in res:
-        rl2, dil, nrec = accuracy(res[64]["series"], res[32]["series"], deck.dt)
-        acc = {"fp32_vs_fp64_rel_l2": rl2, "max_dIL_dB_50_1000Hz": round(dil, 5), "receivers_with_signal": nrec,
-               "steps": total}
-        print(f"[bench] FP32 vs FP64: rel-L2 {rl2:.3e}, max |dIL| {dil:.4f} dB over 50-1000 Hz", file=sys.stderr,
-              flush=True)
+        full = {"c1": 10000, "c2": 2000}[a.workload] if a.accuracy_steps is None else a.accuracy_steps
+        ser, steps_acc, src = {p: res[p]["series"] for p in (32, 64)}, total, "the timed runs"
+        if total < full:
+            # the timed run is shorter than the workload: the accuracy comes from an untimed
+            # pass of the whole workload, so the figure is the same whatever --steps is
+            adeck = (lee.deck.config_c1 if a.workload == "c1" else lee.deck.config_c2)(time_domain=full)
+            for p in (32, 64):
+                print(f"[bench] accuracy pass {wl} RK4 fp{p}: {full} untimed steps", file=sys.stderr, flush=True)
+                ser[p] = run_series(lee, adeck, rec, p, full)
+            steps_acc, src = full, "an untimed pass of the whole workload"
+        m = accuracy(ser[64], ser[32], deck.dt)
+        if m is not None:
+            rl2, dil, nrec = m
+            acc = {"fp32_vs_fp64_rel_l2": rl2, "max_dIL_dB_50_1000Hz": round(dil, 5), "receivers_with_signal": nrec,
+                   "receivers": len(rec), "steps": steps_acc, "series_from": src}
+            print(f"[bench] FP32 vs FP64 ({steps_acc} steps, {nrec} receivers with signal): rel-L2 {rl2:.3e}, "
+                  f"max |dIL| {dil:.4f} dB over 50-1000 Hz", file=sys.stderr, flush=True)
     fb = {}
     if not a.no_fb and a.workload == "c1":
         # the reference's own scheme on the same grid: FB_p_v, flat ground, no wind
@@ -391,19 +448,43 @@ def main():
             fb[f"fb_fp{p}_pt_updates_per_s"] = rr["points"] * 200 / rr["wall"]
             fb[f"fb_fp{p}_ms_per_step"] = 1e3 * rr["wall"] / 200
             fb[f"fb_fp{p}_kernels"] = kernel_table(rr["kernels"])
-    cpu = None
+    cpu = cpu_ref = None
     if not a.no_cpu:
         from oracle import oracle as orc
         nthr = orc.get_threads()
-        print(f"[bench] cpu baseline (oracle, {nthr} threads, then 1)", file=sys.stderr, flush=True)
+        model = cpu_model()
+        print(f"[bench] cpu baseline (oracle, {nthr} threads, then 1; {model})", file=sys.stderr, flush=True)
         c = cpu_baseline(deck, rec, threads=nthr)
         c1 = cpu_baseline(deck, rec, budget_s=15.0, threads=1)
-        orc.set_threads(nthr)
         cpu = {"value": round(c["value"], 1), "unit": "grid-pt updates/s", "cores": c["threads"], "kind": "port",
                "sample": f"{wl} (RK4, terrain, porous ground) FP64, first {c['steps']} step(s) "
-                         f"({c['seconds']:.1f} s) on {c['threads']} OpenMP threads",
+                         f"({c['seconds']:.1f} s) on {c['threads']} OpenMP threads of {model}",
                "single_thread_value": round(c1["value"], 1),
                "single_thread_sample": f"first {c1['steps']} step(s) ({c1['seconds']:.1f} s), one thread"}
+        if a.workload == "c1" and not a.no_fb:
+            # the reference's own scheme (FB_p_v) on the CPU, beside the GPU FB lines: the C1
+            # grid flat (a bounded sample), and the 96x64x48 workload in full (2000 steps) on
+            # every thread and a sample of it on one
+            print("[bench] cpu baseline, reference scheme FB_p_v (C1 grid; 96x64x48 x 2000 steps)", file=sys.stderr,
+                  flush=True)
+            fdeck = lee.deck.config_c1(time_domain=220, reference_scheme=True)
+            f_all = cpu_baseline(fdeck, rec, budget_s=12.0, threads=nthr)
+            f_one = cpu_baseline(fdeck, rec, budget_s=10.0, threads=1, max_steps=2)
+            c0 = lee.deck.config_c0(time_domain=2000, reference_scheme=True)
+            c0_all = cpu_baseline(c0, lee.deck.receivers_c0(), budget_s=1e9, threads=nthr, max_steps=2000)
+            c0_one = cpu_baseline(c0, lee.deck.receivers_c0(), budget_s=12.0, threads=1, max_steps=2000)
+            cpu_ref = {"cpu": model, "unit": "grid-pt updates/s", "kind": "port",
+                       "c1_grid_fb": {"threads": f_all["threads"], "value": round(f_all["value"], 1),
+                                      "sample": f"first {f_all['steps']} step(s) ({f_all['seconds']:.1f} s)",
+                                      "single_thread_value": round(f_one["value"], 1),
+                                      "single_thread_sample": f"first {f_one['steps']} step(s) "
+                                                              f"({f_one['seconds']:.1f} s)"},
+                       "c0_fb": {"threads": c0_all["threads"], "value": round(c0_all["value"], 1),
+                                 "steps": c0_all["steps"], "seconds": round(c0_all["seconds"], 2),
+                                 "single_thread_value": round(c0_one["value"], 1),
+                                 "single_thread_sample": f"first {c0_one['steps']} step(s) "
+                                                         f"({c0_one['seconds']:.1f} s)"}}
+        orc.set_threads(nthr)
     extra = {}
     for p, rr in res.items():
         extra[f"fp{p}_pt_updates_per_s"] = rr["points"] * a.steps / rr["wall"]
@@ -431,6 +512,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "cpu_baseline_reference_scheme": cpu_ref,
         "accuracy": acc,
         **extra,
         **fb,

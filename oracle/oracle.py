@@ -99,9 +99,15 @@ class Oracle:
     def __init__(self, deck, receivers=None, fill_nn_nan=False, integrator=None):
         L = lib()
         self.deck = deck
-        self._cdeck = deck.to_c()
-        # the sidecar options (receivers, integrator, terrain, log wind) of the deck
-        self._opt = _deck_module().make_options(deck, receivers=receivers, integrator=integrator)
+        dm = _deck_module()
+        if isinstance(deck, dm.LeeDeck):
+            # a deck parsed from files (input.txt + coordi_*): no sidecar extensions
+            self._cdeck = deck
+            self._opt = dm.make_options(None, receivers=receivers, integrator=integrator)
+        else:
+            self._cdeck = deck.to_c()
+            # the sidecar options (receivers, integrator, terrain, log wind) of the deck
+            self._opt = dm.make_options(deck, receivers=receivers, integrator=integrator)
         h = L.orc_create_opt(ctypes.byref(self._cdeck), ctypes.byref(self._opt), 1 if fill_nn_nan else 0)
         if not h:
             raise ValueError("oracle: " + L.orc_error().decode())

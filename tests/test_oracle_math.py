@@ -124,6 +124,78 @@ def test_tdbc_recursion_invariant(lee):
     assert seen > 0
 
 
+# The six-pole table of the TDBC (calculation_2D.h:73-75): a constant table, the model's data
+TD_A = np.array([8.226223820e-01, 3.732304064e-01, 9.106232372e-02, 4.819445335e-02, 4.225250937e-01,
+                 1.837970986e-01])
+TD_R = np.array([8.681310138e-01, 1.953722814e-01, 1.108067950e-02, 2.566498621e-03, 1.457297589e-04,
+                 4.648583789e-02])
+
+
+def _tdbc_run(lee, sigma, steps):
+    """East-face velocity and histories of an oracle run with the East TDBC, step by step."""
+    dk = lee.deck.box(10, 14, 10, 0.05, bc=(1, 1, 5, 1, 1, 1), src=(0.0, 0.5, 0.25), gauss_width=0.2)
+    dk.pore_q, dk.pore_porosity, dk.pore_resistivity = 3.0 ** 0.5, 0.3, sigma
+    o = O.Oracle(dk)
+    o.frame_setup(0)
+    vs, ps, psis = [], [], []
+    for _ in range(steps):
+        o.step(1)
+        u, v, w, p = o.state()
+        vs.append(v[1:-1, -1, 1:-1].T.copy())  # (i, k) of the face's interior
+        ps.append(p[1:-1, -1, 1:-1].T.copy())
+        psis.append(o.psi()[1:-1, 1:-1, :].transpose(1, 0, 2).copy())  # (i, k, pole)
+    return dk, np.array(vs), np.array(ps), np.array(psis)
+
+
+def _continuous_histories(v, dt, tau, a, r):
+    """The histories of the continuous impedance Z(s) = Z_inf (1 + sum_k a_k/(r_k + s tau)):
+    psi_k(t) = (a_k/tau) int_0^t exp(-r_k (t - t')/tau) v(t') dt', integrated exactly for a
+    velocity held at v_n over (t_{n-1}, t_n]."""
+    x = r * dt / tau
+    decay, gain = np.exp(-x), a / r * -np.expm1(-x)
+    psi = np.zeros(v.shape[1:] + (len(a),))
+    out = []
+    for n in range(v.shape[0]):
+        psi = decay * psi + gain * v[n][..., None]
+        out.append(psi.copy())
+    return np.array(out)
+
+
+def test_tdbc_histories_follow_the_continuous_impedance(lee):
+    """The East TDBC discretises the impedance Z(s) = Z_inf (1 + sum_k a_k/(r_k + s tau)) with
+    Z_inf = rho c q/Omega and tau = gamma rho q^2/(sigma Omega), rho = gamma P0/c^2 and
+    c = 340.29 m/s (airpore.cpp:252-254): p/Z_inf = v + sum_k psi_k with psi_k the memory
+    integral of v.  The oracle's histories must equal the continuous memory integral of its own
+    face velocity up to the discretisation error, which is first order in r_k dt/tau (the
+    recursion's gain a_k dt/tau against the exact a_k/r_k (1 - exp(-r_k dt/tau)), so the
+    relative error per pole is below r_k dt/tau / 2), and halve with dt/tau.  A wrong decay
+    (sign, a missing gamma in tau, dt for dt/tau) or gain fails by orders of magnitude more."""
+    errs = []
+    for sigma in (4000.0, 2000.0):  # tau doubles: dt/tau halves
+        dk, v, p, psi = _tdbc_run(lee, sigma, 60)
+        rho = dk.gamma * dk.Pav / 340.29 ** 2
+        zinf = rho * 340.29 * dk.pore_q / dk.pore_porosity
+        tau = dk.gamma * rho * dk.pore_q ** 2 / sigma / dk.pore_porosity
+        tb = dk.dt / tau
+        assert 1e-3 < tb < 2e-2
+        assert np.abs(v).max() > 1e-6
+        cont = _continuous_histories(v, dk.dt, tau, TD_A, TD_R)
+        # per pole: the relative difference stays inside the first-order bound
+        for q in range(6):
+            e = np.linalg.norm(psi[..., q] - cont[..., q]) / np.linalg.norm(cont[..., q])
+            assert e <= 0.5 * TD_R[q] * tb * 1.02 + 1e-12, (sigma, q, e)
+        errs.append(np.linalg.norm(psi - cont) / np.linalg.norm(cont))
+        # the face relation of the model, with Z_inf from its closed form
+        lhs = v + psi.sum(axis=-1)
+        assert np.abs(lhs - p / zinf).max() <= 1e-12 * np.abs(p / zinf).max()
+        # plausible mistakes in the decay or the time constant are far outside that bound
+        for wrong_tau in (tau / dk.gamma, tau * dk.gamma, 1.0):
+            bad = _continuous_histories(v, dk.dt, wrong_tau, TD_A, TD_R)
+            assert np.linalg.norm(psi - bad) / np.linalg.norm(bad) > 20 * errs[-1]
+    # first order in dt/tau
+    assert 0.4 < errs[1] / errs[0] < 0.6, errs
+
+
 def _interface_pair(lee, Kp=7, Ka=12, n=(14, 16)):
     d = 0.05
     common = dict(bc=(1, 3, 1, 1, 1, 1), gauss_width=0.2, src=(0.05, 0.4, 0.1))
