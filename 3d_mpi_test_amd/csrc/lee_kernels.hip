@@ -255,74 +255,17 @@ __device__ __forceinline__ void load_nb(const AirArgs<T> &a, int i, int j, int k
     }
 }
 
-// ------------------------------------------------------------------------------------
-// Pressure increment dt*R_p at one interior point: air::cal_fp (air.cpp:1953-2322).
-// Reads u, v, w, p of the time level in g.un ... g.pn and the auxiliaries a.Qo.  Terms
-// follow the reference's association; the TERRAIN terms are those of its nine-metric chain
-// (air.cpp:1975-2292) that the separable grid multiplies by zero, inserted in chain order.
-// ------------------------------------------------------------------------------------
-// CEN (the RK4 path): the convective derivatives V d/dY ... are central differences instead
-// of the reference's one-sided pair (backward for p, forward for u v w).  The one-sided pair
-// advects the velocity components across the flow with a downwind difference, whose modes
-// grow at up to 2|V|/dy; the central form is neutral.
-template <typename T, int WM, bool TERRAIN, bool CEN = false>
-__device__ __forceinline__ T air_dp(const AirArgs<T> &a, int i, int j, int k)
+// The layer terms of the pressure increment (air.cpp:2044-2292): the x, y and z slabs with
+// their corner cross terms, added to r in the reference's order.  Reads the auxiliaries a.Qo.
+template <typename T, bool TERRAIN>
+__device__ __forceinline__ void dp_layers(const AirArgs<T> &a, int i, int j, int k, T mX, T mY, T mZ, T zX, T zY, T pc,
+                                          T ddx, T ddy, T dwz, T &r)
 {
     const Grid<T> &g = a.g;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const size_t sxy = (size_t)nx * ny;
-    const size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
-    T mX, mY, mZ, zX = T(0), zY = T(0);
-    if constexpr (TERRAIN) metric_terrain(g, i, j, k, mX, mY, mZ, zX, zY);
-    else metric(g, i, j, k, mX, mY, mZ);
     const T dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, dt = g.dt, gP = a.gP;
-    const T pc = g.pn[id];
-    T ddx, ddy, dwz, r;
-    if constexpr (WM != 1) {
-        Nb<T> n;
-        load_nb<T, WM, TERRAIN, CEN, true, false>(a, i, j, k, n);
-        r = dp_core<T, WM, TERRAIN, CEN>(a, n, ddx, ddy, dwz);
-    } else {
-        const T dux = g.un[id + 1] - g.un[id];
-        const T dvy = g.vn[id + nx] - g.vn[id];
-        dwz = g.wn[id + sxy] - g.wn[id];
-        T duz = T(0), dvz = T(0);
-        T div;
-        if constexpr (TERRAIN) {
-            duz = g.un[id + sxy] - g.un[id];
-            dvz = g.vn[id + sxy] - g.vn[id];
-            div = (((((mX * dux) * dxi) + ((zX * duz) * dzi)) + ((mY * dvy) * dyi)) + ((zY * dvz) * dzi)) + ((mZ * dwz) * dzi);
-        } else {
-            div = (((mX * dux) * dxi) + ((mY * dvy) * dyi)) + ((mZ * dwz) * dzi);
-        }
-        // d/dX of u and d/dY of v (the damped terms of the x and y layers)
-        ddx = (mX * dux) * dxi;
-        ddy = (mY * dvy) * dyi;
-        if constexpr (TERRAIN) {
-            ddx = ddx + (zX * duz) * dzi;
-            ddy = ddy + (zY * dvz) * dzi;
-        }
-            const T Uc = a.U[id], Vc = a.V[id], Wc = a.W[id];
-            const T c1 = ((Uc * mX) * dxi) * dt, c2 = ((Vc * mY) * dyi) * dt;
-            T c3;
-            T UX = ((mX * (a.U[id + 1] - a.U[id - 1])) * dxi) * T(0.5);
-            T VY = ((mY * (a.V[id + nx] - a.V[id - nx])) * dyi) * T(0.5);
-            const T WZ = ((mZ * (a.W[id + sxy] - a.W[id - sxy])) * dzi) * T(0.5);
-            if constexpr (TERRAIN) {
-                c3 = ((((Uc * zX) + (Vc * zY)) + (Wc * mZ)) * dzi) * dt;
-                UX = UX + ((zX * (a.U[id + sxy] - a.U[id - sxy])) * dzi) * T(0.5);
-                VY = VY + ((zY * (a.V[id + sxy] - a.V[id - sxy])) * dzi) * T(0.5);
-            } else {
-                c3 = ((Wc * mZ) * dzi) * dt;
-            }
-            if constexpr (CEN)
-                r = ((-c1) * ((g.pn[id + 1] - g.pn[id - 1]) * T(0.5)) - c2 * ((g.pn[id + nx] - g.pn[id - nx]) * T(0.5))) -
-                    c3 * ((g.pn[id + sxy] - g.pn[id - sxy]) * T(0.5));
-            else
-                r = ((-c1) * (pc - g.pn[id - 1]) - c2 * (pc - g.pn[id - nx])) - c3 * (pc - g.pn[id - sxy]);
-            r += (((-a.gam) * pc) * dt) * ((UX + VY) + WZ);
-            r += (((-a.gam) * a.Pav) * dt) * div;
-    }
+    (void)sxy;
     const int n0 = a.qn[0], n1 = a.qn[1], n2 = a.qn[2], n3 = a.qn[3], n4 = a.qn[4], n5 = a.qn[5];
     // forward differences of a slab field along y or z with the metric, plus the cross-metric
     // z term over terrain: X-slab (stride n*ny per plane), Y-slab (nx*n), Z-slab (nx*ny)
@@ -457,6 +400,187 @@ __device__ __forceinline__ T air_dp(const AirArgs<T> &a, int i, int j, int k)
         ycross_z(sz, a.z2.d[k]);
     }
     (void)n4;
+}
+
+// The layer terms of the velocity increments (air.cpp:1579-1905): the x, y and z slabs with
+// their corner cross terms, added to du, dv, dw in the reference's order.  Reads the pressure
+// auxiliaries a.Qn[.][3] and the velocity auxiliaries a.Qo.
+template <typename T, bool TERRAIN>
+__device__ __forceinline__ void duvw_layers(const AirArgs<T> &a, int i, int j, int k, T mX, T mY, T mZ, T zX, T zY,
+                                            T uc, T vc, T wc, T gpx, T gpy, T gpz, T &du, T &dv, T &dw)
+{
+    const Grid<T> &g = a.g;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const size_t sxy = (size_t)nx * ny;
+    const T dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, dt = g.dt, Aav = a.Aav;
+    (void)sxy;
+    const int n0 = a.qn[0], n1 = a.qn[1], n2 = a.qn[2], n3 = a.qn[3], n5 = a.qn[5];
+    // backward differences of a slab field with the metric: d/dX (x and cross-z), d/dY
+    // (y and cross-z); `sy`, `sz` are the slab strides in y and z
+    auto bX = [&](gptr<const T> Q, size_t q, size_t sz) {
+        T v = (mX * (Q[q] - Q[q - 1])) * dxi;
+        if constexpr (TERRAIN) v = v + (zX * (Q[q] - Q[q - sz])) * dzi;
+        return v;
+    };
+    auto bY = [&](gptr<const T> Q, size_t q, size_t sy, size_t sz) {
+        T v = (mY * (Q[q] - Q[q - sy])) * dyi;
+        if constexpr (TERRAIN) v = v + (zY * (Q[q] - Q[q - sz])) * dzi;
+        return v;
+    };
+    // X1 / X2 (air.cpp:1579-1619)
+    if (i < a.I1) {
+        const T sx = a.x1.s[i], cx = a.x1.c[i];
+        const size_t q = qxi(i, j, k, n0, ny), sz = (size_t)n0 * ny;
+        gptr<const T> Qp = a.Qn[0][3];
+        du += ((-sx) * uc) * dt + (cx * (Aav * gpx)) * dt;
+        dv += ((-sx) * (vc + Aav * bY(Qp, q, n0, sz))) * dt;
+        dw += ((-sx) * (wc + Aav * ((mZ * (Qp[q] - Qp[q - sz])) * dzi))) * dt;
+    }
+    if (i >= nx - a.I2) {
+        const int ii = i - (nx - n1);
+        const T sx = a.x2.s[i], cx = a.x2.c[i];
+        const size_t q = qxi(ii, j, k, n1, ny), sz = (size_t)n1 * ny;
+        gptr<const T> Qp = a.Qn[1][3];
+        du += ((-sx) * uc) * dt + (cx * (Aav * gpx)) * dt;
+        dv += ((-sx) * (vc + Aav * bY(Qp, q, n1, sz))) * dt;
+        dw += ((-sx) * (wc + Aav * ((mZ * (Qp[q] - Qp[q - sz])) * dzi))) * dt;
+    }
+    // y layer with x cross terms (air.cpp:1621-1728)
+    auto ylayer = [&](T sy, T cy, T dym, gptr<const T> Qp, size_t qy, int nq) {
+        const size_t szy = (size_t)nx * nq;
+        du += ((-sy) * (uc + Aav * bX(Qp, qy, szy))) * dt;
+        dv += ((cy * Aav) * gpy) * dt - (sy * vc) * dt;
+        dw += ((-sy) * (wc + Aav * ((mZ * (Qp[qy] - Qp[qy - szy])) * dzi))) * dt;
+        if (i < a.I1) {
+            const T sx = a.x1.s[i], dxm = a.x1.d[i];
+            const size_t q = qxi(i, j, k, n0, ny), sz = (size_t)n0 * ny;
+            gptr<const T> Xp = a.Qn[0][3];
+            du += (-((a.Qo[0][0][q] * sx) * sy + ((dxm * sy) * Aav) * bX(Xp, q, sz))) * dt;
+            dv += (-((a.Qo[0][1][q] * sx) * sy + ((dym * sx) * Aav) * bY(Xp, q, n0, sz))) * dt;
+            dw += (((-a.Qo[0][2][q]) * sx) * sy) * dt;
+        } else if (i >= nx - a.I2) {
+            const int ii = i - (nx - n1);
+            const T sx = a.x2.s[i], dxm = a.x2.d[i];
+            const size_t q = qxi(ii, j, k, n1, ny), sz = (size_t)n1 * ny;
+            gptr<const T> Xp = a.Qn[1][3];
+            du += (-((a.Qo[1][0][q] * sx) * sy + ((dxm * sy) * Aav) * bX(Xp, q, sz))) * dt;
+            dv += (-((a.Qo[1][1][q] * sx) * sy + ((dym * sx) * Aav) * bY(Xp, q, n1, sz))) * dt;
+            dw += (((-a.Qo[1][2][q]) * sx) * sy) * dt;
+        }
+    };
+    if (j < a.J1) ylayer(a.y1.s[j], a.y1.c[j], a.y1.d[j], a.Qn[2][3], qyi(i, j, k, nx, n2), n2);
+    if (j >= ny - a.J2) ylayer(a.y2.s[j], a.y2.c[j], a.y2.d[j], a.Qn[3][3], qyi(i, j - (ny - n3), k, nx, n3), n3);
+    // z layer with x and y cross terms (air.cpp:1730-1905)
+    auto zlayer = [&](T sz, T cz, T dzm, gptr<const T> Qp, size_t qz) {
+        du += ((-sz) * (uc + Aav * bX(Qp, qz, sxy))) * dt;
+        dv += ((-sz) * (vc + Aav * bY(Qp, qz, nx, sxy))) * dt;
+        dw += ((cz * Aav) * gpz) * dt - (sz * wc) * dt;
+        if (i < a.I1) {
+            const T sx = a.x1.s[i], dxm = a.x1.d[i];
+            const size_t q = qxi(i, j, k, n0, ny), s0 = (size_t)n0 * ny;
+            gptr<const T> Xp = a.Qn[0][3];
+            du += (-((a.Qo[0][0][q] * sx) * sz + ((dxm * sz) * Aav) * bX(Xp, q, s0))) * dt;
+            dv += (((-a.Qo[0][1][q]) * sx) * sz) * dt;
+            dw += (-((a.Qo[0][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - s0])) * dzi))) * dt;
+        } else if (i >= nx - a.I2) {
+            const int ii = i - (nx - n1);
+            const T sx = a.x2.s[i], dxm = a.x2.d[i];
+            const size_t q = qxi(ii, j, k, n1, ny), s1 = (size_t)n1 * ny;
+            gptr<const T> Xp = a.Qn[1][3];
+            du += (-((a.Qo[1][0][q] * sx) * sz + ((dxm * sz) * Aav) * bX(Xp, q, s1))) * dt;
+            dv += (((-a.Qo[1][1][q]) * sx) * sz) * dt;
+            dw += (-((a.Qo[1][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - s1])) * dzi))) * dt;
+        }
+        if (j < a.J1) {
+            const T sy = a.y1.s[j], dym = a.y1.d[j];
+            const size_t q = qyi(i, j, k, nx, n2), s2 = (size_t)nx * n2;
+            gptr<const T> Yp = a.Qn[2][3];
+            du += (((-a.Qo[2][0][q]) * sy) * sz) * dt;
+            dv += (-((a.Qo[2][1][q] * sy) * sz + ((dym * sz) * Aav) * bY(Yp, q, nx, s2))) * dt;
+            dw += (-((a.Qo[2][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - s2])) * dzi))) * dt;
+        } else if (j >= ny - a.J2) {
+            const int jj = j - (ny - n3);
+            const T sy = a.y2.s[j], dym = a.y2.d[j];
+            const size_t q = qyi(i, jj, k, nx, n3), s3 = (size_t)nx * n3;
+            gptr<const T> Yp = a.Qn[3][3];
+            du += (((-a.Qo[3][0][q]) * sy) * sz) * dt;
+            dv += (-((a.Qo[3][1][q] * sy) * sz + ((dym * sz) * Aav) * bY(Yp, q, nx, s3))) * dt;
+            dw += (-((a.Qo[3][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - s3])) * dzi))) * dt;
+        }
+    };
+    if (k < a.K1) zlayer(a.z1.s[k], a.z1.c[k], a.z1.d[k], a.Qn[4][3], qzi(i, j, k, nx, ny));
+    if (k >= nz - a.K2) zlayer(a.z2.s[k], a.z2.c[k], a.z2.d[k], a.Qn[5][3], qzi(i, j, k - (nz - n5), nx, ny));
+}
+
+// ------------------------------------------------------------------------------------
+// Pressure increment dt*R_p at one interior point: air::cal_fp (air.cpp:1953-2322).
+// Reads u, v, w, p of the time level in g.un ... g.pn and the auxiliaries a.Qo.  Terms
+// follow the reference's association; the TERRAIN terms are those of its nine-metric chain
+// (air.cpp:1975-2292) that the separable grid multiplies by zero, inserted in chain order.
+// ------------------------------------------------------------------------------------
+// CEN (the RK4 path): the convective derivatives V d/dY ... are central differences instead
+// of the reference's one-sided pair (backward for p, forward for u v w).  The one-sided pair
+// advects the velocity components across the flow with a downwind difference, whose modes
+// grow at up to 2|V|/dy; the central form is neutral.
+template <typename T, int WM, bool TERRAIN, bool CEN = false>
+__device__ __forceinline__ T air_dp(const AirArgs<T> &a, int i, int j, int k)
+{
+    const Grid<T> &g = a.g;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const size_t sxy = (size_t)nx * ny;
+    const size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+    T mX, mY, mZ, zX = T(0), zY = T(0);
+    if constexpr (TERRAIN) metric_terrain(g, i, j, k, mX, mY, mZ, zX, zY);
+    else metric(g, i, j, k, mX, mY, mZ);
+    const T dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, dt = g.dt, gP = a.gP;
+    const T pc = g.pn[id];
+    T ddx, ddy, dwz, r;
+    if constexpr (WM != 1) {
+        Nb<T> n;
+        load_nb<T, WM, TERRAIN, CEN, true, false>(a, i, j, k, n);
+        r = dp_core<T, WM, TERRAIN, CEN>(a, n, ddx, ddy, dwz);
+    } else {
+        const T dux = g.un[id + 1] - g.un[id];
+        const T dvy = g.vn[id + nx] - g.vn[id];
+        dwz = g.wn[id + sxy] - g.wn[id];
+        T duz = T(0), dvz = T(0);
+        T div;
+        if constexpr (TERRAIN) {
+            duz = g.un[id + sxy] - g.un[id];
+            dvz = g.vn[id + sxy] - g.vn[id];
+            div = (((((mX * dux) * dxi) + ((zX * duz) * dzi)) + ((mY * dvy) * dyi)) + ((zY * dvz) * dzi)) + ((mZ * dwz) * dzi);
+        } else {
+            div = (((mX * dux) * dxi) + ((mY * dvy) * dyi)) + ((mZ * dwz) * dzi);
+        }
+        // d/dX of u and d/dY of v (the damped terms of the x and y layers)
+        ddx = (mX * dux) * dxi;
+        ddy = (mY * dvy) * dyi;
+        if constexpr (TERRAIN) {
+            ddx = ddx + (zX * duz) * dzi;
+            ddy = ddy + (zY * dvz) * dzi;
+        }
+            const T Uc = a.U[id], Vc = a.V[id], Wc = a.W[id];
+            const T c1 = ((Uc * mX) * dxi) * dt, c2 = ((Vc * mY) * dyi) * dt;
+            T c3;
+            T UX = ((mX * (a.U[id + 1] - a.U[id - 1])) * dxi) * T(0.5);
+            T VY = ((mY * (a.V[id + nx] - a.V[id - nx])) * dyi) * T(0.5);
+            const T WZ = ((mZ * (a.W[id + sxy] - a.W[id - sxy])) * dzi) * T(0.5);
+            if constexpr (TERRAIN) {
+                c3 = ((((Uc * zX) + (Vc * zY)) + (Wc * mZ)) * dzi) * dt;
+                UX = UX + ((zX * (a.U[id + sxy] - a.U[id - sxy])) * dzi) * T(0.5);
+                VY = VY + ((zY * (a.V[id + sxy] - a.V[id - sxy])) * dzi) * T(0.5);
+            } else {
+                c3 = ((Wc * mZ) * dzi) * dt;
+            }
+            if constexpr (CEN)
+                r = ((-c1) * ((g.pn[id + 1] - g.pn[id - 1]) * T(0.5)) - c2 * ((g.pn[id + nx] - g.pn[id - nx]) * T(0.5))) -
+                    c3 * ((g.pn[id + sxy] - g.pn[id - sxy]) * T(0.5));
+            else
+                r = ((-c1) * (pc - g.pn[id - 1]) - c2 * (pc - g.pn[id - nx])) - c3 * (pc - g.pn[id - sxy]);
+            r += (((-a.gam) * pc) * dt) * ((UX + VY) + WZ);
+            r += (((-a.gam) * a.Pav) * dt) * div;
+    }
+    dp_layers<T, TERRAIN>(a, i, j, k, mX, mY, mZ, zX, zY, pc, ddx, ddy, dwz, r);
     return r;
 }
 
@@ -573,102 +697,7 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
             dw += (((-uc) * WX) * dt - (vc * WY) * dt) - (wc * WZ) * dt;
             dw += ((-Aav) * dt) * gpz;
     }
-    const int n0 = a.qn[0], n1 = a.qn[1], n2 = a.qn[2], n3 = a.qn[3], n5 = a.qn[5];
-    // backward differences of a slab field with the metric: d/dX (x and cross-z), d/dY
-    // (y and cross-z); `sy`, `sz` are the slab strides in y and z
-    auto bX = [&](gptr<const T> Q, size_t q, size_t sz) {
-        T v = (mX * (Q[q] - Q[q - 1])) * dxi;
-        if constexpr (TERRAIN) v = v + (zX * (Q[q] - Q[q - sz])) * dzi;
-        return v;
-    };
-    auto bY = [&](gptr<const T> Q, size_t q, size_t sy, size_t sz) {
-        T v = (mY * (Q[q] - Q[q - sy])) * dyi;
-        if constexpr (TERRAIN) v = v + (zY * (Q[q] - Q[q - sz])) * dzi;
-        return v;
-    };
-    // X1 / X2 (air.cpp:1579-1619)
-    if (i < a.I1) {
-        const T sx = a.x1.s[i], cx = a.x1.c[i];
-        const size_t q = qxi(i, j, k, n0, ny), sz = (size_t)n0 * ny;
-        gptr<const T> Qp = a.Qn[0][3];
-        du += ((-sx) * uc) * dt + (cx * (Aav * gpx)) * dt;
-        dv += ((-sx) * (vc + Aav * bY(Qp, q, n0, sz))) * dt;
-        dw += ((-sx) * (wc + Aav * ((mZ * (Qp[q] - Qp[q - sz])) * dzi))) * dt;
-    }
-    if (i >= nx - a.I2) {
-        const int ii = i - (nx - n1);
-        const T sx = a.x2.s[i], cx = a.x2.c[i];
-        const size_t q = qxi(ii, j, k, n1, ny), sz = (size_t)n1 * ny;
-        gptr<const T> Qp = a.Qn[1][3];
-        du += ((-sx) * uc) * dt + (cx * (Aav * gpx)) * dt;
-        dv += ((-sx) * (vc + Aav * bY(Qp, q, n1, sz))) * dt;
-        dw += ((-sx) * (wc + Aav * ((mZ * (Qp[q] - Qp[q - sz])) * dzi))) * dt;
-    }
-    // y layer with x cross terms (air.cpp:1621-1728)
-    auto ylayer = [&](T sy, T cy, T dym, gptr<const T> Qp, size_t qy, int nq) {
-        const size_t szy = (size_t)nx * nq;
-        du += ((-sy) * (uc + Aav * bX(Qp, qy, szy))) * dt;
-        dv += ((cy * Aav) * gpy) * dt - (sy * vc) * dt;
-        dw += ((-sy) * (wc + Aav * ((mZ * (Qp[qy] - Qp[qy - szy])) * dzi))) * dt;
-        if (i < a.I1) {
-            const T sx = a.x1.s[i], dxm = a.x1.d[i];
-            const size_t q = qxi(i, j, k, n0, ny), sz = (size_t)n0 * ny;
-            gptr<const T> Xp = a.Qn[0][3];
-            du += (-((a.Qo[0][0][q] * sx) * sy + ((dxm * sy) * Aav) * bX(Xp, q, sz))) * dt;
-            dv += (-((a.Qo[0][1][q] * sx) * sy + ((dym * sx) * Aav) * bY(Xp, q, n0, sz))) * dt;
-            dw += (((-a.Qo[0][2][q]) * sx) * sy) * dt;
-        } else if (i >= nx - a.I2) {
-            const int ii = i - (nx - n1);
-            const T sx = a.x2.s[i], dxm = a.x2.d[i];
-            const size_t q = qxi(ii, j, k, n1, ny), sz = (size_t)n1 * ny;
-            gptr<const T> Xp = a.Qn[1][3];
-            du += (-((a.Qo[1][0][q] * sx) * sy + ((dxm * sy) * Aav) * bX(Xp, q, sz))) * dt;
-            dv += (-((a.Qo[1][1][q] * sx) * sy + ((dym * sx) * Aav) * bY(Xp, q, n1, sz))) * dt;
-            dw += (((-a.Qo[1][2][q]) * sx) * sy) * dt;
-        }
-    };
-    if (j < a.J1) ylayer(a.y1.s[j], a.y1.c[j], a.y1.d[j], a.Qn[2][3], qyi(i, j, k, nx, n2), n2);
-    if (j >= ny - a.J2) ylayer(a.y2.s[j], a.y2.c[j], a.y2.d[j], a.Qn[3][3], qyi(i, j - (ny - n3), k, nx, n3), n3);
-    // z layer with x and y cross terms (air.cpp:1730-1905)
-    auto zlayer = [&](T sz, T cz, T dzm, gptr<const T> Qp, size_t qz) {
-        du += ((-sz) * (uc + Aav * bX(Qp, qz, sxy))) * dt;
-        dv += ((-sz) * (vc + Aav * bY(Qp, qz, nx, sxy))) * dt;
-        dw += ((cz * Aav) * gpz) * dt - (sz * wc) * dt;
-        if (i < a.I1) {
-            const T sx = a.x1.s[i], dxm = a.x1.d[i];
-            const size_t q = qxi(i, j, k, n0, ny), s0 = (size_t)n0 * ny;
-            gptr<const T> Xp = a.Qn[0][3];
-            du += (-((a.Qo[0][0][q] * sx) * sz + ((dxm * sz) * Aav) * bX(Xp, q, s0))) * dt;
-            dv += (((-a.Qo[0][1][q]) * sx) * sz) * dt;
-            dw += (-((a.Qo[0][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - s0])) * dzi))) * dt;
-        } else if (i >= nx - a.I2) {
-            const int ii = i - (nx - n1);
-            const T sx = a.x2.s[i], dxm = a.x2.d[i];
-            const size_t q = qxi(ii, j, k, n1, ny), s1 = (size_t)n1 * ny;
-            gptr<const T> Xp = a.Qn[1][3];
-            du += (-((a.Qo[1][0][q] * sx) * sz + ((dxm * sz) * Aav) * bX(Xp, q, s1))) * dt;
-            dv += (((-a.Qo[1][1][q]) * sx) * sz) * dt;
-            dw += (-((a.Qo[1][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - s1])) * dzi))) * dt;
-        }
-        if (j < a.J1) {
-            const T sy = a.y1.s[j], dym = a.y1.d[j];
-            const size_t q = qyi(i, j, k, nx, n2), s2 = (size_t)nx * n2;
-            gptr<const T> Yp = a.Qn[2][3];
-            du += (((-a.Qo[2][0][q]) * sy) * sz) * dt;
-            dv += (-((a.Qo[2][1][q] * sy) * sz + ((dym * sz) * Aav) * bY(Yp, q, nx, s2))) * dt;
-            dw += (-((a.Qo[2][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - s2])) * dzi))) * dt;
-        } else if (j >= ny - a.J2) {
-            const int jj = j - (ny - n3);
-            const T sy = a.y2.s[j], dym = a.y2.d[j];
-            const size_t q = qyi(i, jj, k, nx, n3), s3 = (size_t)nx * n3;
-            gptr<const T> Yp = a.Qn[3][3];
-            du += (((-a.Qo[3][0][q]) * sy) * sz) * dt;
-            dv += (-((a.Qo[3][1][q] * sy) * sz + ((dym * sz) * Aav) * bY(Yp, q, nx, s3))) * dt;
-            dw += (-((a.Qo[3][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - s3])) * dzi))) * dt;
-        }
-    };
-    if (k < a.K1) zlayer(a.z1.s[k], a.z1.c[k], a.z1.d[k], a.Qn[4][3], qzi(i, j, k, nx, ny));
-    if (k >= nz - a.K2) zlayer(a.z2.s[k], a.z2.c[k], a.z2.d[k], a.Qn[5][3], qzi(i, j, k - (nz - n5), nx, ny));
+    duvw_layers<T, TERRAIN>(a, i, j, k, mX, mY, mZ, zX, zY, uc, vc, wc, gpx, gpy, gpz, du, dv, dw);
     du_ = du; dv_ = dv; dw_ = dw;
 }
 
@@ -733,6 +762,169 @@ __global__ __launch_bounds__(BX * BY) void k_air_v(const AirArgs<T> a)
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
     if (i < 1 || i >= a.g.nx - 1 || j >= a.g.ny - 1) return;
     air_v_point<T, WIND, TERRAIN>(a, i, j, k);
+}
+
+// ------------------------------------------------------------------------------------
+// The two phases with 16-byte accesses: lane l of a wavefront holds the VW = 16/sizeof(T)
+// consecutive x points i0 = VW*(64*bx + l) ... i0 + VW - 1 of one row (4 in FP32, 2 in
+// FP64), so every field access of a wavefront is one 1 KB segment loaded or stored by one
+// instruction per lane instead of VW.  A lane's points are computed one after the other with
+// the same per-point arithmetic as k_air_p / k_air_v (dp_core / duvw_core and the layer
+// terms), so both give the same bits.  The x neighbour outside a lane's vector (u at i0 + VW,
+// p at i0 - 1) is one 4-byte read of a line the wavefront loads anyway.  For a flat grid
+// without mean flow or obstacle, nx a multiple of VW (rows start on 16-byte boundaries).
+// ------------------------------------------------------------------------------------
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using type = f32x4; };
+template <> struct Vec16<double> { using type = f64x2; };
+
+template <typename T> __device__ __forceinline__ void ld16(gptr<const T> p, size_t id, T (&x)[16 / sizeof(T)])
+{
+    using V = typename Vec16<T>::type;
+    const V v = *(const __attribute__((address_space(1))) V *)(p.p + id);
+    __builtin_memcpy(x, &v, 16);
+}
+template <typename T> __device__ __forceinline__ void st16(gptr<T> p, size_t id, const T (&x)[16 / sizeof(T)])
+{
+    using V = typename Vec16<T>::type;
+    V v;
+    __builtin_memcpy(&v, x, 16);
+    *(__attribute__((address_space(1))) V *)(p.p + id) = v;
+}
+
+// Whether (i, j, k) lies in a layer whose terms air_dp / air_duvw add.
+template <typename T> __device__ __forceinline__ bool in_layers(const AirArgs<T> &a, int i, int j, int k)
+{
+    return i < a.I1 || i >= a.g.nx - a.I2 || j < a.J1 || j >= a.g.ny - a.J2 || k < a.K1 || k >= a.g.nz - a.K2;
+}
+
+template <typename T> __device__ __forceinline__ bool in_slabs(const OwnQ<T> &o)
+{
+    return o.in[0] || o.in[1] || o.in[2] || o.in[3] || o.in[4] || o.in[5];
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX * BY) void k_air_p16(const AirArgs<T> a)
+{
+    constexpr int VW = 16 / sizeof(T);
+    const Grid<T> &g = a.g;
+    const int nx = g.nx, ny = g.ny;
+    const int i0 = (blockIdx.x * BX + threadIdx.x) * VW, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    if (i0 >= nx || j >= ny - 1) return;
+    const size_t sxy = (size_t)nx * ny;
+    const size_t id = (size_t)i0 + (size_t)nx * ((size_t)j + (size_t)ny * k);
+    T u[VW], v[VW], vj[VW], w[VW], wk[VW], p[VW], out[VW];
+    ld16<T>(g.un, id, u);
+    ld16<T>(g.vn, id, v);
+    ld16<T>(g.vn, id + nx, vj);
+    ld16<T>(g.wn, id, w);
+    ld16<T>(g.wn, id + sxy, wk);
+    ld16<T>(g.pn, id, p);
+    const T ue = i0 + VW < nx ? g.un[id + VW] : T(0);
+    bool all = true;
+#pragma unroll
+    for (int c = 0; c < VW; ++c) {
+        const int i = i0 + c;
+        out[c] = p[c];
+        if (i < 1 || i > nx - 2) {
+            all = false;
+            continue;
+        }
+        Nb<T> n;
+        n.zX = T(0); n.zY = T(0);
+        metric(g, i, j, k, n.mX, n.mY, n.mZ);
+        n.uc = u[c]; n.u_ip = c + 1 < VW ? u[c + 1] : ue;
+        n.vc = v[c]; n.v_jp = vj[c];
+        n.wc = w[c]; n.w_kp = wk[c];
+        n.pc = p[c];
+        T ddx, ddy, dwz;
+        T r = dp_core<T, 0, false, false>(a, n, ddx, ddy, dwz);
+        if (in_layers(a, i, j, k)) dp_layers<T, false>(a, i, j, k, n.mX, n.mY, n.mZ, T(0), T(0), n.pc, ddx, ddy, dwz, r);
+        const T pnew = p[c] + r;
+        out[c] = pnew;
+        const OwnQ<T> own = own_slabs(a, i, j, k);
+        if (in_slabs(own)) {
+#pragma unroll
+            for (int s = 0; s < 6; ++s)
+                if (own.in[s]) a.Qn[s][3][own.idx[s]] = a.Qo[s][3][own.idx[s]] + g.dt * pnew;
+        }
+    }
+    if (all) {
+        st16<T>(g.pnn, id, out);
+    } else {
+#pragma unroll
+        for (int c = 0; c < VW; ++c)
+            if (i0 + c >= 1 && i0 + c <= nx - 2) g.pnn[id + c] = out[c];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX * BY) void k_air_v16(const AirArgs<T> a)
+{
+    constexpr int VW = 16 / sizeof(T);
+    const Grid<T> &g = a.g;
+    const int nx = g.nx, ny = g.ny;
+    const int i0 = (blockIdx.x * BX + threadIdx.x) * VW, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    if (i0 >= nx || j >= ny - 1) return;
+    const size_t sxy = (size_t)nx * ny;
+    const size_t id = (size_t)i0 + (size_t)nx * ((size_t)j + (size_t)ny * k);
+    T u[VW], v[VW], w[VW], p[VW], pj[VW], pk[VW], uo[VW], vo[VW], wo[VW];
+    ld16<T>(g.un, id, u);
+    ld16<T>(g.vn, id, v);
+    ld16<T>(g.wn, id, w);
+    ld16<T>(g.pnn, id, p);
+    ld16<T>(g.pnn, id - nx, pj);
+    ld16<T>(g.pnn, id - sxy, pk);
+    const T pw = i0 > 0 ? g.pnn[id - 1] : T(0);
+    const T dt = g.dt;
+    bool all = true;
+#pragma unroll
+    for (int c = 0; c < VW; ++c) {
+        const int i = i0 + c;
+        uo[c] = u[c]; vo[c] = v[c]; wo[c] = w[c];
+        if (i < 1 || i > nx - 2) {
+            all = false;
+            continue;
+        }
+        Nb<T> n;
+        n.zX = T(0); n.zY = T(0);
+        metric(g, i, j, k, n.mX, n.mY, n.mZ);
+        n.uc = u[c]; n.vc = v[c]; n.wc = w[c];
+        n.pc = p[c]; n.p_im = c > 0 ? p[c - 1] : pw; n.p_jm = pj[c]; n.p_km = pk[c];
+        T gpx, gpy, gpz, du, dv, dw;
+        duvw_core<T, 0, false, false>(a, n, gpx, gpy, gpz, du, dv, dw);
+        if (in_layers(a, i, j, k))
+            duvw_layers<T, false>(a, i, j, k, n.mX, n.mY, n.mZ, T(0), T(0), n.uc, n.vc, n.wc, gpx, gpy, gpz, du, dv, dw);
+        const T un = u[c] + du, vn = v[c] + dv, wn = w[c] + dw;
+        uo[c] = un; vo[c] = vn; wo[c] = wn;
+        const OwnQ<T> own = own_slabs(a, i, j, k);
+        if (in_slabs(own)) {
+#pragma unroll
+            for (int s = 0; s < 6; ++s)
+                if (own.in[s]) {
+                    const size_t q = own.idx[s];
+                    const T qu = a.Qo[s][0][q], qv = a.Qo[s][1][q], qw = a.Qo[s][2][q];
+                    a.Qn[s][0][q] = qu + dt * un;
+                    a.Qn[s][1][q] = qv + dt * vn;
+                    a.Qn[s][2][q] = qw + dt * wn;
+                }
+        }
+    }
+    if (all) {
+        st16<T>(g.unn, id, uo);
+        st16<T>(g.vnn, id, vo);
+        st16<T>(g.wnn, id, wo);
+    } else {
+#pragma unroll
+        for (int c = 0; c < VW; ++c)
+            if (i0 + c >= 1 && i0 + c <= nx - 2) {
+                g.unn[id + c] = uo[c];
+                g.vnn[id + c] = vo[c];
+                g.wnn[id + c] = wo[c];
+            }
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -1346,8 +1538,25 @@ unsigned flat_blocks(size_t total)
 
 }  // namespace
 
+// The 16-byte kernels cover a flat grid without mean flow or obstacle whose rows start on
+// 16-byte boundaries; every other deck runs the one-point kernels.
+template <typename T> bool wide_fits(const AirArgs<T> &a, bool wind)
+{
+    constexpr int VW = 16 / sizeof(T);
+    return !wind && !a.g.tsh && !a.omask && a.g.nx % VW == 0;
+}
+template <typename T> dim3 wide_grid(int nx, int ny, int nz)
+{
+    constexpr int VW = 16 / sizeof(T);
+    return dim3((nx + BX * VW - 1) / (BX * VW), (ny - 2 + BY - 1) / BY, nz - 2);
+}
+
 template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s)
 {
+    if (wide_fits(a, wind)) {
+        hipLaunchKernelGGL((k_air_p16<T>), wide_grid<T>(a.g.nx, a.g.ny, a.g.nz), dim3(BX, BY), 0, s, a);
+        return;
+    }
     dim3 g = interior_grid(a.g.nx, a.g.ny, a.g.nz), b(BX, BY);
     const bool ter = a.g.tsh != nullptr;
     if (ter) {
@@ -1360,6 +1569,10 @@ template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStrea
 }
 template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStream_t s)
 {
+    if (wide_fits(a, wind)) {
+        hipLaunchKernelGGL((k_air_v16<T>), wide_grid<T>(a.g.nx, a.g.ny, a.g.nz), dim3(BX, BY), 0, s, a);
+        return;
+    }
     dim3 g = interior_grid(a.g.nx, a.g.ny, a.g.nz), b(BX, BY);
     const bool ter = a.g.tsh != nullptr;
     if (ter) {

@@ -36,6 +36,11 @@ def cases(lee):
     # long in y: the fused pass splits the planes into chunks of several planes
     out["long"] = B(24, 200, 20, 0.05, pml=5, bc=(2, 3, 5, 2, 2, 2), porous_cells=6, src=(0.0, 1.0, 0.4),
                     gauss_width=0.25, time_domain=30, pml_y_time=5)
+    # 264 points across x: two x tiles of the 16-byte kernels in FP32 (256 points per
+    # wavefront) and three in FP64 (128), the last one partial; x = 0 at i = 256, so the
+    # pulse crosses the tile boundary into the x2 layer (i >= 259)
+    out["wide"] = B(264, 30, 12, 0.05, pml=5, bc=(2, 3, 2, 2, 2, 2), porous_cells=5, x0=-12.8,
+                    src=(0.0, 0.6, 0.3), gauss_width=0.25, time_domain=30)
     return out
 
 
@@ -43,7 +48,7 @@ PATHS = {"split": 1, "fused": 2}
 
 
 @pytest.mark.parametrize("path", ["split", "fused"])
-@pytest.mark.parametrize("name", ["rigid", "pml5", "tdbc", "porous", "wind5", "wind2", "long"])
+@pytest.mark.parametrize("name", ["rigid", "pml5", "tdbc", "porous", "wind5", "wind2", "long", "wide"])
 def test_fb_fp64_parity(lee, orc, name, path):
     d = cases(lee)[name]
     print(f"[gpu parity] case={name} path={path} shape={d.shape}", flush=True)
@@ -105,7 +110,7 @@ def test_fp32_tracks_fp64(lee):
 @pytest.mark.parametrize("prec", [32, 64])
 def test_fused_equals_split_bitwise(lee, prec):
     """Both kernel paths give the same bits, FP32 included."""
-    for name in ("pml5", "porous", "wind5", "long"):
+    for name in ("pml5", "porous", "wind5", "long", "wide"):
         d = cases(lee)[name]
         print(f"[gpu] fused vs split fp{prec} case={name}", flush=True)
         a = lee.Solver(d, precision=prec, receivers=REC, path=PATHS["split"])
