@@ -109,6 +109,7 @@ def lib():
         L.lee_save_restart.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
         L.lee_load_restart.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
         L.lee_write_contour.argtypes = [vp, ctypes.c_char_p]
+        L.lee_write_contour_noalloc.argtypes = [vp, ctypes.c_char_p]
         L.lee_set_source_series.argtypes = [vp, _dp, ctypes.c_longlong, ctypes.c_int]
         L.lee_get_ground_psi.argtypes = [vp, _dp, ctypes.c_longlong]
         L.lee_ensemble_create.argtypes = [ctypes.c_int, ctypes.POINTER(LeeDeck), ctypes.POINTER(LeeOptions),
@@ -338,6 +339,11 @@ class Solver:
 
     def write_contour(self, dirpath: str) -> None:
         _check(lib().lee_write_contour(self.h, dirpath.encode()), self.h)
+
+    def write_contour_noalloc(self, dirpath: str) -> None:
+        """The same file through lee_write_contour_noalloc (no device allocation: the subsample
+        is staged in the library's fixed per-GPU array)."""
+        _check(lib().lee_write_contour_noalloc(self.h, dirpath.encode()), self.h)
 
     def ground_psi(self, K: Optional[int] = None) -> np.ndarray:
         """Ground impedance histories, shape (ny, nx, K), K = the context's pole count (a

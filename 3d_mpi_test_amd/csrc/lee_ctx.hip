@@ -15,6 +15,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -90,8 +91,9 @@ struct lee_ctx {
     long long *rec_idx = nullptr;
     double *rec_buf = nullptr;
     int *flag = nullptr;
-    void *scratch = nullptr;
-    size_t scratch_bytes = 0;
+    void *shift_tmp = nullptr;  // frame shift: one field or slab copied out of place
+    size_t shift_tmp_bytes = 0;
+    int device = 0;
     int mf_count = -1;
     long long steps = 0, counter = 0;
     float last_ms = 0.f;
@@ -132,6 +134,44 @@ int set_err(lee_ctx *c, int code, const std::string &m)
         if (e_ != hipSuccess)                                                                   \
             return set_err((c), LEE_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+}  // namespace
+
+// ---- device staging of lee_write_contour_noalloc ----------------------------------------
+// A file-scope __device__ array exists once on every GPU the library runs on, from the loading
+// of this code object until the process ends, so that entry point stages through it without
+// allocating.  On the host the array's name is a stand-in, not a device address: host code
+// never passes it to a kernel or a copy.  lee_create looks up each device's address once, with
+// that device current (hipGetSymbolAddress), and the entry point uses that address.  Calls on
+// one GPU share its array: two such calls that overlap on one GPU race.
+constexpr size_t kScratchBytes = (size_t)4 << 20;  // 4 MiB per GPU
+constexpr size_t kScratchDoubles = kScratchBytes / sizeof(double);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_at[kMaxDevices] = {};  // each device's address of `scratch`
+
+// Once per device, with that device current.
+int scratch_setup(lee_ctx *c, int device)
+{
+    if (device < 0 || device >= kMaxDevices) return set_err(c, LEE_ERR_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_at[device]) return LEE_OK;
+    void *p = nullptr;
+    HIPCHK(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_err(c, LEE_ERR_DEVICE, "no device address for the contour staging array");
+    g_scratch_at[device] = (double *)p;
+    return LEE_OK;
+}
+
+double *scratch_of(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch_at[device] : nullptr;
+}
 
 int dalloc(lee_ctx *c, void **out, size_t bytes)
 {
@@ -416,8 +456,8 @@ template <typename T> int frame_shift(lee_ctx *c, int N)
         size_t cnt = (size_t)A * m.ny * C;
         for (int f = 0; f < 4; ++f) {
             void *q = c->Q[c->qcur][sl][f];
-            HIPCHK(c, hipMemcpyAsync(c->scratch, q, cnt * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
-            launch_shift<T>((T *)q, (const T *)c->scratch, A, m.ny, C, tempJ, m.lead, s, c->stream);
+            HIPCHK(c, hipMemcpyAsync(c->shift_tmp, q, cnt * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+            launch_shift<T>((T *)q, (const T *)c->shift_tmp, A, m.ny, C, tempJ, m.lead, s, c->stream);
         }
     }
     // RK4: the y layers move with the frame as well (their auxiliaries meet the shifted x and
@@ -430,8 +470,8 @@ template <typename T> int frame_shift(lee_ctx *c, int N)
             const int lo = g.lo;
             for (int f = 0; f < 4; ++f) {
                 void *q = c->Q[c->qcur][sl][f];
-                HIPCHK(c, hipMemcpyAsync(c->scratch, q, cnt * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
-                launch_shift_yslab<T>((T *)q, (const T *)c->scratch, m.nx, g.n, m.nz, lo, m.ny, tempJ, m.lead, s, c->stream);
+                HIPCHK(c, hipMemcpyAsync(c->shift_tmp, q, cnt * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+                launch_shift_yslab<T>((T *)q, (const T *)c->shift_tmp, m.nx, g.n, m.nz, lo, m.ny, tempJ, m.lead, s, c->stream);
             }
         }
     HIPCHK(c, hipGetLastError());
@@ -868,6 +908,7 @@ template <typename T> int run_steps(lee_ctx *c, int nsteps, double *rec_out)
 int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
 {
     std::unique_ptr<lee_ctx> c(new lee_ctx());
+    int rc0;
     lee_options o;
     lee_options_default(&o);
     if (opt) o = *opt;
@@ -897,6 +938,8 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
         if (nb >= (1LL << 31)) return set_err(nullptr, LEE_ERR_ARG, "grid too large: more than 2^31 boundary nodes");
     }
     HIPCHK(C, hipSetDevice(o.device));
+    C->device = o.device;
+    if ((rc0 = scratch_setup(C, o.device))) return rc0;
     HIPCHK(C, hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
     HIPCHK(C, hipEventCreate(&C->ev0));
     HIPCHK(C, hipEventCreate(&C->ev1));
@@ -982,8 +1025,8 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
         HIPCHK(C, hipMemcpyAsync(C->omask, m.omask.data(), m.omask.size(), hipMemcpyHostToDevice, C->stream));
         HIPCHK(C, hipStreamSynchronize(C->stream));
     }
-    C->scratch_bytes = std::max<size_t>(qmax * es, 8);
-    if ((rc = dalloc(C, &C->scratch, C->scratch_bytes))) return rc;
+    C->shift_tmp_bytes = std::max<size_t>(qmax * es, 8);
+    if ((rc = dalloc(C, &C->shift_tmp, C->shift_tmp_bytes))) return rc;
     // initial condition (calculation_2D.cpp:300-314): velocities zero, Gaussian pressure
     std::vector<double> p0;
     m.initial_condition(p0, false);
@@ -1414,13 +1457,37 @@ int out_count(int n, int stride) { return (n - 1) % stride == 0 ? (n - 1) / stri
 
 // calculation_2D.cpp:207-213 (out_n = 4) and the subsample of mpi_send_data
 // (calculation_2D.cpp:1030-1105); getOstream (calculation_2D.cpp:1165-1182) prints it.
+// The zone's subsample goes to `stage` when given (lee_write_contour_noalloc: the GPU's
+// staging array, copied back in fixed-size chunks through a stack buffer) or to a device
+// buffer allocated for the call (lee_write_contour).
 template <typename T>
 int contour_zone(lee_ctx *c, std::ofstream &o, const T *p, int nx, int ny, int nz, const std::vector<double> &X,
-                 const std::vector<double> &Y, const std::vector<double> &Z)
+                 const std::vector<double> &Y, const std::vector<double> &Z, double yshift, double *stage)
 {
     const int st = 4;
     const int ox = out_count(nx, st), oy = out_count(ny, st), oz = out_count(nz, st);
     const size_t n = (size_t)ox * oy * oz;
+    auto pick = [&](const std::vector<double> &v, int o_, int nn, int q) { return q < o_ - 1 ? v[(size_t)q * st] : v[nn - 1]; };
+    o << "I=" << ox << ",J=" << oy << ",K=" << oz << ",F=POINT" << std::endl;
+    if (stage) {
+        launch_subsample<T>(p, nx, ny, nz, st, ox, oy, oz, stage, c->stream);
+        HIPCHK(c, hipGetLastError());
+        constexpr size_t kChunk = 1024;
+        double h[kChunk];
+        size_t r = 0;
+        for (int k = 0; k < oz; ++k)
+            for (int j = 0; j < oy; ++j)
+                for (int i = 0; i < ox; ++i, ++r) {
+                    if (r % kChunk == 0) {
+                        const size_t cnt = std::min(kChunk, n - r);
+                        HIPCHK(c, hipMemcpyAsync(h, stage + r, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                        HIPCHK(c, hipStreamSynchronize(c->stream));
+                    }
+                    o << pick(X, ox, nx, i) << " " << pick(Y, oy, ny, j) + yshift << " " << pick(Z, oz, nz, k) << " "
+                      << h[r % kChunk] << " " << std::endl;
+                }
+        return LEE_OK;
+    }
     double *d = nullptr;
     HIPCHK(c, hipMalloc(&d, n * sizeof(double)));
     launch_subsample<T>(p, nx, ny, nz, st, ox, oy, oz, d, c->stream);
@@ -1430,29 +1497,41 @@ int contour_zone(lee_ctx *c, std::ofstream &o, const T *p, int nx, int ny, int n
     (void)hipFree(d);
     HIPCHK(c, e1);
     HIPCHK(c, e2);
-    auto pick = [&](const std::vector<double> &v, int o_, int nn, int q) { return q < o_ - 1 ? v[(size_t)q * st] : v[nn - 1]; };
-    o << "I=" << ox << ",J=" << oy << ",K=" << oz << ",F=POINT" << std::endl;
     size_t r = 0;
     for (int k = 0; k < oz; ++k)
         for (int j = 0; j < oy; ++j)
             for (int i = 0; i < ox; ++i, ++r)
-                o << pick(X, ox, nx, i) << " " << pick(Y, oy, ny, j) << " " << pick(Z, oz, nz, k) << " " << h[r] << " "
-                  << std::endl;
+                o << pick(X, ox, nx, i) << " " << pick(Y, oy, ny, j) + yshift << " " << pick(Z, oz, nz, k) << " " << h[r]
+                  << " " << std::endl;
     return LEE_OK;
 }
 
-}  // namespace
-
-extern "C" {
+size_t contour_values(int nx, int ny, int nz)
+{
+    return (size_t)out_count(nx, 4) * out_count(ny, 4) * out_count(nz, 4);
+}
 
 // Writes dir/ptNNN.dat, NNN = (steps_done + 1) / out_difft as the reference numbers its
 // files (get_data_contour receives n + temp_time + 1 and adds one more), from the pressure of
-// the current time level.  The reference writes to a fixed absolute cluster path.
-int lee_write_contour(lee_ctx *c, const char *dir)
+// the current time level.  The reference writes to a fixed absolute cluster path.  With
+// noalloc the subsample is staged in the GPU's staging array; a zone larger than it is
+// refused before anything is written.
+int write_contour(lee_ctx *c, const char *dir, bool noalloc)
 {
     if (!c || !dir) return set_err(c, LEE_ERR_ARG, "bad arguments");
     const Model &m = c->m;
     if (m.d.out_difft <= 0) return set_err(c, LEE_ERR_ARG, "out_difft must be positive");
+    double *stage = nullptr;
+    if (noalloc) {
+        stage = scratch_of(c->device);
+        if (!stage) return set_err(c, LEE_ERR_STATE, "no staging array on this device");
+        const size_t need = std::max(contour_values(m.nx, m.ny, m.nz),
+                                     m.south_porous ? contour_values(m.pnx, m.pny, m.pnz) : (size_t)0);
+        if (need > kScratchDoubles)
+            return set_err(c, LEE_ERR_ARG, "lee_write_contour_noalloc: a zone of " + std::to_string(need) +
+                                               " values exceeds the staging array of " + std::to_string(kScratchDoubles) +
+                                               " values (4 MiB per GPU); use lee_write_contour");
+    }
     const long long nn = (c->steps + 1) / m.d.out_difft;
     char name[64];
     std::snprintf(name, sizeof name, "/pt%03lld.dat", nn);
@@ -1462,32 +1541,39 @@ int lee_write_contour(lee_ctx *c, const char *dir)
     o.setf(std::ios::scientific, std::ios::floatfield);
     o.precision(6);
     const int N = c->mf_count < 0 ? 0 : c->mf_count;
-    std::vector<double> Y(m.ay);
-    for (auto &y : Y) y += N * (m.trail - 1) * m.d.dy;
+    const double ys = N * (m.trail - 1) * m.d.dy;
     const int b = c->cur * 4 + 3;
     int rc;
     if (!m.south_porous) {
         o << "VARIABLES = \"X\", \"Y\", \"Z\", \"P\"" << std::endl;
         o << "ZONE T = \"Air Zone\",";
-        rc = c->prec == 64 ? contour_zone<double>(c, o, (const double *)c->air[b], m.nx, m.ny, m.nz, m.ax, Y, m.az)
-                           : contour_zone<float>(c, o, (const float *)c->air[b], m.nx, m.ny, m.nz, m.ax, Y, m.az);
+        rc = c->prec == 64 ? contour_zone<double>(c, o, (const double *)c->air[b], m.nx, m.ny, m.nz, m.ax, m.ay, m.az, ys, stage)
+                           : contour_zone<float>(c, o, (const float *)c->air[b], m.nx, m.ny, m.nz, m.ax, m.ay, m.az, ys, stage);
         if (rc) return rc;
     } else {
         // the reference omits the VARIABLES line when the ground is porous
         o << "ZONE T = \"Air Zone\",";
-        rc = c->prec == 64 ? contour_zone<double>(c, o, (const double *)c->air[b], m.nx, m.ny, m.nz, m.ax, Y, m.az)
-                           : contour_zone<float>(c, o, (const float *)c->air[b], m.nx, m.ny, m.nz, m.ax, Y, m.az);
+        rc = c->prec == 64 ? contour_zone<double>(c, o, (const double *)c->air[b], m.nx, m.ny, m.nz, m.ax, m.ay, m.az, ys, stage)
+                           : contour_zone<float>(c, o, (const float *)c->air[b], m.nx, m.ny, m.nz, m.ax, m.ay, m.az, ys, stage);
         if (rc) return rc;
-        std::vector<double> PY(m.pay);
-        for (auto &y : PY) y += N * (m.trail - 1) * m.d.pore_dy;
+        const double pys = N * (m.trail - 1) * m.d.pore_dy;
         o << "ZONE T = \"south pore Zone\",";
-        rc = c->prec == 64 ? contour_zone<double>(c, o, (const double *)c->pore[b], m.pnx, m.pny, m.pnz, m.pax, PY, m.paz)
-                           : contour_zone<float>(c, o, (const float *)c->pore[b], m.pnx, m.pny, m.pnz, m.pax, PY, m.paz);
+        rc = c->prec == 64
+                 ? contour_zone<double>(c, o, (const double *)c->pore[b], m.pnx, m.pny, m.pnz, m.pax, m.pay, m.paz, pys, stage)
+                 : contour_zone<float>(c, o, (const float *)c->pore[b], m.pnx, m.pny, m.pnz, m.pax, m.pay, m.paz, pys, stage);
         if (rc) return rc;
     }
     if (!o) return set_err(c, LEE_ERR_IO, "write failed: " + path);
     return LEE_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+int lee_write_contour(lee_ctx *c, const char *dir) { return write_contour(c, dir, false); }
+
+int lee_write_contour_noalloc(lee_ctx *c, const char *dir) { return write_contour(c, dir, true); }
 
 int lee_get_psi(lee_ctx *c, double *psi, long long cap)
 {

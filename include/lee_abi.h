@@ -255,6 +255,14 @@ int  lee_load_restart(lee_ctx *ctx, const char *path, int *index);
  * NNN = (steps done + 1) / out_difft.  The reference calls it when the steps done are a
  * multiple of out_difft (airpore.cpp:615-618) and writes to a fixed absolute path. */
 int  lee_write_contour(lee_ctx *ctx, const char *dir);
+/* The same contour file, written without allocating device memory: no counterpart in the
+ * reference; for callers that cannot allocate (code that runs while a HIP graph is being
+ * captured, tight serving loops, harnesses that forbid hipMalloc).  lee_write_contour
+ * allocates a device buffer per call for the subsample; this variant stages it in a fixed
+ * 4 MiB array per GPU (the library holds it until the process ends), so each zone may have
+ * at most 524288 subsampled nodes (LEE_ERR_ARG beyond that, before anything is written).
+ * Calls that overlap on one GPU share that array and race. */
+int  lee_write_contour_noalloc(lee_ctx *ctx, const char *dir);
 
 /* Time-series source (calculation_2D::UpdateSource calculation_2D.cpp:371-385, the series
  * of source_input.dat read at calculation_2D.cpp:220-235, the call at airpore.cpp:608-611):
