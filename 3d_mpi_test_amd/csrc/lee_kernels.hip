@@ -765,169 +765,6 @@ __global__ __launch_bounds__(BX * BY) void k_air_v(const AirArgs<T> a)
 }
 
 // ------------------------------------------------------------------------------------
-// The two phases with 16-byte accesses: lane l of a wavefront holds the VW = 16/sizeof(T)
-// consecutive x points i0 = VW*(64*bx + l) ... i0 + VW - 1 of one row (4 in FP32, 2 in
-// FP64), so every field access of a wavefront is one 1 KB segment loaded or stored by one
-// instruction per lane instead of VW.  A lane's points are computed one after the other with
-// the same per-point arithmetic as k_air_p / k_air_v (dp_core / duvw_core and the layer
-// terms), so both give the same bits.  The x neighbour outside a lane's vector (u at i0 + VW,
-// p at i0 - 1) is one 4-byte read of a line the wavefront loads anyway.  For a flat grid
-// without mean flow or obstacle, nx a multiple of VW (rows start on 16-byte boundaries).
-// ------------------------------------------------------------------------------------
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x2 __attribute__((ext_vector_type(2)));
-template <typename T> struct Vec16;
-template <> struct Vec16<float> { using type = f32x4; };
-template <> struct Vec16<double> { using type = f64x2; };
-
-template <typename T> __device__ __forceinline__ void ld16(gptr<const T> p, size_t id, T (&x)[16 / sizeof(T)])
-{
-    using V = typename Vec16<T>::type;
-    const V v = *(const __attribute__((address_space(1))) V *)(p.p + id);
-    __builtin_memcpy(x, &v, 16);
-}
-template <typename T> __device__ __forceinline__ void st16(gptr<T> p, size_t id, const T (&x)[16 / sizeof(T)])
-{
-    using V = typename Vec16<T>::type;
-    V v;
-    __builtin_memcpy(&v, x, 16);
-    *(__attribute__((address_space(1))) V *)(p.p + id) = v;
-}
-
-// Whether (i, j, k) lies in a layer whose terms air_dp / air_duvw add.
-template <typename T> __device__ __forceinline__ bool in_layers(const AirArgs<T> &a, int i, int j, int k)
-{
-    return i < a.I1 || i >= a.g.nx - a.I2 || j < a.J1 || j >= a.g.ny - a.J2 || k < a.K1 || k >= a.g.nz - a.K2;
-}
-
-template <typename T> __device__ __forceinline__ bool in_slabs(const OwnQ<T> &o)
-{
-    return o.in[0] || o.in[1] || o.in[2] || o.in[3] || o.in[4] || o.in[5];
-}
-
-template <typename T>
-__global__ __launch_bounds__(BX * BY) void k_air_p16(const AirArgs<T> a)
-{
-    constexpr int VW = 16 / sizeof(T);
-    const Grid<T> &g = a.g;
-    const int nx = g.nx, ny = g.ny;
-    const int i0 = (blockIdx.x * BX + threadIdx.x) * VW, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
-    if (i0 >= nx || j >= ny - 1) return;
-    const size_t sxy = (size_t)nx * ny;
-    const size_t id = (size_t)i0 + (size_t)nx * ((size_t)j + (size_t)ny * k);
-    T u[VW], v[VW], vj[VW], w[VW], wk[VW], p[VW], out[VW];
-    ld16<T>(g.un, id, u);
-    ld16<T>(g.vn, id, v);
-    ld16<T>(g.vn, id + nx, vj);
-    ld16<T>(g.wn, id, w);
-    ld16<T>(g.wn, id + sxy, wk);
-    ld16<T>(g.pn, id, p);
-    const T ue = i0 + VW < nx ? g.un[id + VW] : T(0);
-    bool all = true;
-#pragma unroll
-    for (int c = 0; c < VW; ++c) {
-        const int i = i0 + c;
-        out[c] = p[c];
-        if (i < 1 || i > nx - 2) {
-            all = false;
-            continue;
-        }
-        Nb<T> n;
-        n.zX = T(0); n.zY = T(0);
-        metric(g, i, j, k, n.mX, n.mY, n.mZ);
-        n.uc = u[c]; n.u_ip = c + 1 < VW ? u[c + 1] : ue;
-        n.vc = v[c]; n.v_jp = vj[c];
-        n.wc = w[c]; n.w_kp = wk[c];
-        n.pc = p[c];
-        T ddx, ddy, dwz;
-        T r = dp_core<T, 0, false, false>(a, n, ddx, ddy, dwz);
-        if (in_layers(a, i, j, k)) dp_layers<T, false>(a, i, j, k, n.mX, n.mY, n.mZ, T(0), T(0), n.pc, ddx, ddy, dwz, r);
-        const T pnew = p[c] + r;
-        out[c] = pnew;
-        const OwnQ<T> own = own_slabs(a, i, j, k);
-        if (in_slabs(own)) {
-#pragma unroll
-            for (int s = 0; s < 6; ++s)
-                if (own.in[s]) a.Qn[s][3][own.idx[s]] = a.Qo[s][3][own.idx[s]] + g.dt * pnew;
-        }
-    }
-    if (all) {
-        st16<T>(g.pnn, id, out);
-    } else {
-#pragma unroll
-        for (int c = 0; c < VW; ++c)
-            if (i0 + c >= 1 && i0 + c <= nx - 2) g.pnn[id + c] = out[c];
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(BX * BY) void k_air_v16(const AirArgs<T> a)
-{
-    constexpr int VW = 16 / sizeof(T);
-    const Grid<T> &g = a.g;
-    const int nx = g.nx, ny = g.ny;
-    const int i0 = (blockIdx.x * BX + threadIdx.x) * VW, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
-    if (i0 >= nx || j >= ny - 1) return;
-    const size_t sxy = (size_t)nx * ny;
-    const size_t id = (size_t)i0 + (size_t)nx * ((size_t)j + (size_t)ny * k);
-    T u[VW], v[VW], w[VW], p[VW], pj[VW], pk[VW], uo[VW], vo[VW], wo[VW];
-    ld16<T>(g.un, id, u);
-    ld16<T>(g.vn, id, v);
-    ld16<T>(g.wn, id, w);
-    ld16<T>(g.pnn, id, p);
-    ld16<T>(g.pnn, id - nx, pj);
-    ld16<T>(g.pnn, id - sxy, pk);
-    const T pw = i0 > 0 ? g.pnn[id - 1] : T(0);
-    const T dt = g.dt;
-    bool all = true;
-#pragma unroll
-    for (int c = 0; c < VW; ++c) {
-        const int i = i0 + c;
-        uo[c] = u[c]; vo[c] = v[c]; wo[c] = w[c];
-        if (i < 1 || i > nx - 2) {
-            all = false;
-            continue;
-        }
-        Nb<T> n;
-        n.zX = T(0); n.zY = T(0);
-        metric(g, i, j, k, n.mX, n.mY, n.mZ);
-        n.uc = u[c]; n.vc = v[c]; n.wc = w[c];
-        n.pc = p[c]; n.p_im = c > 0 ? p[c - 1] : pw; n.p_jm = pj[c]; n.p_km = pk[c];
-        T gpx, gpy, gpz, du, dv, dw;
-        duvw_core<T, 0, false, false>(a, n, gpx, gpy, gpz, du, dv, dw);
-        if (in_layers(a, i, j, k))
-            duvw_layers<T, false>(a, i, j, k, n.mX, n.mY, n.mZ, T(0), T(0), n.uc, n.vc, n.wc, gpx, gpy, gpz, du, dv, dw);
-        const T un = u[c] + du, vn = v[c] + dv, wn = w[c] + dw;
-        uo[c] = un; vo[c] = vn; wo[c] = wn;
-        const OwnQ<T> own = own_slabs(a, i, j, k);
-        if (in_slabs(own)) {
-#pragma unroll
-            for (int s = 0; s < 6; ++s)
-                if (own.in[s]) {
-                    const size_t q = own.idx[s];
-                    const T qu = a.Qo[s][0][q], qv = a.Qo[s][1][q], qw = a.Qo[s][2][q];
-                    a.Qn[s][0][q] = qu + dt * un;
-                    a.Qn[s][1][q] = qv + dt * vn;
-                    a.Qn[s][2][q] = qw + dt * wn;
-                }
-        }
-    }
-    if (all) {
-        st16<T>(g.unn, id, uo);
-        st16<T>(g.vnn, id, vo);
-        st16<T>(g.wnn, id, wo);
-    } else {
-#pragma unroll
-        for (int c = 0; c < VW; ++c)
-            if (i0 + c >= 1 && i0 + c <= nx - 2) {
-                g.unn[id + c] = uo[c];
-                g.vnn[id + c] = vo[c];
-                g.wnn[id + c] = wo[c];
-            }
-    }
-}
-
-// ------------------------------------------------------------------------------------
 // Fused step: both phases in one pass.
 //
 // A block is FB_TY + 1 wavefronts.  Wavefront w holds row j = FB_TY*ty + w and lane l holds
@@ -1538,25 +1375,9 @@ unsigned flat_blocks(size_t total)
 
 }  // namespace
 
-// The 16-byte kernels cover a flat grid without mean flow or obstacle whose rows start on
-// 16-byte boundaries; every other deck runs the one-point kernels.
-template <typename T> bool wide_fits(const AirArgs<T> &a, bool wind)
-{
-    constexpr int VW = 16 / sizeof(T);
-    return !wind && !a.g.tsh && !a.omask && a.g.nx % VW == 0;
-}
-template <typename T> dim3 wide_grid(int nx, int ny, int nz)
-{
-    constexpr int VW = 16 / sizeof(T);
-    return dim3((nx + BX * VW - 1) / (BX * VW), (ny - 2 + BY - 1) / BY, nz - 2);
-}
 
 template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s)
 {
-    if (wide_fits(a, wind)) {
-        hipLaunchKernelGGL((k_air_p16<T>), wide_grid<T>(a.g.nx, a.g.ny, a.g.nz), dim3(BX, BY), 0, s, a);
-        return;
-    }
     dim3 g = interior_grid(a.g.nx, a.g.ny, a.g.nz), b(BX, BY);
     const bool ter = a.g.tsh != nullptr;
     if (ter) {
@@ -1569,10 +1390,6 @@ template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStrea
 }
 template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStream_t s)
 {
-    if (wide_fits(a, wind)) {
-        hipLaunchKernelGGL((k_air_v16<T>), wide_grid<T>(a.g.nx, a.g.ny, a.g.nz), dim3(BX, BY), 0, s, a);
-        return;
-    }
     dim3 g = interior_grid(a.g.nx, a.g.ny, a.g.nz), b(BX, BY);
     const bool ter = a.g.tsh != nullptr;
     if (ter) {

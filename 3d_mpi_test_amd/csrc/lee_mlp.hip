@@ -4,8 +4,9 @@
 //
 // Every product of the network is one kernel, C[M x N] = A[M x K] B[K x N] with strided
 // operands (forward: activations x weights^T; backward: dZ^T x activations and dZ x weights),
-// one 64-lane wave per 16 x 16 tile of C, v_mfma_f32_16x16x16_bf16 over K in steps of 16;
-// the FP32 operands are rounded to BF16 (nearest even) as they are loaded.  Elementwise
+// one 64-lane wave per 16 x 16 tile of C, gfx950's v_mfma_f32_16x16x32_bf16 over K in steps
+// of 32 (and v_mfma_f32_16x16x16_bf16 for a last step of 16 when K is an odd multiple of 16);
+// the FP32 operands are rounded to BF16 (nearest even, NaN kept) as they are loaded.  Elementwise
 // kernels do the bias, GELU and its derivative, the loss gradient, the bias sums and Adam.
 #include <hip/hip_runtime.h>
 
@@ -20,19 +21,24 @@
 namespace {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// FP32 -> BF16 bits, round to nearest even; a NaN stays a (quiet) NaN instead of rounding
+// into the exponent field.
 __device__ __forceinline__ short to_bf16(float x)
 {
     unsigned u = __float_as_uint(x);
-    u = u + 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (short)((u >> 16) | 0x0040u);
+    u = u + 0x7fffu + ((u >> 16) & 1u);
     return (short)(u >> 16);
 }
 
 // C(m, n) = sum_k A(m, k) B(k, n) [+ bias(n)]; A(m, k) = pa[m*am + k*ak], B(k, n) = pb[k*bk + n*bn].
-// M, N, K multiples of 16.  Block: 4 waves, one 16x16 tile each.  Lane l of a wave holds
-// A(i, k0+4q .. k0+4q+3) and B(k0+4q .. +3, j) with i = j = l % 16, q = l / 16, and receives
-// C(4q .. 4q+3, j).
+// M, N, K multiples of 16.  Block: 4 waves, one 16x16 tile each.  With r = l % 16 and
+// q = l / 16, lane l of a wave holds A(r, k0+8q .. k0+8q+7) and B(k0+8q .. +7, r) in a step of
+// 32 (A(r, k0+4q .. +3), B(k0+4q .. +3, r) in a step of 16) and receives C(4q .. 4q+3, r).
 __global__ __launch_bounds__(256) void k_gemm_bf16(const float *__restrict__ pa, long am, long ak,
                                                    const float *__restrict__ pb, long bk, long bn,
                                                    float *__restrict__ pc, long cm, const float *__restrict__ bias,
@@ -45,7 +51,18 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const float *__restrict__ pa,
     const int m0 = (tile / tiles_n) << 4, n0 = (tile % tiles_n) << 4;
     const int r = lane & 15, q = lane >> 4;
     v4f acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    int k0 = 0;
+    for (; k0 + 32 <= K; k0 += 32) {
+        v8s a, b;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int k = k0 + 8 * q + t;
+            a[t] = to_bf16(pa[(long)(m0 + r) * am + (long)k * ak]);
+            b[t] = to_bf16(pb[(long)k * bk + (long)(n0 + r) * bn]);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), acc, 0, 0, 0);
+    }
+    if (k0 < K) {
         v4s a, b;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {

@@ -15,10 +15,14 @@ N_IN, N_OUT = 4, 8
 
 
 def bf16(x):
-    """Round float32 values to bfloat16 (nearest even), returned as float64."""
-    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
-    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
-    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    """Round float32 values to bfloat16 (nearest even; a NaN stays a quiet NaN), returned as
+    float64."""
+    u32 = np.asarray(x, dtype=np.float32).view(np.uint32)
+    u = u32.astype(np.uint64)
+    r = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    nan = (u32 & 0x7FFFFFFF) > 0x7F800000
+    r = np.where(nan, (u >> 16) | 0x40, r)
+    return (r.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
 def unpack(w):

@@ -22,6 +22,10 @@ def test_bf16_rounding_known_answers():
     r = (rng.standard_normal(20000) * 10.0 ** rng.uniform(-6, 6, 20000)).astype(np.float32)
     ref = torch.from_numpy(r).to(torch.bfloat16).to(torch.float64).numpy()
     assert np.array_equal(mo.bf16(r), ref)
+    # NaN (quiet and signalling, either sign) stays NaN; infinities stay infinite
+    bits = np.array([0x7FC00000, 0x7F800001, 0xFFBFFFFF, 0x7FFFFFFF, 0x7F800000, 0xFF800000], dtype=np.uint32)
+    z = mo.bf16(bits.view(np.float32))
+    assert np.isnan(z[:4]).all() and z[4] == np.inf and z[5] == -np.inf
 
 
 def _params(seed):
