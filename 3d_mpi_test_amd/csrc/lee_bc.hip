@@ -50,15 +50,29 @@ template <typename T> __device__ __forceinline__ T p_pore_walk(const BCArgs<T> &
 // faces N E W F B then treat its edge nodes as on every other plane, and there is no South
 // face.  The normal velocity (half a cell below the plane) follows the porous momentum
 // equation with the pressure half a cell further down given by the impedance through the
-// TDBC recursion of calculation_2D.cpp:986-1028 with K poles.  Flat grid: the plane's
-// metric is gmx[i] = 1/X_over_x, gmy[j] = 1/Y_over_y, gmz = 1/Z_over_z of the plane k = 1.
+// TDBC recursion of calculation_2D.cpp:986-1028 with K poles.  The plane's metric is that of
+// the plane k = 1: gmx[i] = 1/X_over_x, gmy[j] = 1/Y_over_y, gmz = 1/Z_over_z on a flat grid;
+// on the terrain-following grid 1/Z_over_z and the cross terms zX, zY of the inverse Jacobian
+// per node (gez, gzx, gzy), with the plane's z derivatives one-sided towards k = 1.  With no
+// hill every term reduces to the flat closure's bit for bit (the cross terms are zeros).
 template <typename T> __device__ __forceinline__ T ground_p(const BCArgs<T> &a, int i, int j)
 {
     const Grid<T> &g = a.air;
     const size_t e = gidx(g, i, j, 0);
-    const T ex = (T)a.gmx[i], ey = (T)a.gmy[j], ez = (T)a.gmz;
-    const T div = ex * g.dxi * (g.un[gidx(g, i + 1, j, 0)] - g.un[e]) + ey * g.dyi * (g.vn[gidx(g, i, j + 1, 0)] - g.vn[e]) +
-                  ez * g.dzi * (g.wn[gidx(g, i, j, 1)] - g.wn[e]);
+    const T ex = (T)a.gmx[i], ey = (T)a.gmy[j];
+    T div;
+    if (a.gez) {
+        const size_t c = (size_t)i + (size_t)g.nx * j;
+        const T ez = (T)a.gez[c], zX = (T)a.gzx[c], zY = (T)a.gzy[c];
+        const T dux = g.un[gidx(g, i + 1, j, 0)] - g.un[e], duz = g.un[gidx(g, i, j, 1)] - g.un[e];
+        const T dvy = g.vn[gidx(g, i, j + 1, 0)] - g.vn[e], dvz = g.vn[gidx(g, i, j, 1)] - g.vn[e];
+        div = ((ex * g.dxi * dux + zX * g.dzi * duz) + (ey * g.dyi * dvy + zY * g.dzi * dvz)) +
+              ez * g.dzi * (g.wn[gidx(g, i, j, 1)] - g.wn[e]);
+    } else {
+        const T ez = (T)a.gmz;
+        div = ex * g.dxi * (g.un[gidx(g, i + 1, j, 0)] - g.un[e]) + ey * g.dyi * (g.vn[gidx(g, i, j + 1, 0)] - g.vn[e]) +
+              ez * g.dzi * (g.wn[gidx(g, i, j, 1)] - g.wn[e]);
+    }
     return g.pn[e] + (-(T)a.g_gama * div);
 }
 
@@ -132,7 +146,8 @@ template <typename T> __device__ __forceinline__ double ground_w_of(const BCArgs
     const Grid<T> &g = a.air;
     const size_t e = gidx(g, i, j, 0);
     const double hist = a.gZinf * temp_psi;
-    return ((double)g.wn[e] - a.gcpl * ((double)g.pnn[e] - hist)) * a.g_ib / (1.0 + a.g_ib * a.gcpl * a.gzi);
+    const double cpl = a.gcplv ? a.gcplv[(size_t)i + (size_t)g.nx * j] : a.gcpl;
+    return ((double)g.wn[e] - cpl * ((double)g.pnn[e] - hist)) * a.g_ib / (1.0 + a.g_ib * cpl * a.gzi);
 }
 
 // The ground histories are pole-major, Psi_q of node (i, j) at q * nx * ny + i + nx * j, so
@@ -150,6 +165,19 @@ template <typename T> __device__ __forceinline__ T ground_vel(const BCArgs<T> &a
     const Grid<T> &g = a.air;
     const size_t e = gidx(g, i, j, 0);
     const T ib = (T)a.g_ib, dt = a.dt, eff = (T)a.g_eff;
+    if (a.gez && c < 2) {
+        // terrain: d/dX = ex d/dx + zX d/dz, d/dY = ey d/dy + zY d/dz, d/dz one-sided to k = 1
+        const size_t q = (size_t)i + (size_t)g.nx * j;
+        const T pz = (g.pnn[gidx(g, i, j, 1)] - g.pnn[e]) * g.dzi;
+        if (c == 0) {
+            const T ex = (T)a.gmx[i], zX = (T)a.gzx[q];
+            const T gx = ex * (g.pnn[e] - g.pnn[gidx(g, i - 1, j, 0)]) * g.dxi + zX * pz;
+            return g.un[e] + (T(-1.0) * ib * dt * eff * gx + g.un[e] * ib - g.un[e]);
+        }
+        const T ey = (T)a.gmy[j], zY = (T)a.gzy[q];
+        const T gy = ey * (g.pnn[e] - g.pnn[gidx(g, i, j - 1, 0)]) * g.dyi + zY * pz;
+        return g.vn[e] + (T(-1.0) * ib * dt * eff * gy + g.vn[e] * ib - g.vn[e]);
+    }
     if (c == 0) {
         const T ex = (T)a.gmx[i];
         return g.un[e] + (T(-1.0) * ib * dt * eff * (ex * (g.pnn[e] - g.pnn[gidx(g, i - 1, j, 0)]) * g.dxi) + g.un[e] * ib - g.un[e]);

@@ -104,6 +104,9 @@ template <typename T> struct BCArgs {
     double gdecay[8], ggain[8], ginv, gZinf, gtb;
     gptr<const double> gmx, gmy;
     double gmz, g_gama, g_eff, g_ib, gcpl, gzi;
+    // the same on the terrain-following grid, per ground node c = i + nx*j (NULL when flat):
+    // 1/Z_over_z, the cross terms zX, zY of the inverse Jacobian and 2 dt/(rho_e dz) 1/Z_over_z
+    gptr<const double> gez, gzx, gzy, gcplv;
     gptr<const long long> rec_idx;  // receiver samples of time level n (pressure phase)
     int nrec;
     gptr<double> rec_out;

@@ -248,7 +248,6 @@ std::string Model::build(const lee_deck &deck, const lee_options &o, int &code)
         if (d.bc[LEE_FACE_S] != LEE_BC_RIGID) return "the ground impedance replaces a rigid South face";
         if (d.mf_judge == 1) return "the ground impedance runs with a fixed frame";
         if (obstacle) return "no embedded obstacle with the ground impedance";
-        if (terrain) return "the ground interface plane is flat (no terrain with the ground impedance)";
         code = LEE_ERR_ARG;
         if (!(d.pore_q > 0) || !(d.pore_porosity > 0) || !(d.pore_resistivity >= 0) || !std::isfinite(d.pore_q) ||
             !std::isfinite(d.pore_porosity) || !std::isfinite(d.pore_resistivity))

@@ -493,24 +493,34 @@ def config_c3(time_domain: int = 3000, seed: int = 20261202, n: int = 64):
     return decks, recs
 
 
-def config_c4(ground, kind: str, residues=None, time_domain: int = 8000) -> Deck:
+C4_HILL = (0.0, 12.0, 1.0, 5.0)  # x0, y0, height, sigma [m]: a hill between the C4 receivers
+
+
+def config_c4(ground, kind: str, residues=None, time_domain: int = 8000, flat: bool = False) -> Deck:
     """Learned ground: 1024x512x160 air points at 0.04 m (the spacing is not specified; the
-    512x256x128 benchmark's is used), flat ground, 20-point PML on the sides and top, no
-    wind, no moving frame, Gaussian pulse of half-width 3 dx at 0.5 m, 3 m into the domain,
-    FB_p_v (the ground
-    impedance is discretised for it) at CFL 0.5, 8000 steps.  kind 'porous': the full solve
-    over the porous layer of `ground` (a ground.Ground, depth points + the two interface
-    planes); kind 'impedance': air only, the ground interface plane closed by the impedance
-    with `residues` on ground.R_GRID."""
+    512x256x128 benchmark's is used), curvilinear: terrain-following over a Gaussian hill of
+    height 1 m and sigma 5 m at y = 12 m on the receiver line (the height is not specified;
+    2 m, as in the 512x256x128 workload, breaks the FB_p_v step bound at CFL 0.5 on this
+    160-point column), 20-point PML on the sides and top, no wind, no moving frame, Gaussian
+    pulse of half-width 3 dx at 0.5 m above the ground, 3 m into the domain, FB_p_v (the
+    ground impedance is discretised for it) at CFL 0.5, 8000 steps.  kind 'porous': the full
+    solve over the porous layer of `ground` (a ground.Ground, depth points + the two interface
+    planes, following the terrain); kind 'impedance': air only, the ground interface plane on
+    the terrain closed by the impedance with `residues` on ground.R_GRID.  flat=True drops the
+    hill."""
     from . import ground as G
     d = 0.04
     kw = dict(pml=20, time_domain=time_domain, gauss_width=2 * 3 * d, src=(0.0, 3.0, 0.5), rcv=(0.0, 6.0, 1.2),
               fft_m=14, pml_y_time=0)
     if kind == "porous":
         dk = box(1024, 512, 160, d, bc=(2, 3, 2, 2, 2, 2), porous_cells=ground.depth + 2, **kw)
-        return G.porous_deck(dk, ground)
-    dk = box(1024, 512, 160, d, bc=(2, 1, 2, 2, 2, 2), **kw)
-    return G.impedance_deck(dk, ground, residues)
+        dk = G.porous_deck(dk, ground)
+    else:
+        dk = box(1024, 512, 160, d, bc=(2, 1, 2, 2, 2, 2), **kw)
+        dk = G.impedance_deck(dk, ground, residues)
+    if not flat:
+        dk.hills = [C4_HILL]
+    return dk
 
 
 def receivers_c4():

@@ -1054,18 +1054,42 @@ static void bc_v(ogrid *g, int face)
  * a_k / (r_k + s tau)), through the TDBC recursion of calculation_2D.cpp:986-1028 with K
  * poles.  The interior nodes of the plane are updated like stencil output, before the faces
  * (N E W F B, there is no South face), which then treat the plane's edge nodes as on every
- * other plane.  Flat grid: the plane's metric is 1/X_over_x, 1/Y_over_y and 1/Z_over_z of
- * the plane k = 1. */
+ * other plane.  The plane's metric is that of the plane k = 1: ex = 1/X_over_x, ey =
+ * 1/Y_over_y, ez = 1/Z_over_z and, on the terrain-following grid (Z_over_z = Zz sh, Z_over_x =
+ * hx tk, Z_over_y = hy tk), the cross terms zX = -(Z_over_x ex) ez, zY = -(Z_over_y ey) ez of
+ * the inverse Jacobian; the z derivatives of the plane are one-sided towards k = 1 (the
+ * velocities' pressure gradient d/dX = ex d/dx + zX d/dz takes (p1 - p0) dz^-1).  With no
+ * hill (zX = zY = 0, sh = 1) every term reduces to the flat closure's, bit for bit. */
+static void ground_metric(const ogrid *g, int i, int j, double *ex, double *ey, double *ez, double *zX, double *zY)
+{
+    *ex = 1.0 / g->Xx[i];
+    *ey = 1.0 / g->Yy[j];
+    if (!g->terrain) { *ez = 1.0 / g->Zz[1]; *zX = 0.0; *zY = 0.0; return; }
+    size_t c = (size_t)i + (size_t)g->nx * j;
+    *ez = 1.0 / (g->Zz[1] * g->sh[c]);
+    *zX = -((g->hx[c] * g->tk[1]) * *ex) * *ez;
+    *zY = -((g->hy[c] * g->tk[1]) * *ey) * *ez;
+}
+
 static void bc_p_ground(orc_sim *s)
 {
     ogrid *g = &s->air;
     for (int j = 1; j < g->ny - 1; ++j)
         for (int i = 1; i < g->nx - 1; ++i) {
             size_t e = IDX(g, i, j, 0);
-            double ex = 1.0 / g->Xx[i], ey = 1.0 / g->Yy[j], ez = 1.0 / g->Zz[1];
-            double div = ex * g->dxi * (g->un[IDX(g, i + 1, j, 0)] - g->un[e]) +
-                         ey * g->dyi * (g->vn[IDX(g, i, j + 1, 0)] - g->vn[e]) +
-                         ez * g->dzi * (g->wn[IDX(g, i, j, 1)] - g->wn[e]);
+            double ex, ey, ez, zX, zY;
+            ground_metric(g, i, j, &ex, &ey, &ez, &zX, &zY);
+            double div;
+            if (!g->terrain) {
+                div = ex * g->dxi * (g->un[IDX(g, i + 1, j, 0)] - g->un[e]) +
+                      ey * g->dyi * (g->vn[IDX(g, i, j + 1, 0)] - g->vn[e]) +
+                      ez * g->dzi * (g->wn[IDX(g, i, j, 1)] - g->wn[e]);
+            } else {
+                double dux = g->un[IDX(g, i + 1, j, 0)] - g->un[e], duz = g->un[IDX(g, i, j, 1)] - g->un[e];
+                double dvy = g->vn[IDX(g, i, j + 1, 0)] - g->vn[e], dvz = g->vn[IDX(g, i, j, 1)] - g->vn[e];
+                div = ((ex * g->dxi * dux + zX * g->dzi * duz) + (ey * g->dyi * dvy + zY * g->dzi * dvz)) +
+                      ez * g->dzi * (g->wn[IDX(g, i, j, 1)] - g->wn[e]);
+            }
             g->pnn[e] = g->pn[e] + (-s->g_gama * div);
         }
 }
@@ -1075,15 +1099,23 @@ static void bc_v_ground(orc_sim *s)
     ogrid *g = &s->air;
     int nx = g->nx;
     double ib = s->g_ib, dt = g->dt, eff = s->g_eff;
-    double ez = 1.0 / g->Zz[1];
-    double cpl = 2.0 * dt * eff * ez * g->dzi;   /* dt/rho_e over half a cell */
     double zi = s->gZinf / s->g_inv;             /* Zinf (1 + tb sum_k a_k) */
     for (int j = 1; j < g->ny - 1; ++j)
         for (int i = 1; i < nx - 1; ++i) {
             size_t e = IDX(g, i, j, 0);
-            double ex = 1.0 / g->Xx[i], ey = 1.0 / g->Yy[j];
-            g->unn[e] = g->un[e] + (-1.0 * ib * dt * eff * (ex * (g->pnn[e] - g->pnn[IDX(g, i - 1, j, 0)]) * g->dxi) + g->un[e] * ib - g->un[e]);
-            g->vnn[e] = g->vn[e] + (-1.0 * ib * dt * eff * (ey * (g->pnn[e] - g->pnn[IDX(g, i, j - 1, 0)]) * g->dyi) + g->vn[e] * ib - g->vn[e]);
+            double ex, ey, ez, zX, zY;
+            ground_metric(g, i, j, &ex, &ey, &ez, &zX, &zY);
+            double cpl = 2.0 * dt * eff * ez * g->dzi;   /* dt/rho_e over half a cell */
+            if (!g->terrain) {
+                g->unn[e] = g->un[e] + (-1.0 * ib * dt * eff * (ex * (g->pnn[e] - g->pnn[IDX(g, i - 1, j, 0)]) * g->dxi) + g->un[e] * ib - g->un[e]);
+                g->vnn[e] = g->vn[e] + (-1.0 * ib * dt * eff * (ey * (g->pnn[e] - g->pnn[IDX(g, i, j - 1, 0)]) * g->dyi) + g->vn[e] * ib - g->vn[e]);
+            } else {
+                double pz = (g->pnn[IDX(g, i, j, 1)] - g->pnn[e]) * g->dzi;
+                double gx = ex * (g->pnn[e] - g->pnn[IDX(g, i - 1, j, 0)]) * g->dxi + zX * pz;
+                double gy = ey * (g->pnn[e] - g->pnn[IDX(g, i, j - 1, 0)]) * g->dyi + zY * pz;
+                g->unn[e] = g->un[e] + (-1.0 * ib * dt * eff * gx + g->un[e] * ib - g->un[e]);
+                g->vnn[e] = g->vn[e] + (-1.0 * ib * dt * eff * gy + g->vn[e] * ib - g->vn[e]);
+            }
             double *Psi = &s->Gpsi[((size_t)i + (size_t)nx * j) * s->gK];
             double temp_psi = 0;
             for (int q = 0; q < s->gK; ++q) temp_psi += s->g_decay[q] * Psi[q];
@@ -1508,8 +1540,8 @@ orc_sim *orc_create_opt(const lee_deck *d, const lee_options *opt, int fill_nn_n
         if (opt->ground_poles > LEE_MAX_POLES || !(opt->ground_zinf > 0) || !(opt->ground_tau > 0)) {
             orc_destroy(s); FAIL("bad ground impedance");
         }
-        if (!(d->pore_q > 0) || !(d->pore_porosity > 0) || !(d->pore_resistivity >= 0) || opt->n_hills > 0) {
-            orc_destroy(s); FAIL("the ground impedance needs the top material (pore_q, pore_porosity, pore_resistivity) and a flat ground");
+        if (!(d->pore_q > 0) || !(d->pore_porosity > 0) || !(d->pore_resistivity >= 0)) {
+            orc_destroy(s); FAIL("the ground impedance needs the top material (pore_q, pore_porosity, pore_resistivity)");
         }
         {
             double eff_density = d->pore_q * d->pore_q / d->pore_porosity / s->Aav;

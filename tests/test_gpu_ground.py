@@ -40,6 +40,28 @@ def test_ground_impedance_parity(lee, orc, pml, graph, K):
     assert e <= 1e-10 and fe <= 1e-10 and pe <= 1e-10
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_ground_impedance_on_terrain_parity(lee, orc, graph):
+    """The ground plane on the terrain-following grid (a 0.15 m hill under the source), FP64
+    against the oracle: fields, receivers and histories."""
+    d = deck(lee, 5)
+    d.hills = [(0.1, 0.7, 0.15, 0.3)]
+    print(f"[gpu ground] terrain graph={graph}", flush=True)
+    s = lee.Solver(d, precision=64, receivers=REC, use_graph=graph)
+    s.frame_setup(0)
+    got = s.step(d.time_domain)
+    o = orc.Oracle(d, receivers=REC)
+    o.frame_setup(0)
+    ref = o.step(d.time_domain)
+    e = rel(got, ref)
+    fe = max(rel(a, b) for a, b in zip(s.state(), o.state()))
+    pe = rel(s.ground_psi(), o.ground_psi(8))
+    print(f"  rec {e:.2e} fields {fe:.2e} histories {pe:.2e}", flush=True)
+    s.close()
+    assert np.abs(ref).max() > 0
+    assert e <= 1e-10 and fe <= 1e-10 and pe <= 1e-10
+
+
 def test_ground_impedance_fp32(lee):
     d = deck(lee, 5)
     a = lee.Solver(d, precision=64, receivers=REC)
