@@ -53,7 +53,7 @@ template <typename T> __device__ __forceinline__ T p_pore_walk(const BCArgs<T> &
 // TDBC recursion of calculation_2D.cpp:986-1028 with K poles.  The plane's metric is that of
 // the plane k = 1: gmx[i] = 1/X_over_x, gmy[j] = 1/Y_over_y, gmz = 1/Z_over_z on a flat grid;
 // on the terrain-following grid 1/Z_over_z and the cross terms zX, zY of the inverse Jacobian
-// per node (gez, gzx, gzy), with the plane's z derivatives one-sided towards k = 1.  With no
+// per node (the tables of gter), with the plane's z derivatives one-sided towards k = 1.  With no
 // hill every term reduces to the flat closure's bit for bit (the cross terms are zeros).
 template <typename T> __device__ __forceinline__ T ground_p(const BCArgs<T> &a, int i, int j)
 {
@@ -61,9 +61,9 @@ template <typename T> __device__ __forceinline__ T ground_p(const BCArgs<T> &a, 
     const size_t e = gidx(g, i, j, 0);
     const T ex = (T)a.gmx[i], ey = (T)a.gmy[j];
     T div;
-    if (a.gez) {
-        const size_t c = (size_t)i + (size_t)g.nx * j;
-        const T ez = (T)a.gez[c], zX = (T)a.gzx[c], zY = (T)a.gzy[c];
+    if (a.gter) {
+        const size_t c = (size_t)i + (size_t)g.nx * j, n2 = (size_t)g.nx * g.ny;
+        const T ez = (T)a.gter[c], zX = (T)a.gter[n2 + c], zY = (T)a.gter[2 * n2 + c];
         const T dux = g.un[gidx(g, i + 1, j, 0)] - g.un[e], duz = g.un[gidx(g, i, j, 1)] - g.un[e];
         const T dvy = g.vn[gidx(g, i, j + 1, 0)] - g.vn[e], dvz = g.vn[gidx(g, i, j, 1)] - g.vn[e];
         div = ((ex * g.dxi * dux + zX * g.dzi * duz) + (ey * g.dyi * dvy + zY * g.dzi * dvz)) +
@@ -146,7 +146,7 @@ template <typename T> __device__ __forceinline__ double ground_w_of(const BCArgs
     const Grid<T> &g = a.air;
     const size_t e = gidx(g, i, j, 0);
     const double hist = a.gZinf * temp_psi;
-    const double cpl = a.gcplv ? a.gcplv[(size_t)i + (size_t)g.nx * j] : a.gcpl;
+    const double cpl = a.gter ? a.gter[3 * (size_t)g.nx * g.ny + (size_t)i + (size_t)g.nx * j] : a.gcpl;
     return ((double)g.wn[e] - cpl * ((double)g.pnn[e] - hist)) * a.g_ib / (1.0 + a.g_ib * cpl * a.gzi);
 }
 
@@ -165,26 +165,26 @@ template <typename T> __device__ __forceinline__ T ground_vel(const BCArgs<T> &a
     const Grid<T> &g = a.air;
     const size_t e = gidx(g, i, j, 0);
     const T ib = (T)a.g_ib, dt = a.dt, eff = (T)a.g_eff;
-    if (a.gez && c < 2) {
-        // terrain: d/dX = ex d/dx + zX d/dz, d/dY = ey d/dy + zY d/dz, d/dz one-sided to k = 1
-        const size_t q = (size_t)i + (size_t)g.nx * j;
-        const T pz = (g.pnn[gidx(g, i, j, 1)] - g.pnn[e]) * g.dzi;
-        if (c == 0) {
-            const T ex = (T)a.gmx[i], zX = (T)a.gzx[q];
-            const T gx = ex * (g.pnn[e] - g.pnn[gidx(g, i - 1, j, 0)]) * g.dxi + zX * pz;
-            return g.un[e] + (T(-1.0) * ib * dt * eff * gx + g.un[e] * ib - g.un[e]);
-        }
-        const T ey = (T)a.gmy[j], zY = (T)a.gzy[q];
-        const T gy = ey * (g.pnn[e] - g.pnn[gidx(g, i, j - 1, 0)]) * g.dyi + zY * pz;
-        return g.vn[e] + (T(-1.0) * ib * dt * eff * gy + g.vn[e] * ib - g.vn[e]);
+    // terrain: d/dX = ex d/dx + zX d/dz, d/dY = ey d/dy + zY d/dz with d/dz one-sided to
+    // k = 1; on a flat grid the cross term is absent
+    T pz = T(0), zX = T(0), zY = T(0);
+    if (a.gter) {
+        const size_t q = (size_t)i + (size_t)g.nx * j, n2 = (size_t)g.nx * g.ny;
+        pz = (g.pnn[gidx(g, i, j, 1)] - g.pnn[e]) * g.dzi;
+        zX = (T)a.gter[n2 + q];
+        zY = (T)a.gter[2 * n2 + q];
     }
     if (c == 0) {
         const T ex = (T)a.gmx[i];
-        return g.un[e] + (T(-1.0) * ib * dt * eff * (ex * (g.pnn[e] - g.pnn[gidx(g, i - 1, j, 0)]) * g.dxi) + g.un[e] * ib - g.un[e]);
+        T gx = ex * (g.pnn[e] - g.pnn[gidx(g, i - 1, j, 0)]) * g.dxi;
+        if (a.gter) gx = gx + zX * pz;
+        return g.un[e] + (T(-1.0) * ib * dt * eff * gx + g.un[e] * ib - g.un[e]);
     }
     if (c == 1) {
         const T ey = (T)a.gmy[j];
-        return g.vn[e] + (T(-1.0) * ib * dt * eff * (ey * (g.pnn[e] - g.pnn[gidx(g, i, j - 1, 0)]) * g.dyi) + g.vn[e] * ib - g.vn[e]);
+        T gy = ey * (g.pnn[e] - g.pnn[gidx(g, i, j - 1, 0)]) * g.dyi;
+        if (a.gter) gy = gy + zY * pz;
+        return g.vn[e] + (T(-1.0) * ib * dt * eff * gy + g.vn[e] * ib - g.vn[e]);
     }
     return (T)ground_w(a, i, j);
 }

@@ -88,8 +88,9 @@ struct lee_ctx {
     double *gpsi = nullptr, *gpsi2 = nullptr;  // ground impedance histories (South face)
     double *gmx = nullptr, *gmy = nullptr;     // ground plane metric 1/X_over_x, 1/Y_over_y
     double gmz = 0;                            // and 1/Z_over_z of the plane k = 1
-    // over terrain, per ground node: 1/Z_over_z, zX, zY and the half-cell coupling
-    double *gez = nullptr, *gzx = nullptr, *gzy = nullptr, *gcplv = nullptr;
+    // over terrain, per ground node: 1/Z_over_z, zX, zY and the half-cell coupling, four
+    // tables of nx*ny values one after the other
+    double *gter = nullptr;
     long long *rec_idx = nullptr;
     double *rec_buf = nullptr;
     int *flag = nullptr;
@@ -345,7 +346,7 @@ template <typename T> BCArgs<T> bc_args(lee_ctx *c, const AirArgs<T> &aa, double
     for (int q = 0; q < 8; ++q) { b.gdecay[q] = m.g_decay[q]; b.ggain[q] = m.g_gain[q]; }
     b.ginv = m.g_inv; b.gZinf = m.gZinf; b.gtb = m.g_tb;
     b.gmx = c->gmx; b.gmy = c->gmy; b.gmz = c->gmz;
-    b.gez = c->gez; b.gzx = c->gzx; b.gzy = c->gzy; b.gcplv = c->gcplv;
+    b.gter = c->gter;
     b.g_gama = m.g_gama; b.g_eff = m.g_eff; b.g_ib = m.g_ib;
     b.gcpl = 2.0 * m.dt * m.g_eff * c->gmz * m.dzi;
     b.gzi = m.ground ? m.gZinf / m.g_inv : 0.0;
@@ -384,7 +385,8 @@ int upload_frame(lee_ctx *c, int N)
             // the plane's metric on the terrain-following grid (that of the plane k = 1):
             // Z_over_z = Zz sh, Z_over_x = hx tk, Z_over_y = hy tk
             const size_t n2 = (size_t)m.nx * m.ny;
-            std::vector<double> ez(n2, 0.0), zx(n2, 0.0), zy(n2, 0.0), cp(n2, 0.0);
+            std::vector<double> tab(4 * n2, 0.0);
+            double *ez = tab.data(), *zx = ez + n2, *zy = zx + n2, *cp = zy + n2;
             for (int j = 1; j + 1 < m.ny; ++j)
                 for (int i = 1; i + 1 < m.nx; ++i) {
                     const size_t q = (size_t)i + (size_t)m.nx * j;
@@ -393,9 +395,7 @@ int upload_frame(lee_ctx *c, int N)
                     zy[q] = -((t.ter.hy[q] * t.ter.tk[1]) * gy[j]) * ez[q];
                     cp[q] = 2.0 * m.dt * m.g_eff * ez[q] * m.dzi;
                 }
-            if ((rc = upload_f64(c, c->gez, ez.data(), n2)) || (rc = upload_f64(c, c->gzx, zx.data(), n2)) ||
-                (rc = upload_f64(c, c->gzy, zy.data(), n2)) || (rc = upload_f64(c, c->gcplv, cp.data(), n2)))
-                return rc;
+            if ((rc = upload_f64(c, c->gter, tab.data(), tab.size()))) return rc;
         }
     }
     const Profile *P[6] = {&t.x1, &t.x2, &t.y1, &t.y2, &t.z1, &t.z2};
@@ -995,12 +995,7 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
         if ((rc = dalloc(C, (void **)&C->gpsi, ng)) || (rc = dalloc(C, (void **)&C->gpsi2, ng))) return rc;
         if ((rc = dalloc(C, (void **)&C->gmx, (size_t)m.nx * 8)) || (rc = dalloc(C, (void **)&C->gmy, (size_t)m.ny * 8)))
             return rc;
-        if (m.terrain) {
-            const size_t n2 = (size_t)m.nx * m.ny * 8;
-            if ((rc = dalloc(C, (void **)&C->gez, n2)) || (rc = dalloc(C, (void **)&C->gzx, n2)) ||
-                (rc = dalloc(C, (void **)&C->gzy, n2)) || (rc = dalloc(C, (void **)&C->gcplv, n2)))
-                return rc;
-        }
+        if (m.terrain && (rc = dalloc(C, (void **)&C->gter, (size_t)4 * m.nx * m.ny * 8))) return rc;
     }
     int nr = (int)m.ri.size();
     if ((rc = dalloc(C, (void **)&C->rec_idx, (size_t)std::max(nr, 1) * sizeof(long long)))) return rc;

@@ -104,9 +104,10 @@ template <typename T> struct BCArgs {
     double gdecay[8], ggain[8], ginv, gZinf, gtb;
     gptr<const double> gmx, gmy;
     double gmz, g_gama, g_eff, g_ib, gcpl, gzi;
-    // the same on the terrain-following grid, per ground node c = i + nx*j (NULL when flat):
-    // 1/Z_over_z, the cross terms zX, zY of the inverse Jacobian and 2 dt/(rho_e dz) 1/Z_over_z
-    gptr<const double> gez, gzx, gzy, gcplv;
+    // the same on the terrain-following grid (NULL when flat): four tables of nx*ny values per
+    // ground node c = i + nx*j, one after the other -- 1/Z_over_z, the cross terms zX, zY of
+    // the inverse Jacobian, and 2 dt/(rho_e dz) 1/Z_over_z
+    gptr<const double> gter;
     gptr<const long long> rec_idx;  // receiver samples of time level n (pressure phase)
     int nrec;
     gptr<double> rec_out;
