@@ -50,26 +50,27 @@ template <typename T> __device__ __forceinline__ T p_pore_walk(const BCArgs<T> &
 // faces N E W F B then treat its edge nodes as on every other plane, and there is no South
 // face.  The normal velocity (half a cell below the plane) follows the porous momentum
 // equation with the pressure half a cell further down given by the impedance through the
-// TDBC recursion of calculation_2D.cpp:986-1028 with K poles.  The plane's metric is that of
-// the plane k = 1: gmx[i] = 1/X_over_x, gmy[j] = 1/Y_over_y, gmz = 1/Z_over_z on a flat grid;
-// on the terrain-following grid 1/Z_over_z and the cross terms zX, zY of the inverse Jacobian
-// per node (the tables of gter), with the plane's z derivatives one-sided towards k = 1.  With no
-// hill every term reduces to the flat closure's bit for bit (the cross terms are zeros).
+// TDBC recursion of calculation_2D.cpp:986-1028 with K poles.  The plane's metric is the
+// porous top plane's: gmx[i] = 1/X_over_x, gmy[j] = 1/Y_over_y, gmz = 1/Z_over_z of the plane
+// k = 1, and on the terrain-following grid the cross terms zX, zY of the inverse Jacobian per
+// node (the tables of gter) of a layer shifted rigidly with the ground, as porous.cpp's grid is
+// under airpore.cpp's hills, not the air column's compressed metric.  The plane's z derivatives
+// are one-sided towards k = 1.  With no hill every term reduces to the flat closure's bit for
+// bit (the cross terms are zeros).
 template <typename T> __device__ __forceinline__ T ground_p(const BCArgs<T> &a, int i, int j)
 {
     const Grid<T> &g = a.air;
     const size_t e = gidx(g, i, j, 0);
-    const T ex = (T)a.gmx[i], ey = (T)a.gmy[j];
+    const T ex = (T)a.gmx[i], ey = (T)a.gmy[j], ez = (T)a.gmz;
     T div;
     if (a.gter) {
         const size_t c = (size_t)i + (size_t)g.nx * j, n2 = (size_t)g.nx * g.ny;
-        const T ez = (T)a.gter[c], zX = (T)a.gter[n2 + c], zY = (T)a.gter[2 * n2 + c];
+        const T zX = (T)a.gter[c], zY = (T)a.gter[n2 + c];
         const T dux = g.un[gidx(g, i + 1, j, 0)] - g.un[e], duz = g.un[gidx(g, i, j, 1)] - g.un[e];
         const T dvy = g.vn[gidx(g, i, j + 1, 0)] - g.vn[e], dvz = g.vn[gidx(g, i, j, 1)] - g.vn[e];
         div = ((ex * g.dxi * dux + zX * g.dzi * duz) + (ey * g.dyi * dvy + zY * g.dzi * dvz)) +
               ez * g.dzi * (g.wn[gidx(g, i, j, 1)] - g.wn[e]);
     } else {
-        const T ez = (T)a.gmz;
         div = ex * g.dxi * (g.un[gidx(g, i + 1, j, 0)] - g.un[e]) + ey * g.dyi * (g.vn[gidx(g, i, j + 1, 0)] - g.vn[e]) +
               ez * g.dzi * (g.wn[gidx(g, i, j, 1)] - g.wn[e]);
     }
@@ -146,8 +147,7 @@ template <typename T> __device__ __forceinline__ double ground_w_of(const BCArgs
     const Grid<T> &g = a.air;
     const size_t e = gidx(g, i, j, 0);
     const double hist = a.gZinf * temp_psi;
-    const double cpl = a.gter ? a.gter[3 * (size_t)g.nx * g.ny + (size_t)i + (size_t)g.nx * j] : a.gcpl;
-    return ((double)g.wn[e] - cpl * ((double)g.pnn[e] - hist)) * a.g_ib / (1.0 + a.g_ib * cpl * a.gzi);
+    return ((double)g.wn[e] - a.gcpl * ((double)g.pnn[e] - hist)) * a.g_ib / (1.0 + a.g_ib * a.gcpl * a.gzi);
 }
 
 // The ground histories are pole-major, Psi_q of node (i, j) at q * nx * ny + i + nx * j, so
@@ -171,8 +171,8 @@ template <typename T> __device__ __forceinline__ T ground_vel(const BCArgs<T> &a
     if (a.gter) {
         const size_t q = (size_t)i + (size_t)g.nx * j, n2 = (size_t)g.nx * g.ny;
         pz = (g.pnn[gidx(g, i, j, 1)] - g.pnn[e]) * g.dzi;
-        zX = (T)a.gter[n2 + q];
-        zY = (T)a.gter[2 * n2 + q];
+        zX = (T)a.gter[q];
+        zY = (T)a.gter[n2 + q];
     }
     if (c == 0) {
         const T ex = (T)a.gmx[i];

@@ -88,8 +88,8 @@ struct lee_ctx {
     double *gpsi = nullptr, *gpsi2 = nullptr;  // ground impedance histories (South face)
     double *gmx = nullptr, *gmy = nullptr;     // ground plane metric 1/X_over_x, 1/Y_over_y
     double gmz = 0;                            // and 1/Z_over_z of the plane k = 1
-    // over terrain, per ground node: 1/Z_over_z, zX, zY and the half-cell coupling, four
-    // tables of nx*ny values one after the other
+    // over terrain, the cross terms zX, zY of the plane's inverse Jacobian per ground node,
+    // two tables of nx*ny values one after the other
     double *gter = nullptr;
     long long *rec_idx = nullptr;
     double *rec_buf = nullptr;
@@ -382,18 +382,16 @@ int upload_frame(lee_ctx *c, int N)
         HIPCHK(c, hipMemcpyAsync(c->gmy, gy.data(), gy.size() * 8, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (m.terrain) {
-            // the plane's metric on the terrain-following grid (that of the plane k = 1):
-            // Z_over_z = Zz sh, Z_over_x = hx tk, Z_over_y = hy tk
+            // the plane's metric on the terrain-following grid: that of a porous top plane
+            // shifted rigidly with the ground (Z_over_z = Zz, Z_over_x = hx, Z_over_y = hy)
             const size_t n2 = (size_t)m.nx * m.ny;
-            std::vector<double> tab(4 * n2, 0.0);
-            double *ez = tab.data(), *zx = ez + n2, *zy = zx + n2, *cp = zy + n2;
+            std::vector<double> tab(2 * n2, 0.0);
+            double *zx = tab.data(), *zy = zx + n2;
             for (int j = 1; j + 1 < m.ny; ++j)
                 for (int i = 1; i + 1 < m.nx; ++i) {
                     const size_t q = (size_t)i + (size_t)m.nx * j;
-                    ez[q] = 1.0 / (t.Zz[1] * t.ter.sh[q]);
-                    zx[q] = -((t.ter.hx[q] * t.ter.tk[1]) * gx[i]) * ez[q];
-                    zy[q] = -((t.ter.hy[q] * t.ter.tk[1]) * gy[j]) * ez[q];
-                    cp[q] = 2.0 * m.dt * m.g_eff * ez[q] * m.dzi;
+                    zx[q] = -(t.ter.hx[q] * gx[i]) * c->gmz;
+                    zy[q] = -(t.ter.hy[q] * gy[j]) * c->gmz;
                 }
             if ((rc = upload_f64(c, c->gter, tab.data(), tab.size()))) return rc;
         }
@@ -995,7 +993,7 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
         if ((rc = dalloc(C, (void **)&C->gpsi, ng)) || (rc = dalloc(C, (void **)&C->gpsi2, ng))) return rc;
         if ((rc = dalloc(C, (void **)&C->gmx, (size_t)m.nx * 8)) || (rc = dalloc(C, (void **)&C->gmy, (size_t)m.ny * 8)))
             return rc;
-        if (m.terrain && (rc = dalloc(C, (void **)&C->gter, (size_t)4 * m.nx * m.ny * 8))) return rc;
+        if (m.terrain && (rc = dalloc(C, (void **)&C->gter, (size_t)2 * m.nx * m.ny * 8))) return rc;
     }
     int nr = (int)m.ri.size();
     if ((rc = dalloc(C, (void **)&C->rec_idx, (size_t)std::max(nr, 1) * sizeof(long long)))) return rc;
@@ -1954,13 +1952,12 @@ int lee_ensemble_create(int n, const lee_deck *decks, const lee_options *opts, l
             (rc = ens_alloc(E, &E->bnd_args[st], n * sb)))
             return rc;
     }
-    // block prefixes: interior tiles of the air and porous grids, boundary nodes in 256s
+    // block prefixes: interior columns of the air and porous grids in 256s per plane,
+    // boundary nodes in 256s
     std::vector<int> fa(1, 0), fp(1, 0), fb(1, 0);
     for (lee_ctx *c : E->m) {
         const Model &m = c->m;
-        auto tiles = [](int nx, int ny, int nz) {
-            return (long long)((nx + kTileX - 1) / kTileX) * ((ny - 2 + kTileY - 1) / kTileY) * (nz - 2);
-        };
+        auto tiles = [](int nx, int ny, int nz) { return batch_blocks(nx, ny, nz); };
         long long ta = tiles(m.nx, m.ny, m.nz);
         long long nb = (c->prec == 64 ? boundary_count<double>(m.nx, m.ny, m.nz) : boundary_count<float>(m.nx, m.ny, m.nz));
         if (m.south_porous) {

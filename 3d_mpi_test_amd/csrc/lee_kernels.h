@@ -104,9 +104,8 @@ template <typename T> struct BCArgs {
     double gdecay[8], ggain[8], ginv, gZinf, gtb;
     gptr<const double> gmx, gmy;
     double gmz, g_gama, g_eff, g_ib, gcpl, gzi;
-    // the same on the terrain-following grid (NULL when flat): four tables of nx*ny values per
-    // ground node c = i + nx*j, one after the other -- 1/Z_over_z, the cross terms zX, zY of
-    // the inverse Jacobian, and 2 dt/(rho_e dz) 1/Z_over_z
+    // on the terrain-following grid (NULL when flat): the cross terms zX, zY of the plane's
+    // inverse Jacobian per ground node c = i + nx*j, two tables of nx*ny values one after the other
     gptr<const double> gter;
     gptr<const long long> rec_idx;  // receiver samples of time level n (pressure phase)
     int nrec;
@@ -157,8 +156,15 @@ template <typename T> void launch_rk_pore(const RKPoreArgs<T> &r, bool terrain, 
 // kernel; by value the block was copied to scratch for the slab indexing)
 template <typename T> void launch_rk_bnd(const RKBndArgs<T> &r, const RKBndArgs<T> *rd, hipStream_t s);
 // Ensemble forms: args[d] in device memory for member d, first[d] = its first block
-// (first[n] = nblocks); interior tiles of BX x BY points for air/pore, 256 boundary nodes for bnd.
-constexpr int kTileX = 64, kTileY = 4;
+// (first[n] = nblocks); 256 boundary nodes per block for bnd.  Air and porous blocks of 256
+// threads cover 256 consecutive columns i + nx*(j - 1) of the interior rows j = 1 .. ny-2 of
+// one plane (the member's rows are 64-128 points, so 64-wide tiles would leave about a
+// quarter of the lanes idle): batch_blocks of them per member.
+constexpr int kBatchBlock = 256;
+inline long long batch_blocks(int nx, int ny, int nz)
+{
+    return ((long long)nx * (ny - 2) + kBatchBlock - 1) / kBatchBlock * (nz - 2);
+}
 template <typename T>
 void launch_rk_air_batch(const RKArgs<T> *args, const int *first, int n, int nblocks, int wind, bool terrain, hipStream_t s);
 template <typename T>

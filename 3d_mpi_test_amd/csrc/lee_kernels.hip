@@ -21,7 +21,7 @@ namespace lee {
 namespace {
 
 constexpr int BX = 64, BY = 4;
-static_assert(BX == kTileX && BY == kTileY, "ensemble block prefixes are counted in BX x BY tiles");
+static_assert(BX * BY == kBatchBlock, "ensemble blocks are BX x BY threads");
 
 // Tile t of a flat grid whose blocks go round-robin to the 8 XCDs: XCD x runs the x-th
 // contiguous range of tiles, in order.
@@ -1095,6 +1095,19 @@ __device__ __forceinline__ int batch_member(const int *first, int n, int b)
     return lo;
 }
 
+// Point of thread (threadIdx.x, threadIdx.y) in block `local` of a member: blocks of 256
+// consecutive columns c = i + nx*(j - 1) over the interior rows, plane after plane; false for
+// a face column (i = 0, nx - 1) or past the last row.
+__device__ __forceinline__ bool batch_point(int nx, int ny, int local, int &i, int &j, int &k)
+{
+    const int per = (nx * (ny - 2) + kBatchBlock - 1) / kBatchBlock;
+    const int c = (local % per) * kBatchBlock + threadIdx.x + BX * threadIdx.y;
+    k = local / per + 1;
+    j = c / nx + 1;
+    i = c - (j - 1) * nx;
+    return j < ny - 1 && i >= 1 && i < nx - 1;
+}
+
 template <typename T, int WM, bool TERRAIN, bool COMP>
 __global__ __launch_bounds__(BX * BY) void k_rk_air_batch(const RKArgs<T> *__restrict__ args, const int *__restrict__ first,
                                                          int n)
@@ -1102,11 +1115,8 @@ __global__ __launch_bounds__(BX * BY) void k_rk_air_batch(const RKArgs<T> *__res
     const int d = batch_member(first, n, blockIdx.x);
     const RKArgs<T> &r = args[d];
     const int nx = r.a.g.nx, ny = r.a.g.ny;
-    const int tx = (nx + BX - 1) / BX, ty = (ny - 2 + BY - 1) / BY;
-    const int local = blockIdx.x - first[d];
-    const int i = (local % tx) * BX + threadIdx.x, j = ((local / tx) % ty) * BY + threadIdx.y + 1,
-              k = local / (tx * ty) + 1;
-    if (i < 1 || i >= nx - 1 || j >= ny - 1) return;
+    int i, j, k;
+    if (!batch_point(nx, ny, blockIdx.x - first[d], i, j, k)) return;
     rk_air_point<T, WM, TERRAIN, COMP>(r, i, j, k);
 }
 
@@ -1168,11 +1178,8 @@ __global__ __launch_bounds__(BX * BY) void k_rk_pore_batch(const RKPoreArgs<T> *
     const int d = batch_member(first, n, blockIdx.x);
     const RKPoreArgs<T> &r = args[d];
     const int nx = r.g.nx, ny = r.g.ny;
-    const int tx = (nx + BX - 1) / BX, ty = (ny - 2 + BY - 1) / BY;
-    const int local = blockIdx.x - first[d];
-    const int i = (local % tx) * BX + threadIdx.x, j = ((local / tx) % ty) * BY + threadIdx.y + 1,
-              k = local / (tx * ty) + 1;
-    if (i < 1 || i >= nx - 1 || j >= ny - 1) return;
+    int i, j, k;
+    if (!batch_point(nx, ny, blockIdx.x - first[d], i, j, k)) return;
     rk_pore_point<T, TERRAIN, COMP>(r, i, j, k);
 }
 

@@ -407,6 +407,8 @@ def il_errors(a, b, dt, fmin=F_BAND[0], fmax=F_BAND[1]):
     receivers carrying signal), in dB over [fmin, fmax]: (largest narrow-band difference, bins
     60 dB below the band peak skipped; largest one-third-octave band-level difference, centre
     frequencies 1000 * 2^(n/3) in the band, edges at 2^(+-1/6))."""
+    if not (np.isfinite(a).all() and np.isfinite(b).all()):
+        return float("inf"), float("inf")  # a run that blew up matches nothing
     energy = (a ** 2).sum(axis=0)
     use = np.nonzero(energy > 1e-6 * energy.max())[0]
     n = 1 << int(np.ceil(np.log2(a.shape[0])))
@@ -432,25 +434,30 @@ def il_errors(a, b, dt, fmin=F_BAND[0], fmax=F_BAND[1]):
 JUDGE_REC = [(0.0, y, h) for y in (2.6, 3.4) for h in (0.1, 0.6, 1.2)]
 
 
-def judge_deck(g: Ground, kind: str, residues=None, steps=1024, poles=None):
+def judge_deck(g: Ground, kind: str, residues=None, steps=1024, poles=None, hills=None):
     """A 120 x 120 x 60 box at 0.04 m (20-point PML on the sides and top), Gaussian pulse at
-    0.6 m, over g's porous layer ('porous') or the impedance closure ('impedance')."""
+    0.6 m, over g's porous layer ('porous') or the impedance closure ('impedance'); with
+    `hills` ((x0, y0, height, sigma) each) terrain-following over them."""
     lee = _lee()
     kw = dict(pml=20, src=(0.0, 1.6, 0.6), gauss_width=0.24, time_domain=steps)
     if kind == "porous":
         d = lee.deck.box(120, 120, 60, DZ, bc=(2, 3, 2, 2, 2, 2), porous_cells=g.depth + 2, **kw)
-        return porous_deck(d, g)
-    d = lee.deck.box(120, 120, 60, DZ, bc=(2, 1, 2, 2, 2, 2), **kw)
-    return impedance_deck(d, g, residues, poles=poles)
+        d = porous_deck(d, g)
+    else:
+        d = lee.deck.box(120, 120, 60, DZ, bc=(2, 1, 2, 2, 2, 2), **kw)
+        d = impedance_deck(d, g, residues, poles=poles)
+    if hills:
+        d.hills = list(hills)
+    return d
 
 
-def judge_3d(g: Ground, residues, precision=64, poles=None):
+def judge_3d(g: Ground, residues, precision=64, poles=None, hills=None):
     """IL differences (narrow-band, one-third octave) of the impedance closure with
     `residues` (on R_GRID or `poles`) against the full porous solve in the judge box."""
     lee = _lee()
     out = []
     for kind in ("porous", "impedance"):
-        d = judge_deck(g, kind, residues, poles=poles)
+        d = judge_deck(g, kind, residues, poles=poles, hills=hills)
         s = lee.Solver(d, precision=precision, receivers=JUDGE_REC, use_graph=USE_GRAPH)
         try:
             s.frame_setup(0)

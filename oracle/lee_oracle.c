@@ -1054,21 +1054,23 @@ static void bc_v(ogrid *g, int face)
  * a_k / (r_k + s tau)), through the TDBC recursion of calculation_2D.cpp:986-1028 with K
  * poles.  The interior nodes of the plane are updated like stencil output, before the faces
  * (N E W F B, there is no South face), which then treat the plane's edge nodes as on every
- * other plane.  The plane's metric is that of the plane k = 1: ex = 1/X_over_x, ey =
- * 1/Y_over_y, ez = 1/Z_over_z and, on the terrain-following grid (Z_over_z = Zz sh, Z_over_x =
- * hx tk, Z_over_y = hy tk), the cross terms zX = -(Z_over_x ex) ez, zY = -(Z_over_y ey) ez of
- * the inverse Jacobian; the z derivatives of the plane are one-sided towards k = 1 (the
+ * other plane.  The plane's metric is the porous top plane's, Z_over_z = Zz of the plane k = 1:
+ * ex = 1/X_over_x, ey = 1/Y_over_y, ez = 1/Z_over_z and, on the terrain-following grid, the
+ * cross terms zX = -(hx ex) ez, zY = -(hy ey) ez of the inverse Jacobian of a layer shifted
+ * rigidly with the ground (Z_over_x = hx, Z_over_y = hy; porous.cpp's grid under
+ * airpore.cpp's hills, sh = tk = 1 there), not the air column's compressed one: the closure
+ * stands for that plane.  The z derivatives of the plane are one-sided towards k = 1 (the
  * velocities' pressure gradient d/dX = ex d/dx + zX d/dz takes (p1 - p0) dz^-1).  With no
- * hill (zX = zY = 0, sh = 1) every term reduces to the flat closure's, bit for bit. */
+ * hill (zX = zY = 0) every term reduces to the flat closure's, bit for bit. */
 static void ground_metric(const ogrid *g, int i, int j, double *ex, double *ey, double *ez, double *zX, double *zY)
 {
     *ex = 1.0 / g->Xx[i];
     *ey = 1.0 / g->Yy[j];
-    if (!g->terrain) { *ez = 1.0 / g->Zz[1]; *zX = 0.0; *zY = 0.0; return; }
+    *ez = 1.0 / g->Zz[1];
+    if (!g->terrain) { *zX = 0.0; *zY = 0.0; return; }
     size_t c = (size_t)i + (size_t)g->nx * j;
-    *ez = 1.0 / (g->Zz[1] * g->sh[c]);
-    *zX = -((g->hx[c] * g->tk[1]) * *ex) * *ez;
-    *zY = -((g->hy[c] * g->tk[1]) * *ey) * *ez;
+    *zX = -(g->hx[c] * *ex) * *ez;
+    *zY = -(g->hy[c] * *ey) * *ez;
 }
 
 static void bc_p_ground(orc_sim *s)

@@ -148,3 +148,8 @@ def test_il_errors(lee):
     assert n == 0.0 and t == 0.0
     n, t = G.il_errors(a, 1.1 * a, 1e-4)
     assert n == pytest.approx(20 * np.log10(1.1), rel=1e-9) and t == pytest.approx(20 * np.log10(1.1), rel=1e-9)
+    # a run that blew up (inf, then nan) matches nothing, rather than nothing to compare
+    b = a.copy()
+    b[3000:, 1] = np.inf
+    assert G.il_errors(a, b, 1e-4) == (float("inf"), float("inf"))
+    assert G.il_errors(b * 0.0, a, 1e-4) == (float("inf"), float("inf"))
