@@ -14,6 +14,9 @@
 #include "lee_kernels.h"
 
 #include <algorithm>
+#include <climits>
+#include <utility>
+#include <vector>
 #include <type_traits>
 
 namespace lee {
@@ -31,6 +34,44 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
     return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + (b >> 3);
 }
 
+
+// A box of 64 x 4-point tiles: tile columns bx .. bx+nbx-1 (points i = 64 bx ...), tile
+// rows by .. by+nby-1 (points j = 4 by + 1 ...), planes k .. k+nk-1.  The split launches
+// cover the tile space of a domain with lists of such boxes, one list per set of slab axes
+// its points can meet (see split_regions).
+struct TileBox {
+    int bx, nbx, by, nby, k, nk;
+};
+constexpr int kMaxBoxes = 18;
+template <int NB> struct BoxList {
+    TileBox b[NB];
+    int first[NB + 1];  // first block of each box; first[n] = blocks in all, then INT_MAX
+    int n;
+};
+
+// Point of block t of a list: thread (threadIdx.x, threadIdx.y) of the box's tiles taken x
+// fastest, then y, then planes.  blockIdx is uniform, so this is scalar arithmetic on the
+// argument, read at constant offsets only: the box's fields as sums of products with 0/1
+// (selects between loaded values were merged into one load through a selected address,
+// which put the argument in private memory).
+template <int NB, int... Q>
+__device__ __forceinline__ void list_point_(const BoxList<NB> &l, int t, int &i, int &j, int &k,
+                                            std::integer_sequence<int, Q...>)
+{
+    const int q = (0 + ... + (int)(Q > 0 && t >= l.first[Q]));
+    const int bx = (0 + ... + ((q == Q) * l.b[Q].bx)), nbx = (0 + ... + ((q == Q) * l.b[Q].nbx));
+    const int by = (0 + ... + ((q == Q) * l.b[Q].by)), nby = (0 + ... + ((q == Q) * l.b[Q].nby));
+    const int k0 = (0 + ... + ((q == Q) * l.b[Q].k)), f = (0 + ... + ((q == Q) * l.first[Q]));
+    t -= f;
+    const int per = nbx * nby, kk = t / per, r = t - kk * per, ry = r / nbx;
+    k = k0 + kk;
+    i = (bx + (r - ry * nbx)) * 64 + threadIdx.x;
+    j = (by + ry) * BY + threadIdx.y + 1;
+}
+template <int NB> __device__ __forceinline__ void list_point(const BoxList<NB> &l, int t, int &i, int &j, int &k)
+{
+    list_point_(l, t, i, j, k, std::make_integer_sequence<int, NB>{});
+}
 
 template <typename T> __device__ __forceinline__ void metric(const Grid<T> &g, int i, int j, int k, T &mX, T &mY, T &mZ)
 {
@@ -74,16 +115,17 @@ template <typename T> struct OwnQ {
     size_t idx[6];
     bool in[6];
 };
-template <typename T> __device__ __forceinline__ OwnQ<T> own_slabs(const AirArgs<T> &a, int i, int j, int k)
+// L as for dp_layers: slabs of the axes left out are known not to hold the point.
+template <typename T, int L = 7> __device__ __forceinline__ OwnQ<T> own_slabs(const AirArgs<T> &a, int i, int j, int k)
 {
     const int nx = a.g.nx, ny = a.g.ny;
     OwnQ<T> o;
-    o.in[0] = a.qn[0] && i < a.qlo[0] + a.qn[0];
-    o.in[1] = a.qn[1] && i >= a.qlo[1];
-    o.in[2] = a.qn[2] && a.y1on && j < a.qlo[2] + a.qn[2];
-    o.in[3] = a.qn[3] && j >= a.qlo[3];
-    o.in[4] = a.qn[4] && k < a.qlo[4] + a.qn[4];
-    o.in[5] = a.qn[5] && k >= a.qlo[5];
+    o.in[0] = (L & 1) && a.qn[0] && i < a.qlo[0] + a.qn[0];
+    o.in[1] = (L & 1) && a.qn[1] && i >= a.qlo[1];
+    o.in[2] = (L & 2) && a.qn[2] && a.y1on && j < a.qlo[2] + a.qn[2];
+    o.in[3] = (L & 2) && a.qn[3] && j >= a.qlo[3];
+    o.in[4] = (L & 4) && a.qn[4] && k < a.qlo[4] + a.qn[4];
+    o.in[5] = (L & 4) && a.qn[5] && k >= a.qlo[5];
     o.idx[0] = o.in[0] ? qxi(i - a.qlo[0], j, k, a.qn[0], ny) : 0;
     o.idx[1] = o.in[1] ? qxi(i - a.qlo[1], j, k, a.qn[1], ny) : 0;
     o.idx[2] = o.in[2] ? qyi(i, j - a.qlo[2], k, nx, a.qn[2]) : 0;
@@ -257,7 +299,9 @@ __device__ __forceinline__ void load_nb(const AirArgs<T> &a, int i, int j, int k
 
 // The layer terms of the pressure increment (air.cpp:2044-2292): the x, y and z slabs with
 // their corner cross terms, added to r in the reference's order.  Reads the auxiliaries a.Qo.
-template <typename T, bool TERRAIN>
+// L: the slab axes compiled in (bit 0 x, bit 1 y, bit 2 z), for launches whose tiles can only
+// meet those slabs; 7 compiles all.
+template <typename T, bool TERRAIN, int L = 7>
 __device__ __forceinline__ void dp_layers(const AirArgs<T> &a, int i, int j, int k, T mX, T mY, T mZ, T zX, T zY, T pc,
                                           T ddx, T ddy, T dwz, T &r)
 {
@@ -280,7 +324,7 @@ __device__ __forceinline__ void dp_layers(const AirArgs<T> &a, int i, int j, int
         return v;
     };
     // X1 (air.cpp:2044-2057)
-    if (i < a.I1) {
+    if ((L & 1) && i < a.I1) {
         const T sx = a.x1.s[i], cx = a.x1.c[i];
         const size_t q = qxi(i, j, k, n0, ny), sz = (size_t)n0 * ny;
         gptr<const T> Qv = a.Qo[0][1], Qw = a.Qo[0][2];
@@ -289,7 +333,7 @@ __device__ __forceinline__ void dp_layers(const AirArgs<T> &a, int i, int j, int
         r += A - B;
     }
     // X2 (air.cpp:2059-2074)
-    if (i >= nx - a.I2) {
+    if ((L & 1) && i >= nx - a.I2) {
         const int ii = i - (nx - n1);
         const T sx = a.x2.s[i], cx = a.x2.c[i];
         const size_t q = qxi(ii, j, k, n1, ny), sz = (size_t)n1 * ny;
@@ -301,13 +345,13 @@ __device__ __forceinline__ void dp_layers(const AirArgs<T> &a, int i, int j, int
     // cross term of a y or z layer with an x layer (air.cpp:2091-2115 and the like):
     //   -(Qp_x*sx*so + gP*((1/bx-1)*so*(dQu_x/dX) + (1/bo-1)*sx*(dQ_o/dO)))*dt
     auto xcross_y = [&](T so, T dom) {
-        if (i < a.I1) {
+        if ((L & 1) && i < a.I1) {
             const T sx = a.x1.s[i], dxm = a.x1.d[i];
             const size_t q = qxi(i, j, k, n0, ny), sz = (size_t)n0 * ny;
             const T t1 = (dxm * so) * fX(a.Qo[0][0], q, sz);
             const T t2 = (dom * sx) * fY(a.Qo[0][1], q, n0, sz);
             r += (-((a.Qo[0][3][q] * sx) * so + gP * (t1 + t2))) * dt;
-        } else if (i >= nx - a.I2) {
+        } else if ((L & 1) && i >= nx - a.I2) {
             const int ii = i - (nx - n1);
             const T sx = a.x2.s[i], dxm = a.x2.d[i];
             const size_t q = qxi(ii, j, k, n1, ny), sz = (size_t)n1 * ny;
@@ -317,13 +361,13 @@ __device__ __forceinline__ void dp_layers(const AirArgs<T> &a, int i, int j, int
         }
     };
     auto xcross_z = [&](T so, T dom) {
-        if (i < a.I1) {
+        if ((L & 1) && i < a.I1) {
             const T sx = a.x1.s[i], dxm = a.x1.d[i];
             const size_t q = qxi(i, j, k, n0, ny), sz = (size_t)n0 * ny;
             const T t1 = (dxm * so) * fX(a.Qo[0][0], q, sz);
             const T t2 = (dom * sx) * ((mZ * (a.Qo[0][2][q + sz] - a.Qo[0][2][q])) * dzi);
             r += (-((a.Qo[0][3][q] * sx) * so + gP * (t1 + t2))) * dt;
-        } else if (i >= nx - a.I2) {
+        } else if ((L & 1) && i >= nx - a.I2) {
             const int ii = i - (nx - n1);
             const T sx = a.x2.s[i], dxm = a.x2.d[i];
             const size_t q = qxi(ii, j, k, n1, ny), sz = (size_t)n1 * ny;
@@ -333,7 +377,7 @@ __device__ __forceinline__ void dp_layers(const AirArgs<T> &a, int i, int j, int
         }
     };
     // Y1 (air.cpp:2076-2116)
-    if (j < a.J1) {
+    if ((L & 2) && j < a.J1) {
         const T sy = a.y1.s[j], cy = a.y1.c[j];
         const size_t q = qyi(i, j, k, nx, n2), sz = (size_t)nx * n2;
         gptr<const T> Qu = a.Qo[2][0], Qw = a.Qo[2][2];
@@ -343,7 +387,7 @@ __device__ __forceinline__ void dp_layers(const AirArgs<T> &a, int i, int j, int
         xcross_y(sy, a.y1.d[j]);
     }
     // Y2 (air.cpp:2118-2158)
-    if (j >= ny - a.J2) {
+    if ((L & 2) && j >= ny - a.J2) {
         const int jj = j - (ny - n3);
         const T sy = a.y2.s[j], cy = a.y2.c[j];
         const size_t q = qyi(i, jj, k, nx, n3), sz = (size_t)nx * n3;
@@ -355,13 +399,13 @@ __device__ __forceinline__ void dp_layers(const AirArgs<T> &a, int i, int j, int
     }
     // y-z cross term (air.cpp:2200-2224)
     auto ycross_z = [&](T sz_, T dzm) {
-        if (j < a.J1) {
+        if ((L & 2) && j < a.J1) {
             const T sy = a.y1.s[j], dym = a.y1.d[j];
             const size_t q = qyi(i, j, k, nx, n2), sz = (size_t)nx * n2;
             const T t1 = (dym * sz_) * fY(a.Qo[2][1], q, nx, sz);
             const T t2 = (dzm * sy) * ((mZ * (a.Qo[2][2][q + sz] - a.Qo[2][2][q])) * dzi);
             r += (-((a.Qo[2][3][q] * sy) * sz_ + gP * (t1 + t2))) * dt;
-        } else if (j >= ny - a.J2) {
+        } else if ((L & 2) && j >= ny - a.J2) {
             const int jj = j - (ny - n3);
             const T sy = a.y2.s[j], dym = a.y2.d[j];
             const size_t q = qyi(i, jj, k, nx, n3), sz = (size_t)nx * n3;
@@ -385,14 +429,14 @@ __device__ __forceinline__ void dp_layers(const AirArgs<T> &a, int i, int j, int
         r += A - B;
     };
     // Z1 (air.cpp:2160-2225)
-    if (k < a.K1) {
+    if ((L & 4) && k < a.K1) {
         const T sz = a.z1.s[k], cz = a.z1.c[k];
         zmain(sz, cz, a.Qo[4][0], a.Qo[4][1], qzi(i, j, k, nx, ny));
         xcross_z(sz, a.z1.d[k]);
         ycross_z(sz, a.z1.d[k]);
     }
     // Z2 (air.cpp:2227-2292)
-    if (k >= nz - a.K2) {
+    if ((L & 4) && k >= nz - a.K2) {
         const int kk = k - (nz - n5);
         const T sz = a.z2.s[k], cz = a.z2.c[k];
         zmain(sz, cz, a.Qo[5][0], a.Qo[5][1], qzi(i, j, kk, nx, ny));
@@ -405,7 +449,7 @@ __device__ __forceinline__ void dp_layers(const AirArgs<T> &a, int i, int j, int
 // The layer terms of the velocity increments (air.cpp:1579-1905): the x, y and z slabs with
 // their corner cross terms, added to du, dv, dw in the reference's order.  Reads the pressure
 // auxiliaries a.Qn[.][3] and the velocity auxiliaries a.Qo.
-template <typename T, bool TERRAIN>
+template <typename T, bool TERRAIN, int L = 7>
 __device__ __forceinline__ void duvw_layers(const AirArgs<T> &a, int i, int j, int k, T mX, T mY, T mZ, T zX, T zY,
                                             T uc, T vc, T wc, T gpx, T gpy, T gpz, T &du, T &dv, T &dw)
 {
@@ -428,7 +472,7 @@ __device__ __forceinline__ void duvw_layers(const AirArgs<T> &a, int i, int j, i
         return v;
     };
     // X1 / X2 (air.cpp:1579-1619)
-    if (i < a.I1) {
+    if ((L & 1) && i < a.I1) {
         const T sx = a.x1.s[i], cx = a.x1.c[i];
         const size_t q = qxi(i, j, k, n0, ny), sz = (size_t)n0 * ny;
         gptr<const T> Qp = a.Qn[0][3];
@@ -436,7 +480,7 @@ __device__ __forceinline__ void duvw_layers(const AirArgs<T> &a, int i, int j, i
         dv += ((-sx) * (vc + Aav * bY(Qp, q, n0, sz))) * dt;
         dw += ((-sx) * (wc + Aav * ((mZ * (Qp[q] - Qp[q - sz])) * dzi))) * dt;
     }
-    if (i >= nx - a.I2) {
+    if ((L & 1) && i >= nx - a.I2) {
         const int ii = i - (nx - n1);
         const T sx = a.x2.s[i], cx = a.x2.c[i];
         const size_t q = qxi(ii, j, k, n1, ny), sz = (size_t)n1 * ny;
@@ -451,14 +495,14 @@ __device__ __forceinline__ void duvw_layers(const AirArgs<T> &a, int i, int j, i
         du += ((-sy) * (uc + Aav * bX(Qp, qy, szy))) * dt;
         dv += ((cy * Aav) * gpy) * dt - (sy * vc) * dt;
         dw += ((-sy) * (wc + Aav * ((mZ * (Qp[qy] - Qp[qy - szy])) * dzi))) * dt;
-        if (i < a.I1) {
+        if ((L & 1) && i < a.I1) {
             const T sx = a.x1.s[i], dxm = a.x1.d[i];
             const size_t q = qxi(i, j, k, n0, ny), sz = (size_t)n0 * ny;
             gptr<const T> Xp = a.Qn[0][3];
             du += (-((a.Qo[0][0][q] * sx) * sy + ((dxm * sy) * Aav) * bX(Xp, q, sz))) * dt;
             dv += (-((a.Qo[0][1][q] * sx) * sy + ((dym * sx) * Aav) * bY(Xp, q, n0, sz))) * dt;
             dw += (((-a.Qo[0][2][q]) * sx) * sy) * dt;
-        } else if (i >= nx - a.I2) {
+        } else if ((L & 1) && i >= nx - a.I2) {
             const int ii = i - (nx - n1);
             const T sx = a.x2.s[i], dxm = a.x2.d[i];
             const size_t q = qxi(ii, j, k, n1, ny), sz = (size_t)n1 * ny;
@@ -468,21 +512,21 @@ __device__ __forceinline__ void duvw_layers(const AirArgs<T> &a, int i, int j, i
             dw += (((-a.Qo[1][2][q]) * sx) * sy) * dt;
         }
     };
-    if (j < a.J1) ylayer(a.y1.s[j], a.y1.c[j], a.y1.d[j], a.Qn[2][3], qyi(i, j, k, nx, n2), n2);
-    if (j >= ny - a.J2) ylayer(a.y2.s[j], a.y2.c[j], a.y2.d[j], a.Qn[3][3], qyi(i, j - (ny - n3), k, nx, n3), n3);
+    if ((L & 2) && j < a.J1) ylayer(a.y1.s[j], a.y1.c[j], a.y1.d[j], a.Qn[2][3], qyi(i, j, k, nx, n2), n2);
+    if ((L & 2) && j >= ny - a.J2) ylayer(a.y2.s[j], a.y2.c[j], a.y2.d[j], a.Qn[3][3], qyi(i, j - (ny - n3), k, nx, n3), n3);
     // z layer with x and y cross terms (air.cpp:1730-1905)
     auto zlayer = [&](T sz, T cz, T dzm, gptr<const T> Qp, size_t qz) {
         du += ((-sz) * (uc + Aav * bX(Qp, qz, sxy))) * dt;
         dv += ((-sz) * (vc + Aav * bY(Qp, qz, nx, sxy))) * dt;
         dw += ((cz * Aav) * gpz) * dt - (sz * wc) * dt;
-        if (i < a.I1) {
+        if ((L & 1) && i < a.I1) {
             const T sx = a.x1.s[i], dxm = a.x1.d[i];
             const size_t q = qxi(i, j, k, n0, ny), s0 = (size_t)n0 * ny;
             gptr<const T> Xp = a.Qn[0][3];
             du += (-((a.Qo[0][0][q] * sx) * sz + ((dxm * sz) * Aav) * bX(Xp, q, s0))) * dt;
             dv += (((-a.Qo[0][1][q]) * sx) * sz) * dt;
             dw += (-((a.Qo[0][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - s0])) * dzi))) * dt;
-        } else if (i >= nx - a.I2) {
+        } else if ((L & 1) && i >= nx - a.I2) {
             const int ii = i - (nx - n1);
             const T sx = a.x2.s[i], dxm = a.x2.d[i];
             const size_t q = qxi(ii, j, k, n1, ny), s1 = (size_t)n1 * ny;
@@ -491,14 +535,14 @@ __device__ __forceinline__ void duvw_layers(const AirArgs<T> &a, int i, int j, i
             dv += (((-a.Qo[1][1][q]) * sx) * sz) * dt;
             dw += (-((a.Qo[1][2][q] * sx) * sz + ((dzm * sx) * Aav) * ((mZ * (Xp[q] - Xp[q - s1])) * dzi))) * dt;
         }
-        if (j < a.J1) {
+        if ((L & 2) && j < a.J1) {
             const T sy = a.y1.s[j], dym = a.y1.d[j];
             const size_t q = qyi(i, j, k, nx, n2), s2 = (size_t)nx * n2;
             gptr<const T> Yp = a.Qn[2][3];
             du += (((-a.Qo[2][0][q]) * sy) * sz) * dt;
             dv += (-((a.Qo[2][1][q] * sy) * sz + ((dym * sz) * Aav) * bY(Yp, q, nx, s2))) * dt;
             dw += (-((a.Qo[2][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - s2])) * dzi))) * dt;
-        } else if (j >= ny - a.J2) {
+        } else if ((L & 2) && j >= ny - a.J2) {
             const int jj = j - (ny - n3);
             const T sy = a.y2.s[j], dym = a.y2.d[j];
             const size_t q = qyi(i, jj, k, nx, n3), s3 = (size_t)nx * n3;
@@ -508,8 +552,8 @@ __device__ __forceinline__ void duvw_layers(const AirArgs<T> &a, int i, int j, i
             dw += (-((a.Qo[3][2][q] * sy) * sz + ((dzm * sy) * Aav) * ((mZ * (Yp[q] - Yp[q - s3])) * dzi))) * dt;
         }
     };
-    if (k < a.K1) zlayer(a.z1.s[k], a.z1.c[k], a.z1.d[k], a.Qn[4][3], qzi(i, j, k, nx, ny));
-    if (k >= nz - a.K2) zlayer(a.z2.s[k], a.z2.c[k], a.z2.d[k], a.Qn[5][3], qzi(i, j, k - (nz - n5), nx, ny));
+    if ((L & 4) && k < a.K1) zlayer(a.z1.s[k], a.z1.c[k], a.z1.d[k], a.Qn[4][3], qzi(i, j, k, nx, ny));
+    if ((L & 4) && k >= nz - a.K2) zlayer(a.z2.s[k], a.z2.c[k], a.z2.d[k], a.Qn[5][3], qzi(i, j, k - (nz - n5), nx, ny));
 }
 
 // ------------------------------------------------------------------------------------
@@ -522,7 +566,11 @@ __device__ __forceinline__ void duvw_layers(const AirArgs<T> &a, int i, int j, i
 // of the reference's one-sided pair (backward for p, forward for u v w).  The one-sided pair
 // advects the velocity components across the flow with a downwind difference, whose modes
 // grow at up to 2|V|/dy; the central form is neutral.
-template <typename T, int WM, bool TERRAIN, bool CEN = false>
+// L: the slab axes the point may lie in (bit 0 x, 1 y, 2 z; 7 all, 0 none): the launches over
+// the interior box and over the one-axis parts of the shell below know that the other
+// slabs hold no point of theirs, so those layer terms, which would add nothing, are not
+// compiled in.
+template <typename T, int WM, bool TERRAIN, bool CEN = false, int L = 7>
 __device__ __forceinline__ T air_dp(const AirArgs<T> &a, int i, int j, int k)
 {
     const Grid<T> &g = a.g;
@@ -580,7 +628,7 @@ __device__ __forceinline__ T air_dp(const AirArgs<T> &a, int i, int j, int k)
             r += (((-a.gam) * pc) * dt) * ((UX + VY) + WZ);
             r += (((-a.gam) * a.Pav) * dt) * div;
     }
-    dp_layers<T, TERRAIN>(a, i, j, k, mX, mY, mZ, zX, zY, pc, ddx, ddy, dwz, r);
+    if constexpr (L != 0) dp_layers<T, TERRAIN, L>(a, i, j, k, mX, mY, mZ, zX, zY, pc, ddx, ddy, dwz, r);
     return r;
 }
 
@@ -595,17 +643,24 @@ template <typename T> __device__ __forceinline__ int obstacle_id(const AirArgs<T
 // FB pressure update at one interior point: calculation_2D::Cal_pressure
 // (calculation_2D.cpp:598-608) and Update_PML_Qp (air.cpp:2388-2439).  Writes p^{n+1} and
 // the point's Q_p and returns p^{n+1}.
-template <typename T, bool WIND, bool TERRAIN = false>
+// L as for air_dp; below 7 the point is also outside the obstacle (the split launches run
+// only without one).
+template <typename T, bool WIND, bool TERRAIN = false, int L = 7>
 __device__ __forceinline__ T air_p_point(const AirArgs<T> &a, int i, int j, int k)
 {
     const Grid<T> &g = a.g;
     const size_t id = (size_t)i + (size_t)g.nx * ((size_t)j + (size_t)g.ny * k);
-    const OwnQ<T> own = own_slabs(a, i, j, k);
+    if constexpr (L == 0) {
+        const T pnew = g.pn[id] + air_dp<T, WIND ? 1 : 0, TERRAIN, false, 0>(a, i, j, k);
+        g.pnn[id] = pnew;
+        return pnew;
+    }
+    const OwnQ<T> own = own_slabs<T, L>(a, i, j, k);
     T qp_old[6];
 #pragma unroll
     for (int s = 0; s < 6; ++s) qp_old[s] = own.in[s] ? a.Qo[s][3][own.idx[s]] : T(0);
-    T r = air_dp<T, WIND ? 1 : 0, TERRAIN>(a, i, j, k);
-    if (const int mo = obstacle_id(a, i, j, k)) {
+    T r = air_dp<T, WIND ? 1 : 0, TERRAIN, false, L>(a, i, j, k);
+    if (const int mo = L == 7 ? obstacle_id(a, i, j, k) : 0) {
         // inside the obstacle the porous-medium continuity equation replaces the increment
         // (air.cpp:2294-2318)
         T mX, mY, mZ;
@@ -630,7 +685,7 @@ __device__ __forceinline__ T air_p_point(const AirArgs<T> &a, int i, int j, int 
 // and the pressure auxiliaries in a.Qn[.][3] (FB: p^{n+1}; RK4: the stage state), the
 // velocity auxiliaries in a.Qo.
 // ------------------------------------------------------------------------------------
-template <typename T, int WM, bool TERRAIN, bool CEN = false>
+template <typename T, int WM, bool TERRAIN, bool CEN = false, int L = 7>
 __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int k, T &du_, T &dv_, T &dw_)
 {
     const Grid<T> &g = a.g;
@@ -697,18 +752,26 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
             dw += (((-uc) * WX) * dt - (vc * WY) * dt) - (wc * WZ) * dt;
             dw += ((-Aav) * dt) * gpz;
     }
-    duvw_layers<T, TERRAIN>(a, i, j, k, mX, mY, mZ, zX, zY, uc, vc, wc, gpx, gpy, gpz, du, dv, dw);
+    if constexpr (L != 0) duvw_layers<T, TERRAIN, L>(a, i, j, k, mX, mY, mZ, zX, zY, uc, vc, wc, gpx, gpy, gpz, du, dv, dw);
     du_ = du; dv_ = dv; dw_ = dw;
 }
 
 // FB velocity update at one interior point with p at n+1: calculation_2D::Cal_velocity
 // (calculation_2D.cpp:564-574) and Update_PML_Qvw (air.cpp:2323-2386).
-template <typename T, bool WIND, bool TERRAIN = false>
+template <typename T, bool WIND, bool TERRAIN = false, int L = 7>
 __device__ __forceinline__ void air_v_point(const AirArgs<T> &a, int i, int j, int k)
 {
     const Grid<T> &g = a.g;
     const size_t id = (size_t)i + (size_t)g.nx * ((size_t)j + (size_t)g.ny * k);
-    const OwnQ<T> own = own_slabs(a, i, j, k);
+    if constexpr (L == 0) {
+        T du, dv, dw;
+        air_duvw<T, WIND ? 1 : 0, TERRAIN, false, 0>(a, i, j, k, du, dv, dw);
+        g.unn[id] = g.un[id] + du;
+        g.vnn[id] = g.vn[id] + dv;
+        g.wnn[id] = g.wn[id] + dw;
+        return;
+    }
+    const OwnQ<T> own = own_slabs<T, L>(a, i, j, k);
     T qu_old[6], qv_old[6], qw_old[6];
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
@@ -717,9 +780,9 @@ __device__ __forceinline__ void air_v_point(const AirArgs<T> &a, int i, int j, i
         qw_old[s] = own.in[s] ? a.Qo[s][2][own.idx[s]] : T(0);
     }
     T du, dv, dw;
-    air_duvw<T, WIND ? 1 : 0, TERRAIN>(a, i, j, k, du, dv, dw);
+    air_duvw<T, WIND ? 1 : 0, TERRAIN, false, L>(a, i, j, k, du, dv, dw);
     const T dt = g.dt;
-    if (const int mo = obstacle_id(a, i, j, k)) {
+    if (const int mo = L == 7 ? obstacle_id(a, i, j, k) : 0) {
         // inside the obstacle the porous-medium momentum equation replaces the increments
         // (air.cpp:1907-1945)
         T mX, mY, mZ;
@@ -754,6 +817,28 @@ __global__ __launch_bounds__(BX * BY) void k_air_p(const AirArgs<T> a)
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
     if (i < 1 || i >= a.g.nx - 1 || j >= a.g.ny - 1) return;
     (void)air_p_point<T, WIND, TERRAIN>(a, i, j, k);
+}
+
+// Split forms of the two passes: a launch per box list, with the layer terms of the slab axes
+// L its points can meet (0 for the interior box, one axis for the faces of the shell, 7 for
+// its edges); together they cover the points of k_air_p / k_air_v with the same arithmetic.
+// On the 512 x 256 x 128 grid with its 20-point layers (FP32) the interior launches take
+// 39 + 54 us for 53 % of the points, the layer launches the rest of the 143 + 180 us.
+template <typename T, bool WIND, bool TERRAIN, int L, int NB>
+__global__ __launch_bounds__(BX * BY) void k_air_p_box(const AirArgs<T> a, const BoxList<NB> l)
+{
+    int i, j, k;
+    list_point(l, blockIdx.x, i, j, k);
+    if (i < 1 || i >= a.g.nx - 1 || j >= a.g.ny - 1) return;
+    (void)air_p_point<T, WIND, TERRAIN, L>(a, i, j, k);
+}
+template <typename T, bool WIND, bool TERRAIN, int L, int NB>
+__global__ __launch_bounds__(BX * BY) void k_air_v_box(const AirArgs<T> a, const BoxList<NB> l)
+{
+    int i, j, k;
+    list_point(l, blockIdx.x, i, j, k);
+    if (i < 1 || i >= a.g.nx - 1 || j >= a.g.ny - 1) return;
+    air_v_point<T, WIND, TERRAIN, L>(a, i, j, k);
 }
 
 template <typename T, bool WIND, bool TERRAIN>
@@ -991,15 +1076,15 @@ __device__ __forceinline__ T rk_update_if(int st, T xn, T k, gptr<T> acc, gptr<T
     }
 }
 
-template <typename T, int WM, bool TERRAIN, bool COMP>
+template <typename T, int WM, bool TERRAIN, bool COMP, int L = 7>
 __device__ __forceinline__ void rk_air_point(const RKArgs<T> &r, int i, int j, int k)
 {
     const AirArgs<T> &a = r.a;
     const size_t id = (size_t)i + (size_t)a.g.nx * ((size_t)j + (size_t)a.g.ny * k);
     const int st = r.stage;
     T kf[4];
-    kf[3] = air_dp<T, WM, TERRAIN, true>(a, i, j, k);
-    air_duvw<T, WM, TERRAIN, true>(a, i, j, k, kf[0], kf[1], kf[2]);
+    kf[3] = air_dp<T, WM, TERRAIN, true, L>(a, i, j, k);
+    air_duvw<T, WM, TERRAIN, true, L>(a, i, j, k, kf[0], kf[1], kf[2]);
     const T xc[4] = {a.g.un[id], a.g.vn[id], a.g.wn[id], a.g.pn[id]};  // stage state at the point
     // every register read of the point first, then the arithmetic, then the stores
     T xn[4], ac[4], er[4];
@@ -1019,9 +1104,10 @@ __device__ __forceinline__ void rk_air_point(const RKArgs<T> &r, int i, int j, i
         if (st < 3) r.acc[f][id] = ao[f];
         if (COMP && st == 3) r.err[f][id] = eo[f];
     }
+    if constexpr (L == 0) return;
     // auxiliaries of the point's own slabs: dQ/dt = f (air.cpp:2323-2439 as an ODE), each
     // slab's reads before its stores
-    const OwnQ<T> own = own_slabs(a, i, j, k);
+    const OwnQ<T> own = own_slabs<T, L>(a, i, j, k);
     const T dt = a.g.dt;
 #pragma unroll
     for (int s = 0; s < 6; ++s)
@@ -1080,6 +1166,20 @@ __global__ __launch_bounds__(BX * TY) void k_rk_air_t(const RKArgs<T> *__restric
     const int i = bx * BX + threadIdx.x, j = by * TY + threadIdx.y + 1;
     if (i < 1 || i >= nx - 1 || j >= ny - 1) return;
     rk_air_point<T, WM, TERRAIN, COMP>(r, i, j, k);
+}
+
+// Split form of the RK4 air stage (64 x 4 tiles), as k_air_p_box.  The FP32 interior kernel
+// is held to 72 registers (7 waves per SIMD at least): left free it took 77 and ran 6; the
+// FP64 one, and the FP32 one with a uniform mean flow (WM 1), would spill there.
+template <typename T, int WM, bool TERRAIN, bool COMP, int L, int NB>
+__global__ __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu((COMP && L == 0 && WM != 1) ? 7 : 1, 8))) void k_rk_air_box(
+    const RKArgs<T> *__restrict__ rp, const BoxList<NB> l)
+{
+    const RKArgs<T> &r = *rp;
+    int i, j, k;
+    list_point(l, blockIdx.x, i, j, k);
+    if (i < 1 || i >= r.a.g.nx - 1 || j >= r.a.g.ny - 1) return;
+    rk_air_point<T, WM, TERRAIN, COMP, L>(r, i, j, k);
 }
 
 // Ensemble form: one launch steps every member.  The flat block index selects the member by
@@ -1373,6 +1473,82 @@ __global__ __launch_bounds__(256) void k_copy_words(v4f *dst, const v4f *src, si
 // face threads i = 0 and i = nx - 1 return at once); y and z start at the first interior plane.
 dim3 interior_grid(int nx, int ny, int nz) { return dim3((nx + BX - 1) / BX, (ny - 2 + BY - 1) / BY, nz - 2); }
 
+// The split of a domain's tile space by the slab axes its tiles can meet.  Along each axis the
+// tiles fall in three ranges: before, inside and after the range wholly outside that axis's
+// layers (the terms of dp_layers / duvw_layers and the auxiliaries of own_slabs).  A box of
+// three inside ranges is the interior (no layer terms), one with a single outside range meets
+// only that axis's slabs, the rest (edges and corners of the shell) may meet any.  False when
+// the interior is empty or an obstacle is embedded: the full launch then covers the domain.
+struct Regions {
+    BoxList<1> in;        // the interior
+    BoxList<2> ax[3];     // the shell's parts that meet only x, only y, only z slabs
+    BoxList<12> edge;     // the rest of the shell (edges and corners)
+    BoxList<6> shell;     // the whole shell: planes below and above the interior, then rows
+                          // before and after it, then columns left and right of it
+};
+template <int NB> void pack(BoxList<NB> &l, const std::vector<TileBox> &v)
+{
+    l.n = (int)v.size();
+    l.first[0] = 0;
+    for (int q = 0; q < NB; ++q) {
+        l.b[q] = q < l.n ? v[q] : TileBox{0, 1, 0, 1, 1, 1};
+        l.first[q + 1] = q < l.n ? l.first[q] + v[q].nbx * v[q].nby * v[q].nk : INT_MAX;
+    }
+}
+template <typename T> bool split_regions(const AirArgs<T> &a, Regions &R)
+{
+    const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+    if (a.omask) return false;
+    int xa = std::max(1, a.I1), xb = std::min(nx - 1, nx - a.I2);
+    int ya = std::max(1, a.J1), yb = std::min(ny - 1, ny - a.J2);
+    int za = std::max(1, a.K1), zb = std::min(nz - 1, nz - a.K2);
+    if (a.qn[0]) xa = std::max(xa, a.qlo[0] + a.qn[0]);
+    if (a.qn[1]) xb = std::min(xb, a.qlo[1]);
+    if (a.qn[2]) ya = std::max(ya, a.qlo[2] + a.qn[2]);
+    if (a.qn[3]) yb = std::min(yb, a.qlo[3]);
+    if (a.qn[4]) za = std::max(za, a.qlo[4] + a.qn[4]);
+    if (a.qn[5]) zb = std::min(zb, a.qlo[5]);
+    const int bx0 = (xa + BX - 1) / BX, bx1 = xb / BX;            // tiles [64 bx, 64 bx + 64) in [xa, xb)
+    const int by0 = (ya - 1 + BY - 1) / BY, by1 = (yb - 1) / BY;  // rows [BY by + 1, BY by + BY + 1) in [ya, yb)
+    if (bx1 <= bx0 || by1 <= by0 || zb <= za) return false;
+    const int TX = (nx + BX - 1) / BX, TYt = (ny - 2 + BY - 1) / BY;
+    // ranges (start, count) before / inside / after, per axis
+    const int X[3][2] = {{0, bx0}, {bx0, bx1 - bx0}, {bx1, TX - bx1}};
+    const int Y[3][2] = {{0, by0}, {by0, by1 - by0}, {by1, TYt - by1}};
+    const int Z[3][2] = {{1, za - 1}, {za, zb - za}, {zb, nz - 1 - zb}};
+    std::vector<TileBox> in, ax[3], edge, shell;
+    auto add = [](std::vector<TileBox> &l, int x0, int nbx, int y0, int nby, int z0, int nk) {
+        if (nbx > 0 && nby > 0 && nk > 0) l.push_back(TileBox{x0, nbx, y0, nby, z0, nk});
+    };
+    for (int cz = 0; cz < 3; ++cz)
+        for (int cy = 0; cy < 3; ++cy) {
+            const int outs = (cy != 1) + (cz != 1);
+            if (outs == 2) {  // an edge for every x range: one box across x
+                add(edge, 0, TX, Y[cy][0], Y[cy][1], Z[cz][0], Z[cz][1]);
+                continue;
+            }
+            for (int cx = 0; cx < 3; ++cx) {
+                const int o = outs + (cx != 1);
+                std::vector<TileBox> &l = o == 0 ? in : o == 2 ? edge : cx != 1 ? ax[0] : cy != 1 ? ax[1] : ax[2];
+                add(l, X[cx][0], X[cx][1], Y[cy][0], Y[cy][1], Z[cz][0], Z[cz][1]);
+            }
+        }
+    const int nk = zb - za;
+    add(shell, 0, TX, 0, TYt, 1, za - 1);
+    add(shell, 0, TX, 0, TYt, zb, nz - 1 - zb);
+    add(shell, 0, TX, 0, by0, za, nk);
+    add(shell, 0, TX, by1, TYt - by1, za, nk);
+    add(shell, 0, bx0, by0, by1 - by0, za, nk);
+    add(shell, bx1, TX - bx1, by0, by1 - by0, za, nk);
+    pack(R.in, in);
+    for (int q = 0; q < 3; ++q) pack(R.ax[q], ax[q]);
+    pack(R.edge, edge);
+    pack(R.shell, shell);
+    return true;
+}
+template <int NB> unsigned list_blocks(const BoxList<NB> &l) { return (unsigned)l.first[l.n]; }
+
+
 unsigned flat_blocks(size_t total)
 {
     size_t b = (total + 255) / 256;
@@ -1382,11 +1558,25 @@ unsigned flat_blocks(size_t total)
 
 }  // namespace
 
-
 template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s)
 {
     dim3 g = interior_grid(a.g.nx, a.g.ny, a.g.nz), b(BX, BY);
     const bool ter = a.g.tsh != nullptr;
+    Regions R;
+    if (split_regions(a, R)) {
+#define LEE_PS(W, TER)                                                                                       \
+    do {                                                                                                     \
+        if (R.in.n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 0, 1>), dim3(list_blocks(R.in)), b, 0, s, a, R.in);    \
+        if (R.ax[0].n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 1, 2>), dim3(list_blocks(R.ax[0])), b, 0, s, a, R.ax[0]); \
+        if (R.ax[1].n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 2, 2>), dim3(list_blocks(R.ax[1])), b, 0, s, a, R.ax[1]); \
+        if (R.ax[2].n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 4, 2>), dim3(list_blocks(R.ax[2])), b, 0, s, a, R.ax[2]); \
+        if (R.edge.n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 7, 12>), dim3(list_blocks(R.edge)), b, 0, s, a, R.edge); \
+    } while (0)
+        if (ter) { if (wind) LEE_PS(true, true); else LEE_PS(false, true); }
+        else { if (wind) LEE_PS(true, false); else LEE_PS(false, false); }
+#undef LEE_PS
+        return;
+    }
     if (ter) {
         if (wind) hipLaunchKernelGGL((k_air_p<T, true, true>), g, b, 0, s, a);
         else hipLaunchKernelGGL((k_air_p<T, false, true>), g, b, 0, s, a);
@@ -1399,6 +1589,21 @@ template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStrea
 {
     dim3 g = interior_grid(a.g.nx, a.g.ny, a.g.nz), b(BX, BY);
     const bool ter = a.g.tsh != nullptr;
+    Regions R;
+    if (split_regions(a, R)) {
+#define LEE_VS(W, TER)                                                                                       \
+    do {                                                                                                     \
+        if (R.in.n) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 0, 1>), dim3(list_blocks(R.in)), b, 0, s, a, R.in);    \
+        if (R.ax[0].n) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 1, 2>), dim3(list_blocks(R.ax[0])), b, 0, s, a, R.ax[0]); \
+        if (R.ax[1].n) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 2, 2>), dim3(list_blocks(R.ax[1])), b, 0, s, a, R.ax[1]); \
+        if (R.ax[2].n) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 4, 2>), dim3(list_blocks(R.ax[2])), b, 0, s, a, R.ax[2]); \
+        if (R.edge.n) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 7, 12>), dim3(list_blocks(R.edge)), b, 0, s, a, R.edge); \
+    } while (0)
+        if (ter) { if (wind) LEE_VS(true, true); else LEE_VS(false, true); }
+        else { if (wind) LEE_VS(true, false); else LEE_VS(false, false); }
+#undef LEE_VS
+        return;
+    }
     if (ter) {
         if (wind) hipLaunchKernelGGL((k_air_v<T, true, true>), g, b, 0, s, a);
         else hipLaunchKernelGGL((k_air_v<T, false, true>), g, b, 0, s, a);
@@ -1422,12 +1627,37 @@ void launch_rk_air(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool terra
 }
 
 // Variant v of the tuned stage launch (lee_ctx keeps the autotuner's choice): v = 0 the
-// plane-by-plane 64x4 grid, 1..4 the flat grids of kRkTiles.
+// plane-by-plane 64x4 grid, 1..4 the flat grids of kRkTiles, 5 and 6 the split launches of
+// split_regions: the interior and the shell, or the interior and the shell by slab axes (v = 0
+// when the interior is empty).
 template <typename T>
 void launch_rk_air_v(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool terrain, int v, hipStream_t s)
 {
-    if (v <= 0) { launch_rk_air<T>(r, rd, wind, terrain, s); return; }
+    Regions R;
+    if (v <= 0 || (v >= 5 && !split_regions(r.a, R))) { launch_rk_air<T>(r, rd, wind, terrain, s); return; }
     constexpr bool C = std::is_same<T, float>::value;
+    if (v >= 5) {
+        const dim3 b(BX, BY);
+#define LEE_RKS(WM, TER)                                                                                          \
+    do {                                                                                                        \
+        if (R.in.n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 0, 1>), dim3(list_blocks(R.in)), b, 0, s, rd, R.in); \
+        if (v == 5) {                                                                                           \
+            if (R.shell.n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 6>), dim3(list_blocks(R.shell)), b, 0, s, rd, R.shell); \
+            break;                                                                                              \
+        }                                                                                                       \
+        if (R.ax[0].n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 1, 2>), dim3(list_blocks(R.ax[0])), b, 0, s, rd, R.ax[0]); \
+        if (R.ax[1].n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 2, 2>), dim3(list_blocks(R.ax[1])), b, 0, s, rd, R.ax[1]); \
+        if (R.ax[2].n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 4, 2>), dim3(list_blocks(R.ax[2])), b, 0, s, rd, R.ax[2]); \
+        if (R.edge.n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 12>), dim3(list_blocks(R.edge)), b, 0, s, rd, R.edge); \
+    } while (0)
+        if (terrain) {
+            if (wind == 1) LEE_RKS(1, true); else if (wind == 2) LEE_RKS(2, true); else LEE_RKS(0, true);
+        } else {
+            if (wind == 1) LEE_RKS(1, false); else if (wind == 2) LEE_RKS(2, false); else LEE_RKS(0, false);
+        }
+#undef LEE_RKS
+        return;
+    }
     const int nx = r.a.g.nx, ny = r.a.g.ny, K = r.a.g.nz - 2;
     const int TYv = kRkTiles[v].ty;
     const unsigned nb = (unsigned)(((nx + BX - 1) / BX) * ((ny - 2 + TYv - 1) / TYv) * K);

@@ -51,8 +51,9 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-# launch shapes of the RK4 air stage (lee_options.rk_tile 1..5), as the autotuner reports them
-RK_TILES = ["64x4 planes", "64x4 xcd-columns", "64x8 xcd-columns", "64x8 planes", "64x16 xcd-columns"]
+# launch shapes of the RK4 air stage (lee_options.rk_tile 1..7), as the autotuner reports them
+RK_TILES = ["64x4 planes", "64x4 xcd-columns", "64x8 xcd-columns", "64x8 planes", "64x16 xcd-columns",
+            "64x4 interior + layer shell", "64x4 interior + layers by axis"]
 
 
 def run_gpu(lee, deck, receivers, precision, steps, warmup, record=True):

@@ -110,14 +110,17 @@ typedef struct lee_options {
      * Z(s) = ground_zinf (1 + sum_k ground_a[k] / (ground_r[k] + s ground_tau)) through the
      * East TDBC's recursion (calculation_2D.cpp:986-1028) with K poles: the normal velocity
      * half a cell below the plane follows the porous momentum equation with the impedance's
-     * pressure half a cell further down.  FB_p_v, deck South face rigid, fixed frame, flat
-     * grid, no obstacle; residues and poles >= 0 (passive). */
+     * pressure half a cell further down.  On the terrain-following grid (n_hills > 0) the
+     * plane takes the metric of a porous top plane shifted rigidly with the ground.  FB_p_v,
+     * deck South face rigid, fixed frame, no obstacle; residues and poles >= 0 (passive). */
     int ground_poles;       /* 0: off, else 1..LEE_MAX_POLES pole/residue pairs */
     double ground_a[8], ground_r[8];
     double ground_zinf, ground_tau;
     /* RK4 air-stage launch shape: 0 = autotune on the first steps (times the candidates on the
-     * context's own buffers), 1..5 = candidate 0..4 (64x4 plane order, 64x4 / 64x8 XCD columns,
-     * 64x8 plane order, 64x16 XCD columns).  All give the same bits. */
+     * context's own buffers), 1..7 = candidate 0..6 (64x4 plane order, 64x4 / 64x8 XCD columns,
+     * 64x8 plane order, 64x16 XCD columns, 64x4 tiles split into the interior outside every
+     * layer and the layer shell, the same with the shell split by slab axes).  All give the
+     * same bits. */
     int rk_tile;
 } lee_options;
 
@@ -147,7 +150,7 @@ typedef struct lee_info {
     long long steps_done;           /* time steps advanced so far */
     long long pml_y1_counter;       /* reference PML_JMAX1 step counter */
     int path;                       /* LEE_PATH_SPLIT or LEE_PATH_FUSED, as chosen */
-    int rk_tile;                    /* RK4 air-stage launch shape in use (1..5), 0 before tuning */
+    int rk_tile;                    /* RK4 air-stage launch shape in use (1..7), 0 before tuning */
 } lee_info;
 
 typedef struct lee_ctx lee_ctx;

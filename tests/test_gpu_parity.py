@@ -41,14 +41,27 @@ def cases(lee):
     # pulse crosses the tile boundary into the x2 layer (i >= 259)
     out["wide"] = B(264, 30, 12, 0.05, pml=5, bc=(2, 3, 2, 2, 2, 2), porous_cells=5, x0=-12.8,
                     src=(0.0, 0.6, 0.3), gauss_width=0.25, time_domain=30)
+    # 200 x 30 x 16 with layers of 5 points: two 64 x 4-point tile columns and three tile rows
+    # lie outside every layer, so the split passes run their layer-free interior launch next
+    # to the shell launch; over terrain, and with a mean flow
+    t = B(200, 30, 16, 0.05, pml=5, bc=(2, 3, 2, 2, 2, 2), porous_cells=6, x0=-5.0, src=(0.0, 0.6, 0.3),
+          gauss_width=0.25, time_domain=30)
+    t.hills = [(0.3, 0.7, 0.08, 0.3), (-1.0, 0.9, 0.05, 0.4)]
+    out["wide_terrain"] = t
+    out["wide_wind"] = B(200, 30, 16, 0.05, pml=5, bc=(2, 1, 2, 2, 2, 2), x0=-5.0, src=(0.0, 0.6, 0.4),
+                         gauss_width=0.25, time_domain=30, velocity_method=5, velocity_coef=3.0)
     return out
 
 
 PATHS = {"split": 1, "fused": 2}
 
 
-@pytest.mark.parametrize("path", ["split", "fused"])
-@pytest.mark.parametrize("name", ["rigid", "pml5", "tdbc", "porous", "wind5", "wind2", "long", "wide"])
+# the fused pass runs separable grids only (a terrain deck is refused on it)
+FB_CASES = [(n, p) for n in ["rigid", "pml5", "tdbc", "porous", "wind5", "wind2", "long", "wide", "wide_terrain",
+                             "wide_wind"] for p in ["split", "fused"] if not (n == "wide_terrain" and p == "fused")]
+
+
+@pytest.mark.parametrize("name,path", FB_CASES)
 def test_fb_fp64_parity(lee, orc, name, path):
     d = cases(lee)[name]
     print(f"[gpu parity] case={name} path={path} shape={d.shape}", flush=True)

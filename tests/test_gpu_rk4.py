@@ -37,16 +37,23 @@ def cases(lee):
     # 70 points across x: two x tiles of 64 points, the second one partial, over a porous ground
     out["wide"] = B(70, 24, 16, 0.05, pml=5, bc=(2, 3, 2, 2, 2, 2), porous_cells=6, src=(0.4, 0.6, 0.3),
                     gauss_width=0.25, time_domain=30)
+    # 200 points across x with layers of 5: tiles outside every layer (the interior launch of
+    # launch shape 6) and the shell around them, over terrain with the log wind
+    t = B(200, 32, 22, 0.05, pml=5, bc=(2, 3, 2, 2, 2, 2), porous_cells=8, x0=-5.0, src=(0.0, 0.6, 0.3),
+          gauss_width=0.25, time_domain=30, velocity_method=7)
+    t.hills, t.wind_ustar, t.wind_z0 = [(0.1, 0.7, 0.2, 0.3), (-2.0, 1.1, 0.12, 0.25)], 0.4, 0.02
+    out["split"] = t
     for d in out.values():
         d.integrator = 1
     return out
 
 
-@pytest.mark.parametrize("name", ["pml", "porous", "wind5", "terrain", "overlap", "wide"])
+@pytest.mark.parametrize("name", ["pml", "porous", "wind5", "terrain", "overlap", "wide", "split"])
 def test_rk4_fp64_parity(lee, orc, name):
     d = cases(lee)[name]
     print(f"[gpu rk4] case={name} shape={d.shape}", flush=True)
-    s = lee.Solver(d, precision=64, receivers=REC)
+    # the split case runs launch shape 6 (interior box + layer shell), the others the tuner's
+    s = lee.Solver(d, precision=64, receivers=REC, rk_tile=6 if name == "split" else 0)
     s.frame_setup(0)
     got = s.step(d.time_domain)
     o = orc.Oracle(d, receivers=REC)
@@ -140,10 +147,10 @@ def test_fb_two_x_tiles_match_oracle(lee, orc):
 def test_rk4_launch_shapes_give_the_same_bits(lee, prec):
     """Every launch shape of the RK4 air stage (the autotuner's candidates) computes the same
     per-point arithmetic: identical receivers and fields."""
-    d = cases(lee)["terrain"]
+    d = cases(lee)["split"]
     print(f"[gpu] rk4 launch shapes fp{prec}", flush=True)
     ref = None
-    for v in range(1, 6):
+    for v in range(1, 8):
         s = lee.Solver(d, precision=prec, receivers=REC, integrator=1, rk_tile=v)
         s.frame_setup(0)
         got = (s.step(d.time_domain), s.state())
@@ -158,5 +165,5 @@ def test_rk4_launch_shapes_give_the_same_bits(lee, prec):
     s = lee.Solver(d, precision=prec, receivers=REC, integrator=1)
     s.frame_setup(0)
     s.step(2)
-    assert 1 <= s.info().rk_tile <= 5
+    assert 1 <= s.info().rk_tile <= 7
     s.close()
