@@ -47,13 +47,21 @@ template <int NB> struct BoxList {
     TileBox b[NB];
     int first[NB + 1];  // first block of each box; first[n] = blocks in all, then INT_MAX
     int n;
+    int pad;  // each plane of a box takes a multiple of 8 blocks (list_point)
 };
 
-// Point of block t of a list: thread (threadIdx.x, threadIdx.y) of the box's tiles taken x
+// Point of block t of a list: thread (threadIdx.x, threadIdx.y) of the box's tiles, taken x
 // fastest, then y, then planes.  blockIdx is uniform, so this is scalar arithmetic on the
 // argument, read at constant offsets only: the box's fields as sums of products with 0/1
 // (selects between loaded values were merged into one load through a selected address,
-// which put the argument in private memory).
+// which put the argument in private memory).  With pad each plane of a box takes a multiple
+// of 8 blocks (the extra ones get i = 0 and leave at the bounds check) and the boxes start at
+// multiples of 8: blocks are dealt to the 8 XCDs round-robin, so a tile column then stays on
+// one XCD through its planes, and the planes k - 1, k + 1 a tile reads come from that XCD's
+// L2 (on the 318-tile planes of the 512 x 256 x 128 interior, plane k + 1 of a column went to
+// another XCD and the RK4 stage fetched twice its compulsory reads).  It takes a third off
+// the RK4 stage's HBM reads at the same speed; the FB passes ran 2 % slower with it and
+// stay unpadded.
 template <int NB, int... Q>
 __device__ __forceinline__ void list_point_(const BoxList<NB> &l, int t, int &i, int &j, int &k,
                                             std::integer_sequence<int, Q...>)
@@ -63,9 +71,13 @@ __device__ __forceinline__ void list_point_(const BoxList<NB> &l, int t, int &i,
     const int by = (0 + ... + ((q == Q) * l.b[Q].by)), nby = (0 + ... + ((q == Q) * l.b[Q].nby));
     const int k0 = (0 + ... + ((q == Q) * l.b[Q].k)), f = (0 + ... + ((q == Q) * l.first[Q]));
     t -= f;
-    const int per = nbx * nby, kk = t / per, r = t - kk * per, ry = r / nbx;
+    const int per = nbx * nby, pp = l.pad ? (per + 7) & ~7 : per, kk = t / pp, col = t - kk * pp, ry = col / nbx;
+    if (col >= per) {  // padding: leaves at the caller's bounds check
+        i = 0; j = 1; k = k0;
+        return;
+    }
     k = k0 + kk;
-    i = (bx + (r - ry * nbx)) * 64 + threadIdx.x;
+    i = (bx + (col - ry * nbx)) * 64 + threadIdx.x;
     j = (by + ry) * BY + threadIdx.y + 1;
 }
 template <int NB> __device__ __forceinline__ void list_point(const BoxList<NB> &l, int t, int &i, int &j, int &k)
@@ -1182,6 +1194,127 @@ __global__ __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu((COMP &
     rk_air_point<T, WM, TERRAIN, COMP, L>(r, i, j, k);
 }
 
+// Interior RK4 air stage with the stage state staged through LDS.  Each block copies the
+// fields its 64 x 4 tile reads -- u, v, w, p (and the mean flow V) on plane k with a halo of a
+// 16-byte chunk in x and a row in y, and on planes k - 1, k + 1 the tile's own points -- into
+// LDS with global_load_lds (16 bytes per lane, no registers held while the loads are in
+// flight, so a block has its whole stencil input in flight at once), then every point takes
+// its neighbours from LDS and runs the same dp_core / duvw_core and register update as
+// rk_air_point: the same bits.  Mean flows of velocity_method 0 and 7; rows a multiple of 16
+// bytes (nx % (16 / sizeof(T)) == 0), so every chunk is aligned.
+template <typename T> struct Stage {
+    static constexpr int H = 16 / (int)sizeof(T);   // elements per chunk, the x halo
+    static constexpr int W = BX + 2 * H;            // a plane-k row: i0 - H .. i0 + 63 + H
+    static constexpr int PK = (BY + 2) * W;         // plane k: rows j0 - 1 .. j0 + BY
+    static constexpr int PS = BY * BX;              // planes k -+ 1: the tile's own points
+    static constexpr int SZ = PK + 2 * PS;          // one field
+    static constexpr int CK = PK / H, CS = PS / H;  // 16-byte chunks
+    static constexpr int GK = (CK + 63) / 64, GS = CS / 64;  // wave loads per part
+    static constexpr int G = GK + 2 * GS;           // wave loads per field
+    static_assert(CS % 64 == 0, "side planes are whole wave loads");
+};
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <typename T, int NF>
+__device__ __forceinline__ void stage_fields(gptr<const T> const (&F)[5], T *lds, size_t nx, size_t sxy, int i0, int j0,
+                                             int k)
+{
+    using S = Stage<T>;
+    const int lane = threadIdx.x;
+    // wave w (= threadIdx.y) issues the loads g = f * G + part with g % BY == w; the loops run
+    // at compile time so that every field pointer is read at a constant index
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int part = 0; part < S::G; ++part) {
+            if ((f * S::G + part) % BY != (int)threadIdx.y) continue;
+            int c, limit, off;
+            size_t e;
+            if (part < S::GK) {  // plane k, row by row of W elements
+                c = part * 64 + lane;
+                const int r = c / (S::W / S::H), cc = c - r * (S::W / S::H);
+                e = (size_t)(i0 - S::H + cc * S::H) + nx * (size_t)(j0 - 1 + r) + sxy * (size_t)k;
+                limit = S::CK;
+                off = f * S::SZ + part * 64 * S::H;
+            } else {  // plane k - 1 (side 0) or k + 1 (side 1), the tile's rows
+                const int q = part - S::GK, side = q / S::GS, gq = q - side * S::GS;
+                c = gq * 64 + lane;
+                const int r = c / (BX / S::H), cc = c - r * (BX / S::H);
+                e = (size_t)(i0 + cc * S::H) + nx * (size_t)(j0 + r) + sxy * (size_t)(side ? k + 1 : k - 1);
+                limit = S::CS;
+                off = f * S::SZ + S::PK + side * S::PS + gq * 64 * S::H;
+            }
+            if (c < limit)
+                __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) const void *)(F[f] + e),
+                                                 (lds_void *)(lds + off), 16, 0, 0);
+        }
+}
+
+template <typename T, int WM, bool TERRAIN, bool COMP>
+__global__ __launch_bounds__(BX * BY) void k_rk_air_lds(const RKArgs<T> *__restrict__ rp, const BoxList<1> l)
+{
+    static_assert(WM == 0 || WM == 2, "uniform mean flows take the register path");
+    using S = Stage<T>;
+    constexpr int NF = WM == 2 ? 5 : 4;
+    __shared__ T lds[NF * S::SZ];
+    const RKArgs<T> &r = *rp;
+    const AirArgs<T> &a = r.a;
+    const Grid<T> &g = a.g;
+    int i, j, k;
+    list_point(l, blockIdx.x, i, j, k);
+    if (i < 1) return;  // a padding block (uniform over the block: no barrier is left waiting)
+    const int ti = threadIdx.x, tj = threadIdx.y;
+    const size_t nx = (size_t)g.nx, sxy = (size_t)g.nx * g.ny;
+    const size_t id = (size_t)i + nx * ((size_t)j + (size_t)g.ny * k);
+    {
+        gptr<const T> const F[5] = {g.un, g.vn, g.wn, g.pn, a.V};
+        stage_fields<T, NF>(F, lds, nx, sxy, i - ti, j - tj, k);
+    }
+    // the point's registers while the stage state is in flight
+    const int st = r.stage;
+    T xn[4], ac[4], er[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        xn[f] = (st == 0 || (!COMP && st == 3)) ? T(0) : r.xn[f][id];
+        ac[f] = st == 0 ? T(0) : r.acc[f][id];
+        er[f] = (COMP && st == 3) ? r.err[f][id] : T(0);
+    }
+    Nb<T> n;
+    n.zX = T(0); n.zY = T(0);
+    if constexpr (TERRAIN) metric_terrain(g, i, j, k, n.mX, n.mY, n.mZ, n.zX, n.zY);
+    else metric(g, i, j, k, n.mX, n.mY, n.mZ);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    auto K = [&](int f, int di, int dj) { return lds[f * S::SZ + (tj + 1 + dj) * S::W + (ti + S::H + di)]; };
+    auto M = [&](int f) { return lds[f * S::SZ + S::PK + tj * BX + ti]; };
+    auto P = [&](int f) { return lds[f * S::SZ + S::PK + S::PS + tj * BX + ti]; };
+    n.uc = K(0, 0, 0); n.u_ip = K(0, 1, 0); n.u_jp = K(0, 0, 1); n.u_jm = K(0, 0, -1); n.u_kp = P(0); n.u_km = M(0);
+    n.vc = K(1, 0, 0); n.v_jp = K(1, 0, 1); n.v_jm = K(1, 0, -1); n.v_kp = P(1); n.v_km = M(1);
+    n.wc = K(2, 0, 0); n.w_jp = K(2, 0, 1); n.w_jm = K(2, 0, -1); n.w_kp = P(2); n.w_km = M(2);
+    n.pc = K(3, 0, 0); n.p_im = K(3, -1, 0); n.p_jp = K(3, 0, 1); n.p_jm = K(3, 0, -1); n.p_kp = P(3); n.p_km = M(3);
+    if constexpr (WM == 2) {
+        n.Vc = K(4, 0, 0); n.V_ip = K(4, 1, 0); n.V_im = K(4, -1, 0); n.V_jp = K(4, 0, 1); n.V_jm = K(4, 0, -1);
+        n.V_kp = P(4); n.V_km = M(4);
+    }
+    T kf[4], ddx, ddy, dwz, gpx, gpy, gpz;
+    kf[3] = dp_core<T, WM, TERRAIN, true>(a, n, ddx, ddy, dwz);
+    duvw_core<T, WM, TERRAIN, true>(a, n, gpx, gpy, gpz, kf[0], kf[1], kf[2]);
+    const T xc[4] = {n.uc, n.vc, n.wc, n.pc};
+    if (st == 0) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) xn[f] = xc[f];
+    }
+    T d[4], ao[4], eo[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) d[f] = rk_regs<T, COMP>(st, xn[f], kf[f], ac[f], er[f], ao[f], eo[f]);
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        r.dst[f][id] = d[f];
+        if (st < 3) r.acc[f][id] = ao[f];
+        if (COMP && st == 3) r.err[f][id] = eo[f];
+    }
+}
+
 // Ensemble form: one launch steps every member.  The flat block index selects the member by
 // the block prefix `first` (first[d] .. first[d+1]-1 are member d's BX x BY tiles, x fastest,
 // then y, then interior planes) and reads that member's arguments from device memory.
@@ -1486,16 +1619,18 @@ struct Regions {
     BoxList<6> shell;     // the whole shell: planes below and above the interior, then rows
                           // before and after it, then columns left and right of it
 };
-template <int NB> void pack(BoxList<NB> &l, const std::vector<TileBox> &v)
+template <int NB> void pack(BoxList<NB> &l, const std::vector<TileBox> &v, bool pad)
 {
     l.n = (int)v.size();
+    l.pad = pad ? 1 : 0;
     l.first[0] = 0;
     for (int q = 0; q < NB; ++q) {
+        const int per = q < l.n ? v[q].nbx * v[q].nby : 0;
         l.b[q] = q < l.n ? v[q] : TileBox{0, 1, 0, 1, 1, 1};
-        l.first[q + 1] = q < l.n ? l.first[q] + v[q].nbx * v[q].nby * v[q].nk : INT_MAX;
+        l.first[q + 1] = q < l.n ? l.first[q] + (pad ? (per + 7) & ~7 : per) * v[q].nk : INT_MAX;
     }
 }
-template <typename T> bool split_regions(const AirArgs<T> &a, Regions &R)
+template <typename T> bool split_regions(const AirArgs<T> &a, Regions &R, bool pad)
 {
     const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
     if (a.omask) return false;
@@ -1540,10 +1675,10 @@ template <typename T> bool split_regions(const AirArgs<T> &a, Regions &R)
     add(shell, 0, TX, by1, TYt - by1, za, nk);
     add(shell, 0, bx0, by0, by1 - by0, za, nk);
     add(shell, bx1, TX - bx1, by0, by1 - by0, za, nk);
-    pack(R.in, in);
-    for (int q = 0; q < 3; ++q) pack(R.ax[q], ax[q]);
-    pack(R.edge, edge);
-    pack(R.shell, shell);
+    pack(R.in, in, pad);
+    for (int q = 0; q < 3; ++q) pack(R.ax[q], ax[q], pad);
+    pack(R.edge, edge, pad);
+    pack(R.shell, shell, pad);
     return true;
 }
 template <int NB> unsigned list_blocks(const BoxList<NB> &l) { return (unsigned)l.first[l.n]; }
@@ -1563,7 +1698,7 @@ template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStrea
     dim3 g = interior_grid(a.g.nx, a.g.ny, a.g.nz), b(BX, BY);
     const bool ter = a.g.tsh != nullptr;
     Regions R;
-    if (split_regions(a, R)) {
+    if (split_regions(a, R, false)) {
 #define LEE_PS(W, TER)                                                                                       \
     do {                                                                                                     \
         if (R.in.n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 0, 1>), dim3(list_blocks(R.in)), b, 0, s, a, R.in);    \
@@ -1590,7 +1725,7 @@ template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStrea
     dim3 g = interior_grid(a.g.nx, a.g.ny, a.g.nz), b(BX, BY);
     const bool ter = a.g.tsh != nullptr;
     Regions R;
-    if (split_regions(a, R)) {
+    if (split_regions(a, R, false)) {
 #define LEE_VS(W, TER)                                                                                       \
     do {                                                                                                     \
         if (R.in.n) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 0, 1>), dim3(list_blocks(R.in)), b, 0, s, a, R.in);    \
@@ -1628,14 +1763,29 @@ void launch_rk_air(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool terra
 
 // Variant v of the tuned stage launch (lee_ctx keeps the autotuner's choice): v = 0 the
 // plane-by-plane 64x4 grid, 1..4 the flat grids of kRkTiles, 5 and 6 the split launches of
-// split_regions: the interior and the shell, or the interior and the shell by slab axes (v = 0
-// when the interior is empty).
+// split_regions: the interior and the shell, or the interior and the shell by slab axes; 7 as
+// 5 with the interior staged through LDS (5 for a uniform mean flow or rows not a multiple of
+// 16 bytes).  v = 0 when the interior is empty.
 template <typename T>
 void launch_rk_air_v(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool terrain, int v, hipStream_t s)
 {
     Regions R;
-    if (v <= 0 || (v >= 5 && !split_regions(r.a, R))) { launch_rk_air<T>(r, rd, wind, terrain, s); return; }
+    if (v <= 0 || (v >= 5 && !split_regions(r.a, R, true))) { launch_rk_air<T>(r, rd, wind, terrain, s); return; }
     constexpr bool C = std::is_same<T, float>::value;
+    if (v == 7 && wind != 1 && r.a.g.nx % Stage<T>::H == 0) {
+        // the interior staged through LDS, the shell as v 5
+        const dim3 b(BX, BY);
+#define LEE_RKL(WM, TER)                                                                                          \
+    do {                                                                                                        \
+        if (R.in.n) hipLaunchKernelGGL((k_rk_air_lds<T, WM, TER, C>), dim3(list_blocks(R.in)), b, 0, s, rd, R.in); \
+        if (R.shell.n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 6>), dim3(list_blocks(R.shell)), b, 0, s, rd, R.shell); \
+    } while (0)
+        if (terrain) { if (wind == 2) LEE_RKL(2, true); else LEE_RKL(0, true); }
+        else { if (wind == 2) LEE_RKL(2, false); else LEE_RKL(0, false); }
+#undef LEE_RKL
+        return;
+    }
+    if (v == 7) v = 5;
     if (v >= 5) {
         const dim3 b(BX, BY);
 #define LEE_RKS(WM, TER)                                                                                          \

@@ -51,15 +51,18 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-# launch shapes of the RK4 air stage (lee_options.rk_tile 1..7), as the autotuner reports them
+# launch shapes of the RK4 air stage (lee_options.rk_tile 1..8), as the autotuner reports them
 RK_TILES = ["64x4 planes", "64x4 xcd-columns", "64x8 xcd-columns", "64x8 planes", "64x16 xcd-columns",
-            "64x4 interior + layer shell", "64x4 interior + layers by axis"]
+            "64x4 interior + layer shell", "64x4 interior + layers by axis", "64x4 LDS-staged interior + layer shell"]
+
+
+RK_TILE = 0  # --rk-tile: RK4 air-stage launch shape, 0 = the autotuner's choice
 
 
 def run_gpu(lee, deck, receivers, precision, steps, warmup, record=True):
     """Run warmup + steps steps of `deck`, frames moving as the reference schedules them.
     Returns timing, kernel statistics and the receiver series of the whole run."""
-    s = lee.Solver(deck, precision=precision, receivers=receivers)
+    s = lee.Solver(deck, precision=precision, receivers=receivers, rk_tile=RK_TILE)
     npts = s.nx * s.ny * s.nz + s.pnx * s.pny * s.pnz
     state = {"mf": 0, "temp": 0, "left": 0}
 
@@ -376,9 +379,14 @@ def main():
     ap.add_argument("--accuracy-steps", type=int, default=None,
                     help="C1/C2: steps of the untimed FP32-vs-FP64 accuracy pass when the timed run is shorter "
                          "(default: the whole workload, 10000 / 2000; 0 skips it)")
+    ap.add_argument("--rk-tile", type=int, default=0,
+                    help="RK4 air-stage launch shape 1..8 (lee_options.rk_tile; default 0: autotuned), for "
+                         "profiling runs that should not time the tuner's trials")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "rk4_air_c1_traffic.json"),
                     help="JSON {kernel: HBM bytes per launch} from rocprofv3 FETCH_SIZE/WRITE_SIZE runs")
     a = ap.parse_args()
+    global RK_TILE
+    RK_TILE = a.rk_tile
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): the domain lives in one "
               "GPU's HBM and no multi-GPU path is built", file=sys.stderr)
@@ -419,12 +427,18 @@ def main():
     dom = max(r["kernels"], key=lambda k: k["total_ms"])
     avg_s = dom["total_ms"] / dom["launches"] / 1e3
     achieved = dom["bytes_per_launch"] / avg_s / 1e9
-    traffic = None
-    if a.traffic_json and os.path.exists(a.traffic_json):
+    traffic = traffic_src = None
+    if a.traffic_json and os.path.exists(a.traffic_json) and a.workload == "c1":
+        # HBM bytes per launch of the launch shape this run's autotuner kept (the profiles
+        # measure each shape; null for a shape they do not hold)
         with open(a.traffic_json) as fh:
-            traffic = json.load(fh).get(f"{dom['name']}_fp{head}") if a.workload == "c1" else None
+            tj = json.load(fh)
+        per_shape = tj.get(f"{dom['name']}_fp{head}") or {}
+        traffic = per_shape.get(r["rk_tile"]) if isinstance(per_shape, dict) else None
+        traffic_src = tj.get("_source") if traffic is not None else None
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom["name"],
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
+            "kernel": dom["name"], "launch_shape": r["rk_tile"],
             "bytes_per_launch": dom["bytes_per_launch"], "avg_launch_ms": round(avg_s * 1e3, 4),
             "copy_kernel_GBs": round(2 * (1 << 30) / (r["copy_ms"] / 1e3) / 1e9, 1) if r["copy_ms"] > 0 else None}
     acc = None
