@@ -227,8 +227,11 @@ template <typename T> __device__ __forceinline__ T v_pore_walk(const BCArgs<T> &
 }
 
 // i, j, k of boundary node t of an nx*ny*nz box: the two x planes, then the two y planes
-// without their x edges, then the two z planes without x and y edges.
-__device__ __forceinline__ void boundary_node(long long t64, int nx, int ny, int nz, int &i, int &j, int &k)
+// without their x edges, then the two z planes without x and y edges.  With xring the x
+// planes contribute only their edge rings (rows j = 0 and ny - 1 of every plane k, then planes
+// k = 0 and nz - 1 of the rows between): the nodes of the interior rows belong to the air
+// stage (rk_stage_takes_xfaces).
+__device__ __forceinline__ void boundary_node(long long t64, int nx, int ny, int nz, bool xring, int &i, int &j, int &k)
 {
     // 32-bit divisions: the boundary counts fit (lee_create refuses grids whose counts do not).
     // Every branch sets all three coordinates as values: assigning a different output in each
@@ -238,7 +241,22 @@ __device__ __forceinline__ void boundary_node(long long t64, int nx, int ny, int
     const unsigned uy = (unsigned)ny, ux2 = (unsigned)(nx - 2), uz = (unsigned)nz;
     const unsigned nA = 2u * uy * uz, nB = 2u * ux2 * uz;
     int ii, jj, kk;
-    if (t < nA) {
+    bool ring = false;
+    if (xring) {
+        const unsigned per = 2u * uz + 2u * (uy - 2u);  // edge ring of one x plane
+        if (t < 2u * per) {
+            const unsigned side = t / per, r = t - side * per;
+            const unsigned r2 = r < 2u * uz ? 0u : r - 2u * uz;
+            ring = true;
+            ii = side ? nx - 1 : 0;
+            jj = r < 2u * uz ? ((r & 1u) ? ny - 1 : 0) : 1 + (int)(r2 >> 1);
+            kk = r < 2u * uz ? (int)(r >> 1) : ((r2 & 1u) ? nz - 1 : 0);
+        } else {
+            t = t - 2u * per + nA;  // the y and z planes follow as without the ring
+        }
+    }
+    if (ring) {
+    } else if (t < nA) {
         const unsigned pl = uy * uz, side = t / pl, r = t - side * pl, q = r / uy;
         ii = side ? nx - 1 : 0; jj = (int)(r - q * uy); kk = (int)q;
     } else if (t < nA + nB) {
@@ -296,7 +314,7 @@ __global__ __launch_bounds__(256) void k_bc_p_all(const BCArgs<T> a)
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < a.nb_air) {
         int i, j, k;
-        boundary_node(t, a.air.nx, a.air.ny, a.air.nz, i, j, k);
+        boundary_node(t, a.air.nx, a.air.ny, a.air.nz, a.xring != 0, i, j, k);
         const T v = p_air_walk<T, true>(a, i, j, k, 6);
         QBnd<T, 1> qb;
         qb_read<T, 1>(a, 3, i, j, k, qb);
@@ -305,7 +323,7 @@ __global__ __launch_bounds__(256) void k_bc_p_all(const BCArgs<T> a)
         qb_write<T, 1>(a, 3, qb, vals);
     } else if (t < a.nb_air + a.nb_pore) {
         int i, j, k;
-        boundary_node(t - a.nb_air, a.pore.nx, a.pore.ny, a.pore.nz, i, j, k);
+        boundary_node(t - a.nb_air, a.pore.nx, a.pore.ny, a.pore.nz, false, i, j, k);
         a.pore.pnn[gidx(a.pore, i, j, k)] = p_pore_walk(a, i, j, k, 6);
     } else if (t < a.nb_air + a.nb_pore + a.nrec) {
         // receivers sample time level n, which this step does not modify (airpore.cpp:573-580)
@@ -323,7 +341,7 @@ __global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < a.nb_air) {
         int i, j, k;
-        boundary_node(t, a.air.nx, a.air.ny, a.air.nz, i, j, k);
+        boundary_node(t, a.air.nx, a.air.ny, a.air.nz, a.xring != 0, i, j, k);
         const size_t id = gidx(a.air, i, j, k);
         if (a.ground && k == 0 && i > 0 && i < a.air.nx - 1 && j > 0 && j < a.air.ny - 1) {
             // interior node of the ground plane: its velocities, then Update_Psi
@@ -366,7 +384,7 @@ __global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
         }
     } else if (t < a.nb_air + a.nb_pore) {
         int i, j, k;
-        boundary_node(t - a.nb_air, a.pore.nx, a.pore.ny, a.pore.nz, i, j, k);
+        boundary_node(t - a.nb_air, a.pore.nx, a.pore.ny, a.pore.nz, false, i, j, k);
         const T u = v_pore_walk(a, 0, i, j, k, 6);
         const T v = v_pore_walk(a, 1, i, j, k, 6);
         const T w = v_pore_walk(a, 2, i, j, k, 6);
@@ -393,7 +411,7 @@ __device__ __forceinline__ void rk_bnd_node(const RKBndArgs<T> &r, long long t)
     const BCArgs<T> &a = r.b;
     if (t < a.nb_air) {
         int i, j, k;
-        boundary_node(t, a.air.nx, a.air.ny, a.air.nz, i, j, k);
+        boundary_node(t, a.air.nx, a.air.ny, a.air.nz, a.xring != 0, i, j, k);
         const Grid<T> &g = a.air;
         const size_t id = gidx(g, i, j, k);
         // every read first -- the face walks, the node's stage state, the auxiliaries of each
@@ -438,7 +456,7 @@ __device__ __forceinline__ void rk_bnd_node(const RKBndArgs<T> &r, long long t)
             }
     } else if (t < a.nb_air + a.nb_pore) {
         int i, j, k;
-        boundary_node(t - a.nb_air, a.pore.nx, a.pore.ny, a.pore.nz, i, j, k);
+        boundary_node(t - a.nb_air, a.pore.nx, a.pore.ny, a.pore.nz, false, i, j, k);
         const Grid<T> &g = a.pore;
         const size_t id = gidx(g, i, j, k);
         const T p = p_pore_walk(a, i, j, k, 6);
@@ -485,10 +503,11 @@ template void launch_rk_bnd_batch<double>(const RKBndArgs<double> *, const int *
 template void launch_rk_bnd<float>(const RKBndArgs<float> &, const RKBndArgs<float> *, hipStream_t);
 template void launch_rk_bnd<double>(const RKBndArgs<double> &, const RKBndArgs<double> *, hipStream_t);
 
-template <typename T> long long boundary_count(int nx, int ny, int nz)
+template <typename T> long long boundary_count(int nx, int ny, int nz, bool xring)
 {
     if (nx <= 0) return 0;
-    return 2LL * ny * nz + 2LL * (nx - 2) * nz + 2LL * (nx - 2) * (ny - 2);
+    const long long xplanes = xring ? 2LL * (2LL * nz + 2LL * (ny - 2)) : 2LL * ny * nz;
+    return xplanes + 2LL * (nx - 2) * nz + 2LL * (nx - 2) * (ny - 2);
 }
 
 template <typename T> void launch_bc_p_all(const BCArgs<T> &a, hipStream_t s)
@@ -503,8 +522,8 @@ template <typename T> void launch_bc_v_all(const BCArgs<T> &a, hipStream_t s)
     hipLaunchKernelGGL((k_bc_v_all<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
 }
 
-template long long boundary_count<float>(int, int, int);
-template long long boundary_count<double>(int, int, int);
+template long long boundary_count<float>(int, int, int, bool);
+template long long boundary_count<double>(int, int, int, bool);
 template void launch_bc_p_all<float>(const BCArgs<float> &, hipStream_t);
 template void launch_bc_p_all<double>(const BCArgs<double> &, hipStream_t);
 template void launch_bc_v_all<float>(const BCArgs<float> &, hipStream_t);

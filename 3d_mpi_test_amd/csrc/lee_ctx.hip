@@ -61,6 +61,8 @@ struct lee_ctx {
     int qcur = 0;
     bool fused = false;     // both air phases in one kernel (k_air_fb)
     int rk_variant = -1;    // RK4 air-stage launch shape (kRkTiles index), -1 until tuned
+    bool rk_xfaces = false; // that shape's air stage takes the x-face nodes of the interior rows
+    bool fb_xfaces = false; // the split FB passes take them
     // RK4 air-stage arguments of the four stages in device memory and the host copy they were
     // uploaded from (refreshed when buffer roles or the y1 gate change, never inside a capture)
     void *rk_args_dev = nullptr;
@@ -332,6 +334,7 @@ template <typename T> BCArgs<T> bc_args(lee_ctx *c, const AirArgs<T> &aa, double
     if (m.south_porous) b.pore = pore_grid<T>(c);
     else { std::memset(&b.pore, 0, sizeof b.pore); }
     b.porous = m.south_porous; b.iface = m.south_porous && m.iface_copy; b.tdbc = m.east_tdbc; b.y1on = aa.y1on;
+    b.xring = 0;  // the boundary passes walk whole x planes unless a launch shape takes their rows
     b.nb_air = boundary_count<T>(m.nx, m.ny, m.nz);
     b.nb_pore = m.south_porous ? boundary_count<T>(m.pnx, m.pny, m.pnz) : 0;
     for (int s = 0; s < 6; ++s) {
@@ -610,6 +613,12 @@ void rk4_stage_args(lee_ctx *c, int st, int J1, int y1on, RKArgs<T> &r, RKPoreAr
     b.b = bc_args<T>(c, r.a, nullptr);
     b.b.tdbc = 0;
     b.b.nrec = 0;
+    c->rk_xfaces = rk_stage_takes_xfaces<T>(r, c->rk_variant);
+    if (c->rk_xfaces) {
+        // the air stage's launch shape takes the x-face nodes of the interior rows
+        b.b.xring = 1;
+        b.b.nb_air = boundary_count<T>(m.nx, m.ny, m.nz, true);
+    }
     b.b.air.unn = dst[0]; b.b.air.vnn = dst[1]; b.b.air.wnn = dst[2]; b.b.air.pnn = dst[3];
     if (m.south_porous) {
         b.b.pore.unn = pdst[0]; b.b.pore.vnn = pdst[1]; b.b.pore.wnn = pdst[2]; b.b.pore.pnn = pdst[3];
@@ -625,14 +634,15 @@ int rk_wind_mode(const Model &m) { return m.has_wind ? (m.wind_v_only ? 2 : 1) :
 // Brings the device copy of the four stage arguments (air stage, then boundary stage: four
 // RKArgs followed by four RKBndArgs) up to date for the gate state (J1, y1on).  A change is rare (frame shifts, the y1 gate opening) and waits for the copy.
 // The arguments depend on the buffer roles (level and auxiliary sets, the FP32 error and
-// second stage buffers a frame shift swaps) and the gate state only: that is the key.
+// second stage buffers a frame shift swaps), the gate state and the air stage's launch shape
+// (which may take the x-face nodes from the boundary stage): that is the key.
 template <typename T> int rk_args_sync(lee_ctx *c, int J1, int y1on)
 {
     const size_t sz = sizeof(RKArgs<T>), szb = sizeof(RKBndArgs<T>);
     static_assert(sizeof(RKArgs<T>) % alignof(RKBndArgs<T>) == 0, "boundary blocks follow the air blocks");
-    const uintptr_t key[8] = {(uintptr_t)c->cur, (uintptr_t)c->qcur, (uintptr_t)J1, (uintptr_t)y1on,
+    const uintptr_t key[9] = {(uintptr_t)c->cur, (uintptr_t)c->qcur, (uintptr_t)J1, (uintptr_t)y1on,
                               (uintptr_t)c->rk_sb[0][0], (uintptr_t)c->rk_err[0][0], (uintptr_t)c->rk_sb[1][0],
-                              (uintptr_t)c->rk_err[1][0]};
+                              (uintptr_t)c->rk_err[1][0], (uintptr_t)(intptr_t)c->rk_variant};
     std::vector<unsigned char> now((const unsigned char *)key, (const unsigned char *)key + sizeof key);
     if (c->rk_args_dev && now == c->rk_args_key) return LEE_OK;
     if (c->capturing) return set_err(c, LEE_ERR_STATE, "RK4 stage arguments changed inside a captured chunk");
@@ -712,6 +722,12 @@ template <typename T> void enqueue_fb_step(lee_ctx *c, double *rec_slot)
     PoreArgs<T> pa{};
     if (m.south_porous) pa = PoreArgs<T>{pore_grid<T>(c), (T)m.p_gama, (T)m.p_invbeta, (T)m.p_eff};
     BCArgs<T> b = bc_args<T>(c, a, rec_slot);
+    c->fb_xfaces = !c->fused && fb_passes_take_xfaces<T>(a);
+    if (c->fb_xfaces) {
+        // the split passes take the x-face nodes of the interior rows
+        b.xring = 1;
+        b.nb_air = boundary_count<T>(m.nx, m.ny, m.nz, true);
+    }
     if (c->fused) {
         // porous pressure first: the air South face reads its plane Kp-2 through the interface
         if (m.south_porous) {
@@ -1675,11 +1691,14 @@ int lee_kernel_stats(const lee_ctx *c, lee_kstat *out, int cap)
         qrk += cnt * 16.0;
     }
     const double wrk = m.has_wind ? (m.wind_v_only ? 1.0 : 3.0) * m.nx * m.ny * m.nz : 0.0;
+    // x-face nodes of the interior rows, when the RK4 air stage's launch shape takes them
+    const double nxf = c->rk_xfaces ? 2.0 * (m.ny - 2) * (m.nz - 2) : 0.0;
+    const double fxf = c->fb_xfaces ? 2.0 * (m.ny - 2) * (m.nz - 2) : 0.0;  // the same for the FB passes
     const char *names[KS_N] = {"air_p", "air_v", "pore_p", "pore_v", "air_fb", "bc_p", "bc_v", "rk_air", "rk_pore", "rk_bnd"};
-    double bytes[KS_N] = {(5 * inner + qp + wind) * es, (7 * inner + qv + wind) * es, 5 * pinner * es, 7 * pinner * es,
-                          (8 * inner + qf + wind) * es, 2 * nb * es, 6 * nb * es + psi,
-                          (rkw * inner + qrk + wrk) * es, prkw * pinner * es, 8 * nb * es};
-    double pts[KS_N] = {inner, inner, pinner, pinner, inner, nb, nb, inner, pinner, nb};
+    double bytes[KS_N] = {(5 * inner + qp + wind + 2 * fxf) * es, (7 * inner + qv + wind + 6 * fxf) * es, 5 * pinner * es,
+                          7 * pinner * es, (8 * inner + qf + wind) * es, 2 * (nb - fxf) * es, 6 * (nb - fxf) * es + psi,
+                          (rkw * inner + qrk + wrk + 8 * nxf) * es, prkw * pinner * es, 8 * (nb - nxf) * es};
+    double pts[KS_N] = {inner, inner, pinner, pinner, inner, nb - fxf, nb - fxf, inner, pinner, nb - nxf};
     int n = 0;
     for (int k = 0; k < KS_N && n < cap; ++k) {
         lee_kstat s = c->kstat[k];

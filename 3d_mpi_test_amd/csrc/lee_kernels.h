@@ -84,6 +84,7 @@ template <typename T> struct PoreArgs {
 template <typename T> struct BCArgs {
     Grid<T> air, pore;         // pore.nx == 0 without a porous ground
     int porous, iface, tdbc, y1on;
+    int xring;                 // the x planes' interior-row nodes are the air stage's (rk_stage_takes_xfaces)
     long long nb_air, nb_pore; // boundary nodes of each box
     gptr<const T> Qo[6][4];
     gptr<T> Qn[6][4];
@@ -153,6 +154,12 @@ constexpr RkTile kRkTiles[8] = {{4, false, "64x4 planes"}, {4, true, "64x4 xcd-c
 constexpr int kRkVariants = 8;
 template <typename T>
 void launch_rk_air_v(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool terrain, int v, hipStream_t s);
+// Whether shape v's air stage also takes the x-face nodes of the interior rows (the boundary
+// stage then walks only the x planes' edge rings: BCArgs::xring).
+template <typename T> bool rk_stage_takes_xfaces(const RKArgs<T> &r, int v);
+// Whether the split FB passes (launch_air_p / launch_air_v) take the x-face nodes of the
+// interior rows (the face passes then walk only the x planes' edge rings).
+template <typename T> bool fb_passes_take_xfaces(const AirArgs<T> &a);
 template <typename T> void launch_rk_pore(const RKPoreArgs<T> &r, bool terrain, hipStream_t s);
 // r: the stage arguments on the host (node count), rd: the same in device memory (read by the
 // kernel; by value the block was copied to scratch for the slab indexing)
@@ -174,7 +181,7 @@ void launch_rk_pore_batch(const RKPoreArgs<T> *args, const int *first, int n, in
 template <typename T> void launch_rk_bnd_batch(const RKBndArgs<T> *args, const int *first, int n, int nblocks, hipStream_t s);
 template <typename T> void launch_sample_batch(const T *const *p, const long long *idx, int n, double *out, hipStream_t s);
 
-template <typename T> long long boundary_count(int nx, int ny, int nz);
+template <typename T> long long boundary_count(int nx, int ny, int nz, bool xring = false);
 template <typename T> void launch_bc_p_all(const BCArgs<T> &a, hipStream_t s);
 template <typename T> void launch_bc_v_all(const BCArgs<T> &a, hipStream_t s);
 

@@ -55,7 +55,7 @@ template <int NB> struct BoxList {
 // argument, read at constant offsets only: the box's fields as sums of products with 0/1
 // (selects between loaded values were merged into one load through a selected address,
 // which put the argument in private memory).  With pad each plane of a box takes a multiple
-// of 8 blocks (the extra ones get i = 0 and leave at the bounds check) and the boxes start at
+// of 8 blocks (the extra ones get i = -1 and leave at the bounds check) and the boxes start at
 // multiples of 8: blocks are dealt to the 8 XCDs round-robin, so a tile column then stays on
 // one XCD through its planes, and the planes k - 1, k + 1 a tile reads come from that XCD's
 // L2 (on the 318-tile planes of the 512 x 256 x 128 interior, plane k + 1 of a column went to
@@ -73,7 +73,7 @@ __device__ __forceinline__ void list_point_(const BoxList<NB> &l, int t, int &i,
     t -= f;
     const int per = nbx * nby, pp = l.pad ? (per + 7) & ~7 : per, kk = t / pp, col = t - kk * pp, ry = col / nbx;
     if (col >= per) {  // padding: leaves at the caller's bounds check
-        i = 0; j = 1; k = k0;
+        i = -1; j = 1; k = k0;
         return;
     }
     k = k0 + kk;
@@ -657,7 +657,10 @@ template <typename T> __device__ __forceinline__ int obstacle_id(const AirArgs<T
 // the point's Q_p and returns p^{n+1}.
 // L as for air_dp; below 7 the point is also outside the obstacle (the split launches run
 // only without one).
-template <typename T, bool WIND, bool TERRAIN = false, int L = 7>
+// XF: the lanes on the x faces take the face node p = p(1 or nx - 2) and its Q_p, as
+// rk_air_point (the Front / Back copy of calculation_2D.cpp:696-705 that the pressure face
+// pass gives such a node).
+template <typename T, bool WIND, bool TERRAIN = false, int L = 7, bool XF = false>
 __device__ __forceinline__ T air_p_point(const AirArgs<T> &a, int i, int j, int k)
 {
     const Grid<T> &g = a.g;
@@ -667,12 +670,14 @@ __device__ __forceinline__ T air_p_point(const AirArgs<T> &a, int i, int j, int 
         g.pnn[id] = pnew;
         return pnew;
     }
+    const bool face = XF && (i == 0 || i == g.nx - 1);
     const OwnQ<T> own = own_slabs<T, L>(a, i, j, k);
     T qp_old[6];
 #pragma unroll
     for (int s = 0; s < 6; ++s) qp_old[s] = own.in[s] ? a.Qo[s][3][own.idx[s]] : T(0);
-    T r = air_dp<T, WIND ? 1 : 0, TERRAIN, false, L>(a, i, j, k);
-    if (const int mo = L == 7 ? obstacle_id(a, i, j, k) : 0) {
+    T r = T(0);
+    if (!face) r = air_dp<T, WIND ? 1 : 0, TERRAIN, false, L>(a, i, j, k);
+    if (const int mo = L == 7 && !XF ? obstacle_id(a, i, j, k) : 0) {
         // inside the obstacle the porous-medium continuity equation replaces the increment
         // (air.cpp:2294-2318)
         T mX, mY, mZ;
@@ -682,7 +687,12 @@ __device__ __forceinline__ T air_p_point(const AirArgs<T> &a, int i, int j, int 
                       ((mZ * (g.wn[id + sxy] - g.wn[id])) * g.dzi);
         r = (-a.ogama[mo]) * div;
     }
-    const T pnew = g.pn[id] + r;
+    T pnew = g.pn[id] + r;
+    if constexpr (XF) {
+        const int lane = (int)threadIdx.x;
+        const T v = __shfl(pnew, i == 0 ? lane + 1 : i == g.nx - 1 ? lane - 1 : lane, 64);
+        if (face) pnew = v;
+    }
     g.pnn[id] = pnew;
     // Q_p += dt*p of this point; the value is final, face conditions only touch boundary nodes
 #pragma unroll
@@ -770,7 +780,8 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
 
 // FB velocity update at one interior point with p at n+1: calculation_2D::Cal_velocity
 // (calculation_2D.cpp:564-574) and Update_PML_Qvw (air.cpp:2323-2386).
-template <typename T, bool WIND, bool TERRAIN = false, int L = 7>
+// XF as for air_p_point: u = 0, v and w copied (calculation_2D.cpp:784-864).
+template <typename T, bool WIND, bool TERRAIN = false, int L = 7, bool XF = false>
 __device__ __forceinline__ void air_v_point(const AirArgs<T> &a, int i, int j, int k)
 {
     const Grid<T> &g = a.g;
@@ -783,6 +794,7 @@ __device__ __forceinline__ void air_v_point(const AirArgs<T> &a, int i, int j, i
         g.wnn[id] = g.wn[id] + dw;
         return;
     }
+    const bool face = XF && (i == 0 || i == g.nx - 1);
     const OwnQ<T> own = own_slabs<T, L>(a, i, j, k);
     T qu_old[6], qv_old[6], qw_old[6];
 #pragma unroll
@@ -791,10 +803,10 @@ __device__ __forceinline__ void air_v_point(const AirArgs<T> &a, int i, int j, i
         qv_old[s] = own.in[s] ? a.Qo[s][1][own.idx[s]] : T(0);
         qw_old[s] = own.in[s] ? a.Qo[s][2][own.idx[s]] : T(0);
     }
-    T du, dv, dw;
-    air_duvw<T, WIND ? 1 : 0, TERRAIN, false, L>(a, i, j, k, du, dv, dw);
+    T du = T(0), dv = T(0), dw = T(0);
+    if (!face) air_duvw<T, WIND ? 1 : 0, TERRAIN, false, L>(a, i, j, k, du, dv, dw);
     const T dt = g.dt;
-    if (const int mo = L == 7 ? obstacle_id(a, i, j, k) : 0) {
+    if (const int mo = L == 7 && !XF ? obstacle_id(a, i, j, k) : 0) {
         // inside the obstacle the porous-medium momentum equation replaces the increments
         // (air.cpp:1907-1945)
         T mX, mY, mZ;
@@ -808,7 +820,12 @@ __device__ __forceinline__ void air_v_point(const AirArgs<T> &a, int i, int j, i
         dv = (f * ((mY * (pc - p[id - g.nx])) * g.dyi) + vc * ib) - vc;
         dw = (f * ((mZ * (pc - p[id - sxy])) * g.dzi) + wc * ib) - wc;
     }
-    const T un = g.un[id] + du, vn = g.vn[id] + dv, wn = g.wn[id] + dw;
+    T un = g.un[id] + du, vn = g.vn[id] + dv, wn = g.wn[id] + dw;
+    if constexpr (XF) {
+        const int lane = (int)threadIdx.x, from = i == 0 ? lane + 1 : i == g.nx - 1 ? lane - 1 : lane;
+        const T v1 = __shfl(vn, from, 64), w1 = __shfl(wn, from, 64);
+        if (face) { un = T(0); vn = v1; wn = w1; }
+    }
     g.unn[id] = un;
     g.vnn[id] = vn;
     g.wnn[id] = wn;
@@ -836,21 +853,21 @@ __global__ __launch_bounds__(BX * BY) void k_air_p(const AirArgs<T> a)
 // its edges); together they cover the points of k_air_p / k_air_v with the same arithmetic.
 // On the 512 x 256 x 128 grid with its 20-point layers (FP32) the interior launches take
 // 39 + 54 us for 53 % of the points, the layer launches the rest of the 143 + 180 us.
-template <typename T, bool WIND, bool TERRAIN, int L, int NB>
+template <typename T, bool WIND, bool TERRAIN, int L, int NB, bool XF = false>
 __global__ __launch_bounds__(BX * BY) void k_air_p_box(const AirArgs<T> a, const BoxList<NB> l)
 {
     int i, j, k;
     list_point(l, blockIdx.x, i, j, k);
-    if (i < 1 || i >= a.g.nx - 1 || j >= a.g.ny - 1) return;
-    (void)air_p_point<T, WIND, TERRAIN, L>(a, i, j, k);
+    if (i < (XF ? 0 : 1) || i >= a.g.nx - (XF ? 0 : 1) || j >= a.g.ny - 1) return;
+    (void)air_p_point<T, WIND, TERRAIN, L, XF>(a, i, j, k);
 }
-template <typename T, bool WIND, bool TERRAIN, int L, int NB>
+template <typename T, bool WIND, bool TERRAIN, int L, int NB, bool XF = false>
 __global__ __launch_bounds__(BX * BY) void k_air_v_box(const AirArgs<T> a, const BoxList<NB> l)
 {
     int i, j, k;
     list_point(l, blockIdx.x, i, j, k);
-    if (i < 1 || i >= a.g.nx - 1 || j >= a.g.ny - 1) return;
-    air_v_point<T, WIND, TERRAIN, L>(a, i, j, k);
+    if (i < (XF ? 0 : 1) || i >= a.g.nx - (XF ? 0 : 1) || j >= a.g.ny - 1) return;
+    air_v_point<T, WIND, TERRAIN, L, XF>(a, i, j, k);
 }
 
 template <typename T, bool WIND, bool TERRAIN>
@@ -1088,33 +1105,52 @@ __device__ __forceinline__ T rk_update_if(int st, T xn, T k, gptr<T> acc, gptr<T
     }
 }
 
-template <typename T, int WM, bool TERRAIN, bool COMP, int L = 7>
+// XF: the lanes on the x faces (i = 0 and nx - 1, rows and planes of the interior) take
+// their face node of the destination state -- u = 0, v, w, p copied from the interior
+// neighbour (calculation_2D.cpp:696-725, 784-864, the values the boundary stage's walks give
+// such a node) -- and the node's auxiliaries, in place of the boundary stage: the lines of
+// those nodes are streamed by this pass anyway, and one node per row costs a line per array
+// in the boundary stage.  The lane is idle otherwise; the neighbour's values come across the
+// wavefront (the face's neighbour lane is in the same 64-point row).
+template <typename T, int WM, bool TERRAIN, bool COMP, int L = 7, bool XF = false>
 __device__ __forceinline__ void rk_air_point(const RKArgs<T> &r, int i, int j, int k)
 {
     const AirArgs<T> &a = r.a;
     const size_t id = (size_t)i + (size_t)a.g.nx * ((size_t)j + (size_t)a.g.ny * k);
     const int st = r.stage;
-    T kf[4];
-    kf[3] = air_dp<T, WM, TERRAIN, true, L>(a, i, j, k);
-    air_duvw<T, WM, TERRAIN, true, L>(a, i, j, k, kf[0], kf[1], kf[2]);
+    const bool face = XF && (i == 0 || i == a.g.nx - 1);
+    T kf[4] = {T(0), T(0), T(0), T(0)};
+    if (!face) {
+        kf[3] = air_dp<T, WM, TERRAIN, true, L>(a, i, j, k);
+        air_duvw<T, WM, TERRAIN, true, L>(a, i, j, k, kf[0], kf[1], kf[2]);
+    }
     const T xc[4] = {a.g.un[id], a.g.vn[id], a.g.wn[id], a.g.pn[id]};  // stage state at the point
     // every register read of the point first, then the arithmetic, then the stores
     T xn[4], ac[4], er[4];
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         // level n: the stage state itself at stage 0; not needed by the last FP64 stage
-        xn[f] = st == 0 ? xc[f] : (!COMP && st == 3) ? T(0) : r.xn[f][id];
-        ac[f] = st == 0 ? T(0) : r.acc[f][id];
-        er[f] = (COMP && st == 3) ? r.err[f][id] : T(0);
+        xn[f] = (st == 0 || face) ? xc[f] : (!COMP && st == 3) ? T(0) : r.xn[f][id];
+        ac[f] = (st == 0 || face) ? T(0) : r.acc[f][id];
+        er[f] = (COMP && st == 3 && !face) ? r.err[f][id] : T(0);
     }
     T d[4], ao[4], eo[4];
 #pragma unroll
     for (int f = 0; f < 4; ++f) d[f] = rk_regs<T, COMP>(st, xn[f], kf[f], ac[f], er[f], ao[f], eo[f]);
+    if constexpr (XF) {
+        const int lane = (int)threadIdx.x;
+        const int from = i == 0 ? lane + 1 : i == a.g.nx - 1 ? lane - 1 : lane;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            const T v = __shfl(d[f], from, 64);
+            if (face) d[f] = f == 0 ? T(0) : v;
+        }
+    }
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         r.dst[f][id] = d[f];
-        if (st < 3) r.acc[f][id] = ao[f];
-        if (COMP && st == 3) r.err[f][id] = eo[f];
+        if (st < 3 && !face) r.acc[f][id] = ao[f];
+        if (COMP && st == 3 && !face) r.err[f][id] = eo[f];
     }
     if constexpr (L == 0) return;
     // auxiliaries of the point's own slabs: dQ/dt = f (air.cpp:2323-2439 as an ODE), each
@@ -1183,15 +1219,15 @@ __global__ __launch_bounds__(BX * TY) void k_rk_air_t(const RKArgs<T> *__restric
 // Split form of the RK4 air stage (64 x 4 tiles), as k_air_p_box.  The FP32 interior kernel
 // is held to 72 registers (7 waves per SIMD at least): left free it took 77 and ran 6; the
 // FP64 one, and the FP32 one with a uniform mean flow (WM 1), would spill there.
-template <typename T, int WM, bool TERRAIN, bool COMP, int L, int NB>
+template <typename T, int WM, bool TERRAIN, bool COMP, int L, int NB, bool XF = false>
 __global__ __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu((COMP && L == 0 && WM != 1) ? 7 : 1, 8))) void k_rk_air_box(
     const RKArgs<T> *__restrict__ rp, const BoxList<NB> l)
 {
     const RKArgs<T> &r = *rp;
     int i, j, k;
     list_point(l, blockIdx.x, i, j, k);
-    if (i < 1 || i >= r.a.g.nx - 1 || j >= r.a.g.ny - 1) return;
-    rk_air_point<T, WM, TERRAIN, COMP, L>(r, i, j, k);
+    if (i < (XF ? 0 : 1) || i >= r.a.g.nx - (XF ? 0 : 1) || j >= r.a.g.ny - 1) return;
+    rk_air_point<T, WM, TERRAIN, COMP, L, XF>(r, i, j, k);
 }
 
 // Interior RK4 air stage with the stage state staged through LDS.  Each block copies the
@@ -1693,19 +1729,31 @@ unsigned flat_blocks(size_t total)
 
 }  // namespace
 
+// Whether the split FB passes take the x-face nodes of the interior rows (the interior is
+// not empty and the East face lane has its neighbour in its own 64-point row); the face
+// passes then walk only the x planes' edge rings.
+template <typename T> bool fb_passes_take_xfaces(const AirArgs<T> &a)
+{
+    Regions R;
+    return a.g.nx % BX != 1 && split_regions(a, R, false);
+}
+
 template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s)
 {
     dim3 g = interior_grid(a.g.nx, a.g.ny, a.g.nz), b(BX, BY);
     const bool ter = a.g.tsh != nullptr;
     Regions R;
     if (split_regions(a, R, false)) {
+        const bool xf = fb_passes_take_xfaces<T>(a);
 #define LEE_PS(W, TER)                                                                                       \
     do {                                                                                                     \
         if (R.in.n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 0, 1>), dim3(list_blocks(R.in)), b, 0, s, a, R.in);    \
-        if (R.ax[0].n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 1, 2>), dim3(list_blocks(R.ax[0])), b, 0, s, a, R.ax[0]); \
+        if (R.ax[0].n && xf) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 1, 2, true>), dim3(list_blocks(R.ax[0])), b, 0, s, a, R.ax[0]); \
+        else if (R.ax[0].n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 1, 2>), dim3(list_blocks(R.ax[0])), b, 0, s, a, R.ax[0]); \
         if (R.ax[1].n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 2, 2>), dim3(list_blocks(R.ax[1])), b, 0, s, a, R.ax[1]); \
         if (R.ax[2].n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 4, 2>), dim3(list_blocks(R.ax[2])), b, 0, s, a, R.ax[2]); \
-        if (R.edge.n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 7, 12>), dim3(list_blocks(R.edge)), b, 0, s, a, R.edge); \
+        if (R.edge.n && xf) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 7, 12, true>), dim3(list_blocks(R.edge)), b, 0, s, a, R.edge); \
+        else if (R.edge.n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 7, 12>), dim3(list_blocks(R.edge)), b, 0, s, a, R.edge); \
     } while (0)
         if (ter) { if (wind) LEE_PS(true, true); else LEE_PS(false, true); }
         else { if (wind) LEE_PS(true, false); else LEE_PS(false, false); }
@@ -1726,13 +1774,16 @@ template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStrea
     const bool ter = a.g.tsh != nullptr;
     Regions R;
     if (split_regions(a, R, false)) {
+        const bool xf = fb_passes_take_xfaces<T>(a);
 #define LEE_VS(W, TER)                                                                                       \
     do {                                                                                                     \
         if (R.in.n) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 0, 1>), dim3(list_blocks(R.in)), b, 0, s, a, R.in);    \
-        if (R.ax[0].n) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 1, 2>), dim3(list_blocks(R.ax[0])), b, 0, s, a, R.ax[0]); \
+        if (R.ax[0].n && xf) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 1, 2, true>), dim3(list_blocks(R.ax[0])), b, 0, s, a, R.ax[0]); \
+        else if (R.ax[0].n) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 1, 2>), dim3(list_blocks(R.ax[0])), b, 0, s, a, R.ax[0]); \
         if (R.ax[1].n) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 2, 2>), dim3(list_blocks(R.ax[1])), b, 0, s, a, R.ax[1]); \
         if (R.ax[2].n) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 4, 2>), dim3(list_blocks(R.ax[2])), b, 0, s, a, R.ax[2]); \
-        if (R.edge.n) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 7, 12>), dim3(list_blocks(R.edge)), b, 0, s, a, R.edge); \
+        if (R.edge.n && xf) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 7, 12, true>), dim3(list_blocks(R.edge)), b, 0, s, a, R.edge); \
+        else if (R.edge.n) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 7, 12>), dim3(list_blocks(R.edge)), b, 0, s, a, R.edge); \
     } while (0)
         if (ter) { if (wind) LEE_VS(true, true); else LEE_VS(false, true); }
         else { if (wind) LEE_VS(true, false); else LEE_VS(false, false); }
@@ -1766,11 +1817,21 @@ void launch_rk_air(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool terra
 // split_regions: the interior and the shell, or the interior and the shell by slab axes; 7 as
 // 5 with the interior staged through LDS (5 for a uniform mean flow or rows not a multiple of
 // 16 bytes).  v = 0 when the interior is empty.
+// Whether launch shape v of the stage takes the x-face nodes of the interior rows (the split
+// shapes, when the interior is not empty and the East face lane has its neighbour in its own
+// 64-point row); the boundary stage then walks only the x planes' edge rings.
+template <typename T> bool rk_stage_takes_xfaces(const RKArgs<T> &r, int v)
+{
+    Regions R;
+    return v >= 5 && v <= 7 && r.a.g.nx % BX != 1 && split_regions(r.a, R, true);
+}
+
 template <typename T>
 void launch_rk_air_v(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool terrain, int v, hipStream_t s)
 {
     Regions R;
     if (v <= 0 || (v >= 5 && !split_regions(r.a, R, true))) { launch_rk_air<T>(r, rd, wind, terrain, s); return; }
+    const bool xf = rk_stage_takes_xfaces<T>(r, v);
     constexpr bool C = std::is_same<T, float>::value;
     if (v == 7 && wind != 1 && r.a.g.nx % Stage<T>::H == 0) {
         // the interior staged through LDS, the shell as v 5
@@ -1778,7 +1839,10 @@ void launch_rk_air_v(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool ter
 #define LEE_RKL(WM, TER)                                                                                          \
     do {                                                                                                        \
         if (R.in.n) hipLaunchKernelGGL((k_rk_air_lds<T, WM, TER, C>), dim3(list_blocks(R.in)), b, 0, s, rd, R.in); \
-        if (R.shell.n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 6>), dim3(list_blocks(R.shell)), b, 0, s, rd, R.shell); \
+        if (R.shell.n) {                                                                                        \
+            if (xf) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 6, true>), dim3(list_blocks(R.shell)), b, 0, s, rd, R.shell); \
+            else hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 6>), dim3(list_blocks(R.shell)), b, 0, s, rd, R.shell); \
+        }                                                                                                       \
     } while (0)
         if (terrain) { if (wind == 2) LEE_RKL(2, true); else LEE_RKL(0, true); }
         else { if (wind == 2) LEE_RKL(2, false); else LEE_RKL(0, false); }
@@ -1792,13 +1856,16 @@ void launch_rk_air_v(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool ter
     do {                                                                                                        \
         if (R.in.n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 0, 1>), dim3(list_blocks(R.in)), b, 0, s, rd, R.in); \
         if (v == 5) {                                                                                           \
-            if (R.shell.n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 6>), dim3(list_blocks(R.shell)), b, 0, s, rd, R.shell); \
+            if (R.shell.n && xf) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 6, true>), dim3(list_blocks(R.shell)), b, 0, s, rd, R.shell); \
+            else if (R.shell.n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 6>), dim3(list_blocks(R.shell)), b, 0, s, rd, R.shell); \
             break;                                                                                              \
         }                                                                                                       \
-        if (R.ax[0].n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 1, 2>), dim3(list_blocks(R.ax[0])), b, 0, s, rd, R.ax[0]); \
+        if (R.ax[0].n && xf) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 1, 2, true>), dim3(list_blocks(R.ax[0])), b, 0, s, rd, R.ax[0]); \
+        else if (R.ax[0].n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 1, 2>), dim3(list_blocks(R.ax[0])), b, 0, s, rd, R.ax[0]); \
         if (R.ax[1].n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 2, 2>), dim3(list_blocks(R.ax[1])), b, 0, s, rd, R.ax[1]); \
         if (R.ax[2].n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 4, 2>), dim3(list_blocks(R.ax[2])), b, 0, s, rd, R.ax[2]); \
-        if (R.edge.n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 12>), dim3(list_blocks(R.edge)), b, 0, s, rd, R.edge); \
+        if (R.edge.n && xf) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 12, true>), dim3(list_blocks(R.edge)), b, 0, s, rd, R.edge); \
+        else if (R.edge.n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 12>), dim3(list_blocks(R.edge)), b, 0, s, rd, R.edge); \
     } while (0)
         if (terrain) {
             if (wind == 1) LEE_RKS(1, true); else if (wind == 2) LEE_RKS(2, true); else LEE_RKS(0, true);
@@ -1959,6 +2026,8 @@ void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s)
     template void launch_rk_pore<T>(const RKPoreArgs<T> &, bool, hipStream_t);                               \
     template void launch_rk_air_batch<T>(const RKArgs<T> *, const int *, int, int, int, bool, hipStream_t);   \
     template void launch_rk_air_v<T>(const RKArgs<T> &, const RKArgs<T> *, int, bool, int, hipStream_t);     \
+    template bool rk_stage_takes_xfaces<T>(const RKArgs<T> &, int);                                           \
+    template bool fb_passes_take_xfaces<T>(const AirArgs<T> &);                                              \
     template void launch_rk_pore_batch<T>(const RKPoreArgs<T> *, const int *, int, int, bool, hipStream_t);   \
     template void launch_sample_batch<T>(const T *const *, const long long *, int, double *, hipStream_t);     \
     template void launch_air_fb<T>(const AirArgs<T> &, bool, hipStream_t);                                   \
