@@ -314,7 +314,7 @@ __global__ __launch_bounds__(256) void k_bc_p_all(const BCArgs<T> a)
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < a.nb_air) {
         int i, j, k;
-        boundary_node(t, a.air.nx, a.air.ny, a.air.nz, a.xring != 0, i, j, k);
+        boundary_node(t, a.air.nx, a.air.ny, a.air.nz, a.xring == 1, i, j, k);
         const T v = p_air_walk<T, true>(a, i, j, k, 6);
         QBnd<T, 1> qb;
         qb_read<T, 1>(a, 3, i, j, k, qb);
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256) void k_bc_v_all(const BCArgs<T> a)
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < a.nb_air) {
         int i, j, k;
-        boundary_node(t, a.air.nx, a.air.ny, a.air.nz, a.xring != 0, i, j, k);
+        boundary_node(t, a.air.nx, a.air.ny, a.air.nz, a.xring == 1, i, j, k);
         const size_t id = gidx(a.air, i, j, k);
         if (a.ground && k == 0 && i > 0 && i < a.air.nx - 1 && j > 0 && j < a.air.ny - 1) {
             // interior node of the ground plane: its velocities, then Update_Psi
@@ -411,7 +411,9 @@ __device__ __forceinline__ void rk_bnd_node(const RKBndArgs<T> &r, long long t)
     const BCArgs<T> &a = r.b;
     if (t < a.nb_air) {
         int i, j, k;
-        boundary_node(t, a.air.nx, a.air.ny, a.air.nz, a.xring != 0, i, j, k);
+        boundary_node(t, a.air.nx, a.air.ny, a.air.nz, a.xring == 1, i, j, k);
+        if (a.xring == 2 && j > 0 && j < a.air.ny - 1 && k > 0 && k < a.air.nz - 1 && batch_takes_face(a.air.nx, i, j))
+            return;  // the batched air stage took this face node
         const Grid<T> &g = a.air;
         const size_t id = gidx(g, i, j, k);
         // every read first -- the face walks, the node's stage state, the auxiliaries of each

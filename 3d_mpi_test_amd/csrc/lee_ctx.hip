@@ -1809,6 +1809,8 @@ template <typename T> int ens_tables(lee_ensemble *e, const std::vector<int> &J1
         for (int d = 0; d < n; ++d) {
             RKPoreArgs<T> pr;
             rk4_stage_args<T>(e->m[d], st, J1[d], y1on[d], ra[d], pr, ba[d]);
+            // the batched air stage takes the x-face nodes whose neighbour is in its wavefront
+            ba[d].b.xring = 2;
             if (e->m[d]->m.south_porous) pa.push_back(pr);
         }
         EHIP(e, hipMemcpy(e->air_args[st], ra.data(), n * sizeof(RKArgs<T>), hipMemcpyHostToDevice));

@@ -84,7 +84,10 @@ template <typename T> struct PoreArgs {
 template <typename T> struct BCArgs {
     Grid<T> air, pore;         // pore.nx == 0 without a porous ground
     int porous, iface, tdbc, y1on;
-    int xring;                 // the x planes' interior-row nodes are the air stage's (rk_stage_takes_xfaces)
+    // the x planes' interior-row nodes the air stage takes: 0 none, 1 all (the split launches,
+    // rk_stage_takes_xfaces: the enumeration covers only the edge rings), 2 those of the
+    // batched stage (batch_takes_face; the enumeration skips them)
+    int xring;
     long long nb_air, nb_pore; // boundary nodes of each box
     gptr<const T> Qo[6][4];
     gptr<T> Qn[6][4];
@@ -173,6 +176,16 @@ constexpr int kBatchBlock = 256;
 inline long long batch_blocks(int nx, int ny, int nz)
 {
     return ((long long)nx * (ny - 2) + kBatchBlock - 1) / kBatchBlock * (nz - 2);
+}
+// Whether node (i, j) of an interior row is an x-face node the batched air stage takes: its
+// interior neighbour (i + 1 or i - 1, the next or previous column) is in the same wavefront
+// of 64 consecutive columns (blocks start at multiples of 256 columns).  The boundary stage
+// skips exactly these (BCArgs::xring 2).
+__host__ __device__ inline bool batch_takes_face(int nx, int i, int j)
+{
+    if (i != 0 && i != nx - 1) return false;
+    const int c = (j - 1) * nx + i;
+    return i == 0 ? (c & 63) != 63 : (c & 63) != 0;
 }
 template <typename T>
 void launch_rk_air_batch(const RKArgs<T> *args, const int *first, int n, int nblocks, int wind, bool terrain, hipStream_t s);

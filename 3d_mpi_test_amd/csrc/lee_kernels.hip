@@ -1105,6 +1105,35 @@ __device__ __forceinline__ T rk_update_if(int st, T xn, T k, gptr<T> acc, gptr<T
     }
 }
 
+// Auxiliaries of the point's own slabs: dQ/dt = f (air.cpp:2323-2439 as an ODE) with the RK4
+// register update, f the stage state xc at the point; each slab's reads before its stores.
+template <typename T, int L>
+__device__ __forceinline__ void rk_aux_point(const RKArgs<T> &r, int i, int j, int k, const T (&xc)[4])
+{
+    const AirArgs<T> &a = r.a;
+    const int st = r.stage;
+    const OwnQ<T> own = own_slabs<T, L>(a, i, j, k);
+    const T dt = a.g.dt;
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+        if (own.in[s]) {
+            const size_t q = own.idx[s];
+            T qx[4], qa[4];
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                qx[f] = r.qxn[s][f][q];
+                qa[f] = st != 0 ? r.qacc[s][f][q] : T(0);
+            }
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                T qao, unused;
+                const T qd = rk_regs<T, false>(st, qx[f], dt * xc[f], qa[f], T(0), qao, unused);
+                r.qdst[s][f][q] = qd;
+                if (st < 3) r.qacc[s][f][q] = qao;
+            }
+        }
+}
+
 // XF: the lanes on the x faces (i = 0 and nx - 1, rows and planes of the interior) take
 // their face node of the destination state -- u = 0, v, w, p copied from the interior
 // neighbour (calculation_2D.cpp:696-725, 784-864, the values the boundary stage's walks give
@@ -1152,29 +1181,7 @@ __device__ __forceinline__ void rk_air_point(const RKArgs<T> &r, int i, int j, i
         if (st < 3 && !face) r.acc[f][id] = ao[f];
         if (COMP && st == 3 && !face) r.err[f][id] = eo[f];
     }
-    if constexpr (L == 0) return;
-    // auxiliaries of the point's own slabs: dQ/dt = f (air.cpp:2323-2439 as an ODE), each
-    // slab's reads before its stores
-    const OwnQ<T> own = own_slabs<T, L>(a, i, j, k);
-    const T dt = a.g.dt;
-#pragma unroll
-    for (int s = 0; s < 6; ++s)
-        if (own.in[s]) {
-            const size_t q = own.idx[s];
-            T qx[4], qa[4];
-#pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                qx[f] = r.qxn[s][f][q];
-                qa[f] = st != 0 ? r.qacc[s][f][q] : T(0);
-            }
-#pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                T qao, unused;
-                const T qd = rk_regs<T, false>(st, qx[f], dt * xc[f], qa[f], T(0), qao, unused);
-                r.qdst[s][f][q] = qd;
-                if (st < 3) r.qacc[s][f][q] = qao;
-            }
-        }
+    if constexpr (L != 0) rk_aux_point<T, L>(r, i, j, k, xc);
 }
 
 // The stage arguments (some 200 pointers) live in device memory: passed by value they
@@ -1377,6 +1384,10 @@ __device__ __forceinline__ bool batch_point(int nx, int ny, int local, int &i, i
     return j < ny - 1 && i >= 1 && i < nx - 1;
 }
 
+// The batched stage also takes the x-face nodes of the interior rows whose interior neighbour
+// is in the same wavefront (batch_takes_face), as the split single-domain launches do
+// (rk_air_point with XF): the face lane runs the same code as the others and takes the
+// neighbour's values across the wavefront.
 template <typename T, int WM, bool TERRAIN, bool COMP>
 __global__ __launch_bounds__(BX * BY) void k_rk_air_batch(const RKArgs<T> *__restrict__ args, const int *__restrict__ first,
                                                          int n)
@@ -1385,8 +1396,8 @@ __global__ __launch_bounds__(BX * BY) void k_rk_air_batch(const RKArgs<T> *__res
     const RKArgs<T> &r = args[d];
     const int nx = r.a.g.nx, ny = r.a.g.ny;
     int i, j, k;
-    if (!batch_point(nx, ny, blockIdx.x - first[d], i, j, k)) return;
-    rk_air_point<T, WM, TERRAIN, COMP>(r, i, j, k);
+    const bool in = batch_point(nx, ny, blockIdx.x - first[d], i, j, k);
+    if (in || (j < ny - 1 && batch_takes_face(nx, i, j))) rk_air_point<T, WM, TERRAIN, COMP, 7, true>(r, i, j, k);
 }
 
 // Porous layer: dt*R_p = -gama div u (porous.cpp:77-95); dt*R_u = -dt eff grad p, with the
