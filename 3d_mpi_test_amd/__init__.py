@@ -123,6 +123,8 @@ def lib():
         L.lee_ensemble_last_ms.restype = ctypes.c_double
         L.lee_ensemble_error.argtypes = [vp]
         L.lee_ensemble_error.restype = ctypes.c_char_p
+        L.lee_ensemble_set_launch.argtypes = [vp, ctypes.c_int]
+        L.lee_ensemble_launch.argtypes = [vp]
         L.lee_read_source_series.argtypes = [ctypes.c_char_p, _dp, ctypes.c_longlong]
         L.lee_read_source_series.restype = ctypes.c_longlong
         L.lee_copy_bench.restype = ctypes.c_double
@@ -135,7 +137,7 @@ class Ensemble:
     launch per kernel kind over every member.  `decks` and per-member `receivers` lists;
     members share precision, wind kind and grid kind."""
 
-    def __init__(self, decks, precision: int = 32, receivers=None, use_graph: bool = True):
+    def __init__(self, decks, precision: int = 32, receivers=None, use_graph: bool = True, launch: int = 0):
         L = lib()
         n = len(decks)
         self._cdecks = (LeeDeck * n)()
@@ -166,6 +168,8 @@ class Ensemble:
             m = ctypes.c_void_p()
             self._check(L.lee_ensemble_member(self.h, d, ctypes.byref(m)))
             self.members.append(Solver._member(m, keep[2 * d]))
+        if launch:
+            self.set_launch(launch)
 
     def _check(self, rc):
         if rc != LEE_OK:
@@ -181,6 +185,15 @@ class Ensemble:
 
     def last_ms(self) -> float:
         return lib().lee_ensemble_last_ms(self.h)
+
+    def set_launch(self, form: int):
+        """Launch form of the air and porous stages: 0 autotune (default), 1 a block per
+        tile, 2 tiles padded per plane to multiples of 8, 3 persistent."""
+        self._check(lib().lee_ensemble_set_launch(self.h, form))
+
+    def launch(self) -> int:
+        """The launch form in use (1..3), 0 before the first step."""
+        return lib().lee_ensemble_launch(self.h)
 
     def points(self) -> int:
         return sum(m.nx * m.ny * m.nz + m.pnx * m.pny * m.pnz for m in self.members)

@@ -1748,8 +1748,12 @@ struct lee_ensemble {
     std::string err;
     std::vector<void *> allocs;
     void *air_args[4] = {}, *pore_args[4] = {}, *bnd_args[4] = {};
-    int *first = nullptr;  // [air n+1 | pore np+1 | bnd n+1]
-    int air_blocks = 0, pore_blocks = 0, bnd_blocks = 0, n_pore = 0;
+    // tile prefixes [air n+1 | pore np+1 | bnd n+1 | air np1 | pore np1]: the last two with
+    // each member plane's tiles padded to a multiple of 8 (launch form 1)
+    int *first = nullptr;
+    int air_blocks = 0, pore_blocks = 0, bnd_blocks = 0, n_pore = 0, air_blocks_pad = 0, pore_blocks_pad = 0;
+    int form = -1;     // launch form of the air and porous stages (kEnsForms), -1 until tuned
+    int form_pin = 0;  // lee_ensemble_set_launch: 0 autotune, 1 + form pinned
     std::vector<int> key;  // (J1, y1on) per member the tables hold
     int version = 0;
     void **rec_p = nullptr;
@@ -1823,6 +1827,63 @@ template <typename T> int ens_tables(lee_ensemble *e, const std::vector<int> &J1
     return LEE_OK;
 }
 
+// The air and porous launches of RK4 stage `st` in launch form `form`.
+template <typename T> void ens_stage_cells(lee_ensemble *e, int st, int form, hipStream_t s)
+{
+    const int n = (int)e->m.size();
+    const bool pad = form == 1;
+    const int *fa = e->first, *fp = e->first + n + 1;
+    if (pad) { fa = e->first + 2 * (n + 1) + e->n_pore + 1; fp = fa + n + 1; }
+    launch_rk_air_batch<T>((const RKArgs<T> *)e->air_args[st], fa, n, pad ? e->air_blocks_pad : e->air_blocks, e->wind,
+                           e->terrain, form, s);
+    if (e->n_pore)
+        launch_rk_pore_batch<T>((const RKPoreArgs<T> *)e->pore_args[st], fp, e->n_pore,
+                                pad ? e->pore_blocks_pad : e->pore_blocks, e->terrain, form, s);
+}
+
+// Tables current for the coming step (its gate states peeked, the counters left as they are).
+template <typename T> int ens_tables_peek(lee_ensemble *e)
+{
+    std::vector<int> J1(e->m.size()), y1on(e->m.size());
+    bool stale = false;
+    for (size_t d = 0; d < e->m.size(); ++d) {
+        lee_ctx *c = e->m[d];
+        const long long save = c->counter;
+        gate_tick(c, J1[d], y1on[d]);
+        c->counter = save;
+        stale |= e->key[2 * d] != J1[d] || e->key[2 * d + 1] != y1on[d];
+    }
+    return stale ? ens_tables<T>(e, J1, y1on) : LEE_OK;
+}
+
+// Autotuner of the launch form: times stage 0's air and porous launches of the coming step
+// in each form (stage 0 reads only level n, so repeating it changes nothing) and keeps the
+// fastest; every form gives the same bits.
+template <typename T> int ens_tune(lee_ensemble *e)
+{
+    if (e->form_pin > 0) {
+        e->form = e->form_pin - 1;
+        return LEE_OK;
+    }
+    int rc = ens_tables_peek<T>(e);
+    if (rc) return rc;
+    float best = 1e30f;
+    int pick = 0;
+    for (int f = 0; f < kEnsForms; ++f) {
+        ens_stage_cells<T>(e, 0, f, e->stream);  // warm-up
+        EHIP(e, hipEventRecord(e->ev0, e->stream));
+        for (int rep = 0; rep < 3; ++rep) ens_stage_cells<T>(e, 0, f, e->stream);
+        EHIP(e, hipEventRecord(e->ev1, e->stream));
+        EHIP(e, hipEventSynchronize(e->ev1));
+        float ms = 0.f;
+        EHIP(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        if (ms < best) { best = ms; pick = f; }
+    }
+    EHIP(e, hipGetLastError());
+    e->form = pick;
+    return LEE_OK;
+}
+
 // One ensemble RK4 step (launches only once the tables are current).
 template <typename T> int ens_step(lee_ensemble *e, double *rec_slot)
 {
@@ -1839,11 +1900,9 @@ template <typename T> int ens_step(lee_ensemble *e, double *rec_slot)
     }
     hipStream_t s = e->stream;
     launch_sample_batch<T>((const T *const *)e->rec_p, e->rec_idx, e->nrec, rec_slot, s);
-    const int *fa = e->first, *fp = e->first + n + 1, *fb = e->first + n + 1 + e->n_pore + 1;
+    const int *fb = e->first + n + 1 + e->n_pore + 1;
     for (int st = 0; st < 4; ++st) {
-        launch_rk_air_batch<T>((const RKArgs<T> *)e->air_args[st], fa, n, e->air_blocks, e->wind, e->terrain, s);
-        if (e->n_pore)
-            launch_rk_pore_batch<T>((const RKPoreArgs<T> *)e->pore_args[st], fp, e->n_pore, e->pore_blocks, e->terrain, s);
+        ens_stage_cells<T>(e, st, e->form, s);
         launch_rk_bnd_batch<T>((const RKBndArgs<T> *)e->bnd_args[st], fb, n, e->bnd_blocks, s);
     }
     for (lee_ctx *c : e->m) c->steps += 1;
@@ -1861,6 +1920,10 @@ bool ens_gate_constant(const lee_ensemble *e, int n)
 template <typename T> int ens_run(lee_ensemble *e, int nsteps, double *rec_out, int use_graph)
 {
     hipStream_t s = e->stream;
+    if (e->form < 0) {
+        int rc = ens_tune<T>(e);
+        if (rc) return rc;
+    }
     EHIP(e, hipEventRecord(e->ev0, s));
     int done = 0;
     while (done < nsteps) {
@@ -1868,20 +1931,9 @@ template <typename T> int ens_run(lee_ensemble *e, int nsteps, double *rec_out, 
         bool replayed = false;
         if (use_graph && ens_gate_constant(e, n)) {
             // the gate states hold for the whole chunk: bring the tables up to date before
-            // capturing (peek the coming step's state without advancing the counters)
-            std::vector<int> J1(e->m.size()), y1on(e->m.size());
-            bool stale = false;
-            for (size_t d = 0; d < e->m.size(); ++d) {
-                lee_ctx *c = e->m[d];
-                const long long save = c->counter;
-                gate_tick(c, J1[d], y1on[d]);
-                c->counter = save;
-                stale |= e->key[2 * d] != J1[d] || e->key[2 * d + 1] != y1on[d];
-            }
-            if (stale) {
-                int rc = ens_tables<T>(e, J1, y1on);
-                if (rc) return rc;
-            }
+            // capturing
+            int rc = ens_tables_peek<T>(e);
+            if (rc) return rc;
             hipGraphExec_t exec = nullptr;
             for (auto &g : e->graphs)
                 if (g.n == n && g.version == e->version) exec = g.exec;
@@ -1973,28 +2025,32 @@ int lee_ensemble_create(int n, const lee_deck *decks, const lee_options *opts, l
             (rc = ens_alloc(E, &E->bnd_args[st], n * sb)))
             return rc;
     }
-    // block prefixes: interior columns of the air and porous grids in 256s per plane,
-    // boundary nodes in 256s
-    std::vector<int> fa(1, 0), fp(1, 0), fb(1, 0);
+    // tile prefixes: interior columns of the air and porous grids in 256s per plane (and the
+    // same with each plane's tiles padded to a multiple of 8), boundary nodes in 256s
+    std::vector<int> fa(1, 0), fp(1, 0), fb(1, 0), fa8(1, 0), fp8(1, 0);
     for (lee_ctx *c : E->m) {
         const Model &m = c->m;
-        auto tiles = [](int nx, int ny, int nz) { return batch_blocks(nx, ny, nz); };
-        long long ta = tiles(m.nx, m.ny, m.nz);
+        long long ta = batch_blocks(m.nx, m.ny, m.nz), ta8 = batch_blocks(m.nx, m.ny, m.nz, true);
         long long nb = (c->prec == 64 ? boundary_count<double>(m.nx, m.ny, m.nz) : boundary_count<float>(m.nx, m.ny, m.nz));
         if (m.south_porous) {
-            fp.push_back((int)(fp.back() + tiles(m.pnx, m.pny, m.pnz)));
+            fp.push_back((int)(fp.back() + batch_blocks(m.pnx, m.pny, m.pnz)));
+            fp8.push_back((int)(fp8.back() + batch_blocks(m.pnx, m.pny, m.pnz, true)));
             nb += c->prec == 64 ? boundary_count<double>(m.pnx, m.pny, m.pnz) : boundary_count<float>(m.pnx, m.pny, m.pnz);
         }
-        if (fa.back() + ta > INT_MAX / 2 || fb.back() + (nb + 255) / 256 > INT_MAX / 2)
+        if (fa8.back() + ta8 > INT_MAX / 2 || fp8.back() > INT_MAX / 2 || fb.back() + (nb + 255) / 256 > INT_MAX / 2)
             return ens_err(E, LEE_ERR_ARG, "ensemble too large for one launch");
         fa.push_back((int)(fa.back() + ta));
+        fa8.push_back((int)(fa8.back() + ta8));
         fb.push_back((int)(fb.back() + (nb + 255) / 256));
     }
     E->air_blocks = fa.back(); E->pore_blocks = fp.back(); E->bnd_blocks = fb.back();
+    E->air_blocks_pad = fa8.back(); E->pore_blocks_pad = fp8.back();
     E->n_pore = (int)fp.size() - 1;
     std::vector<int> first(fa);
     first.insert(first.end(), fp.begin(), fp.end());
     first.insert(first.end(), fb.begin(), fb.end());
+    first.insert(first.end(), fa8.begin(), fa8.end());
+    first.insert(first.end(), fp8.begin(), fp8.end());
     if ((rc = ens_alloc(E, (void **)&E->first, first.size() * sizeof(int)))) return rc;
     EHIP(E, hipMemcpy(E->first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice));
     // receivers of all members, concatenated
@@ -2054,6 +2110,19 @@ int lee_ensemble_step(lee_ensemble *e, int nsteps, double *rec_out, int use_grap
 }
 
 double lee_ensemble_last_ms(const lee_ensemble *e) { return e ? (double)e->last_ms : 0.0; }
+
+int lee_ensemble_set_launch(lee_ensemble *e, int form)
+{
+    if (!e || form < 0 || form > kEnsForms) return ens_err(e, LEE_ERR_ARG, "launch form 0 (autotune) .. 3");
+    if (e->stream) EHIP(e, hipStreamSynchronize(e->stream));
+    for (auto &g : e->graphs) (void)hipGraphExecDestroy(g.exec);  // captured with the old form
+    e->graphs.clear();
+    e->form_pin = form;
+    e->form = -1;
+    return LEE_OK;
+}
+
+int lee_ensemble_launch(const lee_ensemble *e) { return e && e->form >= 0 ? e->form + 1 : 0; }
 
 const char *lee_ensemble_error(const lee_ensemble *e) { return e ? e->err.c_str() : g_global_err.c_str(); }
 

@@ -173,9 +173,20 @@ template <typename T> void launch_rk_bnd(const RKBndArgs<T> &r, const RKBndArgs<
 // one plane (the member's rows are 64-128 points, so 64-wide tiles would leave about a
 // quarter of the lanes idle): batch_blocks of them per member.
 constexpr int kBatchBlock = 256;
-inline long long batch_blocks(int nx, int ny, int nz)
+// Launch forms of the batched air and porous stages (same points, same arithmetic, identical
+// bits; the ensemble's autotuner keeps the fastest):
+//   0  one block per tile, member after member, plane after plane;
+//   1  as 0 with each member plane's tiles padded to a multiple of 8, so that a tile and the
+//      tiles above and below it run on one XCD under round-robin dispatch (padding tiles do
+//      nothing);
+//   2  persistent: as many blocks as the chip holds at once, each walking the tiles of form 0
+//      with a stride of the grid size.
+constexpr int kEnsForms = 3;
+inline long long batch_blocks(int nx, int ny, int nz, bool pad = false)
 {
-    return ((long long)nx * (ny - 2) + kBatchBlock - 1) / kBatchBlock * (nz - 2);
+    long long per = ((long long)nx * (ny - 2) + kBatchBlock - 1) / kBatchBlock;
+    if (pad) per = (per + 7) & ~7LL;
+    return per * (nz - 2);
 }
 // Whether node (i, j) of an interior row is an x-face node the batched air stage takes: its
 // interior neighbour (i + 1 or i - 1, the next or previous column) is in the same wavefront
@@ -187,10 +198,13 @@ __host__ __device__ inline bool batch_takes_face(int nx, int i, int j)
     const int c = (j - 1) * nx + i;
     return i == 0 ? (c & 63) != 63 : (c & 63) != 0;
 }
+// form: kEnsForms; first and nblocks count the tiles of that form (padded for form 1)
 template <typename T>
-void launch_rk_air_batch(const RKArgs<T> *args, const int *first, int n, int nblocks, int wind, bool terrain, hipStream_t s);
+void launch_rk_air_batch(const RKArgs<T> *args, const int *first, int n, int nblocks, int wind, bool terrain, int form,
+                         hipStream_t s);
 template <typename T>
-void launch_rk_pore_batch(const RKPoreArgs<T> *args, const int *first, int n, int nblocks, bool terrain, hipStream_t s);
+void launch_rk_pore_batch(const RKPoreArgs<T> *args, const int *first, int n, int nblocks, bool terrain, int form,
+                          hipStream_t s);
 template <typename T> void launch_rk_bnd_batch(const RKBndArgs<T> *args, const int *first, int n, int nblocks, hipStream_t s);
 template <typename T> void launch_sample_batch(const T *const *p, const long long *idx, int n, double *out, hipStream_t s);
 

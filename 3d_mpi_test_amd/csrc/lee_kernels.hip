@@ -1371,12 +1371,14 @@ __device__ __forceinline__ int batch_member(const int *first, int n, int b)
     return lo;
 }
 
-// Point of thread (threadIdx.x, threadIdx.y) in block `local` of a member: blocks of 256
-// consecutive columns c = i + nx*(j - 1) over the interior rows, plane after plane; false for
-// a face column (i = 0, nx - 1) or past the last row.
-__device__ __forceinline__ bool batch_point(int nx, int ny, int local, int &i, int &j, int &k)
+// Point of thread (threadIdx.x, threadIdx.y) in tile `local` of a member: tiles of 256
+// consecutive columns c = i + nx*(j - 1) over the interior rows, plane after plane (a plane's
+// tile count rounded up to a multiple of 8 when `pad`); false for a face column (i = 0,
+// nx - 1) or past the last row (every point of a padding tile).
+__device__ __forceinline__ bool batch_point(int nx, int ny, bool pad, int local, int &i, int &j, int &k)
 {
-    const int per = (nx * (ny - 2) + kBatchBlock - 1) / kBatchBlock;
+    int per = (nx * (ny - 2) + kBatchBlock - 1) / kBatchBlock;
+    if (pad) per = (per + 7) & ~7;
     const int c = (local % per) * kBatchBlock + threadIdx.x + BX * threadIdx.y;
     k = local / per + 1;
     j = c / nx + 1;
@@ -1389,15 +1391,26 @@ __device__ __forceinline__ bool batch_point(int nx, int ny, int local, int &i, i
 // (rk_air_point with XF): the face lane runs the same code as the others and takes the
 // neighbour's values across the wavefront.
 template <typename T, int WM, bool TERRAIN, bool COMP>
-__global__ __launch_bounds__(BX * BY) void k_rk_air_batch(const RKArgs<T> *__restrict__ args, const int *__restrict__ first,
-                                                         int n)
+__device__ __forceinline__ void rk_air_tile(const RKArgs<T> *__restrict__ args, const int *__restrict__ first, int n,
+                                            int tile, bool pad)
 {
-    const int d = batch_member(first, n, blockIdx.x);
+    const int d = batch_member(first, n, tile);
     const RKArgs<T> &r = args[d];
     const int nx = r.a.g.nx, ny = r.a.g.ny;
     int i, j, k;
-    const bool in = batch_point(nx, ny, blockIdx.x - first[d], i, j, k);
+    const bool in = batch_point(nx, ny, pad, tile - first[d], i, j, k);
     if (in || (j < ny - 1 && batch_takes_face(nx, i, j))) rk_air_point<T, WM, TERRAIN, COMP, 7, true>(r, i, j, k);
+}
+template <typename T, int WM, bool TERRAIN, bool COMP, int FORM>
+__global__ __launch_bounds__(BX * BY) void k_rk_air_batch(const RKArgs<T> *__restrict__ args, const int *__restrict__ first,
+                                                         int n, int ntiles)
+{
+    if constexpr (FORM == 2) {
+        // the tile index is uniform over the block, so every wave of it takes the same trips
+        for (int t = blockIdx.x; t < ntiles; t += gridDim.x) rk_air_tile<T, WM, TERRAIN, COMP>(args, first, n, t, false);
+    } else {
+        rk_air_tile<T, WM, TERRAIN, COMP>(args, first, n, blockIdx.x, FORM == 1);
+    }
 }
 
 // Porous layer: dt*R_p = -gama div u (porous.cpp:77-95); dt*R_u = -dt eff grad p, with the
@@ -1452,15 +1465,24 @@ __global__ __launch_bounds__(BX * BY) void k_rk_pore(const RKPoreArgs<T> r)
 }
 
 template <typename T, bool TERRAIN, bool COMP>
-__global__ __launch_bounds__(BX * BY) void k_rk_pore_batch(const RKPoreArgs<T> *__restrict__ args,
-                                                          const int *__restrict__ first, int n)
+__device__ __forceinline__ void rk_pore_tile(const RKPoreArgs<T> *__restrict__ args, const int *__restrict__ first, int n,
+                                             int tile, bool pad)
 {
-    const int d = batch_member(first, n, blockIdx.x);
+    const int d = batch_member(first, n, tile);
     const RKPoreArgs<T> &r = args[d];
     const int nx = r.g.nx, ny = r.g.ny;
     int i, j, k;
-    if (!batch_point(nx, ny, blockIdx.x - first[d], i, j, k)) return;
-    rk_pore_point<T, TERRAIN, COMP>(r, i, j, k);
+    if (batch_point(nx, ny, pad, tile - first[d], i, j, k)) rk_pore_point<T, TERRAIN, COMP>(r, i, j, k);
+}
+template <typename T, bool TERRAIN, bool COMP, int FORM>
+__global__ __launch_bounds__(BX * BY) void k_rk_pore_batch(const RKPoreArgs<T> *__restrict__ args,
+                                                          const int *__restrict__ first, int n, int ntiles)
+{
+    if constexpr (FORM == 2) {
+        for (int t = blockIdx.x; t < ntiles; t += gridDim.x) rk_pore_tile<T, TERRAIN, COMP>(args, first, n, t, false);
+    } else {
+        rk_pore_tile<T, TERRAIN, COMP>(args, first, n, blockIdx.x, FORM == 1);
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -1910,28 +1932,60 @@ void launch_rk_air_v(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool ter
 #undef LEE_RKT
 }
 
-template <typename T>
-void launch_rk_air_batch(const RKArgs<T> *args, const int *first, int n, int nblocks, int wind, bool terrain, hipStream_t s)
+// Blocks of `kernel` (256 threads) the chip holds at once: the grid of a persistent form.
+template <typename K> unsigned resident_blocks(K kernel)
 {
-    constexpr bool C = std::is_same<T, float>::value;
-    if (nblocks <= 0) return;
-    dim3 g((unsigned)nblocks), b(BX, BY);
-#define LEE_RK(WM, TER) hipLaunchKernelGGL((k_rk_air_batch<T, WM, TER, C>), g, b, 0, s, args, first, n)
-    if (terrain) {
-        if (wind == 1) LEE_RK(1, true); else if (wind == 2) LEE_RK(2, true); else LEE_RK(0, true);
-    } else {
-        if (wind == 1) LEE_RK(1, false); else if (wind == 2) LEE_RK(2, false); else LEE_RK(0, false);
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, BX * BY, 0) != hipSuccess)
+        return 0;
+    return (unsigned)std::max(1, cus * per);
+}
+// One launch of a batched kernel in form `form` over `nblocks` tiles: a block per tile, or
+// (form 2) the resident grid walking them.
+template <typename K, typename... A>
+void launch_form(K kernel, int form, int nblocks, hipStream_t s, A... args)
+{
+    unsigned g = (unsigned)nblocks;
+    if (form == 2) {
+        static const unsigned cap = resident_blocks(kernel);  // one per instantiation (one device)
+        if (cap) g = std::min(g, cap);
     }
-#undef LEE_RK
+    hipLaunchKernelGGL(kernel, dim3(g), dim3(BX, BY), 0, s, args...);
 }
 template <typename T>
-void launch_rk_pore_batch(const RKPoreArgs<T> *args, const int *first, int n, int nblocks, bool terrain, hipStream_t s)
+void launch_rk_air_batch(const RKArgs<T> *args, const int *first, int n, int nblocks, int wind, bool terrain, int form,
+                         hipStream_t s)
 {
     constexpr bool C = std::is_same<T, float>::value;
     if (nblocks <= 0) return;
-    dim3 g((unsigned)nblocks), b(BX, BY);
-    if (terrain) hipLaunchKernelGGL((k_rk_pore_batch<T, true, C>), g, b, 0, s, args, first, n);
-    else hipLaunchKernelGGL((k_rk_pore_batch<T, false, C>), g, b, 0, s, args, first, n);
+#define LEE_RKF(WM, TER)                                                                                       \
+    do {                                                                                                     \
+        if (form == 2) launch_form(k_rk_air_batch<T, WM, TER, C, 2>, 2, nblocks, s, args, first, n, nblocks);    \
+        else if (form == 1) launch_form(k_rk_air_batch<T, WM, TER, C, 1>, 1, nblocks, s, args, first, n, nblocks); \
+        else launch_form(k_rk_air_batch<T, WM, TER, C, 0>, 0, nblocks, s, args, first, n, nblocks);             \
+    } while (0)
+    if (terrain) {
+        if (wind == 1) LEE_RKF(1, true); else if (wind == 2) LEE_RKF(2, true); else LEE_RKF(0, true);
+    } else {
+        if (wind == 1) LEE_RKF(1, false); else if (wind == 2) LEE_RKF(2, false); else LEE_RKF(0, false);
+    }
+#undef LEE_RKF
+}
+template <typename T>
+void launch_rk_pore_batch(const RKPoreArgs<T> *args, const int *first, int n, int nblocks, bool terrain, int form,
+                          hipStream_t s)
+{
+    constexpr bool C = std::is_same<T, float>::value;
+    if (nblocks <= 0) return;
+#define LEE_RPF(TER)                                                                                           \
+    do {                                                                                                     \
+        if (form == 2) launch_form(k_rk_pore_batch<T, TER, C, 2>, 2, nblocks, s, args, first, n, nblocks);       \
+        else if (form == 1) launch_form(k_rk_pore_batch<T, TER, C, 1>, 1, nblocks, s, args, first, n, nblocks);  \
+        else launch_form(k_rk_pore_batch<T, TER, C, 0>, 0, nblocks, s, args, first, n, nblocks);                \
+    } while (0)
+    if (terrain) LEE_RPF(true); else LEE_RPF(false);
+#undef LEE_RPF
 }
 template <typename T> void launch_sample_batch(const T *const *p, const long long *idx, int n, double *out, hipStream_t s)
 {
@@ -2035,11 +2089,11 @@ void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s)
     template bool fused_fits<T>(int);                                                                        \
     template void launch_rk_air<T>(const RKArgs<T> &, const RKArgs<T> *, int, bool, hipStream_t);            \
     template void launch_rk_pore<T>(const RKPoreArgs<T> &, bool, hipStream_t);                               \
-    template void launch_rk_air_batch<T>(const RKArgs<T> *, const int *, int, int, int, bool, hipStream_t);   \
+    template void launch_rk_air_batch<T>(const RKArgs<T> *, const int *, int, int, int, bool, int, hipStream_t); \
     template void launch_rk_air_v<T>(const RKArgs<T> &, const RKArgs<T> *, int, bool, int, hipStream_t);     \
     template bool rk_stage_takes_xfaces<T>(const RKArgs<T> &, int);                                           \
     template bool fb_passes_take_xfaces<T>(const AirArgs<T> &);                                              \
-    template void launch_rk_pore_batch<T>(const RKPoreArgs<T> *, const int *, int, int, bool, hipStream_t);   \
+    template void launch_rk_pore_batch<T>(const RKPoreArgs<T> *, const int *, int, int, bool, int, hipStream_t); \
     template void launch_sample_batch<T>(const T *const *, const long long *, int, double *, hipStream_t);     \
     template void launch_air_fb<T>(const AirArgs<T> &, bool, hipStream_t);                                   \
     template void launch_pore_p<T>(const PoreArgs<T> &, hipStream_t);                                        \

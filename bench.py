@@ -57,6 +57,8 @@ RK_TILES = ["64x4 planes", "64x4 xcd-columns", "64x8 xcd-columns", "64x8 planes"
 
 
 RK_TILE = 0  # --rk-tile: RK4 air-stage launch shape, 0 = the autotuner's choice
+# launch forms of the ensemble's air and porous stages (lee_ensemble_set_launch), index = form
+ENS_FORMS = [None, "a block per 256-column tile", "tiles padded per plane to multiples of 8", "persistent grid"]
 
 
 def run_gpu(lee, deck, receivers, precision, steps, warmup, record=True):
@@ -209,7 +211,7 @@ def run_c3(lee, a):
                 if stats[i].name.decode() in ("rk_air", "rk_pore", "rk_bnd"):
                     bytes_step += 4 * stats[i].bytes_per_launch
         series = [np.concatenate([ser[0][d], ser[1][d]]) for d in range(len(decks))]
-        out[p] = dict(points=npts, wall=wall, bytes_step=bytes_step, series=series)
+        out[p] = dict(points=npts, wall=wall, bytes_step=bytes_step, series=series, launch=ENS_FORMS[e.launch()])
         print(f"[bench] C3 fp{p}: {npts * a.steps / wall / 1e9:.2f} G pt-updates/s ({1e3 * wall / a.steps:.3f} ms/step)",
               file=sys.stderr, flush=True)
         e.close()
@@ -240,7 +242,8 @@ def run_c3(lee, a):
                                "(sigma log-U 10-2000 kPa s/m^2, porosity U[0.2,0.5], tortuosity U[1.5,4]), source "
                                "U[0.3,2] m, log wind u* U[0,0.5] m/s, 10-pt PML sides+top, RK4 CFL 0.5, one launch per "
                                "kernel kind per stage, graph-replayed",
-                   "points": r["points"], "domains": len(decks), "parallelism": "single GPU"},
+                   "points": r["points"], "domains": len(decks), "parallelism": "single GPU",
+                   "launch_form": r["launch"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "whole RK4 step (rk_air + rk_pore + rk_bnd, all members)",

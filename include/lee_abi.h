@@ -298,6 +298,15 @@ int  lee_ensemble_receivers(const lee_ensemble *e, int *offsets, int cap);
 int  lee_ensemble_step(lee_ensemble *e, int nsteps, double *rec_out, int use_graph);
 double lee_ensemble_last_ms(const lee_ensemble *e);
 const char *lee_ensemble_error(const lee_ensemble *e);
+/* Launch form of the members' air and porous stages (no counterpart in the reference):
+ * 0 = autotune at the next step (the default: each form timed on the coming step's first
+ * stage, the fastest kept), 1 = a block per 256-column tile, 2 = the same with each member
+ * plane's tiles padded to a multiple of 8 (a tile and its vertical neighbours on one XCD),
+ * 3 = persistent (the resident grid walks the tiles).  All forms give identical bits.
+ * Drops the captured graphs.  lee_ensemble_launch returns the form in use (1..3), 0 before
+ * the first step. */
+int  lee_ensemble_set_launch(lee_ensemble *e, int form);
+int  lee_ensemble_launch(const lee_ensemble *e);
 
 /* Learned ground model (no counterpart in the reference): an MLP 4 -> 64 -> 64 -> 64 -> 8 with
  * GELU, from normalised ground parameters (flow resistivity, porosity, tortuosity, layer

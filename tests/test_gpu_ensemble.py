@@ -69,3 +69,42 @@ def test_ensemble_members_match_single_runs(lee, orc, precision, graph):
             print(f"  member {d}: vs oracle {err:.2e}", flush=True)
             assert err <= 1e-10
     e.close()
+
+
+@pytest.mark.parametrize("form", [1, 2, 3])
+def test_ensemble_launch_forms_match_single_runs(lee, form):
+    """Every launch form of the air and porous stages (a block per tile, tiles padded per
+    plane, persistent grid) steps each member bit for bit as the deck run alone."""
+    ds = members(lee)
+    recs = [receivers(d) for d in ds]
+    for precision, graph in [(64, False), (32, True)]:
+        print(f"[gpu ensemble] launch form {form} fp{precision} graph={graph}", flush=True)
+        e = lee.Ensemble(ds, precision=precision, receivers=recs, use_graph=graph, launch=form)
+        assert e.launch() == 0
+        got = [np.concatenate(parts) for parts in zip(*[e.step(n) for n in (5, 24)])]
+        assert e.launch() == form
+        for d, dk in enumerate(ds):
+            s = lee.Solver(dk, precision=precision, receivers=recs[d], integrator=1)
+            s.frame_setup(0)
+            assert np.array_equal(got[d], s.step(29)), d
+            for a, b in zip(e.members[d].state(), s.state()):
+                assert np.array_equal(a, b), d
+            if dk.has_pore:
+                for a, b in zip(e.members[d].state(1), s.state(1)):
+                    assert np.array_equal(a, b), d
+            s.close()
+        e.close()
+
+
+def test_ensemble_autotuner_keeps_a_form(lee):
+    ds = members(lee)
+    e = lee.Ensemble(ds, precision=32, receivers=[receivers(d) for d in ds], use_graph=True)
+    e.step(3)
+    assert e.launch() in (1, 2, 3)
+    with pytest.raises(lee.LeeError):
+        e.set_launch(4)
+    e.set_launch(2)
+    assert e.launch() == 0
+    e.step(2)
+    assert e.launch() == 2
+    e.close()
