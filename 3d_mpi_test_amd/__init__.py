@@ -106,6 +106,8 @@ def lib():
         L.lee_profile.argtypes = [vp, ctypes.c_int]
         L.lee_kernel_stats.argtypes = [vp, ctypes.POINTER(LeeKstat), ctypes.c_int]
         L.lee_copy_bench.argtypes = [vp, ctypes.c_longlong, ctypes.c_int]
+        L.lee_stream_bench.argtypes = [vp, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.lee_stream_bench.restype = ctypes.c_double
         L.lee_save_restart.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
         L.lee_load_restart.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
         L.lee_write_contour.argtypes = [vp, ctypes.c_char_p]

@@ -229,6 +229,11 @@ void launch_shift_yslab(T *dst, const T *src, int nx, int n, int nz, int lo, int
 template <typename T> void launch_copy_plane_j(T *p, int nx, int ny, int nz, hipStream_t s);
 template <typename T> void launch_sample(const T *p, const long long *idx, int n, double *out, hipStream_t s);
 void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s);
+// Many-stream reference kernel: reads nr and writes nw arrays of n words, one word of each per
+// lane; (nr, nw) one of (1,1) (4,4) (8,4) (12,8) (20,8) (28,8), else nothing is launched and
+// the result is false.
+constexpr int kStreamMaxR = 28, kStreamMaxW = 8;
+bool launch_stream_words(float *const *out, int nw, const float *const *in, int nr, size_t n, hipStream_t s);
 template <typename T> void launch_finite_check(const T *f, size_t n, int *flag, hipStream_t s);
 template <typename T>
 void launch_wind_log(T *V, int nx, int ny, int nz, const double *h, const double *az, const double *tk, double ustar,

@@ -1671,6 +1671,27 @@ __global__ __launch_bounds__(256) void k_copy_words(v4f *dst, const v4f *src, si
     if (t < n4) __builtin_nontemporal_store(__builtin_nontemporal_load(src + t), dst + t);
 }
 
+// Many-stream reference: every lane reads one 4-byte word of each of NR arrays and writes one
+// of each of NW arrays (sums of its reads), all reads in flight before the first store, as the
+// stencil stages do with their fields: the bandwidth a kernel with that many streams can reach.
+struct StreamPtrs { const float *in[kStreamMaxR]; float *out[kStreamMaxW]; };
+template <int NR, int NW>
+__global__ __launch_bounds__(256) void k_stream_words(StreamPtrs p, size_t n)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    float a[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) a[r] = p.in[r][t];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        float s = 0.f;
+#pragma unroll
+        for (int r = w; r < NR; r += NW) s += a[r];
+        p.out[w][t] = s;
+    }
+}
+
 // Interior tiles: x tiles start at i = 0 so that a tile's rows cover whole 128-byte lines (the
 // face threads i = 0 and i = nx - 1 return at once); y and z start at the first interior plane.
 dim3 interior_grid(int nx, int ny, int nz) { return dim3((nx + BX - 1) / BX, (ny - 2 + BY - 1) / BY, nz - 2); }
@@ -2075,6 +2096,19 @@ void launch_add_source(T *p, int nx, int ny, int nz, const double *X, const doub
 {
     hipLaunchKernelGGL((k_add_source<T>), dim3(flat_blocks((size_t)nx * ny * nz)), dim3(256), 0, s, p, nx, ny, nz, X, Y,
                        Z, sx, sy, sz, gc, amp);
+}
+
+bool launch_stream_words(float *const *out, int nw, const float *const *in, int nr, size_t n, hipStream_t s)
+{
+    StreamPtrs p{};
+    if (nr < 1 || nr > kStreamMaxR || nw < 1 || nw > kStreamMaxW) return false;
+    for (int r = 0; r < nr; ++r) p.in[r] = in[r];
+    for (int w = 0; w < nw; ++w) p.out[w] = out[w];
+    const dim3 g((unsigned)((n + 255) / 256)), b(256);
+#define LEE_SW(R, W) if (nr == R && nw == W) { hipLaunchKernelGGL((k_stream_words<R, W>), g, b, 0, s, p, n); return true; }
+    LEE_SW(1, 1) LEE_SW(4, 4) LEE_SW(8, 4) LEE_SW(12, 8) LEE_SW(20, 8) LEE_SW(28, 8)
+#undef LEE_SW
+    return false;
 }
 
 void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s)

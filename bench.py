@@ -57,6 +57,7 @@ RK_TILES = ["64x4 planes", "64x4 xcd-columns", "64x8 xcd-columns", "64x8 planes"
 
 
 RK_TILE = 0  # --rk-tile: RK4 air-stage launch shape, 0 = the autotuner's choice
+STREAMS = (12, 8)  # arrays read and written by the many-stream reference kernel (lee_stream_bench)
 # launch forms of the ensemble's air and porous stages (lee_ensemble_set_launch), index = form
 ENS_FORMS = [None, "a block per 256-column tile", "tiles padded per plane to multiples of 8", "persistent grid"]
 
@@ -101,8 +102,11 @@ def run_gpu(lee, deck, receivers, precision, steps, warmup, record=True):
                             bytes_per_launch=k.bytes_per_launch, points=k.points_per_launch))
     tile = s.info().rk_tile
     copy_ms = lee.lib().lee_copy_bench(s.h, 1 << 28, 10)  # 1 GiB read + 1 GiB write
+    # a kernel with as many streams as an RK4 stage's average (12 read, 8 written, 64 MiB each)
+    stream_ms = lee.lib().lee_stream_bench(s.h, 1 << 24, STREAMS[0], STREAMS[1], 10)
     rec = np.concatenate([x for x in series if x is not None], axis=0) if record else None
-    info = dict(points=npts, wall=t1 - t0, kernels=kst, copy_ms=copy_ms, frames=state["mf"] + 1, series=rec,
+    info = dict(points=npts, wall=t1 - t0, kernels=kst, copy_ms=copy_ms, stream_ms=stream_ms, frames=state["mf"] + 1,
+                series=rec,
                 rk_tile=RK_TILES[tile - 1] if tile > 0 else None)
     s.close()
     return info
@@ -444,6 +448,10 @@ def main():
             "kernel": dom["name"], "launch_shape": r["rk_tile"],
             "bytes_per_launch": dom["bytes_per_launch"], "avg_launch_ms": round(avg_s * 1e3, 4),
             "copy_kernel_GBs": round(2 * (1 << 30) / (r["copy_ms"] / 1e3) / 1e9, 1) if r["copy_ms"] > 0 else None}
+    if r["stream_ms"] > 0:
+        sgb = sum(STREAMS) * (1 << 26) / (r["stream_ms"] / 1e3) / 1e9
+        roof["stream_kernel"] = {"reads": STREAMS[0], "writes": STREAMS[1], "GB/s": round(sgb, 1),
+                                 "achieved_over_it": round(achieved / sgb, 4)}
     acc = None
     if 32 in res and 64 in res:
         full = {"c1": 10000, "c2": 2000}[a.workload] if a.accuracy_steps is None else a.accuracy_steps

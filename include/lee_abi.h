@@ -236,6 +236,12 @@ int  lee_kernel_stats(const lee_ctx *ctx, lee_kstat *out, int cap);
  * HIP events (returns ms).  No reference counterpart: calibrates HBM counters and the
  * copy roofline for bench.py. */
 double lee_copy_bench(lee_ctx *ctx, long long n_words, int repeats);
+/* Many-stream reference (no counterpart in the reference): a kernel that reads `reads` and
+ * writes `writes` arrays of n_words 4-byte words, one word of each per lane, all reads in
+ * flight before its stores -- the bandwidth a stencil stage with that many field streams can
+ * reach.  (reads, writes) one of (1,1) (4,4) (8,4) (12,8) (20,8) (28,8).  Returns ms per
+ * launch (mean over `repeats` after a warm-up), -1 on error.  Allocates and frees its arrays. */
+double lee_stream_bench(lee_ctx *ctx, long long n_words, int reads, int writes, int repeats);
 
 /* Checkpoint in the reference's restart text format (airpore::Save_restart airpore.cpp:664-700,
  * airpore::save_restartfile 783-796, calculation_2D::save_restart_cal calculation_2D.cpp:497-520,
