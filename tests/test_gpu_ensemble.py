@@ -108,3 +108,60 @@ def test_ensemble_autotuner_keeps_a_form(lee):
     e.step(2)
     assert e.launch() == 2
     e.close()
+
+
+def kind_members(lee, kind):
+    """Members sharing a grid and wind kind other than the flat log-wind set above: over
+    terrain (log wind), with U V W wind fields (method 5), and in calm air."""
+    B = lee.deck.box
+    out = []
+    if kind == "terrain":
+        d = B(22, 26, 18, 0.05, pml=4, bc=(2, 3, 2, 2, 2, 2), porous_cells=6, src=(0.0, 0.5, 0.35), gauss_width=0.2,
+              time_domain=30, velocity_method=7)
+        d.hills, d.wind_ustar, d.wind_z0 = [(0.1, 0.6, 0.15, 0.3)], 0.3, 0.02
+        out.append(d)
+        d = B(30, 20, 16, 0.05, pml=4, bc=(2, 1, 2, 2, 2, 2), src=(0.1, 0.4, 0.3), gauss_width=0.25,
+              time_domain=30, velocity_method=7)
+        d.hills, d.wind_ustar, d.wind_z0 = [(-0.2, 0.5, 0.1, 0.25), (0.3, 0.8, 0.08, 0.2)], 0.5, 0.05
+        out.append(d)
+    elif kind == "uvw":
+        out.append(B(24, 28, 18, 0.05, pml=4, bc=(2, 3, 2, 2, 2, 2), porous_cells=6, src=(0.0, 0.6, 0.4),
+                     gauss_width=0.25, time_domain=30, velocity_method=5, velocity_coef=3.0))
+        out.append(B(18, 22, 16, 0.05, pml=4, bc=(2, 1, 2, 2, 2, 2), src=(0.05, 0.5, 0.3), gauss_width=0.2,
+                     time_domain=30, velocity_method=5, velocity_coef=2.0))
+    else:
+        out.append(B(20, 24, 18, 0.05, pml=4, bc=(2, 3, 2, 2, 2, 2), porous_cells=7, src=(0.0, 0.5, 0.4),
+                     gauss_width=0.25, time_domain=30))
+        out.append(B(26, 18, 16, 0.05, pml=4, bc=(2, 1, 2, 2, 2, 2), src=(0.1, 0.4, 0.3), gauss_width=0.2,
+                     time_domain=30))
+    for d in out:
+        d.integrator = 1
+    return out
+
+
+@pytest.mark.parametrize("kind", ["terrain", "uvw", "calm"])
+@pytest.mark.parametrize("form", [0, 3])
+def test_ensemble_grid_and_wind_kinds_match_single_runs(lee, orc, kind, form):
+    """The batched kernels of the other grid and wind kinds (terrain, U V W wind, no wind),
+    autotuned and persistent: each member bit for bit as the deck run alone, FP64 members
+    within 1e-10 of the oracle."""
+    ds = kind_members(lee, kind)
+    recs = [receivers(d) for d in ds]
+    for precision, graph in [(64, False), (32, True)]:
+        print(f"[gpu ensemble] {kind} launch {form} fp{precision} graph={graph}", flush=True)
+        e = lee.Ensemble(ds, precision=precision, receivers=recs, use_graph=graph, launch=form)
+        got = [np.concatenate(parts) for parts in zip(*[e.step(n) for n in (4, 21)])]
+        for d, dk in enumerate(ds):
+            s = lee.Solver(dk, precision=precision, receivers=recs[d], integrator=1)
+            s.frame_setup(0)
+            assert np.array_equal(got[d], s.step(25)), (kind, d)
+            for a, b in zip(e.members[d].state(), s.state()):
+                assert np.array_equal(a, b), (kind, d)
+            s.close()
+            if precision == 64:
+                o = orc.Oracle(dk, receivers=recs[d], integrator=1)
+                o.frame_setup(0)
+                err = rel(got[d], o.step(25))
+                print(f"  member {d}: vs oracle {err:.2e}", flush=True)
+                assert err <= 1e-10
+        e.close()
