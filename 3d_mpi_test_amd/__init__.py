@@ -126,7 +126,7 @@ def lib():
         L.lee_ensemble_error.argtypes = [vp]
         L.lee_ensemble_error.restype = ctypes.c_char_p
         L.lee_ensemble_set_launch.argtypes = [vp, ctypes.c_int]
-        L.lee_ensemble_launch.argtypes = [vp]
+        L.lee_ensemble_launch.argtypes = [vp, ctypes.c_int]
         L.lee_read_source_series.argtypes = [ctypes.c_char_p, _dp, ctypes.c_longlong]
         L.lee_read_source_series.restype = ctypes.c_longlong
         L.lee_copy_bench.restype = ctypes.c_double
@@ -193,9 +193,10 @@ class Ensemble:
         tile, 2 tiles padded per plane to multiples of 8, 3 persistent."""
         self._check(lib().lee_ensemble_set_launch(self.h, form))
 
-    def launch(self) -> int:
-        """The launch form in use (1..3), 0 before the first step."""
-        return lib().lee_ensemble_launch(self.h)
+    def launch(self, porous: bool = False) -> int:
+        """The launch form in use (1..3) of the air stage, or of the porous stage; 0 before
+        the first step."""
+        return lib().lee_ensemble_launch(self.h, 1 if porous else 0)
 
     def points(self) -> int:
         return sum(m.nx * m.ny * m.nz + m.pnx * m.pny * m.pnz for m in self.members)

@@ -1784,8 +1784,9 @@ struct lee_ensemble {
     // each member plane's tiles padded to a multiple of 8 (launch form 1)
     int *first = nullptr;
     int air_blocks = 0, pore_blocks = 0, bnd_blocks = 0, n_pore = 0, air_blocks_pad = 0, pore_blocks_pad = 0;
-    int form = -1;     // launch form of the air and porous stages (kEnsForms), -1 until tuned
-    int form_pin = 0;  // lee_ensemble_set_launch: 0 autotune, 1 + form pinned
+    int form = -1;       // launch form of the air stage (kEnsForms), -1 until tuned
+    int form_pore = -1;  // and of the porous stage (tuned on its own)
+    int form_pin = 0;    // lee_ensemble_set_launch: 0 autotune, 1 + form pinned for both
     std::vector<int> key;  // (J1, y1on) per member the tables hold
     int version = 0;
     void **rec_p = nullptr;
@@ -1859,18 +1860,18 @@ template <typename T> int ens_tables(lee_ensemble *e, const std::vector<int> &J1
     return LEE_OK;
 }
 
-// The air and porous launches of RK4 stage `st` in launch form `form`.
-template <typename T> void ens_stage_cells(lee_ensemble *e, int st, int form, hipStream_t s)
+// The air launch (form fa, -1 none) and the porous launch (form fp, -1 none) of RK4 stage st.
+template <typename T> void ens_stage_cells(lee_ensemble *e, int st, int fa, int fp, hipStream_t s)
 {
     const int n = (int)e->m.size();
-    const bool pad = form == 1;
-    const int *fa = e->first, *fp = e->first + n + 1;
-    if (pad) { fa = e->first + 2 * (n + 1) + e->n_pore + 1; fp = fa + n + 1; }
-    launch_rk_air_batch<T>((const RKArgs<T> *)e->air_args[st], fa, n, pad ? e->air_blocks_pad : e->air_blocks, e->wind,
-                           e->terrain, form, s);
-    if (e->n_pore)
-        launch_rk_pore_batch<T>((const RKPoreArgs<T> *)e->pore_args[st], fp, e->n_pore,
-                                pad ? e->pore_blocks_pad : e->pore_blocks, e->terrain, form, s);
+    const int *pa = e->first + (fa == 1 ? 2 * (n + 1) + e->n_pore + 1 : 0);
+    const int *pp = e->first + (fp == 1 ? 3 * (n + 1) + e->n_pore + 1 : n + 1);
+    if (fa >= 0)
+        launch_rk_air_batch<T>((const RKArgs<T> *)e->air_args[st], pa, n, fa == 1 ? e->air_blocks_pad : e->air_blocks,
+                               e->wind, e->terrain, fa, s);
+    if (fp >= 0 && e->n_pore)
+        launch_rk_pore_batch<T>((const RKPoreArgs<T> *)e->pore_args[st], pp, e->n_pore,
+                                fp == 1 ? e->pore_blocks_pad : e->pore_blocks, e->terrain, fp, s);
 }
 
 // Tables current for the coming step (its gate states peeked, the counters left as they are).
@@ -1888,35 +1889,39 @@ template <typename T> int ens_tables_peek(lee_ensemble *e)
     return stale ? ens_tables<T>(e, J1, y1on) : LEE_OK;
 }
 
-// Autotuner of the launch form: times stage 0's air and porous launches of the coming step
-// in each form (stage 0 reads only level n, so repeating it changes nothing) and keeps the
-// fastest; every form gives the same bits.
+// Autotuner of the launch forms: times stage 0's air launch, then its porous launch, of the
+// coming step in each form (stage 0 reads only level n, so repeating it changes nothing) and
+// keeps the fastest for each kind; every form gives the same bits.
 template <typename T> int ens_tune(lee_ensemble *e)
 {
     if (e->form_pin > 0) {
-        e->form = e->form_pin - 1;
+        e->form = e->form_pore = e->form_pin - 1;
         return LEE_OK;
     }
     int rc = ens_tables_peek<T>(e);
     if (rc) return rc;
-    float t[kEnsForms];
-    for (int f = 0; f < kEnsForms; ++f) t[f] = 1e30f;
-    for (int pass = 0; pass < 2; ++pass)  // each form's best of two passes, as tune_rk
-        for (int f = 0; f < kEnsForms; ++f) {
-            ens_stage_cells<T>(e, 0, f, e->stream);  // warm-up
-            EHIP(e, hipEventRecord(e->ev0, e->stream));
-            for (int rep = 0; rep < 4; ++rep) ens_stage_cells<T>(e, 0, f, e->stream);
-            EHIP(e, hipEventRecord(e->ev1, e->stream));
-            EHIP(e, hipEventSynchronize(e->ev1));
-            float ms = 0.f;
-            EHIP(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
-            t[f] = std::min(t[f], ms);
-        }
+    int pick[2] = {0, 0};
+    for (int kind = 0; kind < (e->n_pore ? 2 : 1); ++kind) {
+        float t[kEnsForms];
+        for (int f = 0; f < kEnsForms; ++f) t[f] = 1e30f;
+        for (int pass = 0; pass < 2; ++pass)  // each form's best of two passes, as tune_rk
+            for (int f = 0; f < kEnsForms; ++f) {
+                const int fa = kind == 0 ? f : -1, fp = kind == 1 ? f : -1;
+                ens_stage_cells<T>(e, 0, fa, fp, e->stream);  // warm-up
+                EHIP(e, hipEventRecord(e->ev0, e->stream));
+                for (int rep = 0; rep < 4; ++rep) ens_stage_cells<T>(e, 0, fa, fp, e->stream);
+                EHIP(e, hipEventRecord(e->ev1, e->stream));
+                EHIP(e, hipEventSynchronize(e->ev1));
+                float ms = 0.f;
+                EHIP(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
+                t[f] = std::min(t[f], ms);
+            }
+        for (int f = 1; f < kEnsForms; ++f)
+            if (t[f] < t[pick[kind]]) pick[kind] = f;
+    }
     EHIP(e, hipGetLastError());
-    int pick = 0;
-    for (int f = 1; f < kEnsForms; ++f)
-        if (t[f] < t[pick]) pick = f;
-    e->form = pick;
+    e->form = pick[0];
+    e->form_pore = pick[1];
     return LEE_OK;
 }
 
@@ -1938,7 +1943,7 @@ template <typename T> int ens_step(lee_ensemble *e, double *rec_slot)
     launch_sample_batch<T>((const T *const *)e->rec_p, e->rec_idx, e->nrec, rec_slot, s);
     const int *fb = e->first + n + 1 + e->n_pore + 1;
     for (int st = 0; st < 4; ++st) {
-        ens_stage_cells<T>(e, st, e->form, s);
+        ens_stage_cells<T>(e, st, e->form, e->form_pore, s);
         launch_rk_bnd_batch<T>((const RKBndArgs<T> *)e->bnd_args[st], fb, n, e->bnd_blocks, s);
     }
     for (lee_ctx *c : e->m) c->steps += 1;
@@ -2154,11 +2159,16 @@ int lee_ensemble_set_launch(lee_ensemble *e, int form)
     for (auto &g : e->graphs) (void)hipGraphExecDestroy(g.exec);  // captured with the old form
     e->graphs.clear();
     e->form_pin = form;
-    e->form = -1;
+    e->form = e->form_pore = -1;
     return LEE_OK;
 }
 
-int lee_ensemble_launch(const lee_ensemble *e) { return e && e->form >= 0 ? e->form + 1 : 0; }
+int lee_ensemble_launch(const lee_ensemble *e, int porous)
+{
+    if (!e) return 0;
+    const int f = porous ? e->form_pore : e->form;
+    return f >= 0 ? f + 1 : 0;
+}
 
 const char *lee_ensemble_error(const lee_ensemble *e) { return e ? e->err.c_str() : g_global_err.c_str(); }
 

@@ -215,7 +215,8 @@ def run_c3(lee, a):
                 if stats[i].name.decode() in ("rk_air", "rk_pore", "rk_bnd"):
                     bytes_step += 4 * stats[i].bytes_per_launch
         series = [np.concatenate([ser[0][d], ser[1][d]]) for d in range(len(decks))]
-        out[p] = dict(points=npts, wall=wall, bytes_step=bytes_step, series=series, launch=ENS_FORMS[e.launch()])
+        out[p] = dict(points=npts, wall=wall, bytes_step=bytes_step, series=series,
+                      launch={"air": ENS_FORMS[e.launch()], "porous": ENS_FORMS[e.launch(porous=True)]})
         print(f"[bench] C3 fp{p}: {npts * a.steps / wall / 1e9:.2f} G pt-updates/s ({1e3 * wall / a.steps:.3f} ms/step)",
               file=sys.stderr, flush=True)
         e.close()

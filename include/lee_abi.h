@@ -309,10 +309,12 @@ const char *lee_ensemble_error(const lee_ensemble *e);
  * stage, the fastest kept), 1 = a block per 256-column tile, 2 = the same with each member
  * plane's tiles padded to a multiple of 8 (a tile and its vertical neighbours on one XCD),
  * 3 = persistent (the resident grid walks the tiles).  All forms give identical bits.
- * Drops the captured graphs.  lee_ensemble_launch returns the form in use (1..3), 0 before
- * the first step. */
+ * The autotuner picks the air stage's and the porous stage's forms separately; a pinned form
+ * applies to both.  Drops the captured graphs.  lee_ensemble_launch returns the form in use
+ * (1..3) of the air stage (porous = 0) or of the porous stage (porous = 1), 0 before the
+ * first step. */
 int  lee_ensemble_set_launch(lee_ensemble *e, int form);
-int  lee_ensemble_launch(const lee_ensemble *e);
+int  lee_ensemble_launch(const lee_ensemble *e, int porous);
 
 /* Learned ground model (no counterpart in the reference): an MLP 4 -> 64 -> 64 -> 64 -> 8 with
  * GELU, from normalised ground parameters (flow resistivity, porosity, tortuosity, layer

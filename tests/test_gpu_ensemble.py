@@ -82,7 +82,7 @@ def test_ensemble_launch_forms_match_single_runs(lee, form):
         e = lee.Ensemble(ds, precision=precision, receivers=recs, use_graph=graph, launch=form)
         assert e.launch() == 0
         got = [np.concatenate(parts) for parts in zip(*[e.step(n) for n in (5, 24)])]
-        assert e.launch() == form
+        assert e.launch() == form and e.launch(porous=True) == form
         for d, dk in enumerate(ds):
             s = lee.Solver(dk, precision=precision, receivers=recs[d], integrator=1)
             s.frame_setup(0)
@@ -100,13 +100,13 @@ def test_ensemble_autotuner_keeps_a_form(lee):
     ds = members(lee)
     e = lee.Ensemble(ds, precision=32, receivers=[receivers(d) for d in ds], use_graph=True)
     e.step(3)
-    assert e.launch() in (1, 2, 3)
+    assert e.launch() in (1, 2, 3) and e.launch(porous=True) in (1, 2, 3)
     with pytest.raises(lee.LeeError):
         e.set_launch(4)
     e.set_launch(2)
     assert e.launch() == 0
     e.step(2)
-    assert e.launch() == 2
+    assert e.launch() == 2 and e.launch(porous=True) == 2
     e.close()
 
 
