@@ -847,9 +847,9 @@ template <typename T> int graph_steps(lee_ctx *c, int n)
     return LEE_OK;
 }
 
-// Autotuner of the RK4 air stage: times each launch shape on stage 0 of the coming step (a
-// stage-0 launch reads only level n and writes the stage registers, so repeating it changes
-// nothing the step itself does not overwrite) and keeps the fastest.
+// Autotuner of the RK4 air stage: times each launch shape on the air launches of stages 0-2
+// of the coming step (they read level n and the registers stage 0 rewrites, so repeating
+// them changes nothing the step itself does not overwrite) and keeps the fastest.
 template <typename T> int tune_rk(lee_ctx *c)
 {
     const Model &m = c->m;
@@ -861,23 +861,29 @@ template <typename T> int tune_rk(lee_ctx *c)
     int J1, y1on;
     gate_tick(c, J1, y1on);
     c->counter = save;
-    RKArgs<T> r;
+    // the air launches of stages 0, 1, 2 of the coming step: stage 0 rewrites the accumulators
+    // and the stage states that 1 and 2 read (from level n, which none of them writes), so the
+    // three can be repeated; stage 3 writes level n and is left out
+    RKArgs<T> r[3];
     RKPoreArgs<T> pr;
     RKBndArgs<T> b;
-    rk4_stage_args<T>(c, 0, J1, y1on, r, pr, b);
+    for (int st = 0; st < 3; ++st) rk4_stage_args<T>(c, st, J1, y1on, r[st], pr, b);
     int rc = rk_args_sync<T>(c, J1, y1on);
     if (rc) return rc;
     const RKArgs<T> *rd = (const RKArgs<T> *)c->rk_args_dev;
     const int wind = rk_wind_mode(m);
+    auto stages = [&](int v) {
+        for (int st = 0; st < 3; ++st) launch_rk_air_v<T>(r[st], rd + st, wind, m.terrain, v, c->stream);
+    };
     // two passes over the shapes, each shape's best pass kept: shapes a few percent apart
     // are not ranked by one pass's noise
     float t[kRkVariants];
     for (int v = 0; v < kRkVariants; ++v) t[v] = 1e30f;
     for (int pass = 0; pass < 2; ++pass)
         for (int v = 0; v < kRkVariants; ++v) {
-            launch_rk_air_v<T>(r, rd, wind, m.terrain, v, c->stream);  // warm-up
+            stages(v);  // warm-up
             HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-            for (int rep = 0; rep < 4; ++rep) launch_rk_air_v<T>(r, rd, wind, m.terrain, v, c->stream);
+            for (int rep = 0; rep < 2; ++rep) stages(v);
             HIPCHK(c, hipEventRecord(c->ev1, c->stream));
             HIPCHK(c, hipEventSynchronize(c->ev1));
             float ms = 0.f;
@@ -968,7 +974,7 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
                            "the fused path runs the FB_p_v step on a separable grid and, with a porous South face, "
                            "needs the interface copy");
         if (o.path < LEE_PATH_AUTO || o.path > LEE_PATH_FUSED) return set_err(nullptr, LEE_ERR_ARG, "unknown path");
-        if (o.rk_tile < 0 || o.rk_tile > kRkVariants) return set_err(nullptr, LEE_ERR_ARG, "rk_tile must be 0..5");
+        if (o.rk_tile < 0 || o.rk_tile > kRkVariants) return set_err(nullptr, LEE_ERR_ARG, "rk_tile must be 0..9");
         // AUTO runs the split passes: on MI355X they are faster than the fused pass (whose
         // per-plane barrier limits the loads in flight)
         C->fused = o.path == LEE_PATH_FUSED;

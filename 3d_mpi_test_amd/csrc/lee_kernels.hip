@@ -1868,35 +1868,47 @@ void launch_rk_air(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool terra
 
 // Variant v of the tuned stage launch (lee_ctx keeps the autotuner's choice): v = 0 the
 // plane-by-plane 64x4 grid, 1..4 the flat grids of kRkTiles, 5 and 6 the split launches of
-// split_regions: the interior and the shell, or the interior and the shell by slab axes; 7 as
-// 5 with the interior staged through LDS (5 for a uniform mean flow or rows not a multiple of
-// 16 bytes).  v = 0 when the interior is empty.
+// split_regions: the interior and the shell, or the interior and the shell by slab axes; 7 and
+// 8 as 5 and 6 with the interior staged through LDS (5 and 6 for a uniform mean flow or rows
+// not a multiple of 16 bytes).  v = 0 when the interior is empty.
 // Whether launch shape v of the stage takes the x-face nodes of the interior rows (the split
 // shapes, when the interior is not empty and the East face lane has its neighbour in its own
 // 64-point row); the boundary stage then walks only the x planes' edge rings.
 template <typename T> bool rk_stage_takes_xfaces(const RKArgs<T> &r, int v)
 {
     Regions R;
-    return v >= 5 && v <= 7 && r.a.g.nx % BX != 1 && split_regions(r.a, R, true);
+    return v >= 5 && v <= 8 && r.a.g.nx % BX != 1 && split_regions(r.a, R, true);
 }
 
 template <typename T>
 void launch_rk_air_v(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool terrain, int v, hipStream_t s)
 {
+    // the layer shell by slab axes: x, y, z layers and the edges (Regions R, block b, C, xf)
+#define LEE_RK_AXES(WM, TER)                                                                                      \
+    do {                                                                                                        \
+        if (R.ax[0].n && xf) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 1, 2, true>), dim3(list_blocks(R.ax[0])), b, 0, s, rd, R.ax[0]); \
+        else if (R.ax[0].n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 1, 2>), dim3(list_blocks(R.ax[0])), b, 0, s, rd, R.ax[0]); \
+        if (R.ax[1].n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 2, 2>), dim3(list_blocks(R.ax[1])), b, 0, s, rd, R.ax[1]); \
+        if (R.ax[2].n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 4, 2>), dim3(list_blocks(R.ax[2])), b, 0, s, rd, R.ax[2]); \
+        if (R.edge.n && xf) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 12, true>), dim3(list_blocks(R.edge)), b, 0, s, rd, R.edge); \
+        else if (R.edge.n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 12>), dim3(list_blocks(R.edge)), b, 0, s, rd, R.edge); \
+    } while (0)
     Regions R;
     if (v <= 0 || (v >= 5 && !split_regions(r.a, R, true))) { launch_rk_air<T>(r, rd, wind, terrain, s); return; }
     const bool xf = rk_stage_takes_xfaces<T>(r, v);
     constexpr bool C = std::is_same<T, float>::value;
-    if (v == 7 && wind != 1 && r.a.g.nx % Stage<T>::H == 0) {
-        // the interior staged through LDS, the shell as v 5
+    if ((v == 7 || v == 8) && wind != 1 && r.a.g.nx % Stage<T>::H == 0) {
+        // the interior staged through LDS, the shell as v 5 (whole) or v 6 (by slab axes)
         const dim3 b(BX, BY);
 #define LEE_RKL(WM, TER)                                                                                          \
     do {                                                                                                        \
         if (R.in.n) hipLaunchKernelGGL((k_rk_air_lds<T, WM, TER, C>), dim3(list_blocks(R.in)), b, 0, s, rd, R.in); \
-        if (R.shell.n) {                                                                                        \
-            if (xf) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 6, true>), dim3(list_blocks(R.shell)), b, 0, s, rd, R.shell); \
-            else hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 6>), dim3(list_blocks(R.shell)), b, 0, s, rd, R.shell); \
+        if (v == 7) {                                                                                           \
+            if (R.shell.n && xf) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 6, true>), dim3(list_blocks(R.shell)), b, 0, s, rd, R.shell); \
+            else if (R.shell.n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 6>), dim3(list_blocks(R.shell)), b, 0, s, rd, R.shell); \
+            break;                                                                                              \
         }                                                                                                       \
+        LEE_RK_AXES(WM, TER);                                                                                   \
     } while (0)
         if (terrain) { if (wind == 2) LEE_RKL(2, true); else LEE_RKL(0, true); }
         else { if (wind == 2) LEE_RKL(2, false); else LEE_RKL(0, false); }
@@ -1904,6 +1916,7 @@ void launch_rk_air_v(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool ter
         return;
     }
     if (v == 7) v = 5;
+    if (v == 8) v = 6;
     if (v >= 5) {
         const dim3 b(BX, BY);
 #define LEE_RKS(WM, TER)                                                                                          \
@@ -1914,12 +1927,7 @@ void launch_rk_air_v(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool ter
             else if (R.shell.n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 6>), dim3(list_blocks(R.shell)), b, 0, s, rd, R.shell); \
             break;                                                                                              \
         }                                                                                                       \
-        if (R.ax[0].n && xf) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 1, 2, true>), dim3(list_blocks(R.ax[0])), b, 0, s, rd, R.ax[0]); \
-        else if (R.ax[0].n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 1, 2>), dim3(list_blocks(R.ax[0])), b, 0, s, rd, R.ax[0]); \
-        if (R.ax[1].n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 2, 2>), dim3(list_blocks(R.ax[1])), b, 0, s, rd, R.ax[1]); \
-        if (R.ax[2].n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 4, 2>), dim3(list_blocks(R.ax[2])), b, 0, s, rd, R.ax[2]); \
-        if (R.edge.n && xf) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 12, true>), dim3(list_blocks(R.edge)), b, 0, s, rd, R.edge); \
-        else if (R.edge.n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 12>), dim3(list_blocks(R.edge)), b, 0, s, rd, R.edge); \
+        LEE_RK_AXES(WM, TER);                                                                                   \
     } while (0)
         if (terrain) {
             if (wind == 1) LEE_RKS(1, true); else if (wind == 2) LEE_RKS(2, true); else LEE_RKS(0, true);
@@ -1951,6 +1959,7 @@ void launch_rk_air_v(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool ter
     default: LEE_RKT(16, true); break;
     }
 #undef LEE_RKT
+#undef LEE_RK_AXES
 }
 
 // Blocks of `kernel` (256 threads) the chip holds at once: the grid of a persistent form.

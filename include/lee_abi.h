@@ -117,10 +117,10 @@ typedef struct lee_options {
     double ground_a[8], ground_r[8];
     double ground_zinf, ground_tau;
     /* RK4 air-stage launch shape: 0 = autotune on the first steps (times the candidates on the
-     * context's own buffers), 1..8 = candidate 0..7 (64x4 plane order, 64x4 / 64x8 XCD columns,
+     * context's own buffers), 1..9 = candidate 0..8 (64x4 plane order, 64x4 / 64x8 XCD columns,
      * 64x8 plane order, 64x16 XCD columns, 64x4 tiles split into the interior outside every
-     * layer and the layer shell, the same with the shell split by slab axes, and with the
-     * interior's stage state staged through LDS).  All give the same bits. */
+     * layer and the layer shell, the same with the shell split by slab axes, and those two
+     * with the interior's stage state staged through LDS).  All give the same bits. */
     int rk_tile;
 } lee_options;
 
@@ -150,7 +150,7 @@ typedef struct lee_info {
     long long steps_done;           /* time steps advanced so far */
     long long pml_y1_counter;       /* reference PML_JMAX1 step counter */
     int path;                       /* LEE_PATH_SPLIT or LEE_PATH_FUSED, as chosen */
-    int rk_tile;                    /* RK4 air-stage launch shape in use (1..8), 0 before tuning */
+    int rk_tile;                    /* RK4 air-stage launch shape in use (1..9), 0 before tuning */
 } lee_info;
 
 typedef struct lee_ctx lee_ctx;
