@@ -48,6 +48,7 @@ class LeeInfo(ctypes.Structure):
         ("steps_done", ctypes.c_longlong), ("pml_y1_counter", ctypes.c_longlong),
         ("path", ctypes.c_int),
         ("rk_tile", ctypes.c_int),
+        ("fb_form", ctypes.c_int),
     ]
 
 
@@ -253,7 +254,7 @@ class Solver:
 
     def __init__(self, d, precision: int = 64, receivers=None, health_every: int = 0, device: int = 0,
                  path: int = PATH_AUTO, integrator: Optional[int] = None, use_graph: bool = False,
-                 rk_tile: int = 0):
+                 rk_tile: int = 0, fb_form: int = 0):
         L = lib()
         if isinstance(d, deck.Deck):
             self._deck_obj = d
@@ -263,7 +264,7 @@ class Solver:
             self._cdeck = d
         opt = deck.make_options(self._deck_obj, precision=precision, receivers=receivers,
                                 health_every=health_every, device=device, path=path, integrator=integrator,
-                                use_graph=1 if use_graph else 0, rk_tile=rk_tile)
+                                use_graph=1 if use_graph else 0, rk_tile=rk_tile, fb_form=fb_form)
         self._opt = opt
         h = ctypes.c_void_p()
         _check(L.lee_create(ctypes.byref(self._cdeck), ctypes.byref(opt), ctypes.byref(h)))

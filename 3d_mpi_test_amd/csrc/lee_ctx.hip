@@ -60,6 +60,8 @@ struct lee_ctx {
     void *Q[2][6][4] = {};  // PML auxiliaries, two sets: Q[qcur] holds the values before a step
     int qcur = 0;
     bool fused = false;     // both air phases in one kernel (k_air_fb)
+    int fb_form = -1;       // FB_p_v air-pass launch form (kFbForms), -1 until tuned
+    int fb_first = 0, fb_last = 0;  // the forms lee_options.path allows, first .. last
     int rk_variant = -1;    // RK4 air-stage launch shape (kRkTiles index), -1 until tuned
     bool rk_xfaces = false; // that shape's air stage takes the x-face nodes of the interior rows
     bool fb_xfaces = false; // the split FB passes take them
@@ -711,6 +713,16 @@ template <typename T> void enqueue_rk4_step(lee_ctx *c, double *rec_slot)
     c->steps += 1;
 }
 
+// Face-walk arguments of the split FB passes: they take the x-face nodes of the interior rows
+// (the face passes then walk only the x planes' edge rings); returns whether they do.
+template <typename T> bool fb_face_args(const lee_ctx *c, const AirArgs<T> &a, BCArgs<T> &b)
+{
+    if (c->fused || !fb_passes_take_xfaces<T>(a)) return false;
+    b.xring = 1;
+    b.nb_air = boundary_count<T>(c->m.nx, c->m.ny, c->m.nz, true);
+    return true;
+}
+
 // One FB_p_v step (airpore.cpp:589-614): launches only, receivers into rec_slot.
 template <typename T> void enqueue_fb_step(lee_ctx *c, double *rec_slot)
 {
@@ -722,12 +734,11 @@ template <typename T> void enqueue_fb_step(lee_ctx *c, double *rec_slot)
     PoreArgs<T> pa{};
     if (m.south_porous) pa = PoreArgs<T>{pore_grid<T>(c), (T)m.p_gama, (T)m.p_invbeta, (T)m.p_eff};
     BCArgs<T> b = bc_args<T>(c, a, rec_slot);
-    c->fb_xfaces = !c->fused && fb_passes_take_xfaces<T>(a);
-    if (c->fb_xfaces) {
-        // the split passes take the x-face nodes of the interior rows
-        b.xring = 1;
-        b.nb_air = boundary_count<T>(m.nx, m.ny, m.nz, true);
-    }
+    if (c->fb_form < 0) c->fb_form = c->fused ? 3 : 0;  // not tuned: the defaults
+    c->fused = c->fb_form >= 2;
+    const bool pad = c->fb_form == 1;
+    const int per_cu = c->fused ? kFbPerCu[c->fb_form - 2] : 0;
+    c->fb_xfaces = fb_face_args<T>(c, a, b);
     if (c->fused) {
         // porous pressure first: the air South face reads its plane Kp-2 through the interface
         if (m.south_porous) {
@@ -736,7 +747,7 @@ template <typename T> void enqueue_fb_step(lee_ctx *c, double *rec_slot)
             prof_mark(c, KS_PORE_P, false);
         }
         prof_mark(c, KS_AIR_FB, true);
-        launch_air_fb<T>(a, m.has_wind, s);
+        launch_air_fb<T>(a, m.has_wind, s, per_cu);
         prof_mark(c, KS_AIR_FB, false);
         prof_mark(c, KS_BC_P, true);
         launch_bc_p_all<T>(b, s);
@@ -744,7 +755,7 @@ template <typename T> void enqueue_fb_step(lee_ctx *c, double *rec_slot)
     } else {
         // ---- pressure phase (airpore.cpp:366-398); receivers sample level n first
         prof_mark(c, KS_AIR_P, true);
-        launch_air_p<T>(a, m.has_wind, s);
+        launch_air_p<T>(a, m.has_wind, s, pad);
         prof_mark(c, KS_AIR_P, false);
         if (m.south_porous) {
             prof_mark(c, KS_PORE_P, true);
@@ -756,7 +767,7 @@ template <typename T> void enqueue_fb_step(lee_ctx *c, double *rec_slot)
         prof_mark(c, KS_BC_P, false);
         // ---- velocity phase (airpore.cpp:324-364)
         prof_mark(c, KS_AIR_V, true);
-        launch_air_v<T>(a, m.has_wind, s);
+        launch_air_v<T>(a, m.has_wind, s, pad);
         prof_mark(c, KS_AIR_V, false);
     }
     if (m.south_porous) {
@@ -898,12 +909,64 @@ template <typename T> int tune_rk(lee_ctx *c)
     return LEE_OK;
 }
 
+// Autotuner of the FB_p_v air passes: times each launch form the path allows on the air and
+// face passes of the coming step (they write only level n+1, its auxiliaries and the
+// receiver buffer, all of which the step itself rewrites) and keeps the fastest.  The porous
+// passes are the same in every form and are left out.
+template <typename T> int tune_fb(lee_ctx *c)
+{
+    const Model &m = c->m;
+    const long long save = c->counter;
+    int J1, y1on;
+    gate_tick(c, J1, y1on);
+    c->counter = save;
+    const AirArgs<T> a = air_args<T>(c, J1, y1on);
+    float t[kFbForms];
+    for (int f = 0; f < kFbForms; ++f) t[f] = 1e30f;
+    for (int pass = 0; pass < 2; ++pass)
+        for (int f = c->fb_first; f <= c->fb_last; ++f) {
+            c->fused = f >= 2;
+            BCArgs<T> b = bc_args<T>(c, a, c->rec_buf);
+            (void)fb_face_args<T>(c, a, b);
+            auto passes = [&]() {
+                if (c->fused) {
+                    launch_air_fb<T>(a, m.has_wind, c->stream, kFbPerCu[f - 2]);
+                    launch_bc_p_all<T>(b, c->stream);
+                } else {
+                    launch_air_p<T>(a, m.has_wind, c->stream, f == 1);
+                    launch_bc_p_all<T>(b, c->stream);
+                    launch_air_v<T>(a, m.has_wind, c->stream, f == 1);
+                }
+                launch_bc_v_all<T>(b, c->stream);
+            };
+            passes();  // warm-up
+            HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+            for (int rep = 0; rep < 3; ++rep) passes();
+            HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+            HIPCHK(c, hipEventSynchronize(c->ev1));
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            t[f] = std::min(t[f], ms);
+        }
+    HIPCHK(c, hipGetLastError());
+    int pick = c->fb_first;
+    for (int f = c->fb_first + 1; f <= c->fb_last; ++f)
+        if (t[f] < t[pick]) pick = f;
+    c->fb_form = pick;
+    c->fused = pick >= 2;
+    return LEE_OK;
+}
+
 template <typename T> int run_steps(lee_ctx *c, int nsteps, double *rec_out)
 {
     Model &m = c->m;
     hipStream_t s = c->stream;
     if (m.rk4 && c->rk_variant < 0) {
         int rc = tune_rk<T>(c);
+        if (rc) return rc;
+    }
+    if (!m.rk4 && c->fb_form < 0) {
+        int rc = tune_fb<T>(c);
         if (rc) return rc;
     }
     const int nr = (int)m.ri.size();
@@ -975,9 +1038,17 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
                            "needs the interface copy");
         if (o.path < LEE_PATH_AUTO || o.path > LEE_PATH_FUSED) return set_err(nullptr, LEE_ERR_ARG, "unknown path");
         if (o.rk_tile < 0 || o.rk_tile > kRkVariants) return set_err(nullptr, LEE_ERR_ARG, "rk_tile must be 0..9");
-        // AUTO runs the split passes: on MI355X they are faster than the fused pass (whose
-        // per-plane barrier limits the loads in flight)
-        C->fused = o.path == LEE_PATH_FUSED;
+        // the FB launch forms the path allows: SPLIT the split passes (plain or padded),
+        // FUSED the fused pass at each pencil depth, AUTO both when the fused pass can run
+        C->fb_first = o.path == LEE_PATH_FUSED ? 2 : 0;
+        C->fb_last = o.path == LEE_PATH_SPLIT || !can ? 1 : kFbForms - 1;
+        if (o.fb_form < 0 || o.fb_form > kFbForms) return set_err(nullptr, LEE_ERR_ARG, "fb_form must be 0..6");
+        if (o.fb_form > 0) {
+            if (o.fb_form - 1 < C->fb_first || o.fb_form - 1 > C->fb_last)
+                return set_err(nullptr, LEE_ERR_UNSUPPORTED, "fb_form is not one of the forms this path and deck allow");
+            C->fb_form = o.fb_form - 1;
+        }
+        C->fused = C->fb_form >= 2 || o.path == LEE_PATH_FUSED;
         // the boundary kernels decode node indices in 32 bits
         const long long nb = boundary_count<double>(mm.nx, mm.ny, mm.nz) +
                              (mm.south_porous ? boundary_count<double>(mm.pnx, mm.pny, mm.pnz) : 0) + (long long)mm.ri.size();
@@ -1126,6 +1197,7 @@ int lee_get_info(const lee_ctx *c, lee_info *info)
     info->pml_y1_counter = c->counter;
     info->path = c->fused ? LEE_PATH_FUSED : LEE_PATH_SPLIT;
     info->rk_tile = c->rk_variant + 1;
+    info->fb_form = c->m.rk4 ? 0 : c->fb_form + 1;
     return LEE_OK;
 }
 

@@ -213,12 +213,20 @@ template <typename T> long long boundary_count(int nx, int ny, int nz, bool xrin
 template <typename T> void launch_bc_p_all(const BCArgs<T> &a, hipStream_t s);
 template <typename T> void launch_bc_v_all(const BCArgs<T> &a, hipStream_t s);
 
-template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s);
-template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStream_t s);
+// pad: the box lists of the split launches with each plane's tiles padded to a multiple of 8
+// (a tile column on one XCD under round-robin dispatch; padding tiles do nothing)
+template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s, bool pad = false);
+template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStream_t s, bool pad = false);
 // Both air phases in one pass (pressure then velocity, plane by plane); see lee_kernels.hip.
 // Returns false when the grid does not fit the fused kernel (nx - 2 > 1024).
 template <typename T> bool fused_fits(int nx);
-template <typename T> void launch_air_fb(const AirArgs<T> &a, bool wind, hipStream_t s);
+// per_cu: the pencil depth is the planes / chunks that give about per_cu blocks per CU
+template <typename T> void launch_air_fb(const AirArgs<T> &a, bool wind, hipStream_t s, int per_cu = 4);
+// Launch forms of the FB_p_v air passes (the FB autotuner picks one; identical bits): 0 the
+// split passes, 1 the split passes with padded box lists, 2.. the fused pass with pencils
+// for kFbPerCu[form - 2] blocks per CU.
+constexpr int kFbPerCu[4] = {2, 4, 8, 16};
+constexpr int kFbForms = 6;
 template <typename T> void launch_pore_p(const PoreArgs<T> &a, hipStream_t s);
 template <typename T> void launch_pore_v(const PoreArgs<T> &a, hipStream_t s);
 // Moving-frame shift along j of an (A x B x C) box (B = frame length), out of place.

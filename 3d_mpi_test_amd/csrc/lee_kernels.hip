@@ -1792,12 +1792,12 @@ template <typename T> bool fb_passes_take_xfaces(const AirArgs<T> &a)
     return a.g.nx % BX != 1 && split_regions(a, R, false);
 }
 
-template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s)
+template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s, bool pad)
 {
     dim3 g = interior_grid(a.g.nx, a.g.ny, a.g.nz), b(BX, BY);
     const bool ter = a.g.tsh != nullptr;
     Regions R;
-    if (split_regions(a, R, false)) {
+    if (split_regions(a, R, pad)) {
         const bool xf = fb_passes_take_xfaces<T>(a);
 #define LEE_PS(W, TER)                                                                                       \
     do {                                                                                                     \
@@ -1822,12 +1822,12 @@ template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStrea
         else hipLaunchKernelGGL((k_air_p<T, false, false>), g, b, 0, s, a);
     }
 }
-template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStream_t s)
+template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStream_t s, bool pad)
 {
     dim3 g = interior_grid(a.g.nx, a.g.ny, a.g.nz), b(BX, BY);
     const bool ter = a.g.tsh != nullptr;
     Regions R;
-    if (split_regions(a, R, false)) {
+    if (split_regions(a, R, pad)) {
         const bool xf = fb_passes_take_xfaces<T>(a);
 #define LEE_VS(W, TER)                                                                                       \
     do {                                                                                                     \
@@ -2031,7 +2031,7 @@ template <typename T> void launch_rk_pore(const RKPoreArgs<T> &r, bool terrain, 
 
 template <typename T> bool fused_fits(int nx) { return nx >= 3; }
 
-template <typename T> void launch_air_fb(const AirArgs<T> &a, bool wind, hipStream_t s)
+template <typename T> void launch_air_fb(const AirArgs<T> &a, bool wind, hipStream_t s, int per_cu)
 {
     static int cus = 0;
     if (!cus) {
@@ -2042,8 +2042,8 @@ template <typename T> void launch_air_fb(const AirArgs<T> &a, bool wind, hipStre
     }
     const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
     const int ntx = (nx - 2 + 62) / 63, nty = (ny - 2 + FB_TY - 1) / FB_TY;
-    // split the planes into chunks until there are about four blocks per CU
-    int nch = std::max(1, std::min(nz - 2, (4 * cus + ntx * nty - 1) / (ntx * nty)));
+    // split the planes into chunks until there are about per_cu blocks per CU
+    int nch = std::max(1, std::min(nz - 2, (std::max(per_cu, 1) * cus + ntx * nty - 1) / (ntx * nty)));
     const int kc = (nz - 2 + nch - 1) / nch;
     nch = (nz - 2 + kc - 1) / kc;
     dim3 grid((unsigned)(ntx * nty * nch)), block(64, FB_TY + 1);
@@ -2127,8 +2127,8 @@ void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s)
 }
 
 #define LEE_INST(T)                                                                                          \
-    template void launch_air_p<T>(const AirArgs<T> &, bool, hipStream_t);                                    \
-    template void launch_air_v<T>(const AirArgs<T> &, bool, hipStream_t);                                    \
+    template void launch_air_p<T>(const AirArgs<T> &, bool, hipStream_t, bool);                              \
+    template void launch_air_v<T>(const AirArgs<T> &, bool, hipStream_t, bool);                              \
     template bool fused_fits<T>(int);                                                                        \
     template void launch_rk_air<T>(const RKArgs<T> &, const RKArgs<T> *, int, bool, hipStream_t);            \
     template void launch_rk_pore<T>(const RKPoreArgs<T> &, bool, hipStream_t);                               \
@@ -2138,7 +2138,7 @@ void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s)
     template bool fb_passes_take_xfaces<T>(const AirArgs<T> &);                                              \
     template void launch_rk_pore_batch<T>(const RKPoreArgs<T> *, const int *, int, int, bool, int, hipStream_t); \
     template void launch_sample_batch<T>(const T *const *, const long long *, int, double *, hipStream_t);     \
-    template void launch_air_fb<T>(const AirArgs<T> &, bool, hipStream_t);                                   \
+    template void launch_air_fb<T>(const AirArgs<T> &, bool, hipStream_t, int);                              \
     template void launch_pore_p<T>(const PoreArgs<T> &, hipStream_t);                                        \
     template void launch_pore_v<T>(const PoreArgs<T> &, hipStream_t);                                        \
     template void launch_shift<T>(T *, const T *, int, int, int, int, int, int, hipStream_t);                \

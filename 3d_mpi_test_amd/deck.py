@@ -81,6 +81,7 @@ class LeeOptions(ctypes.Structure):
         ("ground_zinf", ctypes.c_double),
         ("ground_tau", ctypes.c_double),
         ("rk_tile", ctypes.c_int),
+        ("fb_form", ctypes.c_int),
     ]
 
 
@@ -89,12 +90,12 @@ INT_DECK, INT_RK4 = 0, 1
 
 def make_options(deck: Optional["Deck"] = None, precision: int = 64, receivers=None, health_every: int = 0,
                  device: int = 0, path: int = 0, integrator: Optional[int] = None, use_graph: int = 0,
-                 rk_tile: int = 0) -> LeeOptions:
+                 rk_tile: int = 0, fb_form: int = 0) -> LeeOptions:
     """``lee_options`` for a deck: its extensions (integrator, hills, log wind) plus the run
     settings.  The receiver array is kept alive on the returned struct."""
     o = LeeOptions()
     o.precision, o.health_every, o.device, o.path, o.use_graph = precision, health_every, device, path, use_graph
-    o.rk_tile = rk_tile
+    o.rk_tile, o.fb_form = rk_tile, fb_form
     if receivers is not None:
         rec = np.ascontiguousarray(receivers, dtype=np.float64).reshape(-1)
         o._rec = rec

@@ -122,6 +122,13 @@ typedef struct lee_options {
      * layer and the layer shell, the same with the shell split by slab axes, and those two
      * with the interior's stage state staged through LDS).  All give the same bits. */
     int rk_tile;
+    /* FB_p_v air-pass launch form: 0 = autotune on the first step among the forms `path`
+     * allows (times each form's air and face passes on the coming step), 1 = the split
+     * passes, 2 = the split passes with each plane's tiles padded to a multiple of 8 (a tile
+     * column on one XCD), 3..6 = the fused pass with pencils deep enough for about 2, 4, 8, 16
+     * blocks per CU.  1..2 need path AUTO or SPLIT, 3..6 AUTO or FUSED.  All give the same
+     * bits. */
+    int fb_form;
 } lee_options;
 
 /* Time integrators.  LEE_INT_DECK runs the deck's scheme (the reference implements only
@@ -136,8 +143,9 @@ enum { LEE_INT_DECK = 0, LEE_INT_RK4 = 1 };
 
 /* Kernel path of the time step.  SPLIT runs the reference's two phases as separate passes
  * over the grid; FUSED computes both phases of a plane in one pass (one read and one write
- * of u, v, w, p per point).  Both give the same bits.  AUTO runs SPLIT, the faster of the
- * two on MI355X.  FUSED needs the porous interface copy when South is porous. */
+ * of u, v, w, p per point).  Both give the same bits.  AUTO lets the FB autotuner choose
+ * among the forms of both (lee_options.fb_form); on MI355X it keeps the split passes.  FUSED
+ * needs the porous interface copy when South is porous. */
 enum { LEE_PATH_AUTO = 0, LEE_PATH_SPLIT = 1, LEE_PATH_FUSED = 2 };
 
 typedef struct lee_info {
@@ -151,6 +159,7 @@ typedef struct lee_info {
     long long pml_y1_counter;       /* reference PML_JMAX1 step counter */
     int path;                       /* LEE_PATH_SPLIT or LEE_PATH_FUSED, as chosen */
     int rk_tile;                    /* RK4 air-stage launch shape in use (1..9), 0 before tuning */
+    int fb_form;                    /* FB_p_v air-pass launch form in use (1..6), 0 before tuning */
 } lee_info;
 
 typedef struct lee_ctx lee_ctx;

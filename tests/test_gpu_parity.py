@@ -137,3 +137,35 @@ def test_fused_equals_split_bitwise(lee, prec):
             for x, y in zip(a.state(1), b.state(1)):
                 assert np.array_equal(x, y), name
         a.close(); b.close()
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_fb_launch_forms_give_the_same_bits(lee, prec):
+    """Every launch form of the FB_p_v air passes the autotuner picks from (split, split with
+    padded tile lists, fused at four pencil depths) gives the same bits; the tuner keeps one
+    of the forms the path allows."""
+    for name in ("pml5", "wide"):
+        d = cases(lee)[name]
+        ref = None
+        for form in range(1, 7):
+            print(f"[gpu] fb launch form {form} fp{prec} case={name}", flush=True)
+            s = lee.Solver(d, precision=prec, receivers=REC, fb_form=form)
+            s.frame_setup(0)
+            got = (s.step(d.time_domain), s.state(), s.state(1) if d.has_pore else [])
+            assert s.info().fb_form == form
+            s.close()
+            if ref is None:
+                ref = got
+                continue
+            assert np.array_equal(got[0], ref[0]), (name, form)
+            for x, y in zip(got[1] + got[2], ref[1] + ref[2]):
+                assert np.array_equal(x, y), (name, form)
+    d = cases(lee)["wide"]
+    for path, allowed in (("split", (1, 2)), ("fused", (3, 4, 5, 6)), (None, tuple(range(1, 7)))):
+        s = lee.Solver(d, precision=prec, receivers=REC, path=PATHS[path] if path else 0)
+        s.frame_setup(0)
+        s.step(2)
+        assert s.info().fb_form in allowed, path
+        s.close()
+    with pytest.raises(lee.LeeError):
+        lee.Solver(d, precision=prec, receivers=REC, path=PATHS["split"], fb_form=4)
