@@ -734,7 +734,7 @@ template <typename T> void enqueue_fb_step(lee_ctx *c, double *rec_slot)
     PoreArgs<T> pa{};
     if (m.south_porous) pa = PoreArgs<T>{pore_grid<T>(c), (T)m.p_gama, (T)m.p_invbeta, (T)m.p_eff};
     BCArgs<T> b = bc_args<T>(c, a, rec_slot);
-    if (c->fb_form < 0) c->fb_form = c->fused ? 3 : 0;  // not tuned: the defaults
+    if (c->fb_form < 0) c->fb_form = c->fb_first == 2 ? 3 : 0;  // not tuned: the path's default
     c->fused = c->fb_form >= 2;
     const bool pad = c->fb_form == 1;
     const int per_cu = c->fused ? kFbPerCu[c->fb_form - 2] : 0;
