@@ -1858,8 +1858,9 @@ struct lee_ensemble {
     std::string err;
     std::vector<void *> allocs;
     void *air_args[4] = {}, *pore_args[4] = {}, *bnd_args[4] = {};
-    // tile prefixes [air n+1 | pore np+1 | bnd n+1 | air np1 | pore np1]: the last two with
-    // each member plane's tiles padded to a multiple of 8 (launch form 1)
+    // tile prefixes [air n+1 | pore np+1 | bnd n+1 | air n+1 | pore np+1] (np porous
+    // members): the last two with each member plane's tiles padded to a multiple of 8
+    // (launch form 1)
     int *first = nullptr;
     int air_blocks = 0, pore_blocks = 0, bnd_blocks = 0, n_pore = 0, air_blocks_pad = 0, pore_blocks_pad = 0;
     int form = -1;       // launch form of the air stage (kEnsForms), -1 until tuned
