@@ -106,7 +106,7 @@ struct lee_ctx {
     float last_ms = 0.f;
     std::vector<double> host_rec;
     // profiling: event pairs around the stencil launches, resolved at the end of lee_step
-    bool profile = false;
+    int profile = 0;  // lee_profile: 0 off, 1 every kernel kind, 2 the air stencil launches only
     std::vector<hipEvent_t> evpool;
     size_t evnext = 0;
     struct Pending { int kind; hipEvent_t a, b; };
@@ -249,6 +249,7 @@ hipEvent_t pooled_event(lee_ctx *c)
 void prof_mark(lee_ctx *c, int kind, bool begin)
 {
     if (!c->profile) return;
+    if (c->profile == 2 && kind != KS_RK_AIR && kind != KS_AIR_P && kind != KS_AIR_V && kind != KS_AIR_FB) return;
     hipEvent_t e = pooled_event(c);
     if (!e) return;
     (void)hipEventRecord(e, c->stream);
@@ -1722,7 +1723,8 @@ double lee_last_step_ms(const lee_ctx *c) { return c ? (double)c->last_ms : 0.0;
 int lee_profile(lee_ctx *c, int enable)
 {
     if (!c) return LEE_ERR_ARG;
-    c->profile = enable != 0;
+    if (enable < 0 || enable > 2) return set_err(c, LEE_ERR_ARG, "profile level 0, 1 or 2");
+    c->profile = enable;
     for (auto &k : c->kstat) { k.launches = 0; k.total_ms = 0; }
     return LEE_OK;
 }

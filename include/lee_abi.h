@@ -238,6 +238,8 @@ typedef struct lee_kstat {
     double bytes_per_launch;
     double points_per_launch;
 } lee_kstat;
+/* enable: 0 off, 1 every kernel kind, 2 the air stencil launches only (the dominant kernels,
+ * with fewer timing marks in the stream) */
 int  lee_profile(lee_ctx *ctx, int enable);
 int  lee_kernel_stats(const lee_ctx *ctx, lee_kstat *out, int cap);
 
