@@ -1038,7 +1038,7 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
                            "the fused path runs the FB_p_v step on a separable grid and, with a porous South face, "
                            "needs the interface copy");
         if (o.path < LEE_PATH_AUTO || o.path > LEE_PATH_FUSED) return set_err(nullptr, LEE_ERR_ARG, "unknown path");
-        if (o.rk_tile < 0 || o.rk_tile > kRkVariants) return set_err(nullptr, LEE_ERR_ARG, "rk_tile must be 0..9");
+        if (o.rk_tile < 0 || o.rk_tile > kRkVariants) return set_err(nullptr, LEE_ERR_ARG, "rk_tile must be 0..11");
         // the FB launch forms the path allows: SPLIT the split passes (plain or padded),
         // FUSED the fused pass at each pencil depth, AUTO both when the fused pass can run
         C->fb_first = o.path == LEE_PATH_FUSED ? 2 : 0;

@@ -150,12 +150,14 @@ template <typename T>
 void launch_rk_air(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool terrain, hipStream_t s);
 // Launch shapes of the RK4 air stage the autotuner chooses from (variant 0 = launch_rk_air).
 struct RkTile { int ty; bool columns; const char *name; };
-constexpr RkTile kRkTiles[9] = {{4, false, "64x4 planes"}, {4, true, "64x4 xcd-columns"}, {8, true, "64x8 xcd-columns"},
+constexpr RkTile kRkTiles[11] = {{4, false, "64x4 planes"}, {4, true, "64x4 xcd-columns"}, {8, true, "64x8 xcd-columns"},
                                 {8, false, "64x8 planes"}, {16, true, "64x16 xcd-columns"},
                                 {4, false, "64x4 interior + layer shell"}, {4, false, "64x4 interior + layers by axis"},
                                 {4, false, "64x4 LDS-staged interior + layer shell"},
-                                {4, false, "64x4 LDS-staged interior + layers by axis"}};
-constexpr int kRkVariants = 9;
+                                {4, false, "64x4 LDS-staged interior + layers by axis"},
+                                {4, false, "64x4 LDS-staged interior + LDS-staged layer shell"},
+                                {4, false, "64x4 LDS-staged interior + LDS-staged layers by axis"}};
+constexpr int kRkVariants = 11;
 template <typename T>
 void launch_rk_air_v(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool terrain, int v, hipStream_t s);
 // Whether shape v's air stage also takes the x-face nodes of the interior rows (the boundary

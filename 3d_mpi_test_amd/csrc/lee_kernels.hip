@@ -1258,9 +1258,12 @@ template <typename T> struct Stage {
 };
 typedef __attribute__((address_space(3))) void lds_void;
 
+// Only grid points are loaded: chunks outside x = 0 .. nx - 1 or rows outside 0 .. ny - 1 (the
+// planes k -+ 1: 0 .. ny - 2, the rows of points) are left out, which no point of the grid
+// reads (tiles of the layer shell reach past the faces); the caller keeps nx a multiple of H.
 template <typename T, int NF>
-__device__ __forceinline__ void stage_fields(gptr<const T> const (&F)[5], T *lds, size_t nx, size_t sxy, int i0, int j0,
-                                             int k)
+__device__ __forceinline__ void stage_fields(gptr<const T> const (&F)[5], T *lds, int nx, int ny, size_t sxy, int i0,
+                                             int j0, int k)
 {
     using S = Stage<T>;
     const int lane = threadIdx.x;
@@ -1271,22 +1274,30 @@ __device__ __forceinline__ void stage_fields(gptr<const T> const (&F)[5], T *lds
 #pragma unroll
         for (int part = 0; part < S::G; ++part) {
             if ((f * S::G + part) % BY != (int)threadIdx.y) continue;
-            int c, limit, off;
+            int c, limit, off, x, y, ylim;
             size_t e;
             if (part < S::GK) {  // plane k, row by row of W elements
                 c = part * 64 + lane;
                 const int r = c / (S::W / S::H), cc = c - r * (S::W / S::H);
-                e = (size_t)(i0 - S::H + cc * S::H) + nx * (size_t)(j0 - 1 + r) + sxy * (size_t)k;
+                x = i0 - S::H + cc * S::H;
+                y = j0 - 1 + r;
+                ylim = ny;
+                e = sxy * (size_t)k;
                 limit = S::CK;
                 off = f * S::SZ + part * 64 * S::H;
             } else {  // plane k - 1 (side 0) or k + 1 (side 1), the tile's rows
                 const int q = part - S::GK, side = q / S::GS, gq = q - side * S::GS;
                 c = gq * 64 + lane;
                 const int r = c / (BX / S::H), cc = c - r * (BX / S::H);
-                e = (size_t)(i0 + cc * S::H) + nx * (size_t)(j0 + r) + sxy * (size_t)(side ? k + 1 : k - 1);
+                x = i0 + cc * S::H;
+                y = j0 + r;
+                ylim = ny - 1;
+                e = sxy * (size_t)(side ? k + 1 : k - 1);
                 limit = S::CS;
                 off = f * S::SZ + S::PK + side * S::PS + gq * 64 * S::H;
             }
+            if (x < 0 || x >= nx || y >= ylim) continue;
+            e += (size_t)x + (size_t)nx * (size_t)y;
             if (c < limit)
                 __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) const void *)(F[f] + e),
                                                  (lds_void *)(lds + off), 16, 0, 0);
@@ -1311,7 +1322,7 @@ __global__ __launch_bounds__(BX * BY) void k_rk_air_lds(const RKArgs<T> *__restr
     const size_t id = (size_t)i + nx * ((size_t)j + (size_t)g.ny * k);
     {
         gptr<const T> const F[5] = {g.un, g.vn, g.wn, g.pn, a.V};
-        stage_fields<T, NF>(F, lds, nx, sxy, i - ti, j - tj, k);
+        stage_fields<T, NF>(F, lds, g.nx, g.ny, sxy, i - ti, j - tj, k);
     }
     // the point's registers while the stage state is in flight
     const int st = r.stage;
@@ -1356,6 +1367,94 @@ __global__ __launch_bounds__(BX * BY) void k_rk_air_lds(const RKArgs<T> *__restr
         if (st < 3) r.acc[f][id] = ao[f];
         if (COMP && st == 3) r.err[f][id] = eo[f];
     }
+}
+
+// The layer shell (or one part of it, L its slab axes) with the stage state staged through LDS
+// as k_rk_air_lds stages the interior's: the same core on LDS operands, then the layer terms,
+// the register update, the x-face lanes (XF) and the auxiliaries exactly as rk_air_point has
+// them, so the bits are the same.  Every thread of a block stages and meets the barrier; the
+// points outside the grid's interior rows leave after it.
+template <typename T, int WM, bool TERRAIN, bool COMP, int L, int NB, bool XF>
+__global__ __launch_bounds__(BX * BY) void k_rk_air_lds_box(const RKArgs<T> *__restrict__ rp, const BoxList<NB> l)
+{
+    static_assert(WM == 0 || WM == 2, "uniform mean flows take the register path");
+    using S = Stage<T>;
+    constexpr int NF = WM == 2 ? 5 : 4;
+    __shared__ T lds[NF * S::SZ];
+    const RKArgs<T> &r = *rp;
+    const AirArgs<T> &a = r.a;
+    const Grid<T> &g = a.g;
+    int i, j, k;
+    list_point(l, blockIdx.x, i, j, k);
+    if (i < 0) return;  // a padding block (uniform over the block)
+    const int ti = threadIdx.x, tj = threadIdx.y, nx = g.nx;
+    const size_t sxy = (size_t)nx * g.ny;
+    {
+        gptr<const T> const F[5] = {g.un, g.vn, g.wn, g.pn, a.V};
+        stage_fields<T, NF>(F, lds, nx, g.ny, sxy, i - ti, j - tj, k);
+    }
+    const bool live = i >= (XF ? 0 : 1) && i < nx - (XF ? 0 : 1) && j < g.ny - 1;
+    const bool face = XF && (i == 0 || i == nx - 1);
+    const size_t id = live ? (size_t)i + (size_t)nx * ((size_t)j + (size_t)g.ny * k) : 0;
+    const int st = r.stage;
+    T xn[4], ac[4], er[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        xn[f] = (!live || st == 0 || face || (!COMP && st == 3)) ? T(0) : r.xn[f][id];
+        ac[f] = (!live || st == 0 || face) ? T(0) : r.acc[f][id];
+        er[f] = (live && COMP && st == 3 && !face) ? r.err[f][id] : T(0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (!live) return;
+    auto K = [&](int f, int di, int dj) { return lds[f * S::SZ + (tj + 1 + dj) * S::W + (ti + S::H + di)]; };
+    auto M = [&](int f) { return lds[f * S::SZ + S::PK + tj * BX + ti]; };
+    auto P = [&](int f) { return lds[f * S::SZ + S::PK + S::PS + tj * BX + ti]; };
+    const T xc[4] = {K(0, 0, 0), K(1, 0, 0), K(2, 0, 0), K(3, 0, 0)};  // stage state at the point
+    T kf[4] = {T(0), T(0), T(0), T(0)};
+    if (!face) {
+        Nb<T> n;
+        n.zX = T(0); n.zY = T(0);
+        if constexpr (TERRAIN) metric_terrain(g, i, j, k, n.mX, n.mY, n.mZ, n.zX, n.zY);
+        else metric(g, i, j, k, n.mX, n.mY, n.mZ);
+        n.uc = xc[0]; n.u_ip = K(0, 1, 0); n.u_jp = K(0, 0, 1); n.u_jm = K(0, 0, -1); n.u_kp = P(0); n.u_km = M(0);
+        n.vc = xc[1]; n.v_jp = K(1, 0, 1); n.v_jm = K(1, 0, -1); n.v_kp = P(1); n.v_km = M(1);
+        n.wc = xc[2]; n.w_jp = K(2, 0, 1); n.w_jm = K(2, 0, -1); n.w_kp = P(2); n.w_km = M(2);
+        n.pc = xc[3]; n.p_im = K(3, -1, 0); n.p_jp = K(3, 0, 1); n.p_jm = K(3, 0, -1); n.p_kp = P(3); n.p_km = M(3);
+        if constexpr (WM == 2) {
+            n.Vc = K(4, 0, 0); n.V_ip = K(4, 1, 0); n.V_im = K(4, -1, 0); n.V_jp = K(4, 0, 1); n.V_jm = K(4, 0, -1);
+            n.V_kp = P(4); n.V_km = M(4);
+        }
+        T ddx, ddy, dwz, gpx, gpy, gpz;
+        kf[3] = dp_core<T, WM, TERRAIN, true>(a, n, ddx, ddy, dwz);
+        if constexpr (L != 0) dp_layers<T, TERRAIN, L>(a, i, j, k, n.mX, n.mY, n.mZ, n.zX, n.zY, n.pc, ddx, ddy, dwz, kf[3]);
+        duvw_core<T, WM, TERRAIN, true>(a, n, gpx, gpy, gpz, kf[0], kf[1], kf[2]);
+        if constexpr (L != 0)
+            duvw_layers<T, TERRAIN, L>(a, i, j, k, n.mX, n.mY, n.mZ, n.zX, n.zY, n.uc, n.vc, n.wc, gpx, gpy, gpz, kf[0],
+                                       kf[1], kf[2]);
+    }
+    if (st == 0 || face) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) xn[f] = xc[f];
+    }
+    T d[4], ao[4], eo[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) d[f] = rk_regs<T, COMP>(st, xn[f], kf[f], ac[f], er[f], ao[f], eo[f]);
+    if constexpr (XF) {
+        const int from = i == 0 ? ti + 1 : i == nx - 1 ? ti - 1 : ti;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            const T v = __shfl(d[f], from, 64);
+            if (face) d[f] = f == 0 ? T(0) : v;
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        r.dst[f][id] = d[f];
+        if (st < 3 && !face) r.acc[f][id] = ao[f];
+        if (COMP && st == 3 && !face) r.err[f][id] = eo[f];
+    }
+    if constexpr (L != 0) rk_aux_point<T, L>(r, i, j, k, xc);
 }
 
 // Ensemble form: one launch steps every member.  The flat block index selects the member by
@@ -1869,15 +1968,16 @@ void launch_rk_air(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool terra
 // Variant v of the tuned stage launch (lee_ctx keeps the autotuner's choice): v = 0 the
 // plane-by-plane 64x4 grid, 1..4 the flat grids of kRkTiles, 5 and 6 the split launches of
 // split_regions: the interior and the shell, or the interior and the shell by slab axes; 7 and
-// 8 as 5 and 6 with the interior staged through LDS (5 and 6 for a uniform mean flow or rows
-// not a multiple of 16 bytes).  v = 0 when the interior is empty.
+// 8 as 5 and 6 with the interior staged through LDS, 9 and 10 with the shell staged as well
+// (5 and 6 for a uniform mean flow or rows not a multiple of 16 bytes).  v = 0 when the
+// interior is empty.
 // Whether launch shape v of the stage takes the x-face nodes of the interior rows (the split
 // shapes, when the interior is not empty and the East face lane has its neighbour in its own
 // 64-point row); the boundary stage then walks only the x planes' edge rings.
 template <typename T> bool rk_stage_takes_xfaces(const RKArgs<T> &r, int v)
 {
     Regions R;
-    return v >= 5 && v <= 8 && r.a.g.nx % BX != 1 && split_regions(r.a, R, true);
+    return v >= 5 && v <= 10 && r.a.g.nx % BX != 1 && split_regions(r.a, R, true);
 }
 
 template <typename T>
@@ -1897,12 +1997,26 @@ void launch_rk_air_v(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool ter
     if (v <= 0 || (v >= 5 && !split_regions(r.a, R, true))) { launch_rk_air<T>(r, rd, wind, terrain, s); return; }
     const bool xf = rk_stage_takes_xfaces<T>(r, v);
     constexpr bool C = std::is_same<T, float>::value;
-    if ((v == 7 || v == 8) && wind != 1 && r.a.g.nx % Stage<T>::H == 0) {
-        // the interior staged through LDS, the shell as v 5 (whole) or v 6 (by slab axes)
+    if (v >= 7 && v <= 10 && wind != 1 && r.a.g.nx % Stage<T>::H == 0) {
+        // the interior staged through LDS; the shell as v 5 (whole) or v 6 (by slab axes),
+        // from registers (v 7, 8) or staged through LDS as well (v 9, 10)
         const dim3 b(BX, BY);
+#define LEE_RKLS(WM, TER, LL, NB, LIST)                                                                            \
+    do {                                                                                                        \
+        if (LIST.n && xf) hipLaunchKernelGGL((k_rk_air_lds_box<T, WM, TER, C, LL, NB, true>), dim3(list_blocks(LIST)), b, 0, s, rd, LIST); \
+        else if (LIST.n) hipLaunchKernelGGL((k_rk_air_lds_box<T, WM, TER, C, LL, NB, false>), dim3(list_blocks(LIST)), b, 0, s, rd, LIST); \
+    } while (0)
 #define LEE_RKL(WM, TER)                                                                                          \
     do {                                                                                                        \
         if (R.in.n) hipLaunchKernelGGL((k_rk_air_lds<T, WM, TER, C>), dim3(list_blocks(R.in)), b, 0, s, rd, R.in); \
+        if (v == 9) { LEE_RKLS(WM, TER, 7, 6, R.shell); break; }                                                \
+        if (v == 10) {                                                                                          \
+            LEE_RKLS(WM, TER, 1, 2, R.ax[0]);                                                                   \
+            if (R.ax[1].n) hipLaunchKernelGGL((k_rk_air_lds_box<T, WM, TER, C, 2, 2, false>), dim3(list_blocks(R.ax[1])), b, 0, s, rd, R.ax[1]); \
+            if (R.ax[2].n) hipLaunchKernelGGL((k_rk_air_lds_box<T, WM, TER, C, 4, 2, false>), dim3(list_blocks(R.ax[2])), b, 0, s, rd, R.ax[2]); \
+            LEE_RKLS(WM, TER, 7, 12, R.edge);                                                                   \
+            break;                                                                                              \
+        }                                                                                                       \
         if (v == 7) {                                                                                           \
             if (R.shell.n && xf) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 6, true>), dim3(list_blocks(R.shell)), b, 0, s, rd, R.shell); \
             else if (R.shell.n) hipLaunchKernelGGL((k_rk_air_box<T, WM, TER, C, 7, 6>), dim3(list_blocks(R.shell)), b, 0, s, rd, R.shell); \
@@ -1913,10 +2027,11 @@ void launch_rk_air_v(const RKArgs<T> &r, const RKArgs<T> *rd, int wind, bool ter
         if (terrain) { if (wind == 2) LEE_RKL(2, true); else LEE_RKL(0, true); }
         else { if (wind == 2) LEE_RKL(2, false); else LEE_RKL(0, false); }
 #undef LEE_RKL
+#undef LEE_RKLS
         return;
     }
-    if (v == 7) v = 5;
-    if (v == 8) v = 6;
+    if (v == 7 || v == 9) v = 5;
+    if (v == 8 || v == 10) v = 6;
     if (v >= 5) {
         const dim3 b(BX, BY);
 #define LEE_RKS(WM, TER)                                                                                          \

@@ -51,10 +51,11 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-# launch shapes of the RK4 air stage (lee_options.rk_tile 1..9), as the autotuner reports them
+# launch shapes of the RK4 air stage (lee_options.rk_tile 1..11), as the autotuner reports them
 RK_TILES = ["64x4 planes", "64x4 xcd-columns", "64x8 xcd-columns", "64x8 planes", "64x16 xcd-columns",
             "64x4 interior + layer shell", "64x4 interior + layers by axis", "64x4 LDS-staged interior + layer shell",
-            "64x4 LDS-staged interior + layers by axis"]
+            "64x4 LDS-staged interior + layers by axis", "64x4 LDS-staged interior + LDS-staged layer shell",
+            "64x4 LDS-staged interior + LDS-staged layers by axis"]
 
 
 RK_TILE = 0  # --rk-tile: RK4 air-stage launch shape, 0 = the autotuner's choice
@@ -395,7 +396,7 @@ def main():
                     help="C1/C2: steps of the untimed FP32-vs-FP64 accuracy pass when the timed run is shorter "
                          "(default: the whole workload, 10000 / 2000; 0 skips it)")
     ap.add_argument("--rk-tile", type=int, default=0,
-                    help="RK4 air-stage launch shape 1..9 (lee_options.rk_tile; default 0: autotuned), for "
+                    help="RK4 air-stage launch shape 1..11 (lee_options.rk_tile; default 0: autotuned), for "
                          "profiling runs that should not time the tuner's trials")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "rk4_air_c1_traffic.json"),
                     help="JSON {kernel: HBM bytes per launch} from rocprofv3 FETCH_SIZE/WRITE_SIZE runs")

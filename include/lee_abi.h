@@ -117,10 +117,11 @@ typedef struct lee_options {
     double ground_a[8], ground_r[8];
     double ground_zinf, ground_tau;
     /* RK4 air-stage launch shape: 0 = autotune on the first steps (times the candidates on the
-     * context's own buffers), 1..9 = candidate 0..8 (64x4 plane order, 64x4 / 64x8 XCD columns,
-     * 64x8 plane order, 64x16 XCD columns, 64x4 tiles split into the interior outside every
-     * layer and the layer shell, the same with the shell split by slab axes, and those two
-     * with the interior's stage state staged through LDS).  All give the same bits. */
+     * context's own buffers), 1..11 = candidate 0..10 (64x4 plane order, 64x4 / 64x8 XCD
+     * columns, 64x8 plane order, 64x16 XCD columns, 64x4 tiles split into the interior outside
+     * every layer and the layer shell, the same with the shell split by slab axes, those two
+     * with the interior's stage state staged through LDS, and with the shell's staged as
+     * well).  All give the same bits. */
     int rk_tile;
     /* FB_p_v air-pass launch form: 0 = autotune on the first step among the forms `path`
      * allows (times each form's air and face passes on the coming step), 1 = the split
@@ -158,7 +159,7 @@ typedef struct lee_info {
     long long steps_done;           /* time steps advanced so far */
     long long pml_y1_counter;       /* reference PML_JMAX1 step counter */
     int path;                       /* LEE_PATH_SPLIT or LEE_PATH_FUSED, as chosen */
-    int rk_tile;                    /* RK4 air-stage launch shape in use (1..9), 0 before tuning */
+    int rk_tile;                    /* RK4 air-stage launch shape in use (1..11), 0 before tuning */
     int fb_form;                    /* FB_p_v air-pass launch form in use (1..6), 0 before tuning */
 } lee_info;
 
