@@ -1375,7 +1375,7 @@ __global__ __launch_bounds__(BX * BY) void k_rk_air_lds(const RKArgs<T> *__restr
 // them, so the bits are the same.  Every thread of a block stages and meets the barrier; the
 // points outside the grid's interior rows leave after it.
 template <typename T, int WM, bool TERRAIN, bool COMP, int L, int NB, bool XF>
-__global__ __launch_bounds__(BX * BY) void k_rk_air_lds_box(const RKArgs<T> *__restrict__ rp, const BoxList<NB> l)
+__global__ __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(COMP ? 7 : 1, 8))) void k_rk_air_lds_box(const RKArgs<T> *__restrict__ rp, const BoxList<NB> l)
 {
     static_assert(WM == 0 || WM == 2, "uniform mean flows take the register path");
     using S = Stage<T>;
