@@ -441,20 +441,20 @@ __device__ __forceinline__ void rk_bnd_node(const RKBndArgs<T> &r, long long t)
 #pragma unroll
                 for (int f = 0; f < 4; ++f) {
                     qx[s][f] = r.qxn[s][f][qi[s]];
-                    qa[s][f] = r.stage != 0 ? r.qacc[s][f][qi[s]] : T(0);
+                    qa[s][f] = T(0);
                 }
+                if (r.stage != 0) acc4_load(r.qacc[s][0], qi[s], qa[s]);
             }
         }
         g.pnn[id] = p; g.unn[id] = u; g.vnn[id] = v; g.wnn[id] = w;
 #pragma unroll
         for (int s = 0; s < 6; ++s)
             if (in[s]) {
+                T ao[4];
 #pragma unroll
-                for (int f = 0; f < 4; ++f) {
-                    T ao;
-                    r.qdst[s][f][qi[s]] = rk_q_regs<T>(r.stage, qx[s][f], a.dt * sv[f], qa[s][f], ao);
-                    if (r.stage < 3) r.qacc[s][f][qi[s]] = ao;
-                }
+                for (int f = 0; f < 4; ++f)
+                    r.qdst[s][f][qi[s]] = rk_q_regs<T>(r.stage, qx[s][f], a.dt * sv[f], qa[s][f], ao[f]);
+                if (r.stage < 3) acc4_store(r.qacc[s][0], qi[s], ao);
             }
     } else if (t < a.nb_air + a.nb_pore) {
         int i, j, k;

@@ -74,7 +74,8 @@ struct lee_ctx {
     // RK4 registers: second stage state, accumulator, FP32 accumulator low part and state
     // error ([0] air, [1] porous; u v w p), and of the auxiliaries (stage state, accumulator)
     void *rk_sb[2][4] = {}, *rk_acc[2][4] = {}, *rk_lo[2][4] = {}, *rk_err[2][4] = {};
-    void *rkq_sb[6][4] = {}, *rkq_acc[6][4] = {};
+    void *rkq_sb[6][4] = {};
+    void *rkq_acc[6][4] = {};  // [s][0]: the slab's four accumulators interleaved ([s][1..3] unused)
     // terrain tables (sh, hx, hy per (i,j); tk per k) of the air [0] and porous [1] grids
     void *ter[2][4] = {};
     unsigned char *omask = nullptr;  // embedded obstacle material ids (FB)
@@ -589,7 +590,7 @@ void rk4_stage_args(lee_ctx *c, int st, int J1, int y1on, RKArgs<T> &r, RKPoreAr
             rk_stage_q<T>(c, st, q, f, qs, qd);
             r.a.Qo[q][f] = qs; r.a.Qn[q][f] = qs;
             r.qxn[q][f] = (const T *)c->Q[c->qcur][q][f];
-            r.qacc[q][f] = (T *)c->rkq_acc[q][f];
+            r.qacc[q][f] = (T *)c->rkq_acc[q][0];  // one block of four interleaved fields
             r.qdst[q][f] = qd;
         }
     r.stage = st;
@@ -1115,7 +1116,9 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
         for (int s = 0; s < 6; ++s) {
             size_t cnt = m.slab[s].count(m.nx, m.ny, m.nz);
             for (int f = 0; f < 4; ++f)
-                if ((rc = dalloc(C, &C->rkq_sb[s][f], cnt * es)) || (rc = dalloc(C, &C->rkq_acc[s][f], cnt * es))) return rc;
+                if ((rc = dalloc(C, &C->rkq_sb[s][f], cnt * es))) return rc;
+            // the accumulators of the four fields side by side (acc4_load / acc4_store)
+            if ((rc = dalloc(C, &C->rkq_acc[s][0], 4 * cnt * es))) return rc;
         }
     }
     if (m.terrain) {

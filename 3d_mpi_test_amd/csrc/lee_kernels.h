@@ -34,6 +34,21 @@ template <typename T> struct gptr {
     }
 };
 
+// The RK4 accumulators of a PML slab's auxiliaries hold the four fields of an element side by
+// side (element q at 4q .. 4q + 3: one 16-byte access per element in FP32, two in FP64); the
+// stage arguments' qacc[s][f] all point at the slab's block.
+template <typename T> __device__ __forceinline__ void acc4_load(gptr<T> base, size_t q, T (&a)[4])
+{
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    const v4 v = ((const __attribute__((address_space(1))) v4 *)base.p)[q];
+    a[0] = v.x; a[1] = v.y; a[2] = v.z; a[3] = v.w;
+}
+template <typename T> __device__ __forceinline__ void acc4_store(gptr<T> base, size_t q, const T (&a)[4])
+{
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    ((__attribute__((address_space(1))) v4 *)base.p)[q] = v4{a[0], a[1], a[2], a[3]};
+}
+
 enum Face { F_N = 0, F_S = 1, F_E = 2, F_W = 3, F_F = 4, F_B = 5 };
 
 // Per-index PML profile: sigma, (1 - 1/beta), (1/beta - 1).

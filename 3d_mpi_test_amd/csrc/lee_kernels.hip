@@ -1118,19 +1118,16 @@ __device__ __forceinline__ void rk_aux_point(const RKArgs<T> &r, int i, int j, i
     for (int s = 0; s < 6; ++s)
         if (own.in[s]) {
             const size_t q = own.idx[s];
-            T qx[4], qa[4];
+            T qx[4], qa[4] = {T(0), T(0), T(0), T(0)}, qao[4];
+#pragma unroll
+            for (int f = 0; f < 4; ++f) qx[f] = r.qxn[s][f][q];
+            if (st != 0) acc4_load(r.qacc[s][0], q, qa);
 #pragma unroll
             for (int f = 0; f < 4; ++f) {
-                qx[f] = r.qxn[s][f][q];
-                qa[f] = st != 0 ? r.qacc[s][f][q] : T(0);
+                T unused;
+                r.qdst[s][f][q] = rk_regs<T, false>(st, qx[f], dt * xc[f], qa[f], T(0), qao[f], unused);
             }
-#pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                T qao, unused;
-                const T qd = rk_regs<T, false>(st, qx[f], dt * xc[f], qa[f], T(0), qao, unused);
-                r.qdst[s][f][q] = qd;
-                if (st < 3) r.qacc[s][f][q] = qao;
-            }
+            if (st < 3) acc4_store(r.qacc[s][0], q, qao);
         }
 }
 
