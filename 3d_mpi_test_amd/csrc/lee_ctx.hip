@@ -72,7 +72,8 @@ struct lee_ctx {
     bool capturing = false;
     bool args_fault = false;  // rk_args_sync failed inside an enqueue (reported by run_steps)
     // RK4 registers: second stage state, accumulator, FP32 accumulator low part and state
-    // error ([0] air, [1] porous; u v w p), and of the auxiliaries (stage state, accumulator)
+    // error ([0] air, [1] porous; u v w p; the air accumulator is one block of the four
+    // fields interleaved, at rk_acc[0][0]), and of the auxiliaries (stage state, accumulator)
     void *rk_sb[2][4] = {}, *rk_acc[2][4] = {}, *rk_lo[2][4] = {}, *rk_err[2][4] = {};
     void *rkq_sb[6][4] = {};
     void *rkq_acc[6][4] = {};  // [s][0]: the slab's four accumulators interleaved ([s][1..3] unused)
@@ -581,7 +582,8 @@ void rk4_stage_args(lee_ctx *c, int st, int J1, int y1on, RKArgs<T> &r, RKPoreAr
     g.pnn = src[3];  // the velocity increment reads the stage pressure
     for (int f = 0; f < 4; ++f) {
         r.xn[f] = (const T *)c->air[c->cur * 4 + f];
-        r.acc[f] = (T *)c->rk_acc[0][f]; r.lo[f] = (T *)c->rk_lo[0][f]; r.err[f] = (T *)c->rk_err[0][f];
+        r.acc[f] = (T *)c->rk_acc[0][0];  // one block of four interleaved fields
+        r.lo[f] = (T *)c->rk_lo[0][f]; r.err[f] = (T *)c->rk_err[0][f];
         r.dst[f] = dst[f];
     }
     for (int q = 0; q < 6; ++q)
@@ -1103,8 +1105,12 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
     if (m.rk4) {
         for (int d = 0; d < (m.south_porous ? 2 : 1); ++d) {
             size_t n = d ? np : na;
+            // the air accumulators of the four fields side by side (acc4_load / acc4_store), the
+            // porous ones a field each
+            if (d == 0 && (rc = dalloc(C, &C->rk_acc[0][0], 4 * n * es))) return rc;
             for (int f = 0; f < 4; ++f) {
-                if ((rc = dalloc(C, &C->rk_sb[d][f], n * es)) || (rc = dalloc(C, &C->rk_acc[d][f], n * es))) return rc;
+                if ((rc = dalloc(C, &C->rk_sb[d][f], n * es)) || (d == 1 && (rc = dalloc(C, &C->rk_acc[d][f], n * es))))
+                    return rc;
                 // FP32: the carried state error; the low part of the accumulator only for the
                 // porous velocities (integrating-factor update)
                 if (C->prec == 32) {

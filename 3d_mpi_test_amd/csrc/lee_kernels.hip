@@ -1152,14 +1152,14 @@ __device__ __forceinline__ void rk_air_point(const RKArgs<T> &r, int i, int j, i
     }
     const T xc[4] = {a.g.un[id], a.g.vn[id], a.g.wn[id], a.g.pn[id]};  // stage state at the point
     // every register read of the point first, then the arithmetic, then the stores
-    T xn[4], ac[4], er[4];
+    T xn[4], ac[4] = {T(0), T(0), T(0), T(0)}, er[4];
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         // level n: the stage state itself at stage 0; not needed by the last FP64 stage
         xn[f] = (st == 0 || face) ? xc[f] : (!COMP && st == 3) ? T(0) : r.xn[f][id];
-        ac[f] = (st == 0 || face) ? T(0) : r.acc[f][id];
         er[f] = (COMP && st == 3 && !face) ? r.err[f][id] : T(0);
     }
+    if (st != 0 && !face) acc4_load(r.acc[0], id, ac);
     T d[4], ao[4], eo[4];
 #pragma unroll
     for (int f = 0; f < 4; ++f) d[f] = rk_regs<T, COMP>(st, xn[f], kf[f], ac[f], er[f], ao[f], eo[f]);
@@ -1175,9 +1175,9 @@ __device__ __forceinline__ void rk_air_point(const RKArgs<T> &r, int i, int j, i
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         r.dst[f][id] = d[f];
-        if (st < 3 && !face) r.acc[f][id] = ao[f];
         if (COMP && st == 3 && !face) r.err[f][id] = eo[f];
     }
+    if (st < 3 && !face) acc4_store(r.acc[0], id, ao);
     if constexpr (L != 0) rk_aux_point<T, L>(r, i, j, k, xc);
 }
 
@@ -1323,13 +1323,13 @@ __global__ __launch_bounds__(BX * BY) void k_rk_air_lds(const RKArgs<T> *__restr
     }
     // the point's registers while the stage state is in flight
     const int st = r.stage;
-    T xn[4], ac[4], er[4];
+    T xn[4], ac[4] = {T(0), T(0), T(0), T(0)}, er[4];
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         xn[f] = (st == 0 || (!COMP && st == 3)) ? T(0) : r.xn[f][id];
-        ac[f] = st == 0 ? T(0) : r.acc[f][id];
         er[f] = (COMP && st == 3) ? r.err[f][id] : T(0);
     }
+    if (st != 0) acc4_load(r.acc[0], id, ac);
     Nb<T> n;
     n.zX = T(0); n.zY = T(0);
     if constexpr (TERRAIN) metric_terrain(g, i, j, k, n.mX, n.mY, n.mZ, n.zX, n.zY);
@@ -1361,9 +1361,9 @@ __global__ __launch_bounds__(BX * BY) void k_rk_air_lds(const RKArgs<T> *__restr
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         r.dst[f][id] = d[f];
-        if (st < 3) r.acc[f][id] = ao[f];
         if (COMP && st == 3) r.err[f][id] = eo[f];
     }
+    if (st < 3) acc4_store(r.acc[0], id, ao);
 }
 
 // The layer shell (or one part of it, L its slab axes) with the stage state staged through LDS
@@ -1394,13 +1394,13 @@ __global__ __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(COMP ? 
     const bool face = XF && (i == 0 || i == nx - 1);
     const size_t id = live ? (size_t)i + (size_t)nx * ((size_t)j + (size_t)g.ny * k) : 0;
     const int st = r.stage;
-    T xn[4], ac[4], er[4];
+    T xn[4], ac[4] = {T(0), T(0), T(0), T(0)}, er[4];
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         xn[f] = (!live || st == 0 || face || (!COMP && st == 3)) ? T(0) : r.xn[f][id];
-        ac[f] = (!live || st == 0 || face) ? T(0) : r.acc[f][id];
         er[f] = (live && COMP && st == 3 && !face) ? r.err[f][id] : T(0);
     }
+    if (live && st != 0 && !face) acc4_load(r.acc[0], id, ac);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (!live) return;
@@ -1448,9 +1448,9 @@ __global__ __launch_bounds__(BX * BY) __attribute__((amdgpu_waves_per_eu(COMP ? 
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         r.dst[f][id] = d[f];
-        if (st < 3 && !face) r.acc[f][id] = ao[f];
         if (COMP && st == 3 && !face) r.err[f][id] = eo[f];
     }
+    if (st < 3 && !face) acc4_store(r.acc[0], id, ao);
     if constexpr (L != 0) rk_aux_point<T, L>(r, i, j, k, xc);
 }
 
