@@ -862,9 +862,9 @@ template <typename T> int graph_steps(lee_ctx *c, int n)
     return LEE_OK;
 }
 
-// Autotuner of the RK4 air stage: times each launch shape on the air launches of stages 0-2
-// of the coming step (they read level n and the registers stage 0 rewrites, so repeating
-// them changes nothing the step itself does not overwrite) and keeps the fastest.
+// Autotuner of the RK4 air stage: times each launch shape on the air and boundary launches of
+// stages 0-2 of the coming step (they read level n and the registers stage 0 rewrites, so
+// repeating them changes nothing the step itself does not overwrite) and keeps the fastest.
 template <typename T> int tune_rk(lee_ctx *c)
 {
     const Model &m = c->m;
@@ -876,29 +876,40 @@ template <typename T> int tune_rk(lee_ctx *c)
     int J1, y1on;
     gate_tick(c, J1, y1on);
     c->counter = save;
-    // the air launches of stages 0, 1, 2 of the coming step: stage 0 rewrites the accumulators
-    // and the stage states that 1 and 2 read (from level n, which none of them writes), so the
-    // three can be repeated; stage 3 writes level n and is left out
+    // the air and boundary launches of stages 0, 1, 2 of the coming step: stage 0 rewrites the
+    // accumulators and the stage states that 1 and 2 read (from level n, which none of them
+    // writes), so the three can be repeated; stage 3 writes level n and is left out.  The
+    // boundary stage is timed with the air stage because its walks depend on the shape (the
+    // split shapes take the x-face nodes of the interior rows off it).
+    const int wind = rk_wind_mode(m);
     RKArgs<T> r[3];
     RKPoreArgs<T> pr;
-    RKBndArgs<T> b;
-    for (int st = 0; st < 3; ++st) rk4_stage_args<T>(c, st, J1, y1on, r[st], pr, b);
-    int rc = rk_args_sync<T>(c, J1, y1on);
-    if (rc) return rc;
-    const RKArgs<T> *rd = (const RKArgs<T> *)c->rk_args_dev;
-    const int wind = rk_wind_mode(m);
-    auto stages = [&](int v) {
-        for (int st = 0; st < 3; ++st) launch_rk_air_v<T>(r[st], rd + st, wind, m.terrain, v, c->stream);
-    };
+    RKBndArgs<T> b[3];
     // two passes over the shapes, each shape's best pass kept: shapes a few percent apart
     // are not ranked by one pass's noise
     float t[kRkVariants];
     for (int v = 0; v < kRkVariants; ++v) t[v] = 1e30f;
     for (int pass = 0; pass < 2; ++pass)
         for (int v = 0; v < kRkVariants; ++v) {
-            stages(v);  // warm-up
+            c->rk_variant = v;  // the stage arguments of this shape (the boundary walks follow it)
+            for (int st = 0; st < 3; ++st) rk4_stage_args<T>(c, st, J1, y1on, r[st], pr, b[st]);
+            int rc = rk_args_sync<T>(c, J1, y1on);
+            if (rc) {
+                c->rk_variant = -1;
+                return rc;
+            }
+            const RKArgs<T> *rd = (const RKArgs<T> *)c->rk_args_dev;
+            const RKBndArgs<T> *bd =
+                (const RKBndArgs<T> *)((const unsigned char *)c->rk_args_dev + 4 * sizeof(RKArgs<T>));
+            auto stages = [&]() {
+                for (int st = 0; st < 3; ++st) {
+                    launch_rk_air_v<T>(r[st], rd + st, wind, m.terrain, v, c->stream);
+                    launch_rk_bnd<T>(b[st], bd + st, c->stream);
+                }
+            };
+            stages();  // warm-up
             HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-            for (int rep = 0; rep < 2; ++rep) stages(v);
+            for (int rep = 0; rep < 2; ++rep) stages();
             HIPCHK(c, hipEventRecord(c->ev1, c->stream));
             HIPCHK(c, hipEventSynchronize(c->ev1));
             float ms = 0.f;
