@@ -660,7 +660,7 @@ template <typename T> __device__ __forceinline__ int obstacle_id(const AirArgs<T
 // XF: the lanes on the x faces take the face node p = p(1 or nx - 2) and its Q_p, as
 // rk_air_point (the Front / Back copy of calculation_2D.cpp:696-705 that the pressure face
 // pass gives such a node).
-template <typename T, bool WIND, bool TERRAIN = false, int L = 7, bool XF = false>
+template <typename T, bool WIND, bool TERRAIN = false, int L = 7, bool XF = false, bool OBST = true>
 __device__ __forceinline__ T air_p_point(const AirArgs<T> &a, int i, int j, int k)
 {
     const Grid<T> &g = a.g;
@@ -677,7 +677,7 @@ __device__ __forceinline__ T air_p_point(const AirArgs<T> &a, int i, int j, int 
     for (int s = 0; s < 6; ++s) qp_old[s] = own.in[s] ? a.Qo[s][3][own.idx[s]] : T(0);
     T r = T(0);
     if (!face) r = air_dp<T, WIND ? 1 : 0, TERRAIN, false, L>(a, i, j, k);
-    if (const int mo = L == 7 && !XF ? obstacle_id(a, i, j, k) : 0) {
+    if (const int mo = OBST && L == 7 && !XF ? obstacle_id(a, i, j, k) : 0) {
         // inside the obstacle the porous-medium continuity equation replaces the increment
         // (air.cpp:2294-2318)
         T mX, mY, mZ;
@@ -781,7 +781,7 @@ __device__ __forceinline__ void air_duvw(const AirArgs<T> &a, int i, int j, int 
 // FB velocity update at one interior point with p at n+1: calculation_2D::Cal_velocity
 // (calculation_2D.cpp:564-574) and Update_PML_Qvw (air.cpp:2323-2386).
 // XF as for air_p_point: u = 0, v and w copied (calculation_2D.cpp:784-864).
-template <typename T, bool WIND, bool TERRAIN = false, int L = 7, bool XF = false>
+template <typename T, bool WIND, bool TERRAIN = false, int L = 7, bool XF = false, bool OBST = true>
 __device__ __forceinline__ void air_v_point(const AirArgs<T> &a, int i, int j, int k)
 {
     const Grid<T> &g = a.g;
@@ -806,7 +806,7 @@ __device__ __forceinline__ void air_v_point(const AirArgs<T> &a, int i, int j, i
     T du = T(0), dv = T(0), dw = T(0);
     if (!face) air_duvw<T, WIND ? 1 : 0, TERRAIN, false, L>(a, i, j, k, du, dv, dw);
     const T dt = g.dt;
-    if (const int mo = L == 7 && !XF ? obstacle_id(a, i, j, k) : 0) {
+    if (const int mo = OBST && L == 7 && !XF ? obstacle_id(a, i, j, k) : 0) {
         // inside the obstacle the porous-medium momentum equation replaces the increments
         // (air.cpp:1907-1945)
         T mX, mY, mZ;
@@ -942,7 +942,7 @@ __global__ __launch_bounds__(64 * (FB_TY + 1)) void k_air_fb(const AirArgs<T> a,
         int ii = i, jj = j, ipp = ip, jpp = jp;
         asm volatile("" : "+v"(ii), "+v"(jj), "+v"(ipp), "+v"(jpp));
         if (!ptask || (face && !faces)) return;
-        const T p = air_p_point<T, WIND>(a, ipp, jpp, k);
+        const T p = air_p_point<T, WIND, false, 7, false, false>(a, ipp, jpp, k);
         const size_t c = (size_t)ii + (size_t)nx * jj;
         if (face) {
             // Front (calculation_2D.cpp:696-705) or West (676-685) face node
@@ -966,7 +966,7 @@ __global__ __launch_bounds__(64 * (FB_TY + 1)) void k_air_fb(const AirArgs<T> a,
         if (own) {
             int ii = i, jj = j;
             asm volatile("" : "+v"(ii), "+v"(jj));
-            air_v_point<T, WIND>(a, ii, jj, k);
+            air_v_point<T, WIND, false, 7, false, false>(a, ii, jj, k);
         }
         if (k + 1 < k1) pressure(k + 1, true);
     }
