@@ -43,12 +43,18 @@ def cases(lee):
           gauss_width=0.25, time_domain=30, velocity_method=7)
     t.hills, t.wind_ustar, t.wind_z0 = [(0.1, 0.7, 0.2, 0.3), (-2.0, 1.1, 0.12, 0.25)], 0.4, 0.02
     out["split"] = t
+    # 128 points across x: whole 64-point tiles, so the staged tiles' x halos reach past both
+    # x faces (only grid points are staged)
+    t = B(128, 30, 20, 0.05, pml=5, bc=(2, 3, 2, 2, 2, 2), porous_cells=6, x0=-3.2, src=(0.0, 0.6, 0.3),
+          gauss_width=0.25, time_domain=24, velocity_method=7)
+    t.hills, t.wind_ustar, t.wind_z0 = [(0.2, 0.6, 0.15, 0.3)], 0.4, 0.02
+    out["split128"] = t
     for d in out.values():
         d.integrator = 1
     return out
 
 
-@pytest.mark.parametrize("name", ["pml", "porous", "wind5", "terrain", "overlap", "wide", "split"])
+@pytest.mark.parametrize("name", ["pml", "porous", "wind5", "terrain", "overlap", "wide", "split", "split128"])
 def test_rk4_fp64_parity(lee, orc, name):
     d = cases(lee)[name]
     print(f"[gpu rk4] case={name} shape={d.shape}", flush=True)
@@ -143,25 +149,31 @@ def test_fb_two_x_tiles_match_oracle(lee, orc):
     assert e <= 1e-10 and fe <= 1e-10
 
 
-@pytest.mark.parametrize("prec", [64, 32])
-def test_rk4_launch_shapes_give_the_same_bits(lee, prec):
-    """Every launch shape of the RK4 air stage (the autotuner's candidates) computes the same
-    per-point arithmetic: identical receivers and fields."""
-    d = cases(lee)["split"]
-    print(f"[gpu] rk4 launch shapes fp{prec}", flush=True)
+def _launch_shapes_agree(lee, d, prec, name):
+    print(f"[gpu] rk4 launch shapes fp{prec} case={name}", flush=True)
     ref = None
     for v in range(1, 12):
         s = lee.Solver(d, precision=prec, receivers=REC, integrator=1, rk_tile=v)
         s.frame_setup(0)
-        got = (s.step(d.time_domain), s.state())
+        got = (s.step(d.time_domain), s.state(), s.state(1))
         assert s.info().rk_tile == v
         s.close()
         if ref is None:
             ref = got
             continue
-        assert np.array_equal(got[0], ref[0]), v
-        for a, b in zip(got[1], ref[1]):
-            assert np.array_equal(a, b), v
+        assert np.array_equal(got[0], ref[0]), (name, v)
+        for a, b in zip(got[1] + got[2], ref[1] + ref[2]):
+            assert np.array_equal(a, b), (name, v)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_rk4_launch_shapes_give_the_same_bits(lee, prec):
+    """Every launch shape of the RK4 air stage (the autotuner's candidates) computes the same
+    per-point arithmetic: identical receivers and fields, on a grid whose last x tile is
+    partial and on one of whole tiles."""
+    for name in ("split", "split128"):
+        _launch_shapes_agree(lee, cases(lee)[name], prec, name)
+    d = cases(lee)["split"]
     s = lee.Solver(d, precision=prec, receivers=REC, integrator=1)
     s.frame_setup(0)
     s.step(2)
