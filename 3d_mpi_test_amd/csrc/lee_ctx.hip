@@ -61,7 +61,7 @@ struct lee_ctx {
     int qcur = 0;
     bool fused = false;     // both air phases in one kernel (k_air_fb)
     int fb_form = -1;       // FB_p_v air-pass launch form (kFbForms), -1 until tuned
-    int fb_first = 0, fb_last = 0;  // the forms lee_options.path allows, first .. last
+    unsigned fb_mask = 0;   // the forms lee_options.path allows, bit f for form f
     int rk_variant = -1;    // RK4 air-stage launch shape (kRkTiles index), -1 until tuned
     bool rk_xfaces = false; // that shape's air stage takes the x-face nodes of the interior rows
     bool fb_xfaces = false; // the split FB passes take them
@@ -738,9 +738,9 @@ template <typename T> void enqueue_fb_step(lee_ctx *c, double *rec_slot)
     PoreArgs<T> pa{};
     if (m.south_porous) pa = PoreArgs<T>{pore_grid<T>(c), (T)m.p_gama, (T)m.p_invbeta, (T)m.p_eff};
     BCArgs<T> b = bc_args<T>(c, a, rec_slot);
-    if (c->fb_form < 0) c->fb_form = c->fb_first == 2 ? 3 : 0;  // not tuned: the path's default
-    c->fused = c->fb_form >= 2;
-    const bool pad = c->fb_form == 1;
+    if (c->fb_form < 0) c->fb_form = (c->fb_mask & 1u) ? 0 : 3;  // not tuned: the path's default
+    c->fused = fb_form_fused(c->fb_form);
+    const bool pad = c->fb_form == 1, vec_ok = c->fb_form != 6;
     const int per_cu = c->fused ? kFbPerCu[c->fb_form - 2] : 0;
     c->fb_xfaces = fb_face_args<T>(c, a, b);
     if (c->fused) {
@@ -759,7 +759,7 @@ template <typename T> void enqueue_fb_step(lee_ctx *c, double *rec_slot)
     } else {
         // ---- pressure phase (airpore.cpp:366-398); receivers sample level n first
         prof_mark(c, KS_AIR_P, true);
-        launch_air_p<T>(a, m.has_wind, s, pad);
+        launch_air_p<T>(a, m.has_wind, s, pad, vec_ok);
         prof_mark(c, KS_AIR_P, false);
         if (m.south_porous) {
             prof_mark(c, KS_PORE_P, true);
@@ -771,7 +771,7 @@ template <typename T> void enqueue_fb_step(lee_ctx *c, double *rec_slot)
         prof_mark(c, KS_BC_P, false);
         // ---- velocity phase (airpore.cpp:324-364)
         prof_mark(c, KS_AIR_V, true);
-        launch_air_v<T>(a, m.has_wind, s, pad);
+        launch_air_v<T>(a, m.has_wind, s, pad, vec_ok);
         prof_mark(c, KS_AIR_V, false);
     }
     if (m.south_porous) {
@@ -939,8 +939,9 @@ template <typename T> int tune_fb(lee_ctx *c)
     float t[kFbForms];
     for (int f = 0; f < kFbForms; ++f) t[f] = 1e30f;
     for (int pass = 0; pass < 2; ++pass)
-        for (int f = c->fb_first; f <= c->fb_last; ++f) {
-            c->fused = f >= 2;
+        for (int f = 0; f < kFbForms; ++f) {
+            if (!(c->fb_mask >> f & 1u)) continue;
+            c->fused = fb_form_fused(f);
             BCArgs<T> b = bc_args<T>(c, a, c->rec_buf);
             (void)fb_face_args<T>(c, a, b);
             auto passes = [&]() {
@@ -948,9 +949,9 @@ template <typename T> int tune_fb(lee_ctx *c)
                     launch_air_fb<T>(a, m.has_wind, c->stream, kFbPerCu[f - 2]);
                     launch_bc_p_all<T>(b, c->stream);
                 } else {
-                    launch_air_p<T>(a, m.has_wind, c->stream, f == 1);
+                    launch_air_p<T>(a, m.has_wind, c->stream, f == 1, f != 6);
                     launch_bc_p_all<T>(b, c->stream);
-                    launch_air_v<T>(a, m.has_wind, c->stream, f == 1);
+                    launch_air_v<T>(a, m.has_wind, c->stream, f == 1, f != 6);
                 }
                 launch_bc_v_all<T>(b, c->stream);
             };
@@ -964,11 +965,11 @@ template <typename T> int tune_fb(lee_ctx *c)
             t[f] = std::min(t[f], ms);
         }
     HIPCHK(c, hipGetLastError());
-    int pick = c->fb_first;
-    for (int f = c->fb_first + 1; f <= c->fb_last; ++f)
-        if (t[f] < t[pick]) pick = f;
+    int pick = -1;
+    for (int f = 0; f < kFbForms; ++f)
+        if ((c->fb_mask >> f & 1u) && (pick < 0 || t[f] < t[pick])) pick = f;
     c->fb_form = pick;
-    c->fused = pick >= 2;
+    c->fused = fb_form_fused(pick);
     return LEE_OK;
 }
 
@@ -1053,17 +1054,18 @@ int create_impl(const lee_deck *deck, const lee_options *opt, lee_ctx **out)
                            "needs the interface copy");
         if (o.path < LEE_PATH_AUTO || o.path > LEE_PATH_FUSED) return set_err(nullptr, LEE_ERR_ARG, "unknown path");
         if (o.rk_tile < 0 || o.rk_tile > kRkVariants) return set_err(nullptr, LEE_ERR_ARG, "rk_tile must be 0..11");
-        // the FB launch forms the path allows: SPLIT the split passes (plain or padded),
-        // FUSED the fused pass at each pencil depth, AUTO both when the fused pass can run
-        C->fb_first = o.path == LEE_PATH_FUSED ? 2 : 0;
-        C->fb_last = o.path == LEE_PATH_SPLIT || !can ? 1 : kFbForms - 1;
-        if (o.fb_form < 0 || o.fb_form > kFbForms) return set_err(nullptr, LEE_ERR_ARG, "fb_form must be 0..6");
+        // the FB launch forms the path allows: SPLIT the split passes (plain, padded, or with
+        // the one-point interior), FUSED the fused pass at each pencil depth, AUTO both when
+        // the fused pass can run
+        C->fb_mask = o.path == LEE_PATH_FUSED ? kFbFusedForms
+                     : o.path == LEE_PATH_SPLIT || !can ? kFbSplitForms : kFbSplitForms | kFbFusedForms;
+        if (o.fb_form < 0 || o.fb_form > kFbForms) return set_err(nullptr, LEE_ERR_ARG, "fb_form must be 0..7");
         if (o.fb_form > 0) {
-            if (o.fb_form - 1 < C->fb_first || o.fb_form - 1 > C->fb_last)
+            if (!(C->fb_mask >> (o.fb_form - 1) & 1u))
                 return set_err(nullptr, LEE_ERR_UNSUPPORTED, "fb_form is not one of the forms this path and deck allow");
             C->fb_form = o.fb_form - 1;
         }
-        C->fused = C->fb_form >= 2 || o.path == LEE_PATH_FUSED;
+        C->fused = fb_form_fused(C->fb_form) || o.path == LEE_PATH_FUSED;
         // the boundary kernels decode node indices in 32 bits
         const long long nb = boundary_count<double>(mm.nx, mm.ny, mm.nz) +
                              (mm.south_porous ? boundary_count<double>(mm.pnx, mm.pny, mm.pnz) : 0) + (long long)mm.ri.size();

@@ -231,19 +231,24 @@ template <typename T> void launch_bc_p_all(const BCArgs<T> &a, hipStream_t s);
 template <typename T> void launch_bc_v_all(const BCArgs<T> &a, hipStream_t s);
 
 // pad: the box lists of the split launches with each plane's tiles padded to a multiple of 8
-// (a tile column on one XCD under round-robin dispatch; padding tiles do nothing)
-template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s, bool pad = false);
-template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStream_t s, bool pad = false);
+// (a tile column on one XCD under round-robin dispatch; padding tiles do nothing).
+// vec_ok: the interior box may run the vector form (several points per lane, FP32 only)
+template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s, bool pad = false,
+                                          bool vec_ok = true);
+template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStream_t s, bool pad = false,
+                                          bool vec_ok = true);
 // Both air phases in one pass (pressure then velocity, plane by plane); see lee_kernels.hip.
 // Returns false when the grid does not fit the fused kernel (nx - 2 > 1024).
 template <typename T> bool fused_fits(int nx);
 // per_cu: the pencil depth is the planes / chunks that give about per_cu blocks per CU
 template <typename T> void launch_air_fb(const AirArgs<T> &a, bool wind, hipStream_t s, int per_cu = 4);
 // Launch forms of the FB_p_v air passes (the FB autotuner picks one; identical bits): 0 the
-// split passes, 1 the split passes with padded box lists, 2.. the fused pass with pencils
-// for kFbPerCu[form - 2] blocks per CU.
+// split passes, 1 the split passes with padded box lists, 2 .. 5 the fused pass with pencils
+// for kFbPerCu[form - 2] blocks per CU, 6 the split passes with the one-point interior.
 constexpr int kFbPerCu[4] = {2, 4, 8, 16};
-constexpr int kFbForms = 6;
+constexpr int kFbForms = 7;
+constexpr bool fb_form_fused(int f) { return f >= 2 && f < 6; }
+constexpr unsigned kFbSplitForms = 1u | 2u | 64u, kFbFusedForms = 4u | 8u | 16u | 32u;
 template <typename T> void launch_pore_p(const PoreArgs<T> &a, hipStream_t s);
 template <typename T> void launch_pore_v(const PoreArgs<T> &a, hipStream_t s);
 // Moving-frame shift along j of an (A x B x C) box (B = frame length), out of place.

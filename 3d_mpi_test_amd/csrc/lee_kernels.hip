@@ -872,6 +872,97 @@ __global__ __launch_bounds__(BX * BY) void k_air_v_box(const AirArgs<T> a, const
     air_v_point<T, WIND, TERRAIN, L, XF, false>(a, i, j, k);
 }
 
+// The interior box of the split passes without a mean flow, in vector accesses: a lane takes
+// P consecutive points of a row (P * sizeof(T) a multiple of 16 bytes), LT = BX * BY / P lanes
+// one 64 x BY tile of the list, a block of 256 lanes 256 / LT tiles.  Every field the
+// increments read at the points and at their j, k neighbours (j + 1, k + 1 for the pressure,
+// j - 1, k - 1 for the velocity) is one vector load per lane; the i + 1 (i - 1) neighbour of
+// the lane's last (first) point is a load of its own.  Each point's increment is dp_core /
+// duvw_core over the same neighbour values as the one-point form (air_p_point /
+// air_v_point with L = 0), so the bits are the same.  Rows must start aligned to the vector
+// (fb_interior_vec: nx a multiple of P, the arrays' bases aligned).
+template <typename T, int P> using VecP = T __attribute__((ext_vector_type(P)));
+template <typename T, int P> __device__ __forceinline__ VecP<T, P> ldv(gptr<T> f, size_t q)
+{
+    return *(const __attribute__((address_space(1))) VecP<T, P> *)(f.p + q);
+}
+template <typename T, int P> __device__ __forceinline__ void stv(gptr<T> f, size_t q, const VecP<T, P> &v)
+{
+    *(__attribute__((address_space(1))) VecP<T, P> *)(f.p + q) = v;
+}
+// Point (i, j, k) of the lane's first point, false for a padding tile, a tile past the list's
+// end or a row past the last interior one.
+template <typename T, int P>
+__device__ __forceinline__ bool vec_point(const AirArgs<T> &a, const BoxList<1> &l, int ntiles, int &i, int &j, int &k)
+{
+    constexpr int LR = BX / P;
+    const int t = (int)(blockIdx.x * blockDim.y + threadIdx.y);
+    if (t >= ntiles) return false;
+    list_point(l, t, i, j, k);  // i = tile origin + threadIdx.x, j = first row + threadIdx.y + 1
+    if (i < 0) return false;
+    const int lane = (int)threadIdx.x, row = lane / LR;
+    i += (lane - row * LR) * P - lane;
+    j += row - (int)threadIdx.y;
+    return j < a.g.ny - 1 && i >= 1 && i + P <= a.g.nx - 1;
+}
+template <typename T, bool TERRAIN, int P>
+__global__ __launch_bounds__(256) void k_air_p_vec(const AirArgs<T> a, const BoxList<1> l, int ntiles)
+{
+    typedef VecP<T, P> V;
+    int i, j, k;
+    if (!vec_point<T, P>(a, l, ntiles, i, j, k)) return;
+    const Grid<T> &g = a.g;
+    const size_t nx = (size_t)g.nx, sxy = nx * g.ny, id = (size_t)i + nx * ((size_t)j + (size_t)g.ny * k);
+    const V pc = ldv<T, P>(g.pn, id), uc = ldv<T, P>(g.un, id), vc = ldv<T, P>(g.vn, id), wc = ldv<T, P>(g.wn, id);
+    const V vjp = ldv<T, P>(g.vn, id + nx), wkp = ldv<T, P>(g.wn, id + sxy);
+    V ukp = uc, vkp = vc;
+    if constexpr (TERRAIN) { ukp = ldv<T, P>(g.un, id + sxy); vkp = ldv<T, P>(g.vn, id + sxy); }
+    const T ulast = g.un[id + P];
+    V out;
+#pragma unroll
+    for (int e = 0; e < P; ++e) {
+        Nb<T> n;
+        n.zX = T(0); n.zY = T(0);
+        if constexpr (TERRAIN) metric_terrain(g, i + e, j, k, n.mX, n.mY, n.mZ, n.zX, n.zY);
+        else metric(g, i + e, j, k, n.mX, n.mY, n.mZ);
+        n.pc = pc[e]; n.uc = uc[e]; n.vc = vc[e]; n.wc = wc[e];
+        n.u_ip = e + 1 < P ? uc[(e + 1) & (P - 1)] : ulast;
+        n.v_jp = vjp[e]; n.w_kp = wkp[e];
+        if constexpr (TERRAIN) { n.u_kp = ukp[e]; n.v_kp = vkp[e]; }
+        T ddx, ddy, dwz;
+        out[e] = pc[e] + dp_core<T, 0, TERRAIN, false>(a, n, ddx, ddy, dwz);
+    }
+    stv<T, P>(g.pnn, id, out);
+}
+template <typename T, bool TERRAIN, int P>
+__global__ __launch_bounds__(256) void k_air_v_vec(const AirArgs<T> a, const BoxList<1> l, int ntiles)
+{
+    typedef VecP<T, P> V;
+    int i, j, k;
+    if (!vec_point<T, P>(a, l, ntiles, i, j, k)) return;
+    const Grid<T> &g = a.g;
+    const size_t nx = (size_t)g.nx, sxy = nx * g.ny, id = (size_t)i + nx * ((size_t)j + (size_t)g.ny * k);
+    const V pc = ldv<T, P>(g.pnn, id), pjm = ldv<T, P>(g.pnn, id - nx), pkm = ldv<T, P>(g.pnn, id - sxy);
+    const V uc = ldv<T, P>(g.un, id), vc = ldv<T, P>(g.vn, id), wc = ldv<T, P>(g.wn, id);
+    const T pfirst = g.pnn[id - 1];
+    V uo, vo, wo;
+#pragma unroll
+    for (int e = 0; e < P; ++e) {
+        Nb<T> n;
+        n.zX = T(0); n.zY = T(0);
+        if constexpr (TERRAIN) metric_terrain(g, i + e, j, k, n.mX, n.mY, n.mZ, n.zX, n.zY);
+        else metric(g, i + e, j, k, n.mX, n.mY, n.mZ);
+        n.uc = uc[e]; n.vc = vc[e]; n.wc = wc[e];
+        n.pc = pc[e];
+        n.p_im = e > 0 ? pc[(e + P - 1) & (P - 1)] : pfirst;
+        n.p_jm = pjm[e]; n.p_km = pkm[e];
+        T gpx, gpy, gpz, du, dv, dw;
+        duvw_core<T, 0, TERRAIN, false>(a, n, gpx, gpy, gpz, du, dv, dw);
+        uo[e] = uc[e] + du; vo[e] = vc[e] + dv; wo[e] = wc[e] + dw;
+    }
+    stv<T, P>(g.unn, id, uo); stv<T, P>(g.vnn, id, vo); stv<T, P>(g.wnn, id, wo);
+}
+
 template <typename T, bool WIND, bool TERRAIN>
 __global__ __launch_bounds__(BX * BY) void k_air_v(const AirArgs<T> a)
 {
@@ -1890,16 +1981,42 @@ template <typename T> bool fb_passes_take_xfaces(const AirArgs<T> &a)
     return a.g.nx % BX != 1 && split_regions(a, R, false);
 }
 
-template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s, bool pad)
+// Points per lane of the interior box's vector form (k_air_p_vec / k_air_v_vec), 0 for the
+// one-point form.
+template <typename T> constexpr int kFbVecP = sizeof(T) == 4 ? 4 : 0;
+// Whether the interior box of the split passes runs the vector form: no mean flow, rows of a
+// multiple of P points and every array of the two time levels aligned to the vector.
+template <typename T> bool fb_interior_vec(const AirArgs<T> &a, bool wind)
+{
+    constexpr int P = kFbVecP<T>;
+    if (P == 0) return false;
+    const Grid<T> &g = a.g;
+    const uintptr_t al = (uintptr_t)(T *)g.un | (uintptr_t)(T *)g.vn | (uintptr_t)(T *)g.wn | (uintptr_t)(T *)g.pn |
+                         (uintptr_t)(T *)g.unn | (uintptr_t)(T *)g.vnn | (uintptr_t)(T *)g.wnn | (uintptr_t)(T *)g.pnn;
+    return !wind && g.nx % P == 0 && al % (P * sizeof(T)) == 0;
+}
+template <typename T, bool TERRAIN, bool PRESSURE>
+void launch_air_vec(const AirArgs<T> &a, const BoxList<1> &l, hipStream_t s)
+{
+    constexpr int P = kFbVecP<T> ? kFbVecP<T> : 1;
+    constexpr int LT = BX * BY / P, TPB = 256 / LT;  // lanes per tile, tiles per block
+    const int nt = (int)list_blocks(l);
+    const dim3 grid((unsigned)((nt + TPB - 1) / TPB)), block(LT, TPB);
+    if (PRESSURE) hipLaunchKernelGGL((k_air_p_vec<T, TERRAIN, P>), grid, block, 0, s, a, l, nt);
+    else hipLaunchKernelGGL((k_air_v_vec<T, TERRAIN, P>), grid, block, 0, s, a, l, nt);
+}
+
+template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s, bool pad, bool vec_ok)
 {
     dim3 g = interior_grid(a.g.nx, a.g.ny, a.g.nz), b(BX, BY);
     const bool ter = a.g.tsh != nullptr;
     Regions R;
     if (split_regions(a, R, pad)) {
-        const bool xf = fb_passes_take_xfaces<T>(a);
+        const bool xf = fb_passes_take_xfaces<T>(a), vec = vec_ok && fb_interior_vec<T>(a, wind);
 #define LEE_PS(W, TER)                                                                                       \
     do {                                                                                                     \
-        if (R.in.n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 0, 1>), dim3(list_blocks(R.in)), b, 0, s, a, R.in);    \
+        if (R.in.n && vec) launch_air_vec<T, TER, true>(a, R.in, s);                                            \
+        else if (R.in.n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 0, 1>), dim3(list_blocks(R.in)), b, 0, s, a, R.in); \
         if (R.ax[0].n && xf) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 1, 2, true>), dim3(list_blocks(R.ax[0])), b, 0, s, a, R.ax[0]); \
         else if (R.ax[0].n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 1, 2>), dim3(list_blocks(R.ax[0])), b, 0, s, a, R.ax[0]); \
         if (R.ax[1].n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 2, 2>), dim3(list_blocks(R.ax[1])), b, 0, s, a, R.ax[1]); \
@@ -1920,16 +2037,17 @@ template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStrea
         else hipLaunchKernelGGL((k_air_p<T, false, false>), g, b, 0, s, a);
     }
 }
-template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStream_t s, bool pad)
+template <typename T> void launch_air_v(const AirArgs<T> &a, bool wind, hipStream_t s, bool pad, bool vec_ok)
 {
     dim3 g = interior_grid(a.g.nx, a.g.ny, a.g.nz), b(BX, BY);
     const bool ter = a.g.tsh != nullptr;
     Regions R;
     if (split_regions(a, R, pad)) {
-        const bool xf = fb_passes_take_xfaces<T>(a);
+        const bool xf = fb_passes_take_xfaces<T>(a), vec = vec_ok && fb_interior_vec<T>(a, wind);
 #define LEE_VS(W, TER)                                                                                       \
     do {                                                                                                     \
-        if (R.in.n) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 0, 1>), dim3(list_blocks(R.in)), b, 0, s, a, R.in);    \
+        if (R.in.n && vec) launch_air_vec<T, TER, false>(a, R.in, s);                                           \
+        else if (R.in.n) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 0, 1>), dim3(list_blocks(R.in)), b, 0, s, a, R.in); \
         if (R.ax[0].n && xf) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 1, 2, true>), dim3(list_blocks(R.ax[0])), b, 0, s, a, R.ax[0]); \
         else if (R.ax[0].n) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 1, 2>), dim3(list_blocks(R.ax[0])), b, 0, s, a, R.ax[0]); \
         if (R.ax[1].n) hipLaunchKernelGGL((k_air_v_box<T, W, TER, 2, 2>), dim3(list_blocks(R.ax[1])), b, 0, s, a, R.ax[1]); \
@@ -2241,8 +2359,8 @@ void launch_copy_words(float *dst, const float *src, size_t n, hipStream_t s)
 }
 
 #define LEE_INST(T)                                                                                          \
-    template void launch_air_p<T>(const AirArgs<T> &, bool, hipStream_t, bool);                              \
-    template void launch_air_v<T>(const AirArgs<T> &, bool, hipStream_t, bool);                              \
+    template void launch_air_p<T>(const AirArgs<T> &, bool, hipStream_t, bool, bool);                              \
+    template void launch_air_v<T>(const AirArgs<T> &, bool, hipStream_t, bool, bool);                              \
     template bool fused_fits<T>(int);                                                                        \
     template void launch_rk_air<T>(const RKArgs<T> &, const RKArgs<T> *, int, bool, hipStream_t);            \
     template void launch_rk_pore<T>(const RKPoreArgs<T> &, bool, hipStream_t);                               \

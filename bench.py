@@ -56,6 +56,9 @@ RK_TILES = ["64x4 planes", "64x4 xcd-columns", "64x8 xcd-columns", "64x8 planes"
             "64x4 interior + layer shell", "64x4 interior + layers by axis", "64x4 LDS-staged interior + layer shell",
             "64x4 LDS-staged interior + layers by axis", "64x4 LDS-staged interior + LDS-staged layer shell",
             "64x4 LDS-staged interior + LDS-staged layers by axis"]
+# launch forms of the FB_p_v air passes (lee_options.fb_form 1..7), as the autotuner reports them
+FB_FORMS = ["split", "split, padded tile lists", "fused, 2 blocks/CU", "fused, 4 blocks/CU", "fused, 8 blocks/CU",
+            "fused, 16 blocks/CU", "split, one point per lane"]
 
 
 RK_TILE = 0  # --rk-tile: RK4 air-stage launch shape, 0 = the autotuner's choice
@@ -108,14 +111,14 @@ def run_gpu(lee, deck, receivers, precision, steps, warmup, record=True):
     series += advance(steps)
     t1 = time.perf_counter()
     kst = kernel_stats()
-    tile = s.info().rk_tile
+    tile, form = s.info().rk_tile, s.info().fb_form
     copy_ms = lee.lib().lee_copy_bench(s.h, 1 << 28, 10)  # 1 GiB read + 1 GiB write
     # a kernel with as many streams as an RK4 stage's average (12 read, 8 written, 64 MiB each)
     stream_ms = lee.lib().lee_stream_bench(s.h, 1 << 24, STREAMS[0], STREAMS[1], 10)
     rec = np.concatenate([x for x in series if x is not None], axis=0) if record else None
     info = dict(points=npts, wall=t1 - t0, kernels=kst, kernels_all=kall or kst, copy_ms=copy_ms, stream_ms=stream_ms,
                 frames=state["mf"] + 1, series=rec,
-                rk_tile=RK_TILES[tile - 1] if tile > 0 else None)
+                rk_tile=RK_TILES[tile - 1] if tile > 0 else None, fb_form=FB_FORMS[form - 1] if form > 0 else None)
     s.close()
     return info
 
@@ -490,6 +493,7 @@ def main():
             fb[f"fb_fp{p}_pt_updates_per_s"] = rr["points"] * 200 / rr["wall"]
             fb[f"fb_fp{p}_ms_per_step"] = 1e3 * rr["wall"] / 200
             fb[f"fb_fp{p}_kernels"] = kernel_table(rr["kernels_all"])
+            fb[f"fb_fp{p}_launch_form"] = rr["fb_form"]
     cpu = cpu_ref = None
     if not a.no_cpu:
         from oracle import oracle as orc

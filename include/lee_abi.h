@@ -125,10 +125,12 @@ typedef struct lee_options {
     int rk_tile;
     /* FB_p_v air-pass launch form: 0 = autotune on the first step among the forms `path`
      * allows (times each form's air and face passes on the coming step), 1 = the split
-     * passes, 2 = the split passes with each plane's tiles padded to a multiple of 8 (a tile
-     * column on one XCD), 3..6 = the fused pass with pencils deep enough for about 2, 4, 8, 16
-     * blocks per CU.  1..2 need path AUTO or SPLIT, 3..6 AUTO or FUSED.  All give the same
-     * bits. */
+     * passes (in FP32 the interior box four points per lane, 16-byte accesses, when there is
+     * no mean flow and nx is a multiple of 4), 2 = the same with each plane's tiles padded to
+     * a multiple of 8 (a tile column on one XCD), 3..6 = the fused pass with pencils deep
+     * enough for about 2, 4, 8, 16 blocks per CU, 7 = the split passes with one point per
+     * lane throughout.  1, 2 and 7 need path AUTO or SPLIT, 3..6 AUTO or FUSED.  All give
+     * the same bits. */
     int fb_form;
 } lee_options;
 
@@ -160,7 +162,7 @@ typedef struct lee_info {
     long long pml_y1_counter;       /* reference PML_JMAX1 step counter */
     int path;                       /* LEE_PATH_SPLIT or LEE_PATH_FUSED, as chosen */
     int rk_tile;                    /* RK4 air-stage launch shape in use (1..11), 0 before tuning */
-    int fb_form;                    /* FB_p_v air-pass launch form in use (1..6), 0 before tuning */
+    int fb_form;                    /* FB_p_v air-pass launch form in use (1..7), 0 before tuning */
 } lee_info;
 
 typedef struct lee_ctx lee_ctx;

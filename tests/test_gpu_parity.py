@@ -142,12 +142,14 @@ def test_fused_equals_split_bitwise(lee, prec):
 @pytest.mark.parametrize("prec", [32, 64])
 def test_fb_launch_forms_give_the_same_bits(lee, prec):
     """Every launch form of the FB_p_v air passes the autotuner picks from (split, split with
-    padded tile lists, fused at four pencil depths) gives the same bits; the tuner keeps one
-    of the forms the path allows."""
+    padded tile lists, fused at four pencil depths, split with one point per lane
+    throughout) gives the same bits; the tuner keeps one of the forms the path allows.  On
+    the 264-point-wide deck the FP32 split passes run their interior box four points per
+    lane, so forms 1 and 2 check that form against the one-point forms."""
     for name in ("pml5", "wide"):
         d = cases(lee)[name]
         ref = None
-        for form in range(1, 7):
+        for form in range(1, 8):
             print(f"[gpu] fb launch form {form} fp{prec} case={name}", flush=True)
             s = lee.Solver(d, precision=prec, receivers=REC, fb_form=form)
             s.frame_setup(0)
@@ -161,7 +163,7 @@ def test_fb_launch_forms_give_the_same_bits(lee, prec):
             for x, y in zip(got[1] + got[2], ref[1] + ref[2]):
                 assert np.array_equal(x, y), (name, form)
     d = cases(lee)["wide"]
-    for path, allowed in (("split", (1, 2)), ("fused", (3, 4, 5, 6)), (None, tuple(range(1, 7)))):
+    for path, allowed in (("split", (1, 2, 7)), ("fused", (3, 4, 5, 6)), (None, tuple(range(1, 8)))):
         s = lee.Solver(d, precision=prec, receivers=REC, path=PATHS[path] if path else 0)
         s.frame_setup(0)
         s.step(2)
@@ -169,3 +171,26 @@ def test_fb_launch_forms_give_the_same_bits(lee, prec):
         s.close()
     with pytest.raises(lee.LeeError):
         lee.Solver(d, precision=prec, receivers=REC, path=PATHS["split"], fb_form=4)
+    with pytest.raises(lee.LeeError):
+        lee.Solver(d, precision=prec, receivers=REC, path=PATHS["fused"], fb_form=7)
+    with pytest.raises(lee.LeeError):
+        lee.Solver(d, precision=prec, receivers=REC, fb_form=8)
+
+
+@pytest.mark.parametrize("name", ["wide_terrain", "wide_wind"])
+def test_fb_vector_interior_gives_the_same_bits(lee, name):
+    """FP32 split passes: the interior box four points per lane (form 1; over terrain it also
+    reads u and v at k + 1 as vectors) against one point per lane (form 7), fields and
+    receivers bit for bit; with a mean flow both run the one-point form."""
+    d = cases(lee)[name]
+    got = []
+    for form in (1, 7):
+        print(f"[gpu] fb vector interior form {form} case={name}", flush=True)
+        s = lee.Solver(d, precision=32, receivers=REC, path=PATHS["split"], fb_form=form)
+        s.frame_setup(0)
+        got.append((s.step(d.time_domain), s.state(), s.state(1) if d.has_pore else []))
+        s.close()
+    assert np.array_equal(got[0][0], got[1][0]), name
+    for x, y in zip(got[0][1] + got[0][2], got[1][1] + got[1][2]):
+        assert np.array_equal(x, y), name
+    assert np.abs(got[0][1][3]).max() > 0, name
