@@ -905,7 +905,10 @@ __device__ __forceinline__ bool vec_point(const AirArgs<T> &a, const BoxList<1> 
     j += row - (int)threadIdx.y;
     return j < a.g.ny - 1 && i >= 1 && i + P <= a.g.nx - 1;
 }
-template <typename T, bool TERRAIN, int P>
+// The pressure form runs on a separable grid only: over terrain (u and v at k + 1 and the
+// per-column metric tables as well) the four-point pressure pass was slower than the
+// one-point one, and the velocity pass faster.
+template <typename T, int P>
 __global__ __launch_bounds__(256) void k_air_p_vec(const AirArgs<T> a, const BoxList<1> l, int ntiles)
 {
     typedef VecP<T, P> V;
@@ -915,22 +918,18 @@ __global__ __launch_bounds__(256) void k_air_p_vec(const AirArgs<T> a, const Box
     const size_t nx = (size_t)g.nx, sxy = nx * g.ny, id = (size_t)i + nx * ((size_t)j + (size_t)g.ny * k);
     const V pc = ldv<T, P>(g.pn, id), uc = ldv<T, P>(g.un, id), vc = ldv<T, P>(g.vn, id), wc = ldv<T, P>(g.wn, id);
     const V vjp = ldv<T, P>(g.vn, id + nx), wkp = ldv<T, P>(g.wn, id + sxy);
-    V ukp = uc, vkp = vc;
-    if constexpr (TERRAIN) { ukp = ldv<T, P>(g.un, id + sxy); vkp = ldv<T, P>(g.vn, id + sxy); }
     const T ulast = g.un[id + P];
     V out;
 #pragma unroll
     for (int e = 0; e < P; ++e) {
         Nb<T> n;
         n.zX = T(0); n.zY = T(0);
-        if constexpr (TERRAIN) metric_terrain(g, i + e, j, k, n.mX, n.mY, n.mZ, n.zX, n.zY);
-        else metric(g, i + e, j, k, n.mX, n.mY, n.mZ);
+        metric(g, i + e, j, k, n.mX, n.mY, n.mZ);
         n.pc = pc[e]; n.uc = uc[e]; n.vc = vc[e]; n.wc = wc[e];
         n.u_ip = e + 1 < P ? uc[(e + 1) & (P - 1)] : ulast;
         n.v_jp = vjp[e]; n.w_kp = wkp[e];
-        if constexpr (TERRAIN) { n.u_kp = ukp[e]; n.v_kp = vkp[e]; }
         T ddx, ddy, dwz;
-        out[e] = pc[e] + dp_core<T, 0, TERRAIN, false>(a, n, ddx, ddy, dwz);
+        out[e] = pc[e] + dp_core<T, 0, false, false>(a, n, ddx, ddy, dwz);
     }
     stv<T, P>(g.pnn, id, out);
 }
@@ -2002,8 +2001,12 @@ void launch_air_vec(const AirArgs<T> &a, const BoxList<1> &l, hipStream_t s)
     constexpr int LT = BX * BY / P, TPB = 256 / LT;  // lanes per tile, tiles per block
     const int nt = (int)list_blocks(l);
     const dim3 grid((unsigned)((nt + TPB - 1) / TPB)), block(LT, TPB);
-    if (PRESSURE) hipLaunchKernelGGL((k_air_p_vec<T, TERRAIN, P>), grid, block, 0, s, a, l, nt);
-    else hipLaunchKernelGGL((k_air_v_vec<T, TERRAIN, P>), grid, block, 0, s, a, l, nt);
+    if constexpr (PRESSURE) {
+        static_assert(!TERRAIN, "the pressure pass's vector form runs on a separable grid");
+        hipLaunchKernelGGL((k_air_p_vec<T, P>), grid, block, 0, s, a, l, nt);
+    } else {
+        hipLaunchKernelGGL((k_air_v_vec<T, TERRAIN, P>), grid, block, 0, s, a, l, nt);
+    }
 }
 
 template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStream_t s, bool pad, bool vec_ok)
@@ -2012,10 +2015,11 @@ template <typename T> void launch_air_p(const AirArgs<T> &a, bool wind, hipStrea
     const bool ter = a.g.tsh != nullptr;
     Regions R;
     if (split_regions(a, R, pad)) {
-        const bool xf = fb_passes_take_xfaces<T>(a), vec = vec_ok && fb_interior_vec<T>(a, wind);
+        // the pressure pass's vector form on a separable grid only
+        const bool xf = fb_passes_take_xfaces<T>(a), vec = vec_ok && !ter && fb_interior_vec<T>(a, wind);
 #define LEE_PS(W, TER)                                                                                       \
     do {                                                                                                     \
-        if (R.in.n && vec) launch_air_vec<T, TER, true>(a, R.in, s);                                            \
+        if (R.in.n && vec) launch_air_vec<T, false, true>(a, R.in, s);                                          \
         else if (R.in.n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 0, 1>), dim3(list_blocks(R.in)), b, 0, s, a, R.in); \
         if (R.ax[0].n && xf) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 1, 2, true>), dim3(list_blocks(R.ax[0])), b, 0, s, a, R.ax[0]); \
         else if (R.ax[0].n) hipLaunchKernelGGL((k_air_p_box<T, W, TER, 1, 2>), dim3(list_blocks(R.ax[0])), b, 0, s, a, R.ax[0]); \

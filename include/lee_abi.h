@@ -126,10 +126,10 @@ typedef struct lee_options {
     /* FB_p_v air-pass launch form: 0 = autotune on the first step among the forms `path`
      * allows (times each form's air and face passes on the coming step), 1 = the split
      * passes (in FP32 the interior box four points per lane, 16-byte accesses, when there is
-     * no mean flow and nx is a multiple of 4), 2 = the same with each plane's tiles padded to
-     * a multiple of 8 (a tile column on one XCD), 3..6 = the fused pass with pencils deep
-     * enough for about 2, 4, 8, 16 blocks per CU, 7 = the split passes with one point per
-     * lane throughout.  1, 2 and 7 need path AUTO or SPLIT, 3..6 AUTO or FUSED.  All give
+     * no mean flow and nx is a multiple of 4; over terrain in the velocity pass only), 2 = the
+     * same with each plane's tiles padded to a multiple of 8 (a tile column on one XCD),
+     * 3..6 = the fused pass with pencils deep enough for about 2, 4, 8, 16 blocks per CU,
+     * 7 = the split passes with one point per lane throughout.  1, 2 and 7 need path AUTO or SPLIT, 3..6 AUTO or FUSED.  All give
      * the same bits. */
     int fb_form;
 } lee_options;

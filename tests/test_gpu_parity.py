@@ -179,9 +179,9 @@ def test_fb_launch_forms_give_the_same_bits(lee, prec):
 
 @pytest.mark.parametrize("name", ["wide_terrain", "wide_wind"])
 def test_fb_vector_interior_gives_the_same_bits(lee, name):
-    """FP32 split passes: the interior box four points per lane (form 1; over terrain it also
-    reads u and v at k + 1 as vectors) against one point per lane (form 7), fields and
-    receivers bit for bit; with a mean flow both run the one-point form."""
+    """FP32 split passes: the interior box four points per lane (form 1; over terrain the
+    velocity pass only) against one point per lane (form 7), fields and receivers bit for
+    bit; with a mean flow both run the one-point form."""
     d = cases(lee)[name]
     got = []
     for form in (1, 7):
